@@ -1,0 +1,149 @@
+"""Adversarial geometry corpus for the relation refine.
+
+Grid-aligned polygons (axis-parallel and diagonal edges, holes,
+multipolygons, unclosed rings, collinear vertices) probed by points on
+vertices, on edge midpoints, at vertex heights, one ulp off edges and in
+general position.  Every branch of the reference's ray-cast classifier
+(c/sedona-tg/src/tg/tg.c:1199-1275) and of its collinearity test (861-901)
+is reached by some pair.  Deterministic for a given seed.
+"""
+import struct
+
+import numpy as np
+
+
+def _ring_bytes(pts):
+    return struct.pack("<I", len(pts)) + b"".join(struct.pack("<dd", float(x), float(y)) for x, y in pts)
+
+
+def polygon_wkb(rings, big_endian=False):
+    if big_endian:
+        out = b"\x00" + struct.pack(">I", 3) + struct.pack(">I", len(rings))
+        for r in rings:
+            out += struct.pack(">I", len(r)) + b"".join(struct.pack(">dd", float(x), float(y)) for x, y in r)
+        return out
+    return b"\x01" + struct.pack("<I", 3) + struct.pack("<I", len(rings)) + b"".join(_ring_bytes(r) for r in rings)
+
+
+def multipolygon_wkb(polys):
+    return b"\x01" + struct.pack("<I", 6) + struct.pack("<I", len(polys)) + b"".join(polygon_wkb(p) for p in polys)
+
+
+def point_wkb(x, y, big_endian=False):
+    if big_endian:
+        return b"\x00" + struct.pack(">I", 1) + struct.pack(">dd", x, y)
+    return b"\x01" + struct.pack("<I", 1) + struct.pack("<dd", x, y)
+
+
+def linestring_wkb(pts):
+    return b"\x01" + struct.pack("<I", 2) + _ring_bytes(pts)
+
+
+def column(wkbs):
+    off = np.zeros(len(wkbs) + 1, np.int64)
+    off[1:] = np.cumsum([len(w) for w in wkbs])
+    data = np.frombuffer(b"".join(wkbs), dtype=np.uint8).copy() if wkbs else np.zeros(0, np.uint8)
+    return off, data, None
+
+
+def build_polygons(seed=11):
+    rng = np.random.default_rng(seed)
+    polys = []
+    # axis-aligned squares and rectangles
+    polys.append(polygon_wkb([[(0, 0), (4, 0), (4, 4), (0, 4), (0, 0)]]))
+    polys.append(polygon_wkb([[(1, 1), (7, 1), (7, 3), (1, 3), (1, 1)]]))
+    # square with a hole
+    polys.append(polygon_wkb([[(0, 0), (8, 0), (8, 8), (0, 8), (0, 0)],
+                              [(2, 2), (2, 6), (6, 6), (6, 2), (2, 2)]]))
+    # diamond (diagonal edges)
+    polys.append(polygon_wkb([[(4, 0), (8, 4), (4, 8), (0, 4), (4, 0)]]))
+    # concave L with collinear vertices and horizontal runs
+    polys.append(polygon_wkb([[(0, 0), (3, 0), (6, 0), (6, 2), (2, 2), (2, 6), (0, 6), (0, 3), (0, 0)]]))
+    # triangle with vertex at the probes' y levels
+    polys.append(polygon_wkb([[(1, 1), (7, 4), (1, 7), (1, 1)]]))
+    # big-endian polygon
+    polys.append(polygon_wkb([[(0, 0), (5, 1), (6, 6), (1, 5), (0, 0)]], big_endian=True))
+    # multipolygon: two squares sharing an edge and one apart
+    polys.append(multipolygon_wkb([[[(0, 0), (2, 0), (2, 2), (0, 2), (0, 0)]],
+                                   [[(2, 0), (4, 0), (4, 2), (2, 2), (2, 0)]],
+                                   [[(5, 5), (7, 5), (7, 7), (5, 7), (5, 5)],
+                                    [(5.5, 5.5), (6.5, 5.5), (6.5, 6.5), (5.5, 6.5), (5.5, 5.5)]]]))
+    # spiky star with vertices on a 0.5 grid
+    for _ in range(6):
+        n = int(rng.integers(5, 14))
+        ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+        r = rng.integers(2, 9, n) * 0.5
+        cx, cy = rng.integers(2, 7, 2)
+        pts = [(cx + np.round(ri * np.cos(a) * 2) / 2, cy + np.round(ri * np.sin(a) * 2) / 2) for ri, a in zip(r, ang)]
+        pts.append(pts[0])
+        polys.append(polygon_wkb([pts]))
+    # general-position star with irrational coordinates
+    for _ in range(6):
+        n = int(rng.integers(6, 40))
+        ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+        r = rng.uniform(1.5, 4.0, n)
+        cx, cy = rng.uniform(2, 6, 2)
+        pts = [(cx + ri * np.cos(a), cy + ri * np.sin(a)) for ri, a in zip(r, ang)]
+        pts.append(pts[0])
+        polys.append(polygon_wkb([pts]))
+    # degenerate: empty polygon, polygon with an empty ring
+    polys.append(b"\x01" + struct.pack("<I", 3) + struct.pack("<I", 0))
+    polys.append(polygon_wkb([[]]))
+    return polys
+
+
+def invalid_wkbs():
+    """Rows the reference refuses to parse (tg.c:8380-8399) or that are not
+    2-D WKB: each must raise an error naming the row."""
+    return [
+        polygon_wkb([[(1, 1), (7, 1), (7, 3), (1, 3)]]),  # ring not closed
+        polygon_wkb([[(1, 1), (2, 2)]]),  # ring with two positions
+        linestring_wkb([(1, 1)]),  # line with one position
+        b"\x01" + struct.pack("<I", 1001) + struct.pack("<ddd", 1, 2, 3),  # Z point
+        b"\x01" + struct.pack("<I", 3),  # truncated
+        b"\x02" + struct.pack("<I", 1) + struct.pack("<dd", 1, 2),  # bad byte order
+    ]
+
+
+def probe_points(build_wkbs, seed=12, n_random=600):
+    """Points on a 0.25 grid, on every vertex and edge midpoint of the build
+    polygons, one ulp either side of those, and random points."""
+    rng = np.random.default_rng(seed)
+    pts = []
+    g = np.arange(-0.5, 8.75, 0.25)
+    for x in g[::2]:
+        for y in g:
+            pts.append((float(x), float(y)))
+    import sys
+    import os
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "..", "oracle"))
+    import oracle as O
+    st = O.Store(column(build_wkbs)).soa()
+    xy = st["xy"]
+    vo = st["pt_off"]
+    for r in range(len(vo) - 1):
+        ring = xy[vo[r]:vo[r + 1]]
+        for i in range(len(ring)):
+            a = ring[i]
+            b = ring[(i + 1) % len(ring)]
+            m = (a + b) / 2
+            for p in (a, m):
+                px, py = float(p[0]), float(p[1])
+                pts.append((px, py))
+                pts.append((np.nextafter(px, np.inf), py))
+                pts.append((px, np.nextafter(py, -np.inf)))
+                pts.append((np.nextafter(px, -np.inf), np.nextafter(py, np.inf)))
+            # a point at the vertex's height well left and right of it
+            pts.append((float(a[0]) - 0.37, float(a[1])))
+            pts.append((float(a[0]) + 0.37, float(a[1])))
+    for _ in range(n_random):
+        pts.append((float(rng.uniform(-0.5, 8.5)), float(rng.uniform(-0.5, 8.5))))
+    wk = [point_wkb(x, y, big_endian=(i % 7 == 3)) for i, (x, y) in enumerate(pts)]
+    wk.append(point_wkb(float("nan"), float("nan")))  # empty point
+    return wk
+
+
+def corpus(seed=11):
+    b = build_polygons(seed)
+    p = probe_points(b, seed + 1)
+    return column(b), column(p)
