@@ -1,0 +1,295 @@
+/*
+ * gpuspatial_hip.h -- C ABI of the MI355X spatial-join library (libsjhip.so).
+ *
+ * Part 1 declares exactly the structs and three create functions that the
+ * reference's Rust FFI binds (bindgen over
+ * c/sedona-libgpuspatial/libgpuspatial/include/gpuspatial/gpuspatial_c.h,
+ * driven from c/sedona-libgpuspatial/build.rs:263-279).  The layouts,
+ * member order, names (including the ABI's "precent" spelling) and
+ * semantics are kept identical so that the Rust crates
+ * c/sedona-libgpuspatial/src/libgpuspatial.rs and
+ * rust/sedona-spatial-join-gpu/src/index/ (Rust) link against this library
+ * unchanged.
+ *
+ * Part 2 declares additive extensions that the reference has no GPU
+ * counterpart for: the distance refine (ST_DWithin), the KNN join, and a
+ * device-resident fused join plus request loop.  Each cites the reference
+ * interface it serves.
+ *
+ * Conventions for every entry point: 0 on success, EINVAL (or another
+ * errno value) on failure with the message available from the object's
+ * get_last_error; all input buffers are borrowed for the duration of the
+ * call; the library never releases caller-owned Arrow arrays.
+ */
+#ifndef GPUSPATIAL_HIP_H
+#define GPUSPATIAL_HIP_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Arrow C Data Interface and C Device Data Interface: public Arrow
+ * specification structs, guarded as the specification prescribes so that
+ * they coexist with arrow/c/abi.h or nanoarrow. */
+#ifndef ARROW_C_DATA_INTERFACE
+#define ARROW_C_DATA_INTERFACE
+#define ARROW_FLAG_DICTIONARY_ORDERED 1
+#define ARROW_FLAG_NULLABLE 2
+#define ARROW_FLAG_MAP_KEYS_SORTED 4
+struct ArrowSchema {
+  const char* format;
+  const char* name;
+  const char* metadata;
+  int64_t flags;
+  int64_t n_children;
+  struct ArrowSchema** children;
+  struct ArrowSchema* dictionary;
+  void (*release)(struct ArrowSchema*);
+  void* private_data;
+};
+struct ArrowArray {
+  int64_t length;
+  int64_t null_count;
+  int64_t offset;
+  int64_t n_buffers;
+  int64_t n_children;
+  const void** buffers;
+  struct ArrowArray** children;
+  struct ArrowArray* dictionary;
+  void (*release)(struct ArrowArray*);
+  void* private_data;
+};
+#endif
+
+#ifndef ARROW_C_DEVICE_DATA_INTERFACE
+#define ARROW_C_DEVICE_DATA_INTERFACE
+typedef int32_t ArrowDeviceType;
+#define ARROW_DEVICE_CPU 1
+#define ARROW_DEVICE_ROCM 10
+struct ArrowDeviceArray {
+  struct ArrowArray array;
+  int64_t device_id;
+  ArrowDeviceType device_type;
+  void* sync_event;
+  int64_t reserved[3];
+};
+#endif
+
+/* ======================================================================
+ * Part 1: the reference ABI (gpuspatial_c.h)
+ * ==================================================================== */
+
+/* gpuspatial_c.h:28-37.  ptx_root: the reference loads OptiX PTX from this
+ * directory; this library embeds its gfx950 code objects and only checks
+ * that the directory exists, failing with "No such file or directory" like
+ * the reference (test/c_wrapper_test.cc:45-65). */
+struct GpuSpatialRuntimeConfig {
+  const char* ptx_root;
+  int device_id;
+  bool use_cuda_memory_pool;
+  int cuda_memory_pool_init_precent;
+};
+
+/* gpuspatial_c.h:39-50: one instance per process. */
+struct GpuSpatialRuntime {
+  int (*init)(struct GpuSpatialRuntime* self, struct GpuSpatialRuntimeConfig* config);
+  void (*release)(struct GpuSpatialRuntime* self);
+  const char* (*get_last_error)(struct GpuSpatialRuntime* self);
+  void* private_data;
+};
+
+/* gpuspatial_c.h:53 */
+void GpuSpatialRuntimeCreate(struct GpuSpatialRuntime* runtime);
+
+/* gpuspatial_c.h:55-60 */
+struct GpuSpatialIndexConfig {
+  const struct GpuSpatialRuntime* runtime;
+  uint32_t concurrency;
+};
+
+/* gpuspatial_c.h:62-65 */
+struct SedonaSpatialIndexContext {
+  void* private_data;
+};
+
+/* gpuspatial_c.h:67-122: FP32 2-D rectangle index.  push_build takes
+ * n_rects x [min_x, min_y, max_x, max_y]; NaN or inverted rects are empty
+ * and never match.  probe reports every (build, probe) pair whose rects
+ * intersect (boundaries inclusive) through the callback, possibly in
+ * several calls; a non-zero callback return stops the probe and is
+ * returned by probe. */
+struct SedonaFloatIndex2D {
+  int (*clear)(struct SedonaFloatIndex2D* self);
+  void (*create_context)(struct SedonaSpatialIndexContext* context);
+  void (*destroy_context)(struct SedonaSpatialIndexContext* context);
+  int (*push_build)(struct SedonaFloatIndex2D* self, const float* buf, uint32_t n_rects);
+  int (*finish_building)(struct SedonaFloatIndex2D* self);
+  int (*probe)(struct SedonaFloatIndex2D* self, struct SedonaSpatialIndexContext* context,
+               const float* buf, uint32_t n_rects,
+               int (*callback)(const uint32_t* build_indices, const uint32_t* probe_indices,
+                               uint32_t length, void* user_data),
+               void* user_data);
+  const char* (*get_last_error)(struct SedonaFloatIndex2D* self);
+  const char* (*context_get_last_error)(struct SedonaSpatialIndexContext* context);
+  void (*release)(struct SedonaFloatIndex2D* self);
+  void* private_data;
+};
+
+/* gpuspatial_c.h:127-128 */
+int GpuSpatialIndexFloat2DCreate(struct SedonaFloatIndex2D* index,
+                                 const struct GpuSpatialIndexConfig* config);
+
+/* gpuspatial_c.h:130-140 */
+struct GpuSpatialRefinerConfig {
+  const struct GpuSpatialRuntime* runtime;
+  uint32_t concurrency;
+  bool compress_bvh;
+  uint32_t pipeline_batches;
+};
+
+/* gpuspatial_c.h:142-151 (values 0..7). */
+enum SedonaSpatialRelationPredicate {
+  SedonaSpatialPredicateEquals = 0,
+  SedonaSpatialPredicateDisjoint,
+  SedonaSpatialPredicateTouches,
+  SedonaSpatialPredicateContains,
+  SedonaSpatialPredicateCovers,
+  SedonaSpatialPredicateIntersects,
+  SedonaSpatialPredicateWithin,
+  SedonaSpatialPredicateCoveredBy
+};
+
+/* gpuspatial_c.h:154-211.  Geometry columns are Arrow Binary ("z"),
+ * LargeBinary ("Z") or BinaryView ("vz") arrays of 2-D WKB.  refine
+ * evaluates predicate(build, probe) for each candidate pair, rewrites the
+ * caller's arrays in place with the pairs that hold, sorted by (probe,
+ * build), and sets *new_indices_size.  Supported: Intersects, Contains,
+ * Within, Covers, CoveredBy for (Multi)Polygon x Point, Point x
+ * (Multi)Polygon and Point x Point; Intersects for (Multi)Polygon x
+ * (Multi)Polygon.  Other combinations fail with EINVAL. */
+struct SedonaSpatialRefiner {
+  int (*clear)(struct SedonaSpatialRefiner* self);
+  int (*init_build_schema)(struct SedonaSpatialRefiner* self, const struct ArrowSchema* build_schema);
+  int (*push_build)(struct SedonaSpatialRefiner* self, const struct ArrowArray* build_array);
+  int (*finish_building)(struct SedonaSpatialRefiner* self);
+  int (*refine)(struct SedonaSpatialRefiner* self, const struct ArrowSchema* probe_schema,
+                const struct ArrowArray* probe_array, enum SedonaSpatialRelationPredicate predicate,
+                uint32_t* build_indices, uint32_t* probe_indices, uint32_t indices_size,
+                uint32_t* new_indices_size);
+  const char* (*get_last_error)(struct SedonaSpatialRefiner* self);
+  void (*release)(struct SedonaSpatialRefiner* self);
+  void* private_data;
+};
+
+/* gpuspatial_c.h:216-217 */
+int GpuSpatialRefinerCreate(struct SedonaSpatialRefiner* refiner,
+                            const struct GpuSpatialRefinerConfig* config);
+
+/* ======================================================================
+ * Part 2: extensions
+ * ==================================================================== */
+
+#define SJHIP_EXT_VERSION 1
+int SjhipExtVersion(void);
+
+/* Distance refine for ST_DWithin(build, probe, distance).  Replaces the
+ * reference's CPU Geo refiner (rust/sedona-spatial-join/src/refine/geo.rs:
+ * 312-335: build.distance_ext(probe) <= distance) that the GPU operator
+ * rejects today (rust/sedona-spatial-join-gpu/src/index/gpu_spatial_index.rs:
+ * 119-121).  Probe rows must be points; build rows Point, LineString,
+ * Polygon or their Multi* forms.  In/out contract as refine. */
+int SjhipRefinerRefineDistance(struct SedonaSpatialRefiner* self,
+                               const struct ArrowSchema* probe_schema,
+                               const struct ArrowArray* probe_array, double distance,
+                               uint32_t* build_indices, uint32_t* probe_indices,
+                               uint32_t indices_size, uint32_t* new_indices_size);
+
+/* KNN join: for every probe point the k nearest build points by Euclidean
+ * distance (hypot), ties broken by build id.  Replaces the CPU
+ * SpatialIndex::query_knn (rust/sedona-spatial-join/src/index/
+ * default_spatial_index.rs:306-502) that the GPU index leaves
+ * unimplemented (rust/sedona-spatial-join-gpu/src/index/
+ * gpu_spatial_index.rs:252-264).  Outputs are row-major n_probe x k;
+ * a probe with fewer than k neighbours gets build id 0xFFFFFFFF and
+ * distance +inf in the unused slots.  Build rows must be points. */
+int SjhipRefinerKnn(struct SedonaSpatialRefiner* self, const struct ArrowSchema* probe_schema,
+                    const struct ArrowArray* probe_array, uint32_t k, uint32_t* out_build_indices,
+                    double* out_distances);
+
+/* Result of a device-resident join: device pointers owned by the library,
+ * valid until release is called. */
+struct SjhipJoinResult {
+  uint32_t* build_indices; /* device memory */
+  uint32_t* probe_indices; /* device memory */
+  uint64_t size;
+  uint64_t candidates;      /* filter output count */
+  uint64_t fallback_tests;  /* FP32 segment tests resolved in FP64 */
+  uint64_t segment_tests;   /* segment tests performed */
+  void (*release)(struct SjhipJoinResult* self);
+  void* private_data;
+};
+
+/* Join options for the device-resident join. */
+struct SjhipJoinOptions {
+  int32_t predicate;   /* SedonaSpatialRelationPredicate, or 100 = DWithin */
+  double distance;     /* DWithin threshold */
+  int32_t precision;   /* 0 = FP32 certified + FP64 fallback, 1 = all-FP64 */
+  int32_t sort_output; /* 1 = pairs sorted by (probe, build) */
+  int32_t sort_probes; /* 1 = visit probes in Morton order */
+  void* stream;        /* hipStream_t, or NULL for the library stream */
+};
+
+/* One probe batch of the GPU operator's query_batch
+ * (rust/sedona-spatial-join-gpu/src/index/gpu_spatial_index.rs:163-229)
+ * with filter and refine fused on the device: the probe WKB column is
+ * already in device memory (ArrowDeviceArray with device_type ROCm) or in
+ * host memory (device_type CPU, copied in).  The index and refiner must
+ * have been built from the same build rows in the same order. */
+int SjhipJoinDevice(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner* refiner,
+                    const struct ArrowSchema* probe_schema,
+                    const struct ArrowDeviceArray* probe_array,
+                    const struct SjhipJoinOptions* options, struct SjhipJoinResult* out);
+
+/* Copies a device join result to host arrays of at least result->size
+ * entries. */
+int SjhipJoinResultCopyToHost(const struct SjhipJoinResult* result, uint32_t* build_indices,
+                              uint32_t* probe_indices);
+
+/* Builds the index from the refiner's decoded build geometry, with rects
+ * computed on the device under the operator's semantics (points: nearest
+ * FP32, others: outward FP32).  Device-resident counterpart of the host
+ * rect marshalling in rust/sedona-spatial-join-gpu/src/index/
+ * gpu_spatial_index_builder.rs:193-259 followed by push_build /
+ * finish_building. */
+int SjhipIndexBuildFromRefiner(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner* refiner);
+
+/* push_build for an Arrow array whose buffers are in ROCm device memory
+ * (or CPU memory), decoded on the device; same contract as push_build
+ * (gpuspatial_c.h:168-174). */
+int SjhipRefinerPushBuildDevice(struct SedonaSpatialRefiner* refiner, const struct ArrowSchema* schema,
+                                const struct ArrowDeviceArray* array);
+
+/* Per-row FP32 rects of a WKB column computed on the device: the GPU form
+ * of the operator's EvaluatedGeometryArray bounds
+ * (rust/sedona-spatial-join/src/operand_evaluator.rs:182-217 ->
+ * rust/sedona-geometry/src/bounds.rs:240-249 -> Bounds2D::new,
+ * rust/sedona-spatial-join/src/utils/bounds.rs:49-63, 147-162), with the
+ * GPU operator's point special case when point_nearest != 0
+ * (rust/sedona-spatial-join-gpu/src/join_provider.rs:108-140) and the
+ * distance expansion of operand_evaluator.rs:605-607 when
+ * expand_distance >= 0 (pass NaN for none).  Empty and null rows get NaN
+ * x 4, the encoding push_build and probe expect.  out_rects holds
+ * 4 * length floats. */
+int SjhipComputeRects(const struct ArrowSchema* schema, const struct ArrowArray* array, int point_nearest,
+                      double expand_distance, float* out_rects);
+
+/* Error text of the last failed extension call on this thread. */
+const char* SjhipLastError(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPUSPATIAL_HIP_H */

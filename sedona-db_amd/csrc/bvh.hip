@@ -1,0 +1,385 @@
+// bvh.hip -- linear bounding-volume hierarchy over FP32 rects and the
+// filter kernel that walks it.
+//
+// Replaces the reference's OptiX acceleration structure and ray launches
+// (c/sedona-libgpuspatial/libgpuspatial/src/rt_spatial_index.cu:309-476,
+// src/rt/shaders/point_query.cu:30-73, box_query_forward.cu:30-89,
+// box_query_backward.cu:30-83) and, for the oracle, the CPU R-tree
+// (rust/sedona-spatial-join/src/index/default_spatial_index_builder.rs:
+// 164-234).  Semantics kept: a (build, probe) pair is a candidate when
+// their FP32 rects intersect with inclusive bounds; NaN / inverted rects
+// never match.
+//
+// Build: Morton codes (16 bits per axis) of rect centres, a radix sort,
+// then a bottom-up merge: LEAF consecutive sorted rects form a leaf node,
+// FANOUT consecutive nodes form a parent, level by level.  The tree is
+// implicit (children of node i are i*FANOUT .. i*FANOUT+FANOUT-1), so no
+// child pointers are stored and each level is one contiguous float4 run.
+//
+// Filter: one lane per probe rect; the lane's traversal stack (node and
+// remaining-children bitmask per level) lives in LDS; the loop is
+// wavefront-uniform so that candidates found in one step are compacted
+// with a wavefront prefix sum and a single atomic per wavefront.
+#include <hipcub/hipcub.hpp>
+
+#include "sjhip.h"
+
+namespace sjhip {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxLevels = 14;
+
+__device__ __forceinline__ bool rect_ok(const float4& r) {
+  return r.x <= r.z && r.y <= r.w;  // false for NaN
+}
+
+__device__ __forceinline__ bool hit(const float4& a, const float4& b) {
+  return !(b.x > a.z || b.z < a.x || b.y > a.w || b.w < a.y);
+}
+
+__device__ __forceinline__ uint32_t spread16(uint32_t x) {
+  x &= 0xffff;
+  x = (x | (x << 8)) & 0x00ff00ff;
+  x = (x | (x << 4)) & 0x0f0f0f0f;
+  x = (x | (x << 2)) & 0x33333333;
+  x = (x | (x << 1)) & 0x55555555;
+  return x;
+}
+
+// float <-> order-preserving uint
+__device__ __forceinline__ uint32_t f2o(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__host__ __device__ __forceinline__ float o2f(uint32_t u) {
+  uint32_t b = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+  float f;
+  __builtin_memcpy(&f, &b, 4);
+  return f;
+}
+
+// bounds of valid rect centres: [0] minx [1] miny [2] maxx [3] maxy as
+// ordered uints, [4] valid count
+__global__ void k_center_bounds(const float4* r, int64_t n, unsigned int* acc) {
+  __shared__ unsigned int s[5][kBlock];
+  int t = threadIdx.x;
+  unsigned int mnx = 0xffffffffu, mny = 0xffffffffu, mxx = 0, mxy = 0, cnt = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + t; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float4 b = r[i];
+    if (!rect_ok(b)) continue;
+    float cx = 0.5f * b.x + 0.5f * b.z, cy = 0.5f * b.y + 0.5f * b.w;
+    unsigned int ox = f2o(cx), oy = f2o(cy);
+    mnx = min(mnx, ox); mny = min(mny, oy);
+    mxx = max(mxx, ox); mxy = max(mxy, oy);
+    cnt++;
+  }
+  s[0][t] = mnx; s[1][t] = mny; s[2][t] = mxx; s[3][t] = mxy; s[4][t] = cnt;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (t < o) {
+      s[0][t] = min(s[0][t], s[0][t + o]);
+      s[1][t] = min(s[1][t], s[1][t + o]);
+      s[2][t] = max(s[2][t], s[2][t + o]);
+      s[3][t] = max(s[3][t], s[3][t + o]);
+      s[4][t] += s[4][t + o];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    atomicMin(&acc[0], s[0][0]);
+    atomicMin(&acc[1], s[1][0]);
+    atomicMax(&acc[2], s[2][0]);
+    atomicMax(&acc[3], s[3][0]);
+    atomicAdd(&acc[4], s[4][0]);
+  }
+}
+
+__global__ void k_morton(const float4* r, int64_t n, float x0, float y0, float sx, float sy, uint32_t* keys,
+                         uint32_t* ids) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 b = r[i];
+  uint32_t k = 0xffffffffu;
+  if (rect_ok(b)) {
+    float cx = 0.5f * b.x + 0.5f * b.z, cy = 0.5f * b.y + 0.5f * b.w;
+    float fx = (cx - x0) * sx, fy = (cy - y0) * sy;
+    uint32_t qx = (uint32_t)fminf(fmaxf(fx, 0.f), 65535.f);
+    uint32_t qy = (uint32_t)fminf(fmaxf(fy, 0.f), 65535.f);
+    k = spread16(qx) | (spread16(qy) << 1);
+    if (k == 0xffffffffu) k = 0xfffffffeu;
+  }
+  keys[i] = k;
+  ids[i] = (uint32_t)i;
+}
+
+__global__ void k_gather_leaves(const float4* r, const uint32_t* sorted_ids, int64_t n, float4* out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = r[sorted_ids[i]];
+}
+
+__global__ void k_merge(const float4* child, int64_t n_child, int group, float4* parent, int64_t n_parent) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_parent) return;
+  int64_t c0 = i * group, c1 = min(c0 + group, n_child);
+  float4 u = child[c0];
+  for (int64_t c = c0 + 1; c < c1; c++) {
+    float4 b = child[c];
+    u.x = fminf(u.x, b.x); u.y = fminf(u.y, b.y);
+    u.z = fmaxf(u.z, b.z); u.w = fmaxf(u.w, b.w);
+  }
+  parent[i] = u;
+}
+
+struct TreeView {
+  const float4* nodes;
+  const float4* leaf_rects;
+  const uint32_t* leaf_ids;
+  int64_t n_valid;
+  int n_levels;
+  int64_t off[kMaxLevels];
+  int64_t cnt[kMaxLevels];
+};
+
+template <int F, int LEAF>
+__global__ __launch_bounds__(kBlock) void k_filter(TreeView t, const float4* q, int64_t nq, const uint32_t* qorder,
+                                                   uint32_t* ob, uint32_t* op, uint64_t cap,
+                                                   unsigned long long* counter) {
+  __shared__ uint32_t s_node[kMaxLevels + 1][kBlock];
+  __shared__ uint32_t s_mask[kMaxLevels + 1][kBlock];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + tid;
+  bool active = i < nq && t.n_levels > 0;
+  uint32_t qid = 0;
+  float4 box = make_float4(0, 0, 0, 0);
+  const int top = t.n_levels;
+  int lvl = top;
+  if (active) {
+    qid = qorder ? qorder[i] : (uint32_t)i;
+    box = q[qid];
+    if (!rect_ok(box)) active = false;
+  }
+  if (active) {
+    // virtual root: children are all nodes of level top-1 (<= F of them)
+    uint32_t m = 0;
+    const float4* nd = t.nodes + t.off[top - 1];
+    int64_t nc = t.cnt[top - 1];
+    for (int c = 0; c < F; c++)
+      if (c < nc && hit(nd[c], box)) m |= 1u << c;
+    s_node[top][tid] = 0;
+    s_mask[top][tid] = m;
+  }
+  while (__any(active)) {
+    uint32_t hits = 0;
+    int64_t hbase = 0;
+    if (active) {
+      while (lvl <= top && s_mask[lvl][tid] == 0) lvl++;
+      if (lvl > top) {
+        active = false;
+      } else {
+        uint32_t m = s_mask[lvl][tid];
+        int c = __ffs(m) - 1;
+        s_mask[lvl][tid] = m & (m - 1);
+        int64_t child = (int64_t)s_node[lvl][tid] * F + c;  // index at level lvl-1
+        if (lvl - 1 == 0) {
+          hbase = child * LEAF;
+          int64_t e = min(hbase + LEAF, t.n_valid);
+          const float4* lr = t.leaf_rects + hbase;
+#pragma unroll
+          for (int k = 0; k < LEAF; k++)
+            if (hbase + k < e && hit(lr[k], box)) hits |= 1u << k;
+        } else {
+          int cl = lvl - 2;  // level of the child's children
+          const float4* nd = t.nodes + t.off[cl];
+          int64_t c0 = child * F;
+          int64_t nc = t.cnt[cl] - c0;
+          uint32_t mm = 0;
+#pragma unroll
+          for (int k = 0; k < F; k++)
+            if (k < nc && hit(nd[c0 + k], box)) mm |= 1u << k;
+          lvl = lvl - 1;
+          s_node[lvl][tid] = (uint32_t)child;
+          s_mask[lvl][tid] = mm;
+        }
+      }
+    }
+    // wavefront compaction of this step's candidates
+    uint32_t cnt = __popc(hits);
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      uint32_t v = __shfl_up(incl, o);
+      if (lane >= o) incl += v;
+    }
+    uint32_t total = __shfl(incl, 63);
+    unsigned long long base = 0;
+    if (total) {
+      if (lane == 0) base = atomicAdd(counter, (unsigned long long)total);
+      base = __shfl(base, 0);
+      unsigned long long w = base + incl - cnt;
+      while (hits) {
+        int k = __ffs(hits) - 1;
+        hits &= hits - 1;
+        if (w < cap) {
+          ob[w] = t.leaf_ids[hbase + k];
+          op[w] = qid;
+        }
+        w++;
+      }
+    }
+  }
+}
+
+unsigned int blocks_for(int64_t n) { return (unsigned int)((n + kBlock - 1) / kBlock); }
+
+void center_bounds(const float4* r, int64_t n, float bb[4], int64_t* n_valid, hipStream_t s) {
+  DevArray<unsigned int> acc(5, s);
+  unsigned int init[5] = {0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+  SJ_CHECK(hipMemcpyAsync(acc.p, init, sizeof(init), hipMemcpyHostToDevice, s));
+  unsigned int grid = (unsigned int)std::min<int64_t>(4096, std::max<int64_t>(1, blocks_for(n)));
+  if (n > 0) hipLaunchKernelGGL(k_center_bounds, grid, kBlock, 0, s, r, n, acc.p);
+  unsigned int h[5];
+  SJ_CHECK(hipMemcpyAsync(h, acc.p, sizeof(h), hipMemcpyDeviceToHost, s));
+  SJ_CHECK(hipStreamSynchronize(s));
+  *n_valid = h[4];
+  for (int k = 0; k < 4; k++) bb[k] = h[4] ? o2f(h[k]) : 0.f;
+}
+
+void sort_keys(DevArray<uint32_t>& keys, DevArray<uint32_t>& ids, int64_t n, hipStream_t s) {
+  DevArray<uint32_t> k2(n, s), i2(n, s);
+  size_t tmp = 0;
+  SJ_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, keys.p, k2.p, ids.p, i2.p, (int)n, 0, 32, s));
+  DevArray<uint8_t> t(tmp + 16, s);
+  SJ_CHECK(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, keys.p, k2.p, ids.p, i2.p, (int)n, 0, 32, s));
+  keys = std::move(k2);
+  ids = std::move(i2);
+}
+
+}  // namespace
+
+void build_rect_tree(const float4* rects, int64_t n, RectTree& t, hipStream_t s) {
+  t.level_off.clear();
+  t.level_cnt.clear();
+  t.n_levels = 0;
+  float bb[4];
+  center_bounds(rects, n, bb, &t.n_valid, s);
+  if (t.n_valid == 0) {
+    t.nodes.alloc(1, s);
+    t.leaf_rects.alloc(1, s);
+    t.leaf_ids.alloc(1, s);
+    return;
+  }
+  DevArray<uint32_t> keys(n, s), ids(n, s);
+  float sx = bb[2] > bb[0] ? 65535.f / (bb[2] - bb[0]) : 0.f;
+  float sy = bb[3] > bb[1] ? 65535.f / (bb[3] - bb[1]) : 0.f;
+  hipLaunchKernelGGL(k_morton, blocks_for(n), kBlock, 0, s, rects, n, bb[0], bb[1], sx, sy, keys.p, ids.p);
+  sort_keys(keys, ids, n, s);
+  const int64_t nv = t.n_valid;
+  t.leaf_rects.alloc(nv, s);
+  hipLaunchKernelGGL(k_gather_leaves, blocks_for(nv), kBlock, 0, s, rects, ids.p, nv, t.leaf_rects.p);
+  t.leaf_ids.alloc(nv, s);
+  SJ_CHECK(hipMemcpyAsync(t.leaf_ids.p, ids.p, nv * 4, hipMemcpyDeviceToDevice, s));
+  t.leaf_keys.alloc(nv, s);
+  SJ_CHECK(hipMemcpyAsync(t.leaf_keys.p, keys.p, nv * 4, hipMemcpyDeviceToDevice, s));
+  for (int k = 0; k < 4; k++) t.bb[k] = bb[k];
+  // level sizes
+  std::vector<int64_t> cnt;
+  int64_t c = (nv + RectTree::LEAF - 1) / RectTree::LEAF;
+  cnt.push_back(c);
+  while (c > RectTree::FANOUT) {
+    c = (c + RectTree::FANOUT - 1) / RectTree::FANOUT;
+    cnt.push_back(c);
+  }
+  if ((int)cnt.size() > kMaxLevels - 1) throw Error("rect tree too deep");
+  int64_t tot = 0;
+  for (int64_t x : cnt) {
+    t.level_off.push_back(tot);
+    tot += x;
+  }
+  t.level_cnt = cnt;
+  t.n_levels = (int)cnt.size();
+  t.nodes.alloc(tot, s);
+  hipLaunchKernelGGL(k_merge, blocks_for(cnt[0]), kBlock, 0, s, t.leaf_rects.p, nv, RectTree::LEAF,
+                     t.nodes.p + t.level_off[0], cnt[0]);
+  for (size_t l = 1; l < cnt.size(); l++)
+    hipLaunchKernelGGL(k_merge, blocks_for(cnt[l]), kBlock, 0, s, t.nodes.p + t.level_off[l - 1], cnt[l - 1],
+                       RectTree::FANOUT, t.nodes.p + t.level_off[l], cnt[l]);
+  SJ_CHECK(hipGetLastError());
+  SJ_CHECK(hipStreamSynchronize(s));
+}
+
+uint64_t filter_pairs(const RectTree& t, const float4* q, int64_t nq, const uint32_t* qorder, PairBuf& pb,
+                      hipStream_t s) {
+  if (pb.counter.n < 1) pb.counter.alloc(1, s);
+  if (pb.b.n == 0) {
+    size_t cap0 = std::max<int64_t>(1 << 16, nq * 4);
+    pb.b.alloc(cap0, s);
+    pb.p.alloc(cap0, s);
+  }
+  if (nq == 0 || t.n_levels == 0) return 0;
+  TreeView v{};
+  v.nodes = t.nodes.p;
+  v.leaf_rects = t.leaf_rects.p;
+  v.leaf_ids = t.leaf_ids.p;
+  v.n_valid = t.n_valid;
+  v.n_levels = t.n_levels;
+  for (int l = 0; l < t.n_levels; l++) {
+    v.off[l] = t.level_off[l];
+    v.cnt[l] = t.level_cnt[l];
+  }
+  for (int attempt = 0; attempt < 2; attempt++) {
+    SJ_CHECK(hipMemsetAsync(pb.counter.p, 0, 8, s));
+    hipLaunchKernelGGL((k_filter<RectTree::FANOUT, RectTree::LEAF>), blocks_for(nq), kBlock, 0, s, v, q, nq, qorder,
+                       pb.b.p, pb.p.p, (uint64_t)pb.b.n, pb.counter.p);
+    SJ_CHECK(hipGetLastError());
+    unsigned long long total = 0;
+    SJ_CHECK(hipMemcpyAsync(&total, pb.counter.p, 8, hipMemcpyDeviceToHost, s));
+    SJ_CHECK(hipStreamSynchronize(s));
+    if (total <= pb.b.n) return total;
+    size_t cap = (size_t)(total + total / 8 + 1024);
+    pb.b.alloc(cap, s);
+    pb.p.alloc(cap, s);
+  }
+  throw Error("filter output did not fit after resizing");
+}
+
+namespace {
+__global__ void k_query_keys(const float4* r, int64_t n, float x0, float y0, float sx, float sy, uint32_t* keys,
+                             uint32_t* ids) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 b = r[i];
+  uint32_t k = 0xffffffffu;
+  if (rect_ok(b)) {
+    float cx = 0.5f * b.x + 0.5f * b.z, cy = 0.5f * b.y + 0.5f * b.w;
+    uint32_t qx = (uint32_t)fminf(fmaxf((cx - x0) * sx, 0.f), 65535.f);
+    uint32_t qy = (uint32_t)fminf(fmaxf((cy - y0) * sy, 0.f), 65535.f);
+    k = spread16(qx) | (spread16(qy) << 1);
+    if (k == 0xffffffffu) k = 0xfffffffeu;
+  }
+  keys[i] = k;
+  ids[i] = (uint32_t)i;
+}
+}  // namespace
+
+int64_t morton_order(const float4* rects, int64_t n, DevArray<uint32_t>& order, hipStream_t s) {
+  float bb[4];
+  int64_t nv;
+  center_bounds(rects, n, bb, &nv, s);
+  if (nv == 0) {
+    order.alloc(1, s);
+    return 0;
+  }
+  DevArray<uint32_t> keys(n, s), ids(n, s);
+  float sx = bb[2] > bb[0] ? 65535.f / (bb[2] - bb[0]) : 0.f;
+  float sy = bb[3] > bb[1] ? 65535.f / (bb[3] - bb[1]) : 0.f;
+  hipLaunchKernelGGL(k_query_keys, blocks_for(n), kBlock, 0, s, rects, n, bb[0], bb[1], sx, sy, keys.p, ids.p);
+  sort_keys(keys, ids, n, s);
+  order = std::move(ids);
+  return nv;
+}
+
+}  // namespace sjhip

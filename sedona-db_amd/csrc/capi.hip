@@ -1,0 +1,735 @@
+// capi.hip -- the C ABI exporter: runtime, FP32 rect index, refiner and the
+// extension entry points declared in include/gpuspatial_hip.h.
+//
+// Error convention of the reference exporter
+// (c/sedona-libgpuspatial/libgpuspatial/src/gpuspatial_c.cc:45-58): every
+// entry point returns 0, or EINVAL with the message kept in the object
+// (probe errors in the context, gpuspatial_c.cc:191-214).  Every entry
+// point makes the runtime's device current (gpuspatial_c.cc:247-253).
+#include <errno.h>
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <mutex>
+
+#include "../../include/gpuspatial_hip.h"
+#include "sjhip.h"
+
+namespace sjhip {
+
+namespace {
+
+thread_local std::string t_last_error;
+
+template <typename F>
+int guarded(std::string* err, F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const Error& e) {
+    if (err) *err = e.what();
+    t_last_error = e.what();
+    return e.code ? e.code : EINVAL;
+  } catch (const std::exception& e) {
+    if (err) *err = e.what();
+    t_last_error = e.what();
+    return EINVAL;
+  } catch (...) {
+    if (err) *err = "unknown error";
+    t_last_error = "unknown error";
+    return EINVAL;
+  }
+}
+
+struct Runtime {
+  int device_id = 0;
+  bool initialized = false;
+  std::string last_error;
+};
+
+Runtime* runtime_of(const GpuSpatialRuntime* rt) {
+  if (!rt || !rt->private_data) throw Error("runtime is null or released");
+  Runtime* r = (Runtime*)rt->private_data;
+  if (!r->initialized) throw Error("runtime is not initialized");
+  return r;
+}
+
+void use_device(const Runtime* r) { SJ_CHECK(hipSetDevice(r->device_id)); }
+
+// A stream owned for the duration of one call.
+struct ScopedStream {
+  hipStream_t s = nullptr;
+  ScopedStream() { SJ_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+  ~ScopedStream() {
+    if (s) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Runtime
+
+int rt_init(GpuSpatialRuntime* self, GpuSpatialRuntimeConfig* cfg) {
+  Runtime* r = (Runtime*)self->private_data;
+  return guarded(&r->last_error, [&] {
+    if (!cfg) throw Error("null runtime config");
+    if (cfg->ptx_root && cfg->ptx_root[0]) {
+      struct stat st;
+      if (stat(cfg->ptx_root, &st) != 0)
+        throw Error(std::string("cannot open shader directory '") + cfg->ptx_root + "': " + std::strerror(errno));
+    }
+    int n = 0;
+    SJ_CHECK(hipGetDeviceCount(&n));
+    if (cfg->device_id < 0 || cfg->device_id >= n)
+      throw Error("device_id " + std::to_string(cfg->device_id) + " out of range (" + std::to_string(n) + " devices)");
+    r->device_id = cfg->device_id;
+    SJ_CHECK(hipSetDevice(r->device_id));
+    if (cfg->use_cuda_memory_pool) {
+      hipMemPool_t pool;
+      SJ_CHECK(hipDeviceGetDefaultMemPool(&pool, r->device_id));
+      size_t free_b = 0, total_b = 0;
+      SJ_CHECK(hipMemGetInfo(&free_b, &total_b));
+      int pct = std::min(100, std::max(0, cfg->cuda_memory_pool_init_precent));
+      uint64_t keep = (uint64_t)((double)total_b * pct / 100.0);
+      SJ_CHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+    }
+    r->initialized = true;
+  });
+}
+
+void rt_release(GpuSpatialRuntime* self) {
+  if (!self) return;
+  delete (Runtime*)self->private_data;
+  self->private_data = nullptr;
+}
+
+const char* rt_err(GpuSpatialRuntime* self) {
+  return self && self->private_data ? ((Runtime*)self->private_data)->last_error.c_str() : "runtime released";
+}
+
+// ---------------------------------------------------------------------------
+// FP32 rect index
+
+struct IndexImpl {
+  Runtime* rt = nullptr;
+  std::string last_error;
+  std::vector<float> host_rects;
+  hipStream_t stream = nullptr;
+  DevArray<float4> rects;
+  RectTree tree;
+  bool built = false;
+  std::mutex mu;
+};
+
+struct IndexCtx {
+  std::string last_error;
+  std::vector<uint32_t> hb, hp;
+};
+
+IndexImpl* index_of(SedonaFloatIndex2D* self) {
+  if (!self || !self->private_data) throw Error("index is null or released");
+  return (IndexImpl*)self->private_data;
+}
+
+int ix_clear(SedonaFloatIndex2D* self) {
+  IndexImpl* ix = (IndexImpl*)self->private_data;
+  return guarded(&ix->last_error, [&] {
+    use_device(ix->rt);
+    ix->host_rects.clear();
+    ix->tree = RectTree();
+    ix->rects.release();
+    ix->built = false;
+  });
+}
+
+void ix_create_context(SedonaSpatialIndexContext* ctx) { ctx->private_data = new IndexCtx(); }
+void ix_destroy_context(SedonaSpatialIndexContext* ctx) {
+  delete (IndexCtx*)ctx->private_data;
+  ctx->private_data = nullptr;
+}
+
+int ix_push_build(SedonaFloatIndex2D* self, const float* buf, uint32_t n) {
+  IndexImpl* ix = (IndexImpl*)self->private_data;
+  return guarded(&ix->last_error, [&] {
+    if (ix->built) throw Error("push_build after finish_building; call clear first");
+    if (n && !buf) throw Error("null rect buffer");
+    ix->host_rects.insert(ix->host_rects.end(), buf, buf + 4 * (size_t)n);
+  });
+}
+
+void index_finish(IndexImpl* ix) {
+  use_device(ix->rt);
+  int64_t n = (int64_t)(ix->host_rects.size() / 4);
+  ix->rects.alloc(n > 0 ? n : 1, ix->stream);
+  if (n)
+    SJ_CHECK(hipMemcpyAsync(ix->rects.p, ix->host_rects.data(), n * sizeof(float4), hipMemcpyHostToDevice,
+                            ix->stream));
+  build_rect_tree(ix->rects.p, n, ix->tree, ix->stream);
+  SJ_CHECK(hipStreamSynchronize(ix->stream));
+  ix->built = true;
+}
+
+int ix_finish(SedonaFloatIndex2D* self) {
+  IndexImpl* ix = (IndexImpl*)self->private_data;
+  return guarded(&ix->last_error, [&] { index_finish(ix); });
+}
+
+int ix_probe(SedonaFloatIndex2D* self, SedonaSpatialIndexContext* context, const float* buf, uint32_t n,
+             int (*cb)(const uint32_t*, const uint32_t*, uint32_t, void*), void* ud) {
+  IndexImpl* ix = (IndexImpl*)self->private_data;
+  IndexCtx* cx = context ? (IndexCtx*)context->private_data : nullptr;
+  std::string* err = cx ? &cx->last_error : &ix->last_error;
+  int cb_rc = 0;
+  int rc = guarded(err, [&] {
+    if (!ix->built) throw Error("probe before finish_building");
+    if (!cb) throw Error("null probe callback");
+    use_device(ix->rt);
+    if (n == 0) return;
+    ScopedStream ss;
+    DevArray<float4> q(n, ss.s);
+    SJ_CHECK(hipMemcpyAsync(q.p, buf, (size_t)n * sizeof(float4), hipMemcpyHostToDevice, ss.s));
+    DevArray<uint32_t> order;
+    int64_t nv = morton_order(q.p, n, order, ss.s);
+    PairBuf pb;
+    uint64_t cnt = filter_pairs(ix->tree, q.p, nv, order.p, pb, ss.s);
+    std::vector<uint32_t> hb(cnt), hp(cnt);
+    if (cnt) {
+      SJ_CHECK(hipMemcpyAsync(hb.data(), pb.b.p, cnt * 4, hipMemcpyDeviceToHost, ss.s));
+      SJ_CHECK(hipMemcpyAsync(hp.data(), pb.p.p, cnt * 4, hipMemcpyDeviceToHost, ss.s));
+    }
+    SJ_CHECK(hipStreamSynchronize(ss.s));
+    // deliver in chunks that fit the callback's u32 length
+    const uint64_t chunk = 1ull << 30;
+    for (uint64_t off = 0; off < cnt || (cnt == 0 && off == 0); off += chunk) {
+      uint32_t len = (uint32_t)std::min<uint64_t>(chunk, cnt - off);
+      cb_rc = cb(hb.data() + off, hp.data() + off, len, ud);
+      if (cb_rc != 0 || cnt == 0) break;
+    }
+  });
+  if (rc) return rc;
+  return cb_rc;
+}
+
+const char* ix_err(SedonaFloatIndex2D* self) {
+  return self && self->private_data ? ((IndexImpl*)self->private_data)->last_error.c_str() : "index released";
+}
+const char* ix_ctx_err(SedonaSpatialIndexContext* ctx) {
+  return ctx && ctx->private_data ? ((IndexCtx*)ctx->private_data)->last_error.c_str() : "context released";
+}
+
+void ix_release(SedonaFloatIndex2D* self) {
+  if (!self || !self->private_data) return;
+  IndexImpl* ix = (IndexImpl*)self->private_data;
+  try {
+    use_device(ix->rt);
+    ix->tree = RectTree();
+    ix->rects.release();
+    if (ix->stream) {
+      (void)hipStreamSynchronize(ix->stream);
+      (void)hipStreamDestroy(ix->stream);
+    }
+  } catch (...) {
+  }
+  delete ix;
+  self->private_data = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Refiner
+
+struct RefinerImpl {
+  Runtime* rt = nullptr;
+  std::string last_error;
+  std::string build_format;
+  hipStream_t stream = nullptr;
+  // pushed build batches, decoded on the device as they arrive
+  std::vector<GeomColumn> batches;
+  GeomColumn build;
+  RingIndex ring_index;
+  bool built = false;
+  std::mutex mu;
+};
+
+RefinerImpl* refiner_of(SedonaSpatialRefiner* self) {
+  if (!self || !self->private_data) throw Error("refiner is null or released");
+  return (RefinerImpl*)self->private_data;
+}
+
+int rf_clear(SedonaSpatialRefiner* self) {
+  RefinerImpl* rf = (RefinerImpl*)self->private_data;
+  return guarded(&rf->last_error, [&] {
+    use_device(rf->rt);
+    rf->batches.clear();
+    rf->build = GeomColumn();
+    rf->ring_index = RingIndex();
+    rf->built = false;
+  });
+}
+
+int rf_init_schema(SedonaSpatialRefiner* self, const ArrowSchema* schema) {
+  RefinerImpl* rf = (RefinerImpl*)self->private_data;
+  return guarded(&rf->last_error, [&] {
+    if (!schema || !schema->format) throw Error("null build schema");
+    std::string f = schema->format;
+    if (f != "z" && f != "Z" && f != "vz")
+      throw Error("unsupported build storage type '" + f + "' (expected Binary, LargeBinary or BinaryView WKB)");
+    rf->build_format = f;
+  });
+}
+
+void refiner_push(RefinerImpl* rf, const WkbInput& in) {
+  use_device(rf->rt);
+  if (rf->built) throw Error("push_build after finish_building; call clear first");
+  GeomColumn g;
+  decode_wkb(in, g, rf->stream);
+  rf->batches.push_back(std::move(g));
+}
+
+int rf_push_build(SedonaSpatialRefiner* self, const ArrowArray* array) {
+  RefinerImpl* rf = (RefinerImpl*)self->private_data;
+  return guarded(&rf->last_error, [&] {
+    if (rf->build_format.empty()) throw Error("init_build_schema must be called before push_build");
+    ArrowSchema sch{};
+    sch.format = rf->build_format.c_str();
+    refiner_push(rf, wkb_input_from_arrow(&sch, array, false));
+  });
+}
+
+// Concatenation of decoded batches (build ids are global in push order).
+__global__ void k_shift_copy(const int64_t* in, int64_t n, int64_t delta, int64_t* out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i] + delta;
+}
+__global__ void k_iota64(int64_t n, int64_t base, int64_t* out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = base + i;
+}
+
+void concat_columns(std::vector<GeomColumn>& parts, GeomColumn& out, hipStream_t s) {
+  if (parts.size() == 1) {
+    out = std::move(parts[0]);
+    parts.clear();
+    return;
+  }
+  out = GeomColumn();
+  bool all_points = true;
+  for (auto& g : parts) {
+    out.n_rows += g.n_rows;
+    all_points &= g.points_only;
+    out.type_mask |= g.type_mask;
+  }
+  out.type.alloc(out.n_rows > 0 ? out.n_rows : 1, s);
+  int64_t r = 0;
+  for (auto& g : parts) {
+    if (g.n_rows) SJ_CHECK(hipMemcpyAsync(out.type.p + r, g.type.p, g.n_rows, hipMemcpyDeviceToDevice, s));
+    r += g.n_rows;
+  }
+  if (all_points) {
+    out.points_only = true;
+    out.n_parts = out.n_rings = out.n_pts = out.n_rows;
+    out.xy.alloc(out.n_rows > 0 ? out.n_rows : 1, s);
+    r = 0;
+    for (auto& g : parts) {
+      if (g.n_rows) SJ_CHECK(hipMemcpyAsync(out.xy.p + r, g.xy.p, g.n_rows * 16, hipMemcpyDeviceToDevice, s));
+      r += g.n_rows;
+    }
+  } else {
+    // points-only batches are viewed as one part / ring / point per row;
+    // their NaN (empty) rows keep one NaN point, which every predicate
+    // treats as empty
+    for (auto& g : parts) {
+      out.n_parts += g.n_parts;
+      out.n_rings += g.n_rings;
+      out.n_pts += g.n_pts;
+    }
+    out.part_off.alloc(out.n_rows + 1, s);
+    out.ring_off.alloc(out.n_parts + 1, s);
+    out.pt_off.alloc(out.n_rings + 1, s);
+    out.xy.alloc(out.n_pts > 0 ? out.n_pts : 1, s);
+    int64_t rr = 0, pp = 0, gg = 0, vv = 0;
+    for (auto& g : parts) {
+      unsigned int b;
+      if (g.points_only) {
+        b = (unsigned int)((g.n_rows + 255) / 256);
+        if (g.n_rows) {
+          hipLaunchKernelGGL(k_iota64, b, 256, 0, s, g.n_rows, pp, out.part_off.p + gg);
+          hipLaunchKernelGGL(k_iota64, b, 256, 0, s, g.n_rows, rr, out.ring_off.p + pp);
+          hipLaunchKernelGGL(k_iota64, b, 256, 0, s, g.n_rows, vv, out.pt_off.p + rr);
+        }
+      } else {
+        if (g.n_rows)
+          hipLaunchKernelGGL(k_shift_copy, (unsigned int)((g.n_rows + 255) / 256), 256, 0, s, g.part_off.p, g.n_rows,
+                             pp, out.part_off.p + gg);
+        if (g.n_parts)
+          hipLaunchKernelGGL(k_shift_copy, (unsigned int)((g.n_parts + 255) / 256), 256, 0, s, g.ring_off.p,
+                             g.n_parts, rr, out.ring_off.p + pp);
+        if (g.n_rings)
+          hipLaunchKernelGGL(k_shift_copy, (unsigned int)((g.n_rings + 255) / 256), 256, 0, s, g.pt_off.p, g.n_rings,
+                             vv, out.pt_off.p + rr);
+      }
+      if (g.n_pts) SJ_CHECK(hipMemcpyAsync(out.xy.p + vv, g.xy.p, g.n_pts * 16, hipMemcpyDeviceToDevice, s));
+      gg += g.n_rows;
+      pp += g.n_parts;
+      rr += g.n_rings;
+      vv += g.n_pts;
+    }
+    SJ_CHECK(hipMemcpyAsync(out.part_off.p + out.n_rows, &out.n_parts, 8, hipMemcpyHostToDevice, s));
+    SJ_CHECK(hipMemcpyAsync(out.ring_off.p + out.n_parts, &out.n_rings, 8, hipMemcpyHostToDevice, s));
+    SJ_CHECK(hipMemcpyAsync(out.pt_off.p + out.n_rings, &out.n_pts, 8, hipMemcpyHostToDevice, s));
+  }
+  SJ_CHECK(hipGetLastError());
+  SJ_CHECK(hipStreamSynchronize(s));
+  parts.clear();
+}
+
+void refiner_finish(RefinerImpl* rf) {
+  use_device(rf->rt);
+  if (rf->batches.empty()) {
+    rf->build = GeomColumn();
+    rf->build.type.alloc(1, rf->stream);
+    rf->build.points_only = true;
+    rf->build.xy.alloc(1, rf->stream);
+  } else {
+    concat_columns(rf->batches, rf->build, rf->stream);
+  }
+  RingIndexParams prm;
+  build_ring_index(rf->build, rf->ring_index, prm, rf->stream);
+  SJ_CHECK(hipStreamSynchronize(rf->stream));
+  rf->built = true;
+}
+
+int rf_finish(SedonaSpatialRefiner* self) {
+  RefinerImpl* rf = (RefinerImpl*)self->private_data;
+  return guarded(&rf->last_error, [&] { refiner_finish(rf); });
+}
+
+// Shared body of refine / refine-distance over host candidate arrays.
+void refine_host(RefinerImpl* rf, const ArrowSchema* probe_schema, const ArrowArray* probe_array, int pred,
+                 double distance, uint32_t* bidx, uint32_t* pidx, uint32_t n, uint32_t* new_n) {
+  if (!rf->built) throw Error("refine before finish_building");
+  if (!new_n) throw Error("null new_indices_size");
+  use_device(rf->rt);
+  if (n == 0) {
+    *new_n = 0;
+    return;
+  }
+  if (!bidx || !pidx) throw Error("null candidate index arrays");
+  ScopedStream ss;
+  hipStream_t s = ss.s;
+  GeomColumn probe;
+  decode_wkb(wkb_input_from_arrow(probe_schema, probe_array, false), probe, s);
+  // range checks on the host side (cheap, and they keep kernels in bounds)
+  for (uint32_t i = 0; i < n; i++) {
+    if (bidx[i] >= (uint64_t)rf->build.n_rows) throw Error("build index out of range: " + std::to_string(bidx[i]));
+    if (pidx[i] >= (uint64_t)probe.n_rows) throw Error("probe index out of range: " + std::to_string(pidx[i]));
+  }
+  if (pred != 100) check_relation_types(rf->build.type_mask, probe.type_mask, pred);
+  DevArray<uint32_t> cb(n, s), cp(n, s);
+  SJ_CHECK(hipMemcpyAsync(cb.p, bidx, (size_t)n * 4, hipMemcpyHostToDevice, s));
+  SJ_CHECK(hipMemcpyAsync(cp.p, pidx, (size_t)n * 4, hipMemcpyHostToDevice, s));
+  PairBuf out;
+  uint64_t kept;
+  if (pred == 100)
+    kept = refine_distance(rf->build, probe, distance, cb.p, cp.p, n, out, s);
+  else
+    kept = refine_relation(rf->build, rf->ring_index, probe, pred, cb.p, cp.p, n, 0, out, nullptr, s);
+  sort_pairs(out.b.p, out.p.p, kept, s);
+  if (kept) {
+    SJ_CHECK(hipMemcpyAsync(bidx, out.b.p, kept * 4, hipMemcpyDeviceToHost, s));
+    SJ_CHECK(hipMemcpyAsync(pidx, out.p.p, kept * 4, hipMemcpyDeviceToHost, s));
+  }
+  SJ_CHECK(hipStreamSynchronize(s));
+  *new_n = (uint32_t)kept;
+}
+
+int rf_refine(SedonaSpatialRefiner* self, const ArrowSchema* probe_schema, const ArrowArray* probe_array,
+              SedonaSpatialRelationPredicate pred, uint32_t* bidx, uint32_t* pidx, uint32_t n, uint32_t* new_n) {
+  RefinerImpl* rf = (RefinerImpl*)self->private_data;
+  std::string err;
+  int rc = guarded(&err, [&] { refine_host(rf, probe_schema, probe_array, (int)pred, 0.0, bidx, pidx, n, new_n); });
+  if (rc) {
+    std::lock_guard<std::mutex> g(rf->mu);
+    rf->last_error = err;
+  }
+  return rc;
+}
+
+const char* rf_err(SedonaSpatialRefiner* self) {
+  return self && self->private_data ? ((RefinerImpl*)self->private_data)->last_error.c_str() : "refiner released";
+}
+
+void rf_release(SedonaSpatialRefiner* self) {
+  if (!self || !self->private_data) return;
+  RefinerImpl* rf = (RefinerImpl*)self->private_data;
+  try {
+    use_device(rf->rt);
+    rf->batches.clear();
+    rf->build = GeomColumn();
+    rf->ring_index = RingIndex();
+    if (rf->stream) {
+      (void)hipStreamSynchronize(rf->stream);
+      (void)hipStreamDestroy(rf->stream);
+    }
+  } catch (...) {
+  }
+  delete rf;
+  self->private_data = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Device join result
+
+struct JoinResultImpl {
+  PairBuf pairs;
+  int device_id = 0;
+};
+
+void jr_release(SjhipJoinResult* r) {
+  if (!r || !r->private_data) return;
+  JoinResultImpl* j = (JoinResultImpl*)r->private_data;
+  (void)hipSetDevice(j->device_id);
+  delete j;
+  r->private_data = nullptr;
+  r->build_indices = r->probe_indices = nullptr;
+  r->size = 0;
+}
+
+}  // namespace
+
+// Fused device join used by the extension entry point and the request loop.
+struct JoinEngine {
+  static void run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const SjhipJoinOptions& o, SjhipJoinResult* out);
+};
+
+void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const SjhipJoinOptions& o,
+                     SjhipJoinResult* out) {
+  if (!ix->built) throw Error("index is not built");
+  if (!rf->built) throw Error("refiner is not built");
+  use_device(rf->rt);
+  std::unique_ptr<ScopedStream> own;
+  hipStream_t s = (hipStream_t)o.stream;
+  if (!s) {
+    own.reset(new ScopedStream());
+    s = own->s;
+  }
+  GeomColumn probe;
+  decode_wkb(in, probe, s);
+  const bool dwithin = o.predicate == 100;
+  if (!dwithin) check_relation_types(rf->build.type_mask, probe.type_mask, o.predicate);
+  DevArray<float4> q;
+  // probe rects: GPU operator semantics (points nearest) for relation
+  // predicates; outward + distance expansion for DWithin
+  compute_rects(probe, nullptr, !dwithin, dwithin ? o.distance : NAN, q, s);
+  DevArray<uint32_t> order;
+  int64_t nq = probe.n_rows;
+  const uint32_t* ord = nullptr;
+  if (o.sort_probes) {
+    nq = morton_order(q.p, probe.n_rows, order, s);
+    ord = order.p;
+  }
+  PairBuf cand;
+  uint64_t nc = filter_pairs(ix->tree, q.p, nq, ord, cand, s);
+  std::unique_ptr<JoinResultImpl> jr(new JoinResultImpl());
+  jr->device_id = rf->rt->device_id;
+  RefineStats st;
+  uint64_t kept;
+  if (dwithin)
+    kept = refine_distance(rf->build, probe, o.distance, cand.b.p, cand.p.p, nc, jr->pairs, s);
+  else
+    kept = refine_relation(rf->build, rf->ring_index, probe, o.predicate, cand.b.p, cand.p.p, nc, o.precision,
+                           jr->pairs, &st, s);
+  if (o.sort_output) sort_pairs(jr->pairs.b.p, jr->pairs.p.p, kept, s);
+  SJ_CHECK(hipStreamSynchronize(s));
+  out->build_indices = jr->pairs.b.p;
+  out->probe_indices = jr->pairs.p.p;
+  out->size = kept;
+  out->candidates = nc;
+  out->segment_tests = st.segment_tests;
+  out->fallback_tests = st.fallback_tests;
+  out->release = jr_release;
+  out->private_data = jr.release();
+}
+
+}  // namespace sjhip
+
+using namespace sjhip;
+
+extern "C" {
+
+void GpuSpatialRuntimeCreate(struct GpuSpatialRuntime* runtime) {
+  runtime->init = rt_init;
+  runtime->release = rt_release;
+  runtime->get_last_error = rt_err;
+  runtime->private_data = new Runtime();
+}
+
+int GpuSpatialIndexFloat2DCreate(struct SedonaFloatIndex2D* index, const struct GpuSpatialIndexConfig* config) {
+  std::string err;
+  return guarded(&err, [&] {
+    if (!index || !config) throw Error("null index or config");
+    Runtime* rt = runtime_of(config->runtime);
+    use_device(rt);
+    IndexImpl* ix = new IndexImpl();
+    ix->rt = rt;
+    SJ_CHECK(hipStreamCreateWithFlags(&ix->stream, hipStreamNonBlocking));
+    index->clear = ix_clear;
+    index->create_context = ix_create_context;
+    index->destroy_context = ix_destroy_context;
+    index->push_build = ix_push_build;
+    index->finish_building = ix_finish;
+    index->probe = ix_probe;
+    index->get_last_error = ix_err;
+    index->context_get_last_error = ix_ctx_err;
+    index->release = ix_release;
+    index->private_data = ix;
+  });
+}
+
+int GpuSpatialRefinerCreate(struct SedonaSpatialRefiner* refiner, const struct GpuSpatialRefinerConfig* config) {
+  std::string err;
+  return guarded(&err, [&] {
+    if (!refiner || !config) throw Error("null refiner or config");
+    Runtime* rt = runtime_of(config->runtime);
+    use_device(rt);
+    RefinerImpl* rf = new RefinerImpl();
+    rf->rt = rt;
+    SJ_CHECK(hipStreamCreateWithFlags(&rf->stream, hipStreamNonBlocking));
+    refiner->clear = rf_clear;
+    refiner->init_build_schema = rf_init_schema;
+    refiner->push_build = rf_push_build;
+    refiner->finish_building = rf_finish;
+    refiner->refine = rf_refine;
+    refiner->get_last_error = rf_err;
+    refiner->release = rf_release;
+    refiner->private_data = rf;
+  });
+}
+
+int SjhipExtVersion(void) { return SJHIP_EXT_VERSION; }
+
+const char* SjhipLastError(void) { return t_last_error.c_str(); }
+
+int SjhipRefinerRefineDistance(struct SedonaSpatialRefiner* self, const struct ArrowSchema* probe_schema,
+                               const struct ArrowArray* probe_array, double distance, uint32_t* build_indices,
+                               uint32_t* probe_indices, uint32_t indices_size, uint32_t* new_indices_size) {
+  RefinerImpl* rf = self ? (RefinerImpl*)self->private_data : nullptr;
+  std::string err;
+  int rc = guarded(&err, [&] {
+    if (!rf) throw Error("refiner is null or released");
+    if (!(distance == distance)) throw Error("distance is NaN");
+    refine_host(rf, probe_schema, probe_array, 100, distance, build_indices, probe_indices, indices_size,
+                new_indices_size);
+  });
+  if (rc && rf) {
+    std::lock_guard<std::mutex> g(rf->mu);
+    rf->last_error = err;
+  }
+  return rc;
+}
+
+int SjhipRefinerKnn(struct SedonaSpatialRefiner* self, const struct ArrowSchema* probe_schema,
+                    const struct ArrowArray* probe_array, uint32_t k, uint32_t* out_build_indices,
+                    double* out_distances) {
+  RefinerImpl* rf = self ? (RefinerImpl*)self->private_data : nullptr;
+  std::string err;
+  int rc = guarded(&err, [&] {
+    if (!rf) throw Error("refiner is null or released");
+    if (!rf->built) throw Error("knn before finish_building");
+    if (!out_build_indices || !out_distances) throw Error("null knn output arrays");
+    use_device(rf->rt);
+    ScopedStream ss;
+    GeomColumn probe;
+    decode_wkb(wkb_input_from_arrow(probe_schema, probe_array, false), probe, ss.s);
+    size_t m = (size_t)probe.n_rows * k;
+    if (m == 0) return;
+    DevArray<uint32_t> ids(m, ss.s);
+    DevArray<double> d(m, ss.s);
+    knn_points(rf->build, probe, (int)k, ids.p, d.p, ss.s);
+    SJ_CHECK(hipMemcpyAsync(out_build_indices, ids.p, m * 4, hipMemcpyDeviceToHost, ss.s));
+    SJ_CHECK(hipMemcpyAsync(out_distances, d.p, m * 8, hipMemcpyDeviceToHost, ss.s));
+    SJ_CHECK(hipStreamSynchronize(ss.s));
+  });
+  if (rc && rf) {
+    std::lock_guard<std::mutex> g(rf->mu);
+    rf->last_error = err;
+  }
+  return rc;
+}
+
+int SjhipJoinDevice(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner* refiner,
+                    const struct ArrowSchema* probe_schema, const struct ArrowDeviceArray* probe_array,
+                    const struct SjhipJoinOptions* options, struct SjhipJoinResult* out) {
+  return guarded(nullptr, [&] {
+    if (!index || !index->private_data || !refiner || !refiner->private_data) throw Error("null index or refiner");
+    if (!probe_array || !options || !out) throw Error("null argument");
+    std::memset(out, 0, sizeof(*out));
+    bool dev = probe_array->device_type == ARROW_DEVICE_ROCM;
+    if (!dev && probe_array->device_type != ARROW_DEVICE_CPU)
+      throw Error("probe array must be in ROCm device memory or CPU memory");
+    WkbInput in = wkb_input_from_arrow(probe_schema, &probe_array->array, dev);
+    JoinEngine::run((IndexImpl*)index->private_data, (RefinerImpl*)refiner->private_data, in, *options, out);
+  });
+}
+
+int SjhipJoinResultCopyToHost(const struct SjhipJoinResult* r, uint32_t* b, uint32_t* p) {
+  return guarded(nullptr, [&] {
+    if (!r) throw Error("null result");
+    if (r->size == 0) return;
+    SJ_CHECK(hipMemcpy(b, r->build_indices, r->size * 4, hipMemcpyDeviceToHost));
+    SJ_CHECK(hipMemcpy(p, r->probe_indices, r->size * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+// Builds the index's tree from the refiner's decoded build geometry (rects
+// computed on the device with the operator's semantics) instead of from
+// host rects: the device-resident counterpart of
+// rust/sedona-spatial-join-gpu/src/index/gpu_spatial_index_builder.rs:193-259.
+int SjhipIndexBuildFromRefiner(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner* refiner) {
+  return guarded(nullptr, [&] {
+    IndexImpl* ix = index_of(index);
+    RefinerImpl* rf = refiner_of(refiner);
+    if (!rf->built) throw Error("refiner is not built");
+    use_device(ix->rt);
+    compute_rects(rf->build, &rf->ring_index, true, NAN, ix->rects, ix->stream);
+    build_rect_tree(ix->rects.p, rf->build.n_rows, ix->tree, ix->stream);
+    SJ_CHECK(hipStreamSynchronize(ix->stream));
+    ix->host_rects.clear();
+    ix->built = true;
+  });
+}
+
+// Device-resident build push: decodes an Arrow array whose buffers are in
+// device (or host) memory.
+int SjhipRefinerPushBuildDevice(struct SedonaSpatialRefiner* refiner, const struct ArrowSchema* schema,
+                                const struct ArrowDeviceArray* array) {
+  return guarded(nullptr, [&] {
+    RefinerImpl* rf = refiner_of(refiner);
+    if (!array) throw Error("null array");
+    bool dev = array->device_type == ARROW_DEVICE_ROCM;
+    refiner_push(rf, wkb_input_from_arrow(schema, &array->array, dev));
+  });
+}
+
+int SjhipComputeRects(const struct ArrowSchema* schema, const struct ArrowArray* array, int point_nearest,
+                      double expand_distance, float* out_rects) {
+  return guarded(nullptr, [&] {
+    if (!out_rects) throw Error("null output");
+    WkbInput in = wkb_input_from_arrow(schema, array, false);
+    int n_dev = 0;
+    SJ_CHECK(hipGetDevice(&n_dev));
+    ScopedStream ss;
+    GeomColumn g;
+    decode_wkb(in, g, ss.s);
+    DevArray<float4> r;
+    compute_rects(g, nullptr, point_nearest != 0, expand_distance, r, ss.s);
+    if (g.n_rows)
+      SJ_CHECK(hipMemcpyAsync(out_rects, r.p, g.n_rows * sizeof(float4), hipMemcpyDeviceToHost, ss.s));
+    SJ_CHECK(hipStreamSynchronize(ss.s));
+  });
+}
+
+}  // extern "C"

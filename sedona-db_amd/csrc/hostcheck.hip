@@ -1,0 +1,63 @@
+// hostcheck.hip -- host entry points over geom_math.h (see
+// include/gpuspatial_hip_testing.h).  Compiled with the same flags as the
+// device code, so the formulas the kernels use can be checked against the
+// oracle on machines without a GPU.
+#include "../../include/gpuspatial_hip_testing.h"
+#include "geom_math.h"
+
+using namespace sjhip;
+
+namespace {
+// host copy of the certified FP32 test (the device one is __device__ only
+// because it uses device fminf/fmaxf overloads)
+int raycast_f32_host(float ax, float ay, float bx, float by, float px, float py, float E) {
+  float lo = ay < by ? ay : by, hi = ay > by ? ay : by;
+  float e2 = 2.0f * E;
+  if (py < lo - e2 || py > hi + e2) return kRayOut;
+  if (!(py > lo + e2 && py < hi - e2)) return -1;
+  float xl = ax < bx ? ax : bx, xh = ax > bx ? ax : bx;
+  if (px < xl - e2) return kRayIn;
+  if (px > xh + e2) return kRayOut;
+  if (by < ay) {
+    float t = ax; ax = bx; bx = t;
+    t = ay; ay = by; by = t;
+  }
+  float d1 = py - ay, d2 = bx - ax, d3 = by - ay, d4 = px - ax;
+  float det = d1 * d2 - d3 * d4;
+  float a1 = fabsf(d1), a2 = fabsf(d2), a3 = fabsf(d3), a4 = fabsf(d4);
+  const float u = 5.9604645e-08f;
+  float bound = 3.0f * E * (a1 + a2 + a3 + a4) + 8.0f * u * (a1 * a2 + a3 * a4) + 8.0f * E * E;
+  if (det > bound) return kRayIn;
+  if (det < -bound) return kRayOut;
+  return -1;
+}
+}  // namespace
+
+extern "C" {
+int SjhipHostRaycast(double ax, double ay, double bx, double by, double px, double py) {
+  return raycast(ax, ay, bx, by, px, py);
+}
+int SjhipHostRaycastF32(float ax, float ay, float bx, float by, float px, float py, float e) {
+  return raycast_f32_host(ax, ay, bx, by, px, py, e);
+}
+int SjhipHostCollinear(double x1, double y1, double x2, double y2, double x3, double y3) {
+  return collinear(x1, y1, x2, y2, x3, y3) ? 1 : 0;
+}
+int SjhipHostSegmentIntersects(double ax, double ay, double bx, double by, double cx, double cy, double dx,
+                               double dy) {
+  return segment_intersects(ax, ay, bx, by, cx, cy, dx, dy) ? 1 : 0;
+}
+int SjhipHostOrient2d(double ax, double ay, double bx, double by, double cx, double cy) {
+  return orient2d_sign(ax, ay, bx, by, cx, cy);
+}
+double SjhipHostHypot(double x, double y) { return hypot_ref(x, y); }
+double SjhipHostCoordToLine(double px, double py, double sx, double sy, double ex, double ey) {
+  return coord_to_line(px, py, sx, sy, ex, ey);
+}
+void SjhipHostRaycastBatch(const double* s, int64_t n, int32_t* out) {
+  for (int64_t i = 0; i < n; i++) out[i] = raycast(s[6 * i], s[6 * i + 1], s[6 * i + 2], s[6 * i + 3], s[6 * i + 4], s[6 * i + 5]);
+}
+void SjhipHostHypotBatch(const double* xy, int64_t n, double* out) {
+  for (int64_t i = 0; i < n; i++) out[i] = hypot_ref(xy[2 * i], xy[2 * i + 1]);
+}
+}

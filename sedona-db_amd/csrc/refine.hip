@@ -1,0 +1,731 @@
+// refine.hip -- refine kernels over candidate pairs.
+//
+// Relation predicates follow the reference CPU refine (TG):
+//   rust/sedona-spatial-join/src/refine/tg.rs:322-350 evaluates
+//   Op(build, probe) with tg_geom_{intersects,contains,within,covers,
+//   coveredby}; for (multi)polygon x point these reduce to
+//   poly_contains_point (c/sedona-tg/src/tg/tg.c:3909-3955) over
+//   tg_ring_contains_point (2809-2822), i.e. the ray classification of
+//   every ring segment (raycast, 1199-1275): any ON segment decides
+//   allow_on_edge, otherwise the parity of IN segments decides.
+//   Polygon x polygon intersects follows tg.c:4100-4136 / 3166-3190 /
+//   3089-3111.
+// The distance predicate follows the reference's Geo refiner
+//   (rust/sedona-spatial-join/src/refine/geo.rs:312-335) over
+//   rust/sedona-geo-generic-alg/.../euclidean/utils.rs:127-271.
+//
+// Two precisions for the point-in-polygon refine:
+//   precision 1: every segment in FP64 with the reference's exact
+//                expression sequence (the plain kernel);
+//   precision 0: every segment first in FP32 ring-local coordinates with a
+//                forward error bound; only segments whose outcome the bound
+//                cannot certify are re-evaluated in FP64 exactly as above.
+// Both give the reference's decisions bit for bit.
+//
+// Work split: candidates whose polygon has at most kLongRing segments are
+// refined one per lane; longer ones one per wavefront, with the ring's
+// segments split across the 64 lanes and the ON / IN-parity results
+// combined with wavefront ballots.
+#include <hipcub/hipcub.hpp>
+
+#include "geom_math.h"
+#include "sjhip.h"
+
+namespace sjhip {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kLongRing = 192;
+
+enum PredCode : int {
+  kEquals = 0, kDisjoint = 1, kTouches = 2, kContains = 3, kCovers = 4, kIntersects = 5, kWithin = 6,
+  kCoveredBy = 7, kDWithin = 100
+};
+
+__device__ __forceinline__ bool is_polyish(int t) { return t == kPolygon || t == kMultiPolygon; }
+
+struct RingRef {
+  const double2* xy;  // ring points (double)
+  int64_t v0;         // global index of the first point
+  int64_t npts;
+};
+
+__device__ __forceinline__ int64_t n_parts(const GeomView& g, int64_t row) {
+  return g.part_off[row + 1] - g.part_off[row];
+}
+
+__device__ __forceinline__ RingRef ring_at(const GeomView& g, int64_t ring) {
+  RingRef r;
+  r.v0 = g.pt_off[ring];
+  r.npts = g.pt_off[ring + 1] - r.v0;
+  r.xy = g.xy + r.v0;
+  return r;
+}
+
+__device__ __forceinline__ double4 ring_rect_scan(const RingRef& r) {
+  double4 b = make_double4(INFINITY, INFINITY, -INFINITY, -INFINITY);
+  for (int64_t i = 0; i < r.npts; i++) {
+    double2 p = r.xy[i];
+    b.x = fmin0(b.x, p.x); b.y = fmin0(b.y, p.y);
+    b.z = fmax0(b.z, p.x); b.w = fmax0(b.w, p.y);
+  }
+  return b;
+}
+
+__device__ __forceinline__ bool rect_covers(const double4& b, double px, double py) {
+  return !(px < b.x || py < b.y || px > b.z || py > b.w);
+}
+
+// Accumulates the ray classification of segment (a, b) into (on, in).
+__device__ __forceinline__ void acc_ray(int rc, int& on, int& in) {
+  on |= rc == kRayOn;
+  in ^= rc == kRayIn;
+}
+
+// FP64 plain ring test over segments [s0, s1) with stride.
+__device__ __forceinline__ void ring_segments_f64(const RingRef& r, int64_t s0, int64_t s1, int64_t stride,
+                                                  double px, double py, int& on, int& in, uint64_t& tests) {
+  for (int64_t j = s0; j < s1; j += stride) {
+    double2 a = r.xy[j], b = r.xy[j + 1];
+    if (py < fmin0(a.y, b.y) || py > fmax0(a.y, b.y)) continue;
+    tests++;
+    acc_ray(raycast(a.x, a.y, b.x, b.y, px, py), on, in);
+  }
+}
+
+// FP32-certified ring test over segments [s0, s1) with stride.
+__device__ __forceinline__ void ring_segments_f32(const RingRef& r, const float2* xy32, float E, float pxl, float pyl,
+                                                  double px, double py, int64_t s0, int64_t s1, int64_t stride,
+                                                  int& on, int& in, uint64_t& tests, uint64_t& fallbacks) {
+  const float2* q = xy32 + r.v0;
+  for (int64_t j = s0; j < s1; j += stride) {
+    float2 a = q[j], b = q[j + 1];
+    int rc = raycast_f32_certified(a.x, a.y, b.x, b.y, pxl, pyl, E);
+    if (rc == kRayOut) continue;
+    tests++;
+    if (rc < 0) {
+      fallbacks++;
+      double2 A = r.xy[j], B = r.xy[j + 1];
+      rc = raycast(A.x, A.y, B.x, B.y, px, py);
+    }
+    acc_ray(rc, on, in);
+  }
+}
+
+struct Ctx {
+  GeomView B;         // build column
+  RingIndexView R;    // build ring index
+  GeomView P;         // probe column
+};
+
+// tg_ring_contains_point for a build-side ring (precomputed rect and
+// optional FP32 local coordinates).
+__device__ bool build_ring_contains(const Ctx& c, int64_t ring, double px, double py, bool allow_on_edge,
+                                    int precision, uint64_t& tests, uint64_t& fallbacks) {
+  double4 rb = c.R.ring_rect[ring];
+  if (!rect_covers(rb, px, py)) return false;
+  RingRef r = ring_at(c.B, ring);
+  int64_t ns = r.npts >= 3 ? r.npts - 1 : 0;
+  int on = 0, in = 0;
+  if (precision == 0) {
+    double2 o = c.R.origin[ring];
+    float E = c.R.err[ring];
+    float pxl = (float)(px - o.x), pyl = (float)(py - o.y);
+    ring_segments_f32(r, c.R.xy32, E, pxl, pyl, px, py, 0, ns, 1, on, in, tests, fallbacks);
+  } else {
+    ring_segments_f64(r, 0, ns, 1, px, py, on, in, tests);
+  }
+  return on ? allow_on_edge : (in != 0);
+}
+
+// tg_ring_contains_point for any ring (probe side: rect scanned on the fly).
+__device__ bool any_ring_contains(const GeomView& g, int64_t ring, double px, double py, bool allow_on_edge) {
+  RingRef r = ring_at(g, ring);
+  double4 rb = ring_rect_scan(r);
+  if (!rect_covers(rb, px, py)) return false;
+  int64_t ns = r.npts >= 3 ? r.npts - 1 : 0;
+  int on = 0, in = 0;
+  uint64_t t = 0;
+  ring_segments_f64(r, 0, ns, 1, px, py, on, in, t);
+  return on ? allow_on_edge : (in != 0);
+}
+
+// poly_contains_point (tg.c:3909-3932) for part `part` of a build row.
+__device__ bool build_poly_contains(const Ctx& c, int64_t part, double px, double py, bool allow_on_edge,
+                                    int precision, uint64_t& tests, uint64_t& fallbacks) {
+  int64_t r0 = c.B.ring_off[part], r1 = c.B.ring_off[part + 1];
+  if (r0 == r1) return false;
+  if (r1 - r0 == 1) return build_ring_contains(c, r0, px, py, allow_on_edge, precision, tests, fallbacks);
+  if (c.B.pt_off[r0 + 1] - c.B.pt_off[r0] < 3) return false;  // tg_poly_empty
+  if (!build_ring_contains(c, r0, px, py, allow_on_edge, precision, tests, fallbacks)) return false;
+  for (int64_t h = r0 + 1; h < r1; h++)
+    if (build_ring_contains(c, h, px, py, !allow_on_edge, precision, tests, fallbacks)) return false;
+  return true;
+}
+
+__device__ bool any_poly_contains(const GeomView& g, int64_t part, double px, double py, bool allow_on_edge) {
+  int64_t r0 = g.ring_off[part], r1 = g.ring_off[part + 1];
+  if (r0 == r1) return false;
+  if (r1 - r0 == 1) return any_ring_contains(g, r0, px, py, allow_on_edge);
+  if (g.pt_off[r0 + 1] - g.pt_off[r0] < 3) return false;
+  if (!any_ring_contains(g, r0, px, py, allow_on_edge)) return false;
+  for (int64_t h = r0 + 1; h < r1; h++)
+    if (any_ring_contains(g, h, px, py, !allow_on_edge)) return false;
+  return true;
+}
+
+__device__ __forceinline__ bool point_of(const GeomView& g, int64_t row, double& x, double& y) {
+  if (g.type[row] != kPoint) return false;
+  if (g.points_only) {
+    double2 p = g.xy[row];
+    x = p.x; y = p.y;
+    return p.x == p.x || p.y == p.y;
+  }
+  if (g.part_off[row] == g.part_off[row + 1]) return false;
+  double2 p = g.xy[g.pt_off[g.ring_off[g.part_off[row]]]];
+  x = p.x; y = p.y;
+  // an empty point (NaN, NaN) that entered through a points-only batch
+  return p.x == p.x || p.y == p.y;
+}
+
+// tg_point_covers_poly (tg.c:1150-1154) for every part of a polyish row.
+__device__ bool point_covers_polyish(const GeomView& g, int64_t row, double px, double py) {
+  int64_t p0 = g.part_off[row], p1 = g.part_off[row + 1];
+  if (p0 == p1) return false;
+  for (int64_t q = p0; q < p1; q++) {
+    int64_t r0 = g.ring_off[q];
+    if (g.ring_off[q + 1] == r0) return false;
+    RingRef r = ring_at(g, r0);
+    if (r.npts < 3) return false;
+    double4 b = ring_rect_scan(r);
+    if (!(feq(b.x, px) && feq(b.y, py) && feq(b.z, px) && feq(b.w, py))) return false;
+  }
+  return true;
+}
+
+// ---- polygon x polygon intersects (tg.c:3089-3111, 3166-3190, 4100-4136)
+
+__device__ bool ring_covers_pt(const RingRef& r, const double4& rb, double px, double py) {
+  if (!rect_covers(rb, px, py)) return false;
+  int64_t ns = r.npts >= 3 ? r.npts - 1 : 0;
+  int on = 0, in = 0;
+  uint64_t t = 0;
+  ring_segments_f64(r, 0, ns, 1, px, py, on, in, t);
+  return on ? true : (in != 0);
+}
+
+__device__ bool ring_intersects_segment(const RingRef& ring, const double4& rb, double ax, double ay, double bx,
+                                        double by) {
+  double sx0 = fmin0(ax, bx), sy0 = fmin0(ay, by), sx1 = fmax0(ax, bx), sy1 = fmax0(ay, by);
+  if (rb.x > sx1 || rb.z < sx0 || rb.y > sy1 || rb.w < sy0) return false;
+  if (ring_covers_pt(ring, rb, ax, ay) || ring_covers_pt(ring, rb, bx, by)) return true;
+  int count = 0;
+  int64_t ns = ring.npts >= 3 ? ring.npts - 1 : 0;
+  for (int64_t i = 0; i < ns && count < 2; i++) {
+    double2 c = ring.xy[i], d = ring.xy[i + 1];
+    if (segment_intersects(ax, ay, bx, by, c.x, c.y, d.x, d.y)) count++;
+  }
+  return count >= 2;
+}
+
+__device__ bool ring_intersects_ring(RingRef ring, RingRef other) {
+  if (ring.npts < 3 || other.npts < 3) return false;
+  double4 a = ring_rect_scan(ring), b = ring_rect_scan(other);
+  if (b.x > a.z || b.z < a.x || b.y > a.w || b.w < a.y) return false;
+  if ((b.z - b.x) * (b.w - b.y) > (a.z - a.x) * (a.w - a.y)) {
+    RingRef t = ring; ring = other; other = t;
+    double4 tb = a; a = b; b = tb;
+  }
+  int64_t ns = other.npts - 1;
+  for (int64_t i = 0; i < ns; i++) {
+    double2 p = other.xy[i], q = other.xy[i + 1];
+    if (ring_intersects_segment(ring, a, p.x, p.y, q.x, q.y)) return true;
+  }
+  return false;
+}
+
+// tg_ring_contains_ring(hole, ring, false) as used by poly x poly
+// intersects: every vertex strictly inside and no segment crossing.
+__device__ bool ring_contains_ring_strict(const RingRef& a, const RingRef& b) {
+  if (a.npts < 3 || b.npts < 3) return false;
+  double4 ra = ring_rect_scan(a), rb = ring_rect_scan(b);
+  if (rb.x < ra.x || rb.z > ra.z || rb.y < ra.y || rb.w > ra.w) return false;
+  for (int64_t i = 0; i < b.npts; i++) {
+    double2 p = b.xy[i];
+    int on = 0, in = 0;
+    uint64_t t = 0;
+    if (!rect_covers(ra, p.x, p.y)) return false;
+    ring_segments_f64(a, 0, a.npts - 1, 1, p.x, p.y, on, in, t);
+    if (on || !in) return false;
+  }
+  for (int64_t i = 0; i + 1 < b.npts; i++) {
+    double2 p = b.xy[i], q = b.xy[i + 1];
+    for (int64_t j = 0; j + 1 < a.npts; j++) {
+      double2 c = a.xy[j], d = a.xy[j + 1];
+      if (segment_intersects(p.x, p.y, q.x, q.y, c.x, c.y, d.x, d.y)) return false;
+    }
+  }
+  return true;
+}
+
+__device__ bool poly_intersects_poly(const GeomView& A, int64_t pa, const GeomView& Bv, int64_t pb) {
+  int64_t a0 = A.ring_off[pa], a1 = A.ring_off[pa + 1];
+  int64_t b0 = Bv.ring_off[pb], b1 = Bv.ring_off[pb + 1];
+  if (a0 == a1 || b0 == b1) return false;
+  RingRef ae = ring_at(A, a0), be = ring_at(Bv, b0);
+  if (a1 - a0 == 1 && b1 - b0 == 1) return ring_intersects_ring(ae, be);
+  if (ae.npts < 3 || be.npts < 3) return false;
+  if (!ring_intersects_ring(be, ae)) return false;
+  for (int64_t h = a0 + 1; h < a1; h++)
+    if (ring_contains_ring_strict(ring_at(A, h), be)) return false;
+  for (int64_t h = b0 + 1; h < b1; h++)
+    if (ring_contains_ring_strict(ring_at(Bv, h), ae)) return false;
+  return true;
+}
+
+// Evaluates pred(build b, probe p).  Returns 0/1, or -1 when unsupported.
+__device__ int eval_relation(const Ctx& c, int64_t b, int64_t p, int pred, int precision, uint64_t& tests,
+                             uint64_t& fallbacks) {
+  int tb = c.B.type[b], tp = c.P.type[p];
+  if (tb == kNull || tp == kNull) return 0;
+  double px, py, bx, by;
+  if (is_polyish(tb) && tp == kPoint) {
+    if (!point_of(c.P, p, px, py)) return 0;
+    if (n_parts(c.B, b) == 0) return 0;
+    bool allow;
+    switch (pred) {
+      case kIntersects: case kCovers: allow = true; break;
+      case kContains: allow = false; break;
+      case kWithin: return 0;
+      case kCoveredBy: return point_covers_polyish(c.B, b, px, py) ? 1 : 0;
+      default: return -1;
+    }
+    for (int64_t q = c.B.part_off[b]; q < c.B.part_off[b + 1]; q++)
+      if (build_poly_contains(c, q, px, py, allow, precision, tests, fallbacks)) return 1;
+    return 0;
+  }
+  if (tb == kPoint && is_polyish(tp)) {
+    if (!point_of(c.B, b, bx, by)) return 0;
+    if (n_parts(c.P, p) == 0) return 0;
+    bool allow;
+    switch (pred) {
+      case kIntersects: case kCoveredBy: allow = true; break;
+      case kWithin: allow = false; break;
+      case kContains: return 0;
+      case kCovers: return point_covers_polyish(c.P, p, bx, by) ? 1 : 0;
+      default: return -1;
+    }
+    for (int64_t q = c.P.part_off[p]; q < c.P.part_off[p + 1]; q++)
+      if (any_poly_contains(c.P, q, bx, by, allow)) return 1;
+    return 0;
+  }
+  if (tb == kPoint && tp == kPoint) {
+    if (!point_of(c.B, b, bx, by) || !point_of(c.P, p, px, py)) return 0;
+    switch (pred) {
+      case kIntersects: case kContains: case kWithin: case kCovers: case kCoveredBy: case kEquals:
+        return (feq(bx, px) && feq(by, py)) ? 1 : 0;
+      default: return -1;
+    }
+  }
+  if (is_polyish(tb) && is_polyish(tp) && pred == kIntersects) {
+    for (int64_t qa = c.B.part_off[b]; qa < c.B.part_off[b + 1]; qa++)
+      for (int64_t qb = c.P.part_off[p]; qb < c.P.part_off[p + 1]; qb++)
+        if (poly_intersects_poly(c.B, qa, c.P, qb)) return 1;
+    return 0;
+  }
+  return -1;
+}
+
+// Whether candidate i is handled by the wavefront-cooperative kernel.
+__device__ __forceinline__ bool is_long(const Ctx& c, int64_t b, int64_t p) {
+  if (c.B.type[b] != kPolygon || c.P.type[p] != kPoint) return false;
+  int64_t part = c.B.part_off[b];
+  if (c.B.ring_off[part + 1] - c.B.ring_off[part] != 1) return false;
+  int64_t ring = c.B.ring_off[part];
+  return c.B.pt_off[ring + 1] - c.B.pt_off[ring] > kLongRing;
+}
+
+// Wavefront compaction: lanes with keep=1 append (b, p) to the output.
+__device__ __forceinline__ void emit(bool keep, uint32_t b, uint32_t p, uint32_t* ob, uint32_t* op, uint64_t cap,
+                                     unsigned long long* counter) {
+  uint64_t m = __ballot(keep);
+  if (m == 0) return;
+  int lane = threadIdx.x & 63;
+  unsigned long long base = 0;
+  int leader = __ffsll((unsigned long long)m) - 1;
+  if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+  base = __shfl(base, leader);
+  if (keep) {
+    uint64_t below = m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+    unsigned long long w = base + __popcll(below);
+    if (w < cap) { ob[w] = b; op[w] = p; }
+  }
+}
+
+// One lane per candidate; long-ring candidates are deferred into a list.
+__global__ __launch_bounds__(kBlock) void k_refine_lane(Ctx c, int pred, int precision, const uint32_t* cb,
+                                                        const uint32_t* cp, uint64_t n, uint32_t* ob, uint32_t* op,
+                                                        uint64_t cap, unsigned long long* counter,
+                                                        uint32_t* long_list, unsigned long long* long_counter,
+                                                        unsigned long long* stats, int* bad) {
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t tests = 0, fallbacks = 0;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+    uint64_t i = base + threadIdx.x;
+    bool keep = false;
+    uint32_t b = 0, p = 0;
+    if (i < n) {
+      b = cb[i];
+      p = cp[i];
+      if (long_list && is_long(c, b, p)) {
+        unsigned long long w = atomicAdd(long_counter, 1ull);
+        long_list[w] = (uint32_t)i;
+      } else {
+        int r = eval_relation(c, b, p, pred, precision, tests, fallbacks);
+        if (r < 0) atomicOr(bad, 1);
+        keep = r > 0;
+      }
+    }
+    emit(keep, b, p, ob, op, cap, counter);
+  }
+  // per-wave statistics
+  for (int o = 32; o > 0; o >>= 1) {
+    tests += __shfl_xor(tests, o);
+    fallbacks += __shfl_xor(fallbacks, o);
+  }
+  if ((threadIdx.x & 63) == 0 && stats) {
+    atomicAdd(&stats[0], (unsigned long long)tests);
+    atomicAdd(&stats[1], (unsigned long long)fallbacks);
+  }
+}
+
+// One wavefront per long-ring candidate: lanes split the segments.
+__global__ __launch_bounds__(kBlock) void k_refine_wave(Ctx c, int pred, int precision, const uint32_t* cb,
+                                                        const uint32_t* cp, const uint32_t* list,
+                                                        const unsigned long long* list_n, uint32_t* ob,
+                                                        uint32_t* op, uint64_t cap, unsigned long long* counter,
+                                                        unsigned long long* stats) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t nlist = *list_n;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  uint64_t tests = 0, fallbacks = 0;
+  for (uint64_t k = wave; k < nlist; k += nwaves) {
+    uint64_t i = list[k];
+    uint32_t b = cb[i], p = cp[i];
+    double px, py;
+    bool valid = point_of(c.P, p, px, py);
+    bool allow = (pred == kIntersects || pred == kCovers);
+    int64_t ring = c.B.ring_off[c.B.part_off[b]];
+    bool keep = false;
+    if (valid && (pred == kIntersects || pred == kCovers || pred == kContains)) {
+      double4 rb = c.R.ring_rect[ring];
+      if (rect_covers(rb, px, py)) {
+        RingRef r = ring_at(c.B, ring);
+        int64_t ns = r.npts - 1;
+        int on = 0, in = 0;
+        if (precision == 0) {
+          double2 o = c.R.origin[ring];
+          float E = c.R.err[ring];
+          float pxl = (float)(px - o.x), pyl = (float)(py - o.y);
+          ring_segments_f32(r, c.R.xy32, E, pxl, pyl, px, py, lane, ns, 64, on, in, tests, fallbacks);
+        } else {
+          ring_segments_f64(r, lane, ns, 64, px, py, on, in, tests);
+        }
+        bool any_on = __ballot(on) != 0;
+        bool parity = (__popcll(__ballot(in)) & 1) != 0;
+        keep = any_on ? allow : parity;
+      }
+    }
+    if (lane == 0 && keep) {
+      unsigned long long w = atomicAdd(counter, 1ull);
+      if (w < cap) { ob[w] = b; op[w] = p; }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    tests += __shfl_xor(tests, o);
+    fallbacks += __shfl_xor(fallbacks, o);
+  }
+  if (lane == 0 && stats) {
+    atomicAdd(&stats[0], (unsigned long long)tests);
+    atomicAdd(&stats[1], (unsigned long long)fallbacks);
+  }
+}
+
+// ---- distance ----------------------------------------------------------
+
+// Exact point position w.r.t. a closed ring: 2 boundary, 1 inside, 0 outside.
+__device__ int exact_ring_position(const RingRef& r, double px, double py) {
+  int inside = 0;
+  for (int64_t i = 0; i < r.npts; i++) {
+    int64_t j = i + 1 == r.npts ? 0 : i + 1;
+    double2 a = r.xy[i], b = r.xy[j];
+    if (line_intersects_coord(a.x, a.y, b.x, b.y, px, py)) return 2;
+    if ((a.y > py) != (b.y > py)) {
+      int o = orient2d_sign(a.x, a.y, b.x, b.y, px, py);
+      if ((b.y > a.y) ? (o > 0) : (o < 0)) inside ^= 1;
+    }
+  }
+  return inside;
+}
+
+__device__ double point_to_run(const RingRef& r, double px, double py) {
+  // utils.rs:184-210 (open run of points)
+  if (r.npts < 2) return 0.0;
+  for (int64_t i = 0; i + 1 < r.npts; i++) {
+    double2 a = r.xy[i], b = r.xy[i + 1];
+    if (line_intersects_coord(a.x, a.y, b.x, b.y, px, py)) return 0.0;
+  }
+  double2 a = r.xy[0], b = r.xy[1];
+  double m = coord_to_line(px, py, a.x, a.y, b.x, b.y);
+  for (int64_t i = 1; i + 1 < r.npts; i++) {
+    a = r.xy[i];
+    b = r.xy[i + 1];
+    m = rmin(m, coord_to_line(px, py, a.x, a.y, b.x, b.y));
+  }
+  return m;
+}
+
+__device__ double point_to_polygon(const GeomView& g, int64_t part, double px, double py) {
+  int64_t r0 = g.ring_off[part], r1 = g.ring_off[part + 1];
+  if (r0 == r1) return 0.0;
+  RingRef ext = ring_at(g, r0);
+  int st = exact_ring_position(ext, px, py);
+  if (st == 2) return 0.0;
+  if (st == 1) {
+    bool in_hole = false;
+    for (int64_t h = r0 + 1; h < r1; h++) {
+      int hs = exact_ring_position(ring_at(g, h), px, py);
+      if (hs == 2) return 0.0;
+      if (hs == 1) in_hole = true;
+    }
+    if (!in_hole) return 0.0;
+  }
+  double m = INFINITY;
+  for (int64_t k = r0; k < r1; k++) {
+    RingRef r = ring_at(g, k);
+    for (int64_t i = 0; i + 1 < r.npts; i++) {
+      double2 a = r.xy[i], b = r.xy[i + 1];
+      m = rmin(m, coord_to_line(px, py, a.x, a.y, b.x, b.y));
+    }
+  }
+  return m;
+}
+
+__device__ double distance_to_point(const GeomView& g, int64_t row, double px, double py) {
+  int t = g.type[row];
+  if (g.points_only) {
+    double2 q = g.xy[row];
+    return hypot_ref(q.x - px, q.y - py);
+  }
+  double m = INFINITY;
+  for (int64_t q = g.part_off[row]; q < g.part_off[row + 1]; q++) {
+    int64_t r0 = g.ring_off[q];
+    RingRef r = ring_at(g, r0);
+    double d;
+    if (t == kPoint || t == kMultiPoint) d = hypot_ref(r.xy[0].x - px, r.xy[0].y - py);
+    else if (t == kLine || t == kMultiLine) d = point_to_run(r, px, py);
+    else d = point_to_polygon(g, q, px, py);
+    m = rmin(m, d);
+  }
+  return m;
+}
+
+__global__ __launch_bounds__(kBlock) void k_refine_distance(GeomView B, GeomView P, double dist, const uint32_t* cb,
+                                                            const uint32_t* cp, uint64_t n, uint32_t* ob,
+                                                            uint32_t* op, uint64_t cap, unsigned long long* counter) {
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+    uint64_t i = base + threadIdx.x;
+    bool keep = false;
+    uint32_t b = 0, p = 0;
+    if (i < n) {
+      b = cb[i];
+      p = cp[i];
+      double px, py;
+      bool bok = B.type[b] != kNull && (B.points_only || B.part_off[b] != B.part_off[b + 1]);
+      if (bok && point_of(P, p, px, py)) keep = distance_to_point(B, b, px, py) <= dist;
+    }
+    emit(keep, b, p, ob, op, cap, counter);
+  }
+}
+
+// ---- ring index ------------------------------------------------------------
+
+__global__ void k_ring_rects(GeomView g, int64_t n_rings, double4* rr, double2* origin, float* err) {
+  int64_t ring = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  int lane = threadIdx.x & 63;
+  if (ring >= n_rings) return;
+  int64_t v0 = g.pt_off[ring], v1 = g.pt_off[ring + 1];
+  double xl = INFINITY, yl = INFINITY, xh = -INFINITY, yh = -INFINITY;
+  for (int64_t v = v0 + lane; v < v1; v += 64) {
+    double2 p = g.xy[v];
+    xl = fmin0(xl, p.x); xh = fmax0(xh, p.x);
+    yl = fmin0(yl, p.y); yh = fmax0(yh, p.y);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    xl = fmin0(xl, __shfl_xor(xl, o));
+    yl = fmin0(yl, __shfl_xor(yl, o));
+    xh = fmax0(xh, __shfl_xor(xh, o));
+    yh = fmax0(yh, __shfl_xor(yh, o));
+  }
+  if (lane == 0) {
+    rr[ring] = make_double4(xl, yl, xh, yh);
+    double ox = 0.5 * xl + 0.5 * xh, oy = 0.5 * yl + 0.5 * yh;
+    double H = fmax(xh - xl, yh - yl) * 0.5;
+    float E;
+    // ring-local FP32 coordinates are within 2^-23 H of exact; use 2^-22 H
+    if (v1 - v0 < 3 || !(H > 1e-15) || !(H < 1e15) || ox != ox || oy != oy) E = INFINITY;
+    else E = (float)(H * 2.384185791015625e-07) * 1.0001f;
+    if (!(ox == ox)) { ox = 0; oy = 0; }
+    origin[ring] = make_double2(ox, oy);
+    err[ring] = E;
+  }
+}
+
+__global__ void k_local32(GeomView g, int64_t n_rings, const double2* origin, float2* xy32) {
+  int64_t ring = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  int lane = threadIdx.x & 63;
+  if (ring >= n_rings) return;
+  int64_t v0 = g.pt_off[ring], v1 = g.pt_off[ring + 1];
+  double2 o = origin[ring];
+  for (int64_t v = v0 + lane; v < v1; v += 64) {
+    double2 p = g.xy[v];
+    xy32[v] = make_float2((float)(p.x - o.x), (float)(p.y - o.y));
+  }
+}
+
+unsigned int blocks_for(uint64_t n) { return (unsigned int)((n + kBlock - 1) / kBlock); }
+
+unsigned int grid_for(uint64_t n) {
+  uint64_t b = blocks_for(n);
+  return (unsigned int)std::max<uint64_t>(1, std::min<uint64_t>(b, 1u << 16));
+}
+
+}  // namespace
+
+void build_ring_index(const GeomColumn& g, RingIndex& ri, const RingIndexParams& prm, hipStream_t s) {
+  (void)prm;
+  ri = RingIndex();
+  if (g.points_only || g.n_rings == 0) return;
+  ri.ring_rect.alloc(g.n_rings, s);
+  ri.origin.alloc(g.n_rings, s);
+  ri.err.alloc(g.n_rings, s);
+  ri.xy32.alloc(g.n_pts > 0 ? g.n_pts : 1, s);
+  uint64_t threads = (uint64_t)g.n_rings * 64;
+  hipLaunchKernelGGL(k_ring_rects, blocks_for(threads), kBlock, 0, s, g.view(), g.n_rings, ri.ring_rect.p,
+                     ri.origin.p, ri.err.p);
+  hipLaunchKernelGGL(k_local32, blocks_for(threads), kBlock, 0, s, g.view(), g.n_rings, ri.origin.p, ri.xy32.p);
+  SJ_CHECK(hipGetLastError());
+}
+
+void check_relation_types(uint32_t bm, uint32_t pm, int pred) {
+  const uint32_t poly = (1u << kPolygon) | (1u << kMultiPolygon), pt = 1u << kPoint;
+  bool ok = true;
+  auto only = [](uint32_t m, uint32_t allowed) { return (m & ~allowed) == 0; };
+  switch (pred) {
+    case kIntersects:
+      ok = only(bm, poly | pt) && only(pm, poly | pt);
+      break;
+    case kContains: case kWithin: case kCovers: case kCoveredBy:
+      ok = only(bm, poly | pt) && only(pm, poly | pt) && !((bm & poly) && (pm & poly));
+      break;
+    case kEquals:
+      ok = only(bm, pt) && only(pm, pt);
+      break;
+    default:
+      ok = false;
+  }
+  if (!ok)
+    throw Error("unsupported predicate / geometry type combination for the GPU refine (predicate " +
+                std::to_string(pred) + ")");
+}
+
+uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, int predicate,
+                         const uint32_t* cb, const uint32_t* cp, uint64_t n, int precision, PairBuf& out,
+                         RefineStats* stats, hipStream_t s) {
+  out.counter.reserve(1, s);
+  out.b.reserve(n > 0 ? n : 1, s);
+  out.p.reserve(n > 0 ? n : 1, s);
+  if (n == 0) return 0;
+  if (build.points_only || ri.ring_rect.p == nullptr) precision = 1;
+  Ctx c;
+  c.B = build.view();
+  c.R = ri.view();
+  c.P = probe.view();
+  DevArray<unsigned long long> aux(4, s);  // counter of long list, stats[2], bad flag
+  DevArray<uint32_t> long_list;
+  bool use_long = !build.points_only && probe.points_only && c.R.ring_rect != nullptr &&
+                  (predicate == kIntersects || predicate == kCovers || predicate == kContains);
+  if (use_long) long_list.alloc(n, s);
+  SJ_CHECK(hipMemsetAsync(out.counter.p, 0, 8, s));
+  SJ_CHECK(hipMemsetAsync(aux.p, 0, 32, s));
+  hipLaunchKernelGGL(k_refine_lane, grid_for(n), kBlock, 0, s, c, predicate, precision, cb, cp, n, out.b.p, out.p.p,
+                     (uint64_t)out.b.n, out.counter.p, use_long ? long_list.p : nullptr, aux.p, aux.p + 1,
+                     (int*)(aux.p + 3));
+  SJ_CHECK(hipGetLastError());
+  if (use_long) {
+    // enough waves to fill the chip several times over
+    hipLaunchKernelGGL(k_refine_wave, 256 * 16, kBlock, 0, s, c, predicate, precision, cb, cp, long_list.p, aux.p,
+                       out.b.p, out.p.p, (uint64_t)out.b.n, out.counter.p, aux.p + 1);
+    SJ_CHECK(hipGetLastError());
+  }
+  unsigned long long h[4], cnt = 0;
+  SJ_CHECK(hipMemcpyAsync(h, aux.p, 32, hipMemcpyDeviceToHost, s));
+  SJ_CHECK(hipMemcpyAsync(&cnt, out.counter.p, 8, hipMemcpyDeviceToHost, s));
+  SJ_CHECK(hipStreamSynchronize(s));
+  if (h[3]) throw Error("unsupported geometry type combination in refine");
+  if (stats) {
+    stats->segment_tests += h[1];
+    stats->fallback_tests += h[2];
+  }
+  return cnt;
+}
+
+uint64_t refine_distance(const GeomColumn& build, const GeomColumn& probe, double d, const uint32_t* cb,
+                         const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s) {
+  out.counter.reserve(1, s);
+  out.b.reserve(n > 0 ? n : 1, s);
+  out.p.reserve(n > 0 ? n : 1, s);
+  if (n == 0) return 0;
+  if (!probe.points_only && (probe.type_mask & ~(1u << kPoint)))
+    throw Error("distance refine: probe rows must be points");
+  SJ_CHECK(hipMemsetAsync(out.counter.p, 0, 8, s));
+  hipLaunchKernelGGL(k_refine_distance, grid_for(n), kBlock, 0, s, build.view(), probe.view(), d, cb, cp, n, out.b.p,
+                     out.p.p, (uint64_t)out.b.n, out.counter.p);
+  SJ_CHECK(hipGetLastError());
+  unsigned long long cnt = 0;
+  SJ_CHECK(hipMemcpyAsync(&cnt, out.counter.p, 8, hipMemcpyDeviceToHost, s));
+  SJ_CHECK(hipStreamSynchronize(s));
+  return cnt;
+}
+
+namespace {
+__global__ void k_pack(const uint32_t* b, const uint32_t* p, uint64_t n, uint64_t* k) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) k[i] = ((uint64_t)p[i] << 32) | b[i];
+}
+__global__ void k_unpack(const uint64_t* k, uint64_t n, uint32_t* b, uint32_t* p) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    uint64_t v = k[i];
+    b[i] = (uint32_t)v;
+    p[i] = (uint32_t)(v >> 32);
+  }
+}
+}  // namespace
+
+void sort_pairs(uint32_t* b, uint32_t* p, uint64_t n, hipStream_t s) {
+  if (n < 2) return;
+  DevArray<uint64_t> k1(n, s), k2(n, s);
+  hipLaunchKernelGGL(k_pack, blocks_for(n), kBlock, 0, s, b, p, n, k1.p);
+  size_t tmp = 0;
+  SJ_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, k1.p, k2.p, (int)n, 0, 64, s));
+  DevArray<uint8_t> t(tmp + 16, s);
+  SJ_CHECK(hipcub::DeviceRadixSort::SortKeys(t.p, tmp, k1.p, k2.p, (int)n, 0, 64, s));
+  hipLaunchKernelGGL(k_unpack, blocks_for(n), kBlock, 0, s, k2.p, n, b, p);
+  SJ_CHECK(hipGetLastError());
+}
+
+}  // namespace sjhip

@@ -1,0 +1,248 @@
+// sjhip.h -- internal declarations of the MI355X spatial-join library.
+//
+// Device data layout (all in HBM, structure of arrays):
+//   geometry column  : type[row] (u8), part_off[row+1], ring_off[part+1],
+//                      pt_off[ring+1] (int64 prefix offsets), xy[pt] (double2)
+//                      A column holding only points keeps xy[row] directly
+//                      (NaN for null / empty rows) and no offsets.
+//   ring refine index: ring_rect[ring] (double4, the reference's ring->rect),
+//                      y-stripes per ring (ring_stripe[ring] -> stripe_off ->
+//                      stripe_seg), FP32 ring-local vertices xy32[pt] (float2)
+//                      with a per-ring origin and error bound for the
+//                      certified FP32 fast path.
+//   rect tree        : Morton-sorted FP32 leaf rects (float4) in groups of
+//                      LEAF, internal levels of fan-out FANOUT, all levels
+//                      in one float4 array; sorted build ids (u32).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace sjhip {
+
+// ---------------------------------------------------------------------------
+// Errors
+
+struct Error : std::runtime_error {
+  int code;
+  explicit Error(const std::string& m, int c = 22) : std::runtime_error(m), code(c) {}
+};
+
+#define SJ_CHECK(expr)                                                                    \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      throw ::sjhip::Error(std::string("HIP error ") + hipGetErrorString(e_) + " at " +  \
+                           __FILE__ + ":" + std::to_string(__LINE__));                    \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Device memory (stream-ordered)
+
+template <typename T>
+struct DevArray {
+  T* p = nullptr;
+  size_t n = 0;
+  hipStream_t s = nullptr;
+  DevArray() = default;
+  DevArray(size_t count, hipStream_t stream) { alloc(count, stream); }
+  DevArray(const DevArray&) = delete;
+  DevArray& operator=(const DevArray&) = delete;
+  DevArray(DevArray&& o) noexcept : p(o.p), n(o.n), s(o.s) { o.p = nullptr; o.n = 0; }
+  DevArray& operator=(DevArray&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; s = o.s; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevArray() { release(); }
+  void alloc(size_t count, hipStream_t stream) {
+    release();
+    s = stream;
+    n = count;
+    if (count) SJ_CHECK(hipMallocAsync((void**)&p, (count * sizeof(T) + 255) & ~size_t(255), stream));
+  }
+  // grows (without preserving contents) when smaller than count
+  void reserve(size_t count, hipStream_t stream) {
+    if (count > n || p == nullptr) alloc(count < 1 ? 1 : count, stream);
+  }
+  void release() {
+    if (p) (void)hipFreeAsync(p, s);
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// ---------------------------------------------------------------------------
+// Geometry type codes (2-D WKB base types)
+
+enum GeomType : uint8_t { kNull = 0, kPoint = 1, kLine = 2, kPolygon = 3, kMultiPoint = 4, kMultiLine = 5, kMultiPolygon = 6 };
+
+// Device view of a geometry column.
+struct GeomView {
+  int64_t n_rows = 0;
+  int points_only = 0;
+  const uint8_t* type = nullptr;
+  const int64_t* part_off = nullptr;
+  const int64_t* ring_off = nullptr;
+  const int64_t* pt_off = nullptr;
+  const double2* xy = nullptr;
+};
+
+// Device view of the ring refine index (build side).
+struct RingIndexView {
+  const double4* ring_rect = nullptr;    // xmin, ymin, xmax, ymax (double)
+  const int64_t* stripe_base = nullptr;  // per ring: first entry in stripe_off, -1 = no stripes
+  const int32_t* nstripes = nullptr;     // per ring
+  const double2* stripe_map = nullptr;   // per ring: (y0, scale) with stripe = (y - y0) * scale
+  const int64_t* stripe_off = nullptr;   // stripe -> first entry in stripe_seg
+  const uint32_t* stripe_seg = nullptr;  // local segment index within the ring
+  const float2* xy32 = nullptr;          // per point: ring-local FP32 vertex
+  const double2* origin = nullptr;       // per ring: local origin (double)
+  const float* err = nullptr;            // per ring: FP32 certification bound E
+};
+
+// Owned geometry column on the device.
+struct GeomColumn {
+  int64_t n_rows = 0, n_parts = 0, n_rings = 0, n_pts = 0;
+  bool points_only = false;
+  uint32_t type_mask = 0;  // bit t set when a row of type t is present
+  DevArray<uint8_t> type;
+  DevArray<int64_t> part_off, ring_off, pt_off;
+  DevArray<double2> xy;
+  GeomView view() const {
+    GeomView v;
+    v.n_rows = n_rows;
+    v.points_only = points_only ? 1 : 0;
+    v.type = type.p;
+    v.part_off = part_off.p;
+    v.ring_off = ring_off.p;
+    v.pt_off = pt_off.p;
+    v.xy = xy.p;
+    return v;
+  }
+};
+
+struct RingIndex {
+  DevArray<double4> ring_rect;
+  DevArray<int64_t> stripe_base;
+  DevArray<int32_t> nstripes;
+  DevArray<double2> stripe_map;
+  DevArray<int64_t> stripe_off;
+  DevArray<uint32_t> stripe_seg;
+  DevArray<float2> xy32;
+  DevArray<double2> origin;
+  DevArray<float> err;
+  RingIndexView view() const {
+    RingIndexView v;
+    v.ring_rect = ring_rect.p;
+    v.stripe_base = stripe_base.p;
+    v.nstripes = nstripes.p;
+    v.stripe_map = stripe_map.p;
+    v.stripe_off = stripe_off.p;
+    v.stripe_seg = stripe_seg.p;
+    v.xy32 = xy32.p;
+    v.origin = origin.p;
+    v.err = err.p;
+    return v;
+  }
+};
+
+// Host description of an Arrow WKB column (host or device buffers).
+struct WkbInput {
+  int64_t length = 0;
+  int64_t offset = 0;         // Arrow array offset (rows)
+  const uint8_t* validity = nullptr;
+  const void* offsets = nullptr;  // int32 or int64 (see large)
+  bool large = false;
+  const uint8_t* data = nullptr;
+  // BinaryView: views (16 bytes each) and variadic data buffers
+  bool view = false;
+  const uint8_t* views = nullptr;
+  std::vector<const uint8_t*> view_buffers;
+  std::vector<int64_t> view_buffer_sizes;
+  bool on_device = false;
+};
+
+// Parses an ArrowSchema format ("z", "Z", "vz") and an ArrowArray.
+WkbInput wkb_input_from_arrow(const struct ArrowSchema* schema, const struct ArrowArray* array, bool on_device);
+
+// Decodes a WKB column into a device geometry column (copying host buffers in).
+void decode_wkb(const WkbInput& in, GeomColumn& out, hipStream_t s);
+
+// Per-row FP32 rects: rounding outward; points round to nearest when
+// point_nearest is set (the GPU operator's point special case).  Empty rows
+// get NaN rects.  Optionally expands by a distance (f64) and rounds outward.
+void compute_rects(const GeomColumn& g, const RingIndex* ri, bool point_nearest, double expand,
+                   DevArray<float4>& rects, hipStream_t s);
+
+// Ring refine index for a build column.
+struct RingIndexParams {
+  int segs_per_stripe = 4;     // target segments per y-stripe
+  int min_segs_for_stripes = 16;
+  int max_stripes = 4096;
+};
+void build_ring_index(const GeomColumn& g, RingIndex& ri, const RingIndexParams& p, hipStream_t s);
+
+// Rect tree over FP32 build rects.
+struct RectTree {
+  static constexpr int FANOUT = 8;
+  static constexpr int LEAF = 8;
+  int64_t n_valid = 0;
+  int n_levels = 0;                 // internal + leaf-node levels
+  std::vector<int64_t> level_off;   // start of level l in nodes (l = 0: leaf nodes)
+  std::vector<int64_t> level_cnt;
+  DevArray<float4> nodes;           // all node boxes
+  DevArray<float4> leaf_rects;      // Morton-sorted valid rects
+  DevArray<uint32_t> leaf_ids;      // their build ids
+  DevArray<uint32_t> leaf_keys;     // their Morton keys (sorted)
+  float bb[4] = {0, 0, 0, 0};       // bounds of rect centres (Morton frame)
+};
+void build_rect_tree(const float4* rects_dev, int64_t n, RectTree& t, hipStream_t s);
+
+// Candidate buffers
+struct PairBuf {
+  DevArray<uint32_t> b, p;
+  DevArray<unsigned long long> counter;
+};
+
+// Runs the filter of queries against the tree into pb; returns the count.
+// qorder (optional) gives the visiting order of the queries.
+uint64_t filter_pairs(const RectTree& t, const float4* q, int64_t nq, const uint32_t* qorder, PairBuf& pb,
+                      hipStream_t s);
+
+// Morton order of query rects (a permutation of valid queries; invalid ones
+// are dropped).  Returns the number of valid queries.
+int64_t morton_order(const float4* rects, int64_t n, DevArray<uint32_t>& order, hipStream_t s);
+
+// Refine statistics
+struct RefineStats {
+  uint64_t segment_tests = 0;
+  uint64_t fallback_tests = 0;
+};
+
+// Relation refine of candidate pairs; compacts kept pairs into out (count
+// returned).  precision: 0 = FP32 certified + FP64 fallback, 1 = all-FP64.
+uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, int predicate,
+                         const uint32_t* cb, const uint32_t* cp, uint64_t n, int precision, PairBuf& out,
+                         RefineStats* stats, hipStream_t s);
+
+// Distance refine (build.distance(probe) <= d); probe must be points.
+uint64_t refine_distance(const GeomColumn& build, const GeomColumn& probe, double d, const uint32_t* cb,
+                         const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s);
+
+// Sorts pairs by (probe, build) in place.
+void sort_pairs(uint32_t* b, uint32_t* p, uint64_t n, hipStream_t s);
+
+// KNN over point columns.
+void knn_points(const GeomColumn& build, const GeomColumn& probe, int k, uint32_t* out_ids_dev, double* out_d_dev,
+                hipStream_t s);
+
+// Checks that every pair's geometry types are supported by the relation
+// refine; throws otherwise.
+void check_relation_types(uint32_t build_mask, uint32_t probe_mask, int predicate);
+
+}  // namespace sjhip

@@ -1,0 +1,428 @@
+"""Safe host-side wrappers over the C ABI, mirroring the reference's Rust
+FFI layer (c/sedona-libgpuspatial/src/lib.rs:41-191,
+c/sedona-libgpuspatial/src/libgpuspatial.rs:239-526,
+c/sedona-libgpuspatial/src/options.rs:19-32,
+c/sedona-libgpuspatial/src/predicate.rs:21-46): a process-global runtime
+per device, ``GpuSpatialIndex`` (FP32 rect filter) and ``GpuSpatialRefiner``
+(exact refine), errors raised with the library's message.
+"""
+import ctypes
+import enum
+import threading
+
+import numpy as np
+
+from . import _lib as L
+
+
+class GpuSpatialError(RuntimeError):
+    pass
+
+
+class GpuSpatialRelationPredicate(enum.IntEnum):
+    """predicate.rs:21-46 / gpuspatial_c.h:142-151"""
+    Equals = 0
+    Disjoint = 1
+    Touches = 2
+    Contains = 3
+    Covers = 4
+    Intersects = 5
+    Within = 6
+    CoveredBy = 7
+
+
+PREDICATE_BY_NAME = {p.name.lower(): p for p in GpuSpatialRelationPredicate}
+
+
+class GpuSpatialOptions:
+    """options.rs:19-32"""
+
+    def __init__(self, concurrency=1, device_id=0, compress_bvh=False, pipeline_batches=1,
+                 cuda_use_memory_pool=True, cuda_memory_pool_init_percent=10):
+        self.concurrency = concurrency
+        self.device_id = device_id
+        self.compress_bvh = compress_bvh
+        self.pipeline_batches = pipeline_batches
+        self.cuda_use_memory_pool = cuda_use_memory_pool
+        self.cuda_memory_pool_init_percent = cuda_memory_pool_init_percent
+
+
+# ---------------------------------------------------------------------------
+# Process-global runtime (lib.rs:41-90): one per process; a different device
+# id than the first one is an error.
+
+_runtime_lock = threading.Lock()
+_runtime = None
+_runtime_device = None
+
+
+def _check(rc, msg_fn):
+    if rc != 0:
+        raise GpuSpatialError(msg_fn())
+
+
+def global_runtime(options):
+    global _runtime, _runtime_device
+    with _runtime_lock:
+        if _runtime is not None:
+            if _runtime_device != options.device_id:
+                raise GpuSpatialError(f"GPU runtime already initialised on device {_runtime_device}, "
+                                      f"requested device {options.device_id}")
+            return _runtime
+        lib = L.lib()
+        rt = L.GpuSpatialRuntime()
+        lib.GpuSpatialRuntimeCreate(ctypes.byref(rt))
+        cfg = L.GpuSpatialRuntimeConfig(None, options.device_id, options.cuda_use_memory_pool,
+                                        options.cuda_memory_pool_init_percent)
+        rc = rt.init(ctypes.byref(rt), ctypes.byref(cfg))
+        if rc:
+            msg = rt.get_last_error(ctypes.byref(rt)).decode()
+            rt.release(ctypes.byref(rt))
+            raise GpuSpatialError(f"runtime init failed: {msg}")
+        _runtime = rt
+        _runtime_device = options.device_id
+        return rt
+
+
+class Runtime:
+    """A standalone runtime (the C-ABI object itself), for ABI tests."""
+
+    def __init__(self, ptx_root=None, device_id=0, use_pool=False, pool_percent=10):
+        self.lib = L.lib()
+        self.rt = L.GpuSpatialRuntime()
+        self.lib.GpuSpatialRuntimeCreate(ctypes.byref(self.rt))
+        self.cfg = L.GpuSpatialRuntimeConfig(ptx_root.encode() if ptx_root else None, device_id, use_pool,
+                                             pool_percent)
+
+    def init(self):
+        return self.rt.init(ctypes.byref(self.rt), ctypes.byref(self.cfg))
+
+    def last_error(self):
+        return self.rt.get_last_error(ctypes.byref(self.rt)).decode()
+
+    def release(self):
+        if self.rt.private_data:
+            self.rt.release(ctypes.byref(self.rt))
+
+
+# ---------------------------------------------------------------------------
+# Arrow C data export
+
+
+class ExportedArrow:
+    """A pyarrow array exported through the Arrow C data interface; the
+    library only borrows it for the duration of each call."""
+
+    def __init__(self, arr):
+        import pyarrow as pa
+        if isinstance(arr, pa.ChunkedArray):
+            arr = arr.combine_chunks()
+        if not isinstance(arr, pa.Array):
+            arr = arr.to_arrow()  # datagen.WkbColumn
+        self.arr = arr
+        self.schema = L.ArrowSchema()
+        self.array = L.ArrowArray()
+        arr._export_to_c(ctypes.addressof(self.array), ctypes.addressof(self.schema))
+
+    def __del__(self):
+        try:
+            if self.array.release:
+                self.array.release(ctypes.byref(self.array))
+            if self.schema.release:
+                self.schema.release(ctypes.byref(self.schema))
+        except Exception:
+            pass
+
+
+_noop_release = ctypes.CFUNCTYPE(None, ctypes.POINTER(L.ArrowArray))(lambda p: None)
+
+
+class DeviceWkb:
+    """An Arrow LargeBinary WKB column whose offsets (int64) and data (uint8)
+    live in device memory (e.g. torch tensors on cuda), passed as an
+    ArrowDeviceArray with device_type ROCm."""
+
+    def __init__(self, offsets_ptr, data_ptr, length, device_id=0, keepalive=None):
+        self._keep = keepalive
+        self.bufs = (ctypes.c_void_p * 3)(None, offsets_ptr, data_ptr)
+        self.schema = L.ArrowSchema()
+        self.schema.format = b"Z"
+        self.dev = L.ArrowDeviceArray()
+        a = self.dev.array
+        a.length = length
+        a.null_count = 0
+        a.offset = 0
+        a.n_buffers = 3
+        a.n_children = 0
+        a.buffers = ctypes.cast(self.bufs, ctypes.POINTER(ctypes.c_void_p))
+        a.release = _noop_release
+        self.dev.device_id = device_id
+        self.dev.device_type = L.ARROW_DEVICE_ROCM
+
+    @classmethod
+    def from_torch(cls, col, device="cuda"):
+        import torch
+        off = torch.from_numpy(col.offsets).to(device)
+        data = torch.from_numpy(col.data).to(device)
+        return cls(off.data_ptr(), data.data_ptr(), len(col), keepalive=(off, data))
+
+
+def host_device_array(exported):
+    """Wraps an ExportedArrow as an ArrowDeviceArray with device_type CPU."""
+    d = L.ArrowDeviceArray()
+    d.array = exported.array
+    d.device_id = -1
+    d.device_type = L.ARROW_DEVICE_CPU
+    return d
+
+
+def compute_rects(arr, point_nearest=True, expand_distance=float("nan")):
+    """Device-computed FP32 rects of a WKB column (n x 4, NaN for empty)."""
+    ex = ExportedArrow(arr)
+    out = np.zeros((ex.array.length, 4), np.float32)
+    rc = L.lib().SjhipComputeRects(ctypes.byref(ex.schema), ctypes.byref(ex.array), 1 if point_nearest else 0,
+                                   float(expand_distance), out.ctypes.data)
+    if rc:
+        raise GpuSpatialError(L.lib().SjhipLastError().decode())
+    return out
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _ptr32(a):
+    return a.ctypes.data_as(L.c_u32p)
+
+
+# ---------------------------------------------------------------------------
+# Index (lib.rs:94-134)
+
+
+class GpuSpatialIndex:
+    def __init__(self, options=None, runtime=None):
+        self.options = options or GpuSpatialOptions()
+        self.lib = L.lib()
+        rt = runtime.rt if runtime is not None else global_runtime(self.options)
+        self._rt = rt
+        self.cfg = L.GpuSpatialIndexConfig(ctypes.pointer(rt), self.options.concurrency)
+        self.ix = L.SedonaFloatIndex2D()
+        rc = self.lib.GpuSpatialIndexFloat2DCreate(ctypes.byref(self.ix), ctypes.byref(self.cfg))
+        if rc:
+            raise GpuSpatialError(f"index create failed: {self.lib.SjhipLastError().decode()}")
+
+    try_new = classmethod(lambda cls, options=None: cls(options))
+
+    def _err(self):
+        return self.ix.get_last_error(ctypes.byref(self.ix)).decode()
+
+    def clear(self):
+        _check(self.ix.clear(ctypes.byref(self.ix)), self._err)
+
+    def push_build(self, rects):
+        r = np.ascontiguousarray(rects, dtype=np.float32).reshape(-1, 4)
+        _check(self.ix.push_build(ctypes.byref(self.ix), r.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(r)),
+               self._err)
+
+    def finish_building(self):
+        _check(self.ix.finish_building(ctypes.byref(self.ix)), self._err)
+
+    def build_from_refiner(self, refiner):
+        rc = self.lib.SjhipIndexBuildFromRefiner(ctypes.byref(self.ix), ctypes.byref(refiner.rf))
+        if rc:
+            raise GpuSpatialError(self.lib.SjhipLastError().decode())
+
+    def probe(self, rects, callback_rc=0):
+        """(build_indices, probe_indices) of all candidate pairs
+        (libgpuspatial.rs:239-308 + 532-572)."""
+        r = np.ascontiguousarray(rects, dtype=np.float32).reshape(-1, 4)
+        outs_b, outs_p = [], []
+
+        def cb(b, p, n, ud):
+            if n:
+                outs_b.append(np.ctypeslib.as_array(b, shape=(n,)).copy())
+                outs_p.append(np.ctypeslib.as_array(p, shape=(n,)).copy())
+            return callback_rc
+
+        ccb = L.PROBE_CB(cb)
+        ctx = L.SedonaSpatialIndexContext()
+        self.ix.create_context(ctypes.byref(ctx))
+        try:
+            rc = self.ix.probe(ctypes.byref(self.ix), ctypes.byref(ctx),
+                               r.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(r), ccb, None)
+            if rc:
+                msg = self.ix.context_get_last_error(ctypes.byref(ctx)).decode()
+                raise GpuSpatialError(f"probe failed (rc={rc}): {msg}")
+        finally:
+            self.ix.destroy_context(ctypes.byref(ctx))
+        if not outs_b:
+            return np.zeros(0, np.uint32), np.zeros(0, np.uint32)
+        return np.concatenate(outs_b), np.concatenate(outs_p)
+
+    def release(self):
+        if self.ix.private_data:
+            self.ix.release(ctypes.byref(self.ix))
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Refiner (lib.rs:137-191)
+
+
+class GpuSpatialRefiner:
+    def __init__(self, options=None, runtime=None):
+        self.options = options or GpuSpatialOptions()
+        self.lib = L.lib()
+        rt = runtime.rt if runtime is not None else global_runtime(self.options)
+        self._rt = rt
+        self.cfg = L.GpuSpatialRefinerConfig(ctypes.pointer(rt), self.options.concurrency, self.options.compress_bvh,
+                                             self.options.pipeline_batches)
+        self.rf = L.SedonaSpatialRefiner()
+        rc = self.lib.GpuSpatialRefinerCreate(ctypes.byref(self.rf), ctypes.byref(self.cfg))
+        if rc:
+            raise GpuSpatialError(f"refiner create failed: {self.lib.SjhipLastError().decode()}")
+
+    try_new = classmethod(lambda cls, options=None: cls(options))
+
+    def _err(self):
+        return self.rf.get_last_error(ctypes.byref(self.rf)).decode()
+
+    def clear(self):
+        _check(self.rf.clear(ctypes.byref(self.rf)), self._err)
+
+    def init_build_schema(self, format_or_array):
+        if isinstance(format_or_array, (bytes, str)):
+            sch = L.ArrowSchema()
+            sch.format = format_or_array.encode() if isinstance(format_or_array, str) else format_or_array
+            _check(self.rf.init_build_schema(ctypes.byref(self.rf), ctypes.byref(sch)), self._err)
+        else:
+            ex = ExportedArrow(format_or_array)
+            _check(self.rf.init_build_schema(ctypes.byref(self.rf), ctypes.byref(ex.schema)), self._err)
+
+    def push_build(self, arr):
+        ex = ExportedArrow(arr)
+        _check(self.rf.push_build(ctypes.byref(self.rf), ctypes.byref(ex.array)), self._err)
+
+    def push_build_device(self, dev):
+        rc = self.lib.SjhipRefinerPushBuildDevice(ctypes.byref(self.rf), ctypes.byref(dev.schema),
+                                                  ctypes.byref(dev.dev))
+        if rc:
+            raise GpuSpatialError(self.lib.SjhipLastError().decode())
+
+    def finish_building(self):
+        _check(self.rf.finish_building(ctypes.byref(self.rf)), self._err)
+
+    def refine(self, probe_array, predicate, build_indices, probe_indices):
+        """Filters the candidate pairs in place (libgpuspatial.rs:453-486):
+        returns the kept (build, probe) arrays sorted by (probe, build)."""
+        ex = ExportedArrow(probe_array)
+        b = _u32(build_indices).copy()
+        p = _u32(probe_indices).copy()
+        new_n = ctypes.c_uint32(0)
+        pred = PREDICATE_BY_NAME[predicate] if isinstance(predicate, str) else predicate
+        _check(self.rf.refine(ctypes.byref(self.rf), ctypes.byref(ex.schema), ctypes.byref(ex.array), int(pred),
+                              _ptr32(b), _ptr32(p), len(b), ctypes.byref(new_n)), self._err)
+        return b[:new_n.value], p[:new_n.value]
+
+    def refine_distance(self, probe_array, distance, build_indices, probe_indices):
+        ex = ExportedArrow(probe_array)
+        b = _u32(build_indices).copy()
+        p = _u32(probe_indices).copy()
+        new_n = ctypes.c_uint32(0)
+        _check(self.lib.SjhipRefinerRefineDistance(ctypes.byref(self.rf), ctypes.byref(ex.schema),
+                                                   ctypes.byref(ex.array), float(distance), _ptr32(b), _ptr32(p),
+                                                   len(b), ctypes.byref(new_n)), self._err)
+        return b[:new_n.value], p[:new_n.value]
+
+    def knn(self, probe_array, k):
+        ex = ExportedArrow(probe_array)
+        n = ex.array.length
+        ids = np.zeros((n, k), np.uint32)
+        d = np.zeros((n, k), np.float64)
+        _check(self.lib.SjhipRefinerKnn(ctypes.byref(self.rf), ctypes.byref(ex.schema), ctypes.byref(ex.array), k,
+                                        _ptr32(ids), d.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), self._err)
+        return ids, d
+
+    def release(self):
+        if self.rf.private_data:
+            self.rf.release(ctypes.byref(self.rf))
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Device-resident fused join
+
+
+class JoinResult:
+    def __init__(self, res):
+        self.res = res
+
+    @property
+    def size(self):
+        return self.res.size
+
+    @property
+    def candidates(self):
+        return self.res.candidates
+
+    @property
+    def segment_tests(self):
+        return self.res.segment_tests
+
+    @property
+    def fallback_tests(self):
+        return self.res.fallback_tests
+
+    def to_host(self):
+        n = self.res.size
+        b = np.zeros(max(n, 1), np.uint32)
+        p = np.zeros(max(n, 1), np.uint32)
+        rc = L.lib().SjhipJoinResultCopyToHost(ctypes.byref(self.res), _ptr32(b), _ptr32(p))
+        if rc:
+            raise GpuSpatialError(L.lib().SjhipLastError().decode())
+        return b[:n], p[:n]
+
+    def release(self):
+        if self.res.private_data:
+            self.res.release(ctypes.byref(self.res))
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def join_device(index, refiner, probe, predicate, distance=0.0, precision=0, sort_output=True, sort_probes=True,
+                stream=None):
+    """One probe batch through the fused device join.  `probe` is a
+    DeviceWkb (device memory) or any Arrow-exportable host column."""
+    if isinstance(predicate, str):
+        pred = 100 if predicate == "dwithin" else int(PREDICATE_BY_NAME[predicate])
+    else:
+        pred = int(predicate)
+    opts = L.SjhipJoinOptions(pred, float(distance), int(precision), 1 if sort_output else 0, 1 if sort_probes else 0,
+                              stream)
+    res = L.SjhipJoinResult()
+    if isinstance(probe, DeviceWkb):
+        sch, dev = probe.schema, probe.dev
+        keep = None
+    else:
+        keep = ExportedArrow(probe)
+        sch, dev = keep.schema, host_device_array(keep)
+    rc = L.lib().SjhipJoinDevice(ctypes.byref(index.ix), ctypes.byref(refiner.rf), ctypes.byref(sch),
+                                 ctypes.byref(dev), ctypes.byref(opts), ctypes.byref(res))
+    del keep
+    if rc:
+        raise GpuSpatialError(L.lib().SjhipLastError().decode())
+    return JoinResult(res)

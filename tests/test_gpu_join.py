@@ -1,0 +1,290 @@
+"""Parity tests of the HIP library on the GPU, through the C ABI.
+
+Each test compares the library's output with the golden vectors minted by
+the reference's TG library, or with the oracle on the same seeded inputs,
+bit for bit (pair sets, neighbour ids, distances)."""
+import sys
+
+import numpy as np
+import pytest
+
+import corpus as C
+import oracle as O
+from helpers import expected, pairs_equal, read_fixture
+from sjhip import datagen as D
+from sjhip import gpuspatial as G
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def opts():
+    return G.GpuSpatialOptions(concurrency=1, device_id=0)
+
+
+def build_pair(build_col, opts, from_refiner=True, point_nearest=True):
+    """Index + refiner built from the same build rows (the operator's
+    finish(), rust/sedona-spatial-join-gpu/src/index/gpu_spatial_index_builder.rs:150-259)."""
+    rf = G.GpuSpatialRefiner(opts)
+    rf.init_build_schema(build_col)
+    rf.push_build(build_col)
+    rf.finish_building()
+    ix = G.GpuSpatialIndex(opts)
+    if from_refiner:
+        ix.build_from_refiner(rf)
+    else:
+        ix.push_build(G.compute_rects(build_col, point_nearest=point_nearest))
+        ix.finish_building()
+    return ix, rf
+
+
+def arrow(col):
+    return col.to_arrow() if isinstance(col, D.WkbColumn) else col
+
+
+def corpus_arrow(cols):
+    import pyarrow as pa
+    off, data, _ = cols
+    return pa.Array.from_buffers(pa.binary(), len(off) - 1,
+                                 [None, pa.py_buffer(off.astype(np.int32)), pa.py_buffer(data)])
+
+
+FIXTURES = [
+    ("pip", "synthetic_pip_polygons.parquet", "synthetic_pip_points.parquet",
+     ["intersects", "contains", "covers", "within", "coveredby"]),
+    ("countries", "countries.parquet", "countries_points.parquet",
+     ["intersects", "contains", "covers", "within", "coveredby"]),
+    ("points_in_countries_rev", "countries_points.parquet", "countries.parquet", ["within", "intersects", "coveredby"]),
+    ("polypoly", "synthetic_pip_polygons.parquet", "synthetic_poly_polygons2.parquet", ["intersects"]),
+]
+
+
+@pytest.mark.parametrize("name,bf,pf,preds", FIXTURES)
+def test_c_abi_join_matches_reference_goldens(name, bf, pf, preds, opts):
+    """The flow of the reference's c_wrapper_test.cc:151-329 (index push /
+    finish / probe, refiner push / finish / refine), checked against the
+    pair sets of the reference's TG refine."""
+    say("case", name)
+    b, p = read_fixture(bf), read_fixture(pf)
+    ix, rf = build_pair(b, opts, from_refiner=False)
+    cb, cp = ix.probe(G.compute_rects(p, point_nearest=True))
+    E = expected()
+    for pred in preds:
+        say("  predicate", pred, "candidates", len(cb))
+        ob, op = rf.refine(p, pred, cb, cp)
+        want = (E[f"{name}/{pred}/build"], E[f"{name}/{pred}/probe"])
+        eq, miss, extra = pairs_equal((ob, op), want)
+        assert eq, (name, pred, miss, extra)
+        # sorted by (probe, build) like the reference's refiner output
+        key = (op.astype(np.uint64) << 32) | ob
+        assert np.all(np.diff(key.astype(np.int64)) > 0) or len(key) < 2
+
+
+def test_filter_matches_oracle_candidates(opts):
+    b = read_fixture("synthetic_pip_polygons.parquet")
+    p = read_fixture("synthetic_pip_points.parquet")
+    br, pr = G.compute_rects(b), G.compute_rects(p)
+    ob, op = O.filter_pairs(br, pr)
+    ix = G.GpuSpatialIndex(opts)
+    ix.push_build(br)
+    ix.finish_building()
+    gb, gp = ix.probe(pr)
+    eq, miss, extra = pairs_equal((gb, gp), (ob, op))
+    assert eq, (miss, extra)
+
+
+def test_device_rects_match_reference_rounding():
+    """bounds.rs:147-162 outward rounding and the GPU operator's nearest
+    rounding for points, as restated by the oracle."""
+    for f in ["countries.parquet", "synthetic_pip_polygons.parquet", "countries_points.parquet"]:
+        col = read_fixture(f)
+        st = O.Store(col)
+        for nearest in (True, False):
+            got = G.compute_rects(col, point_nearest=nearest)
+            want = st.rects(point_nearest=nearest)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (f, nearest)
+    pts = read_fixture("countries_points.parquet")
+    got = G.compute_rects(pts, point_nearest=False, expand_distance=0.05)
+    want = O.expand_rects(O.Store(pts).rects(point_nearest=False), 0.05)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_corpus_all_pairs_match_reference(precision, opts):
+    """Every (polygon, point) pair of the adversarial corpus, both roles."""
+    cb, cp = C.corpus()
+    b, p = corpus_arrow(cb), corpus_arrow(cp)
+    nb, np_ = len(b), len(p)
+    bi = np.repeat(np.arange(nb), np_).astype(np.uint32)
+    pi = np.tile(np.arange(np_), nb).astype(np.uint32)
+    E = expected()
+    rf = G.GpuSpatialRefiner(opts)
+    rf.init_build_schema(b)
+    rf.push_build(b)
+    rf.finish_building()
+    for pred in ["intersects", "contains", "covers", "within", "coveredby"]:
+        say("corpus", pred)
+        want = np.unpackbits(E[f"corpus/{pred}/mask"])[: len(bi)].astype(bool)
+        ob, op = rf.refine(p, pred, bi, pi)
+        eq, miss, extra = pairs_equal((ob, op), (bi[want], pi[want]))
+        assert eq, (pred, miss, extra)
+    rrf = G.GpuSpatialRefiner(opts)
+    rrf.init_build_schema(p)
+    rrf.push_build(p)
+    rrf.finish_building()
+    for pred in ["intersects", "within", "coveredby"]:
+        want = np.unpackbits(E[f"corpus_rev/{pred}/mask"])[: len(bi)].astype(bool)
+        ob, op = rrf.refine(b, pred, pi, bi)
+        eq, miss, extra = pairs_equal((ob, op), (pi[want], bi[want]))
+        assert eq, ("rev", pred, miss, extra)
+    bb = np.repeat(np.arange(nb), nb).astype(np.uint32)
+    pp = np.tile(np.arange(nb), nb).astype(np.uint32)
+    want = np.unpackbits(E["corpus_polypoly/intersects/mask"])[: len(bb)].astype(bool)
+    ob, op = rf.refine(b, "intersects", bb, pp)
+    assert pairs_equal((ob, op), (bb[want], pp[want]))[0]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("sort_probes", [True, False])
+def test_device_join_uniform_stars_golden(precision, sort_probes, opts):
+    b, p = D.workload_pip_uniform(20_000, 200, seed=7, extent=100.0 * np.sqrt(200 / 10_000))
+    ix, rf = build_pair(b.to_arrow(), opts)
+    r = G.join_device(ix, rf, p.to_arrow(), "intersects", precision=precision, sort_probes=sort_probes)
+    gb, gp = r.to_host()
+    E = expected()
+    assert np.array_equal(gb, E["uniform_small/intersects/build"])
+    assert np.array_equal(gp, E["uniform_small/intersects/probe"])
+
+
+@pytest.mark.parametrize("pred", ["contains", "intersects", "covers"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_device_join_zipf_stars_vs_oracle(pred, precision, opts):
+    """The 100M x 1M workload's distributions (Zipf vertex counts up to 4096,
+    clustered points) at 1/1000 scale, device-resident probe batch."""
+    b, p = D.workload_pip_zipf(100_000, 1_000, seed=11)
+    say("zipf", pred, precision, "vertices", int(b.offsets[-1]))
+    ix, rf = build_pair(b.to_arrow(), opts)
+    dev = G.DeviceWkb.from_torch(p)
+    r = G.join_device(ix, rf, dev, pred, precision=precision)
+    gb, gp = r.to_host()
+    ob, op = O.join(O.Store(b.as_tuple()), O.Store(p.as_tuple()), pred, nthreads=8)
+    assert np.array_equal(gb, ob) and np.array_equal(gp, op), (len(gb), len(ob))
+    if precision == 0:
+        assert r.segment_tests > 0
+        say("  fallback rate", r.fallback_tests / max(1, r.segment_tests))
+
+
+def test_dwithin_random_walks_vs_oracle(opts):
+    b, p = D.workload_dwithin_walks(200_000, 20_000, seed=13)
+    ix, rf = build_pair(b.to_arrow(), opts)
+    r = G.join_device(ix, rf, p.to_arrow(), "dwithin", distance=0.05)
+    gb, gp = r.to_host()
+    ob, op = O.join(O.Store(b.as_tuple()), O.Store(p.as_tuple()), "dwithin", distance=0.05, nthreads=8)
+    assert np.array_equal(gb, ob) and np.array_equal(gp, op), (len(gb), len(ob))
+    assert len(gb) > 1000
+
+
+def test_dwithin_polygons_and_refine_distance_abi(opts):
+    b, p = D.workload_pip_uniform(20_000, 200, seed=17, extent=100.0 * np.sqrt(200 / 10_000))
+    ix, rf = build_pair(b.to_arrow(), opts)
+    r = G.join_device(ix, rf, p.to_arrow(), "dwithin", distance=0.05)
+    gb, gp = r.to_host()
+    ob, op = O.join(O.Store(b.as_tuple()), O.Store(p.as_tuple()), "dwithin", distance=0.05, nthreads=8)
+    assert np.array_equal(gb, ob) and np.array_equal(gp, op)
+    # the ABI extension over oracle candidates of the expanded filter
+    pr = O.expand_rects(O.Store(p.as_tuple()).rects(False), 0.05)
+    cb, cp = O.filter_pairs(O.Store(b.as_tuple()).rects(False), pr)
+    kb, kp = rf.refine_distance(p.to_arrow(), 0.05, cb, cp)
+    assert pairs_equal((kb, kp), (ob, op))[0]
+
+
+@pytest.mark.parametrize("k", [1, 8])
+def test_knn_exact_with_id_ties(k, opts):
+    rng = np.random.default_rng(19)
+    obj = np.round(rng.uniform(0, 50, size=(200_000, 2)) * 8) / 8  # grid: exact ties
+    q = D.clustered_xy(rng, 20_000, 50.0, n_clusters=8)
+    q[::5] = np.round(q[::5] * 8) / 8
+    bcol = D.points_wkb(obj[:, 0], obj[:, 1])
+    qcol = D.points_wkb(q[:, 0], q[:, 1])
+    rf = G.GpuSpatialRefiner(opts)
+    rf.init_build_schema(bcol.to_arrow())
+    rf.push_build(bcol.to_arrow())
+    rf.finish_building()
+    ids, d = rf.knn(qcol.to_arrow(), k)
+    oid, od = O.knn(O.Store(bcol.as_tuple()), O.Store(qcol.as_tuple()), k, nthreads=8)
+    assert np.array_equal(ids, oid)
+    assert np.array_equal(d.view(np.uint64), od.view(np.uint64))
+
+
+def test_decode_errors_name_the_row(opts):
+    rf = G.GpuSpatialRefiner(opts)
+    for bad in C.invalid_wkbs():
+        col = corpus_arrow(C.column([C.point_wkb(0, 0), C.point_wkb(1, 1), bad]))
+        rf.clear()
+        rf.init_build_schema(col)
+        with pytest.raises(G.GpuSpatialError, match="row 2"):
+            rf.push_build(col)
+
+
+def test_nulls_empties_big_endian_large_and_view(opts):
+    import pyarrow as pa
+    polys = [C.polygon_wkb([[(0, 0), (4, 0), (4, 4), (0, 4), (0, 0)]], big_endian=True), None,
+             C.polygon_wkb([[(5, 5), (9, 5), (9, 9), (5, 9), (5, 5)]]),
+             b"\x01" + (3).to_bytes(4, "little") + (0).to_bytes(4, "little")]
+    pts = [C.point_wkb(1, 1, big_endian=True), C.point_wkb(6, 6), None, C.point_wkb(float("nan"), float("nan")),
+           C.point_wkb(4, 2), C.point_wkb(7, 7, big_endian=True)]
+    want = {(0, 0), (2, 1), (0, 4), (2, 5)}
+    for typ in (pa.binary(), pa.large_binary(), pa.binary_view()):
+        b = pa.array(polys, type=typ)
+        p = pa.array(pts, type=typ)
+        ix, rf = build_pair(b, opts)
+        cb, cp = ix.probe(G.compute_rects(p))
+        ob, op = rf.refine(p, "intersects", cb, cp)
+        assert set(zip(ob.tolist(), op.tolist())) == want, typ
+        ob, op = rf.refine(p, "contains", cb, cp)
+        assert set(zip(ob.tolist(), op.tolist())) == {(0, 0), (2, 1), (2, 5)}, typ
+
+
+def test_empty_build_and_probe(opts):
+    import pyarrow as pa
+    empty = pa.array([], type=pa.binary())
+    ix, rf = build_pair(empty, opts)
+    p = D.points_wkb(np.array([1.0]), np.array([2.0])).to_arrow()
+    cb, cp = ix.probe(G.compute_rects(p))
+    assert len(cb) == 0
+    ob, op = rf.refine(p, "intersects", cb, cp)
+    assert len(ob) == 0
+    b = D.points_wkb(np.array([1.0]), np.array([2.0])).to_arrow()
+    ix2, rf2 = build_pair(b, opts)
+    cb, cp = ix2.probe(np.zeros((0, 4), np.float32))
+    assert len(cb) == 0
+
+
+def test_multi_batch_build_ids_are_global(opts):
+    """Build ids are global in push order across push_build calls
+    (gpuspatial_c.h:186-192)."""
+    b, p = D.workload_pip_uniform(5_000, 60, seed=23, extent=100.0 * np.sqrt(60 / 10_000))
+    parts = [b.slice(0, 20), b.slice(20, 45), b.slice(45, 60)]
+    rf = G.GpuSpatialRefiner(G.GpuSpatialOptions())
+    rf.init_build_schema(parts[0].to_arrow())
+    for q in parts:
+        rf.push_build(q.to_arrow())
+    rf.finish_building()
+    ix = G.GpuSpatialIndex(G.GpuSpatialOptions())
+    ix.build_from_refiner(rf)
+    gb, gp = G.join_device(ix, rf, p.to_arrow(), "intersects").to_host()
+    ob, op = O.join(O.Store(b.as_tuple()), O.Store(p.as_tuple()), "intersects")
+    assert np.array_equal(gb, ob) and np.array_equal(gp, op)
+
+
+def test_probe_callback_stop_propagates(opts):
+    b = read_fixture("synthetic_pip_polygons.parquet")
+    p = read_fixture("synthetic_pip_points.parquet")
+    ix, _ = build_pair(b, opts, from_refiner=False)
+    with pytest.raises(G.GpuSpatialError):
+        ix.probe(G.compute_rects(p), callback_rc=7)

@@ -1,0 +1,239 @@
+"""Pins the CPU oracle (oracle/sjoin_oracle.c) before anything is checked
+against it: golden vectors minted with the reference's own TG library, the
+reference's hand-written known answers, closed forms and brute force."""
+import struct
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import corpus as C
+import oracle as O
+from helpers import expected, read_fixture
+
+FIXTURES = [
+    ("pip", "synthetic_pip_polygons.parquet", "synthetic_pip_points.parquet",
+     ["intersects", "contains", "covers", "within", "coveredby"]),
+    ("countries", "countries.parquet", "countries_points.parquet",
+     ["intersects", "contains", "covers", "within", "coveredby"]),
+    ("polypoly", "synthetic_pip_polygons.parquet", "synthetic_poly_polygons2.parquet", ["intersects"]),
+    ("points_in_countries_rev", "countries_points.parquet", "countries.parquet", ["within", "intersects", "coveredby"]),
+]
+
+
+@pytest.mark.parametrize("name,bf,pf,preds", FIXTURES)
+def test_oracle_matches_reference_goldens_on_reference_fixtures(name, bf, pf, preds):
+    B, P = O.Store(read_fixture(bf)), O.Store(read_fixture(pf))
+    E = expected()
+    for pred in preds:
+        ob, op = O.join(B, P, pred, nthreads=4)
+        assert np.array_equal(ob, E[f"{name}/{pred}/build"]), (name, pred)
+        assert np.array_equal(op, E[f"{name}/{pred}/probe"]), (name, pred)
+
+
+def test_oracle_matches_reference_on_adversarial_corpus():
+    cb, cp = C.corpus()
+    B, P = O.Store(cb), O.Store(cp)
+    nb, np_ = B.n, P.n
+    bi = np.repeat(np.arange(nb), np_).astype(np.uint32)
+    pi = np.tile(np.arange(np_), nb).astype(np.uint32)
+    E = expected()
+    for pred in ["intersects", "contains", "covers", "within", "coveredby"]:
+        want = np.unpackbits(E[f"corpus/{pred}/mask"])[: len(bi)].astype(bool)
+        assert np.array_equal(O.refine(B, P, pred, bi, pi), want), pred
+        want2 = np.unpackbits(E[f"corpus_rev/{pred}/mask"])[: len(bi)].astype(bool)
+        assert np.array_equal(O.refine(P, B, pred, pi, bi), want2), pred
+    bb = np.repeat(np.arange(nb), nb).astype(np.uint32)
+    pp = np.tile(np.arange(nb), nb).astype(np.uint32)
+    want = np.unpackbits(E["corpus_polypoly/intersects/mask"])[: len(bb)].astype(bool)
+    assert np.array_equal(O.refine(B, B, "intersects", bb, pp), want)
+
+
+def test_corpus_reaches_every_ray_branch():
+    """The corpus must exercise ON results and vertex-height nudges, or the
+    pins above would not constrain those branches."""
+    cb, cp = C.corpus()
+    st_b, st_p = O.Store(cb).soa(), O.Store(cp).soa()
+    pts = st_p["xy"][st_p["pt_off"][:-1]]
+    xy = st_b["xy"]
+    counts = {0: 0, 1: 0, 2: 0}
+    vertex_height = 0
+    for r in range(len(st_b["pt_off"]) - 1):
+        ring = xy[st_b["pt_off"][r]:st_b["pt_off"][r + 1]]
+        for i in range(len(ring) - 1):
+            a, b = ring[i], ring[i + 1]
+            for p in pts[::7]:
+                rc = O.raycast(a[0], a[1], b[0], b[1], p[0], p[1])
+                counts[rc] += 1
+                if rc == 1 and (p[1] == a[1] or p[1] == b[1]):
+                    vertex_height += 1
+    assert counts[0] > 1000 and counts[1] > 100 and counts[2] > 50
+    assert vertex_height > 0
+
+
+def test_oracle_matches_reference_on_uniform_stars():
+    from sjhip import datagen as D
+    b, p = D.workload_pip_uniform(20_000, 200, seed=7, extent=100.0 * np.sqrt(200 / 10_000))
+    ob, op = O.join(O.Store(b.as_tuple()), O.Store(p.as_tuple()), "intersects")
+    E = expected()
+    assert np.array_equal(ob, E["uniform_small/intersects/build"])
+    assert np.array_equal(op, E["uniform_small/intersects/probe"])
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference TG library not built here")
+@pytest.mark.parametrize("seed", [21, 22])
+def test_oracle_matches_reference_tg_on_fresh_workloads(seed):
+    from sjhip import datagen as D
+    b, p = D.workload_pip_zipf(30_000, 300, seed=seed, max_vertices=512)
+    B, P = O.Store(b.as_tuple()), O.Store(p.as_tuple())
+    tb, tp = O.TgColumn(b.as_tuple(), index=1), O.TgColumn(p.as_tuple(), index=0)
+    for pred in ["intersects", "contains"]:
+        rb, rp = O.ref_join(tb, tp, B.rects(True), P.rects(True), pred)
+        ob, op = O.join(B, P, pred)
+        assert np.array_equal(ob, rb) and np.array_equal(op, rp)
+
+
+def _poly(rings):
+    return C.polygon_wkb(rings)
+
+
+def test_reference_known_answers_tg_predicates():
+    """c/sedona-tg/src/tg.rs:299-359 (point / polygon cases)."""
+    tri = _poly([[(0, 0), (5, 0), (0, 5), (0, 0)]])
+    holed = _poly([[(0, 0), (3, 0), (3, 3), (0, 3), (0, 0)], [(1, 1), (1, 2), (2, 1), (1, 1)]])
+    cases = [
+        ("contains", tri, C.point_wkb(1, 1), True),
+        ("contains", tri, C.point_wkb(-1, -1), False),
+        ("covers", tri, C.point_wkb(1, 1), True),
+        ("covers", tri, C.point_wkb(-1, -1), False),
+        ("intersects", C.point_wkb(0, 1), C.point_wkb(0, 1), True),
+        ("intersects", C.point_wkb(0, 1), C.point_wkb(0, 0), False),
+        ("equals", C.point_wkb(0, 1), C.point_wkb(0, 1), True),
+        ("within", C.point_wkb(1, 1), tri, True),
+        ("within", C.point_wkb(-1, -1), tri, False),
+        ("coveredby", C.point_wkb(1, 1), tri, True),
+        ("coveredby", C.point_wkb(-1, -1), tri, False),
+        # a point inside the hole is not contained
+        ("contains", holed, C.point_wkb(1.25, 1.25), False),
+        ("contains", holed, C.point_wkb(2.5, 2.5), True),
+        # on the hole's boundary: covered, not contained
+        ("covers", holed, C.point_wkb(1.0, 1.5), True),
+        ("contains", holed, C.point_wkb(1.0, 1.5), False),
+    ]
+    for pred, a, b, want in cases:
+        A, B = O.Store(C.column([a])), O.Store(C.column([b]))
+        got = O.refine(A, B, pred, np.zeros(1, np.uint32), np.zeros(1, np.uint32))[0]
+        assert got == want, (pred, want)
+
+
+def test_reference_known_answer_touch_top_edge():
+    """c/sedona-tg/src/tg.rs:419-445 (tidwall/tg#16)."""
+    pts = [(0.0, 0.0), (1.0, 0.0), (1.0, 1.0), (0.0, 1.0)] + [(0.0, i / 100.0) for i in range(99, -1, -1)]
+    A = O.Store(C.column([_poly([pts])]))
+    B = O.Store(C.column([C.point_wkb(0.5, 1.0)]))
+    assert O.refine(A, B, "intersects", np.zeros(1, np.uint32), np.zeros(1, np.uint32))[0]
+
+
+def test_reference_known_answer_intersects_vertex():
+    """c/sedona-tg/src/tg.rs:381-417: a point on a vertex of a real polygon."""
+    coords = [(-118.2936175, 34.026516), (-118.2935834, 34.026516), (-118.2935834, 34.0264634),
+              (-118.2935834, 34.0263308), (-118.2935834, 34.0262071), (-118.2935834, 34.0260835),
+              (-118.2935834, 34.0259598), (-118.2935835, 34.02592), (-118.2934886, 34.02592),
+              (-118.2934886, 34.0259073), (-118.2934259, 34.0259073), (-118.2934259, 34.02592),
+              (-118.2933447, 34.0259205), (-118.2933443, 34.0260829), (-118.2933442, 34.0262066),
+              (-118.2933442, 34.0263303), (-118.2933442, 34.0264637), (-118.2933442, 34.0264907),
+              (-118.2933259, 34.0264907), (-118.2933259, 34.0265669), (-118.2933035, 34.0265669),
+              (-118.2931745, 34.0265669), (-118.2931745, 34.0265216), (-118.2931693, 34.0265191),
+              (-118.2931876, 34.0263539), (-118.2931384, 34.0263608), (-118.2929819, 34.0263827),
+              (-118.2929795, 34.0264157), (-118.292977, 34.0266764), (-118.2929762, 34.026758),
+              (-118.2929659, 34.026758), (-118.2929658, 34.0267856), (-118.2931085, 34.0267856),
+              (-118.2931085, 34.0267629), (-118.2931722, 34.0267629), (-118.2931722, 34.0267683),
+              (-118.293303, 34.0267683), (-118.2934675, 34.0267684), (-118.2934675, 34.0267859),
+              (-118.2934684, 34.0267859), (-118.2935959, 34.0267859), (-118.2935959, 34.0267078),
+              (-118.2936175, 34.0267078), (-118.2936175, 34.0265897), (-118.2936175, 34.026516)]
+    A = O.Store(C.column([_poly([coords])]))
+    B = O.Store(C.column([C.point_wkb(-118.2934684, 34.0267859)]))
+    assert O.refine(A, B, "intersects", np.zeros(1, np.uint32), np.zeros(1, np.uint32))[0]
+
+
+def test_invalid_wkb_rejected_like_reference():
+    for i, w in enumerate(C.invalid_wkbs()):
+        with pytest.raises(ValueError):
+            O.Store(C.column([C.point_wkb(0, 0), w]))
+
+
+# ---------------------------------------------------------------------------
+# Distance semantics (reference: rust/sedona-geo/src/st_dwithin.rs:114-175)
+
+
+def _dw(a, b, d):
+    A, B = O.Store(C.column([a])), O.Store(C.column([b]))
+    return O.refine(A, B, "dwithin", np.zeros(1, np.uint32), np.zeros(1, np.uint32), distance=d)[0]
+
+
+def test_dwithin_known_answers():
+    assert _dw(C.point_wkb(0, 0), C.point_wkb(3, 4), 5.0)
+    assert not _dw(C.point_wkb(0, 0), C.point_wkb(3, 4), 4.0)
+    sq = _poly([[(0, 0), (0, 1), (1, 1), (1, 0), (0, 0)]])
+    assert _dw(sq, C.point_wkb(0.5, 0.5), 1.0)
+    assert not _dw(sq, C.point_wkb(5, 5), 1.0)
+    line = C.linestring_wkb([(0, 0), (10, 0)])
+    assert _dw(line, C.point_wkb(5, 0.05), 0.05)
+    assert not _dw(line, C.point_wkb(5, 0.0500001), 0.05)
+    # on the line: exact zero (robust orientation), also for d = 0
+    assert _dw(C.linestring_wkb([(0, 0), (3, 1)]), C.point_wkb(1.5, 0.5), 0.0)
+
+
+def _exact_seg_dist2(px, py, sx, sy, ex, ey):
+    P, S, E = [Fraction(v) for v in (px, py)], [Fraction(v) for v in (sx, sy)], [Fraction(v) for v in (ex, ey)]
+    dx, dy = E[0] - S[0], E[1] - S[1]
+    d2 = dx * dx + dy * dy
+    if d2 == 0:
+        return (P[0] - S[0]) ** 2 + (P[1] - S[1]) ** 2
+    r = ((P[0] - S[0]) * dx + (P[1] - S[1]) * dy) / d2
+    r = min(max(r, Fraction(0)), Fraction(1))
+    qx, qy = S[0] + r * dx, S[1] + r * dy
+    return (P[0] - qx) ** 2 + (P[1] - qy) ** 2
+
+
+def test_distance_against_exact_rationals():
+    """Point-to-linestring distance within 1e-12 relative of the exact
+    distance (closed form on rationals)."""
+    rng = np.random.default_rng(5)
+    for _ in range(300):
+        pts = [tuple(rng.uniform(0, 10, 2)) for _ in range(int(rng.integers(2, 6)))]
+        p = tuple(rng.uniform(0, 10, 2))
+        A, B = O.Store(C.column([C.linestring_wkb(pts)])), O.Store(C.column([C.point_wkb(*p)]))
+        got = O.distance(A, 0, B, 0)
+        ex2 = min(_exact_seg_dist2(p[0], p[1], *pts[i], *pts[i + 1]) for i in range(len(pts) - 1))
+        ex = float(ex2) ** 0.5
+        assert abs(got - ex) <= 1e-12 * max(ex, 1e-300)
+
+
+# ---------------------------------------------------------------------------
+# KNN (reference ground truth: spatial_join_integration.rs:1373-1414)
+
+
+def test_knn_matches_brute_force_with_id_ties():
+    rng = np.random.default_rng(9)
+    obj = np.round(rng.uniform(0, 20, size=(3000, 2)) * 4) / 4  # grid: many exact ties
+    q = np.round(rng.uniform(0, 20, size=(400, 2)) * 4) / 4
+    from sjhip import datagen as D
+    B = O.Store(D.points_wkb(obj[:, 0], obj[:, 1]).as_tuple())
+    P = O.Store(D.points_wkb(q[:, 0], q[:, 1]).as_tuple())
+    for k in (1, 5, 8):
+        ids, d = O.knn(B, P, k)
+        dd = np.hypot(q[:, None, 0] - obj[None, :, 0], q[:, None, 1] - obj[None, :, 1])
+        order = np.lexsort((np.broadcast_to(np.arange(len(obj)), dd.shape), dd), axis=1)[:, :k]
+        assert np.array_equal(ids, order.astype(np.uint32))
+        assert np.array_equal(d, np.take_along_axis(dd, order, axis=1))
+
+
+def test_rect_rounding_is_outward_and_tight():
+    """bounds.rs:147-162: lo rounds down, hi rounds up, at most one ulp."""
+    rng = np.random.default_rng(3)
+    for v in rng.uniform(-1e4, 1e4, 2000):
+        lo, hi = O.f64_interval_to_f32(v, v)
+        assert lo <= v <= hi
+        assert np.nextafter(np.float32(lo), np.float32(np.inf)) >= np.float32(v) or lo == v
+        assert hi == lo or np.nextafter(np.float32(lo), np.float32(np.inf)) == np.float32(hi)
