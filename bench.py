@@ -1,0 +1,239 @@
+"""Benchmark: refined join pairs/s of the spatial join on one MI355X.
+
+Workload (default): 100M probe points (80 % from 64 isotropic Gaussians,
+sigma 2, centres U[0,1000]^2; 20 % uniform) x 1M build star polygons
+(vertex counts clip(Zipf(1.2), 4, 4096), radius lognormal median 1),
+ST_Contains(build polygon, probe point), FP64 WKB columns, seeded.
+
+One step = one probe batch through the device-resident join: device WKB
+decode of the 100M probe points + FP32 rects, Morton order of the probes,
+filter over the resident polygon index, refine (certified FP32 + FP64
+fallback), compaction of the refined pairs.  The build side (decode, ring
+index, rect tree) is built once before the timed region and reported as
+build_ms.  Inputs are resident in HBM when the timed region starts.
+
+The CPU baseline is the reference's own TG refine (oracle/_ref, built from
+the reference sources) behind a grid filter, timed on this host's cores over
+a bounded vertical strip of the same workload; the GPU's full-size pair set
+restricted to that strip must equal the CPU pairs exactly.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "sedona-db_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+FLOPS_PER_SEGMENT_TEST = 8  # two subtractions x2, cross product (3), compare
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=float, default=1.0, help="fraction of the full workload (both sides)")
+    ap.add_argument("--precision", type=int, default=0, help="0 = FP32 certified + FP64 fallback, 1 = all FP64")
+    ap.add_argument("--cpu-strip", type=float, default=0.004, help="CPU sample: strip width / extent")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the all-FP64 comparison step")
+    ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=2)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one MI355X: every box in this build has one GPU, so --gpus must be 1 "
+              "(the multi-GPU sharding plan is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from sjhip import datagen as D
+
+    n_points = int(round(100_000_000 * args.scale))
+    n_polys = int(round(1_000_000 * args.scale))
+    t0 = time.time()
+    wl = D.workload_pip_zipf_parallel(n_points, n_polys, seed=args.seed, workers=args.workers)
+    build, probe = wl["build"], wl["probe"]
+    log(f"generated {n_polys} polygons ({int(wl['counts'].sum()) + n_polys} vertices, "
+        f"{build.offsets[-1] / 1e9:.1f} GB WKB) and {n_points} points in {time.time() - t0:.1f}s")
+
+    # CPU baseline sample (strip) is cut before any GPU work
+    strip = None
+    if not args.no_cpu:
+        ext = wl["extent"]
+        x0 = 0.5 * ext
+        w = args.cpu_strip * ext
+        brows = np.nonzero((wl["centers"][:, 0] >= x0) & (wl["centers"][:, 0] < x0 + w))[0]
+        prows = np.nonzero((wl["probe_xy"][:, 0] >= x0) & (wl["probe_xy"][:, 0] < x0 + w))[0]
+        strip = dict(brows=brows, prows=prows, bcol=D.take_rows(build, brows), pcol=D.take_rows(probe, prows),
+                     x0=x0, w=w)
+        log(f"cpu strip: {len(brows)} polygons, {len(prows)} points")
+
+    from sjhip import gpuspatial as G
+
+    opts = G.GpuSpatialOptions(device_id=0)
+    tb = time.time()
+    rf = G.GpuSpatialRefiner(opts)
+    rf.init_build_schema("Z")
+    bdev = G.DeviceWkb.from_host(build)
+    rf.push_build_device(bdev)
+    rf.finish_building()
+    del bdev
+    ix = G.GpuSpatialIndex(opts)
+    ix.build_from_refiner(rf)
+    build_ms = (time.time() - tb) * 1000
+    log(f"build side resident: {build_ms:.0f} ms (includes host->device copy)")
+    pdev = G.DeviceWkb.from_host(probe)
+
+    def step(precision, profile=True):
+        return G.join_device(ix, rf, pdev, "contains", precision=precision, sort_output=False, sort_probes=True,
+                             profile=profile)
+
+    for i in range(args.warmup):
+        r = step(args.precision)
+        log(f"warmup {i}: {r.size} pairs, {r.candidates} candidates, stages {list(r.res.stage_ms)}")
+        r.release()
+    stage = np.zeros(5)
+    sizes = []
+    t = time.time()
+    for i in range(args.steps):
+        r = step(args.precision)
+        stage += np.array(list(r.res.stage_ms))
+        sizes.append((r.size, r.candidates, r.segment_tests, r.fallback_tests))
+        if i < args.steps - 1:
+            r.release()
+    elapsed = time.time() - t
+    last = r
+    ms_per_step = elapsed * 1000 / args.steps
+    pairs, cands, seg_tests, fb = sizes[-1]
+    assert all(s[0] == pairs for s in sizes), "pair count changed between steps"
+    stage /= args.steps
+    value = pairs / (ms_per_step / 1000)
+    log(f"timed: {ms_per_step:.2f} ms/step, {pairs} pairs, stages(ms) decode {stage[0]:.2f} morton {stage[1]:.2f} "
+        f"filter {stage[2]:.2f} refine {stage[3]:.2f}")
+
+    gb_pairs = gp_pairs = None
+    if strip is not None:
+        gb_pairs, gp_pairs = last.to_host()
+    last.release()
+
+    plain = None
+    if not args.no_plain:
+        r1 = step(1 - args.precision)
+        t1 = time.time()
+        r1b = step(1 - args.precision)
+        plain_ms = (time.time() - t1) * 1000
+        plain = dict(precision=1 - args.precision, ms_per_step=plain_ms, pairs=int(r1b.size),
+                     stage_ms=list(r1b.res.stage_ms), same_pair_count=bool(r1b.size == pairs))
+        r1.release()
+        r1b.release()
+
+    # roofline of the dominant kernel (filter: HBM; refine: FP32 VALU)
+    filt_ms, ref_ms = stage[2], stage[3]
+    filter_bytes = 16 * n_points + 8 * cands
+    verts32 = 8 * (int(wl["counts"].sum()) + n_polys)
+    refine_bytes = 8 * cands + 8 * pairs + verts32
+    refine_flops = FLOPS_PER_SEGMENT_TEST * seg_tests
+    roof_filter = dict(bound="hbm", achieved=filter_bytes / (filt_ms / 1e3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
+                       traffic=None, kernel="k_filter", ms=filt_ms)
+    roof_filter["frac"] = roof_filter["achieved"] / HBM_PEAK_GBS
+    ach_tf = refine_flops / (ref_ms / 1e3) / 1e12
+    roof_refine = dict(bound="valu-fp32", achieved=ach_tf, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s", traffic=None,
+                       kernel="k_refine_lane+k_refine_wave", ms=ref_ms,
+                       hbm_gbs=refine_bytes / (ref_ms / 1e3) / 1e9)
+    roof_refine["frac"] = ach_tf / FP32_PEAK_TFLOPS
+    dominant = roof_refine if ref_ms >= filt_ms else roof_filter
+
+    cpu = None
+    if strip is not None:
+        cpu = cpu_baseline(strip, gb_pairs, gp_pairs, args)
+
+    out = {
+        "metric": "refined join pairs/s (ST_Contains, 100M points x 1M Zipf star polygons, device-resident)",
+        "value": value,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == 1 else "f32+f64",
+        "data": "synthetic (seeded, generated on the host)",
+        "config": {"workload": "pip_zipf_contains", "probe_points": n_points, "build_polygons": n_polys,
+                   "build_vertices": int(wl["counts"].sum()) + n_polys, "predicate": "ST_Contains",
+                   "precision": "fp32-certified+fp64-fallback" if args.precision == 0 else "fp64-plain",
+                   "parallelism": "single-gpu"},
+        "roofline": {k: dominant[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
+        "roofline_kernel": dominant["kernel"],
+        "roofline_stages": {"filter": roof_filter, "refine": roof_refine},
+        "cpu_baseline": cpu,
+        "pairs_per_step": int(pairs),
+        "candidates_per_step": int(cands),
+        "probe_points_per_s": n_points / (ms_per_step / 1000),
+        "stage_ms": {"decode_rects": stage[0], "morton": stage[1], "filter": stage[2], "refine": stage[3]},
+        "segment_tests": int(seg_tests),
+        "fallback_rate": fb / max(1, seg_tests),
+        "build_ms": build_ms,
+        "plain_fp64": plain,
+    }
+    if plain and args.precision == 0:
+        out["speedup_vs_plain_fp64"] = plain["ms_per_step"] / ms_per_step
+    print(json.dumps(out), flush=True)
+    D.release_shared(wl["keep"])
+
+
+def cpu_baseline(strip, gb, gp, args):
+    """The reference's TG refine (oracle/_ref) behind a grid filter on the
+    strip sample; pairs/s on this host's cores.  Also checks that the GPU's
+    full-size pairs restricted to the strip equal the CPU pairs."""
+    import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    kind = "reference" if O.ref_available() else "port"
+    bcol, pcol = strip["bcol"], strip["pcol"]
+    t = time.time()
+    if kind == "reference":
+        st_b, st_p = O.Store(bcol.as_tuple()), O.Store(pcol.as_tuple())
+        br, pr = st_b.rects(True), st_p.rects(True)
+        tb = O.TgColumn(bcol.as_tuple(), index=1, nthreads=threads)
+        tp = O.TgColumn(pcol.as_tuple(), index=0, nthreads=threads)
+        cb, cp = O.ref_join(tb, tp, br, pr, "contains", nthreads=threads)
+    else:
+        st_b, st_p = O.Store(bcol.as_tuple()), O.Store(pcol.as_tuple())
+        cb, cp = O.join(st_b, st_p, "contains", nthreads=threads)
+    secs = time.time() - t
+    res = {"value": len(cb) / secs, "unit": "pairs/s", "cores": threads, "kind": kind,
+           "sample": f"vertical strip x in [{strip['x0']:.1f}, {strip['x0'] + strip['w']:.1f}): "
+                     f"{len(strip['brows'])} polygons x {len(strip['prows'])} points, {len(cb)} pairs, "
+                     f"{secs:.2f} s (WKB parse + filter + refine)"}
+    if gb is not None:
+        bmap = np.full(int(gb.max()) + 1 if len(gb) else 1, -1, np.int64)
+        sel_b = strip["brows"]
+        bmap[sel_b[sel_b < len(bmap)]] = np.nonzero(sel_b < len(bmap))[0]
+        pset = np.full(int(gp.max()) + 1 if len(gp) else 1, -1, np.int64)
+        sel_p = strip["prows"]
+        pset[sel_p[sel_p < len(pset)]] = np.nonzero(sel_p < len(pset))[0]
+        lb = bmap[gb]
+        lp = pset[gp]
+        keep = (lb >= 0) & (lp >= 0)
+        gk = np.sort((lp[keep].astype(np.uint64) << 32) | lb[keep].astype(np.uint64))
+        ck = np.sort((cp.astype(np.uint64) << 32) | cb.astype(np.uint64))
+        res["gpu_pairs_in_sample"] = int(len(gk))
+        res["pair_set_diff"] = int(len(np.setxor1d(gk, ck)))
+    return res
+
+
+if __name__ == "__main__":
+    main()

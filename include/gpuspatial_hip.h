@@ -228,6 +228,10 @@ struct SjhipJoinResult {
   uint64_t candidates;      /* filter output count */
   uint64_t fallback_tests;  /* FP32 segment tests resolved in FP64 */
   uint64_t segment_tests;   /* segment tests performed */
+  /* device time of the stages (ms, HIP events on the join's stream) when
+   * options.profile is set: [0] probe decode + rects, [1] Morton order of
+   * probes, [2] filter kernel, [3] refine kernels, [4] output sort */
+  double stage_ms[5];
   void (*release)(struct SjhipJoinResult* self);
   void* private_data;
 };
@@ -239,7 +243,8 @@ struct SjhipJoinOptions {
   int32_t precision;   /* 0 = FP32 certified + FP64 fallback, 1 = all-FP64 */
   int32_t sort_output; /* 1 = pairs sorted by (probe, build) */
   int32_t sort_probes; /* 1 = visit probes in Morton order */
-  void* stream;        /* hipStream_t, or NULL for the library stream */
+  int32_t profile;     /* 1 = time the stages with HIP events (stage_ms) */
+  void* stream;        /* hipStream_t, or NULL for a stream of the call */
 };
 
 /* One probe batch of the GPU operator's query_batch
@@ -285,6 +290,15 @@ int SjhipRefinerPushBuildDevice(struct SedonaSpatialRefiner* refiner, const stru
  * 4 * length floats. */
 int SjhipComputeRects(const struct ArrowSchema* schema, const struct ArrowArray* array, int point_nearest,
                       double expand_distance, float* out_rects);
+
+/* Device-memory plumbing for callers without a device allocator of their
+ * own (no counterpart in the reference): allocate / free / copy on the
+ * current device, synchronous. */
+int SjhipDeviceAlloc(uint64_t bytes, void** out_ptr);
+int SjhipDeviceFree(void* ptr);
+int SjhipCopyToDevice(void* dst, const void* src, uint64_t bytes);
+int SjhipCopyToHost(void* dst, const void* src, uint64_t bytes);
+int SjhipSetDevice(int device_id);
 
 /* Error text of the last failed extension call on this thread. */
 const char* SjhipLastError(void);

@@ -312,7 +312,7 @@ void build_rect_tree(const float4* rects, int64_t n, RectTree& t, hipStream_t s)
 }
 
 uint64_t filter_pairs(const RectTree& t, const float4* q, int64_t nq, const uint32_t* qorder, PairBuf& pb,
-                      hipStream_t s) {
+                      hipStream_t s, KernelTimer* timer) {
   if (pb.counter.n < 1) pb.counter.alloc(1, s);
   if (pb.b.n == 0) {
     size_t cap0 = std::max<int64_t>(1 << 16, nq * 4);
@@ -332,9 +332,11 @@ uint64_t filter_pairs(const RectTree& t, const float4* q, int64_t nq, const uint
   }
   for (int attempt = 0; attempt < 2; attempt++) {
     SJ_CHECK(hipMemsetAsync(pb.counter.p, 0, 8, s));
+    if (timer) timer->record_start(s);
     hipLaunchKernelGGL((k_filter<RectTree::FANOUT, RectTree::LEAF>), blocks_for(nq), kBlock, 0, s, v, q, nq, qorder,
                        pb.b.p, pb.p.p, (uint64_t)pb.b.n, pb.counter.p);
     SJ_CHECK(hipGetLastError());
+    if (timer) timer->record_stop(s);
     unsigned long long total = 0;
     SJ_CHECK(hipMemcpyAsync(&total, pb.counter.p, 8, hipMemcpyDeviceToHost, s));
     SJ_CHECK(hipStreamSynchronize(s));

@@ -516,6 +516,28 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
     own.reset(new ScopedStream());
     s = own->s;
   }
+  // stage events: decode+rects, Morton order, filter kernel, refine
+  // kernels, output sort
+  struct Events {
+    hipEvent_t e[10] = {};
+    bool on = false;
+    ~Events() {
+      for (auto& x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } ev;
+  ev.on = o.profile != 0;
+  if (ev.on)
+    for (auto& x : ev.e) SJ_CHECK(hipEventCreate(&x));
+  auto mark = [&](int i) {
+    if (ev.on) SJ_CHECK(hipEventRecord(ev.e[i], s));
+  };
+  KernelTimer ft, rt;
+  if (ev.on) {
+    ft.start = ev.e[4]; ft.stop = ev.e[5];
+    rt.start = ev.e[6]; rt.stop = ev.e[7];
+  }
+  mark(0);
   GeomColumn probe;
   decode_wkb(in, probe, s);
   const bool dwithin = o.predicate == 100;
@@ -524,6 +546,7 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
   // probe rects: GPU operator semantics (points nearest) for relation
   // predicates; outward + distance expansion for DWithin
   compute_rects(probe, nullptr, !dwithin, dwithin ? o.distance : NAN, q, s);
+  mark(1);
   DevArray<uint32_t> order;
   int64_t nq = probe.n_rows;
   const uint32_t* ord = nullptr;
@@ -531,19 +554,35 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
     nq = morton_order(q.p, probe.n_rows, order, s);
     ord = order.p;
   }
+  mark(2);
   PairBuf cand;
-  uint64_t nc = filter_pairs(ix->tree, q.p, nq, ord, cand, s);
+  uint64_t nc = filter_pairs(ix->tree, q.p, nq, ord, cand, s, ev.on ? &ft : nullptr);
   std::unique_ptr<JoinResultImpl> jr(new JoinResultImpl());
   jr->device_id = rf->rt->device_id;
   RefineStats st;
   uint64_t kept;
   if (dwithin)
-    kept = refine_distance(rf->build, probe, o.distance, cand.b.p, cand.p.p, nc, jr->pairs, s);
+    kept = refine_distance(rf->build, probe, o.distance, cand.b.p, cand.p.p, nc, jr->pairs, s, ev.on ? &rt : nullptr);
   else
     kept = refine_relation(rf->build, rf->ring_index, probe, o.predicate, cand.b.p, cand.p.p, nc, o.precision,
-                           jr->pairs, &st, s);
+                           jr->pairs, &st, s, ev.on ? &rt : nullptr);
+  mark(8);
   if (o.sort_output) sort_pairs(jr->pairs.b.p, jr->pairs.p.p, kept, s);
+  mark(9);
   SJ_CHECK(hipStreamSynchronize(s));
+  if (ev.on) {
+    float ms;
+    SJ_CHECK(hipEventElapsedTime(&ms, ev.e[0], ev.e[1])); out->stage_ms[0] = ms;
+    SJ_CHECK(hipEventElapsedTime(&ms, ev.e[1], ev.e[2])); out->stage_ms[1] = ms;
+    if (nc || nq) { SJ_CHECK(hipEventElapsedTime(&ms, ev.e[4], ev.e[5])); out->stage_ms[2] = ms; }
+    if (nc) { SJ_CHECK(hipEventElapsedTime(&ms, ev.e[6], ev.e[7])); out->stage_ms[3] = ms; }
+    SJ_CHECK(hipEventElapsedTime(&ms, ev.e[8], ev.e[9])); out->stage_ms[4] = ms;
+  }
+  // the result outlives the call's stream: all its work is complete, so its
+  // buffers are released on the null stream later
+  jr->pairs.b.s = nullptr;
+  jr->pairs.p.s = nullptr;
+  jr->pairs.counter.s = nullptr;
   out->build_indices = jr->pairs.b.p;
   out->probe_indices = jr->pairs.p.p;
   out->size = kept;
@@ -673,6 +712,36 @@ int SjhipJoinDevice(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefine
     WkbInput in = wkb_input_from_arrow(probe_schema, &probe_array->array, dev);
     JoinEngine::run((IndexImpl*)index->private_data, (RefinerImpl*)refiner->private_data, in, *options, out);
   });
+}
+
+int SjhipDeviceAlloc(uint64_t bytes, void** out_ptr) {
+  return guarded(nullptr, [&] {
+    if (!out_ptr) throw Error("null output pointer");
+    *out_ptr = nullptr;
+    SJ_CHECK(hipMalloc(out_ptr, bytes ? bytes : 1));
+  });
+}
+
+int SjhipDeviceFree(void* ptr) {
+  return guarded(nullptr, [&] {
+    if (ptr) SJ_CHECK(hipFree(ptr));
+  });
+}
+
+int SjhipCopyToDevice(void* dst, const void* src, uint64_t bytes) {
+  return guarded(nullptr, [&] {
+    if (bytes) SJ_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  });
+}
+
+int SjhipCopyToHost(void* dst, const void* src, uint64_t bytes) {
+  return guarded(nullptr, [&] {
+    if (bytes) SJ_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  });
+}
+
+int SjhipSetDevice(int device_id) {
+  return guarded(nullptr, [&] { SJ_CHECK(hipSetDevice(device_id)); });
 }
 
 int SjhipJoinResultCopyToHost(const struct SjhipJoinResult* r, uint32_t* b, uint32_t* p) {

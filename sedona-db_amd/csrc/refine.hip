@@ -644,7 +644,7 @@ void check_relation_types(uint32_t bm, uint32_t pm, int pred) {
 
 uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, int predicate,
                          const uint32_t* cb, const uint32_t* cp, uint64_t n, int precision, PairBuf& out,
-                         RefineStats* stats, hipStream_t s) {
+                         RefineStats* stats, hipStream_t s, KernelTimer* timer) {
   out.counter.reserve(1, s);
   out.b.reserve(n > 0 ? n : 1, s);
   out.p.reserve(n > 0 ? n : 1, s);
@@ -661,6 +661,7 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
   if (use_long) long_list.alloc(n, s);
   SJ_CHECK(hipMemsetAsync(out.counter.p, 0, 8, s));
   SJ_CHECK(hipMemsetAsync(aux.p, 0, 32, s));
+  if (timer) timer->record_start(s);
   hipLaunchKernelGGL(k_refine_lane, grid_for(n), kBlock, 0, s, c, predicate, precision, cb, cp, n, out.b.p, out.p.p,
                      (uint64_t)out.b.n, out.counter.p, use_long ? long_list.p : nullptr, aux.p, aux.p + 1,
                      (int*)(aux.p + 3));
@@ -671,6 +672,7 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
                        out.b.p, out.p.p, (uint64_t)out.b.n, out.counter.p, aux.p + 1);
     SJ_CHECK(hipGetLastError());
   }
+  if (timer) timer->record_stop(s);
   unsigned long long h[4], cnt = 0;
   SJ_CHECK(hipMemcpyAsync(h, aux.p, 32, hipMemcpyDeviceToHost, s));
   SJ_CHECK(hipMemcpyAsync(&cnt, out.counter.p, 8, hipMemcpyDeviceToHost, s));
@@ -684,7 +686,7 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
 }
 
 uint64_t refine_distance(const GeomColumn& build, const GeomColumn& probe, double d, const uint32_t* cb,
-                         const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s) {
+                         const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s, KernelTimer* timer) {
   out.counter.reserve(1, s);
   out.b.reserve(n > 0 ? n : 1, s);
   out.p.reserve(n > 0 ? n : 1, s);
@@ -692,9 +694,11 @@ uint64_t refine_distance(const GeomColumn& build, const GeomColumn& probe, doubl
   if (!probe.points_only && (probe.type_mask & ~(1u << kPoint)))
     throw Error("distance refine: probe rows must be points");
   SJ_CHECK(hipMemsetAsync(out.counter.p, 0, 8, s));
+  if (timer) timer->record_start(s);
   hipLaunchKernelGGL(k_refine_distance, grid_for(n), kBlock, 0, s, build.view(), probe.view(), d, cb, cp, n, out.b.p,
                      out.p.p, (uint64_t)out.b.n, out.counter.p);
   SJ_CHECK(hipGetLastError());
+  if (timer) timer->record_stop(s);
   unsigned long long cnt = 0;
   SJ_CHECK(hipMemcpyAsync(&cnt, out.counter.p, 8, hipMemcpyDeviceToHost, s));
   SJ_CHECK(hipStreamSynchronize(s));

@@ -203,6 +203,13 @@ struct RectTree {
 };
 void build_rect_tree(const float4* rects_dev, int64_t n, RectTree& t, hipStream_t s);
 
+// Optional HIP-event bracket around a kernel launch (on its stream).
+struct KernelTimer {
+  hipEvent_t start = nullptr, stop = nullptr;
+  void record_start(hipStream_t s) { if (start) (void)hipEventRecord(start, s); }
+  void record_stop(hipStream_t s) { if (stop) (void)hipEventRecord(stop, s); }
+};
+
 // Candidate buffers
 struct PairBuf {
   DevArray<uint32_t> b, p;
@@ -212,7 +219,7 @@ struct PairBuf {
 // Runs the filter of queries against the tree into pb; returns the count.
 // qorder (optional) gives the visiting order of the queries.
 uint64_t filter_pairs(const RectTree& t, const float4* q, int64_t nq, const uint32_t* qorder, PairBuf& pb,
-                      hipStream_t s);
+                      hipStream_t s, KernelTimer* timer = nullptr);
 
 // Morton order of query rects (a permutation of valid queries; invalid ones
 // are dropped).  Returns the number of valid queries.
@@ -228,11 +235,11 @@ struct RefineStats {
 // returned).  precision: 0 = FP32 certified + FP64 fallback, 1 = all-FP64.
 uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, int predicate,
                          const uint32_t* cb, const uint32_t* cp, uint64_t n, int precision, PairBuf& out,
-                         RefineStats* stats, hipStream_t s);
+                         RefineStats* stats, hipStream_t s, KernelTimer* timer = nullptr);
 
 // Distance refine (build.distance(probe) <= d); probe must be points.
 uint64_t refine_distance(const GeomColumn& build, const GeomColumn& probe, double d, const uint32_t* cb,
-                         const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s);
+                         const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s, KernelTimer* timer = nullptr);
 
 // Sorts pairs by (probe, build) in place.
 void sort_pairs(uint32_t* b, uint32_t* p, uint64_t n, hipStream_t s);

@@ -154,6 +154,7 @@ SjhipJoinResult._fields_ = [
     ("candidates", ctypes.c_uint64),
     ("fallback_tests", ctypes.c_uint64),
     ("segment_tests", ctypes.c_uint64),
+    ("stage_ms", ctypes.c_double * 5),
     ("release", ctypes.CFUNCTYPE(None, ctypes.POINTER(SjhipJoinResult))),
     ("private_data", ctypes.c_void_p),
 ]
@@ -166,6 +167,7 @@ class SjhipJoinOptions(ctypes.Structure):
         ("precision", ctypes.c_int32),
         ("sort_output", ctypes.c_int32),
         ("sort_probes", ctypes.c_int32),
+        ("profile", ctypes.c_int32),
         ("stream", ctypes.c_void_p),
     ]
 
@@ -175,7 +177,8 @@ EXPORTED = [
     "GpuSpatialRuntimeCreate", "GpuSpatialIndexFloat2DCreate", "GpuSpatialRefinerCreate",
     "SjhipExtVersion", "SjhipRefinerRefineDistance", "SjhipRefinerKnn", "SjhipJoinDevice",
     "SjhipJoinResultCopyToHost", "SjhipIndexBuildFromRefiner", "SjhipRefinerPushBuildDevice", "SjhipLastError",
-    "SjhipComputeRects",
+    "SjhipComputeRects", "SjhipDeviceAlloc", "SjhipDeviceFree", "SjhipCopyToDevice", "SjhipCopyToHost",
+    "SjhipSetDevice",
     "SjhipHostRaycast", "SjhipHostRaycastF32", "SjhipHostCollinear", "SjhipHostSegmentIntersects",
     "SjhipHostOrient2d", "SjhipHostHypot", "SjhipHostCoordToLine", "SjhipHostRaycastBatch", "SjhipHostHypotBatch",
 ]
@@ -207,6 +210,11 @@ def lib():
         L.SjhipRefinerPushBuildDevice.argtypes = [_RF, _SCH, ctypes.POINTER(ArrowDeviceArray)]
         L.SjhipLastError.restype = ctypes.c_char_p
         L.SjhipComputeRects.argtypes = [_SCH, _ARR, ctypes.c_int, ctypes.c_double, ctypes.c_void_p]
+        L.SjhipDeviceAlloc.argtypes = [ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]
+        L.SjhipDeviceFree.argtypes = [ctypes.c_void_p]
+        L.SjhipCopyToDevice.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
+        L.SjhipCopyToHost.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
+        L.SjhipSetDevice.argtypes = [ctypes.c_int]
         d6 = [ctypes.c_double] * 6
         L.SjhipHostRaycast.argtypes = d6
         L.SjhipHostRaycastF32.argtypes = [ctypes.c_float] * 7

@@ -299,3 +299,152 @@ def serving_requests(n_requests, seed=6, lo=1_000, hi=10_000_000, extent=1000.0,
     sizes = np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), size=n_requests))).astype(np.int64)
     preds = [mix[i % len(mix)] for i in range(n_requests)]
     return sizes, preds, rng
+
+
+# ---------------------------------------------------------------------------
+# Parallel generation for the full-size workloads: per-row parameters are
+# drawn in the parent; vertex-level work runs in forked workers (one child
+# RNG per chunk, numpy.random.SeedSequence.spawn) writing straight into
+# shared memory.  Same distributions as the sequential generators above,
+# different random streams.  Call before any GPU runtime is initialised in
+# the process (the workers are forked).
+
+
+def _shm_array(shape, dtype):
+    from multiprocessing import shared_memory
+    nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+    shm = shared_memory.SharedMemory(create=True, size=max(nbytes, 1))
+    arr = np.ndarray(shape, dtype=dtype, buffer=shm.buf)
+    return shm, arr
+
+
+def _run_parallel(fn, tasks, workers):
+    import multiprocessing as mp
+    ctx = mp.get_context("fork")
+    with ctx.Pool(workers) as pool:
+        for _ in pool.imap_unordered(fn, tasks):
+            pass
+
+
+_PAR = {}
+
+
+def _star_chunk(task):
+    lo, hi, seed = task
+    g = _PAR["star"]
+    rng = np.random.default_rng(seed)
+    counts = g["counts"][lo:hi]
+    ang = _sorted_angles(rng, counts)
+    gid = np.repeat(np.arange(lo, hi), counts)
+    r = g["R"][gid] * (0.7 + 0.3 * rng.random(len(ang)))
+    x = g["centers"][gid, 0] + r * np.cos(ang)
+    y = g["centers"][gid, 1] + r * np.sin(ang)
+    cc = counts + 1
+    vstart = np.zeros(len(counts) + 1, np.int64)
+    np.cumsum(counts, out=vstart[1:])
+    idx = np.arange(int(cc.sum())) - np.repeat(np.cumsum(cc) - cc, cc)
+    gsel = np.repeat(np.arange(len(counts)), cc)
+    src = vstart[gsel] + np.where(idx == counts[gsel], 0, idx)
+    col = rings_wkb(cc, np.stack([x[src], y[src]], axis=1), 3)
+    off = g["offsets"]
+    g["data"][off[lo]:off[hi]] = col.data
+
+
+def star_polygons_parallel(seed, counts, centers, radius_mu, radius_sigma=LOGN_SIGMA, workers=16):
+    """Star polygons (as star_polygons) generated by forked workers; returns
+    (WkbColumn over shared memory, keepalive handles)."""
+    counts = np.asarray(counts, dtype=np.int64)
+    n = len(counts)
+    ss = np.random.SeedSequence(seed)
+    prm_rng = np.random.default_rng(ss.spawn(1)[0])
+    R = prm_rng.lognormal(mean=radius_mu, sigma=radius_sigma, size=n)
+    sizes = 13 + 16 * (counts + 1)
+    off = np.zeros(n + 1, np.int64)
+    np.cumsum(sizes, out=off[1:])
+    shm, data = _shm_array((int(off[-1]),), np.uint8)
+    _PAR["star"] = dict(counts=counts, R=R, centers=centers, offsets=off, data=data)
+    per = max(1, int(4e6 // max(1.0, counts.mean())))
+    bounds = list(range(0, n, per)) + [n]
+    seeds = ss.spawn(len(bounds))[1:]
+    tasks = [(bounds[i], bounds[i + 1], seeds[i]) for i in range(len(bounds) - 1)]
+    _run_parallel(_star_chunk, tasks, workers)
+    _PAR.pop("star", None)
+    return WkbColumn(off, data), (shm,)
+
+
+def _points_chunk(task):
+    lo, hi, seed = task
+    g = _PAR["pts"]
+    rng = np.random.default_rng(seed)
+    n = hi - lo
+    nc = int(round(n * g["frac"]))
+    pts = np.empty((n, 2))
+    if g["centers"] is not None:
+        which = rng.integers(0, len(g["centers"]), size=nc)
+        pts[:nc] = g["centers"][which] + rng.normal(0, g["sigma"], size=(nc, 2))
+    else:
+        nc = 0
+    pts[nc:] = rng.uniform(0, g["extent"], size=(n - nc, 2))
+    pts = pts[rng.permutation(n)]
+    g["xy"][lo:hi] = pts
+    rec = np.ndarray((n,), dtype=POINT_REC, buffer=g["data"], offset=21 * lo)
+    rec["bo"] = 1
+    rec["t"] = 1
+    rec["x"] = pts[:, 0]
+    rec["y"] = pts[:, 1]
+
+
+def points_parallel(seed, n, extent, centers=None, sigma=2.0, frac_clustered=0.8, workers=16):
+    """Points (clustered like clustered_xy when centers is given, uniform
+    otherwise) by forked workers: (WkbColumn, xy array, keepalive)."""
+    shm_d, data = _shm_array((21 * n,), np.uint8)
+    shm_x, xy = _shm_array((n, 2), np.float64)
+    _PAR["pts"] = dict(centers=centers, sigma=sigma, frac=frac_clustered if centers is not None else 0.0,
+                       extent=extent, xy=xy, data=data)
+    ss = np.random.SeedSequence(seed)
+    per = 2_000_000
+    bounds = list(range(0, n, per)) + [n]
+    seeds = ss.spawn(len(bounds))
+    tasks = [(bounds[i], bounds[i + 1], seeds[i]) for i in range(len(bounds) - 1)]
+    _run_parallel(_points_chunk, tasks, workers)
+    _PAR.pop("pts", None)
+    return WkbColumn(np.arange(n + 1, dtype=np.int64) * 21, data), xy, (shm_d, shm_x)
+
+
+def workload_pip_zipf_parallel(n_points=100_000_000, n_polys=1_000_000, seed=2, workers=16, max_vertices=4096):
+    """The 100M clustered points x 1M Zipf(1.2) stars workload, generated in
+    parallel.  Returns dict(build, probe, centers, probe_xy, keep)."""
+    extent = 1000.0 * np.sqrt(n_polys / 1_000_000)
+    rng = np.random.default_rng(seed)
+    centers = rng.uniform(0, extent, size=(n_polys, 2))
+    counts = zipf_counts(rng, n_polys, hi=max_vertices)
+    n_clusters = max(1, int(round(64 * n_polys / 1_000_000))) if n_polys < 1_000_000 else 64
+    ccent = rng.uniform(0, extent, size=(n_clusters, 2))
+    build, k1 = star_polygons_parallel(seed + 1000, counts, centers, 0.0, workers=workers)
+    probe, xy, k2 = points_parallel(seed + 2000, n_points, extent, centers=ccent, workers=workers)
+    return dict(build=build, probe=probe, centers=centers, counts=counts, probe_xy=xy, extent=extent,
+                keep=k1 + k2)
+
+
+def release_shared(keep):
+    for shm in keep:
+        try:
+            shm.close()
+            shm.unlink()
+        except Exception:
+            pass
+
+
+def take_rows(col, rows):
+    """WkbColumn of the given rows (in order)."""
+    rows = np.asarray(rows, dtype=np.int64)
+    lens = col.offsets[rows + 1] - col.offsets[rows]
+    off = np.zeros(len(rows) + 1, np.int64)
+    np.cumsum(lens, out=off[1:])
+    if len(rows) and np.all(lens == lens[0]):
+        w = int(lens[0])
+        base = col.offsets[rows][:, None] + np.arange(w)[None, :]
+        return WkbColumn(off, col.data[base.reshape(-1)])
+    data = np.concatenate([col.data[col.offsets[r]:col.offsets[r + 1]] for r in rows]) if len(rows) else \
+        np.zeros(0, np.uint8)
+    return WkbColumn(off, data)

@@ -160,11 +160,54 @@ class DeviceWkb:
         self.dev.device_type = L.ARROW_DEVICE_ROCM
 
     @classmethod
-    def from_torch(cls, col, device="cuda"):
-        import torch
-        off = torch.from_numpy(col.offsets).to(device)
-        data = torch.from_numpy(col.data).to(device)
-        return cls(off.data_ptr(), data.data_ptr(), len(col), keepalive=(off, data))
+    def from_host(cls, col, device_id=0):
+        """Copies a datagen.WkbColumn (int64 offsets + data) into device
+        buffers owned by the returned object."""
+        off = DeviceBuffer.upload(np.ascontiguousarray(col.offsets, dtype=np.int64), device_id)
+        data = DeviceBuffer.upload(np.ascontiguousarray(col.data, dtype=np.uint8), device_id)
+        return cls(off.ptr, data.ptr, len(col), device_id=device_id, keepalive=(off, data))
+
+
+class DeviceBuffer:
+    """Device memory owned through the library's plumbing entry points."""
+
+    def __init__(self, nbytes, device_id=0):
+        lib = L.lib()
+        rc = lib.SjhipSetDevice(device_id)
+        if rc:
+            raise GpuSpatialError(lib.SjhipLastError().decode())
+        p = ctypes.c_void_p()
+        rc = lib.SjhipDeviceAlloc(int(nbytes), ctypes.byref(p))
+        if rc:
+            raise GpuSpatialError(lib.SjhipLastError().decode())
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    @classmethod
+    def upload(cls, arr, device_id=0):
+        b = cls(max(arr.nbytes, 1), device_id)
+        rc = L.lib().SjhipCopyToDevice(b.ptr, arr.ctypes.data, arr.nbytes)
+        if rc:
+            raise GpuSpatialError(L.lib().SjhipLastError().decode())
+        return b
+
+    def download(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        rc = L.lib().SjhipCopyToHost(out.ctypes.data, self.ptr, out.nbytes)
+        if rc:
+            raise GpuSpatialError(L.lib().SjhipLastError().decode())
+        return out
+
+    def free(self):
+        if self.ptr:
+            L.lib().SjhipDeviceFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def host_device_array(exported):
@@ -404,7 +447,7 @@ class JoinResult:
 
 
 def join_device(index, refiner, probe, predicate, distance=0.0, precision=0, sort_output=True, sort_probes=True,
-                stream=None):
+                profile=False, stream=None):
     """One probe batch through the fused device join.  `probe` is a
     DeviceWkb (device memory) or any Arrow-exportable host column."""
     if isinstance(predicate, str):
@@ -412,7 +455,7 @@ def join_device(index, refiner, probe, predicate, distance=0.0, precision=0, sor
     else:
         pred = int(predicate)
     opts = L.SjhipJoinOptions(pred, float(distance), int(precision), 1 if sort_output else 0, 1 if sort_probes else 0,
-                              stream)
+                              1 if profile else 0, stream)
     res = L.SjhipJoinResult()
     if isinstance(probe, DeviceWkb):
         sch, dev = probe.schema, probe.dev

@@ -169,7 +169,7 @@ def test_device_join_zipf_stars_vs_oracle(pred, precision, opts):
     b, p = D.workload_pip_zipf(100_000, 1_000, seed=11)
     say("zipf", pred, precision, "vertices", int(b.offsets[-1]))
     ix, rf = build_pair(b.to_arrow(), opts)
-    dev = G.DeviceWkb.from_torch(p)
+    dev = G.DeviceWkb.from_host(p)
     r = G.join_device(ix, rf, dev, pred, precision=precision)
     gb, gp = r.to_host()
     ob, op = O.join(O.Store(b.as_tuple()), O.Store(p.as_tuple()), pred, nthreads=8)
