@@ -740,6 +740,11 @@ static int grow(const double* e, int n, double b, double* h) {
 
 /* Sign of (ax-cx)(by-cy) - (ay-cy)(bx-cx), exactly. */
 int ora_orient2d_sign(double ax, double ay, double bx, double by, double cx, double cy) {
+  /* Shewchuk's first-stage filter (errboundA): the plain evaluation's sign
+   * is exact whenever |det| exceeds this bound. */
+  double detl = (ax - cx) * (by - cy), detr = (ay - cy) * (bx - cx);
+  double det = detl - detr, detsum = fabs(detl) + fabs(detr);
+  if (fabs(det) > 3.3306690738754716e-16 * detsum) return det > 0 ? 1 : -1;
   double acx, acx_e, bcy, bcy_e, acy, acy_e, bcx, bcx_e;
   two_sum(ax, -cx, &acx, &acx_e);
   two_sum(by, -cy, &bcy, &bcy_e);

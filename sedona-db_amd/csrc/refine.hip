@@ -121,6 +121,92 @@ struct Ctx {
 
 // tg_ring_contains_point for a build-side ring (precomputed rect and
 // optional FP32 local coordinates).
+// ---- segment-tree walk ----------------------------------------------------
+//
+// A segment can classify the +x ray as IN or ON only when the point's
+// height lies within the segment's y-range and the segment is not entirely
+// left of the point (tg.c:1203-1213: a segment whose max x is below p.x is
+// OUT, degenerate segments included).  The reference's own natural index
+// prunes the same way (branch_maybe_in, tg.c:2762-2772).  Boxes are FP32
+// ring-local and the point's local coordinates are within E of exact, so a
+// margin of 2E keeps the pruning conservative.
+
+struct TreeWalk {
+  const float4* box;     // ring's boxes, level 0 first
+  int64_t cnt[kSegMaxLevels];
+  int64_t off[kSegMaxLevels];
+  int levels;
+  float pxl, pyl, m;
+  // segment scan
+  RingRef r;
+  const float2* q32;
+  float E;
+  double px, py;
+  int64_t ns;
+  int precision;
+  int on, in;
+  uint64_t tests, fallbacks;
+};
+
+__device__ __forceinline__ bool box_pass(const TreeWalk& w, const float4& b) {
+  return w.pyl >= b.y - w.m && w.pyl <= b.w + w.m && w.pxl <= b.z + w.m;
+}
+
+__device__ __forceinline__ void scan_leaf(TreeWalk& w, int64_t leaf) {
+  int64_t s0 = leaf * kSegLeaf, s1 = min(s0 + kSegLeaf, w.ns);
+  if (w.precision == 0) {
+    for (int64_t j = s0; j < s1; j++) {
+      float2 a = w.q32[j], b = w.q32[j + 1];
+      int rc = raycast_f32_certified(a.x, a.y, b.x, b.y, w.pxl, w.pyl, w.E);
+      if (rc == kRayOut) continue;
+      w.tests++;
+      if (rc < 0) {
+        w.fallbacks++;
+        double2 A = w.r.xy[j], B = w.r.xy[j + 1];
+        rc = raycast(A.x, A.y, B.x, B.y, w.px, w.py);
+      }
+      acc_ray(rc, w.on, w.in);
+    }
+  } else {
+    ring_segments_f64(w.r, s0, s1, 1, w.px, w.py, w.on, w.in, w.tests);
+  }
+}
+
+template <int LVL>
+__device__ __forceinline__ void visit(TreeWalk& w, int64_t node) {
+  if constexpr (LVL == 0) {
+    scan_leaf(w, node);
+  } else {
+    int64_t c0 = node * kSegFanout, c1 = min(c0 + kSegFanout, w.cnt[LVL - 1]);
+    const float4* b = w.box + w.off[LVL - 1];
+    for (int64_t c = c0; c < c1; c++)
+      if (box_pass(w, b[c])) visit<LVL - 1>(w, c);
+  }
+}
+
+template <int L>
+__device__ __forceinline__ void walk_root(TreeWalk& w) {
+  const float4* b = w.box + w.off[L - 1];
+  for (int64_t c = 0; c < w.cnt[L - 1]; c++)
+    if (box_pass(w, b[c])) visit<L - 1>(w, c);
+}
+
+// Level sizes of a ring with ns segments; returns the number of levels.
+__host__ __device__ __forceinline__ int seg_tree_shape(int64_t ns, int64_t* cnt, int64_t* off) {
+  int64_t c = (ns + kSegLeaf - 1) / kSegLeaf;
+  int L = 0;
+  int64_t o = 0;
+  while (true) {
+    cnt[L] = c;
+    off[L] = o;
+    o += c;
+    L++;
+    if (c <= kSegFanout || L == kSegMaxLevels) break;
+    c = (c + kSegFanout - 1) / kSegFanout;
+  }
+  return L;
+}
+
 __device__ bool build_ring_contains(const Ctx& c, int64_t ring, double px, double py, bool allow_on_edge,
                                     int precision, uint64_t& tests, uint64_t& fallbacks) {
   double4 rb = c.R.ring_rect[ring];
@@ -128,10 +214,40 @@ __device__ bool build_ring_contains(const Ctx& c, int64_t ring, double px, doubl
   RingRef r = ring_at(c.B, ring);
   int64_t ns = r.npts >= 3 ? r.npts - 1 : 0;
   int on = 0, in = 0;
+  double2 o = c.R.origin[ring];
+  float E = c.R.err[ring];
+  float pxl = (float)(px - o.x), pyl = (float)(py - o.y);
+  int64_t toff = c.R.seg_tree_off ? c.R.seg_tree_off[ring] : -1;
+  if (toff >= 0) {
+    TreeWalk w;
+    w.box = c.R.seg_tree + toff;
+    w.levels = seg_tree_shape(ns, w.cnt, w.off);
+    w.pxl = pxl;
+    w.pyl = pyl;
+    w.m = 2.0f * E;
+    w.r = r;
+    w.q32 = c.R.xy32 + r.v0;
+    w.E = E;
+    w.px = px;
+    w.py = py;
+    w.ns = ns;
+    w.precision = precision;
+    w.on = 0;
+    w.in = 0;
+    w.tests = 0;
+    w.fallbacks = 0;
+    switch (w.levels) {
+      case 1: walk_root<1>(w); break;
+      case 2: walk_root<2>(w); break;
+      case 3: walk_root<3>(w); break;
+      case 4: walk_root<4>(w); break;
+      default: walk_root<5>(w); break;
+    }
+    tests += w.tests;
+    fallbacks += w.fallbacks;
+    return w.on ? allow_on_edge : (w.in != 0);
+  }
   if (precision == 0) {
-    double2 o = c.R.origin[ring];
-    float E = c.R.err[ring];
-    float pxl = (float)(px - o.x), pyl = (float)(py - o.y);
     ring_segments_f32(r, c.R.xy32, E, pxl, pyl, px, py, 0, ns, 1, on, in, tests, fallbacks);
   } else {
     ring_segments_f64(r, 0, ns, 1, px, py, on, in, tests);
@@ -343,6 +459,7 @@ __device__ __forceinline__ bool is_long(const Ctx& c, int64_t b, int64_t p) {
   int64_t part = c.B.part_off[b];
   if (c.B.ring_off[part + 1] - c.B.ring_off[part] != 1) return false;
   int64_t ring = c.B.ring_off[part];
+  if (c.R.seg_tree_off && c.R.seg_tree_off[ring] >= 0) return false;  // the tree walk handles it
   return c.B.pt_off[ring + 1] - c.B.pt_off[ring] > kLongRing;
 }
 
@@ -596,6 +713,75 @@ __global__ void k_local32(GeomView g, int64_t n_rings, const double2* origin, fl
   }
 }
 
+// Boxes per ring (0 for rings scanned whole).
+__global__ void k_seg_counts(GeomView g, int64_t n_rings, int min_segs, int64_t* cnt) {
+  int64_t ring = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ring >= n_rings) return;
+  int64_t npts = g.pt_off[ring + 1] - g.pt_off[ring];
+  int64_t ns = npts >= 3 ? npts - 1 : 0;
+  int64_t total = 0;
+  if (ns >= min_segs) {
+    int64_t c[kSegMaxLevels], o[kSegMaxLevels];
+    int L = seg_tree_shape(ns, c, o);
+    total = o[L - 1] + c[L - 1];
+  }
+  cnt[ring] = total;
+}
+
+__global__ void k_seg_offsets(const int64_t* cnt, const int64_t* scan, int64_t n_rings, int64_t* off) {
+  int64_t ring = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ring >= n_rings) return;
+  off[ring] = cnt[ring] ? scan[ring] : -1;
+}
+
+// Level 0: one wavefront per ring, lanes over leaves.
+__global__ void k_seg_leaves(GeomView g, int64_t n_rings, const float2* xy32, const int64_t* off, float4* tree) {
+  int64_t ring = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  int lane = threadIdx.x & 63;
+  if (ring >= n_rings) return;
+  int64_t o = off[ring];
+  if (o < 0) return;
+  int64_t v0 = g.pt_off[ring];
+  int64_t ns = g.pt_off[ring + 1] - v0 - 1;
+  int64_t nleaf = (ns + kSegLeaf - 1) / kSegLeaf;
+  for (int64_t k = lane; k < nleaf; k += 64) {
+    int64_t a = k * kSegLeaf, e = min(a + kSegLeaf, ns);  // vertices a..e
+    float2 p = xy32[v0 + a];
+    float4 b = make_float4(p.x, p.y, p.x, p.y);
+    for (int64_t v = a + 1; v <= e; v++) {
+      p = xy32[v0 + v];
+      b.x = fminf(b.x, p.x); b.y = fminf(b.y, p.y);
+      b.z = fmaxf(b.z, p.x); b.w = fmaxf(b.w, p.y);
+    }
+    tree[o + k] = b;
+  }
+}
+
+// Upper levels: one thread per ring.
+__global__ void k_seg_upper(GeomView g, int64_t n_rings, const int64_t* off, float4* tree) {
+  int64_t ring = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ring >= n_rings) return;
+  int64_t o = off[ring];
+  if (o < 0) return;
+  int64_t ns = g.pt_off[ring + 1] - g.pt_off[ring] - 1;
+  int64_t c[kSegMaxLevels], lo[kSegMaxLevels];
+  int L = seg_tree_shape(ns, c, lo);
+  for (int l = 1; l < L; l++) {
+    const float4* ch = tree + o + lo[l - 1];
+    float4* pa = tree + o + lo[l];
+    for (int64_t i = 0; i < c[l]; i++) {
+      int64_t c0 = i * kSegFanout, c1 = min(c0 + kSegFanout, c[l - 1]);
+      float4 u = ch[c0];
+      for (int64_t k = c0 + 1; k < c1; k++) {
+        float4 b = ch[k];
+        u.x = fminf(u.x, b.x); u.y = fminf(u.y, b.y);
+        u.z = fmaxf(u.z, b.z); u.w = fmaxf(u.w, b.w);
+      }
+      pa[i] = u;
+    }
+  }
+}
+
 unsigned int blocks_for(uint64_t n) { return (unsigned int)((n + kBlock - 1) / kBlock); }
 
 unsigned int grid_for(uint64_t n) {
@@ -606,7 +792,6 @@ unsigned int grid_for(uint64_t n) {
 }  // namespace
 
 void build_ring_index(const GeomColumn& g, RingIndex& ri, const RingIndexParams& prm, hipStream_t s) {
-  (void)prm;
   ri = RingIndex();
   if (g.points_only || g.n_rings == 0) return;
   ri.ring_rect.alloc(g.n_rings, s);
@@ -617,7 +802,32 @@ void build_ring_index(const GeomColumn& g, RingIndex& ri, const RingIndexParams&
   hipLaunchKernelGGL(k_ring_rects, blocks_for(threads), kBlock, 0, s, g.view(), g.n_rings, ri.ring_rect.p,
                      ri.origin.p, ri.err.p);
   hipLaunchKernelGGL(k_local32, blocks_for(threads), kBlock, 0, s, g.view(), g.n_rings, ri.origin.p, ri.xy32.p);
+  // segment trees
+  DevArray<int64_t> cnt(g.n_rings, s);
+  hipLaunchKernelGGL(k_seg_counts, blocks_for(g.n_rings), kBlock, 0, s, g.view(), g.n_rings, prm.min_segs_for_tree,
+                     cnt.p);
+  DevArray<int64_t> scan(g.n_rings + 1, s);
+  size_t tmp = 0;
+  SJ_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, cnt.p, scan.p, (int)g.n_rings, s));
+  DevArray<uint8_t> t(tmp + 16, s);
+  SJ_CHECK(hipcub::DeviceScan::ExclusiveSum(t.p, tmp, cnt.p, scan.p, (int)g.n_rings, s));
+  int64_t last[2];
+  SJ_CHECK(hipMemcpyAsync(&last[0], scan.p + g.n_rings - 1, 8, hipMemcpyDeviceToHost, s));
+  SJ_CHECK(hipMemcpyAsync(&last[1], cnt.p + g.n_rings - 1, 8, hipMemcpyDeviceToHost, s));
+  SJ_CHECK(hipStreamSynchronize(s));
+  ri.n_boxes = last[0] + last[1];
+  ri.seg_tree_off.alloc(g.n_rings, s);
+  hipLaunchKernelGGL(k_seg_offsets, blocks_for(g.n_rings), kBlock, 0, s, cnt.p, scan.p, g.n_rings,
+                     ri.seg_tree_off.p);
+  ri.seg_tree.alloc(ri.n_boxes > 0 ? ri.n_boxes : 1, s);
+  if (ri.n_boxes > 0) {
+    hipLaunchKernelGGL(k_seg_leaves, blocks_for(threads), kBlock, 0, s, g.view(), g.n_rings, ri.xy32.p,
+                       ri.seg_tree_off.p, ri.seg_tree.p);
+    hipLaunchKernelGGL(k_seg_upper, blocks_for(g.n_rings), kBlock, 0, s, g.view(), g.n_rings, ri.seg_tree_off.p,
+                       ri.seg_tree.p);
+  }
   SJ_CHECK(hipGetLastError());
+  SJ_CHECK(hipStreamSynchronize(s));
 }
 
 void check_relation_types(uint32_t bm, uint32_t pm, int pred) {

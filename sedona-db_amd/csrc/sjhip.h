@@ -6,10 +6,11 @@
 //                      A column holding only points keeps xy[row] directly
 //                      (NaN for null / empty rows) and no offsets.
 //   ring refine index: ring_rect[ring] (double4, the reference's ring->rect),
-//                      y-stripes per ring (ring_stripe[ring] -> stripe_off ->
-//                      stripe_seg), FP32 ring-local vertices xy32[pt] (float2)
-//                      with a per-ring origin and error bound for the
-//                      certified FP32 fast path.
+//                      FP32 ring-local vertices xy32[pt] (float2) with a
+//                      per-ring origin and error bound for the certified
+//                      FP32 fast path, and per ring of 32+ segments a
+//                      segment tree in natural ring order (boxes over 16
+//                      consecutive segments, fan-out 8).
 //   rect tree        : Morton-sorted FP32 leaf rects (float4) in groups of
 //                      LEAF, internal levels of fan-out FANOUT, all levels
 //                      in one float4 array; sorted build ids (u32).
@@ -92,17 +93,21 @@ struct GeomView {
   const double2* xy = nullptr;
 };
 
+// Segment tree of one ring: level 0 holds one FP32 ring-local box per
+// kSegLeaf consecutive segments, level l+1 one box per kSegFanout nodes of
+// level l, up to a top level of at most kSegFanout nodes.
+constexpr int kSegLeaf = 16;
+constexpr int kSegFanout = 8;
+constexpr int kSegMaxLevels = 5;
+
 // Device view of the ring refine index (build side).
 struct RingIndexView {
-  const double4* ring_rect = nullptr;    // xmin, ymin, xmax, ymax (double)
-  const int64_t* stripe_base = nullptr;  // per ring: first entry in stripe_off, -1 = no stripes
-  const int32_t* nstripes = nullptr;     // per ring
-  const double2* stripe_map = nullptr;   // per ring: (y0, scale) with stripe = (y - y0) * scale
-  const int64_t* stripe_off = nullptr;   // stripe -> first entry in stripe_seg
-  const uint32_t* stripe_seg = nullptr;  // local segment index within the ring
-  const float2* xy32 = nullptr;          // per point: ring-local FP32 vertex
-  const double2* origin = nullptr;       // per ring: local origin (double)
-  const float* err = nullptr;            // per ring: FP32 certification bound E
+  const double4* ring_rect = nullptr;   // xmin, ymin, xmax, ymax (double)
+  const float2* xy32 = nullptr;         // per point: ring-local FP32 vertex
+  const double2* origin = nullptr;      // per ring: local origin (double)
+  const float* err = nullptr;           // per ring: FP32 certification bound E
+  const int64_t* seg_tree_off = nullptr;  // per ring: first box in seg_tree, -1 = not indexed
+  const float4* seg_tree = nullptr;       // boxes (xmin, ymin, xmax, ymax), ring-local FP32
 };
 
 // Owned geometry column on the device.
@@ -128,25 +133,20 @@ struct GeomColumn {
 
 struct RingIndex {
   DevArray<double4> ring_rect;
-  DevArray<int64_t> stripe_base;
-  DevArray<int32_t> nstripes;
-  DevArray<double2> stripe_map;
-  DevArray<int64_t> stripe_off;
-  DevArray<uint32_t> stripe_seg;
   DevArray<float2> xy32;
   DevArray<double2> origin;
   DevArray<float> err;
+  DevArray<int64_t> seg_tree_off;
+  DevArray<float4> seg_tree;
+  int64_t n_boxes = 0;
   RingIndexView view() const {
     RingIndexView v;
     v.ring_rect = ring_rect.p;
-    v.stripe_base = stripe_base.p;
-    v.nstripes = nstripes.p;
-    v.stripe_map = stripe_map.p;
-    v.stripe_off = stripe_off.p;
-    v.stripe_seg = stripe_seg.p;
     v.xy32 = xy32.p;
     v.origin = origin.p;
     v.err = err.p;
+    v.seg_tree_off = seg_tree_off.p;
+    v.seg_tree = seg_tree.p;
     return v;
   }
 };
@@ -181,9 +181,7 @@ void compute_rects(const GeomColumn& g, const RingIndex* ri, bool point_nearest,
 
 // Ring refine index for a build column.
 struct RingIndexParams {
-  int segs_per_stripe = 4;     // target segments per y-stripe
-  int min_segs_for_stripes = 16;
-  int max_stripes = 4096;
+  int min_segs_for_tree = 32;  // rings with fewer segments are scanned whole
 };
 void build_ring_index(const GeomColumn& g, RingIndex& ri, const RingIndexParams& p, hipStream_t s);
 

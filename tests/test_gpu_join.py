@@ -180,7 +180,9 @@ def test_device_join_zipf_stars_vs_oracle(pred, precision, opts):
 
 
 def test_dwithin_random_walks_vs_oracle(opts):
-    b, p = D.workload_dwithin_walks(200_000, 20_000, seed=13)
+    # ~160 candidate boxes per point at this line density (as at full size)
+    b, p = D.workload_dwithin_walks(10_000, 2_000, seed=13)
+    say("dwithin walks", len(b), len(p))
     ix, rf = build_pair(b.to_arrow(), opts)
     r = G.join_device(ix, rf, p.to_arrow(), "dwithin", distance=0.05)
     gb, gp = r.to_host()
