@@ -51,6 +51,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the all-FP64 comparison step")
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--refine-order", type=int, default=1, help="1 = group candidates by polygon before refine")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -98,13 +99,14 @@ def main():
 
     def step(precision, profile=True):
         return G.join_device(ix, rf, pdev, "contains", precision=precision, sort_output=False, sort_probes=True,
+                             refine_order=args.refine_order,
                              profile=profile)
 
     for i in range(args.warmup):
         r = step(args.precision)
         log(f"warmup {i}: {r.size} pairs, {r.candidates} candidates, stages {list(r.res.stage_ms)}")
         r.release()
-    stage = np.zeros(5)
+    stage = np.zeros(6)
     sizes = []
     t = time.time()
     for i in range(args.steps):
@@ -183,7 +185,8 @@ def main():
         "pairs_per_step": int(pairs),
         "candidates_per_step": int(cands),
         "probe_points_per_s": n_points / (ms_per_step / 1000),
-        "stage_ms": {"decode_rects": stage[0], "morton": stage[1], "filter": stage[2], "refine": stage[3]},
+        "stage_ms": {"decode_rects": stage[0], "morton": stage[1], "filter": stage[2],
+                     "group_by_polygon": stage[5], "refine": stage[3]},
         "segment_tests": int(seg_tests),
         "fallback_rate": fb / max(1, seg_tests),
         "build_ms": build_ms,

@@ -230,8 +230,9 @@ struct SjhipJoinResult {
   uint64_t segment_tests;   /* segment tests performed */
   /* device time of the stages (ms, HIP events on the join's stream) when
    * options.profile is set: [0] probe decode + rects, [1] Morton order of
-   * probes, [2] filter kernel, [3] refine kernels, [4] output sort */
-  double stage_ms[5];
+   * probes, [2] filter kernel, [3] refine kernels, [4] output sort,
+   * [5] candidate regrouping (refine_order 1) */
+  double stage_ms[6];
   void (*release)(struct SjhipJoinResult* self);
   void* private_data;
 };
@@ -244,6 +245,8 @@ struct SjhipJoinOptions {
   int32_t sort_output; /* 1 = pairs sorted by (probe, build) */
   int32_t sort_probes; /* 1 = visit probes in Morton order */
   int32_t profile;     /* 1 = time the stages with HIP events (stage_ms) */
+  int32_t refine_order; /* 0 = refine candidates in filter order, 1 = group
+                           them by build row first (polygon-major) */
   void* stream;        /* hipStream_t, or NULL for a stream of the call */
 };
 

@@ -314,10 +314,13 @@ void build_rect_tree(const float4* rects, int64_t n, RectTree& t, hipStream_t s)
 uint64_t filter_pairs(const RectTree& t, const float4* q, int64_t nq, const uint32_t* qorder, PairBuf& pb,
                       hipStream_t s, KernelTimer* timer) {
   if (pb.counter.n < 1) pb.counter.alloc(1, s);
-  if (pb.b.n == 0) {
-    size_t cap0 = std::max<int64_t>(1 << 16, nq * 4);
-    pb.b.alloc(cap0, s);
-    pb.p.alloc(cap0, s);
+  // first capacity: the tree's observed candidates per query (16ths) from
+  // earlier probes, else 4 per query; one resize-and-rerun on overflow
+  uint64_t per16 = t.cand_per_query_x16.load(std::memory_order_relaxed);
+  size_t want = std::max<int64_t>(1 << 16, per16 ? (int64_t)((per16 * (uint64_t)nq) / 16 * 9 / 8 + 1024) : nq * 4);
+  if (pb.b.n < want) {
+    pb.b.alloc(want, s);
+    pb.p.alloc(want, s);
   }
   if (nq == 0 || t.n_levels == 0) return 0;
   TreeView v{};
@@ -340,6 +343,7 @@ uint64_t filter_pairs(const RectTree& t, const float4* q, int64_t nq, const uint
     unsigned long long total = 0;
     SJ_CHECK(hipMemcpyAsync(&total, pb.counter.p, 8, hipMemcpyDeviceToHost, s));
     SJ_CHECK(hipStreamSynchronize(s));
+    if (nq > 0) t.cand_per_query_x16.store((uint64_t)((total * 16) / (uint64_t)nq + 1), std::memory_order_relaxed);
     if (total <= pb.b.n) return total;
     size_t cap = (size_t)(total + total / 8 + 1024);
     pb.b.alloc(cap, s);

@@ -557,6 +557,9 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
   mark(2);
   PairBuf cand;
   uint64_t nc = filter_pairs(ix->tree, q.p, nq, ord, cand, s, ev.on ? &ft : nullptr);
+  mark(3);
+  // polygon-major order: one wavefront then walks few distinct rings
+  if (o.refine_order == 1) sort_pairs_by_build(cand, nc, rf->build.n_rows, s);
   std::unique_ptr<JoinResultImpl> jr(new JoinResultImpl());
   jr->device_id = rf->rt->device_id;
   RefineStats st;
@@ -577,6 +580,7 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
     if (nc || nq) { SJ_CHECK(hipEventElapsedTime(&ms, ev.e[4], ev.e[5])); out->stage_ms[2] = ms; }
     if (nc) { SJ_CHECK(hipEventElapsedTime(&ms, ev.e[6], ev.e[7])); out->stage_ms[3] = ms; }
     SJ_CHECK(hipEventElapsedTime(&ms, ev.e[8], ev.e[9])); out->stage_ms[4] = ms;
+    if (nc) { SJ_CHECK(hipEventElapsedTime(&ms, ev.e[3], ev.e[6])); out->stage_ms[5] = ms; }
   }
   // the result outlives the call's stream: all its work is complete, so its
   // buffers are released on the null stream later

@@ -83,6 +83,13 @@ __device__ __forceinline__ void acc_ray(int rc, int& on, int& in) {
   in ^= rc == kRayIn;
 }
 
+// FP64 re-evaluation of one segment, kept out of line so that the FP32 scan
+// loops do not carry the FP64 path's registers.
+__device__ __noinline__ int raycast_fallback(const double2* xy, int j, double px, double py) {
+  double2 A = xy[j], B = xy[j + 1];
+  return raycast(A.x, A.y, B.x, B.y, px, py);
+}
+
 // FP64 plain ring test over segments [s0, s1) with stride.
 __device__ __forceinline__ void ring_segments_f64(const RingRef& r, int64_t s0, int64_t s1, int64_t stride,
                                                   double px, double py, int& on, int& in, uint64_t& tests) {
@@ -106,8 +113,7 @@ __device__ __forceinline__ void ring_segments_f32(const RingRef& r, const float2
     tests++;
     if (rc < 0) {
       fallbacks++;
-      double2 A = r.xy[j], B = r.xy[j + 1];
-      rc = raycast(A.x, A.y, B.x, B.y, px, py);
+      rc = raycast_fallback(r.xy, (int)j, px, py);
     }
     acc_ray(rc, on, in);
   }
@@ -131,80 +137,112 @@ struct Ctx {
 // ring-local and the point's local coordinates are within E of exact, so a
 // margin of 2E keeps the pruning conservative.
 
-struct TreeWalk {
-  const float4* box;     // ring's boxes, level 0 first
-  int64_t cnt[kSegMaxLevels];
-  int64_t off[kSegMaxLevels];
+// Level sizes of a ring's tree (levels <= kSegMaxLevels), as scalars so that
+// nothing is indexed dynamically (dynamic indexing would place them in
+// scratch memory).
+struct TreeShape {
+  int c0, c1, c2, c3, c4;  // node counts per level
   int levels;
+  template <int L>
+  __host__ __device__ __forceinline__ int cnt() const {
+    return L == 0 ? c0 : L == 1 ? c1 : L == 2 ? c2 : L == 3 ? c3 : c4;
+  }
+  template <int L>
+  __host__ __device__ __forceinline__ int off() const {
+    return L == 0 ? 0 : L == 1 ? c0 : L == 2 ? c0 + c1 : L == 3 ? c0 + c1 + c2 : c0 + c1 + c2 + c3;
+  }
+  __host__ __device__ __forceinline__ int total() const { return c0 + c1 + c2 + c3 + c4; }
+};
+
+__host__ __device__ __forceinline__ int up_level(int c) { return (c + kSegFanout - 1) / kSegFanout; }
+
+__host__ __device__ __forceinline__ TreeShape seg_tree_shape(int ns) {
+  TreeShape t;
+  t.c0 = (ns + kSegLeaf - 1) / kSegLeaf;
+  t.c1 = t.c2 = t.c3 = t.c4 = 0;
+  t.levels = 1;
+  if (t.c0 > kSegFanout) {
+    t.c1 = up_level(t.c0);
+    t.levels = 2;
+    if (t.c1 > kSegFanout) {
+      t.c2 = up_level(t.c1);
+      t.levels = 3;
+      if (t.c2 > kSegFanout) {
+        t.c3 = up_level(t.c2);
+        t.levels = 4;
+        if (t.c3 > kSegFanout) {
+          t.c4 = up_level(t.c3);
+          t.levels = 5;
+        }
+      }
+    }
+  }
+  return t;
+}
+
+struct TreeWalk {
+  const float4* box;  // ring's boxes, level 0 first
+  TreeShape shape;
   float pxl, pyl, m;
-  // segment scan
-  RingRef r;
-  const float2* q32;
+  const double2* xy;  // ring's FP64 points
+  const float2* q32;  // ring's FP32 local points
   float E;
   double px, py;
-  int64_t ns;
+  int ns;
   int precision;
   int on, in;
-  uint64_t tests, fallbacks;
+  uint32_t tests, fallbacks;
 };
 
 __device__ __forceinline__ bool box_pass(const TreeWalk& w, const float4& b) {
   return w.pyl >= b.y - w.m && w.pyl <= b.w + w.m && w.pxl <= b.z + w.m;
 }
 
-__device__ __forceinline__ void scan_leaf(TreeWalk& w, int64_t leaf) {
-  int64_t s0 = leaf * kSegLeaf, s1 = min(s0 + kSegLeaf, w.ns);
+__device__ __forceinline__ void scan_leaf(TreeWalk& w, int leaf) {
+  int s0 = leaf * kSegLeaf, s1 = min(s0 + kSegLeaf, w.ns);
   if (w.precision == 0) {
-    for (int64_t j = s0; j < s1; j++) {
+    for (int j = s0; j < s1; j++) {
       float2 a = w.q32[j], b = w.q32[j + 1];
       int rc = raycast_f32_certified(a.x, a.y, b.x, b.y, w.pxl, w.pyl, w.E);
       if (rc == kRayOut) continue;
       w.tests++;
       if (rc < 0) {
         w.fallbacks++;
-        double2 A = w.r.xy[j], B = w.r.xy[j + 1];
-        rc = raycast(A.x, A.y, B.x, B.y, w.px, w.py);
+        rc = raycast_fallback(w.xy, j, w.px, w.py);
       }
-      acc_ray(rc, w.on, w.in);
+      w.on |= rc == kRayOn;
+      w.in ^= rc == kRayIn;
     }
   } else {
-    ring_segments_f64(w.r, s0, s1, 1, w.px, w.py, w.on, w.in, w.tests);
+    for (int j = s0; j < s1; j++) {
+      double2 a = w.xy[j], b = w.xy[j + 1];
+      if (w.py < fmin0(a.y, b.y) || w.py > fmax0(a.y, b.y)) continue;
+      w.tests++;
+      int rc = raycast(a.x, a.y, b.x, b.y, w.px, w.py);
+      w.on |= rc == kRayOn;
+      w.in ^= rc == kRayIn;
+    }
   }
 }
 
 template <int LVL>
-__device__ __forceinline__ void visit(TreeWalk& w, int64_t node) {
+__device__ __forceinline__ void visit(TreeWalk& w, int node) {
   if constexpr (LVL == 0) {
     scan_leaf(w, node);
   } else {
-    int64_t c0 = node * kSegFanout, c1 = min(c0 + kSegFanout, w.cnt[LVL - 1]);
-    const float4* b = w.box + w.off[LVL - 1];
-    for (int64_t c = c0; c < c1; c++)
+    int c0 = node * kSegFanout, c1 = min(c0 + kSegFanout, w.shape.template cnt<LVL - 1>());
+    const float4* b = w.box + w.shape.template off<LVL - 1>();
+    for (int c = c0; c < c1; c++)
       if (box_pass(w, b[c])) visit<LVL - 1>(w, c);
   }
 }
 
 template <int L>
 __device__ __forceinline__ void walk_root(TreeWalk& w) {
-  const float4* b = w.box + w.off[L - 1];
-  for (int64_t c = 0; c < w.cnt[L - 1]; c++)
+  const float4* b = w.box + w.shape.template off<L - 1>();
+  int n = w.shape.template cnt<L - 1>();
+  for (int c = 0; c < n; c++)
     if (box_pass(w, b[c])) visit<L - 1>(w, c);
-}
-
-// Level sizes of a ring with ns segments; returns the number of levels.
-__host__ __device__ __forceinline__ int seg_tree_shape(int64_t ns, int64_t* cnt, int64_t* off) {
-  int64_t c = (ns + kSegLeaf - 1) / kSegLeaf;
-  int L = 0;
-  int64_t o = 0;
-  while (true) {
-    cnt[L] = c;
-    off[L] = o;
-    o += c;
-    L++;
-    if (c <= kSegFanout || L == kSegMaxLevels) break;
-    c = (c + kSegFanout - 1) / kSegFanout;
-  }
-  return L;
 }
 
 __device__ bool build_ring_contains(const Ctx& c, int64_t ring, double px, double py, bool allow_on_edge,
@@ -221,22 +259,22 @@ __device__ bool build_ring_contains(const Ctx& c, int64_t ring, double px, doubl
   if (toff >= 0) {
     TreeWalk w;
     w.box = c.R.seg_tree + toff;
-    w.levels = seg_tree_shape(ns, w.cnt, w.off);
+    w.shape = seg_tree_shape((int)ns);
     w.pxl = pxl;
     w.pyl = pyl;
     w.m = 2.0f * E;
-    w.r = r;
+    w.xy = r.xy;
     w.q32 = c.R.xy32 + r.v0;
     w.E = E;
     w.px = px;
     w.py = py;
-    w.ns = ns;
+    w.ns = (int)ns;
     w.precision = precision;
     w.on = 0;
     w.in = 0;
     w.tests = 0;
     w.fallbacks = 0;
-    switch (w.levels) {
+    switch (w.shape.levels) {
       case 1: walk_root<1>(w); break;
       case 2: walk_root<2>(w); break;
       case 3: walk_root<3>(w); break;
@@ -480,6 +518,43 @@ __device__ __forceinline__ void emit(bool keep, uint32_t b, uint32_t p, uint32_t
   }
 }
 
+// Point-in-polygon refine (build polygons / multipolygons x probe points,
+// the point-probe case of Intersects, Covers and Contains): one lane per
+// candidate, precision fixed at compile time, nothing but the ring walk
+// inlined.
+template <int PRECISION>
+__global__ __launch_bounds__(kBlock) void k_refine_pip(Ctx c, int allow_on_edge, const uint32_t* cb,
+                                                       const uint32_t* cp, uint64_t n, uint32_t* ob, uint32_t* op,
+                                                       uint64_t cap, unsigned long long* counter,
+                                                       unsigned long long* stats) {
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t tests = 0, fallbacks = 0;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+    uint64_t i = base + threadIdx.x;
+    bool keep = false;
+    uint32_t b = 0, p = 0;
+    if (i < n) {
+      b = cb[i];
+      p = cp[i];
+      double2 pt = c.P.xy[p];
+      if (c.B.type[b] != kNull && c.P.type[p] == kPoint && (pt.x == pt.x || pt.y == pt.y)) {
+        int64_t q1 = c.B.part_off[b + 1];
+        for (int64_t q = c.B.part_off[b]; q < q1 && !keep; q++)
+          keep = build_poly_contains(c, q, pt.x, pt.y, allow_on_edge != 0, PRECISION, tests, fallbacks);
+      }
+    }
+    emit(keep, b, p, ob, op, cap, counter);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    tests += __shfl_xor(tests, o);
+    fallbacks += __shfl_xor(fallbacks, o);
+  }
+  if ((threadIdx.x & 63) == 0 && stats) {
+    atomicAdd(&stats[0], (unsigned long long)tests);
+    atomicAdd(&stats[1], (unsigned long long)fallbacks);
+  }
+}
+
 // One lane per candidate; long-ring candidates are deferred into a list.
 __global__ __launch_bounds__(kBlock) void k_refine_lane(Ctx c, int pred, int precision, const uint32_t* cb,
                                                         const uint32_t* cp, uint64_t n, uint32_t* ob, uint32_t* op,
@@ -720,11 +795,7 @@ __global__ void k_seg_counts(GeomView g, int64_t n_rings, int min_segs, int64_t*
   int64_t npts = g.pt_off[ring + 1] - g.pt_off[ring];
   int64_t ns = npts >= 3 ? npts - 1 : 0;
   int64_t total = 0;
-  if (ns >= min_segs) {
-    int64_t c[kSegMaxLevels], o[kSegMaxLevels];
-    int L = seg_tree_shape(ns, c, o);
-    total = o[L - 1] + c[L - 1];
-  }
+  if (ns >= min_segs && ns < (1ll << 30)) total = seg_tree_shape((int)ns).total();
   cnt[ring] = total;
 }
 
@@ -764,8 +835,10 @@ __global__ void k_seg_upper(GeomView g, int64_t n_rings, const int64_t* off, flo
   int64_t o = off[ring];
   if (o < 0) return;
   int64_t ns = g.pt_off[ring + 1] - g.pt_off[ring] - 1;
-  int64_t c[kSegMaxLevels], lo[kSegMaxLevels];
-  int L = seg_tree_shape(ns, c, lo);
+  TreeShape sh = seg_tree_shape((int)ns);
+  int64_t c[kSegMaxLevels] = {sh.c0, sh.c1, sh.c2, sh.c3, sh.c4};
+  int64_t lo[kSegMaxLevels] = {sh.off<0>(), sh.off<1>(), sh.off<2>(), sh.off<3>(), sh.off<4>()};
+  int L = sh.levels;
   for (int l = 1; l < L; l++) {
     const float4* ch = tree + o + lo[l - 1];
     float4* pa = tree + o + lo[l];
@@ -866,11 +939,36 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
   c.P = probe.view();
   DevArray<unsigned long long> aux(4, s);  // counter of long list, stats[2], bad flag
   DevArray<uint32_t> long_list;
+  const uint32_t polyish = (1u << kPolygon) | (1u << kMultiPolygon);
+  bool pip = !build.points_only && probe.points_only && c.R.ring_rect != nullptr &&
+             (build.type_mask & ~polyish) == 0 &&
+             (predicate == kIntersects || predicate == kCovers || predicate == kContains);
+  SJ_CHECK(hipMemsetAsync(out.counter.p, 0, 8, s));
+  SJ_CHECK(hipMemsetAsync(aux.p, 0, 32, s));
+  if (pip) {
+    int allow = predicate == kContains ? 0 : 1;
+    if (timer) timer->record_start(s);
+    if (precision == 0)
+      hipLaunchKernelGGL(k_refine_pip<0>, grid_for(n), kBlock, 0, s, c, allow, cb, cp, n, out.b.p, out.p.p,
+                         (uint64_t)out.b.n, out.counter.p, aux.p + 1);
+    else
+      hipLaunchKernelGGL(k_refine_pip<1>, grid_for(n), kBlock, 0, s, c, allow, cb, cp, n, out.b.p, out.p.p,
+                         (uint64_t)out.b.n, out.counter.p, aux.p + 1);
+    SJ_CHECK(hipGetLastError());
+    if (timer) timer->record_stop(s);
+    unsigned long long h[4], cnt = 0;
+    SJ_CHECK(hipMemcpyAsync(h, aux.p, 32, hipMemcpyDeviceToHost, s));
+    SJ_CHECK(hipMemcpyAsync(&cnt, out.counter.p, 8, hipMemcpyDeviceToHost, s));
+    SJ_CHECK(hipStreamSynchronize(s));
+    if (stats) {
+      stats->segment_tests += h[1];
+      stats->fallback_tests += h[2];
+    }
+    return cnt;
+  }
   bool use_long = !build.points_only && probe.points_only && c.R.ring_rect != nullptr &&
                   (predicate == kIntersects || predicate == kCovers || predicate == kContains);
   if (use_long) long_list.alloc(n, s);
-  SJ_CHECK(hipMemsetAsync(out.counter.p, 0, 8, s));
-  SJ_CHECK(hipMemsetAsync(aux.p, 0, 32, s));
   if (timer) timer->record_start(s);
   hipLaunchKernelGGL(k_refine_lane, grid_for(n), kBlock, 0, s, c, predicate, precision, cb, cp, n, out.b.p, out.p.p,
                      (uint64_t)out.b.n, out.counter.p, use_long ? long_list.p : nullptr, aux.p, aux.p + 1,
@@ -929,6 +1027,19 @@ __global__ void k_unpack(const uint64_t* k, uint64_t n, uint32_t* b, uint32_t* p
   }
 }
 }  // namespace
+
+void sort_pairs_by_build(PairBuf& pb, uint64_t n, int64_t n_build, hipStream_t s) {
+  if (n < 2) return;
+  int bits = 1;
+  while (bits < 32 && (1ll << bits) < n_build) bits++;
+  DevArray<uint32_t> kb(n, s), vp(n, s);
+  size_t tmp = 0;
+  SJ_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, pb.b.p, kb.p, pb.p.p, vp.p, (int)n, 0, bits, s));
+  DevArray<uint8_t> t(tmp + 16, s);
+  SJ_CHECK(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, pb.b.p, kb.p, pb.p.p, vp.p, (int)n, 0, bits, s));
+  pb.b = std::move(kb);
+  pb.p = std::move(vp);
+}
 
 void sort_pairs(uint32_t* b, uint32_t* p, uint64_t n, hipStream_t s) {
   if (n < 2) return;

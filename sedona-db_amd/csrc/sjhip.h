@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -198,6 +199,23 @@ struct RectTree {
   DevArray<uint32_t> leaf_ids;      // their build ids
   DevArray<uint32_t> leaf_keys;     // their Morton keys (sorted)
   float bb[4] = {0, 0, 0, 0};       // bounds of rect centres (Morton frame)
+  // observed candidates per query x 16 (sizes the next probe's output)
+  mutable std::atomic<uint64_t> cand_per_query_x16{0};
+  RectTree() = default;
+  RectTree(RectTree&& o) noexcept { *this = std::move(o); }
+  RectTree& operator=(RectTree&& o) noexcept {
+    n_valid = o.n_valid;
+    n_levels = o.n_levels;
+    level_off = std::move(o.level_off);
+    level_cnt = std::move(o.level_cnt);
+    nodes = std::move(o.nodes);
+    leaf_rects = std::move(o.leaf_rects);
+    leaf_ids = std::move(o.leaf_ids);
+    leaf_keys = std::move(o.leaf_keys);
+    for (int k = 0; k < 4; k++) bb[k] = o.bb[k];
+    cand_per_query_x16.store(o.cand_per_query_x16.load());
+    return *this;
+  }
 };
 void build_rect_tree(const float4* rects_dev, int64_t n, RectTree& t, hipStream_t s);
 
@@ -238,6 +256,9 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
 // Distance refine (build.distance(probe) <= d); probe must be points.
 uint64_t refine_distance(const GeomColumn& build, const GeomColumn& probe, double d, const uint32_t* cb,
                          const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s, KernelTimer* timer = nullptr);
+
+// Regroups candidate pairs by build row (stable radix sort on the build id).
+void sort_pairs_by_build(PairBuf& pb, uint64_t n, int64_t n_build, hipStream_t s);
 
 // Sorts pairs by (probe, build) in place.
 void sort_pairs(uint32_t* b, uint32_t* p, uint64_t n, hipStream_t s);

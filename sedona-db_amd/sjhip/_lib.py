@@ -154,7 +154,7 @@ SjhipJoinResult._fields_ = [
     ("candidates", ctypes.c_uint64),
     ("fallback_tests", ctypes.c_uint64),
     ("segment_tests", ctypes.c_uint64),
-    ("stage_ms", ctypes.c_double * 5),
+    ("stage_ms", ctypes.c_double * 6),
     ("release", ctypes.CFUNCTYPE(None, ctypes.POINTER(SjhipJoinResult))),
     ("private_data", ctypes.c_void_p),
 ]
@@ -168,6 +168,7 @@ class SjhipJoinOptions(ctypes.Structure):
         ("sort_output", ctypes.c_int32),
         ("sort_probes", ctypes.c_int32),
         ("profile", ctypes.c_int32),
+        ("refine_order", ctypes.c_int32),
         ("stream", ctypes.c_void_p),
     ]
 

@@ -447,7 +447,7 @@ class JoinResult:
 
 
 def join_device(index, refiner, probe, predicate, distance=0.0, precision=0, sort_output=True, sort_probes=True,
-                profile=False, stream=None):
+                profile=False, refine_order=1, stream=None):
     """One probe batch through the fused device join.  `probe` is a
     DeviceWkb (device memory) or any Arrow-exportable host column."""
     if isinstance(predicate, str):
@@ -455,7 +455,7 @@ def join_device(index, refiner, probe, predicate, distance=0.0, precision=0, sor
     else:
         pred = int(predicate)
     opts = L.SjhipJoinOptions(pred, float(distance), int(precision), 1 if sort_output else 0, 1 if sort_probes else 0,
-                              1 if profile else 0, stream)
+                              1 if profile else 0, int(refine_order), stream)
     res = L.SjhipJoinResult()
     if isinstance(probe, DeviceWkb):
         sch, dev = probe.schema, probe.dev
