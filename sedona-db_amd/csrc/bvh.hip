@@ -23,6 +23,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "sjhip.h"
+#include "wave_out.h"
 
 namespace sjhip {
 
@@ -148,9 +149,10 @@ __global__ __launch_bounds__(kBlock) void k_filter(TreeView t, const float4* q, 
                                                    uint32_t* ob, uint32_t* op, uint64_t cap,
                                                    unsigned long long* counter) {
   __shared__ uint32_t s_node[kMaxLevels + 1][kBlock];
-  __shared__ uint32_t s_mask[kMaxLevels + 1][kBlock];
+  __shared__ uint8_t s_mask[kMaxLevels + 1][kBlock];
+  __shared__ uint32_t s_ob[kBlock / 64][kWaveOutCap], s_op[kBlock / 64][kWaveOutCap];
   const int tid = threadIdx.x;
-  const int lane = tid & 63;
+  WaveOut wo{s_ob[tid >> 6], s_op[tid >> 6], 0};
   int64_t i = (int64_t)blockIdx.x * blockDim.x + tid;
   bool active = i < nq && t.n_levels > 0;
   uint32_t qid = 0;
@@ -206,31 +208,10 @@ __global__ __launch_bounds__(kBlock) void k_filter(TreeView t, const float4* q, 
         }
       }
     }
-    // wavefront compaction of this step's candidates
-    uint32_t cnt = __popc(hits);
-    uint32_t incl = cnt;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      uint32_t v = __shfl_up(incl, o);
-      if (lane >= o) incl += v;
-    }
-    uint32_t total = __shfl(incl, 63);
-    unsigned long long base = 0;
-    if (total) {
-      if (lane == 0) base = atomicAdd(counter, (unsigned long long)total);
-      base = __shfl(base, 0);
-      unsigned long long w = base + incl - cnt;
-      while (hits) {
-        int k = __ffs(hits) - 1;
-        hits &= hits - 1;
-        if (w < cap) {
-          ob[w] = t.leaf_ids[hbase + k];
-          op[w] = qid;
-        }
-        w++;
-      }
-    }
+    // wavefront compaction of this step's candidates into the LDS buffer
+    if (__ballot(hits != 0)) wave_append_mask(wo, hits, t.leaf_ids, hbase, qid, ob, op, cap, counter);
   }
+  wave_flush(wo, ob, op, cap, counter);
 }
 
 unsigned int blocks_for(int64_t n) { return (unsigned int)((n + kBlock - 1) / kBlock); }

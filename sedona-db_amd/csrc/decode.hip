@@ -105,6 +105,17 @@ __device__ __forceinline__ double2 words_to_xy(const uint32_t w[4], bool le) {
   return make_double2(bitsd(x), bitsd(y));
 }
 
+// ORs the geometry types seen by the active lanes into *mask with one
+// atomic per wavefront (a per-row atomic on one address serialises).
+__device__ __forceinline__ void or_type_mask(unsigned int* mask, uint8_t t) {
+  unsigned int m = 0;
+#pragma unroll
+  for (int k = 1; k <= 6; k++)
+    if (__ballot(t == k)) m |= 1u << k;
+  uint64_t act = __ballot(1);
+  if (m && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)act) - 1) atomicOr(mask, m);
+}
+
 __device__ __forceinline__ void report(unsigned long long* err, int64_t row, uint32_t code) {
   atomicMin(err, ((unsigned long long)row << 8) | code);
 }
@@ -226,7 +237,7 @@ __global__ void k_scan(Src s, int64_t n, uint8_t* type, int64_t* cp, int64_t* cr
   cp[r] = np;
   cr[r] = nr;
   cv[r] = nv;
-  if (t) atomicOr(type_mask, 1u << t);
+  or_type_mask(type_mask, t);
 }
 
 __global__ void k_fill(Src s, int64_t n, const int64_t* part_start, const int64_t* ring_start,
@@ -304,7 +315,7 @@ __global__ void k_points(Src s, int64_t n, double2* xy, uint8_t* type, unsigned 
   }
   xy[r] = out;
   type[r] = t;
-  if (t) atomicOr(type_mask, 1u << t);
+  or_type_mask(type_mask, t);
 }
 
 // Whether every valid row is a 21-byte point record (cheap pre-check).

@@ -30,6 +30,7 @@
 
 #include "geom_math.h"
 #include "sjhip.h"
+#include "wave_out.h"
 
 namespace sjhip {
 
@@ -501,21 +502,15 @@ __device__ __forceinline__ bool is_long(const Ctx& c, int64_t b, int64_t p) {
   return c.B.pt_off[ring + 1] - c.B.pt_off[ring] > kLongRing;
 }
 
+// Per-wavefront LDS output buffer of a kernel (see wave_out.h).
+#define SJ_WAVE_OUT(wo)                                                                   \
+  __shared__ uint32_t wo##_b[kBlock / 64][kWaveOutCap], wo##_p[kBlock / 64][kWaveOutCap]; \
+  WaveOut wo { wo##_b[threadIdx.x >> 6], wo##_p[threadIdx.x >> 6], 0 }
+
 // Wavefront compaction: lanes with keep=1 append (b, p) to the output.
-__device__ __forceinline__ void emit(bool keep, uint32_t b, uint32_t p, uint32_t* ob, uint32_t* op, uint64_t cap,
-                                     unsigned long long* counter) {
-  uint64_t m = __ballot(keep);
-  if (m == 0) return;
-  int lane = threadIdx.x & 63;
-  unsigned long long base = 0;
-  int leader = __ffsll((unsigned long long)m) - 1;
-  if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
-  base = __shfl(base, leader);
-  if (keep) {
-    uint64_t below = m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
-    unsigned long long w = base + __popcll(below);
-    if (w < cap) { ob[w] = b; op[w] = p; }
-  }
+__device__ __forceinline__ void emit(WaveOut& wo, bool keep, uint32_t b, uint32_t p, uint32_t* ob, uint32_t* op,
+                                     uint64_t cap, unsigned long long* counter) {
+  wave_append(wo, keep, b, p, ob, op, cap, counter);
 }
 
 // Point-in-polygon refine (build polygons / multipolygons x probe points,
@@ -527,6 +522,7 @@ __global__ __launch_bounds__(kBlock) void k_refine_pip(Ctx c, int allow_on_edge,
                                                        const uint32_t* cp, uint64_t n, uint32_t* ob, uint32_t* op,
                                                        uint64_t cap, unsigned long long* counter,
                                                        unsigned long long* stats) {
+  SJ_WAVE_OUT(wo);
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   uint64_t tests = 0, fallbacks = 0;
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -543,8 +539,9 @@ __global__ __launch_bounds__(kBlock) void k_refine_pip(Ctx c, int allow_on_edge,
           keep = build_poly_contains(c, q, pt.x, pt.y, allow_on_edge != 0, PRECISION, tests, fallbacks);
       }
     }
-    emit(keep, b, p, ob, op, cap, counter);
+    emit(wo, keep, b, p, ob, op, cap, counter);
   }
+  wave_flush(wo, ob, op, cap, counter);
   for (int o = 32; o > 0; o >>= 1) {
     tests += __shfl_xor(tests, o);
     fallbacks += __shfl_xor(fallbacks, o);
@@ -561,6 +558,7 @@ __global__ __launch_bounds__(kBlock) void k_refine_lane(Ctx c, int pred, int pre
                                                         uint64_t cap, unsigned long long* counter,
                                                         uint32_t* long_list, unsigned long long* long_counter,
                                                         unsigned long long* stats, int* bad) {
+  SJ_WAVE_OUT(wo);
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   uint64_t tests = 0, fallbacks = 0;
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -579,8 +577,9 @@ __global__ __launch_bounds__(kBlock) void k_refine_lane(Ctx c, int pred, int pre
         keep = r > 0;
       }
     }
-    emit(keep, b, p, ob, op, cap, counter);
+    emit(wo, keep, b, p, ob, op, cap, counter);
   }
+  wave_flush(wo, ob, op, cap, counter);
   // per-wave statistics
   for (int o = 32; o > 0; o >>= 1) {
     tests += __shfl_xor(tests, o);
@@ -727,6 +726,7 @@ __device__ double distance_to_point(const GeomView& g, int64_t row, double px, d
 __global__ __launch_bounds__(kBlock) void k_refine_distance(GeomView B, GeomView P, double dist, const uint32_t* cb,
                                                             const uint32_t* cp, uint64_t n, uint32_t* ob,
                                                             uint32_t* op, uint64_t cap, unsigned long long* counter) {
+  SJ_WAVE_OUT(wo);
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
     uint64_t i = base + threadIdx.x;
@@ -739,8 +739,9 @@ __global__ __launch_bounds__(kBlock) void k_refine_distance(GeomView B, GeomView
       bool bok = B.type[b] != kNull && (B.points_only || B.part_off[b] != B.part_off[b + 1]);
       if (bok && point_of(P, p, px, py)) keep = distance_to_point(B, b, px, py) <= dist;
     }
-    emit(keep, b, p, ob, op, cap, counter);
+    emit(wo, keep, b, p, ob, op, cap, counter);
   }
+  wave_flush(wo, ob, op, cap, counter);
 }
 
 // ---- ring index ------------------------------------------------------------
