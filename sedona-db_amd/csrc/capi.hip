@@ -559,7 +559,10 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
   uint64_t nc = filter_pairs(ix->tree, q.p, nq, ord, cand, s, ev.on ? &ft : nullptr);
   mark(3);
   // polygon-major order: one wavefront then walks few distinct rings
-  if (o.refine_order == 1) sort_pairs_by_build(cand, nc, rf->build.n_rows, s);
+  if (o.refine_order == 1) {
+    const float4* br = ix->rects.n >= (size_t)rf->build.n_rows ? ix->rects.p : nullptr;
+    sort_pairs_by_build(cand, nc, rf->build.n_rows, br, probe, s);
+  }
   std::unique_ptr<JoinResultImpl> jr(new JoinResultImpl());
   jr->device_id = rf->rt->device_id;
   RefineStats st;

@@ -286,6 +286,22 @@ __global__ void k_coords(const uint8_t* data, const int64_t* pt_off, const int64
   }
 }
 
+// A 21-byte WKB point record at any byte offset, from six aligned dword
+// loads (the last one holds the record's last byte, so nothing past the
+// record is read).  w[0] byte 0 is the byte order, bytes 1-4 the type,
+// bytes 5-20 the coordinates.
+__device__ __forceinline__ void ld_point_record(const uint8_t* d, int64_t o, uint32_t w[6]) {
+  const uint32_t* base = (const uint32_t*)((uintptr_t)(d + o) & ~(uintptr_t)3);
+  uint32_t sh = (uint32_t)((uintptr_t)(d + o) & 3);
+  uint32_t a0 = base[0], a1 = base[1], a2 = base[2], a3 = base[3], a4 = base[4], a5 = base[5];
+  w[0] = __builtin_amdgcn_alignbyte(a1, a0, sh);
+  w[1] = __builtin_amdgcn_alignbyte(a2, a1, sh);
+  w[2] = __builtin_amdgcn_alignbyte(a3, a2, sh);
+  w[3] = __builtin_amdgcn_alignbyte(a4, a3, sh);
+  w[4] = __builtin_amdgcn_alignbyte(a5, a4, sh);
+  w[5] = __builtin_amdgcn_alignbyte(0u, a5, sh);
+}
+
 // Points-only fast path: one thread per row.
 __global__ void k_points(Src s, int64_t n, double2* xy, uint8_t* type, unsigned long long* err,
                          unsigned int* type_mask) {
@@ -298,15 +314,20 @@ __global__ void k_points(Src s, int64_t n, double2* xy, uint8_t* type, unsigned 
     if (b + 21 > e) {
       report(err, r, kErrTruncated);
     } else {
-      uint8_t bo = s.data[b];
+      uint32_t w[6];
+      ld_point_record(s.data, b, w);
+      uint32_t bo = w[0] & 0xff;
       bool le = bo == 1;
-      uint32_t ty = ld_u32(s.data, b + 1, le);
+      // shift the stream by one byte: c[k] = bytes 1+4k .. 4+4k
+      uint32_t c0 = __builtin_amdgcn_alignbyte(w[1], w[0], 1), c1 = __builtin_amdgcn_alignbyte(w[2], w[1], 1);
+      uint32_t c2 = __builtin_amdgcn_alignbyte(w[3], w[2], 1), c3 = __builtin_amdgcn_alignbyte(w[4], w[3], 1);
+      uint32_t c4 = __builtin_amdgcn_alignbyte(w[5], w[4], 1);
+      uint32_t ty = le ? c0 : __builtin_bswap32(c0);
       if (bo > 1) report(err, r, kErrByteOrder);
       else if (ty != 1) report(err, r, kErrType);
       else {
-        uint32_t w[4];
-        ld_16(s.data, b + 5, w);
-        out = words_to_xy(w, le);
+        uint32_t cw[4] = {c1, c2, c3, c4};
+        out = words_to_xy(cw, le);
         bool empty = (out.x != out.x) && (out.y != out.y);
         t = kPoint;
         if (empty) out = make_double2(NAN, NAN);

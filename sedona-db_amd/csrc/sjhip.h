@@ -257,8 +257,11 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
 uint64_t refine_distance(const GeomColumn& build, const GeomColumn& probe, double d, const uint32_t* cb,
                          const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s, KernelTimer* timer = nullptr);
 
-// Regroups candidate pairs by build row (stable radix sort on the build id).
-void sort_pairs_by_build(PairBuf& pb, uint64_t n, int64_t n_build, hipStream_t s);
+// Regroups candidate pairs by build row and, for point probes, by the
+// point's height within the build rect (radix sort), so that a wavefront
+// refines points of one polygon lying in one horizontal slab.
+void sort_pairs_by_build(PairBuf& pb, uint64_t n, int64_t n_build, const float4* build_rects,
+                         const GeomColumn& probe, hipStream_t s);
 
 // Sorts pairs by (probe, build) in place.
 void sort_pairs(uint32_t* b, uint32_t* p, uint64_t n, hipStream_t s);
