@@ -190,7 +190,6 @@ struct TreeWalk {
   float E;
   double px, py;
   int ns;
-  int precision;
   int on, in;
   uint32_t tests, fallbacks;
 };
@@ -199,24 +198,37 @@ __device__ __forceinline__ bool box_pass(const TreeWalk& w, const float4& b) {
   return w.pyl >= b.y - w.m && w.pyl <= b.w + w.m && w.pxl <= b.z + w.m;
 }
 
+// A leaf's 16 segments: all 17 vertex loads are issued before any test so
+// that the leaf costs one memory round trip.
+template <int PREC>
 __device__ __forceinline__ void scan_leaf(TreeWalk& w, int leaf) {
-  int s0 = leaf * kSegLeaf, s1 = min(s0 + kSegLeaf, w.ns);
-  if (w.precision == 0) {
-    for (int j = s0; j < s1; j++) {
-      float2 a = w.q32[j], b = w.q32[j + 1];
-      int rc = raycast_f32_certified(a.x, a.y, b.x, b.y, w.pxl, w.pyl, w.E);
+  const int s0 = leaf * kSegLeaf;
+  const int m = min(kSegLeaf, w.ns - s0);  // segments in this leaf
+  if constexpr (PREC == 0) {
+    float2 v[kSegLeaf + 1];
+#pragma unroll
+    for (int k = 0; k <= kSegLeaf; k++) v[k] = k <= m ? w.q32[s0 + k] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < kSegLeaf; k++) {
+      if (k >= m) break;
+      int rc = raycast_f32_certified(v[k].x, v[k].y, v[k + 1].x, v[k + 1].y, w.pxl, w.pyl, w.E);
       if (rc == kRayOut) continue;
       w.tests++;
       if (rc < 0) {
         w.fallbacks++;
-        rc = raycast_fallback(w.xy, j, w.px, w.py);
+        rc = raycast_fallback(w.xy, s0 + k, w.px, w.py);
       }
       w.on |= rc == kRayOn;
       w.in ^= rc == kRayIn;
     }
   } else {
-    for (int j = s0; j < s1; j++) {
-      double2 a = w.xy[j], b = w.xy[j + 1];
+    double2 v[kSegLeaf + 1];
+#pragma unroll
+    for (int k = 0; k <= kSegLeaf; k++) v[k] = k <= m ? w.xy[s0 + k] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < kSegLeaf; k++) {
+      if (k >= m) break;
+      double2 a = v[k], b = v[k + 1];
       if (w.py < fmin0(a.y, b.y) || w.py > fmax0(a.y, b.y)) continue;
       w.tests++;
       int rc = raycast(a.x, a.y, b.x, b.y, w.px, w.py);
@@ -226,28 +238,59 @@ __device__ __forceinline__ void scan_leaf(TreeWalk& w, int leaf) {
   }
 }
 
-template <int LVL>
+// Tests up to kSegFanout boxes b[0..n) with all loads issued first.
+__device__ __forceinline__ uint32_t children_mask(const TreeWalk& w, const float4* b, int n) {
+  float4 bx[kSegFanout];
+#pragma unroll
+  for (int k = 0; k < kSegFanout; k++) bx[k] = k < n ? b[k] : make_float4(1.f, 1.f, 0.f, 0.f);
+  uint32_t mask = 0;
+#pragma unroll
+  for (int k = 0; k < kSegFanout; k++)
+    if (k < n && box_pass(w, bx[k])) mask |= 1u << k;
+  return mask;
+}
+
+template <int PREC, int LVL>
 __device__ __forceinline__ void visit(TreeWalk& w, int node) {
   if constexpr (LVL == 0) {
-    scan_leaf(w, node);
+    scan_leaf<PREC>(w, node);
   } else {
-    int c0 = node * kSegFanout, c1 = min(c0 + kSegFanout, w.shape.template cnt<LVL - 1>());
-    const float4* b = w.box + w.shape.template off<LVL - 1>();
-    for (int c = c0; c < c1; c++)
-      if (box_pass(w, b[c])) visit<LVL - 1>(w, c);
+    const int c0 = node * kSegFanout;
+    const int n = min(kSegFanout, w.shape.template cnt<LVL - 1>() - c0);
+    uint32_t mask = children_mask(w, w.box + w.shape.template off<LVL - 1>() + c0, n);
+    while (mask) {
+      int k = __ffs(mask) - 1;
+      mask &= mask - 1;
+      visit<PREC, LVL - 1>(w, c0 + k);
+    }
   }
 }
 
-template <int L>
+template <int PREC, int L>
 __device__ __forceinline__ void walk_root(TreeWalk& w) {
   const float4* b = w.box + w.shape.template off<L - 1>();
-  int n = w.shape.template cnt<L - 1>();
-  for (int c = 0; c < n; c++)
-    if (box_pass(w, b[c])) visit<L - 1>(w, c);
+  const int n = w.shape.template cnt<L - 1>();
+  for (int c0 = 0; c0 < n; c0 += kSegFanout) {
+    uint32_t mask = children_mask(w, b + c0, min(kSegFanout, n - c0));
+    while (mask) {
+      int k = __ffs(mask) - 1;
+      mask &= mask - 1;
+      visit<PREC, L - 1>(w, c0 + k);
+    }
+  }
 }
 
-__device__ bool build_ring_contains(const Ctx& c, int64_t ring, double px, double py, bool allow_on_edge,
-                                    int precision, uint64_t& tests, uint64_t& fallbacks) {
+// Rings of more than 8192 segments (4-5 levels): out of line, so that the
+// common shallow walks keep their registers.
+template <int PREC>
+__device__ __noinline__ void walk_deep(TreeWalk& w) {
+  if (w.shape.levels == 4) walk_root<PREC, 4>(w);
+  else walk_root<PREC, 5>(w);
+}
+
+template <int PREC>
+__device__ __forceinline__ bool build_ring_contains(const Ctx& c, int64_t ring, double px, double py,
+                                                    bool allow_on_edge, uint64_t& tests, uint64_t& fallbacks) {
   double4 rb = c.R.ring_rect[ring];
   if (!rect_covers(rb, px, py)) return false;
   RingRef r = ring_at(c.B, ring);
@@ -270,23 +313,21 @@ __device__ bool build_ring_contains(const Ctx& c, int64_t ring, double px, doubl
     w.px = px;
     w.py = py;
     w.ns = (int)ns;
-    w.precision = precision;
     w.on = 0;
     w.in = 0;
     w.tests = 0;
     w.fallbacks = 0;
     switch (w.shape.levels) {
-      case 1: walk_root<1>(w); break;
-      case 2: walk_root<2>(w); break;
-      case 3: walk_root<3>(w); break;
-      case 4: walk_root<4>(w); break;
-      default: walk_root<5>(w); break;
+      case 1: walk_root<PREC, 1>(w); break;
+      case 2: walk_root<PREC, 2>(w); break;
+      case 3: walk_root<PREC, 3>(w); break;
+      default: walk_deep<PREC>(w); break;
     }
     tests += w.tests;
     fallbacks += w.fallbacks;
     return w.on ? allow_on_edge : (w.in != 0);
   }
-  if (precision == 0) {
+  if constexpr (PREC == 0) {
     ring_segments_f32(r, c.R.xy32, E, pxl, pyl, px, py, 0, ns, 1, on, in, tests, fallbacks);
   } else {
     ring_segments_f64(r, 0, ns, 1, px, py, on, in, tests);
@@ -307,15 +348,16 @@ __device__ bool any_ring_contains(const GeomView& g, int64_t ring, double px, do
 }
 
 // poly_contains_point (tg.c:3909-3932) for part `part` of a build row.
-__device__ bool build_poly_contains(const Ctx& c, int64_t part, double px, double py, bool allow_on_edge,
-                                    int precision, uint64_t& tests, uint64_t& fallbacks) {
+template <int PREC>
+__device__ __forceinline__ bool build_poly_contains(const Ctx& c, int64_t part, double px, double py,
+                                                    bool allow_on_edge, uint64_t& tests, uint64_t& fallbacks) {
   int64_t r0 = c.B.ring_off[part], r1 = c.B.ring_off[part + 1];
   if (r0 == r1) return false;
-  if (r1 - r0 == 1) return build_ring_contains(c, r0, px, py, allow_on_edge, precision, tests, fallbacks);
+  if (r1 - r0 == 1) return build_ring_contains<PREC>(c, r0, px, py, allow_on_edge, tests, fallbacks);
   if (c.B.pt_off[r0 + 1] - c.B.pt_off[r0] < 3) return false;  // tg_poly_empty
-  if (!build_ring_contains(c, r0, px, py, allow_on_edge, precision, tests, fallbacks)) return false;
+  if (!build_ring_contains<PREC>(c, r0, px, py, allow_on_edge, tests, fallbacks)) return false;
   for (int64_t h = r0 + 1; h < r1; h++)
-    if (build_ring_contains(c, h, px, py, !allow_on_edge, precision, tests, fallbacks)) return false;
+    if (build_ring_contains<PREC>(c, h, px, py, !allow_on_edge, tests, fallbacks)) return false;
   return true;
 }
 
@@ -457,7 +499,9 @@ __device__ int eval_relation(const Ctx& c, int64_t b, int64_t p, int pred, int p
       default: return -1;
     }
     for (int64_t q = c.B.part_off[b]; q < c.B.part_off[b + 1]; q++)
-      if (build_poly_contains(c, q, px, py, allow, precision, tests, fallbacks)) return 1;
+      if (precision == 0 ? build_poly_contains<0>(c, q, px, py, allow, tests, fallbacks)
+                         : build_poly_contains<1>(c, q, px, py, allow, tests, fallbacks))
+        return 1;
     return 0;
   }
   if (tb == kPoint && is_polyish(tp)) {
@@ -536,7 +580,7 @@ __global__ __launch_bounds__(kBlock) void k_refine_pip(Ctx c, int allow_on_edge,
       if (c.B.type[b] != kNull && c.P.type[p] == kPoint && (pt.x == pt.x || pt.y == pt.y)) {
         int64_t q1 = c.B.part_off[b + 1];
         for (int64_t q = c.B.part_off[b]; q < q1 && !keep; q++)
-          keep = build_poly_contains(c, q, pt.x, pt.y, allow_on_edge != 0, PRECISION, tests, fallbacks);
+          keep = build_poly_contains<PRECISION>(c, q, pt.x, pt.y, allow_on_edge != 0, tests, fallbacks);
       }
     }
     emit(wo, keep, b, p, ob, op, cap, counter);
