@@ -198,40 +198,35 @@ __device__ __forceinline__ bool box_pass(const TreeWalk& w, const float4& b) {
   return w.pyl >= b.y - w.m && w.pyl <= b.w + w.m && w.pxl <= b.z + w.m;
 }
 
-// A leaf's 16 segments: all 17 vertex loads are issued before any test so
-// that the leaf costs one memory round trip.
+// A leaf's (up to) 16 segments, walking the vertex run once.
 template <int PREC>
 __device__ __forceinline__ void scan_leaf(TreeWalk& w, int leaf) {
   const int s0 = leaf * kSegLeaf;
-  const int m = min(kSegLeaf, w.ns - s0);  // segments in this leaf
+  const int s1 = min(s0 + kSegLeaf, w.ns);
   if constexpr (PREC == 0) {
-    float2 v[kSegLeaf + 1];
-#pragma unroll
-    for (int k = 0; k <= kSegLeaf; k++) v[k] = k <= m ? w.q32[s0 + k] : make_float2(0.f, 0.f);
-#pragma unroll
-    for (int k = 0; k < kSegLeaf; k++) {
-      if (k >= m) break;
-      int rc = raycast_f32_certified(v[k].x, v[k].y, v[k + 1].x, v[k + 1].y, w.pxl, w.pyl, w.E);
+    float2 a = w.q32[s0];
+    for (int j = s0; j < s1; j++) {
+      float2 b = w.q32[j + 1];
+      int rc = raycast_f32_certified(a.x, a.y, b.x, b.y, w.pxl, w.pyl, w.E);
+      a = b;
       if (rc == kRayOut) continue;
       w.tests++;
       if (rc < 0) {
         w.fallbacks++;
-        rc = raycast_fallback(w.xy, s0 + k, w.px, w.py);
+        rc = raycast_fallback(w.xy, j, w.px, w.py);
       }
       w.on |= rc == kRayOn;
       w.in ^= rc == kRayIn;
     }
   } else {
-    double2 v[kSegLeaf + 1];
-#pragma unroll
-    for (int k = 0; k <= kSegLeaf; k++) v[k] = k <= m ? w.xy[s0 + k] : make_double2(0.0, 0.0);
-#pragma unroll
-    for (int k = 0; k < kSegLeaf; k++) {
-      if (k >= m) break;
-      double2 a = v[k], b = v[k + 1];
-      if (w.py < fmin0(a.y, b.y) || w.py > fmax0(a.y, b.y)) continue;
+    double2 a = w.xy[s0];
+    for (int j = s0; j < s1; j++) {
+      double2 b = w.xy[j + 1];
+      double2 aa = a;
+      a = b;
+      if (w.py < fmin0(aa.y, b.y) || w.py > fmax0(aa.y, b.y)) continue;
       w.tests++;
-      int rc = raycast(a.x, a.y, b.x, b.y, w.px, w.py);
+      int rc = raycast(aa.x, aa.y, b.x, b.y, w.px, w.py);
       w.on |= rc == kRayOn;
       w.in ^= rc == kRayIn;
     }
@@ -250,42 +245,54 @@ __device__ __forceinline__ uint32_t children_mask(const TreeWalk& w, const float
   return mask;
 }
 
-template <int PREC, int LVL>
-__device__ __forceinline__ void visit(TreeWalk& w, int node) {
-  if constexpr (LVL == 0) {
-    scan_leaf<PREC>(w, node);
-  } else {
-    const int c0 = node * kSegFanout;
-    const int n = min(kSegFanout, w.shape.template cnt<LVL - 1>() - c0);
-    uint32_t mask = children_mask(w, w.box + w.shape.template off<LVL - 1>() + c0, n);
-    while (mask) {
-      int k = __ffs(mask) - 1;
-      mask &= mask - 1;
-      visit<PREC, LVL - 1>(w, c0 + k);
-    }
-  }
+__device__ __forceinline__ int level_cnt(const TreeShape& t, int l) {
+  return l == 0 ? t.c0 : l == 1 ? t.c1 : l == 2 ? t.c2 : l == 3 ? t.c3 : t.c4;
+}
+__device__ __forceinline__ int level_off(const TreeShape& t, int l) {
+  int o = 0;
+  if (l > 0) o += t.c0;
+  if (l > 1) o += t.c1;
+  if (l > 2) o += t.c2;
+  if (l > 3) o += t.c3;
+  return o;
 }
 
-template <int PREC, int L>
-__device__ __forceinline__ void walk_root(TreeWalk& w) {
-  const float4* b = w.box + w.shape.template off<L - 1>();
-  const int n = w.shape.template cnt<L - 1>();
-  for (int c0 = 0; c0 < n; c0 += kSegFanout) {
-    uint32_t mask = children_mask(w, b + c0, min(kSegFanout, n - c0));
-    while (mask) {
-      int k = __ffs(mask) - 1;
-      mask &= mask - 1;
-      visit<PREC, L - 1>(w, c0 + k);
-    }
-  }
-}
-
-// Rings of more than 8192 segments (4-5 levels): out of line, so that the
-// common shallow walks keep their registers.
+// Depth-first walk without recursion.  The pending children of every level
+// are one byte each of `masks` (bit k of byte l: node base_l + k of level l
+// still to visit).  Children of node v are 8v .. 8v+7, so the base of the
+// level above is recovered as (base >> 3) & ~7.
 template <int PREC>
-__device__ __noinline__ void walk_deep(TreeWalk& w) {
-  if (w.shape.levels == 4) walk_root<PREC, 4>(w);
-  else walk_root<PREC, 5>(w);
+__device__ __forceinline__ void walk_tree(TreeWalk& w) {
+  const int top = w.shape.levels - 1;
+  const int ntop = level_cnt(w.shape, top);
+  const float4* btop = w.box + level_off(w.shape, top);
+  for (int c0 = 0; c0 < ntop; c0 += kSegFanout) {
+    uint64_t masks = (uint64_t)children_mask(w, btop + c0, min(kSegFanout, ntop - c0)) << (8 * top);
+    int lvl = top, base = c0;
+    while (masks) {
+      uint32_t m = (uint32_t)(masks >> (8 * lvl)) & 0xffu;
+      if (m == 0) {
+        lvl++;
+        base = (base >> 3) & ~(kSegFanout - 1);
+        continue;
+      }
+      int k = __ffs(m) - 1;
+      masks &= ~(1ull << (8 * lvl + k));
+      int node = base + k;
+      if (lvl == 0) {
+        scan_leaf<PREC>(w, node);
+        continue;
+      }
+      int cb = node * kSegFanout;
+      int n = min(kSegFanout, level_cnt(w.shape, lvl - 1) - cb);
+      uint32_t cm = children_mask(w, w.box + level_off(w.shape, lvl - 1) + cb, n);
+      if (cm) {
+        lvl--;
+        base = cb;
+        masks |= (uint64_t)cm << (8 * lvl);
+      }
+    }
+  }
 }
 
 template <int PREC>
@@ -317,12 +324,7 @@ __device__ __forceinline__ bool build_ring_contains(const Ctx& c, int64_t ring, 
     w.in = 0;
     w.tests = 0;
     w.fallbacks = 0;
-    switch (w.shape.levels) {
-      case 1: walk_root<PREC, 1>(w); break;
-      case 2: walk_root<PREC, 2>(w); break;
-      case 3: walk_root<PREC, 3>(w); break;
-      default: walk_deep<PREC>(w); break;
-    }
+    walk_tree<PREC>(w);
     tests += w.tests;
     fallbacks += w.fallbacks;
     return w.on ? allow_on_edge : (w.in != 0);
