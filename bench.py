@@ -52,12 +52,18 @@ def main():
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--refine-order", type=int, default=1, help="1 = group candidates by polygon before refine")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only time the plain copy kernel (HBM calibration for the counter summaries)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: every box in this build has one GPU, so --gpus must be 1 "
               "(the multi-GPU sharding plan is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
 
     from sjhip import datagen as D
 
@@ -141,20 +147,31 @@ def main():
         r1.release()
         r1b.release()
 
-    # roofline of the dominant kernel (filter: HBM; refine: FP32 VALU)
+    # Rooflines.  Algorithmic bytes per launch:
+    #   filter: 16 B per probe rect read + 8 B per candidate pair written;
+    #   refine: 8 B per candidate read + 8 B per kept pair written + the
+    #           FP32 vertex runs of the build polygons (8 B per vertex, each
+    #           distinct polygon counted once).
+    # The refine's arithmetic roof (FP32 VALU, 8 flops per segment test) is
+    # reported beside its HBM roof.
     filt_ms, ref_ms = stage[2], stage[3]
     filter_bytes = 16 * n_points + 8 * cands
     verts32 = 8 * (int(wl["counts"].sum()) + n_polys)
     refine_bytes = 8 * cands + 8 * pairs + verts32
     refine_flops = FLOPS_PER_SEGMENT_TEST * seg_tests
+    traffic = read_traffic(os.path.join(ROOT, "profiles", "traffic_pip_zipf_100Mx1M_contains.md"))
+    full = args.scale == 1.0 and args.precision == 0
     roof_filter = dict(bound="hbm", achieved=filter_bytes / (filt_ms / 1e3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
-                       traffic=None, kernel="k_filter", ms=filt_ms)
+                       traffic=traffic.get("k_filter") if full else None, kernel="k_filter", ms=filt_ms,
+                       algorithmic_bytes=filter_bytes)
     roof_filter["frac"] = roof_filter["achieved"] / HBM_PEAK_GBS
     ach_tf = refine_flops / (ref_ms / 1e3) / 1e12
-    roof_refine = dict(bound="valu-fp32", achieved=ach_tf, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s", traffic=None,
-                       kernel="k_refine_lane+k_refine_wave", ms=ref_ms,
-                       hbm_gbs=refine_bytes / (ref_ms / 1e3) / 1e9)
-    roof_refine["frac"] = ach_tf / FP32_PEAK_TFLOPS
+    roof_refine = dict(bound="hbm", achieved=refine_bytes / (ref_ms / 1e3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
+                       traffic=traffic.get("k_refine_pip") if full else None, kernel="k_refine_pip", ms=ref_ms,
+                       algorithmic_bytes=refine_bytes,
+                       valu_fp32={"achieved": ach_tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                  "frac": ach_tf / FP32_PEAK_TFLOPS, "flops": refine_flops})
+    roof_refine["frac"] = roof_refine["achieved"] / HBM_PEAK_GBS
     dominant = roof_refine if ref_ms >= filt_ms else roof_filter
 
     cpu = None
@@ -180,6 +197,7 @@ def main():
                    "parallelism": "single-gpu"},
         "roofline": {k: dominant[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
         "roofline_kernel": dominant["kernel"],
+        "roofline_valu_fp32": dominant.get("valu_fp32"),
         "roofline_stages": {"filter": roof_filter, "refine": roof_refine},
         "cpu_baseline": cpu,
         "pairs_per_step": int(pairs),
@@ -196,6 +214,26 @@ def main():
         out["speedup_vs_plain_fp64"] = plain["ms_per_step"] / ms_per_step
     print(json.dumps(out), flush=True)
     D.release_shared(wl["keep"])
+
+
+def read_traffic(path):
+    """HBM bytes per launch of each profiled kernel: lines 'traffic <kernel> <bytes>'
+    of the committed counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration)."""
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic":
+                out[f[1]] = int(f[2])
+    return out
+
+
+def calibrate_copy(nbytes=1 << 30, reps=20):
+    from sjhip import gpuspatial as G
+
+    ms = G.copy_kernel_ms(nbytes, reps)
+    print(json.dumps({"copy_bytes": nbytes, "ms": ms, "hbm_gbs": 2 * nbytes / (ms / 1e3) / 1e9}), flush=True)
 
 
 def cpu_baseline(strip, gb, gp, args):

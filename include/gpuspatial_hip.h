@@ -303,6 +303,13 @@ int SjhipCopyToDevice(void* dst, const void* src, uint64_t bytes);
 int SjhipCopyToHost(void* dst, const void* src, uint64_t bytes);
 int SjhipSetDevice(int device_id);
 
+/* Bandwidth calibration (no counterpart in the reference): copies `bytes`
+ * (a multiple of 16) from src to dst with a plain vector-copy kernel,
+ * `reps` times on one stream, and returns the mean kernel time in *ms.
+ * Its HBM traffic is exactly 2 * bytes per launch, which scales the
+ * profiler's FETCH_SIZE / WRITE_SIZE readings of the other kernels. */
+int SjhipCopyKernel(void* dst, const void* src, uint64_t bytes, int reps, double* ms);
+
 /* Error text of the last failed extension call on this thread. */
 const char* SjhipLastError(void);
 

@@ -498,6 +498,12 @@ void jr_release(SjhipJoinResult* r) {
   r->size = 0;
 }
 
+// Plain 16-byte vector copy (bandwidth calibration).
+__global__ __launch_bounds__(256) void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n) {
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 }  // namespace
 
 // Fused device join used by the extension entry point and the request loop.
@@ -749,6 +755,31 @@ int SjhipCopyToHost(void* dst, const void* src, uint64_t bytes) {
 
 int SjhipSetDevice(int device_id) {
   return guarded(nullptr, [&] { SJ_CHECK(hipSetDevice(device_id)); });
+}
+
+int SjhipCopyKernel(void* dst, const void* src, uint64_t bytes, int reps, double* ms) {
+  return guarded(nullptr, [&] {
+    if (!dst || !src || !ms || reps < 1 || bytes == 0 || bytes % 16) throw Error("bad copy arguments");
+    hipStream_t s;
+    SJ_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    SJ_CHECK(hipEventCreate(&e0));
+    SJ_CHECK(hipEventCreate(&e1));
+    uint64_t n = bytes / 16;
+    SJ_CHECK(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; r++) {
+      hipLaunchKernelGGL(k_copy16, 256 * 32, 256, 0, s, (uint4*)dst, (const uint4*)src, n);
+      SJ_CHECK(hipGetLastError());
+    }
+    SJ_CHECK(hipEventRecord(e1, s));
+    SJ_CHECK(hipEventSynchronize(e1));
+    float t = 0;
+    SJ_CHECK(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+  });
 }
 
 int SjhipJoinResultCopyToHost(const struct SjhipJoinResult* r, uint32_t* b, uint32_t* p) {

@@ -105,15 +105,19 @@ __device__ __forceinline__ double2 words_to_xy(const uint32_t w[4], bool le) {
   return make_double2(bitsd(x), bitsd(y));
 }
 
-// ORs the geometry types seen by the active lanes into *mask with one
-// atomic per wavefront (a per-row atomic on one address serialises).
+// ORs the geometry types seen by the active lanes into *mask with at most
+// one atomic per wavefront.
 __device__ __forceinline__ void or_type_mask(unsigned int* mask, uint8_t t) {
   unsigned int m = 0;
 #pragma unroll
   for (int k = 1; k <= 6; k++)
     if (__ballot(t == k)) m |= 1u << k;
   uint64_t act = __ballot(1);
-  if (m && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)act) - 1) atomicOr(mask, m);
+  // bits already set need no atomic (most waves of a column see the same
+  // types; same-address atomics from every wave serialise in L2)
+  if (m && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)act) - 1 &&
+      (__atomic_load_n(mask, __ATOMIC_RELAXED) & m) != m)
+    atomicOr(mask, m);
 }
 
 __device__ __forceinline__ void report(unsigned long long* err, int64_t row, uint32_t code) {

@@ -179,7 +179,7 @@ EXPORTED = [
     "SjhipExtVersion", "SjhipRefinerRefineDistance", "SjhipRefinerKnn", "SjhipJoinDevice",
     "SjhipJoinResultCopyToHost", "SjhipIndexBuildFromRefiner", "SjhipRefinerPushBuildDevice", "SjhipLastError",
     "SjhipComputeRects", "SjhipDeviceAlloc", "SjhipDeviceFree", "SjhipCopyToDevice", "SjhipCopyToHost",
-    "SjhipSetDevice",
+    "SjhipSetDevice", "SjhipCopyKernel",
     "SjhipHostRaycast", "SjhipHostRaycastF32", "SjhipHostCollinear", "SjhipHostSegmentIntersects",
     "SjhipHostOrient2d", "SjhipHostHypot", "SjhipHostCoordToLine", "SjhipHostRaycastBatch", "SjhipHostHypotBatch",
 ]
@@ -216,6 +216,8 @@ def lib():
         L.SjhipCopyToDevice.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
         L.SjhipCopyToHost.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
         L.SjhipSetDevice.argtypes = [ctypes.c_int]
+        L.SjhipCopyKernel.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_double)]
         d6 = [ctypes.c_double] * 6
         L.SjhipHostRaycast.argtypes = d6
         L.SjhipHostRaycastF32.argtypes = [ctypes.c_float] * 7

@@ -210,6 +210,20 @@ class DeviceBuffer:
             pass
 
 
+def copy_kernel_ms(nbytes, reps=10, device_id=0):
+    """Mean time (ms) of the library's plain copy kernel over nbytes
+    (2 * nbytes of HBM traffic per launch)."""
+    nbytes = int(nbytes) // 16 * 16
+    src, dst = DeviceBuffer(nbytes, device_id), DeviceBuffer(nbytes, device_id)
+    ms = ctypes.c_double()
+    rc = L.lib().SjhipCopyKernel(dst.ptr, src.ptr, nbytes, int(reps), ctypes.byref(ms))
+    src.free()
+    dst.free()
+    if rc:
+        raise GpuSpatialError(L.lib().SjhipLastError().decode())
+    return ms.value
+
+
 def host_device_array(exported):
     """Wraps an ExportedArrow as an ArrowDeviceArray with device_type CPU."""
     d = L.ArrowDeviceArray()
