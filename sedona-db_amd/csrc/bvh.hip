@@ -146,7 +146,7 @@ struct TreeView {
 
 template <int F, int LEAF>
 __global__ __launch_bounds__(kBlock) void k_filter(TreeView t, const float4* q, int64_t nq, const uint32_t* qorder,
-                                                   uint32_t* ob, uint32_t* op, uint64_t cap,
+                                                   int emit_rank, uint32_t* ob, uint32_t* op, uint64_t cap,
                                                    unsigned long long* counter) {
   __shared__ uint32_t s_node[kMaxLevels + 1][kBlock];
   __shared__ uint8_t s_mask[kMaxLevels + 1][kBlock];
@@ -155,12 +155,13 @@ __global__ __launch_bounds__(kBlock) void k_filter(TreeView t, const float4* q, 
   WaveOut wo{s_ob[tid >> 6], s_op[tid >> 6], 0};
   int64_t i = (int64_t)blockIdx.x * blockDim.x + tid;
   bool active = i < nq && t.n_levels > 0;
-  uint32_t qid = 0;
+  uint32_t qid = 0, pid = 0;  // query row; probe id emitted (row, or position i in qorder)
   float4 box = make_float4(0, 0, 0, 0);
   const int top = t.n_levels;
   int lvl = top;
   if (active) {
     qid = qorder ? qorder[i] : (uint32_t)i;
+    pid = emit_rank ? (uint32_t)i : qid;
     box = q[qid];
     if (!rect_ok(box)) active = false;
   }
@@ -209,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_filter(TreeView t, const float4* q, 
       }
     }
     // wavefront compaction of this step's candidates into the LDS buffer
-    if (__ballot(hits != 0)) wave_append_mask(wo, hits, t.leaf_ids, hbase, qid, ob, op, cap, counter);
+    if (__ballot(hits != 0)) wave_append_mask(wo, hits, t.leaf_ids, hbase, pid, ob, op, cap, counter);
   }
   wave_flush(wo, ob, op, cap, counter);
 }
@@ -293,7 +294,7 @@ void build_rect_tree(const float4* rects, int64_t n, RectTree& t, hipStream_t s)
 }
 
 uint64_t filter_pairs(const RectTree& t, const float4* q, int64_t nq, const uint32_t* qorder, PairBuf& pb,
-                      hipStream_t s, KernelTimer* timer) {
+                      hipStream_t s, KernelTimer* timer, bool emit_rank) {
   if (pb.counter.n < 1) pb.counter.alloc(1, s);
   // first capacity: the tree's observed candidates per query (16ths) from
   // earlier probes, else 4 per query; one resize-and-rerun on overflow
@@ -318,7 +319,7 @@ uint64_t filter_pairs(const RectTree& t, const float4* q, int64_t nq, const uint
     SJ_CHECK(hipMemsetAsync(pb.counter.p, 0, 8, s));
     if (timer) timer->record_start(s);
     hipLaunchKernelGGL((k_filter<RectTree::FANOUT, RectTree::LEAF>), blocks_for(nq), kBlock, 0, s, v, q, nq, qorder,
-                       pb.b.p, pb.p.p, (uint64_t)pb.b.n, pb.counter.p);
+                       emit_rank ? 1 : 0, pb.b.p, pb.p.p, (uint64_t)pb.b.n, pb.counter.p);
     SJ_CHECK(hipGetLastError());
     if (timer) timer->record_stop(s);
     unsigned long long total = 0;

@@ -560,14 +560,22 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
     nq = morton_order(q.p, probe.n_rows, order, s);
     ord = order.p;
   }
+  // Point probes refined by the point-in-polygon kernel are gathered into
+  // Morton order: candidates then name positions in that order, so that the
+  // points of one polygon's candidates lie together in memory.  The refine
+  // maps kept positions back to probe rows.
+  const bool ranked = ord != nullptr && !dwithin && pip_refine_applies(rf->build, rf->ring_index, probe, o.predicate);
+  GeomColumn rprobe;
+  if (ranked) gather_points(probe, ord, nq, rprobe, s);
+  const GeomColumn& rp = ranked ? rprobe : probe;
   mark(2);
   PairBuf cand;
-  uint64_t nc = filter_pairs(ix->tree, q.p, nq, ord, cand, s, ev.on ? &ft : nullptr);
+  uint64_t nc = filter_pairs(ix->tree, q.p, nq, ord, cand, s, ev.on ? &ft : nullptr, ranked);
   mark(3);
   // polygon-major order: one wavefront then walks few distinct rings
   if (o.refine_order == 1) {
     const float4* br = ix->rects.n >= (size_t)rf->build.n_rows ? ix->rects.p : nullptr;
-    sort_pairs_by_build(cand, nc, rf->build.n_rows, br, probe, s);
+    sort_pairs_by_build(cand, nc, rf->build.n_rows, br, rp, s);
   }
   std::unique_ptr<JoinResultImpl> jr(new JoinResultImpl());
   jr->device_id = rf->rt->device_id;
@@ -576,8 +584,8 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
   if (dwithin)
     kept = refine_distance(rf->build, probe, o.distance, cand.b.p, cand.p.p, nc, jr->pairs, s, ev.on ? &rt : nullptr);
   else
-    kept = refine_relation(rf->build, rf->ring_index, probe, o.predicate, cand.b.p, cand.p.p, nc, o.precision,
-                           jr->pairs, &st, s, ev.on ? &rt : nullptr);
+    kept = refine_relation(rf->build, rf->ring_index, rp, o.predicate, cand.b.p, cand.p.p, nc, o.precision,
+                           jr->pairs, &st, s, ev.on ? &rt : nullptr, ranked ? ord : nullptr);
   mark(8);
   if (o.sort_output) sort_pairs(jr->pairs.b.p, jr->pairs.p.p, kept, s);
   mark(9);

@@ -618,7 +618,23 @@ __global__ void k_rects_rows(GeomView g, int nearest, double expand, float4* out
   }
 }
 
+__global__ void k_gather_points(const double2* xy, const uint32_t* order, int64_t n, double2* out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = xy[order[i]];
+}
+
 }  // namespace
+
+void gather_points(const GeomColumn& probe, const uint32_t* order, int64_t n, GeomColumn& out, hipStream_t s) {
+  if (!probe.points_only) throw Error("gather_points needs a point column");
+  out = GeomColumn();
+  out.n_rows = out.n_parts = out.n_rings = out.n_pts = n;
+  out.points_only = true;
+  out.type_mask = probe.type_mask;
+  out.xy.alloc(n > 0 ? n : 1, s);
+  if (n > 0) hipLaunchKernelGGL(k_gather_points, blocks_for(n), kBlock, 0, s, probe.xy.p, order, n, out.xy.p);
+  SJ_CHECK(hipGetLastError());
+}
 
 void compute_rects(const GeomColumn& g, const RingIndex* ri, bool point_nearest, double expand,
                    DevArray<float4>& rects, hipStream_t s) {

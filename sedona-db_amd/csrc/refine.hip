@@ -567,7 +567,7 @@ template <int PRECISION>
 __global__ __launch_bounds__(kBlock) void k_refine_pip(Ctx c, int allow_on_edge, const uint32_t* cb,
                                                        const uint32_t* cp, uint64_t n, uint32_t* ob, uint32_t* op,
                                                        uint64_t cap, unsigned long long* counter,
-                                                       unsigned long long* stats) {
+                                                       unsigned long long* stats, const uint32_t* probe_ids) {
   SJ_WAVE_OUT(wo);
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   uint64_t tests = 0, fallbacks = 0;
@@ -578,12 +578,14 @@ __global__ __launch_bounds__(kBlock) void k_refine_pip(Ctx c, int allow_on_edge,
     if (i < n) {
       b = cb[i];
       p = cp[i];
+      // the probe is a point column: null and empty rows hold NaN
       double2 pt = c.P.xy[p];
-      if (c.B.type[b] != kNull && c.P.type[p] == kPoint && (pt.x == pt.x || pt.y == pt.y)) {
+      if (c.B.type[b] != kNull && (pt.x == pt.x || pt.y == pt.y)) {
         int64_t q1 = c.B.part_off[b + 1];
         for (int64_t q = c.B.part_off[b]; q < q1 && !keep; q++)
           keep = build_poly_contains<PRECISION>(c, q, pt.x, pt.y, allow_on_edge != 0, tests, fallbacks);
       }
+      if (keep && probe_ids) p = probe_ids[p];
     }
     emit(wo, keep, b, p, ob, op, cap, counter);
   }
@@ -972,9 +974,15 @@ void check_relation_types(uint32_t bm, uint32_t pm, int pred) {
                 std::to_string(pred) + ")");
 }
 
+bool pip_refine_applies(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, int predicate) {
+  const uint32_t polyish = (1u << kPolygon) | (1u << kMultiPolygon);
+  return !build.points_only && probe.points_only && ri.ring_rect.p != nullptr && (build.type_mask & ~polyish) == 0 &&
+         (predicate == kIntersects || predicate == kCovers || predicate == kContains);
+}
+
 uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, int predicate,
                          const uint32_t* cb, const uint32_t* cp, uint64_t n, int precision, PairBuf& out,
-                         RefineStats* stats, hipStream_t s, KernelTimer* timer) {
+                         RefineStats* stats, hipStream_t s, KernelTimer* timer, const uint32_t* probe_ids) {
   out.counter.reserve(1, s);
   out.b.reserve(n > 0 ? n : 1, s);
   out.p.reserve(n > 0 ? n : 1, s);
@@ -986,10 +994,8 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
   c.P = probe.view();
   DevArray<unsigned long long> aux(4, s);  // counter of long list, stats[2], bad flag
   DevArray<uint32_t> long_list;
-  const uint32_t polyish = (1u << kPolygon) | (1u << kMultiPolygon);
-  bool pip = !build.points_only && probe.points_only && c.R.ring_rect != nullptr &&
-             (build.type_mask & ~polyish) == 0 &&
-             (predicate == kIntersects || predicate == kCovers || predicate == kContains);
+  bool pip = pip_refine_applies(build, ri, probe, predicate);
+  if (probe_ids && !pip) throw Error("probe id mapping needs the point-in-polygon refine");
   SJ_CHECK(hipMemsetAsync(out.counter.p, 0, 8, s));
   SJ_CHECK(hipMemsetAsync(aux.p, 0, 32, s));
   if (pip) {
@@ -997,10 +1003,10 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
     if (timer) timer->record_start(s);
     if (precision == 0)
       hipLaunchKernelGGL(k_refine_pip<0>, grid_for(n), kBlock, 0, s, c, allow, cb, cp, n, out.b.p, out.p.p,
-                         (uint64_t)out.b.n, out.counter.p, aux.p + 1);
+                         (uint64_t)out.b.n, out.counter.p, aux.p + 1, probe_ids);
     else
       hipLaunchKernelGGL(k_refine_pip<1>, grid_for(n), kBlock, 0, s, c, allow, cb, cp, n, out.b.p, out.p.p,
-                         (uint64_t)out.b.n, out.counter.p, aux.p + 1);
+                         (uint64_t)out.b.n, out.counter.p, aux.p + 1, probe_ids);
     SJ_CHECK(hipGetLastError());
     if (timer) timer->record_stop(s);
     unsigned long long h[4], cnt = 0;

@@ -233,9 +233,16 @@ struct PairBuf {
 };
 
 // Runs the filter of queries against the tree into pb; returns the count.
-// qorder (optional) gives the visiting order of the queries.
+// qorder (optional) gives the visiting order of the queries; with
+// emit_rank the pairs carry the query's position i in qorder instead of
+// its row qorder[i].
 uint64_t filter_pairs(const RectTree& t, const float4* q, int64_t nq, const uint32_t* qorder, PairBuf& pb,
-                      hipStream_t s, KernelTimer* timer = nullptr);
+                      hipStream_t s, KernelTimer* timer = nullptr, bool emit_rank = false);
+
+// Gathers the points of a point column in the given row order
+// (out.xy[i] = probe.xy[order[i]]), producing a point column of n rows
+// without a type array (its rows are valid points or NaN).
+void gather_points(const GeomColumn& probe, const uint32_t* order, int64_t n, GeomColumn& out, hipStream_t s);
 
 // Morton order of query rects (a permutation of valid queries; invalid ones
 // are dropped).  Returns the number of valid queries.
@@ -247,11 +254,18 @@ struct RefineStats {
   uint64_t fallback_tests = 0;
 };
 
+// Whether refine_relation takes its point-in-polygon kernel (polygon
+// build rows, point probes, Intersects / Covers / Contains).
+bool pip_refine_applies(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, int predicate);
+
 // Relation refine of candidate pairs; compacts kept pairs into out (count
 // returned).  precision: 0 = FP32 certified + FP64 fallback, 1 = all-FP64.
+// probe_ids (point-in-polygon kernel only): cp holds positions in probe,
+// and the kept pairs carry probe_ids[position].
 uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, int predicate,
                          const uint32_t* cb, const uint32_t* cp, uint64_t n, int precision, PairBuf& out,
-                         RefineStats* stats, hipStream_t s, KernelTimer* timer = nullptr);
+                         RefineStats* stats, hipStream_t s, KernelTimer* timer = nullptr,
+                         const uint32_t* probe_ids = nullptr);
 
 // Distance refine (build.distance(probe) <= d); probe must be points.
 uint64_t refine_distance(const GeomColumn& build, const GeomColumn& probe, double d, const uint32_t* cb,
