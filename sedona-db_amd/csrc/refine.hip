@@ -295,18 +295,20 @@ __device__ __forceinline__ void walk_tree(TreeWalk& w) {
   }
 }
 
+// tg_ring_contains_point over a build ring given its rect, vertex run,
+// FP32 frame (origin, bound E) and segment tree (toff < 0: none).
 template <int PREC>
-__device__ __forceinline__ bool build_ring_contains(const Ctx& c, int64_t ring, double px, double py,
-                                                    bool allow_on_edge, uint64_t& tests, uint64_t& fallbacks) {
-  double4 rb = c.R.ring_rect[ring];
+__device__ __forceinline__ bool ring_contains_at(const Ctx& c, const double4& rb, int64_t v0, int64_t npts, double2 o,
+                                                 float E, int64_t toff, double px, double py, bool allow_on_edge,
+                                                 uint64_t& tests, uint64_t& fallbacks) {
   if (!rect_covers(rb, px, py)) return false;
-  RingRef r = ring_at(c.B, ring);
+  RingRef r;
+  r.v0 = v0;
+  r.npts = npts;
+  r.xy = c.B.xy + v0;
   int64_t ns = r.npts >= 3 ? r.npts - 1 : 0;
   int on = 0, in = 0;
-  double2 o = c.R.origin[ring];
-  float E = c.R.err[ring];
   float pxl = (float)(px - o.x), pyl = (float)(py - o.y);
-  int64_t toff = c.R.seg_tree_off ? c.R.seg_tree_off[ring] : -1;
   if (toff >= 0) {
     TreeWalk w;
     w.box = c.R.seg_tree + toff;
@@ -335,6 +337,25 @@ __device__ __forceinline__ bool build_ring_contains(const Ctx& c, int64_t ring, 
     ring_segments_f64(r, 0, ns, 1, px, py, on, in, tests);
   }
   return on ? allow_on_edge : (in != 0);
+}
+
+template <int PREC>
+__device__ __forceinline__ bool build_ring_contains(const Ctx& c, int64_t ring, double px, double py,
+                                                    bool allow_on_edge, uint64_t& tests, uint64_t& fallbacks) {
+  double4 rb = c.R.ring_rect[ring];
+  if (!rect_covers(rb, px, py)) return false;
+  int64_t v0 = c.B.pt_off[ring];
+  int64_t toff = c.R.seg_tree_off ? c.R.seg_tree_off[ring] : -1;
+  return ring_contains_at<PREC>(c, rb, v0, c.B.pt_off[ring + 1] - v0, c.R.origin[ring], c.R.err[ring], toff, px, py,
+                                allow_on_edge, tests, fallbacks);
+}
+
+// Single-ring build rows (one part, one ring) as one 64-byte record, so
+// that the point-in-polygon kernel reaches a row's ring in one load.
+__device__ __forceinline__ double2 ring_origin(const double4& rb) {
+  double ox = 0.5 * rb.x + 0.5 * rb.z, oy = 0.5 * rb.y + 0.5 * rb.w;
+  if (!(ox == ox)) { ox = 0; oy = 0; }
+  return make_double2(ox, oy);
 }
 
 // tg_ring_contains_point for any ring (probe side: rect scanned on the fly).
@@ -580,10 +601,16 @@ __global__ __launch_bounds__(kBlock) void k_refine_pip(Ctx c, int allow_on_edge,
       p = cp[i];
       // the probe is a point column: null and empty rows hold NaN
       double2 pt = c.P.xy[p];
-      if (c.B.type[b] != kNull && (pt.x == pt.x || pt.y == pt.y)) {
-        int64_t q1 = c.B.part_off[b + 1];
-        for (int64_t q = c.B.part_off[b]; q < q1 && !keep; q++)
-          keep = build_poly_contains<PRECISION>(c, q, pt.x, pt.y, allow_on_edge != 0, tests, fallbacks);
+      if (pt.x == pt.x || pt.y == pt.y) {
+        const PolyDesc d = c.R.desc[b];
+        if (d.npts >= 0) {
+          keep = ring_contains_at<PRECISION>(c, d.rect, d.v0, d.npts, ring_origin(d.rect), d.err, d.tree_off, pt.x,
+                                             pt.y, allow_on_edge != 0, tests, fallbacks);
+        } else if (c.B.type[b] != kNull) {
+          int64_t q1 = c.B.part_off[b + 1];
+          for (int64_t q = c.B.part_off[b]; q < q1 && !keep; q++)
+            keep = build_poly_contains<PRECISION>(c, q, pt.x, pt.y, allow_on_edge != 0, tests, fallbacks);
+        }
       }
       if (keep && probe_ids) p = probe_ids[p];
     }
@@ -812,15 +839,15 @@ __global__ void k_ring_rects(GeomView g, int64_t n_rings, double4* rr, double2* 
     yh = fmax0(yh, __shfl_xor(yh, o));
   }
   if (lane == 0) {
-    rr[ring] = make_double4(xl, yl, xh, yh);
+    double4 rb = make_double4(xl, yl, xh, yh);
+    rr[ring] = rb;
     double ox = 0.5 * xl + 0.5 * xh, oy = 0.5 * yl + 0.5 * yh;
     double H = fmax(xh - xl, yh - yl) * 0.5;
     float E;
     // ring-local FP32 coordinates are within 2^-23 H of exact; use 2^-22 H
     if (v1 - v0 < 3 || !(H > 1e-15) || !(H < 1e15) || ox != ox || oy != oy) E = INFINITY;
     else E = (float)(H * 2.384185791015625e-07) * 1.0001f;
-    if (!(ox == ox)) { ox = 0; oy = 0; }
-    origin[ring] = make_double2(ox, oy);
+    origin[ring] = ring_origin(rb);
     err[ring] = E;
   }
 }
@@ -911,6 +938,33 @@ unsigned int grid_for(uint64_t n) {
   return (unsigned int)std::max<uint64_t>(1, std::min<uint64_t>(b, 1u << 16));
 }
 
+__global__ void k_poly_desc(GeomView g, RingIndexView R, PolyDesc* out) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= g.n_rows) return;
+  PolyDesc d;
+  d.rect = make_double4(NAN, NAN, NAN, NAN);
+  d.v0 = 0;
+  d.tree_off = -1;
+  d.npts = -1;
+  d.err = INFINITY;
+  d.pad = 0;
+  if (is_polyish(g.type[r]) && g.part_off[r + 1] - g.part_off[r] == 1) {
+    int64_t q = g.part_off[r];
+    if (g.ring_off[q + 1] - g.ring_off[q] == 1) {
+      int64_t ring = g.ring_off[q];
+      int64_t v0 = g.pt_off[ring], n = g.pt_off[ring + 1] - v0;
+      if (n <= 0x7fffffff) {
+        d.rect = R.ring_rect[ring];
+        d.v0 = v0;
+        d.npts = (int32_t)n;
+        d.err = R.err[ring];
+        d.tree_off = R.seg_tree_off[ring];
+      }
+    }
+  }
+  out[r] = d;
+}
+
 }  // namespace
 
 void build_ring_index(const GeomColumn& g, RingIndex& ri, const RingIndexParams& prm, hipStream_t s) {
@@ -948,6 +1002,8 @@ void build_ring_index(const GeomColumn& g, RingIndex& ri, const RingIndexParams&
     hipLaunchKernelGGL(k_seg_upper, blocks_for(g.n_rings), kBlock, 0, s, g.view(), g.n_rings, ri.seg_tree_off.p,
                        ri.seg_tree.p);
   }
+  ri.desc.alloc(g.n_rows > 0 ? g.n_rows : 1, s);
+  if (g.n_rows > 0) hipLaunchKernelGGL(k_poly_desc, blocks_for(g.n_rows), kBlock, 0, s, g.view(), ri.view(), ri.desc.p);
   SJ_CHECK(hipGetLastError());
   SJ_CHECK(hipStreamSynchronize(s));
 }
@@ -976,7 +1032,8 @@ void check_relation_types(uint32_t bm, uint32_t pm, int pred) {
 
 bool pip_refine_applies(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, int predicate) {
   const uint32_t polyish = (1u << kPolygon) | (1u << kMultiPolygon);
-  return !build.points_only && probe.points_only && ri.ring_rect.p != nullptr && (build.type_mask & ~polyish) == 0 &&
+  return !build.points_only && probe.points_only && ri.ring_rect.p != nullptr && ri.desc.p != nullptr &&
+         (build.type_mask & ~polyish) == 0 &&
          (predicate == kIntersects || predicate == kCovers || predicate == kContains);
 }
 

@@ -101,6 +101,18 @@ constexpr int kSegLeaf = 16;
 constexpr int kSegFanout = 8;
 constexpr int kSegMaxLevels = 5;
 
+// Record of a build row that is one polygon with one ring (npts >= 0): its
+// ring's rect, first vertex, vertex count, FP32 bound and segment-tree
+// offset (-1: none).  npts = -1 marks every other row.
+struct alignas(32) PolyDesc {
+  double4 rect;
+  int64_t v0;
+  int64_t tree_off;
+  int32_t npts;
+  float err;
+  int64_t pad;
+};
+
 // Device view of the ring refine index (build side).
 struct RingIndexView {
   const double4* ring_rect = nullptr;   // xmin, ymin, xmax, ymax (double)
@@ -109,6 +121,7 @@ struct RingIndexView {
   const float* err = nullptr;           // per ring: FP32 certification bound E
   const int64_t* seg_tree_off = nullptr;  // per ring: first box in seg_tree, -1 = not indexed
   const float4* seg_tree = nullptr;       // boxes (xmin, ymin, xmax, ymax), ring-local FP32
+  const PolyDesc* desc = nullptr;         // per build row
 };
 
 // Owned geometry column on the device.
@@ -139,6 +152,7 @@ struct RingIndex {
   DevArray<float> err;
   DevArray<int64_t> seg_tree_off;
   DevArray<float4> seg_tree;
+  DevArray<PolyDesc> desc;
   int64_t n_boxes = 0;
   RingIndexView view() const {
     RingIndexView v;
@@ -148,6 +162,7 @@ struct RingIndex {
     v.err = err.p;
     v.seg_tree_off = seg_tree_off.p;
     v.seg_tree = seg_tree.p;
+    v.desc = desc.p;
     return v;
   }
 };
