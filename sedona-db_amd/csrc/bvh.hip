@@ -148,7 +148,8 @@ template <int F, int LEAF>
 __global__ __launch_bounds__(kBlock) void k_filter(TreeView t, const float4* q, int64_t nq, const uint32_t* qorder,
                                                    int emit_rank, uint32_t* ob, uint32_t* op, uint64_t cap,
                                                    unsigned long long* counter) {
-  __shared__ uint32_t s_node[kMaxLevels + 1][kBlock];
+  // per lane and level: the children of the level's current node still to
+  // visit (the node itself is `cur`, its parent cur / F)
   __shared__ uint8_t s_mask[kMaxLevels + 1][kBlock];
   __shared__ uint32_t s_ob[kBlock / 64][kWaveOutCap], s_op[kBlock / 64][kWaveOutCap];
   const int tid = threadIdx.x;
@@ -159,6 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_filter(TreeView t, const float4* q, 
   float4 box = make_float4(0, 0, 0, 0);
   const int top = t.n_levels;
   int lvl = top;
+  uint32_t cur = 0;  // node at level lvl (level top: the virtual root)
   if (active) {
     qid = qorder ? qorder[i] : (uint32_t)i;
     pid = emit_rank ? (uint32_t)i : qid;
@@ -172,21 +174,23 @@ __global__ __launch_bounds__(kBlock) void k_filter(TreeView t, const float4* q, 
     int64_t nc = t.cnt[top - 1];
     for (int c = 0; c < F; c++)
       if (c < nc && hit(nd[c], box)) m |= 1u << c;
-    s_node[top][tid] = 0;
     s_mask[top][tid] = m;
   }
   while (__any(active)) {
     uint32_t hits = 0;
     int64_t hbase = 0;
     if (active) {
-      while (lvl <= top && s_mask[lvl][tid] == 0) lvl++;
+      while (lvl <= top && s_mask[lvl][tid] == 0) {
+        lvl++;
+        cur /= F;
+      }
       if (lvl > top) {
         active = false;
       } else {
         uint32_t m = s_mask[lvl][tid];
         int c = __ffs(m) - 1;
         s_mask[lvl][tid] = m & (m - 1);
-        int64_t child = (int64_t)s_node[lvl][tid] * F + c;  // index at level lvl-1
+        int64_t child = (int64_t)cur * F + c;  // index at level lvl-1
         if (lvl - 1 == 0) {
           hbase = child * LEAF;
           int64_t e = min(hbase + LEAF, t.n_valid);
@@ -204,7 +208,7 @@ __global__ __launch_bounds__(kBlock) void k_filter(TreeView t, const float4* q, 
           for (int k = 0; k < F; k++)
             if (k < nc && hit(nd[c0 + k], box)) mm |= 1u << k;
           lvl = lvl - 1;
-          s_node[lvl][tid] = (uint32_t)child;
+          cur = (uint32_t)child;
           s_mask[lvl][tid] = mm;
         }
       }

@@ -102,7 +102,10 @@ __device__ __forceinline__ void ring_segments_f64(const RingRef& r, int64_t s0, 
   }
 }
 
-// FP32-certified ring test over segments [s0, s1) with stride.
+// FP32-certified ring test over segments [s0, s1) with stride.  With
+// Defer, an uncertain segment is only counted (the caller re-refines the
+// candidate in FP64); otherwise it is decided by the FP64 test at once.
+template <bool Defer = false>
 __device__ __forceinline__ void ring_segments_f32(const RingRef& r, const float2* xy32, float E, float pxl, float pyl,
                                                   double px, double py, int64_t s0, int64_t s1, int64_t stride,
                                                   int& on, int& in, uint64_t& tests, uint64_t& fallbacks) {
@@ -114,7 +117,8 @@ __device__ __forceinline__ void ring_segments_f32(const RingRef& r, const float2
     tests++;
     if (rc < 0) {
       fallbacks++;
-      rc = raycast_fallback(r.xy, (int)j, px, py);
+      if constexpr (Defer) rc = kRayOut;
+      else rc = raycast_fallback(r.xy, (int)j, px, py);
     }
     acc_ray(rc, on, in);
   }
@@ -198,25 +202,43 @@ __device__ __forceinline__ bool box_pass(const TreeWalk& w, const float4& b) {
   return w.pyl >= b.y - w.m && w.pyl <= b.w + w.m && w.pxl <= b.z + w.m;
 }
 
+// Ring-walk precision modes (template parameter PREC): 0 = certified FP32,
+// uncertain segments decided by the FP64 test inline; 1 = FP64 throughout;
+// kPrecDeferred = certified FP32, uncertain segments only counted, so that
+// the caller re-refines the candidate with mode 1.
+constexpr int kPrecDeferred = 2;
+
+#ifndef SJ_LEAF_BATCH
+#define SJ_LEAF_BATCH 8
+#endif
+constexpr int kLeafBatch = SJ_LEAF_BATCH;
+
 // A leaf's (up to) 16 segments, walking the vertex run once.
 template <int PREC>
 __device__ __forceinline__ void scan_leaf(TreeWalk& w, int leaf) {
   const int s0 = leaf * kSegLeaf;
   const int s1 = min(s0 + kSegLeaf, w.ns);
-  if constexpr (PREC == 0) {
-    float2 a = w.q32[s0];
-    for (int j = s0; j < s1; j++) {
-      float2 b = w.q32[j + 1];
-      int rc = raycast_f32_certified(a.x, a.y, b.x, b.y, w.pxl, w.pyl, w.E);
-      a = b;
-      if (rc == kRayOut) continue;
-      w.tests++;
-      if (rc < 0) {
-        w.fallbacks++;
-        rc = raycast_fallback(w.xy, j, w.px, w.py);
+  if constexpr (PREC != 1) {
+    // the vertices of kLeafBatch segments are requested before the first
+    // test, so that their latencies overlap (the walk is latency-bound)
+    for (int c0 = s0; c0 < s1; c0 += kLeafBatch) {
+      float2 v[kLeafBatch + 1];
+#pragma unroll
+      for (int k = 0; k <= kLeafBatch; k++) v[k] = c0 + k <= s1 ? w.q32[c0 + k] : make_float2(0.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < kLeafBatch; k++) {
+        if (c0 + k >= s1) break;
+        int rc = raycast_f32_certified(v[k].x, v[k].y, v[k + 1].x, v[k + 1].y, w.pxl, w.pyl, w.E);
+        if (rc == kRayOut) continue;
+        w.tests++;
+        if (rc < 0) {
+          w.fallbacks++;
+          if constexpr (PREC == kPrecDeferred) rc = kRayOut;
+          else rc = raycast_fallback(w.xy, c0 + k, w.px, w.py);
+        }
+        w.on |= rc == kRayOn;
+        w.in ^= rc == kRayIn;
       }
-      w.on |= rc == kRayOn;
-      w.in ^= rc == kRayIn;
     }
   } else {
     double2 a = w.xy[s0];
@@ -233,15 +255,23 @@ __device__ __forceinline__ void scan_leaf(TreeWalk& w, int leaf) {
   }
 }
 
-// Tests up to kSegFanout boxes b[0..n) with all loads issued first.
+#ifndef SJ_BOX_BATCH
+#define SJ_BOX_BATCH 8
+#endif
+constexpr int kBoxBatch = SJ_BOX_BATCH;
+
+// Tests up to kSegFanout boxes b[0..n), kBoxBatch loads in flight at once.
 __device__ __forceinline__ uint32_t children_mask(const TreeWalk& w, const float4* b, int n) {
-  float4 bx[kSegFanout];
-#pragma unroll
-  for (int k = 0; k < kSegFanout; k++) bx[k] = k < n ? b[k] : make_float4(1.f, 1.f, 0.f, 0.f);
   uint32_t mask = 0;
 #pragma unroll
-  for (int k = 0; k < kSegFanout; k++)
-    if (k < n && box_pass(w, bx[k])) mask |= 1u << k;
+  for (int c0 = 0; c0 < kSegFanout; c0 += kBoxBatch) {
+    float4 bx[kBoxBatch];
+#pragma unroll
+    for (int k = 0; k < kBoxBatch; k++) bx[k] = c0 + k < n ? b[c0 + k] : make_float4(1.f, 1.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < kBoxBatch; k++)
+      if (c0 + k < n && box_pass(w, bx[k])) mask |= 1u << (c0 + k);
+  }
   return mask;
 }
 
@@ -331,8 +361,8 @@ __device__ __forceinline__ bool ring_contains_at(const Ctx& c, const double4& rb
     fallbacks += w.fallbacks;
     return w.on ? allow_on_edge : (w.in != 0);
   }
-  if constexpr (PREC == 0) {
-    ring_segments_f32(r, c.R.xy32, E, pxl, pyl, px, py, 0, ns, 1, on, in, tests, fallbacks);
+  if constexpr (PREC != 1) {
+    ring_segments_f32<PREC == kPrecDeferred>(r, c.R.xy32, E, pxl, pyl, px, py, 0, ns, 1, on, in, tests, fallbacks);
   } else {
     ring_segments_f64(r, 0, ns, 1, px, py, on, in, tests);
   }
@@ -584,12 +614,25 @@ __device__ __forceinline__ void emit(WaveOut& wo, bool keep, uint32_t b, uint32_
 // the point-probe case of Intersects, Covers and Contains): one lane per
 // candidate, precision fixed at compile time, nothing but the ring walk
 // inlined.
+//
+// With PRECISION == kPrecDeferred a candidate that met an uncertain FP32
+// test is not decided here: its index goes to `redo` (count in *n_redo)
+// and the FP64 instance refines it from that list (`list`: candidate
+// indices to refine instead of 0..n-1).
+#ifndef SJ_PIP_WAVES
+#define SJ_PIP_WAVES 8
+#endif
+// The walk is latency-bound: more resident waves per SIMD beat the few
+// registers spilled to reach them.
 template <int PRECISION>
-__global__ __launch_bounds__(kBlock) void k_refine_pip(Ctx c, int allow_on_edge, const uint32_t* cb,
-                                                       const uint32_t* cp, uint64_t n, uint32_t* ob, uint32_t* op,
-                                                       uint64_t cap, unsigned long long* counter,
-                                                       unsigned long long* stats, const uint32_t* probe_ids) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SJ_PIP_WAVES, 8))) void k_refine_pip(Ctx c, int allow_on_edge, const uint32_t* cb,
+                                                       const uint32_t* cp, uint64_t n, const uint32_t* list,
+                                                       uint32_t* ob, uint32_t* op, uint64_t cap,
+                                                       unsigned long long* counter, unsigned long long* stats,
+                                                       const uint32_t* probe_ids, uint32_t* redo,
+                                                       unsigned long long* n_redo, const unsigned long long* n_dev) {
   SJ_WAVE_OUT(wo);
+  if (n_dev) n = *n_dev;  // candidate count produced on the device
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   uint64_t tests = 0, fallbacks = 0;
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -597,8 +640,10 @@ __global__ __launch_bounds__(kBlock) void k_refine_pip(Ctx c, int allow_on_edge,
     bool keep = false;
     uint32_t b = 0, p = 0;
     if (i < n) {
+      if (list) i = list[i];
       b = cb[i];
       p = cp[i];
+      const uint64_t fb0 = fallbacks;
       // the probe is a point column: null and empty rows hold NaN
       double2 pt = c.P.xy[p];
       if (pt.x == pt.x || pt.y == pt.y) {
@@ -610,6 +655,12 @@ __global__ __launch_bounds__(kBlock) void k_refine_pip(Ctx c, int allow_on_edge,
           int64_t q1 = c.B.part_off[b + 1];
           for (int64_t q = c.B.part_off[b]; q < q1 && !keep; q++)
             keep = build_poly_contains<PRECISION>(c, q, pt.x, pt.y, allow_on_edge != 0, tests, fallbacks);
+        }
+      }
+      if constexpr (PRECISION == kPrecDeferred) {
+        if (fallbacks != fb0) {
+          keep = false;
+          redo[atomicAdd(n_redo, 1ull)] = (uint32_t)i;
         }
       }
       if (keep && probe_ids) p = probe_ids[p];
@@ -1058,12 +1109,22 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
   if (pip) {
     int allow = predicate == kContains ? 0 : 1;
     if (timer) timer->record_start(s);
-    if (precision == 0)
-      hipLaunchKernelGGL(k_refine_pip<0>, grid_for(n), kBlock, 0, s, c, allow, cb, cp, n, out.b.p, out.p.p,
-                         (uint64_t)out.b.n, out.counter.p, aux.p + 1, probe_ids);
-    else
-      hipLaunchKernelGGL(k_refine_pip<1>, grid_for(n), kBlock, 0, s, c, allow, cb, cp, n, out.b.p, out.p.p,
-                         (uint64_t)out.b.n, out.counter.p, aux.p + 1, probe_ids);
+    if (precision == 0 && n <= 0xffffffffull) {
+      // certified FP32 pass; candidates with an uncertain test (aux[0] of
+      // them) are refined again in FP64 by the second launch
+      DevArray<uint32_t> redo(n, s);
+      hipLaunchKernelGGL(k_refine_pip<kPrecDeferred>, grid_for(n), kBlock, 0, s, c, allow, cb, cp, n, nullptr,
+                         out.b.p, out.p.p, (uint64_t)out.b.n, out.counter.p, aux.p + 1, probe_ids, redo.p, aux.p,
+                         nullptr);
+      hipLaunchKernelGGL(k_refine_pip<1>, 1024, kBlock, 0, s, c, allow, cb, cp, n, redo.p, out.b.p, out.p.p,
+                         (uint64_t)out.b.n, out.counter.p, nullptr, probe_ids, nullptr, nullptr, aux.p);
+    } else if (precision == 0) {
+      hipLaunchKernelGGL(k_refine_pip<0>, grid_for(n), kBlock, 0, s, c, allow, cb, cp, n, nullptr, out.b.p,
+                         out.p.p, (uint64_t)out.b.n, out.counter.p, aux.p + 1, probe_ids, nullptr, nullptr, nullptr);
+    } else {
+      hipLaunchKernelGGL(k_refine_pip<1>, grid_for(n), kBlock, 0, s, c, allow, cb, cp, n, nullptr, out.b.p,
+                         out.p.p, (uint64_t)out.b.n, out.counter.p, aux.p + 1, probe_ids, nullptr, nullptr, nullptr);
+    }
     SJ_CHECK(hipGetLastError());
     if (timer) timer->record_stop(s);
     unsigned long long h[4], cnt = 0;
