@@ -46,7 +46,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the full workload (both sides)")
     ap.add_argument("--precision", type=int, default=0, help="0 = FP32 certified + FP64 fallback, 1 = all FP64")
-    ap.add_argument("--cpu-strip", type=float, default=0.004, help="CPU sample: strip width / extent")
+    ap.add_argument("--cpu-strip", type=float, default=0.12, help="CPU sample: strip width / extent")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the all-FP64 comparison step")
     ap.add_argument("--workers", type=int, default=16)
@@ -212,6 +212,7 @@ def main():
     }
     if plain and args.precision == 0:
         out["speedup_vs_plain_fp64"] = plain["ms_per_step"] / ms_per_step
+        out["refine_speedup_vs_plain_fp64"] = plain["stage_ms"][3] / stage[3]
     print(json.dumps(out), flush=True)
     D.release_shared(wl["keep"])
 
