@@ -226,7 +226,9 @@ struct SjhipJoinResult {
   uint32_t* probe_indices; /* device memory */
   uint64_t size;
   uint64_t candidates;      /* filter output count */
-  uint64_t fallback_tests;  /* FP32 segment tests resolved in FP64 */
+  uint64_t fallback_tests;  /* FP32 segment tests too close to call: decided
+                               in FP64 (their candidate re-refined in FP64
+                               by the point-in-polygon kernel) */
   uint64_t segment_tests;   /* segment tests performed */
   /* device time of the stages (ms, HIP events on the join's stream) when
    * options.profile is set: [0] probe decode + rects, [1] Morton order of
