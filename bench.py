@@ -52,6 +52,9 @@ def main():
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--refine-order", type=int, default=1, help="1 = group candidates by polygon before refine")
+    ap.add_argument("--serve-requests", type=int, default=96,
+                    help="requests in the serving measurement (0 = skip it)")
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only time the plain copy kernel (HBM calibration for the counter summaries)")
     args = ap.parse_args()
@@ -86,6 +89,15 @@ def main():
         strip = dict(brows=brows, prows=prows, bcol=D.take_rows(build, brows), pcol=D.take_rows(probe, prows),
                      x0=x0, w=w)
         log(f"cpu strip: {len(brows)} polygons, {len(prows)} points")
+
+    # serving requests (generated by forked workers, so before the GPU is used)
+    serve = None
+    if args.serve_requests > 0:
+        sizes, _, _ = D.serving_requests(args.serve_requests, seed=args.seed + 4)
+        scol, _, skeep = D.points_parallel(args.seed + 3000, int(sizes.sum()), wl["extent"],
+                                           centers=wl["cluster_centers"], workers=args.workers)
+        serve = dict(sizes=sizes, col=scol, keep=skeep)
+        log(f"serving: {len(sizes)} requests, {int(sizes.sum())} points")
 
     from sjhip import gpuspatial as G
 
@@ -174,6 +186,11 @@ def main():
     roof_refine["frac"] = roof_refine["achieved"] / HBM_PEAK_GBS
     dominant = roof_refine if ref_ms >= filt_ms else roof_filter
 
+    serving = None
+    if serve is not None:
+        serving = serving_measurement(G, ix, rf, serve, args)
+        D.release_shared(serve["keep"])
+
     cpu = None
     if strip is not None:
         cpu = cpu_baseline(strip, gb_pairs, gp_pairs, args)
@@ -209,12 +226,86 @@ def main():
         "fallback_rate": fb / max(1, seg_tests),
         "build_ms": build_ms,
         "plain_fp64": plain,
+        "serving": serving,
     }
     if plain and args.precision == 0:
         out["speedup_vs_plain_fp64"] = plain["ms_per_step"] / ms_per_step
         out["refine_speedup_vs_plain_fp64"] = plain["stage_ms"][3] / stage[3]
     print(json.dumps(out), flush=True)
     D.release_shared(wl["keep"])
+
+
+SERVE_MIX = (("intersects", 0.0), ("dwithin", 0.05))
+
+
+def serving_measurement(G, ix, rf, serve, args):
+    """Latency of probe requests (log-uniform 1k..10M clustered points,
+    ST_Intersects and ST_DWithin(0.05) alternating) arriving as a Poisson
+    process against the resident polygon index.  Each arrival rate is run
+    twice: through the request loop with `slots` concurrent requests, and
+    one request at a time (a single slot).  Latency = finish - arrival."""
+    import threading
+
+    sizes = serve["sizes"]
+    dev = G.DeviceWkb.from_host(serve["col"])
+    starts = np.concatenate([[0], np.cumsum(sizes)])
+    reqs = [(dev.slice(starts[k], starts[k + 1]),) + SERVE_MIX[k % len(SERVE_MIX)] for k in range(len(sizes))]
+    # warm-up (code objects, memory pool) and the one-at-a-time service time
+    for q, pred, d in reqs[:4]:
+        G.join_device(ix, rf, q, pred, distance=d, sort_output=False).release()
+    t = time.time()
+    pairs = 0
+    for q, pred, d in reqs:
+        r = G.join_device(ix, rf, q, pred, distance=d, sort_output=False)
+        pairs += r.size
+        r.release()
+    service_s = time.time() - t
+    capacity = len(reqs) / service_s
+    log(f"serving: one-at-a-time service {service_s * 1000:.1f} ms for {len(reqs)} requests ({pairs} pairs)")
+
+    def run(rate, slots, seed):
+        rng = np.random.default_rng(seed)
+        gaps = rng.exponential(1.0 / rate, size=len(reqs))
+        sv = G.GpuSpatialServer(ix, rf, slots=slots)
+        ids = []
+        t0 = time.perf_counter()
+        due = 0.0
+        for (q, pred, d), g in zip(reqs, gaps):
+            due += g
+            wait = due - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            ids.append(sv.submit(q, pred, distance=d, sort_output=False))
+        lat, first, last, npairs = [], None, None, 0
+        for rid in ids:
+            res, tm = sv.wait(rid)
+            npairs += res.size
+            res.release()
+            lat.append(tm["done_ms"] - tm["submit_ms"])
+            first = tm["submit_ms"] if first is None else min(first, tm["submit_ms"])
+            last = tm["done_ms"] if last is None else max(last, tm["done_ms"])
+        sv.close()
+        span = (last - first) / 1000
+        lat = np.array(lat)
+        return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "mean_ms": float(lat.mean()), "throughput_rps": len(reqs) / span,
+                "points_per_s": float(sizes.sum()) / span, "pairs_per_s": npairs / span}
+
+    rates = []
+    for rel in (0.5, 0.9, 1.5):
+        rate = rel * capacity
+        row = {"rel_to_one_at_a_time_capacity": rel, "rate_rps": rate,
+               "batched": run(rate, args.serve_slots, 7 + int(rel * 10)),
+               "one_at_a_time": run(rate, 1, 7 + int(rel * 10))}
+        log(f"serving rate {rate:.1f}/s: batched p50 {row['batched']['p50_ms']:.2f} p99 {row['batched']['p99_ms']:.2f} ms;"
+            f" one-at-a-time p50 {row['one_at_a_time']['p50_ms']:.2f} p99 {row['one_at_a_time']['p99_ms']:.2f} ms")
+        rates.append(row)
+    del reqs, dev
+    return {"requests": len(sizes), "points": int(sizes.sum()), "sizes": "log-uniform 1k..10M clustered points",
+            "mix": "ST_Intersects / ST_DWithin(0.05) alternating", "slots": args.serve_slots,
+            "memory_budget": "90% of free HBM", "one_at_a_time_service_ms": service_s * 1000,
+            "one_at_a_time_capacity_rps": capacity, "rates": rates,
+            "p50_ms": rates[1]["batched"]["p50_ms"], "p99_ms": rates[1]["batched"]["p99_ms"]}
 
 
 def read_traffic(path):

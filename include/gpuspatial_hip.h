@@ -263,6 +263,49 @@ int SjhipJoinDevice(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefine
                     const struct ArrowDeviceArray* probe_array,
                     const struct SjhipJoinOptions* options, struct SjhipJoinResult* out);
 
+/* Request loop over a resident index and refiner (the serving path of the
+ * GPU operator: rust/sedona-spatial-join-gpu/src/index/gpu_spatial_index.rs:
+ * 163-229 called for many probe batches at once).  Requests are joined
+ * with SjhipJoinDevice semantics; up to `slots` of them run concurrently on
+ * the device, each on its own stream, admitted in arrival order when a
+ * slot is free and the request's device-memory estimate fits
+ * `memory_budget` next to the running ones (a request larger than the
+ * whole budget runs alone).  No counterpart in the reference. */
+struct SjhipServerConfig {
+  uint32_t slots;          /* concurrent requests (>= 1) */
+  uint64_t memory_budget;  /* bytes; 0 = 90 % of the device memory free at creation */
+};
+
+/* Timing of one request, ms on a steady clock since the server's
+ * creation: submitted, admitted to a slot, finished. */
+struct SjhipRequestTiming {
+  double submit_ms;
+  double admit_ms;
+  double done_ms;
+  uint64_t estimate_bytes; /* the admission estimate */
+};
+
+int SjhipServerCreate(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner* refiner,
+                      const struct SjhipServerConfig* config, void** out_server);
+
+/* Queues one request.  The probe arrays are borrowed until the request's
+ * SjhipServerWait has returned; options->stream is ignored. */
+int SjhipServerSubmit(void* server, const struct ArrowSchema* probe_schema,
+                      const struct ArrowDeviceArray* probe_array, const struct SjhipJoinOptions* options,
+                      uint64_t* out_request_id);
+
+/* Waits up to timeout_ms (-1: no limit, 0: poll) for a request.  Returns 0
+ * with the result moved into *out (release it as for SjhipJoinDevice),
+ * EAGAIN while it is queued or running, or the request's own error code
+ * (message from SjhipLastError).  A request is forgotten once 0 or its
+ * error has been returned. */
+int SjhipServerWait(void* server, uint64_t request_id, int64_t timeout_ms, struct SjhipJoinResult* out,
+                    struct SjhipRequestTiming* timing);
+
+/* Finishes the queued and running requests, then frees the server;
+ * results never waited for are released. */
+int SjhipServerDestroy(void* server);
+
 /* Copies a device join result to host arrays of at least result->size
  * entries. */
 int SjhipJoinResultCopyToHost(const struct SjhipJoinResult* result, uint32_t* build_indices,

@@ -10,9 +10,14 @@
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
+#include <thread>
 
 #include "../../include/gpuspatial_hip.h"
 #include "sjhip.h"
@@ -434,7 +439,7 @@ void refine_host(RefinerImpl* rf, const ArrowSchema* probe_schema, const ArrowAr
   PairBuf out;
   uint64_t kept;
   if (pred == 100)
-    kept = refine_distance(rf->build, probe, distance, cb.p, cp.p, n, out, s);
+    kept = refine_distance(rf->build, rf->ring_index, probe, distance, cb.p, cp.p, n, out, s);
   else
     kept = refine_relation(rf->build, rf->ring_index, probe, pred, cb.p, cp.p, n, 0, out, nullptr, s);
   sort_pairs(out.b.p, out.p.p, kept, s);
@@ -582,7 +587,8 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
   RefineStats st;
   uint64_t kept;
   if (dwithin)
-    kept = refine_distance(rf->build, probe, o.distance, cand.b.p, cand.p.p, nc, jr->pairs, s, ev.on ? &rt : nullptr);
+    kept = refine_distance(rf->build, rf->ring_index, probe, o.distance, cand.b.p, cand.p.p, nc, jr->pairs, s,
+                           ev.on ? &rt : nullptr);
   else
     kept = refine_relation(rf->build, rf->ring_index, rp, o.predicate, cand.b.p, cand.p.p, nc, o.precision,
                            jr->pairs, &st, s, ev.on ? &rt : nullptr, ranked ? ord : nullptr);
@@ -613,6 +619,106 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
   out->release = jr_release;
   out->private_data = jr.release();
 }
+
+
+// ---------------------------------------------------------------------------
+// Request loop over a resident index (continuous batching)
+//
+// `slots` worker threads, each with its own stream, take requests in
+// arrival order.  A request is admitted when a worker is free and its
+// device-memory estimate fits the budget next to the requests already
+// running (a request larger than the whole budget runs alone).  Admitted
+// requests run their join concurrently on the device; a finished request
+// frees its slot and its share of the budget for the next.
+
+struct ServerReq {
+  uint64_t id = 0;
+  WkbInput in;
+  SjhipJoinOptions opt{};
+  uint64_t estimate = 0;
+  int status = 0;  // 0 queued, 1 running, 2 done, 3 failed
+  int rc = 0;
+  std::string error;
+  SjhipJoinResult res{};
+  double t_submit = 0, t_admit = 0, t_done = 0;
+};
+
+struct Server {
+  IndexImpl* ix = nullptr;
+  RefinerImpl* rf = nullptr;
+  uint64_t budget = 0;
+  uint64_t reserved = 0;
+  int running = 0;
+  bool stop = false;
+  uint64_t next_id = 1;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<ServerReq*> queue;
+  std::map<uint64_t, std::unique_ptr<ServerReq>> reqs;
+  std::vector<std::thread> workers;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+
+  double now_ms() const {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+
+  // Device bytes a request holds at its peak: the decoded probe column,
+  // rects, Morton keys and order, the ordered copy of points, and per
+  // candidate the pair arrays, their regrouping copies and keys, the
+  // deferred list and the kept pairs.  Candidates per probe come from the
+  // index's last probe (8 before any).
+  uint64_t estimate(const WkbInput& in, const SjhipJoinOptions& o) const {
+    uint64_t n = (uint64_t)in.length;
+    uint64_t per16 = ix->tree.cand_per_query_x16.load(std::memory_order_relaxed);
+    double cpq = per16 ? per16 / 16.0 : 8.0;
+    if (o.predicate == 100) cpq *= 2.0;  // expanded probe rects
+    uint64_t cand = (uint64_t)(cpq * (double)n * 1.125) + 1024;
+    uint64_t wkb = in.on_device ? 0 : (uint64_t)(in.large ? 8 : 4) * (n + 1) + 32 * n;
+    return wkb + 96 * n + 44 * cand + (1u << 20);
+  }
+
+  void work() {
+    use_device(rf->rt);
+    hipStream_t s;
+    SJ_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    for (;;) {
+      ServerReq* r = nullptr;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] {
+          if (queue.empty()) return stop;
+          ServerReq* h = queue.front();
+          return running == 0 || reserved + h->estimate <= budget;
+        });
+        if (queue.empty()) break;
+        r = queue.front();
+        queue.pop_front();
+        reserved += r->estimate;
+        running++;
+        r->status = 1;
+        r->t_admit = now_ms();
+      }
+      SjhipJoinOptions o = r->opt;
+      o.stream = s;
+      SjhipJoinResult res{};
+      std::string err;
+      int rc = guarded(&err, [&] { JoinEngine::run(ix, rf, r->in, o, &res); });
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        reserved -= r->estimate;
+        running--;
+        r->rc = rc;
+        r->error = err;
+        r->res = res;
+        r->status = rc ? 3 : 2;
+        r->t_done = now_ms();
+      }
+      cv.notify_all();
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+  }
+};
 
 }  // namespace sjhip
 
@@ -732,6 +838,115 @@ int SjhipJoinDevice(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefine
       throw Error("probe array must be in ROCm device memory or CPU memory");
     WkbInput in = wkb_input_from_arrow(probe_schema, &probe_array->array, dev);
     JoinEngine::run((IndexImpl*)index->private_data, (RefinerImpl*)refiner->private_data, in, *options, out);
+  });
+}
+
+int SjhipServerCreate(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner* refiner,
+                      const struct SjhipServerConfig* config, void** out_server) {
+  return guarded(nullptr, [&] {
+    if (!index || !index->private_data || !refiner || !refiner->private_data) throw Error("null index or refiner");
+    if (!config || !out_server) throw Error("null argument");
+    if (config->slots < 1 || config->slots > 64) throw Error("slots must be in 1..64");
+    *out_server = nullptr;
+    std::unique_ptr<Server> sv(new Server());
+    sv->ix = (IndexImpl*)index->private_data;
+    sv->rf = (RefinerImpl*)refiner->private_data;
+    if (!sv->ix->built || !sv->rf->built) throw Error("index and refiner must be built");
+    use_device(sv->rf->rt);
+    sv->budget = config->memory_budget;
+    if (sv->budget == 0) {
+      size_t free_b = 0, total_b = 0;
+      SJ_CHECK(hipMemGetInfo(&free_b, &total_b));
+      sv->budget = (uint64_t)(0.9 * (double)free_b);
+    }
+    Server* p = sv.get();
+    for (uint32_t k = 0; k < config->slots; k++)
+      sv->workers.emplace_back([p] {
+        try {
+          p->work();
+        } catch (...) {
+        }
+      });
+    *out_server = sv.release();
+  });
+}
+
+int SjhipServerSubmit(void* server, const struct ArrowSchema* probe_schema, const struct ArrowDeviceArray* probe_array,
+                      const struct SjhipJoinOptions* options, uint64_t* out_request_id) {
+  return guarded(nullptr, [&] {
+    Server* sv = (Server*)server;
+    if (!sv || !probe_array || !options || !out_request_id) throw Error("null argument");
+    bool dev = probe_array->device_type == ARROW_DEVICE_ROCM;
+    if (!dev && probe_array->device_type != ARROW_DEVICE_CPU)
+      throw Error("probe array must be in ROCm device memory or CPU memory");
+    std::unique_ptr<ServerReq> r(new ServerReq());
+    r->in = wkb_input_from_arrow(probe_schema, &probe_array->array, dev);
+    r->opt = *options;
+    r->estimate = sv->estimate(r->in, r->opt);
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      if (sv->stop) throw Error("server is shutting down");
+      r->id = sv->next_id++;
+      r->t_submit = sv->now_ms();
+      *out_request_id = r->id;
+      sv->queue.push_back(r.get());
+      sv->reqs[r->id] = std::move(r);
+    }
+    sv->cv.notify_all();
+  });
+}
+
+int SjhipServerWait(void* server, uint64_t request_id, int64_t timeout_ms, struct SjhipJoinResult* out,
+                    struct SjhipRequestTiming* timing) {
+  int req_rc = 0;
+  std::string req_err;
+  int rc = guarded(nullptr, [&] {
+    Server* sv = (Server*)server;
+    if (!sv || !out) throw Error("null argument");
+    std::memset(out, 0, sizeof(*out));
+    std::unique_lock<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(request_id);
+    if (it == sv->reqs.end()) throw Error("unknown request id " + std::to_string(request_id));
+    ServerReq* r = it->second.get();
+    auto done = [&] { return r->status >= 2; };
+    if (timeout_ms < 0) sv->cv.wait(lk, done);
+    else if (timeout_ms > 0) sv->cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), done);
+    if (!done()) {
+      req_rc = EAGAIN;
+      return;
+    }
+    if (timing) {
+      timing->submit_ms = r->t_submit;
+      timing->admit_ms = r->t_admit;
+      timing->done_ms = r->t_done;
+      timing->estimate_bytes = r->estimate;
+    }
+    if (r->status == 2) {
+      *out = r->res;
+    } else {
+      req_rc = r->rc ? r->rc : EINVAL;
+      req_err = r->error;
+    }
+    sv->reqs.erase(it);
+  });
+  if (rc) return rc;
+  if (req_rc && req_rc != EAGAIN) t_last_error = req_err;
+  return req_rc;
+}
+
+int SjhipServerDestroy(void* server) {
+  return guarded(nullptr, [&] {
+    Server* sv = (Server*)server;
+    if (!sv) return;
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      sv->stop = true;
+    }
+    sv->cv.notify_all();
+    for (auto& t : sv->workers) t.join();
+    for (auto& kv : sv->reqs)
+      if (kv.second->status == 2 && kv.second->res.release) kv.second->res.release(&kv.second->res);
+    delete sv;
   });
 }
 

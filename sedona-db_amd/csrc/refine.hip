@@ -849,9 +849,151 @@ __device__ double distance_to_point(const GeomView& g, int64_t row, double px, d
   return m;
 }
 
-__global__ __launch_bounds__(kBlock) void k_refine_distance(GeomView B, GeomView P, double dist, const uint32_t* cb,
+// Depth-first walk over a ring's segment tree (the layout walk_tree
+// uses) visiting the leaves whose path of boxes passes `pass`; stops and
+// returns true as soon as `leaf(leaf_index)` does.
+template <class Pass, class Leaf>
+__device__ __forceinline__ bool walk_any(const float4* box, const TreeShape& shape, Pass pass, Leaf leaf) {
+  const int top = shape.levels - 1;
+  const int ntop = level_cnt(shape, top);
+  const float4* btop = box + level_off(shape, top);
+  for (int c0 = 0; c0 < ntop; c0 += kSegFanout) {
+    uint32_t m0 = 0;
+    const int n0 = min(kSegFanout, ntop - c0);
+    for (int k = 0; k < n0; k++)
+      if (pass(btop[c0 + k])) m0 |= 1u << k;
+    uint64_t masks = (uint64_t)m0 << (8 * top);
+    int lvl = top, base = c0;
+    while (masks) {
+      uint32_t m = (uint32_t)(masks >> (8 * lvl)) & 0xffu;
+      if (m == 0) {
+        lvl++;
+        base = (base >> 3) & ~(kSegFanout - 1);
+        continue;
+      }
+      int k = __ffs(m) - 1;
+      masks &= ~(1ull << (8 * lvl + k));
+      int node = base + k;
+      if (lvl == 0) {
+        if (leaf(node)) return true;
+        continue;
+      }
+      int cb = node * kSegFanout;
+      int n = min(kSegFanout, level_cnt(shape, lvl - 1) - cb);
+      const float4* bc = box + level_off(shape, lvl - 1) + cb;
+      uint32_t cm = 0;
+      for (int q = 0; q < n; q++)
+        if (pass(bc[q])) cm |= 1u << q;
+      if (cm) {
+        lvl--;
+        base = cb;
+        masks |= (uint64_t)cm << (8 * lvl);
+      }
+    }
+  }
+  return false;
+}
+
+// Segment j of a vertex run is a term <= d of the reference distance (on
+// the segment: 0, for lines; else distance_coord_to_line).
+__device__ __forceinline__ bool segment_within(const double2* xy, int64_t j, double px, double py, double d,
+                                               bool check_on) {
+  double2 a = xy[j], b = xy[j + 1];
+  if (check_on && line_intersects_coord(a.x, a.y, b.x, b.y, px, py)) return true;
+  return coord_to_line(px, py, a.x, a.y, b.x, b.y) <= d;
+}
+
+// Whether some segment of a build ring (or line) is within d of the point.
+// Segments whose FP32 tree boxes lie farther than d from the point, plus
+// the boxes' and the point's FP32 error bounds (4E) and the rounding of d,
+// cannot be within d and are skipped.
+__device__ bool ring_any_within(const Ctx& c, int64_t ring, double px, double py, double d, bool check_on) {
+  const int64_t v0 = c.B.pt_off[ring], npts = c.B.pt_off[ring + 1] - v0;
+  const double2* xy = c.B.xy + v0;
+  const int64_t ns = npts - 1;
+  const int64_t toff = c.R.seg_tree_off ? c.R.seg_tree_off[ring] : -1;
+  const float E = c.R.err[ring];
+  if (toff < 0 || ns < 1 || !(E < INFINITY)) {
+    for (int64_t j = 0; j < ns; j++)
+      if (segment_within(xy, j, px, py, d, check_on)) return true;
+    return false;
+  }
+  const double2 o = c.R.origin[ring];
+  const float D = (float)(d * 1.0001) + 4.0f * E;
+  const float pxl = (float)(px - o.x), pyl = (float)(py - o.y);
+  const float qx0 = pxl - D, qx1 = pxl + D, qy0 = pyl - D, qy1 = pyl + D;
+  const int nsi = (int)ns;
+  auto pass = [&](const float4& b) { return b.x <= qx1 && b.z >= qx0 && b.y <= qy1 && b.w >= qy0; };
+  auto leaf = [&](int lf) {
+    const int s0 = lf * kSegLeaf, s1 = min(s0 + kSegLeaf, nsi);
+    for (int j = s0; j < s1; j++)
+      if (segment_within(xy, j, px, py, d, check_on)) return true;
+    return false;
+  };
+  return walk_any(c.R.seg_tree + toff, seg_tree_shape(nsi), pass, leaf);
+}
+
+// Position of the point w.r.t. a closed build ring, exactly: 2 boundary,
+// 1 inside, 0 outside (geo's coord_pos_relative_to_ring).  The certified
+// FP32 crossing count over the segment tree decides it when no test is
+// uncertain (then no vertex or edge lies within its margin of the point's
+// ray, so every crossing-number rule counts the same crossings); otherwise
+// the exact scan does.
+__device__ int build_ring_position(const Ctx& c, int64_t ring, double px, double py) {
+  const int64_t v0 = c.B.pt_off[ring], npts = c.B.pt_off[ring + 1] - v0;
+  const double4 rb = c.R.ring_rect[ring];
+  if (!rect_covers(rb, px, py)) return 0;
+  if (npts >= 4) {
+    uint64_t tests = 0, fb = 0;
+    const int64_t toff = c.R.seg_tree_off ? c.R.seg_tree_off[ring] : -1;
+    bool in = ring_contains_at<kPrecDeferred>(c, rb, v0, npts, c.R.origin[ring], c.R.err[ring], toff, px, py, true,
+                                              tests, fb);
+    if (fb == 0) return in ? 1 : 0;
+  }
+  return exact_ring_position(ring_at(c.B, ring), px, py);
+}
+
+// The reference decision distance(build row, point) <= d
+// (rust/sedona-spatial-join/src/refine/geo.rs:312-335) without the
+// distance itself: the distance is a minimum over parts and segments (0
+// inside a polygon and on a line; NaN terms ignored), and such a minimum is
+// <= d exactly when one of its terms is.
+__device__ bool within_distance(const Ctx& c, int64_t row, double px, double py, double d) {
+  if (!(d >= 0)) return false;  // every distance is >= 0 or NaN
+  const GeomView& g = c.B;
+  const int t = g.type[row];
+  if (g.points_only || t == kPoint || t == kMultiPoint || c.R.ring_rect == nullptr)
+    return distance_to_point(g, row, px, py) <= d;
+  for (int64_t q = g.part_off[row]; q < g.part_off[row + 1]; q++) {
+    const int64_t r0 = g.ring_off[q], r1 = g.ring_off[q + 1];
+    if (t == kLine || t == kMultiLine) {
+      if (g.pt_off[r0 + 1] - g.pt_off[r0] < 2) return true;  // point_to_run: 0
+      if (ring_any_within(c, r0, px, py, d, true)) return true;
+      continue;
+    }
+    if (r0 == r1) return true;  // point_to_polygon: 0
+    int st = build_ring_position(c, r0, px, py);
+    if (st == 2) return true;
+    if (st == 1) {
+      bool in_hole = false;
+      for (int64_t h = r0 + 1; h < r1; h++) {
+        int hs = build_ring_position(c, h, px, py);
+        if (hs == 2) return true;
+        if (hs == 1) in_hole = true;
+      }
+      if (!in_hole) return true;
+    }
+    for (int64_t k = r0; k < r1; k++)
+      if (ring_any_within(c, k, px, py, d, false)) return true;
+  }
+  return false;
+}
+
+__global__ __launch_bounds__(kBlock) void k_refine_distance(Ctx c, double dist, const uint32_t* cb,
                                                             const uint32_t* cp, uint64_t n, uint32_t* ob,
                                                             uint32_t* op, uint64_t cap, unsigned long long* counter) {
+  const GeomView& B = c.B;
+  const GeomView& P = c.P;
   SJ_WAVE_OUT(wo);
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -863,7 +1005,7 @@ __global__ __launch_bounds__(kBlock) void k_refine_distance(GeomView B, GeomView
       p = cp[i];
       double px, py;
       bool bok = B.type[b] != kNull && (B.points_only || B.part_off[b] != B.part_off[b + 1]);
-      if (bok && point_of(P, p, px, py)) keep = distance_to_point(B, b, px, py) <= dist;
+      if (bok && point_of(P, p, px, py)) keep = within_distance(c, b, px, py, dist);
     }
     emit(wo, keep, b, p, ob, op, cap, counter);
   }
@@ -1168,7 +1310,8 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
   return cnt;
 }
 
-uint64_t refine_distance(const GeomColumn& build, const GeomColumn& probe, double d, const uint32_t* cb,
+uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
+                         const uint32_t* cb,
                          const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s, KernelTimer* timer) {
   out.counter.reserve(1, s);
   out.b.reserve(n > 0 ? n : 1, s);
@@ -1178,8 +1321,12 @@ uint64_t refine_distance(const GeomColumn& build, const GeomColumn& probe, doubl
     throw Error("distance refine: probe rows must be points");
   SJ_CHECK(hipMemsetAsync(out.counter.p, 0, 8, s));
   if (timer) timer->record_start(s);
-  hipLaunchKernelGGL(k_refine_distance, grid_for(n), kBlock, 0, s, build.view(), probe.view(), d, cb, cp, n, out.b.p,
-                     out.p.p, (uint64_t)out.b.n, out.counter.p);
+  Ctx c;
+  c.B = build.view();
+  c.R = ri.view();
+  c.P = probe.view();
+  hipLaunchKernelGGL(k_refine_distance, grid_for(n), kBlock, 0, s, c, d, cb, cp, n, out.b.p, out.p.p,
+                     (uint64_t)out.b.n, out.counter.p);
   SJ_CHECK(hipGetLastError());
   if (timer) timer->record_stop(s);
   unsigned long long cnt = 0;

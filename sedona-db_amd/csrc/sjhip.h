@@ -283,7 +283,8 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
                          const uint32_t* probe_ids = nullptr);
 
 // Distance refine (build.distance(probe) <= d); probe must be points.
-uint64_t refine_distance(const GeomColumn& build, const GeomColumn& probe, double d, const uint32_t* cb,
+uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
+                         const uint32_t* cb,
                          const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s, KernelTimer* timer = nullptr);
 
 // Regroups candidate pairs by build row and, for point probes, by the

@@ -173,6 +173,15 @@ class SjhipJoinOptions(ctypes.Structure):
     ]
 
 
+class SjhipServerConfig(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_uint32), ("memory_budget", ctypes.c_uint64)]
+
+
+class SjhipRequestTiming(ctypes.Structure):
+    _fields_ = [("submit_ms", ctypes.c_double), ("admit_ms", ctypes.c_double), ("done_ms", ctypes.c_double),
+                ("estimate_bytes", ctypes.c_uint64)]
+
+
 # Every symbol the public headers declare (checked by the CPU test suite).
 EXPORTED = [
     "GpuSpatialRuntimeCreate", "GpuSpatialIndexFloat2DCreate", "GpuSpatialRefinerCreate",
@@ -180,6 +189,7 @@ EXPORTED = [
     "SjhipJoinResultCopyToHost", "SjhipIndexBuildFromRefiner", "SjhipRefinerPushBuildDevice", "SjhipLastError",
     "SjhipComputeRects", "SjhipDeviceAlloc", "SjhipDeviceFree", "SjhipCopyToDevice", "SjhipCopyToHost",
     "SjhipSetDevice", "SjhipCopyKernel",
+    "SjhipServerCreate", "SjhipServerSubmit", "SjhipServerWait", "SjhipServerDestroy",
     "SjhipHostRaycast", "SjhipHostRaycastF32", "SjhipHostCollinear", "SjhipHostSegmentIntersects",
     "SjhipHostOrient2d", "SjhipHostHypot", "SjhipHostCoordToLine", "SjhipHostRaycastBatch", "SjhipHostHypotBatch",
 ]
@@ -207,6 +217,12 @@ def lib():
         L.SjhipJoinDevice.argtypes = [_IX, _RF, _SCH, ctypes.POINTER(ArrowDeviceArray),
                                       ctypes.POINTER(SjhipJoinOptions), ctypes.POINTER(SjhipJoinResult)]
         L.SjhipJoinResultCopyToHost.argtypes = [ctypes.POINTER(SjhipJoinResult), c_u32p, c_u32p]
+        L.SjhipServerCreate.argtypes = [_IX, _RF, ctypes.POINTER(SjhipServerConfig), ctypes.POINTER(ctypes.c_void_p)]
+        L.SjhipServerSubmit.argtypes = [ctypes.c_void_p, _SCH, ctypes.POINTER(ArrowDeviceArray),
+                                        ctypes.POINTER(SjhipJoinOptions), ctypes.POINTER(ctypes.c_uint64)]
+        L.SjhipServerWait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(SjhipJoinResult),
+                                      ctypes.POINTER(SjhipRequestTiming)]
+        L.SjhipServerDestroy.argtypes = [ctypes.c_void_p]
         L.SjhipIndexBuildFromRefiner.argtypes = [_IX, _RF]
         L.SjhipRefinerPushBuildDevice.argtypes = [_RF, _SCH, ctypes.POINTER(ArrowDeviceArray)]
         L.SjhipLastError.restype = ctypes.c_char_p

@@ -423,7 +423,7 @@ def workload_pip_zipf_parallel(n_points=100_000_000, n_polys=1_000_000, seed=2, 
     build, k1 = star_polygons_parallel(seed + 1000, counts, centers, 0.0, workers=workers)
     probe, xy, k2 = points_parallel(seed + 2000, n_points, extent, centers=ccent, workers=workers)
     return dict(build=build, probe=probe, centers=centers, counts=counts, probe_xy=xy, extent=extent,
-                keep=k1 + k2)
+                cluster_centers=ccent, keep=k1 + k2)
 
 
 def release_shared(keep):

@@ -7,6 +7,7 @@ per device, ``GpuSpatialIndex`` (FP32 rect filter) and ``GpuSpatialRefiner``
 (exact refine), errors raised with the library's message.
 """
 import ctypes
+import errno
 import enum
 import threading
 
@@ -158,6 +159,15 @@ class DeviceWkb:
         a.release = _noop_release
         self.dev.device_id = device_id
         self.dev.device_type = L.ARROW_DEVICE_ROCM
+
+    def slice(self, start, stop):
+        """Rows [start, stop) as a DeviceWkb over the same device buffers
+        (Arrow offsets need not start at 0)."""
+        start, stop = int(start), int(stop)
+        if not (0 <= start <= stop <= self.dev.array.length):
+            raise ValueError("slice out of range")
+        return DeviceWkb(self.bufs[1] + 8 * start, self.bufs[2], stop - start, device_id=self.dev.device_id,
+                         keepalive=self)
 
     @classmethod
     def from_host(cls, col, device_id=0):
@@ -460,16 +470,20 @@ class JoinResult:
             pass
 
 
-def join_device(index, refiner, probe, predicate, distance=0.0, precision=0, sort_output=True, sort_probes=True,
-                profile=False, refine_order=1, stream=None):
-    """One probe batch through the fused device join.  `probe` is a
-    DeviceWkb (device memory) or any Arrow-exportable host column."""
+def _join_options(predicate, distance, precision, sort_output, sort_probes, profile, refine_order, stream):
     if isinstance(predicate, str):
         pred = 100 if predicate == "dwithin" else int(PREDICATE_BY_NAME[predicate])
     else:
         pred = int(predicate)
-    opts = L.SjhipJoinOptions(pred, float(distance), int(precision), 1 if sort_output else 0, 1 if sort_probes else 0,
+    return L.SjhipJoinOptions(pred, float(distance), int(precision), 1 if sort_output else 0, 1 if sort_probes else 0,
                               1 if profile else 0, int(refine_order), stream)
+
+
+def join_device(index, refiner, probe, predicate, distance=0.0, precision=0, sort_output=True, sort_probes=True,
+                profile=False, refine_order=1, stream=None):
+    """One probe batch through the fused device join.  `probe` is a
+    DeviceWkb (device memory) or any Arrow-exportable host column."""
+    opts = _join_options(predicate, distance, precision, sort_output, sort_probes, profile, refine_order, stream)
     res = L.SjhipJoinResult()
     if isinstance(probe, DeviceWkb):
         sch, dev = probe.schema, probe.dev
@@ -483,3 +497,63 @@ def join_device(index, refiner, probe, predicate, distance=0.0, precision=0, sor
     if rc:
         raise GpuSpatialError(L.lib().SjhipLastError().decode())
     return JoinResult(res)
+
+
+class GpuSpatialServer:
+    """Request loop over a built index + refiner: up to `slots` probe
+    batches joined concurrently on the device, admitted in arrival order by
+    free slots and a device-memory budget (SjhipServer* entry points)."""
+
+    def __init__(self, index, refiner, slots=4, memory_budget=0):
+        self._index, self._refiner = index, refiner
+        cfg = L.SjhipServerConfig(int(slots), int(memory_budget))
+        h = ctypes.c_void_p()
+        rc = L.lib().SjhipServerCreate(ctypes.byref(index.ix), ctypes.byref(refiner.rf), ctypes.byref(cfg),
+                                       ctypes.byref(h))
+        if rc:
+            raise GpuSpatialError(L.lib().SjhipLastError().decode())
+        self.h = h.value
+        self._keep = {}
+
+    def submit(self, probe, predicate, distance=0.0, precision=0, sort_output=False, sort_probes=True,
+               refine_order=1):
+        opts = _join_options(predicate, distance, precision, sort_output, sort_probes, False, refine_order, None)
+        if isinstance(probe, DeviceWkb):
+            keep, sch, dev = probe, probe.schema, probe.dev
+        else:
+            exp = ExportedArrow(probe)
+            keep, sch, dev = exp, exp.schema, host_device_array(exp)
+            keep = (exp, dev)
+        rid = ctypes.c_uint64()
+        rc = L.lib().SjhipServerSubmit(self.h, ctypes.byref(sch), ctypes.byref(dev), ctypes.byref(opts),
+                                       ctypes.byref(rid))
+        if rc:
+            raise GpuSpatialError(L.lib().SjhipLastError().decode())
+        self._keep[rid.value] = keep
+        return rid.value
+
+    def wait(self, rid, timeout_ms=-1):
+        """(JoinResult, timing dict), or None while the request is not done."""
+        res = L.SjhipJoinResult()
+        t = L.SjhipRequestTiming()
+        rc = L.lib().SjhipServerWait(self.h, int(rid), int(timeout_ms), ctypes.byref(res), ctypes.byref(t))
+        if rc == errno.EAGAIN:
+            return None
+        self._keep.pop(int(rid), None)
+        if rc:
+            raise GpuSpatialError(L.lib().SjhipLastError().decode())
+        timing = {"submit_ms": t.submit_ms, "admit_ms": t.admit_ms, "done_ms": t.done_ms,
+                  "estimate_bytes": int(t.estimate_bytes)}
+        return JoinResult(res), timing
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().SjhipServerDestroy(self.h)
+            self.h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -191,6 +191,42 @@ def test_dwithin_random_walks_vs_oracle(opts):
     assert len(gb) > 1000
 
 
+def test_corpus_dwithin_all_pairs_vs_oracle(opts):
+    """Every (polygon, point) pair of the adversarial corpus (points on
+    vertices and edges, one ulp off them, in holes, rings of up to 39
+    segments with segment trees) under the distance refine."""
+    cb, cp = C.corpus()
+    b, p = corpus_arrow(cb), corpus_arrow(cp)
+    nb, np_ = len(b), len(p)
+    bi = np.repeat(np.arange(nb), np_).astype(np.uint32)
+    pi = np.tile(np.arange(np_), nb).astype(np.uint32)
+    rf = G.GpuSpatialRefiner(opts)
+    rf.init_build_schema(b)
+    rf.push_build(b)
+    rf.finish_building()
+    sb, sp = O.Store(cb), O.Store(cp)
+    for d in [0.0, 1e-9, 0.25, 1.0]:
+        say("corpus dwithin", d)
+        want = O.refine(sb, sp, "dwithin", bi, pi, distance=d, nthreads=8)
+        ob, op = rf.refine_distance(p, d, bi, pi)
+        eq, miss, extra = pairs_equal((ob, op), (bi[want], pi[want]))
+        assert eq, (d, miss, extra)
+
+
+@pytest.mark.parametrize("d", [0.0, 0.05, 0.7])
+def test_dwithin_zipf_polygons_vs_oracle(d, opts):
+    """Large rings (up to 4096 vertices, segment trees) under the distance
+    refine: inside, on-boundary-free outside within d, and beyond d."""
+    b, p = D.workload_pip_zipf(60_000, 600, seed=19)
+    say("dwithin zipf", d)
+    ix, rf = build_pair(b.to_arrow(), opts)
+    r = G.join_device(ix, rf, G.DeviceWkb.from_host(p), "dwithin", distance=d)
+    gb, gp = r.to_host()
+    ob, op = O.join(O.Store(b.as_tuple()), O.Store(p.as_tuple()), "dwithin", distance=d, nthreads=8)
+    assert np.array_equal(gb, ob) and np.array_equal(gp, op), (len(gb), len(ob))
+    assert len(gb) > 100
+
+
 def test_dwithin_polygons_and_refine_distance_abi(opts):
     b, p = D.workload_pip_uniform(20_000, 200, seed=17, extent=100.0 * np.sqrt(200 / 10_000))
     ix, rf = build_pair(b.to_arrow(), opts)
