@@ -51,13 +51,21 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the all-FP64 comparison step")
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--refine-order", type=int, default=1, help="1 = group candidates by polygon before refine")
+    ap.add_argument("--refine-order", type=int, default=None,
+                    help="0 = refine in filter order, 1 = group candidates by build row first, 2 = (DWithin) fused "
+                         "into the filter walk; default 1 for pip_zipf_contains, 0 for dwithin_walks")
+    ap.add_argument("--workload", default="pip_zipf_contains", choices=["pip_zipf_contains", "dwithin_walks"],
+                    help="pip_zipf_contains: 100M points x 1M Zipf stars, ST_Contains (the headline); "
+                         "dwithin_walks: 50M points x 5M random-walk lines, ST_DWithin 0.05")
+    ap.add_argument("--probe-batch", type=int, default=5_000_000, help="dwithin_walks: probe points per batch")
     ap.add_argument("--serve-requests", type=int, default=96,
                     help="requests in the serving measurement (0 = skip it)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only time the plain copy kernel (HBM calibration for the counter summaries)")
     args = ap.parse_args()
+    if args.refine_order is None:
+        args.refine_order = 0 if args.workload == "dwithin_walks" else 1
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: every box in this build has one GPU, so --gpus must be 1 "
@@ -66,6 +74,9 @@ def main():
 
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.workload == "dwithin_walks":
+        run_dwithin_walks(args)
         return
 
     from sjhip import datagen as D
@@ -326,6 +337,136 @@ def calibrate_copy(nbytes=1 << 30, reps=20):
 
     ms = G.copy_kernel_ms(nbytes, reps)
     print(json.dumps({"copy_bytes": nbytes, "ms": ms, "hbm_gbs": 2 * nbytes / (ms / 1e3) / 1e9}), flush=True)
+
+
+def strip_pairs_mask(strip, gb, gp):
+    """Local (strip-row) keys (probe << 32 | build) of the GPU pairs whose
+    build and probe rows both lie in the strip sample."""
+    bmap = np.full(int(gb.max()) + 1 if len(gb) else 1, -1, np.int64)
+    sel_b = strip["brows"]
+    bmap[sel_b[sel_b < len(bmap)]] = np.nonzero(sel_b < len(bmap))[0]
+    pset = np.full(int(gp.max()) + 1 if len(gp) else 1, -1, np.int64)
+    sel_p = strip["prows"]
+    pset[sel_p[sel_p < len(pset)]] = np.nonzero(sel_p < len(pset))[0]
+    lb, lp = bmap[gb], pset[gp]
+    keep = (lb >= 0) & (lp >= 0)
+    return (lp[keep].astype(np.uint64) << 32) | lb[keep].astype(np.uint64)
+
+
+def cut_strip(D, wl, build, probe, frac):
+    ext = wl["extent"]
+    x0 = 0.5 * ext
+    w = frac * ext
+    brows = np.nonzero((wl["centers"][:, 0] >= x0) & (wl["centers"][:, 0] < x0 + w))[0]
+    prows = np.nonzero((wl["probe_xy"][:, 0] >= x0) & (wl["probe_xy"][:, 0] < x0 + w))[0]
+    return dict(brows=brows, prows=prows, bcol=D.take_rows(build, brows), pcol=D.take_rows(probe, prows), x0=x0, w=w)
+
+
+def run_dwithin_walks(args):
+    """50M uniform points x 5M random-walk linestrings (2..256 vertices,
+    steps N(0, 0.5^2)), ST_DWithin(line, point, 0.05).  One step = the 50M
+    probe points through the device-resident join in batches of
+    --probe-batch points."""
+    from sjhip import datagen as D
+
+    dist = 0.05
+    n_points = int(round(50_000_000 * args.scale))
+    n_lines = int(round(5_000_000 * args.scale))
+    t0 = time.time()
+    wl = D.workload_dwithin_walks_parallel(n_points, n_lines, seed=args.seed + 1, workers=args.workers)
+    build, probe = wl["build"], wl["probe"]
+    log(f"generated {n_lines} lines ({int(wl['counts'].sum())} vertices, {build.offsets[-1] / 1e9:.1f} GB WKB) and "
+        f"{n_points} points in {time.time() - t0:.1f}s")
+    strip = None if args.no_cpu else cut_strip(D, wl, build, probe, args.cpu_strip / 4)
+    if strip is not None:
+        log(f"cpu strip: {len(strip['brows'])} lines, {len(strip['prows'])} points")
+
+    from sjhip import gpuspatial as G
+
+    opts = G.GpuSpatialOptions(device_id=0)
+    tb = time.time()
+    rf = G.GpuSpatialRefiner(opts)
+    rf.init_build_schema("Z")
+    bdev = G.DeviceWkb.from_host(build)
+    rf.push_build_device(bdev)
+    rf.finish_building()
+    del bdev
+    ix = G.GpuSpatialIndex(opts)
+    ix.build_from_refiner(rf)
+    build_ms = (time.time() - tb) * 1000
+    pdev = G.DeviceWkb.from_host(probe)
+    log(f"build side resident: {build_ms:.0f} ms")
+    bsz = args.probe_batch
+
+    def step(precision, keep_pairs=False):
+        tot = np.zeros(4, np.int64)
+        stage = np.zeros(6)
+        kept = []
+        for a in range(0, n_points, bsz):
+            r = G.join_device(ix, rf, pdev.slice(a, min(n_points, a + bsz)), "dwithin", distance=dist,
+                              precision=precision, sort_output=False, refine_order=args.refine_order, profile=True)
+            tot += (r.size, r.candidates, r.segment_tests, r.fallback_tests)
+            stage += np.array(list(r.res.stage_ms))
+            if keep_pairs:
+                gb, gp = r.to_host()
+                kept.append(strip_pairs_mask(strip, gb, (gp + a).astype(np.int64)))
+            r.release()
+        return tot, stage, kept
+
+    for i in range(args.warmup):
+        tot, stage, _ = step(args.precision)
+        log(f"warmup {i}: {tot[0]} pairs, {tot[1]} candidates, stages {list(stage)}")
+    t = time.time()
+    for i in range(args.steps):
+        tot, stage, _ = step(args.precision)
+    ms_per_step = (time.time() - t) * 1000 / args.steps
+    pairs, cands, tests, scans = (int(x) for x in tot)
+    log(f"timed: {ms_per_step:.1f} ms/step, {pairs} pairs, {cands} candidates, stages(ms) {list(stage)}")
+    plain = None
+    if not args.no_plain:
+        t1 = time.time()
+        ptot, pstage, _ = step(1)
+        plain = dict(ms_per_step=(time.time() - t1) * 1000, pairs=int(ptot[0]), same_pair_count=bool(ptot[0] == pairs),
+                     refine_ms=float(pstage[3]))
+        log(f"plain fp64: {plain}")
+    cpu = None
+    if strip is not None:
+        _, _, kept = step(args.precision, keep_pairs=True)
+        gk = np.sort(np.concatenate(kept)) if kept else np.zeros(0, np.uint64)
+        import oracle as O
+        threads = min(16, os.cpu_count() or 1)
+        tc = time.time()
+        cb, cp = O.join(O.Store(strip["bcol"].as_tuple()), O.Store(strip["pcol"].as_tuple()), "dwithin",
+                        distance=dist, nthreads=threads)
+        secs = time.time() - tc
+        ck = np.sort((cp.astype(np.uint64) << 32) | cb.astype(np.uint64))
+        cpu = {"value": len(cb) / secs, "unit": "pairs/s", "cores": threads, "kind": "port",
+               "sample": f"vertical strip x in [{strip['x0']:.1f}, {strip['x0'] + strip['w']:.1f}) of line starts "
+                         f"and points: {len(strip['brows'])} lines x {len(strip['prows'])} points, {len(cb)} pairs, "
+                         f"{secs:.2f} s (WKB parse + grid filter + geo-semantics distance refine)",
+               "gpu_pairs_in_sample": int(len(gk)), "pair_set_diff": int(len(np.setxor1d(gk, ck)))}
+    filter_bytes = 16 * n_points + 8 * cands
+    out = {
+        "metric": "refined join pairs/s (ST_DWithin 0.05, 50M points x 5M random-walk lines, device-resident)",
+        "value": pairs / (ms_per_step / 1000), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if args.precision == 1 else "f32+f64",
+        "data": "synthetic (seeded, generated on the host)",
+        "config": {"workload": "dwithin_walks", "probe_points": n_points, "build_lines": n_lines,
+                   "build_vertices": int(wl["counts"].sum()), "predicate": "ST_DWithin", "distance": dist,
+                   "probe_batch": bsz, "parallelism": "single-gpu"},
+        "pairs_per_step": pairs, "candidates_per_step": cands, "probe_points_per_s": n_points / (ms_per_step / 1000),
+        "stage_ms": {"decode_rects": stage[0], "morton": stage[1], "filter": stage[2], "group_by_line": stage[5],
+                     "refine": stage[3]},
+        "roofline_filter": {"bound": "hbm", "achieved": filter_bytes / (stage[2] / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": filter_bytes / (stage[2] / 1e3) / 1e9 / HBM_PEAK_GBS},
+        "fp64_segment_tests": tests, "exact_ring_scans": scans,
+        "build_ms": build_ms, "plain_fp64": plain, "cpu_baseline": cpu,
+    }
+    if plain:
+        out["speedup_vs_plain_fp64"] = plain["ms_per_step"] / ms_per_step
+    print(json.dumps(out), flush=True)
+    D.release_shared(wl["keep"])
 
 
 def cpu_baseline(strip, gb, gp, args):

@@ -248,7 +248,9 @@ struct SjhipJoinOptions {
   int32_t sort_probes; /* 1 = visit probes in Morton order */
   int32_t profile;     /* 1 = time the stages with HIP events (stage_ms) */
   int32_t refine_order; /* 0 = refine candidates in filter order, 1 = group
-                           them by build row first (polygon-major) */
+                           them by build row first (polygon-major), 2 =
+                           (DWithin, point probes) decide the distance inside
+                           the filter walk, writing no candidates */
   void* stream;        /* hipStream_t, or NULL for a stream of the call */
 };
 

@@ -22,6 +22,7 @@
 // with a wavefront prefix sum and a single atomic per wavefront.
 #include <hipcub/hipcub.hpp>
 
+#include "lbvh_filter.h"
 #include "sjhip.h"
 #include "wave_out.h"
 
@@ -29,16 +30,7 @@ namespace sjhip {
 
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kMaxLevels = 14;
-
-__device__ __forceinline__ bool rect_ok(const float4& r) {
-  return r.x <= r.z && r.y <= r.w;  // false for NaN
-}
-
-__device__ __forceinline__ bool hit(const float4& a, const float4& b) {
-  return !(b.x > a.z || b.z < a.x || b.y > a.w || b.w < a.y);
-}
+constexpr int kBlock = kFilterBlock;
 
 __device__ __forceinline__ uint32_t spread16(uint32_t x) {
   x &= 0xffff;
@@ -134,91 +126,6 @@ __global__ void k_merge(const float4* child, int64_t n_child, int group, float4*
   parent[i] = u;
 }
 
-struct TreeView {
-  const float4* nodes;
-  const float4* leaf_rects;
-  const uint32_t* leaf_ids;
-  int64_t n_valid;
-  int n_levels;
-  int64_t off[kMaxLevels];
-  int64_t cnt[kMaxLevels];
-};
-
-template <int F, int LEAF>
-__global__ __launch_bounds__(kBlock) void k_filter(TreeView t, const float4* q, int64_t nq, const uint32_t* qorder,
-                                                   int emit_rank, uint32_t* ob, uint32_t* op, uint64_t cap,
-                                                   unsigned long long* counter) {
-  // per lane and level: the children of the level's current node still to
-  // visit (the node itself is `cur`, its parent cur / F)
-  __shared__ uint8_t s_mask[kMaxLevels + 1][kBlock];
-  __shared__ uint32_t s_ob[kBlock / 64][kWaveOutCap], s_op[kBlock / 64][kWaveOutCap];
-  const int tid = threadIdx.x;
-  WaveOut wo{s_ob[tid >> 6], s_op[tid >> 6], 0};
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + tid;
-  bool active = i < nq && t.n_levels > 0;
-  uint32_t qid = 0, pid = 0;  // query row; probe id emitted (row, or position i in qorder)
-  float4 box = make_float4(0, 0, 0, 0);
-  const int top = t.n_levels;
-  int lvl = top;
-  uint32_t cur = 0;  // node at level lvl (level top: the virtual root)
-  if (active) {
-    qid = qorder ? qorder[i] : (uint32_t)i;
-    pid = emit_rank ? (uint32_t)i : qid;
-    box = q[qid];
-    if (!rect_ok(box)) active = false;
-  }
-  if (active) {
-    // virtual root: children are all nodes of level top-1 (<= F of them)
-    uint32_t m = 0;
-    const float4* nd = t.nodes + t.off[top - 1];
-    int64_t nc = t.cnt[top - 1];
-    for (int c = 0; c < F; c++)
-      if (c < nc && hit(nd[c], box)) m |= 1u << c;
-    s_mask[top][tid] = m;
-  }
-  while (__any(active)) {
-    uint32_t hits = 0;
-    int64_t hbase = 0;
-    if (active) {
-      while (lvl <= top && s_mask[lvl][tid] == 0) {
-        lvl++;
-        cur /= F;
-      }
-      if (lvl > top) {
-        active = false;
-      } else {
-        uint32_t m = s_mask[lvl][tid];
-        int c = __ffs(m) - 1;
-        s_mask[lvl][tid] = m & (m - 1);
-        int64_t child = (int64_t)cur * F + c;  // index at level lvl-1
-        if (lvl - 1 == 0) {
-          hbase = child * LEAF;
-          int64_t e = min(hbase + LEAF, t.n_valid);
-          const float4* lr = t.leaf_rects + hbase;
-#pragma unroll
-          for (int k = 0; k < LEAF; k++)
-            if (hbase + k < e && hit(lr[k], box)) hits |= 1u << k;
-        } else {
-          int cl = lvl - 2;  // level of the child's children
-          const float4* nd = t.nodes + t.off[cl];
-          int64_t c0 = child * F;
-          int64_t nc = t.cnt[cl] - c0;
-          uint32_t mm = 0;
-#pragma unroll
-          for (int k = 0; k < F; k++)
-            if (k < nc && hit(nd[c0 + k], box)) mm |= 1u << k;
-          lvl = lvl - 1;
-          cur = (uint32_t)child;
-          s_mask[lvl][tid] = mm;
-        }
-      }
-    }
-    // wavefront compaction of this step's candidates into the LDS buffer
-    if (__ballot(hits != 0)) wave_append_mask(wo, hits, t.leaf_ids, hbase, pid, ob, op, cap, counter);
-  }
-  wave_flush(wo, ob, op, cap, counter);
-}
-
 unsigned int blocks_for(int64_t n) { return (unsigned int)((n + kBlock - 1) / kBlock); }
 
 void center_bounds(const float4* r, int64_t n, float bb[4], int64_t* n_valid, hipStream_t s) {
@@ -309,21 +216,12 @@ uint64_t filter_pairs(const RectTree& t, const float4* q, int64_t nq, const uint
     pb.p.alloc(want, s);
   }
   if (nq == 0 || t.n_levels == 0) return 0;
-  TreeView v{};
-  v.nodes = t.nodes.p;
-  v.leaf_rects = t.leaf_rects.p;
-  v.leaf_ids = t.leaf_ids.p;
-  v.n_valid = t.n_valid;
-  v.n_levels = t.n_levels;
-  for (int l = 0; l < t.n_levels; l++) {
-    v.off[l] = t.level_off[l];
-    v.cnt[l] = t.level_cnt[l];
-  }
+  TreeView v = tree_view(t);
   for (int attempt = 0; attempt < 2; attempt++) {
     SJ_CHECK(hipMemsetAsync(pb.counter.p, 0, 8, s));
     if (timer) timer->record_start(s);
-    hipLaunchKernelGGL((k_filter<RectTree::FANOUT, RectTree::LEAF>), blocks_for(nq), kBlock, 0, s, v, q, nq, qorder,
-                       emit_rank ? 1 : 0, pb.b.p, pb.p.p, (uint64_t)pb.b.n, pb.counter.p);
+    hipLaunchKernelGGL((k_filter<RectTree::FANOUT, RectTree::LEAF, KeepAll>), blocks_for(nq), kBlock, 0, s, v, q, nq,
+                       qorder, emit_rank ? 1 : 0, KeepAll{}, pb.b.p, pb.p.p, (uint64_t)pb.b.n, pb.counter.p);
     SJ_CHECK(hipGetLastError());
     if (timer) timer->record_stop(s);
     unsigned long long total = 0;

@@ -574,24 +574,36 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
   if (ranked) gather_points(probe, ord, nq, rprobe, s);
   const GeomColumn& rp = ranked ? rprobe : probe;
   mark(2);
-  PairBuf cand;
-  uint64_t nc = filter_pairs(ix->tree, q.p, nq, ord, cand, s, ev.on ? &ft : nullptr, ranked);
-  mark(3);
-  // polygon-major order: one wavefront then walks few distinct rings
-  if (o.refine_order == 1) {
-    const float4* br = ix->rects.n >= (size_t)rf->build.n_rows ? ix->rects.p : nullptr;
-    sort_pairs_by_build(cand, nc, rf->build.n_rows, br, rp, s);
-  }
   std::unique_ptr<JoinResultImpl> jr(new JoinResultImpl());
   jr->device_id = rf->rt->device_id;
   RefineStats st;
-  uint64_t kept;
-  if (dwithin)
-    kept = refine_distance(rf->build, rf->ring_index, probe, o.distance, cand.b.p, cand.p.p, nc, jr->pairs, s,
-                           ev.on ? &rt : nullptr);
-  else
-    kept = refine_relation(rf->build, rf->ring_index, rp, o.predicate, cand.b.p, cand.p.p, nc, o.precision,
-                           jr->pairs, &st, s, ev.on ? &rt : nullptr, ranked ? ord : nullptr);
+  uint64_t kept = 0, nc = 0;
+  // refine_order 2 (DWithin over point probes): filter and distance refine
+  // in one kernel, candidates never written out
+  const bool fused = dwithin && probe.points_only && o.precision != 1 && o.refine_order == 2;
+  if (fused) {
+    kept = filter_refine_distance(ix->tree, q.p, nq, ord, rf->build, rf->ring_index, probe, o.distance, jr->pairs, &nc,
+                                  &st, s, ev.on ? &ft : nullptr);
+    mark(3);
+    mark(6);
+    mark(7);
+  } else {
+    PairBuf cand;
+    nc = filter_pairs(ix->tree, q.p, nq, ord, cand, s, ev.on ? &ft : nullptr, ranked);
+    mark(3);
+    // polygon-major order: one wavefront then walks few distinct rings
+    // (height slabs only for the point-in-polygon refine)
+    if (o.refine_order == 1) {
+      const float4* br = !dwithin && ix->rects.n >= (size_t)rf->build.n_rows ? ix->rects.p : nullptr;
+      sort_pairs_by_build(cand, nc, rf->build.n_rows, br, rp, s);
+    }
+    if (dwithin)
+      kept = refine_distance(rf->build, rf->ring_index, probe, o.distance, cand.b.p, cand.p.p, nc, jr->pairs, s,
+                             ev.on ? &rt : nullptr, o.precision, &st);
+    else
+      kept = refine_relation(rf->build, rf->ring_index, rp, o.predicate, cand.b.p, cand.p.p, nc, o.precision,
+                             jr->pairs, &st, s, ev.on ? &rt : nullptr, ranked ? ord : nullptr);
+  }
   mark(8);
   if (o.sort_output) sort_pairs(jr->pairs.b.p, jr->pairs.p.p, kept, s);
   mark(9);

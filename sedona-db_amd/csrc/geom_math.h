@@ -201,6 +201,11 @@ SJ_HD int expansion_grow(double* e, int n, double b) {
   return k;
 }
 
+// The exact expansion sum, out of line: rarely reached, and its 40-term
+// buffer would otherwise occupy every caller's stack.
+__host__ __device__ __noinline__ int orient2d_exact(double ax, double ay, double bx, double by, double cx,
+                                                     double cy);
+
 SJ_HD int orient2d_sign(double ax, double ay, double bx, double by, double cx, double cy) {
   // fast filter (Shewchuk's ccwerrboundA)
   double detl = (ax - cx) * (by - cy);
@@ -209,6 +214,11 @@ SJ_HD int orient2d_sign(double ax, double ay, double bx, double by, double cx, d
   double detsum = fabs(detl) + fabs(detr);
   const double errA = 3.3306690738754716e-16;  // (3 + 16 eps) eps, eps = 2^-53
   if (fabs(det) > errA * detsum) return det > 0 ? 1 : -1;
+  return orient2d_exact(ax, ay, bx, by, cx, cy);
+}
+
+__host__ __device__ __noinline__ inline int orient2d_exact(double ax, double ay, double bx, double by, double cx,
+                                                            double cy) {
   double u0, u1, v0, v1, w0, w1, z0, z1;
   two_sum(ax, -cx, u1, u0);
   two_sum(by, -cy, v1, v0);

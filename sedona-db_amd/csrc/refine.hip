@@ -29,6 +29,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "geom_math.h"
+#include "lbvh_filter.h"
 #include "sjhip.h"
 #include "wave_out.h"
 
@@ -380,8 +381,8 @@ __device__ __forceinline__ bool build_ring_contains(const Ctx& c, int64_t ring, 
                                 allow_on_edge, tests, fallbacks);
 }
 
-// Single-ring build rows (one part, one ring) as one 64-byte record, so
-// that the point-in-polygon kernel reaches a row's ring in one load.
+// The FP32 frame origin of a ring: the centre of its rect (the record of a
+// single-ring row stores the rect, so the origin is recomputed from it).
 __device__ __forceinline__ double2 ring_origin(const double4& rb) {
   double ox = 0.5 * rb.x + 0.5 * rb.z, oy = 0.5 * rb.y + 0.5 * rb.w;
   if (!(ox == ox)) { ox = 0; oy = 0; }
@@ -895,10 +896,17 @@ __device__ __forceinline__ bool walk_any(const float4* box, const TreeShape& sha
 }
 
 // Segment j of a vertex run is a term <= d of the reference distance (on
-// the segment: 0, for lines; else distance_coord_to_line).
+// the segment: 0, for lines; else distance_coord_to_line).  A segment
+// whose bounding box is more than dm away on either axis is rejected first:
+// dm = d (1 + 1e-4) + 4E exceeds d by more than the rounding of the
+// distance computation, which is relative to the ring's extent (E =
+// 2^-22 H) and to d.
 __device__ __forceinline__ bool segment_within(const double2* xy, int64_t j, double px, double py, double d,
-                                               bool check_on) {
+                                               double dm, bool check_on) {
   double2 a = xy[j], b = xy[j + 1];
+  double gx = fmax(fmin(a.x, b.x) - px, px - fmax(a.x, b.x));
+  double gy = fmax(fmin(a.y, b.y) - py, py - fmax(a.y, b.y));
+  if (fmax(gx, gy) > dm) return false;
   if (check_on && line_intersects_coord(a.x, a.y, b.x, b.y, px, py)) return true;
   return coord_to_line(px, py, a.x, a.y, b.x, b.y) <= d;
 }
@@ -907,18 +915,18 @@ __device__ __forceinline__ bool segment_within(const double2* xy, int64_t j, dou
 // Segments whose FP32 tree boxes lie farther than d from the point, plus
 // the boxes' and the point's FP32 error bounds (4E) and the rounding of d,
 // cannot be within d and are skipped.
-__device__ bool ring_any_within(const Ctx& c, int64_t ring, double px, double py, double d, bool check_on) {
-  const int64_t v0 = c.B.pt_off[ring], npts = c.B.pt_off[ring + 1] - v0;
+__device__ bool ring_any_within_at(const Ctx& c, int64_t v0, int64_t npts, int64_t toff, float E, double2 o,
+                                   double px, double py, double d, bool check_on, uint64_t& tests) {
   const double2* xy = c.B.xy + v0;
   const int64_t ns = npts - 1;
-  const int64_t toff = c.R.seg_tree_off ? c.R.seg_tree_off[ring] : -1;
-  const float E = c.R.err[ring];
+  const double dm = E < INFINITY ? d * 1.0001 + 4.0 * (double)E : INFINITY;
   if (toff < 0 || ns < 1 || !(E < INFINITY)) {
-    for (int64_t j = 0; j < ns; j++)
-      if (segment_within(xy, j, px, py, d, check_on)) return true;
+    for (int64_t j = 0; j < ns; j++) {
+      tests++;
+      if (segment_within(xy, j, px, py, d, dm, check_on)) return true;
+    }
     return false;
   }
-  const double2 o = c.R.origin[ring];
   const float D = (float)(d * 1.0001) + 4.0f * E;
   const float pxl = (float)(px - o.x), pyl = (float)(py - o.y);
   const float qx0 = pxl - D, qx1 = pxl + D, qy0 = pyl - D, qy1 = pyl + D;
@@ -926,11 +934,20 @@ __device__ bool ring_any_within(const Ctx& c, int64_t ring, double px, double py
   auto pass = [&](const float4& b) { return b.x <= qx1 && b.z >= qx0 && b.y <= qy1 && b.w >= qy0; };
   auto leaf = [&](int lf) {
     const int s0 = lf * kSegLeaf, s1 = min(s0 + kSegLeaf, nsi);
-    for (int j = s0; j < s1; j++)
-      if (segment_within(xy, j, px, py, d, check_on)) return true;
+    for (int j = s0; j < s1; j++) {
+      tests++;
+      if (segment_within(xy, j, px, py, d, dm, check_on)) return true;
+    }
     return false;
   };
   return walk_any(c.R.seg_tree + toff, seg_tree_shape(nsi), pass, leaf);
+}
+
+__device__ __forceinline__ bool ring_any_within(const Ctx& c, int64_t ring, double px, double py, double d,
+                                                bool check_on, uint64_t& tests) {
+  const int64_t v0 = c.B.pt_off[ring];
+  return ring_any_within_at(c, v0, c.B.pt_off[ring + 1] - v0, c.R.seg_tree_off ? c.R.seg_tree_off[ring] : -1,
+                            c.R.err[ring], c.R.origin[ring], px, py, d, check_on, tests);
 }
 
 // Position of the point w.r.t. a closed build ring, exactly: 2 boundary,
@@ -939,7 +956,7 @@ __device__ bool ring_any_within(const Ctx& c, int64_t ring, double px, double py
 // uncertain (then no vertex or edge lies within its margin of the point's
 // ray, so every crossing-number rule counts the same crossings); otherwise
 // the exact scan does.
-__device__ int build_ring_position(const Ctx& c, int64_t ring, double px, double py) {
+__device__ int build_ring_position(const Ctx& c, int64_t ring, double px, double py, uint64_t& exact_scans) {
   const int64_t v0 = c.B.pt_off[ring], npts = c.B.pt_off[ring + 1] - v0;
   const double4 rb = c.R.ring_rect[ring];
   if (!rect_covers(rb, px, py)) return 0;
@@ -950,6 +967,7 @@ __device__ int build_ring_position(const Ctx& c, int64_t ring, double px, double
                                               tests, fb);
     if (fb == 0) return in ? 1 : 0;
   }
+  exact_scans++;
   return exact_ring_position(ring_at(c.B, ring), px, py);
 }
 
@@ -958,43 +976,77 @@ __device__ int build_ring_position(const Ctx& c, int64_t ring, double px, double
 // distance itself: the distance is a minimum over parts and segments (0
 // inside a polygon and on a line; NaN terms ignored), and such a minimum is
 // <= d exactly when one of its terms is.
-__device__ bool within_distance(const Ctx& c, int64_t row, double px, double py, double d) {
+//
+// KIND narrows the build column's geometry types at compile time (the
+// kernel instance is chosen from the column's type mask), so that a
+// column of lines does not pay the registers of the polygon branch.
+enum DistKind : int { kDistAny = 0, kDistLines = 1, kDistPolygons = 2 };
+
+template <int KIND>
+__device__ __forceinline__ bool within_distance(const Ctx& c, int64_t row, double px, double py, double d,
+                                                uint64_t& tests, uint64_t& exact_scans) {
   if (!(d >= 0)) return false;  // every distance is >= 0 or NaN
   const GeomView& g = c.B;
   const int t = g.type[row];
-  if (g.points_only || t == kPoint || t == kMultiPoint || c.R.ring_rect == nullptr)
-    return distance_to_point(g, row, px, py) <= d;
+  if constexpr (KIND == kDistAny) {
+    if (g.points_only || t == kPoint || t == kMultiPoint || c.R.ring_rect == nullptr)
+      return distance_to_point(g, row, px, py) <= d;
+  }
+  if constexpr (KIND == kDistLines) {
+    // a single line: its record holds the vertex run, tree and frame
+    const PolyDesc r = c.R.desc[row];
+    if (r.npts >= 0) {
+      if (r.npts < 2) return true;  // point_to_run: 0
+      return ring_any_within_at(c, r.v0, r.npts, r.tree_off, r.err, ring_origin(r.rect), px, py, d, true, tests);
+    }
+  }
   for (int64_t q = g.part_off[row]; q < g.part_off[row + 1]; q++) {
     const int64_t r0 = g.ring_off[q], r1 = g.ring_off[q + 1];
-    if (t == kLine || t == kMultiLine) {
+    if (KIND == kDistLines || (KIND == kDistAny && (t == kLine || t == kMultiLine))) {
       if (g.pt_off[r0 + 1] - g.pt_off[r0] < 2) return true;  // point_to_run: 0
-      if (ring_any_within(c, r0, px, py, d, true)) return true;
+      if (ring_any_within(c, r0, px, py, d, true, tests)) return true;
       continue;
     }
+    if constexpr (KIND == kDistLines) continue;
     if (r0 == r1) return true;  // point_to_polygon: 0
-    int st = build_ring_position(c, r0, px, py);
+    int st = build_ring_position(c, r0, px, py, exact_scans);
     if (st == 2) return true;
     if (st == 1) {
       bool in_hole = false;
       for (int64_t h = r0 + 1; h < r1; h++) {
-        int hs = build_ring_position(c, h, px, py);
+        int hs = build_ring_position(c, h, px, py, exact_scans);
         if (hs == 2) return true;
         if (hs == 1) in_hole = true;
       }
       if (!in_hole) return true;
     }
     for (int64_t k = r0; k < r1; k++)
-      if (ring_any_within(c, k, px, py, d, false)) return true;
+      if (ring_any_within(c, k, px, py, d, false, tests)) return true;
   }
   return false;
 }
 
-__global__ __launch_bounds__(kBlock) void k_refine_distance(Ctx c, double dist, const uint32_t* cb,
+// Kernel instance for a build column's type mask.
+inline int dist_kind(uint32_t type_mask, bool points_only, const RingIndex& ri) {
+  if (points_only || ri.ring_rect.p == nullptr || ri.desc.p == nullptr) return kDistAny;
+  const uint32_t lines = (1u << kLine) | (1u << kMultiLine), polys = (1u << kPolygon) | (1u << kMultiPolygon);
+  if (type_mask && (type_mask & ~lines) == 0) return kDistLines;
+  if (type_mask && (type_mask & ~polys) == 0) return kDistPolygons;
+  return kDistAny;
+}
+
+// plain != 0: the all-FP64 form (the distance itself over every segment,
+// then <=), for comparison.  (Forcing 8 waves per SIMD here, as for the
+// point-in-polygon walk, measured slower: the spills cost more.)
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_refine_distance(Ctx c, double dist, int plain, const uint32_t* cb,
                                                             const uint32_t* cp, uint64_t n, uint32_t* ob,
-                                                            uint32_t* op, uint64_t cap, unsigned long long* counter) {
+                                                            uint32_t* op, uint64_t cap, unsigned long long* counter,
+                                                            unsigned long long* stats) {
   const GeomView& B = c.B;
   const GeomView& P = c.P;
   SJ_WAVE_OUT(wo);
+  uint64_t tests = 0, exact_scans = 0;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
     uint64_t i = base + threadIdx.x;
@@ -1005,12 +1057,58 @@ __global__ __launch_bounds__(kBlock) void k_refine_distance(Ctx c, double dist, 
       p = cp[i];
       double px, py;
       bool bok = B.type[b] != kNull && (B.points_only || B.part_off[b] != B.part_off[b + 1]);
-      if (bok && point_of(P, p, px, py)) keep = within_distance(c, b, px, py, dist);
+      if (bok && point_of(P, p, px, py))
+        keep = plain ? distance_to_point(B, b, px, py) <= dist
+                     : within_distance<KIND>(c, b, px, py, dist, tests, exact_scans);
     }
     emit(wo, keep, b, p, ob, op, cap, counter);
   }
   wave_flush(wo, ob, op, cap, counter);
+  for (int o = 32; o > 0; o >>= 1) {
+    tests += __shfl_xor(tests, o);
+    exact_scans += __shfl_xor(exact_scans, o);
+  }
+  if ((threadIdx.x & 63) == 0 && stats) {
+    atomicAdd(&stats[0], (unsigned long long)tests);
+    atomicAdd(&stats[1], (unsigned long long)exact_scans);
+  }
 }
+
+// Fused filter + distance refine: the filter walk over the rect tree keeps
+// a rect hit (build row b, probe point qid) only when within_distance
+// holds, so candidates are never written out.
+template <int KIND>
+struct KeepWithin {
+  Ctx c;
+  double d;
+  unsigned long long* stats;  // [0] rect hits, [1] FP64 segment tests, [2] exact ring scans
+  uint64_t hits_seen, tests, scans;
+  __device__ __forceinline__ uint32_t operator()(uint32_t hits, const uint32_t* ids, uint32_t qid) {
+    hits_seen += __popc(hits);
+    const double2 pt = c.P.xy[qid];  // a point column: null and empty rows hold NaN
+    if (!(pt.x == pt.x || pt.y == pt.y)) return 0;
+    uint32_t kept = 0;
+    for (uint32_t m = hits; m; m &= m - 1) {
+      const int k = __ffs(m) - 1;
+      const uint32_t b = ids[k];
+      const bool bok = c.B.type[b] != kNull && (c.B.points_only || c.B.part_off[b] != c.B.part_off[b + 1]);
+      if (bok && within_distance<KIND>(c, b, pt.x, pt.y, d, tests, scans)) kept |= 1u << k;
+    }
+    return kept;
+  }
+  __device__ __forceinline__ void finish() {
+    for (int o = 32; o > 0; o >>= 1) {
+      hits_seen += __shfl_xor(hits_seen, o);
+      tests += __shfl_xor(tests, o);
+      scans += __shfl_xor(scans, o);
+    }
+    if ((threadIdx.x & 63) == 0 && stats) {
+      atomicAdd(&stats[0], (unsigned long long)hits_seen);
+      atomicAdd(&stats[1], (unsigned long long)tests);
+      atomicAdd(&stats[2], (unsigned long long)scans);
+    }
+  }
+};
 
 // ---- ring index ------------------------------------------------------------
 
@@ -1131,6 +1229,9 @@ unsigned int grid_for(uint64_t n) {
   return (unsigned int)std::max<uint64_t>(1, std::min<uint64_t>(b, 1u << 16));
 }
 
+// Single-ring build rows (one polygon part with one ring, or one line) as
+// one 64-byte record, so that the refine kernels reach a row's vertex run
+// with one load.
 __global__ void k_poly_desc(GeomView g, RingIndexView R, PolyDesc* out) {
   int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= g.n_rows) return;
@@ -1141,7 +1242,8 @@ __global__ void k_poly_desc(GeomView g, RingIndexView R, PolyDesc* out) {
   d.npts = -1;
   d.err = INFINITY;
   d.pad = 0;
-  if (is_polyish(g.type[r]) && g.part_off[r + 1] - g.part_off[r] == 1) {
+  const int t = g.type[r];
+  if ((is_polyish(t) || t == kLine || t == kMultiLine) && g.part_off[r + 1] - g.part_off[r] == 1) {
     int64_t q = g.part_off[r];
     if (g.ring_off[q + 1] - g.ring_off[q] == 1) {
       int64_t ring = g.ring_off[q];
@@ -1311,8 +1413,8 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
 }
 
 uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
-                         const uint32_t* cb,
-                         const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s, KernelTimer* timer) {
+                         const uint32_t* cb, const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s,
+                         KernelTimer* timer, int precision, RefineStats* stats) {
   out.counter.reserve(1, s);
   out.b.reserve(n > 0 ? n : 1, s);
   out.p.reserve(n > 0 ? n : 1, s);
@@ -1325,14 +1427,90 @@ uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const Geo
   c.B = build.view();
   c.R = ri.view();
   c.P = probe.view();
-  hipLaunchKernelGGL(k_refine_distance, grid_for(n), kBlock, 0, s, c, d, cb, cp, n, out.b.p, out.p.p,
-                     (uint64_t)out.b.n, out.counter.p);
+  DevArray<unsigned long long> aux(2, s);
+  SJ_CHECK(hipMemsetAsync(aux.p, 0, 16, s));
+  const int plain = precision == 1 ? 1 : 0;
+  switch (plain ? kDistAny : dist_kind(build.type_mask, build.points_only, ri)) {
+    case kDistLines:
+      hipLaunchKernelGGL(k_refine_distance<kDistLines>, grid_for(n), kBlock, 0, s, c, d, 0, cb, cp, n, out.b.p,
+                         out.p.p, (uint64_t)out.b.n, out.counter.p, aux.p);
+      break;
+    case kDistPolygons:
+      hipLaunchKernelGGL(k_refine_distance<kDistPolygons>, grid_for(n), kBlock, 0, s, c, d, 0, cb, cp, n, out.b.p,
+                         out.p.p, (uint64_t)out.b.n, out.counter.p, aux.p);
+      break;
+    default:
+      hipLaunchKernelGGL(k_refine_distance<kDistAny>, grid_for(n), kBlock, 0, s, c, d, plain, cb, cp, n, out.b.p,
+                         out.p.p, (uint64_t)out.b.n, out.counter.p, aux.p);
+  }
   SJ_CHECK(hipGetLastError());
   if (timer) timer->record_stop(s);
-  unsigned long long cnt = 0;
+  unsigned long long cnt = 0, h[2];
   SJ_CHECK(hipMemcpyAsync(&cnt, out.counter.p, 8, hipMemcpyDeviceToHost, s));
+  SJ_CHECK(hipMemcpyAsync(h, aux.p, 16, hipMemcpyDeviceToHost, s));
   SJ_CHECK(hipStreamSynchronize(s));
+  if (stats) {
+    stats->segment_tests += h[0];
+    stats->fallback_tests += h[1];
+  }
   return cnt;
+}
+
+uint64_t filter_refine_distance(const RectTree& t, const float4* q, int64_t nq, const uint32_t* qorder,
+                                const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
+                                PairBuf& out, uint64_t* candidates, RefineStats* stats, hipStream_t s,
+                                KernelTimer* timer) {
+  if (!probe.points_only) throw Error("fused distance join: probe rows must be points");
+  out.counter.reserve(1, s);
+  // capacity: the tree's observed kept pairs per query (16ths), else 4
+  uint64_t per16 = t.kept_per_query_x16.load(std::memory_order_relaxed);
+  size_t want = std::max<int64_t>(1 << 16, per16 ? (int64_t)((per16 * (uint64_t)nq) / 16 * 9 / 8 + 1024) : nq * 4);
+  out.b.reserve(want, s);
+  out.p.reserve(want, s);
+  if (candidates) *candidates = 0;
+  if (nq == 0 || t.n_levels == 0) return 0;
+  Ctx c;
+  c.B = build.view();
+  c.R = ri.view();
+  c.P = probe.view();
+  DevArray<unsigned long long> aux(3, s);
+  TreeView v = tree_view(t);
+  const int kind = dist_kind(build.type_mask, build.points_only, ri);
+  auto launch = [&](auto kw) {
+    kw.c = c;
+    kw.d = d;
+    kw.hits_seen = kw.tests = kw.scans = 0;
+    kw.stats = aux.p;
+    hipLaunchKernelGGL((k_filter<RectTree::FANOUT, RectTree::LEAF, decltype(kw)>), blocks_for(nq), kFilterBlock, 0,
+                       s, v, q, nq, qorder, 0, kw, out.b.p, out.p.p, (uint64_t)out.b.n, out.counter.p);
+  };
+  for (int attempt = 0; attempt < 2; attempt++) {
+    SJ_CHECK(hipMemsetAsync(out.counter.p, 0, 8, s));
+    SJ_CHECK(hipMemsetAsync(aux.p, 0, 24, s));
+    if (timer) timer->record_start(s);
+    if (kind == kDistLines) launch(KeepWithin<kDistLines>{});
+    else if (kind == kDistPolygons) launch(KeepWithin<kDistPolygons>{});
+    else launch(KeepWithin<kDistAny>{});
+    SJ_CHECK(hipGetLastError());
+    if (timer) timer->record_stop(s);
+    unsigned long long total = 0, h[3];
+    SJ_CHECK(hipMemcpyAsync(&total, out.counter.p, 8, hipMemcpyDeviceToHost, s));
+    SJ_CHECK(hipMemcpyAsync(h, aux.p, 24, hipMemcpyDeviceToHost, s));
+    SJ_CHECK(hipStreamSynchronize(s));
+    t.kept_per_query_x16.store((uint64_t)((total * 16) / (uint64_t)nq + 1), std::memory_order_relaxed);
+    if (total <= out.b.n) {
+      if (candidates) *candidates = h[0];
+      if (stats) {
+        stats->segment_tests += h[1];
+        stats->fallback_tests += h[2];
+      }
+      return total;
+    }
+    size_t cap = (size_t)(total + total / 8 + 1024);
+    out.b.alloc(cap, s);
+    out.p.alloc(cap, s);
+  }
+  throw Error("distance join output did not fit after resizing");
 }
 
 namespace {

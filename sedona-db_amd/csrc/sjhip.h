@@ -214,8 +214,10 @@ struct RectTree {
   DevArray<uint32_t> leaf_ids;      // their build ids
   DevArray<uint32_t> leaf_keys;     // their Morton keys (sorted)
   float bb[4] = {0, 0, 0, 0};       // bounds of rect centres (Morton frame)
-  // observed candidates per query x 16 (sizes the next probe's output)
+  // observed candidates (and fused-join kept pairs) per query x 16: sizes
+  // the next probe's output
   mutable std::atomic<uint64_t> cand_per_query_x16{0};
+  mutable std::atomic<uint64_t> kept_per_query_x16{0};
   RectTree() = default;
   RectTree(RectTree&& o) noexcept { *this = std::move(o); }
   RectTree& operator=(RectTree&& o) noexcept {
@@ -229,6 +231,7 @@ struct RectTree {
     leaf_keys = std::move(o.leaf_keys);
     for (int k = 0; k < 4; k++) bb[k] = o.bb[k];
     cand_per_query_x16.store(o.cand_per_query_x16.load());
+    kept_per_query_x16.store(o.kept_per_query_x16.load());
     return *this;
   }
 };
@@ -283,9 +286,20 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
                          const uint32_t* probe_ids = nullptr);
 
 // Distance refine (build.distance(probe) <= d); probe must be points.
+// precision 1: the all-FP64 form over every segment.  stats: FP64 segment
+// distance tests, and exact ring scans (uncertain certified crossing counts).
 uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
-                         const uint32_t* cb,
-                         const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s, KernelTimer* timer = nullptr);
+                         const uint32_t* cb, const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s,
+                         KernelTimer* timer = nullptr, int precision = 0, RefineStats* stats = nullptr);
+
+// ST_DWithin join of point probes in one kernel: the filter walk decides
+// the distance predicate for each rect hit, so only kept pairs are
+// written.  q: probe rects expanded by d; qorder: Morton visiting order.
+// *candidates receives the number of rect hits.
+uint64_t filter_refine_distance(const RectTree& t, const float4* q, int64_t nq, const uint32_t* qorder,
+                                const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
+                                PairBuf& out, uint64_t* candidates, RefineStats* stats, hipStream_t s,
+                                KernelTimer* timer = nullptr);
 
 // Regroups candidate pairs by build row and, for point probes, by the
 // point's height within the build rect (radix sort), so that a wavefront

@@ -372,6 +372,56 @@ def star_polygons_parallel(seed, counts, centers, radius_mu, radius_sigma=LOGN_S
     return WkbColumn(off, data), (shm,)
 
 
+def _walk_chunk(task):
+    lo, hi, seed = task
+    g = _PAR["walk"]
+    rng = np.random.default_rng(seed)
+    c = g["counts"][lo:hi]
+    steps = rng.normal(0.0, g["sigma"], size=(int(c.sum()), 2))
+    vstart = np.zeros(len(c) + 1, np.int64)
+    np.cumsum(c, out=vstart[1:])
+    steps[vstart[:-1]] = g["starts"][lo:hi]
+    cs = np.cumsum(steps, axis=0)
+    prev = np.concatenate([[[0.0, 0.0]], cs[vstart[1:-1] - 1]])
+    gid = np.repeat(np.arange(len(c)), c)
+    col = rings_wkb(c, cs - prev[gid], 2)
+    off = g["offsets"]
+    g["data"][off[lo]:off[hi]] = col.data
+
+
+def random_walks_parallel(seed, counts, starts, step_sigma, workers=16):
+    """Random-walk linestrings (as random_walks) generated by forked
+    workers; returns (WkbColumn over shared memory, keepalive handles)."""
+    counts = np.asarray(counts, dtype=np.int64)
+    n = len(counts)
+    off = np.zeros(n + 1, np.int64)
+    np.cumsum(9 + 16 * counts, out=off[1:])
+    shm, data = _shm_array((int(off[-1]),), np.uint8)
+    _PAR["walk"] = dict(counts=counts, starts=starts, sigma=step_sigma, offsets=off, data=data)
+    ss = np.random.SeedSequence(seed)
+    per = max(1, int(4e6 // max(1.0, counts.mean())))
+    bounds = list(range(0, n, per)) + [n]
+    seeds = ss.spawn(len(bounds))
+    tasks = [(bounds[i], bounds[i + 1], seeds[i]) for i in range(len(bounds) - 1)]
+    _run_parallel(_walk_chunk, tasks, workers)
+    _PAR.pop("walk", None)
+    return WkbColumn(off, data), (shm,)
+
+
+def workload_dwithin_walks_parallel(n_points=50_000_000, n_lines=5_000_000, seed=3, workers=16):
+    """50M uniform points x 5M random-walk linestrings (2..256 vertices,
+    steps N(0, 0.5^2) per axis, starts uniform), generated in parallel.
+    Returns dict(build, probe, starts, counts, probe_xy, extent, keep)."""
+    extent = 1000.0 * np.sqrt(n_lines / 5_000_000)
+    rng = np.random.default_rng(seed)
+    counts = rng.integers(2, 257, size=n_lines)
+    starts = rng.uniform(0, extent, size=(n_lines, 2))
+    build, k1 = random_walks_parallel(seed + 1000, counts, starts, 0.5, workers=workers)
+    probe, xy, k2 = points_parallel(seed + 2000, n_points, extent, centers=None, workers=workers)
+    return dict(build=build, probe=probe, starts=starts, centers=starts, counts=counts, probe_xy=xy, extent=extent,
+                keep=k1 + k2)
+
+
 def _points_chunk(task):
     lo, hi, seed = task
     g = _PAR["pts"]

@@ -179,12 +179,13 @@ def test_device_join_zipf_stars_vs_oracle(pred, precision, opts):
         say("  fallback rate", r.fallback_tests / max(1, r.segment_tests))
 
 
-def test_dwithin_random_walks_vs_oracle(opts):
+@pytest.mark.parametrize("refine_order", [1, 2])
+def test_dwithin_random_walks_vs_oracle(refine_order, opts):
     # ~160 candidate boxes per point at this line density (as at full size)
     b, p = D.workload_dwithin_walks(10_000, 2_000, seed=13)
-    say("dwithin walks", len(b), len(p))
+    say("dwithin walks", len(b), len(p), refine_order)
     ix, rf = build_pair(b.to_arrow(), opts)
-    r = G.join_device(ix, rf, p.to_arrow(), "dwithin", distance=0.05)
+    r = G.join_device(ix, rf, p.to_arrow(), "dwithin", distance=0.05, refine_order=refine_order)
     gb, gp = r.to_host()
     ob, op = O.join(O.Store(b.as_tuple()), O.Store(p.as_tuple()), "dwithin", distance=0.05, nthreads=8)
     assert np.array_equal(gb, ob) and np.array_equal(gp, op), (len(gb), len(ob))
@@ -213,14 +214,16 @@ def test_corpus_dwithin_all_pairs_vs_oracle(opts):
         assert eq, (d, miss, extra)
 
 
+@pytest.mark.parametrize("refine_order", [0, 1, 2])
 @pytest.mark.parametrize("d", [0.0, 0.05, 0.7])
-def test_dwithin_zipf_polygons_vs_oracle(d, opts):
+def test_dwithin_zipf_polygons_vs_oracle(d, refine_order, opts):
     """Large rings (up to 4096 vertices, segment trees) under the distance
-    refine: inside, on-boundary-free outside within d, and beyond d."""
+    refine: inside, on-boundary-free outside within d, and beyond d; in
+    filter order, grouped by polygon, and fused into the filter walk."""
     b, p = D.workload_pip_zipf(60_000, 600, seed=19)
-    say("dwithin zipf", d)
+    say("dwithin zipf", d, refine_order)
     ix, rf = build_pair(b.to_arrow(), opts)
-    r = G.join_device(ix, rf, G.DeviceWkb.from_host(p), "dwithin", distance=d)
+    r = G.join_device(ix, rf, G.DeviceWkb.from_host(p), "dwithin", distance=d, refine_order=refine_order)
     gb, gp = r.to_host()
     ob, op = O.join(O.Store(b.as_tuple()), O.Store(p.as_tuple()), "dwithin", distance=d, nthreads=8)
     assert np.array_equal(gb, ob) and np.array_equal(gp, op), (len(gb), len(ob))
