@@ -54,9 +54,13 @@ def main():
     ap.add_argument("--refine-order", type=int, default=None,
                     help="0 = refine in filter order, 1 = group candidates by build row first, 2 = (DWithin) fused "
                          "into the filter walk; default 1 for pip_zipf_contains, 0 for dwithin_walks")
-    ap.add_argument("--workload", default="pip_zipf_contains", choices=["pip_zipf_contains", "dwithin_walks"],
+    ap.add_argument("--workload", default="pip_zipf_contains",
+                    choices=["pip_zipf_contains", "dwithin_walks", "knn", "poly_poly"],
                     help="pip_zipf_contains: 100M points x 1M Zipf stars, ST_Contains (the headline); "
-                         "dwithin_walks: 50M points x 5M random-walk lines, ST_DWithin 0.05")
+                         "dwithin_walks: 50M points x 5M random-walk lines, ST_DWithin 0.05; "
+                         "knn: 20M queries x 200M clustered points, k=8; "
+                         "poly_poly: 2 x 5M clustered stars, ST_Intersects (use --scale)")
+    ap.add_argument("--knn-sample", type=int, default=200_000, help="knn: queries checked against the oracle")
     ap.add_argument("--probe-batch", type=int, default=5_000_000, help="dwithin_walks: probe points per batch")
     ap.add_argument("--serve-requests", type=int, default=96,
                     help="requests in the serving measurement (0 = skip it)")
@@ -77,6 +81,12 @@ def main():
         return
     if args.workload == "dwithin_walks":
         run_dwithin_walks(args)
+        return
+    if args.workload == "knn":
+        run_knn(args)
+        return
+    if args.workload == "poly_poly":
+        run_poly_poly(args)
         return
 
     from sjhip import datagen as D
@@ -469,7 +479,157 @@ def run_dwithin_walks(args):
     D.release_shared(wl["keep"])
 
 
-def cpu_baseline(strip, gb, gp, args):
+def run_knn(args):
+    """KNN join, k = 8: 20M query points x 200M object points (the 100M-point
+    mixture), exact neighbour lists with ties by id, FP64 distances.  One
+    step = the KNN kernel over all queries (device-resident queries and
+    outputs; the object tree is built once, with the build side)."""
+    from sjhip import datagen as D
+
+    k = 8
+    nq = int(round(20_000_000 * args.scale))
+    no = int(round(200_000_000 * args.scale))
+    t0 = time.time()
+    wl = D.workload_knn_parallel(nq, no, seed=args.seed + 3, workers=args.workers)
+    log(f"generated {no} objects and {nq} queries in {time.time() - t0:.1f}s")
+    from sjhip import gpuspatial as G
+
+    opts = G.GpuSpatialOptions(device_id=0)
+    tb = time.time()
+    rf = G.GpuSpatialRefiner(opts)
+    rf.init_build_schema("Z")
+    bdev = G.DeviceWkb.from_host(wl["build"])
+    rf.push_build_device(bdev)
+    rf.finish_building()
+    del bdev
+    qdev = G.DeviceWkb.from_host(wl["probe"])
+    ids, dd, ms0 = rf.knn_device(qdev, k)  # builds the object tree (resident from here on)
+    build_ms = (time.time() - tb) * 1000
+    log(f"build side resident (incl. object tree): {build_ms:.0f} ms; first call kernel {ms0:.1f} ms")
+    ids.free()
+    dd.free()
+    for i in range(args.warmup):
+        a, b, ms = rf.knn_device(qdev, k)
+        a.free()
+        b.free()
+    t = time.time()
+    kms = 0.0
+    for i in range(args.steps):
+        ids, dd, ms = rf.knn_device(qdev, k)
+        kms += ms
+        if i < args.steps - 1:
+            ids.free()
+            dd.free()
+    ms_per_step = (time.time() - t) * 1000 / args.steps
+    kms /= args.steps
+    log(f"timed: {ms_per_step:.1f} ms/step, kernel {kms:.1f} ms")
+    # exactness on a sample of queries
+    rng = np.random.default_rng(args.seed)
+    sample = np.sort(rng.choice(nq, size=min(args.knn_sample, nq), replace=False))
+    gi = ids.download(np.uint32, nq * k).reshape(nq, k)[sample]
+    gd = dd.download(np.float64, nq * k).reshape(nq, k)[sample]
+    ids.free()
+    dd.free()
+    import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    tc = time.time()
+    sq = D.take_rows(wl["probe"], sample)
+    oid, od = O.knn(O.Store(wl["build"].as_tuple()), O.Store(sq.as_tuple()), k, nthreads=threads)
+    secs = time.time() - tc
+    rel = np.abs(gd - od) / np.maximum(np.abs(od), 1e-300)
+    cpu = {"value": len(sample) / secs, "unit": "queries/s", "cores": threads, "kind": "port",
+           "sample": f"{len(sample)} random queries against all {no} objects (grid build included), {secs:.2f} s"}
+    # neighbours examined: a query's k results; bytes: 16 B per query + 12 B per neighbour written
+    out_bytes = 16 * nq + 12 * nq * k
+    out = {
+        "metric": "KNN join queries/s (k=8, 20M queries x 200M clustered points, device-resident)",
+        "value": nq / (ms_per_step / 1000), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic (seeded, generated on the host)",
+        "config": {"workload": "knn", "queries": nq, "objects": no, "k": k, "parallelism": "single-gpu"},
+        "kernel_ms": kms, "neighbours_per_s": nq * k / (kms / 1000),
+        "roofline": {"bound": "hbm", "achieved": out_bytes / (kms / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": out_bytes / (kms / 1e3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
+        "build_ms": build_ms, "cpu_baseline": cpu,
+        "sample_exact_ids": bool(np.array_equal(gi, oid)),
+        "sample_max_rel_distance_error": float(rel.max()) if rel.size else 0.0,
+    }
+    print(json.dumps(out), flush=True)
+    D.release_shared(wl["keep"])
+
+
+def run_poly_poly(args):
+    """Polygon-polygon ST_Intersects: two sets of clustered star polygons
+    (vertex counts log-uniform 4..1024, radius lognormal median 0.8,
+    centres from the 100M-point mixture), at --scale of 5M x 5M."""
+    from sjhip import datagen as D
+
+    na = int(round(5_000_000 * args.scale))
+    t0 = time.time()
+    a, b = D.workload_poly_poly(na, na, seed=args.seed + 2)
+    log(f"generated 2 x {na} polygons ({a.offsets[-1] / 1e9:.2f} + {b.offsets[-1] / 1e9:.2f} GB WKB) "
+        f"in {time.time() - t0:.1f}s")
+    strip = None
+    if not args.no_cpu:
+        import oracle as O
+        ra, rb = O.Store(a.as_tuple()).rects(True), O.Store(b.as_tuple()).rects(True)
+        ext = 1000.0 * np.sqrt(na / 5_000_000)
+        x0, w = 0.5 * ext, args.cpu_strip * ext / 8
+        ca = 0.5 * (ra[:, 0] + ra[:, 2])
+        cb = 0.5 * (rb[:, 0] + rb[:, 2])
+        brows = np.nonzero((ca >= x0) & (ca < x0 + w))[0]
+        prows = np.nonzero((cb >= x0) & (cb < x0 + w))[0]
+        strip = dict(brows=brows, prows=prows, bcol=D.take_rows(a, brows), pcol=D.take_rows(b, prows), x0=x0, w=w)
+    from sjhip import gpuspatial as G
+
+    opts = G.GpuSpatialOptions(device_id=0)
+    rf = G.GpuSpatialRefiner(opts)
+    rf.init_build_schema("Z")
+    adev = G.DeviceWkb.from_host(a)
+    rf.push_build_device(adev)
+    rf.finish_building()
+    del adev
+    ix = G.GpuSpatialIndex(opts)
+    ix.build_from_refiner(rf)
+    pdev = G.DeviceWkb.from_host(b)
+
+    def step():
+        return G.join_device(ix, rf, pdev, "intersects", sort_output=False, refine_order=args.refine_order,
+                             profile=True)
+
+    for i in range(args.warmup):
+        r = step()
+        log(f"warmup {i}: {r.size} pairs, {r.candidates} candidates, stages {list(r.res.stage_ms)}")
+        r.release()
+    t = time.time()
+    for i in range(args.steps):
+        r = step()
+        if i < args.steps - 1:
+            r.release()
+    ms_per_step = (time.time() - t) * 1000 / args.steps
+    pairs, cands, stage = int(r.size), int(r.candidates), list(r.res.stage_ms)
+    log(f"timed: {ms_per_step:.1f} ms/step, {pairs} pairs, {cands} candidates, stages {stage}")
+    cpu = None
+    if strip is not None:
+        gb, gp = r.to_host()
+        cpu = cpu_baseline(strip, gb, gp, args, predicate="intersects", what="polygons")
+    r.release()
+    out = {
+        "metric": "refined join pairs/s (ST_Intersects, clustered star polygons x star polygons, device-resident)",
+        "value": pairs / (ms_per_step / 1000), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded, generated on the host)",
+        "config": {"workload": "poly_poly", "polygons_each_side": na, "scale_of_5M": args.scale,
+                   "predicate": "ST_Intersects", "parallelism": "single-gpu"},
+        "pairs_per_step": pairs, "candidates_per_step": cands,
+        "stage_ms": {"decode_rects": stage[0], "morton": stage[1], "filter": stage[2], "group_by_polygon": stage[5],
+                     "refine": stage[3]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(strip, gb, gp, args, predicate="contains", what="polygons"):
     """The reference's TG refine (oracle/_ref) behind a grid filter on the
     strip sample; pairs/s on this host's cores.  Also checks that the GPU's
     full-size pairs restricted to the strip equal the CPU pairs."""
@@ -482,15 +642,16 @@ def cpu_baseline(strip, gb, gp, args):
         st_b, st_p = O.Store(bcol.as_tuple()), O.Store(pcol.as_tuple())
         br, pr = st_b.rects(True), st_p.rects(True)
         tb = O.TgColumn(bcol.as_tuple(), index=1, nthreads=threads)
-        tp = O.TgColumn(pcol.as_tuple(), index=0, nthreads=threads)
-        cb, cp = O.ref_join(tb, tp, br, pr, "contains", nthreads=threads)
+        tp = O.TgColumn(pcol.as_tuple(), index=1 if what == "polygons" and predicate == "intersects" else 0,
+                        nthreads=threads)
+        cb, cp = O.ref_join(tb, tp, br, pr, predicate, nthreads=threads)
     else:
         st_b, st_p = O.Store(bcol.as_tuple()), O.Store(pcol.as_tuple())
-        cb, cp = O.join(st_b, st_p, "contains", nthreads=threads)
+        cb, cp = O.join(st_b, st_p, predicate, nthreads=threads)
     secs = time.time() - t
     res = {"value": len(cb) / secs, "unit": "pairs/s", "cores": threads, "kind": kind,
            "sample": f"vertical strip x in [{strip['x0']:.1f}, {strip['x0'] + strip['w']:.1f}): "
-                     f"{len(strip['brows'])} polygons x {len(strip['prows'])} points, {len(cb)} pairs, "
+                     f"{len(strip['brows'])} polygons x {len(strip['prows'])} {'points' if predicate == 'contains' else 'polygons'}, {len(cb)} pairs, "
                      f"{secs:.2f} s (WKB parse + filter + refine)"}
     if gb is not None:
         bmap = np.full(int(gb.max()) + 1 if len(gb) else 1, -1, np.int64)

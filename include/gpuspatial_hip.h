@@ -219,6 +219,15 @@ int SjhipRefinerKnn(struct SedonaSpatialRefiner* self, const struct ArrowSchema*
                     const struct ArrowArray* probe_array, uint32_t k, uint32_t* out_build_indices,
                     double* out_distances);
 
+/* SjhipRefinerKnn for a probe column in ROCm device (or CPU) memory, with
+ * the outputs (n_probe x k, as for SjhipRefinerKnn) in device memory the
+ * caller allocated.  The object tree over the build points is built by the
+ * first KNN call and stays resident with the refiner.  *kernel_ms (if not
+ * NULL) receives the device time of the KNN kernel. */
+int SjhipRefinerKnnDevice(struct SedonaSpatialRefiner* self, const struct ArrowSchema* probe_schema,
+                          const struct ArrowDeviceArray* probe_array, uint32_t k, uint32_t* out_build_indices,
+                          double* out_distances, double* kernel_ms);
+
 /* Result of a device-resident join: device pointers owned by the library,
  * valid until release is called. */
 struct SjhipJoinResult {

@@ -255,6 +255,8 @@ struct RefinerImpl {
   std::vector<GeomColumn> batches;
   GeomColumn build;
   RingIndex ring_index;
+  KnnIndex knn;  // object tree of a point build side, built by the first KNN call
+  std::mutex knn_mu;
   bool built = false;
   std::mutex mu;
 };
@@ -271,6 +273,7 @@ int rf_clear(SedonaSpatialRefiner* self) {
     rf->batches.clear();
     rf->build = GeomColumn();
     rf->ring_index = RingIndex();
+    rf->knn = KnnIndex();
     rf->built = false;
   });
 }
@@ -402,6 +405,7 @@ void refiner_finish(RefinerImpl* rf) {
     concat_columns(rf->batches, rf->build, rf->stream);
   }
   RingIndexParams prm;
+  rf->knn = KnnIndex();
   build_ring_index(rf->build, rf->ring_index, prm, rf->stream);
   SJ_CHECK(hipStreamSynchronize(rf->stream));
   rf->built = true;
@@ -475,6 +479,7 @@ void rf_release(SedonaSpatialRefiner* self) {
     rf->batches.clear();
     rf->build = GeomColumn();
     rf->ring_index = RingIndex();
+    rf->knn = KnnIndex();
     if (rf->stream) {
       (void)hipStreamSynchronize(rf->stream);
       (void)hipStreamDestroy(rf->stream);
@@ -826,10 +831,53 @@ int SjhipRefinerKnn(struct SedonaSpatialRefiner* self, const struct ArrowSchema*
     if (m == 0) return;
     DevArray<uint32_t> ids(m, ss.s);
     DevArray<double> d(m, ss.s);
-    knn_points(rf->build, probe, (int)k, ids.p, d.p, ss.s);
+    {
+      std::lock_guard<std::mutex> g(rf->knn_mu);
+      knn_points(rf->build, rf->knn, probe, (int)k, ids.p, d.p, ss.s);
+    }
     SJ_CHECK(hipMemcpyAsync(out_build_indices, ids.p, m * 4, hipMemcpyDeviceToHost, ss.s));
     SJ_CHECK(hipMemcpyAsync(out_distances, d.p, m * 8, hipMemcpyDeviceToHost, ss.s));
     SJ_CHECK(hipStreamSynchronize(ss.s));
+  });
+  if (rc && rf) {
+    std::lock_guard<std::mutex> g(rf->mu);
+    rf->last_error = err;
+  }
+  return rc;
+}
+
+int SjhipRefinerKnnDevice(struct SedonaSpatialRefiner* self, const struct ArrowSchema* probe_schema,
+                          const struct ArrowDeviceArray* probe_array, uint32_t k, uint32_t* out_build_indices,
+                          double* out_distances, double* kernel_ms) {
+  RefinerImpl* rf = self ? (RefinerImpl*)self->private_data : nullptr;
+  std::string err;
+  int rc = guarded(&err, [&] {
+    if (!rf) throw Error("refiner is null or released");
+    if (!rf->built) throw Error("knn before finish_building");
+    if (!probe_array || !out_build_indices || !out_distances) throw Error("null knn argument");
+    bool dev = probe_array->device_type == ARROW_DEVICE_ROCM;
+    if (!dev && probe_array->device_type != ARROW_DEVICE_CPU)
+      throw Error("probe array must be in ROCm device memory or CPU memory");
+    use_device(rf->rt);
+    ScopedStream ss;
+    GeomColumn probe;
+    decode_wkb(wkb_input_from_arrow(probe_schema, &probe_array->array, dev), probe, ss.s);
+    KernelTimer kt;
+    if (kernel_ms) {
+      SJ_CHECK(hipEventCreate(&kt.start));
+      SJ_CHECK(hipEventCreate(&kt.stop));
+    }
+    {
+      std::lock_guard<std::mutex> g(rf->knn_mu);
+      knn_points(rf->build, rf->knn, probe, (int)k, out_build_indices, out_distances, ss.s, kernel_ms ? &kt : nullptr);
+    }
+    if (kernel_ms) {
+      float ms = 0;
+      if (probe.n_rows) SJ_CHECK(hipEventElapsedTime(&ms, kt.start, kt.stop));
+      *kernel_ms = ms;
+      (void)hipEventDestroy(kt.start);
+      (void)hipEventDestroy(kt.stop);
+    }
   });
   if (rc && rf) {
     std::lock_guard<std::mutex> g(rf->mu);

@@ -212,17 +212,23 @@ void launch_knn(const KnnTree& kt, const uint32_t* keys, const float bb[4], floa
 }  // namespace
 
 // bounds helper shared with bvh.hip semantics: centre bounds of valid rects
-void knn_points(const GeomColumn& build, const GeomColumn& probe, int k, uint32_t* out_ids, double* out_d,
-                hipStream_t s) {
+void knn_points(const GeomColumn& build, KnnIndex& index, const GeomColumn& probe, int k, uint32_t* out_ids,
+                double* out_d, hipStream_t s, KernelTimer* timer) {
   if (k < 1 || k > 32) throw Error("knn: k must be in 1..32");
   if (!build.points_only) throw Error("knn: build rows must be points");
   if (!probe.points_only) throw Error("knn: probe rows must be points");
   const int64_t nb = build.n_rows, nq = probe.n_rows;
   if (nq == 0) return;
-  DevArray<float4> rects(nb > 0 ? nb : 1, s);
-  if (nb > 0) hipLaunchKernelGGL(k_point_rects, blocks_for(nb), kBlock, 0, s, build.xy.p, build.type.p, nb, rects.p);
-  RectTree tree;
-  build_rect_tree(rects.p, nb, tree, s);
+  if (!index.built) {
+    DevArray<float4> rects(nb > 0 ? nb : 1, s);
+    if (nb > 0)
+      hipLaunchKernelGGL(k_point_rects, blocks_for(nb), kBlock, 0, s, build.xy.p, build.type.p, nb, rects.p);
+    build_rect_tree(rects.p, nb, index.tree, s);
+    SJ_CHECK(hipStreamSynchronize(s));
+    index.tree.detach_streams();  // the index outlives the call's stream
+    index.built = true;
+  }
+  const RectTree& tree = index.tree;
   const float* bb = tree.bb;
   float sx = bb[2] > bb[0] ? 65535.f / (bb[2] - bb[0]) : 0.f;
   float sy = bb[3] > bb[1] ? 65535.f / (bb[3] - bb[1]) : 0.f;
@@ -245,6 +251,7 @@ void knn_points(const GeomColumn& build, const GeomColumn& probe, int k, uint32_
   int64_t nvq = morton_order(qrects.p, nq, order, s);
   // invalid queries still need their output rows: run them unordered
   const uint32_t* ord = nvq == nq ? order.p : nullptr;
+  if (timer) timer->record_start(s);
   if (k <= 1) launch_knn<1>(kt, keys, bb, sx, sy, probe, ord, nq, k, out_ids, out_d, s);
   else if (k <= 2) launch_knn<2>(kt, keys, bb, sx, sy, probe, ord, nq, k, out_ids, out_d, s);
   else if (k <= 4) launch_knn<4>(kt, keys, bb, sx, sy, probe, ord, nq, k, out_ids, out_d, s);
@@ -252,6 +259,7 @@ void knn_points(const GeomColumn& build, const GeomColumn& probe, int k, uint32_
   else if (k <= 16) launch_knn<16>(kt, keys, bb, sx, sy, probe, ord, nq, k, out_ids, out_d, s);
   else launch_knn<32>(kt, keys, bb, sx, sy, probe, ord, nq, k, out_ids, out_d, s);
   SJ_CHECK(hipGetLastError());
+  if (timer) timer->record_stop(s);
   SJ_CHECK(hipStreamSynchronize(s));
 }
 

@@ -220,6 +220,15 @@ struct RectTree {
   mutable std::atomic<uint64_t> kept_per_query_x16{0};
   RectTree() = default;
   RectTree(RectTree&& o) noexcept { *this = std::move(o); }
+  // For a tree that outlives the stream it was built on (after that
+  // stream has been synchronised): its buffers are then freed on the null
+  // stream instead of the destroyed one.
+  void detach_streams() {
+    nodes.s = nullptr;
+    leaf_rects.s = nullptr;
+    leaf_ids.s = nullptr;
+    leaf_keys.s = nullptr;
+  }
   RectTree& operator=(RectTree&& o) noexcept {
     n_valid = o.n_valid;
     n_levels = o.n_levels;
@@ -310,9 +319,14 @@ void sort_pairs_by_build(PairBuf& pb, uint64_t n, int64_t n_build, const float4*
 // Sorts pairs by (probe, build) in place.
 void sort_pairs(uint32_t* b, uint32_t* p, uint64_t n, hipStream_t s);
 
-// KNN over point columns.
-void knn_points(const GeomColumn& build, const GeomColumn& probe, int k, uint32_t* out_ids_dev, double* out_d_dev,
-                hipStream_t s);
+// KNN over point columns.  The object tree over the build points is built
+// on first use and kept in `index` (resident with the build side).
+struct KnnIndex {
+  RectTree tree;
+  bool built = false;
+};
+void knn_points(const GeomColumn& build, KnnIndex& index, const GeomColumn& probe, int k, uint32_t* out_ids_dev,
+                double* out_d_dev, hipStream_t s, KernelTimer* timer = nullptr);
 
 // Checks that every pair's geometry types are supported by the relation
 // refine; throws otherwise.

@@ -189,7 +189,7 @@ EXPORTED = [
     "SjhipJoinResultCopyToHost", "SjhipIndexBuildFromRefiner", "SjhipRefinerPushBuildDevice", "SjhipLastError",
     "SjhipComputeRects", "SjhipDeviceAlloc", "SjhipDeviceFree", "SjhipCopyToDevice", "SjhipCopyToHost",
     "SjhipSetDevice", "SjhipCopyKernel",
-    "SjhipServerCreate", "SjhipServerSubmit", "SjhipServerWait", "SjhipServerDestroy",
+    "SjhipServerCreate", "SjhipServerSubmit", "SjhipServerWait", "SjhipServerDestroy", "SjhipRefinerKnnDevice",
     "SjhipHostRaycast", "SjhipHostRaycastF32", "SjhipHostCollinear", "SjhipHostSegmentIntersects",
     "SjhipHostOrient2d", "SjhipHostHypot", "SjhipHostCoordToLine", "SjhipHostRaycastBatch", "SjhipHostHypotBatch",
 ]
@@ -223,6 +223,8 @@ def lib():
         L.SjhipServerWait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(SjhipJoinResult),
                                       ctypes.POINTER(SjhipRequestTiming)]
         L.SjhipServerDestroy.argtypes = [ctypes.c_void_p]
+        L.SjhipRefinerKnnDevice.argtypes = [_RF, _SCH, ctypes.POINTER(ArrowDeviceArray), ctypes.c_uint32,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.SjhipIndexBuildFromRefiner.argtypes = [_IX, _RF]
         L.SjhipRefinerPushBuildDevice.argtypes = [_RF, _SCH, ctypes.POINTER(ArrowDeviceArray)]
         L.SjhipLastError.restype = ctypes.c_char_p

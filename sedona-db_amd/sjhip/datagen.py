@@ -461,6 +461,18 @@ def points_parallel(seed, n, extent, centers=None, sigma=2.0, frac_clustered=0.8
     return WkbColumn(np.arange(n + 1, dtype=np.int64) * 21, data), xy, (shm_d, shm_x)
 
 
+def workload_knn_parallel(n_queries=20_000_000, n_objects=200_000_000, seed=5, workers=16):
+    """KNN k=8: 20M query points vs 200M object points, both drawn from the
+    100M-point mixture (64 Gaussians, sigma 2, centres U[0,1000]^2, 20 %
+    uniform).  Returns dict(build, probe, probe_xy, build_xy, extent, keep)."""
+    extent = 1000.0 * np.sqrt(n_objects / 200_000_000)
+    rng = np.random.default_rng(seed)
+    ccent = rng.uniform(0, extent, size=(64, 2))
+    obj, oxy, k1 = points_parallel(seed + 1000, n_objects, extent, centers=ccent, workers=workers)
+    q, qxy, k2 = points_parallel(seed + 2000, n_queries, extent, centers=ccent, workers=workers)
+    return dict(build=obj, probe=q, build_xy=oxy, probe_xy=qxy, extent=extent, keep=k1 + k2)
+
+
 def workload_pip_zipf_parallel(n_points=100_000_000, n_polys=1_000_000, seed=2, workers=16, max_vertices=4096):
     """The 100M clustered points x 1M Zipf(1.2) stars workload, generated in
     parallel.  Returns dict(build, probe, centers, probe_xy, keep)."""

@@ -415,6 +415,18 @@ class GpuSpatialRefiner:
                                         _ptr32(ids), d.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), self._err)
         return ids, d
 
+    def knn_device(self, probe, k):
+        """KNN of a DeviceWkb probe column with device-resident outputs:
+        (ids DeviceBuffer, distances DeviceBuffer, kernel ms)."""
+        n = probe.dev.array.length
+        ids = DeviceBuffer(max(1, n * k * 4))
+        d = DeviceBuffer(max(1, n * k * 8))
+        ms = ctypes.c_double()
+        _check(self.lib.SjhipRefinerKnnDevice(ctypes.byref(self.rf), ctypes.byref(probe.schema),
+                                              ctypes.byref(probe.dev), int(k), ids.ptr, d.ptr, ctypes.byref(ms)),
+               self._err)
+        return ids, d, ms.value
+
     def release(self):
         if self.rf.private_data:
             self.rf.release(ctypes.byref(self.rf))

@@ -262,6 +262,30 @@ def test_knn_exact_with_id_ties(k, opts):
     assert np.array_equal(d.view(np.uint64), od.view(np.uint64))
 
 
+def test_knn_device_outputs_and_resident_tree(opts):
+    """The device-output KNN entry point, and repeated calls over the
+    object tree kept by the first call, give the oracle's exact lists."""
+    rng = np.random.default_rng(23)
+    obj = D.clustered_xy(rng, 300_000, 100.0, n_clusters=16)
+    q = D.clustered_xy(rng, 30_000, 100.0, n_clusters=16)
+    bcol = D.points_wkb(obj[:, 0], obj[:, 1])
+    qcol = D.points_wkb(q[:, 0], q[:, 1])
+    rf = G.GpuSpatialRefiner(opts)
+    rf.init_build_schema(bcol.to_arrow())
+    rf.push_build(bcol.to_arrow())
+    rf.finish_building()
+    oid, od = O.knn(O.Store(bcol.as_tuple()), O.Store(qcol.as_tuple()), 8, nthreads=8)
+    for _ in range(2):
+        ids, db, ms = rf.knn_device(G.DeviceWkb.from_host(qcol), 8)
+        gi = ids.download(np.uint32, len(q) * 8).reshape(len(q), 8)
+        gd = db.download(np.float64, len(q) * 8).reshape(len(q), 8)
+        assert np.array_equal(gi, oid)
+        assert np.array_equal(gd.view(np.uint64), od.view(np.uint64))
+        assert ms > 0
+    hi, hd = rf.knn(qcol.to_arrow(), 8)
+    assert np.array_equal(hi, oid)
+
+
 def test_decode_errors_name_the_row(opts):
     rf = G.GpuSpatialRefiner(opts)
     for bad in C.invalid_wkbs():
