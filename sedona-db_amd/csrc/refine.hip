@@ -129,6 +129,7 @@ struct Ctx {
   GeomView B;         // build column
   RingIndexView R;    // build ring index
   GeomView P;         // probe column
+  RingIndexView PR;   // probe ring index (polygon probes of polygon x polygon; else empty)
 };
 
 // tg_ring_contains_point for a build-side ring (precomputed rect and
@@ -326,29 +327,76 @@ __device__ __forceinline__ void walk_tree(TreeWalk& w) {
   }
 }
 
-// tg_ring_contains_point over a build ring given its rect, vertex run,
-// FP32 frame (origin, bound E) and segment tree (toff < 0: none).
+// Depth-first walk over a ring's segment tree (the layout walk_tree
+// uses) visiting the leaves whose path of boxes passes `pass`; stops and
+// returns true as soon as `leaf(leaf_index)` does.
+template <class Pass, class Leaf>
+__device__ __forceinline__ bool walk_any(const float4* box, const TreeShape& shape, Pass pass, Leaf leaf) {
+  const int top = shape.levels - 1;
+  const int ntop = level_cnt(shape, top);
+  const float4* btop = box + level_off(shape, top);
+  for (int c0 = 0; c0 < ntop; c0 += kSegFanout) {
+    uint32_t m0 = 0;
+    const int n0 = min(kSegFanout, ntop - c0);
+    for (int k = 0; k < n0; k++)
+      if (pass(btop[c0 + k])) m0 |= 1u << k;
+    uint64_t masks = (uint64_t)m0 << (8 * top);
+    int lvl = top, base = c0;
+    while (masks) {
+      uint32_t m = (uint32_t)(masks >> (8 * lvl)) & 0xffu;
+      if (m == 0) {
+        lvl++;
+        base = (base >> 3) & ~(kSegFanout - 1);
+        continue;
+      }
+      int k = __ffs(m) - 1;
+      masks &= ~(1ull << (8 * lvl + k));
+      int node = base + k;
+      if (lvl == 0) {
+        if (leaf(node)) return true;
+        continue;
+      }
+      int cb = node * kSegFanout;
+      int n = min(kSegFanout, level_cnt(shape, lvl - 1) - cb);
+      const float4* bc = box + level_off(shape, lvl - 1) + cb;
+      uint32_t cm = 0;
+      for (int q = 0; q < n; q++)
+        if (pass(bc[q])) cm |= 1u << q;
+      if (cm) {
+        lvl--;
+        base = cb;
+        masks |= (uint64_t)cm << (8 * lvl);
+      }
+    }
+  }
+  return false;
+}
+
+// tg_ring_contains_point over a ring given its rect, vertex run (FP64 xy
+// and FP32 ring-local q32, both starting at the ring's first vertex), FP32
+// frame (origin, bound E) and segment tree (nullptr: none).
 template <int PREC>
-__device__ __forceinline__ bool ring_contains_at(const Ctx& c, const double4& rb, int64_t v0, int64_t npts, double2 o,
-                                                 float E, int64_t toff, double px, double py, bool allow_on_edge,
-                                                 uint64_t& tests, uint64_t& fallbacks) {
+__device__ __forceinline__ bool ring_contains_run(const double2* xy, const float2* q32, const float4* tree,
+                                                  int64_t npts, const double4& rb, double2 o, float E, double px,
+                                                  double py, bool allow_on_edge, uint64_t& tests,
+                                                  uint64_t& fallbacks) {
   if (!rect_covers(rb, px, py)) return false;
   RingRef r;
-  r.v0 = v0;
+  r.v0 = 0;
   r.npts = npts;
-  r.xy = c.B.xy + v0;
+  r.xy = xy;
   int64_t ns = r.npts >= 3 ? r.npts - 1 : 0;
   int on = 0, in = 0;
   float pxl = (float)(px - o.x), pyl = (float)(py - o.y);
-  if (toff >= 0) {
+  if (tree) {
     TreeWalk w;
-    w.box = c.R.seg_tree + toff;
+    w.box = tree;
     w.shape = seg_tree_shape((int)ns);
     w.pxl = pxl;
     w.pyl = pyl;
     w.m = 2.0f * E;
-    w.xy = r.xy;
-    w.q32 = c.R.xy32 + r.v0;
+    w.xy = xy;
+    w.q32 = q32;
     w.E = E;
     w.px = px;
     w.py = py;
@@ -363,11 +411,21 @@ __device__ __forceinline__ bool ring_contains_at(const Ctx& c, const double4& rb
     return w.on ? allow_on_edge : (w.in != 0);
   }
   if constexpr (PREC != 1) {
-    ring_segments_f32<PREC == kPrecDeferred>(r, c.R.xy32, E, pxl, pyl, px, py, 0, ns, 1, on, in, tests, fallbacks);
+    ring_segments_f32<PREC == kPrecDeferred>(r, q32, E, pxl, pyl, px, py, 0, ns, 1, on, in, tests, fallbacks);
   } else {
     ring_segments_f64(r, 0, ns, 1, px, py, on, in, tests);
   }
   return on ? allow_on_edge : (in != 0);
+}
+
+// The same over a build ring (offsets into the build column and its
+// ring index; toff < 0: no segment tree).
+template <int PREC>
+__device__ __forceinline__ bool ring_contains_at(const Ctx& c, const double4& rb, int64_t v0, int64_t npts, double2 o,
+                                                 float E, int64_t toff, double px, double py, bool allow_on_edge,
+                                                 uint64_t& tests, uint64_t& fallbacks) {
+  return ring_contains_run<PREC>(c.B.xy + v0, c.R.xy32 + v0, toff >= 0 ? c.R.seg_tree + toff : nullptr, npts, rb, o,
+                                 E, px, py, allow_on_edge, tests, fallbacks);
 }
 
 template <int PREC>
@@ -496,6 +554,96 @@ __device__ bool ring_intersects_ring(RingRef ring, RingRef other) {
   return false;
 }
 
+// A ring of a column with a ring index: the vertex runs, segment tree and
+// FP32 frame of tg_ring_intersects_ring's operands.
+struct IRing {
+  const double2* xy;
+  const float2* q32;
+  const float4* tree;  // nullptr: none
+  int64_t npts;
+  double4 rect;
+  double2 o;
+  float E;
+};
+
+__device__ __forceinline__ IRing iring(const GeomView& g, const RingIndexView& R, int64_t ring) {
+  IRing r;
+  const int64_t v0 = g.pt_off[ring];
+  r.xy = g.xy + v0;
+  r.q32 = R.xy32 + v0;
+  r.npts = g.pt_off[ring + 1] - v0;
+  const int64_t toff = R.seg_tree_off ? R.seg_tree_off[ring] : -1;
+  r.tree = toff >= 0 ? R.seg_tree + toff : nullptr;
+  r.rect = R.ring_rect[ring];
+  r.o = R.origin[ring];
+  r.E = R.err[ring];
+  return r;
+}
+
+// Whether at least two segments of r intersect segment (a, b) in the
+// reference's segment x segment semantics (its exact bounding-box test
+// comes first, so the tree's boxes, within 2E of the exact ones, prune
+// conservatively with a margin of 4E).
+__device__ __forceinline__ bool ring_two_crossings(const double2* xy, int64_t npts, const float4* tree, float E,
+                                                   double2 o, double ax, double ay, double bx, double by) {
+  const int64_t ns = npts >= 3 ? npts - 1 : 0;
+  int count = 0;
+  if (!tree || !(E < INFINITY)) {
+    for (int64_t i = 0; i < ns && count < 2; i++) {
+      double2 c = xy[i], d = xy[i + 1];
+      if (segment_intersects(ax, ay, bx, by, c.x, c.y, d.x, d.y)) count++;
+    }
+    return count >= 2;
+  }
+  const float m = 4.0f * E;
+  const float x0 = (float)(fmin(ax, bx) - o.x) - m, x1 = (float)(fmax(ax, bx) - o.x) + m;
+  const float y0 = (float)(fmin(ay, by) - o.y) - m, y1 = (float)(fmax(ay, by) - o.y) + m;
+  const int nsi = (int)ns;
+  auto pass = [=](const float4& b) { return b.x <= x1 && b.z >= x0 && b.y <= y1 && b.w >= y0; };
+  auto leaf = [&count, xy, nsi, ax, ay, bx, by](int lf) {
+    const int s0 = lf * kSegLeaf, s1 = min(s0 + kSegLeaf, nsi);
+    for (int j = s0; j < s1; j++) {
+      double2 c = xy[j], d = xy[j + 1];
+      if (segment_intersects(ax, ay, bx, by, c.x, c.y, d.x, d.y)) {
+        count++;
+        if (count >= 2) return true;
+      }
+    }
+    return false;
+  };
+  return walk_any(tree, seg_tree_shape(nsi), pass, leaf);
+}
+
+// tg_ring_intersects_ring (tg.c:3166-3190) over indexed rings: some
+// segment of the smaller-rect ring has an endpoint covered by the other
+// ring or crosses it at least twice (tg.c:3089-3111).  The point tests walk
+// the segment tree (certified FP32, FP64 for uncertain segments) and the
+// crossing counts query it, instead of scanning every segment; the
+// decision is the same.
+__device__ bool iring_intersects_iring(IRing ring, IRing other) {
+  if (ring.npts < 3 || other.npts < 3) return false;
+  double4 a = ring.rect, b = other.rect;
+  if (b.x > a.z || b.z < a.x || b.y > a.w || b.w < a.y) return false;
+  if ((b.z - b.x) * (b.w - b.y) > (a.z - a.x) * (a.w - a.y)) {
+    IRing t = ring; ring = other; other = t;
+    a = ring.rect;
+  }
+  uint64_t tests = 0, fallbacks = 0;
+  const int64_t ns = other.npts - 1;
+  for (int64_t i = 0; i < ns; i++) {
+    double2 p = other.xy[i], q = other.xy[i + 1];
+    double sx0 = fmin0(p.x, q.x), sy0 = fmin0(p.y, q.y), sx1 = fmax0(p.x, q.x), sy1 = fmax0(p.y, q.y);
+    if (a.x > sx1 || a.z < sx0 || a.y > sy1 || a.w < sy0) continue;
+    if (ring_contains_run<0>(ring.xy, ring.q32, ring.tree, ring.npts, a, ring.o, ring.E, p.x, p.y, true, tests,
+                             fallbacks) ||
+        ring_contains_run<0>(ring.xy, ring.q32, ring.tree, ring.npts, a, ring.o, ring.E, q.x, q.y, true, tests,
+                             fallbacks))
+      return true;
+    if (ring_two_crossings(ring.xy, ring.npts, ring.tree, ring.E, ring.o, p.x, p.y, q.x, q.y)) return true;
+  }
+  return false;
+}
+
 // tg_ring_contains_ring(hole, ring, false) as used by poly x poly
 // intersects: every vertex strictly inside and no segment crossing.
 __device__ bool ring_contains_ring_strict(const RingRef& a, const RingRef& b) {
@@ -520,14 +668,18 @@ __device__ bool ring_contains_ring_strict(const RingRef& a, const RingRef& b) {
   return true;
 }
 
-__device__ bool poly_intersects_poly(const GeomView& A, int64_t pa, const GeomView& Bv, int64_t pb) {
+__device__ bool poly_intersects_poly(const GeomView& A, int64_t pa, const GeomView& Bv, int64_t pb,
+                                     const RingIndexView* RA = nullptr, const RingIndexView* RB = nullptr) {
   int64_t a0 = A.ring_off[pa], a1 = A.ring_off[pa + 1];
   int64_t b0 = Bv.ring_off[pb], b1 = Bv.ring_off[pb + 1];
   if (a0 == a1 || b0 == b1) return false;
   RingRef ae = ring_at(A, a0), be = ring_at(Bv, b0);
-  if (a1 - a0 == 1 && b1 - b0 == 1) return ring_intersects_ring(ae, be);
+  const bool indexed = RA && RB && RA->ring_rect && RB->ring_rect;
+  if (a1 - a0 == 1 && b1 - b0 == 1)
+    return indexed ? iring_intersects_iring(iring(A, *RA, a0), iring(Bv, *RB, b0)) : ring_intersects_ring(ae, be);
   if (ae.npts < 3 || be.npts < 3) return false;
-  if (!ring_intersects_ring(be, ae)) return false;
+  if (!(indexed ? iring_intersects_iring(iring(Bv, *RB, b0), iring(A, *RA, a0)) : ring_intersects_ring(be, ae)))
+    return false;
   for (int64_t h = a0 + 1; h < a1; h++)
     if (ring_contains_ring_strict(ring_at(A, h), be)) return false;
   for (int64_t h = b0 + 1; h < b1; h++)
@@ -584,7 +736,7 @@ __device__ int eval_relation(const Ctx& c, int64_t b, int64_t p, int pred, int p
   if (is_polyish(tb) && is_polyish(tp) && pred == kIntersects) {
     for (int64_t qa = c.B.part_off[b]; qa < c.B.part_off[b + 1]; qa++)
       for (int64_t qb = c.P.part_off[p]; qb < c.P.part_off[p + 1]; qb++)
-        if (poly_intersects_poly(c.B, qa, c.P, qb)) return 1;
+        if (poly_intersects_poly(c.B, qa, c.P, qb, &c.R, &c.PR)) return 1;
     return 0;
   }
   return -1;
@@ -848,51 +1000,6 @@ __device__ double distance_to_point(const GeomView& g, int64_t row, double px, d
     m = rmin(m, d);
   }
   return m;
-}
-
-// Depth-first walk over a ring's segment tree (the layout walk_tree
-// uses) visiting the leaves whose path of boxes passes `pass`; stops and
-// returns true as soon as `leaf(leaf_index)` does.
-template <class Pass, class Leaf>
-__device__ __forceinline__ bool walk_any(const float4* box, const TreeShape& shape, Pass pass, Leaf leaf) {
-  const int top = shape.levels - 1;
-  const int ntop = level_cnt(shape, top);
-  const float4* btop = box + level_off(shape, top);
-  for (int c0 = 0; c0 < ntop; c0 += kSegFanout) {
-    uint32_t m0 = 0;
-    const int n0 = min(kSegFanout, ntop - c0);
-    for (int k = 0; k < n0; k++)
-      if (pass(btop[c0 + k])) m0 |= 1u << k;
-    uint64_t masks = (uint64_t)m0 << (8 * top);
-    int lvl = top, base = c0;
-    while (masks) {
-      uint32_t m = (uint32_t)(masks >> (8 * lvl)) & 0xffu;
-      if (m == 0) {
-        lvl++;
-        base = (base >> 3) & ~(kSegFanout - 1);
-        continue;
-      }
-      int k = __ffs(m) - 1;
-      masks &= ~(1ull << (8 * lvl + k));
-      int node = base + k;
-      if (lvl == 0) {
-        if (leaf(node)) return true;
-        continue;
-      }
-      int cb = node * kSegFanout;
-      int n = min(kSegFanout, level_cnt(shape, lvl - 1) - cb);
-      const float4* bc = box + level_off(shape, lvl - 1) + cb;
-      uint32_t cm = 0;
-      for (int q = 0; q < n; q++)
-        if (pass(bc[q])) cm |= 1u << q;
-      if (cm) {
-        lvl--;
-        base = cb;
-        masks |= (uint64_t)cm << (8 * lvl);
-      }
-    }
-  }
-  return false;
 }
 
 // Segment j of a vertex run is a term <= d of the reference distance (on
@@ -1384,6 +1491,15 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
       stats->fallback_tests += h[2];
     }
     return cnt;
+  }
+  // polygon x polygon: the probe's rings get segment trees too
+  const uint32_t polys = (1u << kPolygon) | (1u << kMultiPolygon);
+  RingIndex probe_ri;
+  if (predicate == kIntersects && (build.type_mask & polys) && (probe.type_mask & polys) && !probe.points_only &&
+      c.R.ring_rect != nullptr) {
+    RingIndexParams prm;
+    build_ring_index(probe, probe_ri, prm, s);
+    c.PR = probe_ri.view();
   }
   bool use_long = !build.points_only && probe.points_only && c.R.ring_rect != nullptr &&
                   (predicate == kIntersects || predicate == kCovers || predicate == kContains);

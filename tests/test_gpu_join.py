@@ -230,6 +230,20 @@ def test_dwithin_zipf_polygons_vs_oracle(d, refine_order, opts):
     assert len(gb) > 100
 
 
+def test_poly_poly_large_rings_vs_oracle(opts):
+    """Polygon x polygon intersects with rings of up to 1024 vertices (both
+    sides get segment trees): the indexed crossing counts and point tests
+    decide every pair as the oracle does."""
+    a, b = D.workload_poly_poly(3_000, 3_000, seed=29)
+    say("polypoly large", int(a.offsets[-1]), int(b.offsets[-1]))
+    ix, rf = build_pair(a.to_arrow(), opts)
+    r = G.join_device(ix, rf, G.DeviceWkb.from_host(b), "intersects")
+    gb, gp = r.to_host()
+    ob, op = O.join(O.Store(a.as_tuple()), O.Store(b.as_tuple()), "intersects", nthreads=8)
+    assert np.array_equal(gb, ob) and np.array_equal(gp, op), (len(gb), len(ob))
+    assert len(gb) > 1000
+
+
 def test_dwithin_polygons_and_refine_distance_abi(opts):
     b, p = D.workload_pip_uniform(20_000, 200, seed=17, extent=100.0 * np.sqrt(200 / 10_000))
     ix, rf = build_pair(b.to_arrow(), opts)
