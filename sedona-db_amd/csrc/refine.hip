@@ -831,6 +831,67 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SJ_PIP_W
   }
 }
 
+// tg_ring_intersects_ring for one candidate with the whole wavefront: the
+// lanes take the smaller-rect ring's segments with stride 64 and the
+// wavefront votes.  Every lane returns the same value.
+__device__ bool wave_iring_intersects(IRing ring, IRing other, int lane) {
+  if (ring.npts < 3 || other.npts < 3) return false;
+  double4 a = ring.rect, b = other.rect;
+  if (b.x > a.z || b.z < a.x || b.y > a.w || b.w < a.y) return false;
+  if ((b.z - b.x) * (b.w - b.y) > (a.z - a.x) * (a.w - a.y)) {
+    IRing t = ring; ring = other; other = t;
+    a = ring.rect;
+  }
+  uint64_t tests = 0, fallbacks = 0;
+  const int64_t ns = other.npts - 1;
+  for (int64_t base = 0; base < ns; base += 64) {
+    const int64_t i = base + lane;
+    bool hit = false;
+    if (i < ns) {
+      double2 p = other.xy[i], q = other.xy[i + 1];
+      double sx0 = fmin0(p.x, q.x), sy0 = fmin0(p.y, q.y), sx1 = fmax0(p.x, q.x), sy1 = fmax0(p.y, q.y);
+      if (!(a.x > sx1 || a.z < sx0 || a.y > sy1 || a.w < sy0))
+        hit = ring_contains_run<0>(ring.xy, ring.q32, ring.tree, ring.npts, a, ring.o, ring.E, p.x, p.y, true, tests,
+                                   fallbacks) ||
+              ring_contains_run<0>(ring.xy, ring.q32, ring.tree, ring.npts, a, ring.o, ring.E, q.x, q.y, true, tests,
+                                   fallbacks) ||
+              ring_two_crossings(ring.xy, ring.npts, ring.tree, ring.E, ring.o, p.x, p.y, q.x, q.y);
+    }
+    if (__ballot(hit)) return true;
+  }
+  return false;
+}
+
+// Polygon x polygon ST_Intersects with rings indexed on both sides, one
+// wavefront per candidate (single-ring parts voted by the wavefront;
+// parts with holes decided by every lane alike).
+__global__ __launch_bounds__(kBlock) void k_refine_polypoly(Ctx c, const uint32_t* cb, const uint32_t* cp,
+                                                            uint64_t n, uint32_t* ob, uint32_t* op, uint64_t cap,
+                                                            unsigned long long* counter) {
+  SJ_WAVE_OUT(wo);
+  const int lane = threadIdx.x & 63;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t i = wave; i < n; i += nwaves) {
+    const uint32_t b = cb[i], p = cp[i];
+    bool keep = false;
+    if (is_polyish(c.B.type[b]) && is_polyish(c.P.type[p])) {
+      for (int64_t qa = c.B.part_off[b]; qa < c.B.part_off[b + 1] && !keep; qa++)
+        for (int64_t qb = c.P.part_off[p]; qb < c.P.part_off[p + 1] && !keep; qb++) {
+          const int64_t a0 = c.B.ring_off[qa], a1 = c.B.ring_off[qa + 1];
+          const int64_t b0 = c.P.ring_off[qb], b1 = c.P.ring_off[qb + 1];
+          if (a0 == a1 || b0 == b1) continue;
+          if (a1 - a0 == 1 && b1 - b0 == 1)
+            keep = wave_iring_intersects(iring(c.B, c.R, a0), iring(c.P, c.PR, b0), lane);
+          else
+            keep = poly_intersects_poly(c.B, qa, c.P, qb, &c.R, &c.PR);
+        }
+    }
+    emit(wo, lane == 0 && keep, b, p, ob, op, cap, counter);
+  }
+  wave_flush(wo, ob, op, cap, counter);
+}
+
 // One lane per candidate; long-ring candidates are deferred into a list.
 __global__ __launch_bounds__(kBlock) void k_refine_lane(Ctx c, int pred, int precision, const uint32_t* cb,
                                                         const uint32_t* cp, uint64_t n, uint32_t* ob, uint32_t* op,
@@ -1500,6 +1561,18 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
     RingIndexParams prm;
     build_ring_index(probe, probe_ri, prm, s);
     c.PR = probe_ri.view();
+  }
+  if (c.PR.ring_rect != nullptr && (build.type_mask & ~polys) == 0 && (probe.type_mask & ~polys) == 0) {
+    if (timer) timer->record_start(s);
+    const uint64_t waves = std::min<uint64_t>(n, 1u << 18);
+    hipLaunchKernelGGL(k_refine_polypoly, (unsigned int)((waves * 64 + kBlock - 1) / kBlock), kBlock, 0, s, c, cb, cp,
+                       n, out.b.p, out.p.p, (uint64_t)out.b.n, out.counter.p);
+    SJ_CHECK(hipGetLastError());
+    if (timer) timer->record_stop(s);
+    unsigned long long cnt = 0;
+    SJ_CHECK(hipMemcpyAsync(&cnt, out.counter.p, 8, hipMemcpyDeviceToHost, s));
+    SJ_CHECK(hipStreamSynchronize(s));
+    return cnt;
   }
   bool use_long = !build.points_only && probe.points_only && c.R.ring_rect != nullptr &&
                   (predicate == kIntersects || predicate == kCovers || predicate == kContains);
