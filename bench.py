@@ -61,6 +61,8 @@ def main():
                          "knn: 20M queries x 200M clustered points, k=8; "
                          "poly_poly: 2 x 5M clustered stars, ST_Intersects (use --scale)")
     ap.add_argument("--knn-sample", type=int, default=200_000, help="knn: queries checked against the oracle")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="pip_zipf_contains: skip the autotuner (use --refine-order and the default kernels)")
     ap.add_argument("--probe-batch", type=int, default=5_000_000, help="dwithin_walks: probe points per batch")
     ap.add_argument("--serve-requests", type=int, default=96,
                     help="requests in the serving measurement (0 = skip it)")
@@ -135,10 +137,16 @@ def main():
     build_ms = (time.time() - tb) * 1000
     log(f"build side resident: {build_ms:.0f} ms (includes host->device copy)")
     pdev = G.DeviceWkb.from_host(probe)
+    tuned = None
+    if args.autotune:
+        ta = time.time()
+        tuned = G.autotune(ix, rf, pdev.slice(0, min(n_points, 10_000_000)), "contains")
+        tuned["tune_s"] = time.time() - ta
+        log(f"autotune (10M-point sample): {tuned}")
 
     def step(precision, profile=True):
         return G.join_device(ix, rf, pdev, "contains", precision=precision, sort_output=False, sort_probes=True,
-                             refine_order=args.refine_order,
+                             refine_order=-1 if args.autotune else args.refine_order,
                              profile=profile)
 
     for i in range(args.warmup):
@@ -247,6 +255,7 @@ def main():
         "fallback_rate": fb / max(1, seg_tests),
         "build_ms": build_ms,
         "plain_fp64": plain,
+        "autotune": tuned,
         "serving": serving,
     }
     if plain and args.precision == 0:

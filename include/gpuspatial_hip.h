@@ -259,7 +259,8 @@ struct SjhipJoinOptions {
   int32_t refine_order; /* 0 = refine candidates in filter order, 1 = group
                            them by build row first (polygon-major), 2 =
                            (DWithin, point probes) decide the distance inside
-                           the filter walk, writing no candidates */
+                           the filter walk, writing no candidates; -1 = the
+                           order SjhipAutotune chose (else 1, DWithin 0) */
   void* stream;        /* hipStream_t, or NULL for a stream of the call */
 };
 
@@ -273,6 +274,27 @@ int SjhipJoinDevice(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefine
                     const struct ArrowSchema* probe_schema,
                     const struct ArrowDeviceArray* probe_array,
                     const struct SjhipJoinOptions* options, struct SjhipJoinResult* out);
+
+/* Parameters chosen by SjhipAutotune. */
+struct SjhipTuneResult {
+  int32_t leaf;         /* rect tree leaf size (4, 8 or 16 rects) */
+  int32_t refine_order; /* candidate order before the refine (SjhipJoinOptions.refine_order) */
+  int32_t pip_waves;    /* resident waves per SIMD of the point-in-polygon refine */
+  int32_t evaluations;  /* configurations timed */
+  double best_ms;       /* wall time of the chosen configuration on the sample */
+};
+
+/* Autotuner (no counterpart in the reference): times joins of a probe
+ * sample against the built index and refiner under the options' predicate
+ * and distance, choosing in turn the rect tree's leaf size (the tree is
+ * rebuilt), the candidate order before the refine (per kind of refine:
+ * point in polygon, distance, other) and the point-in-polygon kernel's
+ * resident waves.  The choices stay with the index: joins whose
+ * options.refine_order is -1 use the tuned order.  Must not run
+ * concurrently with joins on the same index. */
+int SjhipAutotune(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner* refiner,
+                  const struct ArrowSchema* probe_schema, const struct ArrowDeviceArray* probe_sample,
+                  const struct SjhipJoinOptions* options, struct SjhipTuneResult* out);
 
 /* Request loop over a resident index and refiner (the serving path of the
  * GPU operator: rust/sedona-spatial-join-gpu/src/index/gpu_spatial_index.rs:

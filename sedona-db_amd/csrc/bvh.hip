@@ -153,7 +153,7 @@ void sort_keys(DevArray<uint32_t>& keys, DevArray<uint32_t>& ids, int64_t n, hip
 
 }  // namespace
 
-void build_rect_tree(const float4* rects, int64_t n, RectTree& t, hipStream_t s) {
+void build_rect_tree(const float4* rects, int64_t n, RectTree& t, hipStream_t s, int leaf) {
   t.level_off.clear();
   t.level_cnt.clear();
   t.n_levels = 0;
@@ -180,7 +180,9 @@ void build_rect_tree(const float4* rects, int64_t n, RectTree& t, hipStream_t s)
   for (int k = 0; k < 4; k++) t.bb[k] = bb[k];
   // level sizes
   std::vector<int64_t> cnt;
-  int64_t c = (nv + RectTree::LEAF - 1) / RectTree::LEAF;
+  if (leaf != 4 && leaf != 8 && leaf != 16) throw Error("rect tree leaf size must be 4, 8 or 16");
+  t.leaf = leaf;
+  int64_t c = (nv + leaf - 1) / leaf;
   cnt.push_back(c);
   while (c > RectTree::FANOUT) {
     c = (c + RectTree::FANOUT - 1) / RectTree::FANOUT;
@@ -195,7 +197,7 @@ void build_rect_tree(const float4* rects, int64_t n, RectTree& t, hipStream_t s)
   t.level_cnt = cnt;
   t.n_levels = (int)cnt.size();
   t.nodes.alloc(tot, s);
-  hipLaunchKernelGGL(k_merge, blocks_for(cnt[0]), kBlock, 0, s, t.leaf_rects.p, nv, RectTree::LEAF,
+  hipLaunchKernelGGL(k_merge, blocks_for(cnt[0]), kBlock, 0, s, t.leaf_rects.p, nv, leaf,
                      t.nodes.p + t.level_off[0], cnt[0]);
   for (size_t l = 1; l < cnt.size(); l++)
     hipLaunchKernelGGL(k_merge, blocks_for(cnt[l]), kBlock, 0, s, t.nodes.p + t.level_off[l - 1], cnt[l - 1],
@@ -220,8 +222,15 @@ uint64_t filter_pairs(const RectTree& t, const float4* q, int64_t nq, const uint
   for (int attempt = 0; attempt < 2; attempt++) {
     SJ_CHECK(hipMemsetAsync(pb.counter.p, 0, 8, s));
     if (timer) timer->record_start(s);
-    hipLaunchKernelGGL((k_filter<RectTree::FANOUT, RectTree::LEAF, KeepAll>), blocks_for(nq), kBlock, 0, s, v, q, nq,
-                       qorder, emit_rank ? 1 : 0, KeepAll{}, pb.b.p, pb.p.p, (uint64_t)pb.b.n, pb.counter.p);
+    if (t.leaf == 4)
+      hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 4, KeepAll>), blocks_for(nq), kBlock, 0, s, v, q, nq, qorder,
+                         emit_rank ? 1 : 0, KeepAll{}, pb.b.p, pb.p.p, (uint64_t)pb.b.n, pb.counter.p);
+    else if (t.leaf == 16)
+      hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 16, KeepAll>), blocks_for(nq), kBlock, 0, s, v, q, nq, qorder,
+                         emit_rank ? 1 : 0, KeepAll{}, pb.b.p, pb.p.p, (uint64_t)pb.b.n, pb.counter.p);
+    else
+      hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 8, KeepAll>), blocks_for(nq), kBlock, 0, s, v, q, nq, qorder,
+                         emit_rank ? 1 : 0, KeepAll{}, pb.b.p, pb.p.p, (uint64_t)pb.b.n, pb.counter.p);
     SJ_CHECK(hipGetLastError());
     if (timer) timer->record_stop(s);
     unsigned long long total = 0;

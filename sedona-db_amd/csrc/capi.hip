@@ -119,6 +119,13 @@ const char* rt_err(GpuSpatialRuntime* self) {
 // ---------------------------------------------------------------------------
 // FP32 rect index
 
+// Join parameters chosen by SjhipAutotune, per kind of refine: 0 point in
+// polygon, 1 distance, 2 other relations.  refine_order < 0: not tuned.
+struct Tuning {
+  int refine_order[3] = {-1, -1, -1};
+  int pip_waves = 8;
+};
+
 struct IndexImpl {
   Runtime* rt = nullptr;
   std::string last_error;
@@ -126,6 +133,7 @@ struct IndexImpl {
   hipStream_t stream = nullptr;
   DevArray<float4> rects;
   RectTree tree;
+  Tuning tuning;
   bool built = false;
   std::mutex mu;
 };
@@ -583,9 +591,15 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
   jr->device_id = rf->rt->device_id;
   RefineStats st;
   uint64_t kept = 0, nc = 0;
+  // refine order: the caller's, or (refine_order < 0) the tuned one, else
+  // the default of the refine kind
+  const int kind = dwithin ? 1 : ranked ? 0 : 2;
+  int order_used = o.refine_order;
+  if (order_used < 0) order_used = ix->tuning.refine_order[kind];
+  if (order_used < 0) order_used = dwithin ? 0 : 1;
   // refine_order 2 (DWithin over point probes): filter and distance refine
   // in one kernel, candidates never written out
-  const bool fused = dwithin && probe.points_only && o.precision != 1 && o.refine_order == 2;
+  const bool fused = dwithin && probe.points_only && o.precision != 1 && order_used == 2;
   if (fused) {
     kept = filter_refine_distance(ix->tree, q.p, nq, ord, rf->build, rf->ring_index, probe, o.distance, jr->pairs, &nc,
                                   &st, s, ev.on ? &ft : nullptr);
@@ -598,7 +612,7 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
     mark(3);
     // polygon-major order: one wavefront then walks few distinct rings
     // (height slabs only for the point-in-polygon refine)
-    if (o.refine_order == 1) {
+    if (order_used == 1) {
       const float4* br = !dwithin && ix->rects.n >= (size_t)rf->build.n_rows ? ix->rects.p : nullptr;
       sort_pairs_by_build(cand, nc, rf->build.n_rows, br, rp, s);
     }
@@ -607,7 +621,7 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
                              ev.on ? &rt : nullptr, o.precision, &st);
     else
       kept = refine_relation(rf->build, rf->ring_index, rp, o.predicate, cand.b.p, cand.p.p, nc, o.precision,
-                             jr->pairs, &st, s, ev.on ? &rt : nullptr, ranked ? ord : nullptr);
+                             jr->pairs, &st, s, ev.on ? &rt : nullptr, ranked ? ord : nullptr, ix->tuning.pip_waves);
   }
   mark(8);
   if (o.sort_output) sort_pairs(jr->pairs.b.p, jr->pairs.p.p, kept, s);
@@ -898,6 +912,102 @@ int SjhipJoinDevice(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefine
       throw Error("probe array must be in ROCm device memory or CPU memory");
     WkbInput in = wkb_input_from_arrow(probe_schema, &probe_array->array, dev);
     JoinEngine::run((IndexImpl*)index->private_data, (RefinerImpl*)refiner->private_data, in, *options, out);
+  });
+}
+
+int SjhipAutotune(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner* refiner,
+                  const struct ArrowSchema* probe_schema, const struct ArrowDeviceArray* probe_sample,
+                  const struct SjhipJoinOptions* options, struct SjhipTuneResult* out) {
+  return guarded(nullptr, [&] {
+    if (!index || !index->private_data || !refiner || !refiner->private_data) throw Error("null index or refiner");
+    if (!probe_sample || !options || !out) throw Error("null argument");
+    IndexImpl* ix = (IndexImpl*)index->private_data;
+    RefinerImpl* rf = (RefinerImpl*)refiner->private_data;
+    if (!ix->built || !rf->built) throw Error("index and refiner must be built");
+    if (!ix->rects.p) throw Error("autotune needs the index's build rects");
+    bool dev = probe_sample->device_type == ARROW_DEVICE_ROCM;
+    if (!dev && probe_sample->device_type != ARROW_DEVICE_CPU)
+      throw Error("probe sample must be in ROCm device memory or CPU memory");
+    use_device(rf->rt);
+    WkbInput in = wkb_input_from_arrow(probe_schema, &probe_sample->array, dev);
+    const bool dwithin = options->predicate == 100;
+    const int64_t n_rects = (int64_t)(ix->rects.n);
+    SjhipJoinOptions o = *options;
+    o.profile = 0;
+    o.stream = nullptr;
+    o.sort_output = 0;
+    int evals = 0;
+    // best of two runs, wall clock of the whole join
+    auto time_join = [&]() {
+      double best = 1e300;
+      for (int r = 0; r < 2; r++) {
+        SjhipJoinResult res{};
+        auto t0 = std::chrono::steady_clock::now();
+        JoinEngine::run(ix, rf, in, o, &res);
+        double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (res.release) res.release(&res);
+        best = std::min(best, ms);
+      }
+      evals++;
+      return best;
+    };
+    auto rebuild = [&](int leaf) {
+      if (ix->tree.leaf == leaf) return;
+      RectTree t;
+      build_rect_tree(ix->rects.p, n_rects, t, ix->stream, leaf);
+      SJ_CHECK(hipStreamSynchronize(ix->stream));
+      t.cand_per_query_x16.store(ix->tree.cand_per_query_x16.load());
+      t.kept_per_query_x16.store(ix->tree.kept_per_query_x16.load());
+      ix->tree = std::move(t);
+    };
+    // the kind of refine this probe takes (as JoinEngine::run decides); the
+    // decoded column is released before its stream
+    int kind = 1;
+    if (!dwithin) {
+      ScopedStream ss;
+      GeomColumn probe_types;
+      decode_wkb(in, probe_types, ss.s);
+      kind = pip_refine_applies(rf->build, rf->ring_index, probe_types, o.predicate) ? 0 : 2;
+    }
+    o.refine_order = ix->tuning.refine_order[kind] >= 0 ? ix->tuning.refine_order[kind] : (dwithin ? 0 : 1);
+    (void)time_join();  // warm-up: code objects, memory pool, output-size hints
+    // 1. rect tree leaf size
+    double best = 1e300;
+    int best_leaf = ix->tree.leaf;
+    for (int leaf : {4, 8, 16}) {
+      rebuild(leaf);
+      double ms = time_join();
+      if (ms < best) { best = ms; best_leaf = leaf; }
+    }
+    rebuild(best_leaf);
+    // 2. candidate order before the refine
+    std::vector<int> orders = {0, 1};
+    if (dwithin) orders.push_back(2);
+    int best_order = o.refine_order;
+    best = 1e300;
+    for (int ro : orders) {
+      o.refine_order = ro;
+      double ms = time_join();
+      if (ms < best) { best = ms; best_order = ro; }
+    }
+    o.refine_order = best_order;
+    ix->tuning.refine_order[kind] = best_order;
+    // 3. resident waves of the point-in-polygon refine
+    if (kind == 0) {
+      int best_w = ix->tuning.pip_waves;
+      best = 1e300;
+      for (int w : {4, 6, 8}) {
+        ix->tuning.pip_waves = w;
+        double ms = time_join();
+        if (ms < best) { best = ms; best_w = w; }
+      }
+      ix->tuning.pip_waves = best_w;
+    }
+    out->leaf = ix->tree.leaf;
+    out->refine_order = best_order;
+    out->pip_waves = ix->tuning.pip_waves;
+    out->evaluations = evals;
+    out->best_ms = best;
   });
 }
 

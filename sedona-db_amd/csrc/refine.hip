@@ -777,8 +777,8 @@ __device__ __forceinline__ void emit(WaveOut& wo, bool keep, uint32_t b, uint32_
 #endif
 // The walk is latency-bound: more resident waves per SIMD beat the few
 // registers spilled to reach them.
-template <int PRECISION>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SJ_PIP_WAVES, 8))) void k_refine_pip(Ctx c, int allow_on_edge, const uint32_t* cb,
+template <int PRECISION, int WAVES = SJ_PIP_WAVES>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_refine_pip(Ctx c, int allow_on_edge, const uint32_t* cb,
                                                        const uint32_t* cp, uint64_t n, const uint32_t* list,
                                                        uint32_t* ob, uint32_t* op, uint64_t cap,
                                                        unsigned long long* counter, unsigned long long* stats,
@@ -1502,7 +1502,8 @@ bool pip_refine_applies(const GeomColumn& build, const RingIndex& ri, const Geom
 
 uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, int predicate,
                          const uint32_t* cb, const uint32_t* cp, uint64_t n, int precision, PairBuf& out,
-                         RefineStats* stats, hipStream_t s, KernelTimer* timer, const uint32_t* probe_ids) {
+                         RefineStats* stats, hipStream_t s, KernelTimer* timer, const uint32_t* probe_ids,
+                         int pip_waves) {
   out.counter.reserve(1, s);
   out.b.reserve(n > 0 ? n : 1, s);
   out.p.reserve(n > 0 ? n : 1, s);
@@ -1525,9 +1526,14 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
       // certified FP32 pass; candidates with an uncertain test (aux[0] of
       // them) are refined again in FP64 by the second launch
       DevArray<uint32_t> redo(n, s);
-      hipLaunchKernelGGL(k_refine_pip<kPrecDeferred>, grid_for(n), kBlock, 0, s, c, allow, cb, cp, n, nullptr,
-                         out.b.p, out.p.p, (uint64_t)out.b.n, out.counter.p, aux.p + 1, probe_ids, redo.p, aux.p,
-                         nullptr);
+      auto fp32_pass = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, grid_for(n), kBlock, 0, s, c, allow, cb, cp, n, nullptr, out.b.p, out.p.p,
+                           (uint64_t)out.b.n, out.counter.p, aux.p + 1, probe_ids, redo.p, aux.p, nullptr);
+      };
+      // resident waves per SIMD (an autotuned choice; registers spill beyond 4)
+      if (pip_waves == 4) fp32_pass(k_refine_pip<kPrecDeferred, 4>);
+      else if (pip_waves == 6) fp32_pass(k_refine_pip<kPrecDeferred, 6>);
+      else fp32_pass(k_refine_pip<kPrecDeferred, 8>);
       // the timer brackets the FP32 pass alone (the kernel the roofline
       // figures describe)
       if (timer) timer->record_stop(s);
@@ -1670,8 +1676,15 @@ uint64_t filter_refine_distance(const RectTree& t, const float4* q, int64_t nq, 
     kw.d = d;
     kw.hits_seen = kw.tests = kw.scans = 0;
     kw.stats = aux.p;
-    hipLaunchKernelGGL((k_filter<RectTree::FANOUT, RectTree::LEAF, decltype(kw)>), blocks_for(nq), kFilterBlock, 0,
-                       s, v, q, nq, qorder, 0, kw, out.b.p, out.p.p, (uint64_t)out.b.n, out.counter.p);
+    if (t.leaf == 4)
+      hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 4, decltype(kw)>), blocks_for(nq), kFilterBlock, 0, s, v, q, nq,
+                         qorder, 0, kw, out.b.p, out.p.p, (uint64_t)out.b.n, out.counter.p);
+    else if (t.leaf == 16)
+      hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 16, decltype(kw)>), blocks_for(nq), kFilterBlock, 0, s, v, q, nq,
+                         qorder, 0, kw, out.b.p, out.p.p, (uint64_t)out.b.n, out.counter.p);
+    else
+      hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 8, decltype(kw)>), blocks_for(nq), kFilterBlock, 0, s, v, q, nq,
+                         qorder, 0, kw, out.b.p, out.p.p, (uint64_t)out.b.n, out.counter.p);
   };
   for (int attempt = 0; attempt < 2; attempt++) {
     SJ_CHECK(hipMemsetAsync(out.counter.p, 0, 8, s));

@@ -204,7 +204,8 @@ void build_ring_index(const GeomColumn& g, RingIndex& ri, const RingIndexParams&
 // Rect tree over FP32 build rects.
 struct RectTree {
   static constexpr int FANOUT = 8;
-  static constexpr int LEAF = 8;
+  static constexpr int LEAF = 8;  // default rects per leaf node
+  int leaf = LEAF;                // rects per leaf node: 4, 8 or 16
   int64_t n_valid = 0;
   int n_levels = 0;                 // internal + leaf-node levels
   std::vector<int64_t> level_off;   // start of level l in nodes (l = 0: leaf nodes)
@@ -230,6 +231,7 @@ struct RectTree {
     leaf_keys.s = nullptr;
   }
   RectTree& operator=(RectTree&& o) noexcept {
+    leaf = o.leaf;
     n_valid = o.n_valid;
     n_levels = o.n_levels;
     level_off = std::move(o.level_off);
@@ -244,7 +246,7 @@ struct RectTree {
     return *this;
   }
 };
-void build_rect_tree(const float4* rects_dev, int64_t n, RectTree& t, hipStream_t s);
+void build_rect_tree(const float4* rects_dev, int64_t n, RectTree& t, hipStream_t s, int leaf = RectTree::LEAF);
 
 // Optional HIP-event bracket around a kernel launch (on its stream).
 struct KernelTimer {
@@ -292,7 +294,7 @@ bool pip_refine_applies(const GeomColumn& build, const RingIndex& ri, const Geom
 uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, int predicate,
                          const uint32_t* cb, const uint32_t* cp, uint64_t n, int precision, PairBuf& out,
                          RefineStats* stats, hipStream_t s, KernelTimer* timer = nullptr,
-                         const uint32_t* probe_ids = nullptr);
+                         const uint32_t* probe_ids = nullptr, int pip_waves = 8);
 
 // Distance refine (build.distance(probe) <= d); probe must be points.
 // precision 1: the all-FP64 form over every segment.  stats: FP64 segment

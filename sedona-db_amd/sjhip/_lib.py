@@ -173,6 +173,11 @@ class SjhipJoinOptions(ctypes.Structure):
     ]
 
 
+class SjhipTuneResult(ctypes.Structure):
+    _fields_ = [("leaf", ctypes.c_int32), ("refine_order", ctypes.c_int32), ("pip_waves", ctypes.c_int32),
+                ("evaluations", ctypes.c_int32), ("best_ms", ctypes.c_double)]
+
+
 class SjhipServerConfig(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_uint32), ("memory_budget", ctypes.c_uint64)]
 
@@ -190,6 +195,7 @@ EXPORTED = [
     "SjhipComputeRects", "SjhipDeviceAlloc", "SjhipDeviceFree", "SjhipCopyToDevice", "SjhipCopyToHost",
     "SjhipSetDevice", "SjhipCopyKernel",
     "SjhipServerCreate", "SjhipServerSubmit", "SjhipServerWait", "SjhipServerDestroy", "SjhipRefinerKnnDevice",
+    "SjhipAutotune",
     "SjhipHostRaycast", "SjhipHostRaycastF32", "SjhipHostCollinear", "SjhipHostSegmentIntersects",
     "SjhipHostOrient2d", "SjhipHostHypot", "SjhipHostCoordToLine", "SjhipHostRaycastBatch", "SjhipHostHypotBatch",
 ]
@@ -223,6 +229,8 @@ def lib():
         L.SjhipServerWait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(SjhipJoinResult),
                                       ctypes.POINTER(SjhipRequestTiming)]
         L.SjhipServerDestroy.argtypes = [ctypes.c_void_p]
+        L.SjhipAutotune.argtypes = [_IX, _RF, _SCH, ctypes.POINTER(ArrowDeviceArray), ctypes.POINTER(SjhipJoinOptions),
+                                    ctypes.POINTER(SjhipTuneResult)]
         L.SjhipRefinerKnnDevice.argtypes = [_RF, _SCH, ctypes.POINTER(ArrowDeviceArray), ctypes.c_uint32,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.SjhipIndexBuildFromRefiner.argtypes = [_IX, _RF]

@@ -511,6 +511,25 @@ def join_device(index, refiner, probe, predicate, distance=0.0, precision=0, sor
     return JoinResult(res)
 
 
+def autotune(index, refiner, probe_sample, predicate, distance=0.0, precision=0):
+    """Tunes the index for joins like join_device(index, refiner, probe,
+    predicate, refine_order=-1): returns the chosen parameters."""
+    opts = _join_options(predicate, distance, precision, False, True, False, -1, None)
+    if isinstance(probe_sample, DeviceWkb):
+        keep, sch, dev = None, probe_sample.schema, probe_sample.dev
+    else:
+        keep = ExportedArrow(probe_sample)
+        sch, dev = keep.schema, host_device_array(keep)
+    out = L.SjhipTuneResult()
+    rc = L.lib().SjhipAutotune(ctypes.byref(index.ix), ctypes.byref(refiner.rf), ctypes.byref(sch), ctypes.byref(dev),
+                               ctypes.byref(opts), ctypes.byref(out))
+    del keep
+    if rc:
+        raise GpuSpatialError(L.lib().SjhipLastError().decode())
+    return {"leaf": out.leaf, "refine_order": out.refine_order, "pip_waves": out.pip_waves,
+            "evaluations": out.evaluations, "best_ms": out.best_ms}
+
+
 class GpuSpatialServer:
     """Request loop over a built index + refiner: up to `slots` probe
     batches joined concurrently on the device, admitted in arrival order by

@@ -230,6 +230,23 @@ def test_dwithin_zipf_polygons_vs_oracle(d, refine_order, opts):
     assert len(gb) > 100
 
 
+@pytest.mark.parametrize("pred,d", [("contains", 0.0), ("dwithin", 0.05)])
+def test_autotuned_join_gives_the_same_pairs(pred, d, opts):
+    """The autotuner rebuilds the tree with its chosen leaf size and picks
+    the refine order and kernel variant; the tuned join (refine_order -1)
+    returns the oracle's pairs."""
+    b, p = D.workload_pip_zipf(80_000, 800, seed=31)
+    ix, rf = build_pair(b.to_arrow(), opts)
+    dev = G.DeviceWkb.from_host(p)
+    t = G.autotune(ix, rf, dev.slice(0, 40_000), pred, distance=d)
+    say("autotune", pred, t)
+    assert t["leaf"] in (4, 8, 16) and t["evaluations"] >= 5
+    r = G.join_device(ix, rf, dev, pred, distance=d, refine_order=-1)
+    gb, gp = r.to_host()
+    ob, op = O.join(O.Store(b.as_tuple()), O.Store(p.as_tuple()), pred, distance=d, nthreads=8)
+    assert np.array_equal(gb, ob) and np.array_equal(gp, op), (len(gb), len(ob))
+
+
 def test_poly_poly_large_rings_vs_oracle(opts):
     """Polygon x polygon intersects with rings of up to 1024 vertices (both
     sides get segment trees): the indexed crossing counts and point tests
