@@ -1003,6 +1003,11 @@ int SjhipAutotune(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner*
       }
       ix->tuning.pip_waves = best_w;
     }
+    // give back the pool memory the sample joins left cached, so that the
+    // first full-size joins do not find it fragmented
+    SJ_CHECK(hipDeviceSynchronize());
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, rf->rt->device_id) == hipSuccess) (void)hipMemPoolTrimTo(pool, 0);
     out->leaf = ix->tree.leaf;
     out->refine_order = best_order;
     out->pip_waves = ix->tuning.pip_waves;
