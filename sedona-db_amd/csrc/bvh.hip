@@ -264,6 +264,50 @@ __global__ void k_query_keys(const float4* r, int64_t n, float x0, float y0, flo
 }
 }  // namespace
 
+__global__ void k_fill_u64(unsigned long long* p, int64_t n, unsigned long long v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_fill_u64, blocks_for(n), kBlock, 0, s, p, n, v);
+  SJ_CHECK(hipGetLastError());
+}
+
+size_t morton_sort_temp_bytes(int64_t n) {
+  size_t tmp = 0;
+  SJ_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                              (uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 32));
+  return tmp;
+}
+
+void morton_order_fixed(const RectTree& t, const float4* rects, int64_t n, uint32_t* keys, uint32_t* ids,
+                        uint32_t* keys_sorted, uint32_t* order, void* temp, size_t temp_bytes, hipStream_t s) {
+  if (n <= 0) return;
+  const float* bb = t.bb;
+  float sx = bb[2] > bb[0] ? 65535.f / (bb[2] - bb[0]) : 0.f;
+  float sy = bb[3] > bb[1] ? 65535.f / (bb[3] - bb[1]) : 0.f;
+  hipLaunchKernelGGL(k_query_keys, blocks_for(n), kBlock, 0, s, rects, n, bb[0], bb[1], sx, sy, keys, ids);
+  SJ_CHECK(hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys, keys_sorted, ids, order, (int)n, 0, 32, s));
+}
+
+void launch_filter(const RectTree& t, const float4* q, int64_t nq, const uint32_t* qorder, bool emit_rank,
+                   uint32_t* ob, uint32_t* op, uint64_t cap, unsigned long long* counter, hipStream_t s) {
+  if (nq <= 0 || t.n_levels == 0) return;
+  TreeView v = tree_view(t);
+  const int er = emit_rank ? 1 : 0;
+  if (t.leaf == 4)
+    hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 4, KeepAll>), blocks_for(nq), kBlock, 0, s, v, q, nq, qorder, er,
+                       KeepAll{}, ob, op, cap, counter);
+  else if (t.leaf == 16)
+    hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 16, KeepAll>), blocks_for(nq), kBlock, 0, s, v, q, nq, qorder, er,
+                       KeepAll{}, ob, op, cap, counter);
+  else
+    hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 8, KeepAll>), blocks_for(nq), kBlock, 0, s, v, q, nq, qorder, er,
+                       KeepAll{}, ob, op, cap, counter);
+  SJ_CHECK(hipGetLastError());
+}
+
 int64_t morton_order(const float4* rects, int64_t n, DevArray<uint32_t>& order, hipStream_t s) {
   float bb[4];
   int64_t nv;

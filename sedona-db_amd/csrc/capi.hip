@@ -637,10 +637,8 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
     if (nc) { SJ_CHECK(hipEventElapsedTime(&ms, ev.e[3], ev.e[6])); out->stage_ms[5] = ms; }
   }
   // the result outlives the call's stream: all its work is complete, so its
-  // buffers are released on the null stream later
-  jr->pairs.b.s = nullptr;
-  jr->pairs.p.s = nullptr;
-  jr->pairs.counter.s = nullptr;
+  // buffers are released on the detached stream later
+  jr->pairs.b.s = jr->pairs.p.s = jr->pairs.counter.s = detached_stream();
   out->build_indices = jr->pairs.b.p;
   out->probe_indices = jr->pairs.p.p;
   out->size = kept;
@@ -662,6 +660,166 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
 // requests run their join concurrently on the device; a finished request
 // frees its slot and its share of the budget for the next.
 
+// A captured per-request launch sequence for point probes of one size
+// bucket (capacity B rows, C candidate pairs), replayed for every request
+// of at most B rows.  The request's column is described in device memory
+// (PointsDesc), the counts are read on the device, every buffer has a
+// fixed capacity, and the counters are cleared by a kernel, so the
+// sequence has no host synchronisation, allocation or memset.  After the
+// replay the host reads the counters: a decode error, a row that is not a
+// point record or more candidates than C send the request down the eager
+// path (JoinEngine::run) instead.
+struct GraphPlan {
+  static constexpr int64_t kMinRows = 4096, kMaxRows = 1 << 22;
+  int kind = 0;  // 0 point in polygon, 1 distance
+  int predicate = 0;
+  double distance = 0;
+  int pip_waves = 8;
+  int leaf = 8;
+  const float4* tree_nodes = nullptr;  // the index tree the plan was captured over
+  int device_id = 0;
+  int64_t B = 0;
+  uint64_t C = 0;
+  DevArray<PointsDesc> desc;
+  GeomColumn pcol, rcol;  // the decoded probe points, and in Morton order
+  DevArray<float4> rects;
+  DevArray<uint32_t> keys, ids, keys_sorted, order;
+  DevArray<uint8_t> sort_tmp;
+  size_t sort_bytes = 0;
+  DevArray<uint32_t> cb, cp, redo, ob, op;
+  DevArray<unsigned long long> cnt;  // [0] error, [1] candidates, [2] deferred, [3] kept, [4..5] refine stats
+  PointsDesc* host_desc = nullptr;   // pinned
+  unsigned long long* host_cnt = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~GraphPlan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (host_desc) (void)hipHostFree(host_desc);
+    if (host_cnt) (void)hipHostFree(host_cnt);
+  }
+};
+
+// Builds and captures a plan on stream s (the stream's work is complete on
+// return; the plan's buffers are freed on the null stream).
+std::unique_ptr<GraphPlan> capture_plan(IndexImpl* ix, RefinerImpl* rf, int kind, const SjhipJoinOptions& o,
+                                        int64_t B, uint64_t C, hipStream_t s) {
+  std::unique_ptr<GraphPlan> g(new GraphPlan());
+  g->kind = kind;
+  g->predicate = o.predicate;
+  g->distance = o.distance;
+  g->pip_waves = ix->tuning.pip_waves;
+  g->leaf = ix->tree.leaf;
+  g->tree_nodes = ix->tree.nodes.p;
+  g->device_id = rf->rt->device_id;
+  g->B = B;
+  g->C = C;
+  g->desc.alloc(1, s);
+  g->pcol.n_rows = g->pcol.n_parts = g->pcol.n_rings = g->pcol.n_pts = B;
+  g->pcol.points_only = true;
+  g->pcol.type_mask = 1u << kPoint;
+  g->pcol.xy.alloc(B, s);
+  g->pcol.type.alloc(B, s);
+  g->rects.alloc(B, s);
+  g->keys.alloc(B, s);
+  g->ids.alloc(B, s);
+  g->keys_sorted.alloc(B, s);
+  g->order.alloc(B, s);
+  g->sort_bytes = morton_sort_temp_bytes(B);
+  g->sort_tmp.alloc(g->sort_bytes + 16, s);
+  if (kind == 0) {
+    g->rcol.n_rows = g->rcol.n_parts = g->rcol.n_rings = g->rcol.n_pts = B;
+    g->rcol.points_only = true;
+    g->rcol.type_mask = 1u << kPoint;
+    g->rcol.xy.alloc(B, s);
+    g->redo.alloc(C, s);
+  }
+  g->cb.alloc(C, s);
+  g->cp.alloc(C, s);
+  g->ob.alloc(C, s);
+  g->op.alloc(C, s);
+  g->cnt.alloc(8, s);
+  SJ_CHECK(hipHostMalloc((void**)&g->host_desc, sizeof(PointsDesc), hipHostMallocDefault));
+  SJ_CHECK(hipHostMalloc((void**)&g->host_cnt, 8 * sizeof(unsigned long long), hipHostMallocDefault));
+  SJ_CHECK(hipStreamSynchronize(s));
+  const bool dwithin = kind == 1;
+  SJ_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    fill_u64(g->cnt.p, 1, ~0ull, s);
+    fill_u64(g->cnt.p + 1, 7, 0ull, s);
+    decode_points_desc(g->desc.p, B, g->pcol.xy.p, g->pcol.type.p, g->cnt.p, s);
+    rects_of_points_into(g->pcol.xy.p, g->pcol.type.p, B, !dwithin, dwithin ? o.distance : NAN, g->rects.p, s);
+    morton_order_fixed(ix->tree, g->rects.p, B, g->keys.p, g->ids.p, g->keys_sorted.p, g->order.p, g->sort_tmp.p,
+                       g->sort_bytes, s);
+    if (!dwithin) {
+      gather_points_into(g->pcol.xy.p, g->order.p, B, g->rcol.xy.p, s);
+      launch_filter(ix->tree, g->rects.p, B, g->order.p, true, g->cb.p, g->cp.p, C, g->cnt.p + 1, s);
+      launch_pip_refine_dev(rf->build, rf->ring_index, g->rcol, o.predicate, g->cb.p, g->cp.p, g->cnt.p + 1, C,
+                            g->order.p, g->redo.p, g->cnt.p + 2, g->ob.p, g->op.p, C, g->cnt.p + 3, g->cnt.p + 4,
+                            g->pip_waves, s);
+    } else {
+      launch_filter(ix->tree, g->rects.p, B, g->order.p, false, g->cb.p, g->cp.p, C, g->cnt.p + 1, s);
+      launch_distance_refine_dev(rf->build, rf->ring_index, g->pcol, o.distance, g->cb.p, g->cp.p, g->cnt.p + 1, C,
+                                 g->ob.p, g->op.p, C, g->cnt.p + 3, g->cnt.p + 4, s);
+    }
+    SJ_CHECK(hipMemcpyAsync(g->host_cnt, g->cnt.p, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  } catch (...) {
+    hipGraph_t dead = nullptr;
+    (void)hipStreamEndCapture(s, &dead);
+    if (dead) (void)hipGraphDestroy(dead);
+    throw;
+  }
+  SJ_CHECK(hipStreamEndCapture(s, &g->graph));
+  SJ_CHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  // the plan outlives the capturing stream's use: free on the detached stream
+  hipStream_t ds = detached_stream();
+  for (auto* a : {&g->keys, &g->ids, &g->keys_sorted, &g->order, &g->cb, &g->cp, &g->redo, &g->ob, &g->op}) a->s = ds;
+  g->desc.s = ds;
+  g->pcol.xy.s = ds;
+  g->pcol.type.s = ds;
+  g->rcol.xy.s = ds;
+  g->rects.s = ds;
+  g->sort_tmp.s = ds;
+  g->cnt.s = ds;
+  return g;
+}
+
+// Runs one request through a plan.  Returns false (nothing produced) when
+// the request must take the eager path instead.
+bool run_plan(GraphPlan& g, const WkbInput& in, SjhipJoinResult* out, hipStream_t s) {
+  g.host_desc->data = in.data;
+  g.host_desc->offsets = in.offsets;
+  g.host_desc->n = in.length;
+  g.host_desc->large = in.large ? 1 : 0;
+  g.host_desc->pad = 0;
+  SJ_CHECK(hipMemcpyAsync(g.desc.p, g.host_desc, sizeof(PointsDesc), hipMemcpyHostToDevice, s));
+  SJ_CHECK(hipGraphLaunch(g.exec, s));
+  SJ_CHECK(hipStreamSynchronize(s));
+  const unsigned long long* h = g.host_cnt;
+  if (h[0] != ~0ull || h[1] > g.C) return false;
+  const uint64_t kept = h[3];
+  std::unique_ptr<JoinResultImpl> jr(new JoinResultImpl());
+  jr->device_id = g.device_id;
+  jr->pairs.b.alloc(kept > 0 ? kept : 1, s);
+  jr->pairs.p.alloc(kept > 0 ? kept : 1, s);
+  if (kept) {
+    SJ_CHECK(hipMemcpyAsync(jr->pairs.b.p, g.ob.p, kept * 4, hipMemcpyDeviceToDevice, s));
+    SJ_CHECK(hipMemcpyAsync(jr->pairs.p.p, g.op.p, kept * 4, hipMemcpyDeviceToDevice, s));
+  }
+  SJ_CHECK(hipStreamSynchronize(s));
+  jr->pairs.b.s = jr->pairs.p.s = detached_stream();
+  std::memset(out, 0, sizeof(*out));
+  out->build_indices = jr->pairs.b.p;
+  out->probe_indices = jr->pairs.p.p;
+  out->size = kept;
+  out->candidates = h[1];
+  out->segment_tests = h[4];
+  out->fallback_tests = h[5];
+  out->release = jr_release;
+  out->private_data = jr.release();
+  return true;
+}
+
 struct ServerReq {
   uint64_t id = 0;
   WkbInput in;
@@ -677,6 +835,7 @@ struct ServerReq {
 struct Server {
   IndexImpl* ix = nullptr;
   RefinerImpl* rf = nullptr;
+  bool graphs = false;
   uint64_t budget = 0;
   uint64_t reserved = 0;
   int running = 0;
@@ -708,10 +867,66 @@ struct Server {
     return wkb + 96 * n + 44 * cand + (1u << 20);
   }
 
+  // The kind of captured sequence a request can replay (0 point in
+  // polygon, 1 distance), or -1: eager path.
+  int graph_kind(const WkbInput& in, const SjhipJoinOptions& o) const {
+    if (!graphs || !in.on_device || in.view || in.validity || in.length <= 0 || in.length > GraphPlan::kMaxRows ||
+        o.precision != 0)
+      return -1;
+    const GeomColumn& b = rf->build;
+    if (o.predicate == 100) return b.n_rows > 0 ? 1 : -1;
+    const uint32_t polys = (1u << kPolygon) | (1u << kMultiPolygon);
+    const bool pip = !b.points_only && rf->ring_index.ring_rect.p && rf->ring_index.desc.p &&
+                     (b.type_mask & ~polys) == 0 &&
+                     (o.predicate == SedonaSpatialPredicateIntersects || o.predicate == SedonaSpatialPredicateCovers ||
+                      o.predicate == SedonaSpatialPredicateContains);
+    return pip ? 0 : -1;
+  }
+
   void work() {
     use_device(rf->rt);
     hipStream_t s;
     SJ_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    // this worker's captured sequences, by (kind, predicate, bucket rows)
+    std::map<std::tuple<int, int, int64_t>, std::unique_ptr<GraphPlan>> plans;
+    auto run_graph = [&](const WkbInput& in, const SjhipJoinOptions& o, SjhipJoinResult* res) {
+      const int kind = graph_kind(in, o);
+      if (kind < 0) return false;
+      int64_t B = GraphPlan::kMinRows;
+      while (B < in.length) B <<= 1;
+      auto key = std::make_tuple(kind, (int)o.predicate, B);
+      auto& plan = plans[key];
+      if (plan && (plan->distance != o.distance || plan->leaf != ix->tree.leaf ||
+                   plan->tree_nodes != ix->tree.nodes.p || plan->pip_waves != ix->tuning.pip_waves))
+        plan.reset();
+      if (!plan) {
+        uint64_t per16 = ix->tree.cand_per_query_x16.load(std::memory_order_relaxed);
+        double cpq = per16 ? per16 / 16.0 : 8.0;
+        if (kind == 1) cpq *= 2.0;
+        plan = capture_plan(ix, rf, kind, o, B, (uint64_t)(cpq * 1.5 * (double)B) + 65536, s);
+      }
+      if (run_plan(*plan, in, res, s)) {
+        if (o.sort_output && res->size) {
+          sort_pairs(res->build_indices, res->probe_indices, res->size, s);
+          SJ_CHECK(hipStreamSynchronize(s));
+        }
+        return true;
+      }
+      // too many candidates for the plan: capture a larger one next time
+      if (plan->host_cnt[0] == ~0ull && plan->host_cnt[1] > plan->C) {
+        uint64_t C = plan->host_cnt[1] + plan->host_cnt[1] / 4 + 65536;
+        plan.reset();
+        plan = capture_plan(ix, rf, kind, o, B, C, s);
+        if (run_plan(*plan, in, res, s)) {
+          if (o.sort_output && res->size) {
+            sort_pairs(res->build_indices, res->probe_indices, res->size, s);
+            SJ_CHECK(hipStreamSynchronize(s));
+          }
+          return true;
+        }
+      }
+      return false;
+    };
     for (;;) {
       ServerReq* r = nullptr;
       {
@@ -733,7 +948,9 @@ struct Server {
       o.stream = s;
       SjhipJoinResult res{};
       std::string err;
-      int rc = guarded(&err, [&] { JoinEngine::run(ix, rf, r->in, o, &res); });
+      int rc = guarded(&err, [&] {
+        if (!run_graph(r->in, o, &res)) JoinEngine::run(ix, rf, r->in, o, &res);
+      });
       {
         std::lock_guard<std::mutex> lk(mu);
         reserved -= r->estimate;
@@ -746,6 +963,7 @@ struct Server {
       }
       cv.notify_all();
     }
+    plans.clear();
     (void)hipStreamSynchronize(s);
     (void)hipStreamDestroy(s);
   }
@@ -1005,7 +1223,8 @@ int SjhipAutotune(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner*
     }
     // give back the pool memory the sample joins left cached, so that the
     // first full-size joins do not find it fragmented
-    SJ_CHECK(hipDeviceSynchronize());
+    SJ_CHECK(hipStreamSynchronize(ix->stream));
+    SJ_CHECK(hipStreamSynchronize(detached_stream()));
     hipMemPool_t pool;
     if (hipDeviceGetDefaultMemPool(&pool, rf->rt->device_id) == hipSuccess) (void)hipMemPoolTrimTo(pool, 0);
     out->leaf = ix->tree.leaf;
@@ -1026,6 +1245,7 @@ int SjhipServerCreate(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefi
     std::unique_ptr<Server> sv(new Server());
     sv->ix = (IndexImpl*)index->private_data;
     sv->rf = (RefinerImpl*)refiner->private_data;
+    sv->graphs = config->graphs != 0;
     if (!sv->ix->built || !sv->rf->built) throw Error("index and refiner must be built");
     use_device(sv->rf->rt);
     sv->budget = config->memory_budget;
@@ -1129,25 +1349,39 @@ int SjhipDeviceAlloc(uint64_t bytes, void** out_ptr) {
   return guarded(nullptr, [&] {
     if (!out_ptr) throw Error("null output pointer");
     *out_ptr = nullptr;
-    SJ_CHECK(hipMalloc(out_ptr, bytes ? bytes : 1));
+    hipStream_t ds = detached_stream();
+    SJ_CHECK(hipMallocAsync(out_ptr, bytes ? bytes : 1, ds));
+    SJ_CHECK(hipStreamSynchronize(ds));
   });
 }
 
 int SjhipDeviceFree(void* ptr) {
   return guarded(nullptr, [&] {
-    if (ptr) SJ_CHECK(hipFree(ptr));
+    if (ptr) {
+      hipStream_t ds = detached_stream();
+      SJ_CHECK(hipFreeAsync(ptr, ds));
+      SJ_CHECK(hipStreamSynchronize(ds));
+    }
   });
 }
 
 int SjhipCopyToDevice(void* dst, const void* src, uint64_t bytes) {
   return guarded(nullptr, [&] {
-    if (bytes) SJ_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    if (bytes) {
+      ScopedStream ss;
+      SJ_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ss.s));
+      SJ_CHECK(hipStreamSynchronize(ss.s));
+    }
   });
 }
 
 int SjhipCopyToHost(void* dst, const void* src, uint64_t bytes) {
   return guarded(nullptr, [&] {
-    if (bytes) SJ_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    if (bytes) {
+      ScopedStream ss;
+      SJ_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ss.s));
+      SJ_CHECK(hipStreamSynchronize(ss.s));
+    }
   });
 }
 
@@ -1184,8 +1418,10 @@ int SjhipJoinResultCopyToHost(const struct SjhipJoinResult* r, uint32_t* b, uint
   return guarded(nullptr, [&] {
     if (!r) throw Error("null result");
     if (r->size == 0) return;
-    SJ_CHECK(hipMemcpy(b, r->build_indices, r->size * 4, hipMemcpyDeviceToHost));
-    SJ_CHECK(hipMemcpy(p, r->probe_indices, r->size * 4, hipMemcpyDeviceToHost));
+    ScopedStream ss;
+    SJ_CHECK(hipMemcpyAsync(b, r->build_indices, r->size * 4, hipMemcpyDeviceToHost, ss.s));
+    SJ_CHECK(hipMemcpyAsync(p, r->probe_indices, r->size * 4, hipMemcpyDeviceToHost, ss.s));
+    SJ_CHECK(hipStreamSynchronize(ss.s));
   });
 }
 

@@ -307,12 +307,12 @@ __device__ __forceinline__ void ld_point_record(const uint8_t* d, int64_t o, uin
 }
 
 // Points-only fast path: one thread per row.
-__global__ void k_points(Src s, int64_t n, double2* xy, uint8_t* type, unsigned long long* err,
-                         unsigned int* type_mask) {
-  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  double2 out = make_double2(NAN, NAN);
-  uint8_t t = 0;
+// One row of a column of 21-byte WKB points: coordinates (NaN for null
+// and empty rows) and type (0 for null rows).
+__device__ __forceinline__ void decode_point_row(const Src& s, int64_t r, double2& out, uint8_t& t,
+                                                 unsigned long long* err) {
+  out = make_double2(NAN, NAN);
+  t = 0;
   if (row_valid(s, r)) {
     int64_t b = row_begin(s, r), e = row_begin(s, r + 1);
     if (b + 21 > e) {
@@ -338,9 +338,42 @@ __global__ void k_points(Src s, int64_t n, double2* xy, uint8_t* type, unsigned 
       }
     }
   }
+}
+
+__global__ void k_points(Src s, int64_t n, double2* xy, uint8_t* type, unsigned long long* err,
+                         unsigned int* type_mask) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double2 out;
+  uint8_t t;
+  decode_point_row(s, r, out, t, err);
   xy[r] = out;
   type[r] = t;
   or_type_mask(type_mask, t);
+}
+
+// The same for a column described in device memory, over a fixed capacity
+// of rows (rows at or beyond d->n are empty): a launch that a captured
+// sequence replays over new inputs.  Rows that are not 21-byte point
+// records report an error.
+__global__ void k_points_desc(const PointsDesc* d, int64_t cap, double2* xy, uint8_t* type,
+                              unsigned long long* err) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= cap) return;
+  double2 out = make_double2(NAN, NAN);
+  uint8_t t = 0;
+  if (r < d->n) {
+    Src s;
+    s.data = d->data;
+    s.offsets = d->offsets;
+    s.large = d->large;
+    s.validity = nullptr;
+    s.bit_offset = 0;
+    if (row_begin(s, r + 1) - row_begin(s, r) != 21) report(err, r, kErrType);
+    else decode_point_row(s, r, out, t, err);
+  }
+  xy[r] = out;
+  type[r] = t;
 }
 
 // Whether every valid row is a 21-byte point record (cheap pre-check).
@@ -624,6 +657,24 @@ __global__ void k_gather_points(const double2* xy, const uint32_t* order, int64_
 }
 
 }  // namespace
+
+void decode_points_desc(const PointsDesc* desc_dev, int64_t cap, double2* xy, uint8_t* type, unsigned long long* err,
+                        hipStream_t s) {
+  if (cap > 0) hipLaunchKernelGGL(k_points_desc, blocks_for(cap), kBlock, 0, s, desc_dev, cap, xy, type, err);
+  SJ_CHECK(hipGetLastError());
+}
+
+void gather_points_into(const double2* xy, const uint32_t* order, int64_t n, double2* out, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_gather_points, blocks_for(n), kBlock, 0, s, xy, order, n, out);
+  SJ_CHECK(hipGetLastError());
+}
+
+void rects_of_points_into(const double2* xy, const uint8_t* type, int64_t n, bool point_nearest, double expand,
+                          float4* out, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_rects_points, blocks_for(n), kBlock, 0, s, xy, type, n, point_nearest ? 1 : 0, expand, out);
+  SJ_CHECK(hipGetLastError());
+}
 
 void gather_points(const GeomColumn& probe, const uint32_t* order, int64_t n, GeomColumn& out, hipStream_t s) {
   if (!probe.points_only) throw Error("gather_points needs a point column");

@@ -785,7 +785,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8
                                                        const uint32_t* probe_ids, uint32_t* redo,
                                                        unsigned long long* n_redo, const unsigned long long* n_dev) {
   SJ_WAVE_OUT(wo);
-  if (n_dev) n = *n_dev;  // candidate count produced on the device
+  if (n_dev) n = min(n, (uint64_t)*n_dev);  // candidate count produced on the device (n: its bound)
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   uint64_t tests = 0, fallbacks = 0;
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -1210,10 +1210,12 @@ template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_refine_distance(Ctx c, double dist, int plain, const uint32_t* cb,
                                                             const uint32_t* cp, uint64_t n, uint32_t* ob,
                                                             uint32_t* op, uint64_t cap, unsigned long long* counter,
-                                                            unsigned long long* stats) {
+                                                            unsigned long long* stats,
+                                                            const unsigned long long* n_dev) {
   const GeomView& B = c.B;
   const GeomView& P = c.P;
   SJ_WAVE_OUT(wo);
+  if (n_dev) n = min(n, (uint64_t)*n_dev);  // candidate count produced on the device (n: its bound)
   uint64_t tests = 0, exact_scans = 0;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -1628,15 +1630,15 @@ uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const Geo
   switch (plain ? kDistAny : dist_kind(build.type_mask, build.points_only, ri)) {
     case kDistLines:
       hipLaunchKernelGGL(k_refine_distance<kDistLines>, grid_for(n), kBlock, 0, s, c, d, 0, cb, cp, n, out.b.p,
-                         out.p.p, (uint64_t)out.b.n, out.counter.p, aux.p);
+                         out.p.p, (uint64_t)out.b.n, out.counter.p, aux.p, nullptr);
       break;
     case kDistPolygons:
       hipLaunchKernelGGL(k_refine_distance<kDistPolygons>, grid_for(n), kBlock, 0, s, c, d, 0, cb, cp, n, out.b.p,
-                         out.p.p, (uint64_t)out.b.n, out.counter.p, aux.p);
+                         out.p.p, (uint64_t)out.b.n, out.counter.p, aux.p, nullptr);
       break;
     default:
       hipLaunchKernelGGL(k_refine_distance<kDistAny>, grid_for(n), kBlock, 0, s, c, d, plain, cb, cp, n, out.b.p,
-                         out.p.p, (uint64_t)out.b.n, out.counter.p, aux.p);
+                         out.p.p, (uint64_t)out.b.n, out.counter.p, aux.p, nullptr);
   }
   SJ_CHECK(hipGetLastError());
   if (timer) timer->record_stop(s);
@@ -1649,6 +1651,55 @@ uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const Geo
     stats->fallback_tests += h[1];
   }
   return cnt;
+}
+
+void launch_pip_refine_dev(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, int predicate,
+                           const uint32_t* cb, const uint32_t* cp, const unsigned long long* n_dev, uint64_t n_cap,
+                           const uint32_t* probe_ids, uint32_t* redo, unsigned long long* redo_n, uint32_t* ob,
+                           uint32_t* op, uint64_t out_cap, unsigned long long* out_n, unsigned long long* stats,
+                           int pip_waves, hipStream_t s) {
+  if (!pip_refine_applies(build, ri, probe, predicate)) throw Error("point-in-polygon refine does not apply");
+  if (n_cap == 0) return;
+  Ctx c;
+  c.B = build.view();
+  c.R = ri.view();
+  c.P = probe.view();
+  const int allow = predicate == kContains ? 0 : 1;
+  auto fp32_pass = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, grid_for(n_cap), kBlock, 0, s, c, allow, cb, cp, n_cap, nullptr, ob, op, out_cap,
+                       out_n, stats, probe_ids, redo, redo_n, n_dev);
+  };
+  if (pip_waves == 4) fp32_pass(k_refine_pip<kPrecDeferred, 4>);
+  else if (pip_waves == 6) fp32_pass(k_refine_pip<kPrecDeferred, 6>);
+  else fp32_pass(k_refine_pip<kPrecDeferred, 8>);
+  hipLaunchKernelGGL(k_refine_pip<1>, 1024, kBlock, 0, s, c, allow, cb, cp, n_cap, redo, ob, op, out_cap, out_n,
+                     nullptr, probe_ids, nullptr, nullptr, redo_n);
+  SJ_CHECK(hipGetLastError());
+}
+
+void launch_distance_refine_dev(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
+                                const uint32_t* cb, const uint32_t* cp, const unsigned long long* n_dev,
+                                uint64_t n_cap, uint32_t* ob, uint32_t* op, uint64_t out_cap,
+                                unsigned long long* out_n, unsigned long long* stats, hipStream_t s) {
+  if (n_cap == 0) return;
+  Ctx c;
+  c.B = build.view();
+  c.R = ri.view();
+  c.P = probe.view();
+  switch (dist_kind(build.type_mask, build.points_only, ri)) {
+    case kDistLines:
+      hipLaunchKernelGGL(k_refine_distance<kDistLines>, grid_for(n_cap), kBlock, 0, s, c, d, 0, cb, cp, n_cap, ob, op,
+                         out_cap, out_n, stats, n_dev);
+      break;
+    case kDistPolygons:
+      hipLaunchKernelGGL(k_refine_distance<kDistPolygons>, grid_for(n_cap), kBlock, 0, s, c, d, 0, cb, cp, n_cap, ob,
+                         op, out_cap, out_n, stats, n_dev);
+      break;
+    default:
+      hipLaunchKernelGGL(k_refine_distance<kDistAny>, grid_for(n_cap), kBlock, 0, s, c, d, 0, cb, cp, n_cap, ob, op,
+                         out_cap, out_n, stats, n_dev);
+  }
+  SJ_CHECK(hipGetLastError());
 }
 
 uint64_t filter_refine_distance(const RectTree& t, const float4* q, int64_t nq, const uint32_t* qorder,

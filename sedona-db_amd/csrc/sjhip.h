@@ -20,6 +20,7 @@
 #include <stdint.h>
 
 #include <atomic>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -44,6 +45,21 @@ struct Error : std::runtime_error {
 
 // ---------------------------------------------------------------------------
 // Device memory (stream-ordered)
+
+// A non-blocking stream per device on which buffers that outlive the
+// stream they were made on are freed and copied.  (The legacy null stream
+// synchronises with every blocking stream, which HIP refuses while another
+// thread captures a graph.)
+inline hipStream_t detached_stream() {
+  static std::mutex mu;
+  static hipStream_t streams[64] = {};
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (dev < 0 || dev >= 64) dev = 0;
+  std::lock_guard<std::mutex> g(mu);
+  if (!streams[dev]) (void)hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking);
+  return streams[dev];
+}
 
 template <typename T>
 struct DevArray {
@@ -222,13 +238,10 @@ struct RectTree {
   RectTree() = default;
   RectTree(RectTree&& o) noexcept { *this = std::move(o); }
   // For a tree that outlives the stream it was built on (after that
-  // stream has been synchronised): its buffers are then freed on the null
-  // stream instead of the destroyed one.
+  // stream has been synchronised): its buffers are then freed on the
+  // detached stream instead of the destroyed one.
   void detach_streams() {
-    nodes.s = nullptr;
-    leaf_rects.s = nullptr;
-    leaf_ids.s = nullptr;
-    leaf_keys.s = nullptr;
+    nodes.s = leaf_rects.s = leaf_ids.s = leaf_keys.s = detached_stream();
   }
   RectTree& operator=(RectTree&& o) noexcept {
     leaf = o.leaf;
@@ -329,6 +342,58 @@ struct KnnIndex {
 };
 void knn_points(const GeomColumn& build, KnnIndex& index, const GeomColumn& probe, int k, uint32_t* out_ids_dev,
                 double* out_d_dev, hipStream_t s, KernelTimer* timer = nullptr);
+
+// ---------------------------------------------------------------------------
+// Launch sequence without host synchronisation or allocation, for capture
+// in a HIP graph and replay over new inputs (the request loop): inputs are
+// described in device memory, counts are read on the device, buffers have
+// fixed capacities.
+
+// A column of 21-byte WKB points in device memory: data, row offsets
+// (int32, or int64 when large), row count.
+struct PointsDesc {
+  const uint8_t* data;
+  const void* offsets;
+  int64_t n;
+  int32_t large;
+  int32_t pad;
+};
+
+// Decodes rows [0, d->n) of the described column into xy/type, rows up to
+// cap as empty; rows that are not point records set *err.
+void decode_points_desc(const PointsDesc* desc_dev, int64_t cap, double2* xy, uint8_t* type, unsigned long long* err,
+                        hipStream_t s);
+void rects_of_points_into(const double2* xy, const uint8_t* type, int64_t n, bool point_nearest, double expand,
+                          float4* out, hipStream_t s);
+void gather_points_into(const double2* xy, const uint32_t* order, int64_t n, double2* out, hipStream_t s);
+
+// Morton order of n query rects in the tree's frame (invalid rects last):
+// keys/ids and their sorted copies, with `temp` of sort_temp_bytes(n).
+size_t morton_sort_temp_bytes(int64_t n);
+void morton_order_fixed(const RectTree& t, const float4* rects, int64_t n, uint32_t* keys, uint32_t* ids,
+                        uint32_t* keys_sorted, uint32_t* order, void* temp, size_t temp_bytes, hipStream_t s);
+
+// One filter launch into fixed buffers (pairs beyond cap are counted, not
+// written).
+void launch_filter(const RectTree& t, const float4* q, int64_t nq, const uint32_t* qorder, bool emit_rank,
+                   uint32_t* ob, uint32_t* op, uint64_t cap, unsigned long long* counter, hipStream_t s);
+
+// Fills n 64-bit words with v (a kernel: captured sequences record no
+// memset nodes).
+void fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
+
+// Refine launches over min(*n_dev, n_cap) candidates.  Point in polygon:
+// certified FP32 pass (uncertain candidates to redo/*redo_n) and the FP64
+// pass over them; stats[0..1] segment tests and uncertain tests.
+void launch_pip_refine_dev(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, int predicate,
+                           const uint32_t* cb, const uint32_t* cp, const unsigned long long* n_dev, uint64_t n_cap,
+                           const uint32_t* probe_ids, uint32_t* redo, unsigned long long* redo_n, uint32_t* ob,
+                           uint32_t* op, uint64_t out_cap, unsigned long long* out_n, unsigned long long* stats,
+                           int pip_waves, hipStream_t s);
+void launch_distance_refine_dev(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
+                                const uint32_t* cb, const uint32_t* cp, const unsigned long long* n_dev,
+                                uint64_t n_cap, uint32_t* ob, uint32_t* op, uint64_t out_cap,
+                                unsigned long long* out_n, unsigned long long* stats, hipStream_t s);
 
 // Checks that every pair's geometry types are supported by the relation
 // refine; throws otherwise.

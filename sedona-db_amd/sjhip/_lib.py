@@ -179,7 +179,7 @@ class SjhipTuneResult(ctypes.Structure):
 
 
 class SjhipServerConfig(ctypes.Structure):
-    _fields_ = [("slots", ctypes.c_uint32), ("memory_budget", ctypes.c_uint64)]
+    _fields_ = [("slots", ctypes.c_uint32), ("memory_budget", ctypes.c_uint64), ("graphs", ctypes.c_int32)]
 
 
 class SjhipRequestTiming(ctypes.Structure):

@@ -535,9 +535,9 @@ class GpuSpatialServer:
     batches joined concurrently on the device, admitted in arrival order by
     free slots and a device-memory budget (SjhipServer* entry points)."""
 
-    def __init__(self, index, refiner, slots=4, memory_budget=0):
+    def __init__(self, index, refiner, slots=4, memory_budget=0, graphs=False):
         self._index, self._refiner = index, refiner
-        cfg = L.SjhipServerConfig(int(slots), int(memory_budget))
+        cfg = L.SjhipServerConfig(int(slots), int(memory_budget), 1 if graphs else 0)
         h = ctypes.c_void_p()
         rc = L.lib().SjhipServerCreate(ctypes.byref(index.ix), ctypes.byref(refiner.rf), ctypes.byref(cfg),
                                        ctypes.byref(h))

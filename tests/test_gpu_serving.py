@@ -133,3 +133,28 @@ def test_request_error_is_reported_by_wait(resident):
         assert np.array_equal(gb, want[0]) and np.array_equal(gp, want[1])
     finally:
         sv.close()
+
+
+def test_graph_replays_equal_eager_joins(resident):
+    """With captured graphs, every replay (three rounds over the same size
+    buckets, with eager joins in between) returns each request's pairs as
+    the eager join does."""
+    R = resident
+    want = [alone(R, q) for q in R["reqs"]]
+    sv = G.GpuSpatialServer(R["ix"], R["rf"], slots=2, graphs=True)
+    try:
+        for rnd in range(3):
+            ids = [sv.submit(R["dev"].slice(q["lo"], q["hi"]), q["pred"], distance=q["d"], sort_output=True)
+                   for q in R["reqs"]]
+            for k, rid in enumerate(ids):
+                res, _ = sv.wait(rid)
+                gb, gp = res.to_host()
+                res.release()
+                assert np.array_equal(gb, want[k][0]) and np.array_equal(gp, want[k][1]), (rnd, k, len(gb))
+            # eager work between the replays
+            q = R["reqs"][rnd]
+            gb, gp = alone(R, q)
+            assert np.array_equal(gb, want[rnd][0]) and np.array_equal(gp, want[rnd][1])
+            say("graph round", rnd, "ok")
+    finally:
+        sv.close()
