@@ -343,8 +343,8 @@ def serving_measurement(G, ix, rf, serve, args):
             "memory_budget": "90% of free HBM", "one_at_a_time_service_ms": service_s * 1000,
             "one_at_a_time_capacity_rps": capacity, "rates": rates,
             "graphs": "batched_graphs replays one captured HIP graph per slot, predicate and power-of-two size "
-                      "bucket (requests up to 4M points); larger requests run the eager pipeline",
-            "p50_ms": rates[1]["batched_graphs"]["p50_ms"], "p99_ms": rates[1]["batched_graphs"]["p99_ms"]}
+                      "bucket for requests of up to 65536 points; larger requests run the eager pipeline",
+            "p50_ms": rates[1]["batched"]["p50_ms"], "p99_ms": rates[1]["batched"]["p99_ms"]}
 
 
 def read_traffic(path):

@@ -308,10 +308,10 @@ struct SjhipServerConfig {
   uint32_t slots;          /* concurrent requests (>= 1) */
   uint64_t memory_budget;  /* bytes; 0 = 90 % of the device memory free at creation */
   int32_t graphs;          /* 1 = replay captured HIP graphs for requests of
-                              point probes in device memory (up to 4M rows;
-                              point-in-polygon predicates and DWithin): one
-                              graph per slot, predicate and power-of-two size
-                              bucket, captured on first use */
+                              point probes in device memory (up to 65536
+                              rows; point-in-polygon predicates and DWithin):
+                              one graph per slot, predicate and power-of-two
+                              size bucket, captured on first use */
 };
 
 /* Timing of one request, ms on a steady clock since the server's

@@ -670,7 +670,7 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
 // point record or more candidates than C send the request down the eager
 // path (JoinEngine::run) instead.
 struct GraphPlan {
-  static constexpr int64_t kMinRows = 4096, kMaxRows = 1 << 22;
+  static constexpr int64_t kMinRows = 4096, kMaxRows = 1 << 16;
   int kind = 0;  // 0 point in polygon, 1 distance
   int predicate = 0;
   double distance = 0;
