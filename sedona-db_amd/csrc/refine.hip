@@ -189,7 +189,7 @@ __host__ __device__ __forceinline__ TreeShape seg_tree_shape(int ns) {
 
 struct TreeWalk {
   const float4* box;  // ring's boxes, level 0 first
-  TreeShape shape;
+  int levels;         // tree levels (level sizes follow from ns; see level_geom)
   float pxl, pyl, m;
   const double2* xy;  // ring's FP64 points
   const float2* q32;  // ring's FP32 local points
@@ -289,15 +289,28 @@ __device__ __forceinline__ int level_off(const TreeShape& t, int l) {
   return o;
 }
 
+// Offset (in boxes) and node count of level l of the tree over ns
+// segments, recomputed instead of held in registers.
+__device__ __forceinline__ void level_geom(int ns, int l, int& off, int& cnt) {
+  int c = (ns + kSegLeaf - 1) / kSegLeaf, o = 0;
+  for (int k = 0; k < l; k++) {
+    o += c;
+    c = (c + kSegFanout - 1) / kSegFanout;
+  }
+  off = o;
+  cnt = c;
+}
+
 // Depth-first walk without recursion.  The pending children of every level
 // are one byte each of `masks` (bit k of byte l: node base_l + k of level l
 // still to visit).  Children of node v are 8v .. 8v+7, so the base of the
 // level above is recovered as (base >> 3) & ~7.
 template <int PREC>
 __device__ __forceinline__ void walk_tree(TreeWalk& w) {
-  const int top = w.shape.levels - 1;
-  const int ntop = level_cnt(w.shape, top);
-  const float4* btop = w.box + level_off(w.shape, top);
+  const int top = w.levels - 1;
+  int otop, ntop;
+  level_geom(w.ns, top, otop, ntop);
+  const float4* btop = w.box + otop;
   for (int c0 = 0; c0 < ntop; c0 += kSegFanout) {
     uint64_t masks = (uint64_t)children_mask(w, btop + c0, min(kSegFanout, ntop - c0)) << (8 * top);
     int lvl = top, base = c0;
@@ -316,8 +329,9 @@ __device__ __forceinline__ void walk_tree(TreeWalk& w) {
         continue;
       }
       int cb = node * kSegFanout;
-      int n = min(kSegFanout, level_cnt(w.shape, lvl - 1) - cb);
-      uint32_t cm = children_mask(w, w.box + level_off(w.shape, lvl - 1) + cb, n);
+      int loff, lcnt;
+      level_geom(w.ns, lvl - 1, loff, lcnt);
+      uint32_t cm = children_mask(w, w.box + loff + cb, min(kSegFanout, lcnt - cb));
       if (cm) {
         lvl--;
         base = cb;
@@ -391,7 +405,7 @@ __device__ __forceinline__ bool ring_contains_run(const double2* xy, const float
   if (tree) {
     TreeWalk w;
     w.box = tree;
-    w.shape = seg_tree_shape((int)ns);
+    w.levels = seg_tree_shape((int)ns).levels;
     w.pxl = pxl;
     w.pyl = pyl;
     w.m = 2.0f * E;
