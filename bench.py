@@ -193,6 +193,10 @@ def main():
     #   refine: 8 B per candidate read + 8 B per kept pair written + the
     #           FP32 vertex runs of the build polygons (8 B per vertex, each
     #           distinct polygon counted once).
+    # The refine stage is three launches on one stream, timed together: the
+    # cell-raster pass (k_raster_pass), the listing of the candidates it left
+    # open (k_todo_scatter) and the certified FP32 walk over them
+    # (k_refine_pip<2>); its measured traffic is the sum of theirs.
     # The refine's arithmetic roof (FP32 VALU, 8 flops per segment test) is
     # reported beside its HBM roof.
     filt_ms, ref_ms = stage[2], stage[3]
@@ -208,7 +212,8 @@ def main():
     roof_filter["frac"] = roof_filter["achieved"] / HBM_PEAK_GBS
     ach_tf = refine_flops / (ref_ms / 1e3) / 1e12
     roof_refine = dict(bound="hbm", achieved=refine_bytes / (ref_ms / 1e3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
-                       traffic=traffic.get("k_refine_pip") if full else None, kernel="k_refine_pip", ms=ref_ms,
+                       traffic=traffic.get("refine_stage") if full else None,
+                       kernel="k_raster_pass+k_todo_scatter+k_refine_pip<2>", ms=ref_ms,
                        algorithmic_bytes=refine_bytes,
                        valu_fp32={"achieved": ach_tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                                   "frac": ach_tf / FP32_PEAK_TFLOPS, "flops": refine_flops})

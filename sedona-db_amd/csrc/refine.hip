@@ -815,7 +815,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8
       double2 pt = c.P.xy[p];
       if (pt.x == pt.x || pt.y == pt.y) {
         const PolyDesc d = c.R.desc[b];
-        if (d.npts >= 0) {
+        // cell raster: inside / outside away from the boundary, no walk
+        int cls = 0;
+        if (d.raster_off >= 0 && rect_covers(d.rect, pt.x, pt.y)) {
+          const double cw = (d.rect.z - d.rect.x) / kRasterCells, ch = (d.rect.w - d.rect.y) / kRasterCells;
+          const int ci = min(max((int)floor((pt.x - d.rect.x) / cw), 0), kRasterCells - 1);
+          const int cj = min(max((int)floor((pt.y - d.rect.y) / ch), 0), kRasterCells - 1);
+          const int cell = cj * kRasterCells + ci;
+          cls = (c.R.raster[d.raster_off + (cell >> 4)] >> (2 * (cell & 15))) & 3;
+        }
+        if (cls) {
+          keep = cls == 1;
+        } else if (d.npts >= 0) {
           keep = ring_contains_at<PRECISION>(c, d.rect, d.v0, d.npts, ring_origin(d.rect), d.err, d.tree_off, pt.x,
                                              pt.y, allow_on_edge != 0, tests, fallbacks);
         } else if (c.B.type[b] != kNull) {
@@ -904,6 +915,104 @@ __global__ __launch_bounds__(kBlock) void k_refine_polypoly(Ctx c, const uint32_
     emit(wo, lane == 0 && keep, b, p, ob, op, cap, counter);
   }
   wave_flush(wo, ob, op, cap, counter);
+}
+
+// Cell-raster pass of the point-in-polygon refine: candidates whose point
+// falls in a clean cell of a single-ring polygon are decided here (kept or
+// dropped); the others are marked for the walk in per-wavefront bit masks
+// (mask[i / 64], bit i % 64) and counted per tile of kRasterTile
+// consecutive candidates (tile_cnt).  k_todo_scatter then lists them in
+// candidate order, so that the walk keeps the polygon-major order of its
+// input and its wavefronts are not held back by candidates that need no
+// walk.
+constexpr int kRasterTile = 4096;
+
+__global__ __launch_bounds__(kBlock) void k_raster_pass(Ctx c, const uint32_t* cb, const uint32_t* cp, uint64_t n,
+                                                        const uint32_t* probe_ids, uint32_t* ob, uint32_t* op,
+                                                        uint64_t cap, unsigned long long* counter, uint64_t* mask,
+                                                        unsigned long long* tile_cnt) {
+  SJ_WAVE_OUT(wo);
+  __shared__ unsigned int block_cnt;
+  const uint64_t n_tiles = (n + kRasterTile - 1) / kRasterTile;
+  const int lane = threadIdx.x & 63;
+  for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    if (threadIdx.x == 0) block_cnt = 0;
+    __syncthreads();
+    unsigned int wave_cnt = 0;
+    for (int it = 0; it < kRasterTile / kBlock; it++) {
+      const uint64_t i = t * kRasterTile + (uint64_t)it * kBlock + threadIdx.x;
+      bool keep = false, walk = false;
+      uint32_t b = 0, p = 0;
+      if (i < n) {
+        b = cb[i];
+        p = cp[i];
+        const double2 pt = c.P.xy[p];
+        if (pt.x == pt.x || pt.y == pt.y) {
+          const PolyDesc d = c.R.desc[b];
+          int cls = 0;
+          if (d.raster_off >= 0) {
+            if (!rect_covers(d.rect, pt.x, pt.y)) {
+              cls = 2;
+            } else {
+              const double cw = (d.rect.z - d.rect.x) / kRasterCells, ch = (d.rect.w - d.rect.y) / kRasterCells;
+              const int ci = min(max((int)floor((pt.x - d.rect.x) / cw), 0), kRasterCells - 1);
+              const int cj = min(max((int)floor((pt.y - d.rect.y) / ch), 0), kRasterCells - 1);
+              const int cell = cj * kRasterCells + ci;
+              cls = (c.R.raster[d.raster_off + (cell >> 4)] >> (2 * (cell & 15))) & 3;
+            }
+          }
+          keep = cls == 1;
+          walk = cls == 0;
+        }
+        if (keep && probe_ids) p = probe_ids[p];
+      }
+      emit(wo, keep, b, p, ob, op, cap, counter);
+      // i - lane is a multiple of 64: one mask word per wavefront step
+      const uint64_t m = __ballot(walk);
+      if (lane == 0 && i < n) mask[i >> 6] = m;
+      wave_cnt += __popcll(m);
+    }
+    if (lane == 0 && wave_cnt) atomicAdd(&block_cnt, wave_cnt);
+    __syncthreads();
+    if (threadIdx.x == 0) tile_cnt[t] = block_cnt;
+  }
+  wave_flush(wo, ob, op, cap, counter);
+}
+
+// Lists the candidates marked by k_raster_pass: tile t's entries start at
+// tile_off[t] (exclusive scan of the tile counts) and keep candidate order.
+__global__ __launch_bounds__(kBlock) void k_todo_scatter(const uint64_t* mask, const unsigned long long* tile_off,
+                                                         uint64_t n, uint32_t* todo) {
+  constexpr int kWords = kRasterTile / 64;
+  __shared__ unsigned int word_off[kWords];
+  const uint64_t n_tiles = (n + kRasterTile - 1) / kRasterTile;
+  const uint64_t n_words = (n + 63) / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    __syncthreads();
+    if (threadIdx.x < kWords) {
+      const uint64_t w = t * kWords + threadIdx.x;
+      word_off[threadIdx.x] = w < n_words ? __popcll(mask[w]) : 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned int acc = 0;
+      for (int j = 0; j < kWords; j++) {
+        const unsigned int v = word_off[j];
+        word_off[j] = acc;
+        acc += v;
+      }
+    }
+    __syncthreads();
+    const unsigned long long base = tile_off[t];
+    for (int j = wave; j < kWords; j += kBlock / 64) {
+      const uint64_t w = t * kWords + j;
+      if (w >= n_words) break;
+      const uint64_t m = mask[w];
+      if ((m >> lane) & 1)
+        todo[base + word_off[j] + __popcll(m & ((1ull << lane) - 1))] = (uint32_t)(w * 64 + lane);
+    }
+  }
 }
 
 // One lane per candidate; long-ring candidates are deferred into a list.
@@ -1425,7 +1534,7 @@ __global__ void k_poly_desc(GeomView g, RingIndexView R, PolyDesc* out) {
   d.tree_off = -1;
   d.npts = -1;
   d.err = INFINITY;
-  d.pad = 0;
+  d.raster_off = -1;
   const int t = g.type[r];
   if ((is_polyish(t) || t == kLine || t == kMultiLine) && g.part_off[r + 1] - g.part_off[r] == 1) {
     int64_t q = g.part_off[r];
@@ -1442,6 +1551,71 @@ __global__ void k_poly_desc(GeomView g, RingIndexView R, PolyDesc* out) {
     }
   }
   out[r] = d;
+}
+
+// Rows that get a cell raster: one polygon ring with a segment tree and a
+// rect of positive width and height.
+__global__ void k_raster_flags(GeomView g, const PolyDesc* desc, int64_t* flag) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= g.n_rows) return;
+  const PolyDesc d = desc[r];
+  flag[r] = is_polyish(g.type[r]) && d.npts >= 4 && d.tree_off >= 0 && d.rect.z > d.rect.x && d.rect.w > d.rect.y &&
+                    d.err < INFINITY
+                ? 1
+                : 0;
+}
+
+// Cell raster of one ring per wavefront.  A cell is left 0 when a segment's
+// bounding box meets it or one of its 8 neighbours; so the point-in-ring
+// result is the same at every point within one cell of a marked-clean
+// cell: the reference's raycast decides each segment there by exact
+// comparisons (the point lies outside every segment's box), and the
+// crossing parity is constant off the boundary.  Clean cells get their
+// centre's result (the certified FP32 walk, FP64 where uncertain).
+__global__ __launch_bounds__(kBlock) void k_raster_build(Ctx c, PolyDesc* desc, const int64_t* flag,
+                                                        const int64_t* roff, uint32_t* raster) {
+  __shared__ uint32_t s_touch[kBlock / 64][kRasterCells * kRasterCells / 32];
+  const int lane = threadIdx.x & 63;
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (row >= c.B.n_rows || !flag[row]) return;  // wave-uniform
+  uint32_t* touch = s_touch[threadIdx.x >> 6];
+  for (int k = lane; k < kRasterCells * kRasterCells / 32; k += 64) touch[k] = 0;
+  wave_lds_sync();
+  PolyDesc d = desc[row];
+  const double4 rb = d.rect;
+  const double cw = (rb.z - rb.x) / kRasterCells, ch = (rb.w - rb.y) / kRasterCells;
+  const double2* xy = c.B.xy + d.v0;
+  const int64_t ns = d.npts - 1;
+  for (int64_t j = lane; j < ns; j += 64) {
+    const double2 a = xy[j], b = xy[j + 1];
+    int i0 = (int)floor((fmin(a.x, b.x) - rb.x) / cw) - 1, i1 = (int)floor((fmax(a.x, b.x) - rb.x) / cw) + 1;
+    int j0 = (int)floor((fmin(a.y, b.y) - rb.y) / ch) - 1, j1 = (int)floor((fmax(a.y, b.y) - rb.y) / ch) + 1;
+    i0 = max(i0, 0); j0 = max(j0, 0);
+    i1 = min(i1, kRasterCells - 1); j1 = min(j1, kRasterCells - 1);
+    for (int cy = j0; cy <= j1; cy++)
+      for (int cx = i0; cx <= i1; cx++) {
+        const int cell = cy * kRasterCells + cx;
+        atomicOr(&touch[cell >> 5], 1u << (cell & 31));
+      }
+  }
+  wave_lds_sync();
+  // lane k < kRasterWords writes word k: 16 cells
+  if (lane < kRasterWords) {
+    uint32_t word = 0;
+    uint64_t t = 0, f = 0;
+    const double2 o = ring_origin(rb);
+    const float4* tree = c.R.seg_tree + d.tree_off;
+    const float2* q32 = c.R.xy32 + d.v0;
+    for (int k = 0; k < 16; k++) {
+      const int cell = lane * 16 + k;
+      if ((touch[cell >> 5] >> (cell & 31)) & 1u) continue;
+      const double px = rb.x + ((cell % kRasterCells) + 0.5) * cw, py = rb.y + ((cell / kRasterCells) + 0.5) * ch;
+      const bool in = ring_contains_run<0>(xy, q32, tree, d.npts, rb, o, d.err, px, py, true, t, f);
+      word |= (in ? 1u : 2u) << (2 * k);
+    }
+    raster[roff[row] * kRasterWords + lane] = word;
+  }
+  if (lane == 0) desc[row].raster_off = roff[row] * kRasterWords;
 }
 
 }  // namespace
@@ -1484,6 +1658,32 @@ void build_ring_index(const GeomColumn& g, RingIndex& ri, const RingIndexParams&
   ri.desc.alloc(g.n_rows > 0 ? g.n_rows : 1, s);
   if (g.n_rows > 0) hipLaunchKernelGGL(k_poly_desc, blocks_for(g.n_rows), kBlock, 0, s, g.view(), ri.view(), ri.desc.p);
   SJ_CHECK(hipGetLastError());
+  // cell rasters of the single-ring polygon rows with segment trees
+  if (g.n_rows > 0 && prm.raster) {
+    DevArray<int64_t> flag(g.n_rows, s), roff(g.n_rows + 1, s);
+    hipLaunchKernelGGL(k_raster_flags, blocks_for(g.n_rows), kBlock, 0, s, g.view(), ri.desc.p, flag.p);
+    size_t tb = 0;
+    SJ_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, flag.p, roff.p, (int)g.n_rows + 1, s));
+    DevArray<uint8_t> tmp(tb + 16, s);
+    // the scan runs over n_rows + 1 entries so that roff[n_rows] is the total
+    DevArray<int64_t> flag1(g.n_rows + 1, s);
+    SJ_CHECK(hipMemcpyAsync(flag1.p, flag.p, g.n_rows * 8, hipMemcpyDeviceToDevice, s));
+    SJ_CHECK(hipMemsetAsync(flag1.p + g.n_rows, 0, 8, s));
+    SJ_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, flag1.p, roff.p, (int)g.n_rows + 1, s));
+    int64_t total = 0;
+    SJ_CHECK(hipMemcpyAsync(&total, roff.p + g.n_rows, 8, hipMemcpyDeviceToHost, s));
+    SJ_CHECK(hipStreamSynchronize(s));
+    ri.n_rasters = total;
+    if (total > 0) {
+      ri.raster.alloc((size_t)total * kRasterWords, s);
+      Ctx c;
+      c.B = g.view();
+      c.R = ri.view();
+      hipLaunchKernelGGL(k_raster_build, blocks_for((uint64_t)g.n_rows * 64), kBlock, 0, s, c, ri.desc.p, flag.p,
+                         roff.p, ri.raster.p);
+      SJ_CHECK(hipGetLastError());
+    }
+  }
   SJ_CHECK(hipStreamSynchronize(s));
 }
 
@@ -1529,12 +1729,12 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
   c.B = build.view();
   c.R = ri.view();
   c.P = probe.view();
-  DevArray<unsigned long long> aux(4, s);  // counter of long list, stats[2], bad flag
+  DevArray<unsigned long long> aux(5, s);  // counter of long list, stats[2], bad flag, raster todo count
   DevArray<uint32_t> long_list;
   bool pip = pip_refine_applies(build, ri, probe, predicate);
   if (probe_ids && !pip) throw Error("probe id mapping needs the point-in-polygon refine");
   SJ_CHECK(hipMemsetAsync(out.counter.p, 0, 8, s));
-  SJ_CHECK(hipMemsetAsync(aux.p, 0, 32, s));
+  SJ_CHECK(hipMemsetAsync(aux.p, 0, 40, s));
   if (pip) {
     int allow = predicate == kContains ? 0 : 1;
     if (timer) timer->record_start(s);
@@ -1542,9 +1742,30 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
       // certified FP32 pass; candidates with an uncertain test (aux[0] of
       // them) are refined again in FP64 by the second launch
       DevArray<uint32_t> redo(n, s);
+      // candidates in clean raster cells are decided first; the walk takes
+      // the rest from a list (count in aux[4])
+      DevArray<uint32_t> todo;
+      if (ri.n_rasters > 0) {
+        const uint64_t n_tiles = (n + kRasterTile - 1) / kRasterTile;
+        DevArray<uint64_t> mask((n + 63) / 64, s);
+        DevArray<unsigned long long> cnt(n_tiles + 1, s), off(n_tiles + 1, s);
+        todo.alloc(n, s);
+        SJ_CHECK(hipMemsetAsync(cnt.p + n_tiles, 0, 8, s));
+        const unsigned int g = (unsigned int)std::min<uint64_t>(n_tiles, 4096);
+        hipLaunchKernelGGL(k_raster_pass, g, kBlock, 0, s, c, cb, cp, n, probe_ids, out.b.p, out.p.p,
+                           (uint64_t)out.b.n, out.counter.p, mask.p, cnt.p);
+        size_t tb = 0;
+        SJ_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt.p, off.p, (int)(n_tiles + 1), s));
+        DevArray<uint8_t> tmp(tb, s);
+        SJ_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, cnt.p, off.p, (int)(n_tiles + 1), s));
+        hipLaunchKernelGGL(k_todo_scatter, g, kBlock, 0, s, mask.p, off.p, n, todo.p);
+        // the walk's candidate count: the scan's last entry
+        SJ_CHECK(hipMemcpyAsync(aux.p + 4, off.p + n_tiles, 8, hipMemcpyDeviceToDevice, s));
+      }
       auto fp32_pass = [&](auto kernel) {
-        hipLaunchKernelGGL(kernel, grid_for(n), kBlock, 0, s, c, allow, cb, cp, n, nullptr, out.b.p, out.p.p,
-                           (uint64_t)out.b.n, out.counter.p, aux.p + 1, probe_ids, redo.p, aux.p, nullptr);
+        hipLaunchKernelGGL(kernel, grid_for(n), kBlock, 0, s, c, allow, cb, cp, n, todo.p, out.b.p, out.p.p,
+                           (uint64_t)out.b.n, out.counter.p, aux.p + 1, probe_ids, redo.p, aux.p,
+                           todo.p ? aux.p + 4 : nullptr);
       };
       // resident waves per SIMD (an autotuned choice; registers spill beyond 4)
       if (pip_waves == 4) fp32_pass(k_refine_pip<kPrecDeferred, 4>);
@@ -1581,6 +1802,7 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
   if (predicate == kIntersects && (build.type_mask & polys) && (probe.type_mask & polys) && !probe.points_only &&
       c.R.ring_rect != nullptr) {
     RingIndexParams prm;
+    prm.raster = false;  // rasters serve point probes only
     build_ring_index(probe, probe_ri, prm, s);
     c.PR = probe_ri.view();
   }

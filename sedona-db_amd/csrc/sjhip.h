@@ -117,17 +117,24 @@ constexpr int kSegLeaf = 16;
 constexpr int kSegFanout = 8;
 constexpr int kSegMaxLevels = 5;
 
-// Record of a build row that is one polygon with one ring (npts >= 0): its
-// ring's rect, first vertex, vertex count, FP32 bound and segment-tree
-// offset (-1: none).  npts = -1 marks every other row.
+// Record of a build row that is one polygon with one ring, or one line
+// (npts >= 0): its ring's rect, first vertex, vertex count, FP32 bound,
+// segment-tree offset (-1: none) and cell-raster offset (-1: none).
+// npts = -1 marks every other row.
 struct alignas(32) PolyDesc {
   double4 rect;
   int64_t v0;
   int64_t tree_off;
   int32_t npts;
   float err;
-  int64_t pad;
+  int64_t raster_off;
 };
+
+// Cell raster of a polygon ring: kRasterCells x kRasterCells cells over
+// the ring's rect, 2 bits each (0: near the boundary, 1: inside, 2:
+// outside), kRasterWords 32-bit words per ring.
+constexpr int kRasterCells = 32;
+constexpr int kRasterWords = kRasterCells * kRasterCells / 16;
 
 // Device view of the ring refine index (build side).
 struct RingIndexView {
@@ -138,6 +145,7 @@ struct RingIndexView {
   const int64_t* seg_tree_off = nullptr;  // per ring: first box in seg_tree, -1 = not indexed
   const float4* seg_tree = nullptr;       // boxes (xmin, ymin, xmax, ymax), ring-local FP32
   const PolyDesc* desc = nullptr;         // per build row
+  const uint32_t* raster = nullptr;       // cell rasters (see PolyDesc::raster_off)
 };
 
 // Owned geometry column on the device.
@@ -169,7 +177,9 @@ struct RingIndex {
   DevArray<int64_t> seg_tree_off;
   DevArray<float4> seg_tree;
   DevArray<PolyDesc> desc;
+  DevArray<uint32_t> raster;
   int64_t n_boxes = 0;
+  int64_t n_rasters = 0;
   RingIndexView view() const {
     RingIndexView v;
     v.ring_rect = ring_rect.p;
@@ -179,6 +189,7 @@ struct RingIndex {
     v.seg_tree_off = seg_tree_off.p;
     v.seg_tree = seg_tree.p;
     v.desc = desc.p;
+    v.raster = raster.p;
     return v;
   }
 };
@@ -214,6 +225,7 @@ void compute_rects(const GeomColumn& g, const RingIndex* ri, bool point_nearest,
 // Ring refine index for a build column.
 struct RingIndexParams {
   int min_segs_for_tree = 32;  // rings with fewer segments are scanned whole
+  bool raster = true;          // cell rasters for single-ring polygon rows with trees
 };
 void build_ring_index(const GeomColumn& g, RingIndex& ri, const RingIndexParams& p, hipStream_t s);
 

@@ -179,6 +179,58 @@ def test_device_join_zipf_stars_vs_oracle(pred, precision, opts):
         say("  fallback rate", r.fallback_tests / max(1, r.segment_tests))
 
 
+def _raster_probe_build():
+    """Single-ring polygons with trees (and so cell rasters): stars of 40 to
+    3000 vertices and a staircase whose edges lie on the raster's cell lines
+    (rect [0, 32]^2, cell width 1).  Probes: the lattice of cell corners and
+    cell centres (one cell beyond the rect on every side), every vertex and
+    edge midpoint, and random points in each rect."""
+    rng = np.random.default_rng(31)
+    rings = []
+    for k, nv in enumerate([40, 200, 1000, 3000]):
+        ang = np.sort(rng.random(nv)) * 2 * np.pi
+        r = 10.0 * (0.6 + 0.4 * rng.random(nv))
+        cx, cy = 100.0 + 40.0 * k, 50.0
+        ring = np.stack([cx + r * np.cos(ang), cy + r * np.sin(ang)], axis=1)
+        rings.append(np.vstack([ring, ring[:1]]))
+    st = [(32.0, 0.0)]
+    for j in range(32):
+        st += [(32.0 - j, j + 1.0), (31.0 - j, j + 1.0)]
+    st += [(0.0, 0.0), (32.0, 0.0)]
+    rings.append(np.array(st))
+    counts = [len(r) for r in rings]
+    b = D.rings_wkb(counts, np.vstack(rings), 3)
+    pts = []
+    for ring in rings:
+        x0, y0 = ring.min(axis=0)
+        x1, y1 = ring.max(axis=0)
+        cw, ch = (x1 - x0) / 32, (y1 - y0) / 32
+        for t in (np.arange(-1, 34, dtype=np.float64), np.arange(-1, 33, dtype=np.float64) + 0.5):
+            gx, gy = np.meshgrid(x0 + t * cw, y0 + t * ch)
+            pts.append(np.stack([gx.ravel(), gy.ravel()], axis=1))
+        pts.append(ring[:-1])
+        pts.append(0.5 * (ring[:-1] + ring[1:]))
+        pts.append(np.stack([x0 + (x1 - x0) * rng.random(2000), y0 + (y1 - y0) * rng.random(2000)], axis=1))
+    xy = np.vstack(pts)
+    return b, D.points_wkb(xy[:, 0].copy(), xy[:, 1].copy())
+
+
+@pytest.mark.parametrize("pred", ["contains", "intersects"])
+@pytest.mark.parametrize("sort_probes", [True, False])
+def test_cell_raster_boundaries_vs_oracle(pred, sort_probes, opts):
+    """Points on the cell lines, in the cell centres, on vertices and edges
+    of polygons with cell rasters: the certified path (raster pass, walk,
+    FP64 redo) gives the oracle's pairs."""
+    b, p = _raster_probe_build()
+    say("raster boundaries", pred, sort_probes, "probes", len(p.offsets) - 1)
+    ix, rf = build_pair(b.to_arrow(), opts)
+    r = G.join_device(ix, rf, p.to_arrow(), pred, precision=0, sort_probes=sort_probes)
+    gb, gp = r.to_host()
+    ob, op = O.join(O.Store(b.as_tuple()), O.Store(p.as_tuple()), pred, nthreads=8)
+    assert len(ob) > 10_000
+    assert np.array_equal(gb, ob) and np.array_equal(gp, op), (len(gb), len(ob))
+
+
 @pytest.mark.parametrize("refine_order", [1, 2])
 def test_dwithin_random_walks_vs_oracle(refine_order, opts):
     # ~160 candidate boxes per point at this line density (as at full size)
