@@ -241,8 +241,9 @@ struct SjhipJoinResult {
   uint64_t segment_tests;   /* segment tests performed */
   /* device time of the stages (ms, HIP events on the join's stream) when
    * options.profile is set: [0] probe decode + rects, [1] Morton order of
-   * probes, [2] filter kernel, [3] refine kernels, [4] output sort,
-   * [5] candidate regrouping (refine_order 1) */
+   * probes, [2] filter kernel, [3] refine kernels, [4] copy of the kept
+   * pairs into the result (exact size) and output sort, [5] candidate
+   * regrouping (refine_order 1) */
   double stage_ms[6];
   void (*release)(struct SjhipJoinResult* self);
   void* private_data;

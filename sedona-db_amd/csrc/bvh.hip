@@ -213,7 +213,7 @@ uint64_t filter_pairs(const RectTree& t, const float4* q, int64_t nq, const uint
   // earlier probes, else 4 per query; one resize-and-rerun on overflow
   uint64_t per16 = t.cand_per_query_x16.load(std::memory_order_relaxed);
   size_t want = std::max<int64_t>(1 << 16, per16 ? (int64_t)((per16 * (uint64_t)nq) / 16 * 9 / 8 + 1024) : nq * 4);
-  if (pb.b.n < want) {
+  if (pb.cap() < want) {
     pb.b.alloc(want, s);
     pb.p.alloc(want, s);
   }
@@ -224,20 +224,20 @@ uint64_t filter_pairs(const RectTree& t, const float4* q, int64_t nq, const uint
     if (timer) timer->record_start(s);
     if (t.leaf == 4)
       hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 4, KeepAll>), blocks_for(nq), kBlock, 0, s, v, q, nq, qorder,
-                         emit_rank ? 1 : 0, KeepAll{}, pb.b.p, pb.p.p, (uint64_t)pb.b.n, pb.counter.p);
+                         emit_rank ? 1 : 0, KeepAll{}, pb.b.p, pb.p.p, pb.cap(), pb.counter.p);
     else if (t.leaf == 16)
       hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 16, KeepAll>), blocks_for(nq), kBlock, 0, s, v, q, nq, qorder,
-                         emit_rank ? 1 : 0, KeepAll{}, pb.b.p, pb.p.p, (uint64_t)pb.b.n, pb.counter.p);
+                         emit_rank ? 1 : 0, KeepAll{}, pb.b.p, pb.p.p, pb.cap(), pb.counter.p);
     else
       hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 8, KeepAll>), blocks_for(nq), kBlock, 0, s, v, q, nq, qorder,
-                         emit_rank ? 1 : 0, KeepAll{}, pb.b.p, pb.p.p, (uint64_t)pb.b.n, pb.counter.p);
+                         emit_rank ? 1 : 0, KeepAll{}, pb.b.p, pb.p.p, pb.cap(), pb.counter.p);
     SJ_CHECK(hipGetLastError());
     if (timer) timer->record_stop(s);
     unsigned long long total = 0;
     SJ_CHECK(hipMemcpyAsync(&total, pb.counter.p, 8, hipMemcpyDeviceToHost, s));
     SJ_CHECK(hipStreamSynchronize(s));
     if (nq > 0) t.cand_per_query_x16.store((uint64_t)((total * 16) / (uint64_t)nq + 1), std::memory_order_relaxed);
-    if (total <= pb.b.n) return total;
+    if (total <= pb.cap()) return total;
     size_t cap = (size_t)(total + total / 8 + 1024);
     pb.b.alloc(cap, s);
     pb.p.alloc(cap, s);

@@ -267,6 +267,38 @@ struct RefinerImpl {
   std::mutex knn_mu;
   bool built = false;
   std::mutex mu;
+  // device scratch of the joins against this build side, one workspace per
+  // join running at a time, kept between joins
+  std::mutex ws_mu;
+  std::vector<std::unique_ptr<Workspace>> ws_free;
+};
+
+// A join's workspace: taken from the refiner's free list (or new) for the
+// join's stream, and given back when the join ends, once the stream has
+// drained (on failure too), with the detached stream as its buffers' owner.
+struct WorkspaceLease {
+  RefinerImpl* rf;
+  hipStream_t s;
+  std::unique_ptr<Workspace> ws;
+  WorkspaceLease(RefinerImpl* r, hipStream_t stream) : rf(r), s(stream) {
+    {
+      std::lock_guard<std::mutex> g(rf->ws_mu);
+      if (!rf->ws_free.empty()) {
+        ws = std::move(rf->ws_free.back());
+        rf->ws_free.pop_back();
+      }
+    }
+    if (!ws) ws.reset(new Workspace());
+    ws->set_stream(s);
+  }
+  ~WorkspaceLease() {
+    (void)hipStreamSynchronize(s);
+    ws->set_stream(detached_stream());
+    std::lock_guard<std::mutex> g(rf->ws_mu);
+    rf->ws_free.push_back(std::move(ws));
+  }
+  WorkspaceLease(const WorkspaceLease&) = delete;
+  WorkspaceLease& operator=(const WorkspaceLease&) = delete;
 };
 
 RefinerImpl* refiner_of(SedonaSpatialRefiner* self) {
@@ -607,7 +639,11 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
     mark(6);
     mark(7);
   } else {
-    PairBuf cand;
+    // candidates and the refine's output live in the workspace; the
+    // result gets the kept pairs at their exact size
+    WorkspaceLease lease(rf, s);
+    Workspace* ws = lease.ws.get();
+    PairBuf& cand = ws->cand;
     nc = filter_pairs(ix->tree, q.p, nq, ord, cand, s, ev.on ? &ft : nullptr, ranked);
     mark(3);
     // polygon-major order: one wavefront then walks few distinct rings
@@ -617,13 +653,21 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
       sort_pairs_by_build(cand, nc, rf->build.n_rows, br, rp, s);
     }
     if (dwithin)
-      kept = refine_distance(rf->build, rf->ring_index, probe, o.distance, cand.b.p, cand.p.p, nc, jr->pairs, s,
-                             ev.on ? &rt : nullptr, o.precision, &st);
+      kept = refine_distance(rf->build, rf->ring_index, probe, o.distance, cand.b.p, cand.p.p, nc, ws->out, s,
+                             ev.on ? &rt : nullptr, o.precision, &st, ws);
     else
       kept = refine_relation(rf->build, rf->ring_index, rp, o.predicate, cand.b.p, cand.p.p, nc, o.precision,
-                             jr->pairs, &st, s, ev.on ? &rt : nullptr, ranked ? ord : nullptr, ix->tuning.pip_waves);
+                             ws->out, &st, s, ev.on ? &rt : nullptr, ranked ? ord : nullptr, ix->tuning.pip_waves,
+                             ws);
+    mark(8);
+    jr->pairs.b.alloc(kept > 0 ? kept : 1, s);
+    jr->pairs.p.alloc(kept > 0 ? kept : 1, s);
+    if (kept) {
+      SJ_CHECK(hipMemcpyAsync(jr->pairs.b.p, ws->out.b.p, kept * 4, hipMemcpyDeviceToDevice, s));
+      SJ_CHECK(hipMemcpyAsync(jr->pairs.p.p, ws->out.p.p, kept * 4, hipMemcpyDeviceToDevice, s));
+    }
   }
-  mark(8);
+  if (fused) mark(8);
   if (o.sort_output) sort_pairs(jr->pairs.b.p, jr->pairs.p.p, kept, s);
   mark(9);
   SJ_CHECK(hipStreamSynchronize(s));
@@ -855,7 +899,7 @@ struct Server {
   // Device bytes a request holds at its peak: the decoded probe column,
   // rects, Morton keys and order, the ordered copy of points, and per
   // candidate the pair arrays, their regrouping copies and keys, the
-  // deferred list and the kept pairs.  Candidates per probe come from the
+  // deferred list, the raster pass's list and marks, and the kept pairs.  Candidates per probe come from the
   // index's last probe (8 before any).
   uint64_t estimate(const WkbInput& in, const SjhipJoinOptions& o) const {
     uint64_t n = (uint64_t)in.length;
@@ -864,7 +908,7 @@ struct Server {
     if (o.predicate == 100) cpq *= 2.0;  // expanded probe rects
     uint64_t cand = (uint64_t)(cpq * (double)n * 1.125) + 1024;
     uint64_t wkb = in.on_device ? 0 : (uint64_t)(in.large ? 8 : 4) * (n + 1) + 32 * n;
-    return wkb + 96 * n + 44 * cand + (1u << 20);
+    return wkb + 96 * n + 49 * cand + (1u << 20);
   }
 
   // The kind of captured sequence a request can replay (0 point in

@@ -1015,6 +1015,59 @@ __global__ __launch_bounds__(kBlock) void k_todo_scatter(const uint64_t* mask, c
   }
 }
 
+// Cell-raster pass of the distance refine over single-line build rows and
+// point probes: a candidate is dropped when no raster cell that the square
+// [p - d, p + d] meets (widened by one cell) may hold a segment, since then
+// every segment is farther than d on one axis; a point outside the line's
+// rect by more than d, or with no coordinates, is dropped as well.  The
+// others are marked for the walk as in k_raster_pass.
+__global__ __launch_bounds__(kBlock) void k_dist_raster_pass(Ctx c, double dist, const uint32_t* cb,
+                                                             const uint32_t* cp, uint64_t n, uint64_t* mask,
+                                                             unsigned long long* tile_cnt) {
+  __shared__ unsigned int block_cnt;
+  const uint64_t n_tiles = (n + kRasterTile - 1) / kRasterTile;
+  const int lane = threadIdx.x & 63;
+  for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    if (threadIdx.x == 0) block_cnt = 0;
+    __syncthreads();
+    unsigned int wave_cnt = 0;
+    for (int it = 0; it < kRasterTile / kBlock; it++) {
+      const uint64_t i = t * kRasterTile + (uint64_t)it * kBlock + threadIdx.x;
+      bool walk = false;
+      if (i < n) {
+        const double2 pt = c.P.xy[cp[i]];
+        if (pt.x == pt.x || pt.y == pt.y) {
+          const PolyDesc d = c.R.desc[cb[i]];
+          walk = true;
+          if (d.raster_off >= 0) {
+            const double cw = (d.rect.z - d.rect.x) / kLineCells, ch = (d.rect.w - d.rect.y) / kLineCells;
+            const double fx0 = floor((pt.x - dist - d.rect.x) / cw) - 1.0, fx1 = floor((pt.x + dist - d.rect.x) / cw) + 1.0;
+            const double fy0 = floor((pt.y - dist - d.rect.y) / ch) - 1.0, fy1 = floor((pt.y + dist - d.rect.y) / ch) + 1.0;
+            if (fx1 < 0.0 || fy1 < 0.0 || fx0 > kLineCells - 1.0 || fy0 > kLineCells - 1.0) {
+              walk = false;
+            } else if (fy1 - fy0 < 16.0) {
+              const int i0 = (int)fmax(fx0, 0.0), i1 = (int)fmin(fx1, kLineCells - 1.0);
+              const int j0 = (int)fmax(fy0, 0.0), j1 = (int)fmin(fy1, kLineCells - 1.0);
+              if (i0 <= i1 && j0 <= j1) {
+                const uint64_t m = (~0ull >> (63 - (i1 - i0))) << i0;
+                walk = false;
+                for (int r = j0; r <= j1 && !walk; r++) walk = (c.R.line_raster[d.raster_off + r] & m) != 0;
+              }
+            }
+          }
+        }
+      }
+      // i - lane is a multiple of 64: one mask word per wavefront step
+      const uint64_t m = __ballot(walk);
+      if (lane == 0 && i < n) mask[i >> 6] = m;
+      wave_cnt += __popcll(m);
+    }
+    if (lane == 0 && wave_cnt) atomicAdd(&block_cnt, wave_cnt);
+    __syncthreads();
+    if (threadIdx.x == 0) tile_cnt[t] = block_cnt;
+  }
+}
+
 // One lane per candidate; long-ring candidates are deferred into a list.
 __global__ __launch_bounds__(kBlock) void k_refine_lane(Ctx c, int pred, int precision, const uint32_t* cb,
                                                         const uint32_t* cp, uint64_t n, uint32_t* ob, uint32_t* op,
@@ -1331,9 +1384,9 @@ inline int dist_kind(uint32_t type_mask, bool points_only, const RingIndex& ri) 
 // point-in-polygon walk, measured slower: the spills cost more.)
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_refine_distance(Ctx c, double dist, int plain, const uint32_t* cb,
-                                                            const uint32_t* cp, uint64_t n, uint32_t* ob,
-                                                            uint32_t* op, uint64_t cap, unsigned long long* counter,
-                                                            unsigned long long* stats,
+                                                            const uint32_t* cp, uint64_t n, const uint32_t* list,
+                                                            uint32_t* ob, uint32_t* op, uint64_t cap,
+                                                            unsigned long long* counter, unsigned long long* stats,
                                                             const unsigned long long* n_dev) {
   const GeomView& B = c.B;
   const GeomView& P = c.P;
@@ -1346,6 +1399,7 @@ __global__ __launch_bounds__(kBlock) void k_refine_distance(Ctx c, double dist, 
     bool keep = false;
     uint32_t b = 0, p = 0;
     if (i < n) {
+      if (list) i = list[i];
       b = cb[i];
       p = cp[i];
       double px, py;
@@ -1618,6 +1672,65 @@ __global__ __launch_bounds__(kBlock) void k_raster_build(Ctx c, PolyDesc* desc, 
   if (lane == 0) desc[row].raster_off = roff[row] * kRasterWords;
 }
 
+__global__ void k_line_raster_flags(GeomView g, const PolyDesc* desc, int64_t* flag) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= g.n_rows) return;
+  const PolyDesc d = desc[r];
+  const double w = d.rect.z - d.rect.x, h = d.rect.w - d.rect.y;
+  flag[r] = (g.type[r] == kLine || g.type[r] == kMultiLine) && d.npts >= 2 && w > 0 && h > 0 && w < INFINITY &&
+                    h < INFINITY
+                ? 1
+                : 0;
+}
+
+// Cell index range [lo, hi] covering [v0, v1] (in cells from the grid's
+// origin), widened by one cell and clamped to the grid; NaN gives cell 0.
+__device__ __forceinline__ int line_cell_lo(double v) { return (int)fmin(fmax(floor(v) - 1.0, 0.0), kLineCells - 1.0); }
+__device__ __forceinline__ int line_cell_hi(double v) { return (int)fmin(fmax(floor(v) + 1.0, 0.0), kLineCells - 1.0); }
+
+// Cell raster of one line per wavefront.  A cell's bit is set when it lies
+// in a segment's cell range (its bounding box, widened by one cell) and
+// its centre is within 1.01 half diagonals of the segment's line (of its
+// point, for a zero-length segment): every cell a segment passes through
+// is set, with a margin far above the rounding of these tests.  Segments
+// with a NaN coordinate set no cell; they are never within a distance.
+__global__ __launch_bounds__(kBlock) void k_line_raster_build(GeomView g, PolyDesc* desc, const int64_t* flag,
+                                                             const int64_t* roff, uint64_t* raster) {
+  static_assert(kLineCells == 64, "one row of cells per lane");
+  __shared__ unsigned long long s_rows[kBlock / 64][kLineCells];
+  const int lane = threadIdx.x & 63;
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (row >= g.n_rows || !flag[row]) return;  // wave-uniform
+  unsigned long long* rows = s_rows[threadIdx.x >> 6];
+  rows[lane] = 0;
+  wave_lds_sync();
+  const PolyDesc d = desc[row];
+  const double4 rb = d.rect;
+  const double cw = (rb.z - rb.x) / kLineCells, ch = (rb.w - rb.y) / kLineCells;
+  const double r = 0.505 * sqrt(cw * cw + ch * ch);
+  const double2* xy = g.xy + d.v0;
+  for (int64_t j = lane; j < (int64_t)d.npts - 1; j += 64) {
+    const double2 a = xy[j], b = xy[j + 1];
+    const int i0 = line_cell_lo((fmin(a.x, b.x) - rb.x) / cw), i1 = line_cell_hi((fmax(a.x, b.x) - rb.x) / cw);
+    const int j0 = line_cell_lo((fmin(a.y, b.y) - rb.y) / ch), j1 = line_cell_hi((fmax(a.y, b.y) - rb.y) / ch);
+    const double ex = b.x - a.x, ey = b.y - a.y;
+    const double len = sqrt(ex * ex + ey * ey);
+    for (int cy = j0; cy <= j1; cy++) {
+      const double yc = rb.y + (cy + 0.5) * ch;
+      unsigned long long bits = 0;
+      for (int cx = i0; cx <= i1; cx++) {
+        const double xc = rb.x + (cx + 0.5) * cw;
+        const double dl = len > 0 ? fabs(ex * (yc - a.y) - ey * (xc - a.x)) / len : sqrt((xc - a.x) * (xc - a.x) + (yc - a.y) * (yc - a.y));
+        if (dl <= r) bits |= 1ull << cx;
+      }
+      if (bits) atomicOr(&rows[cy], bits);
+    }
+  }
+  wave_lds_sync();
+  raster[roff[row] * kLineCells + lane] = rows[lane];
+  if (lane == 0) desc[row].raster_off = roff[row] * kLineCells;
+}
+
 }  // namespace
 
 void build_ring_index(const GeomColumn& g, RingIndex& ri, const RingIndexParams& prm, hipStream_t s) {
@@ -1658,30 +1771,46 @@ void build_ring_index(const GeomColumn& g, RingIndex& ri, const RingIndexParams&
   ri.desc.alloc(g.n_rows > 0 ? g.n_rows : 1, s);
   if (g.n_rows > 0) hipLaunchKernelGGL(k_poly_desc, blocks_for(g.n_rows), kBlock, 0, s, g.view(), ri.view(), ri.desc.p);
   SJ_CHECK(hipGetLastError());
-  // cell rasters of the single-ring polygon rows with segment trees
+  // cell rasters of the single-ring polygon rows with segment trees, and
+  // of the single-line rows
   if (g.n_rows > 0 && prm.raster) {
-    DevArray<int64_t> flag(g.n_rows, s), roff(g.n_rows + 1, s);
-    hipLaunchKernelGGL(k_raster_flags, blocks_for(g.n_rows), kBlock, 0, s, g.view(), ri.desc.p, flag.p);
-    size_t tb = 0;
-    SJ_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, flag.p, roff.p, (int)g.n_rows + 1, s));
-    DevArray<uint8_t> tmp(tb + 16, s);
-    // the scan runs over n_rows + 1 entries so that roff[n_rows] is the total
-    DevArray<int64_t> flag1(g.n_rows + 1, s);
-    SJ_CHECK(hipMemcpyAsync(flag1.p, flag.p, g.n_rows * 8, hipMemcpyDeviceToDevice, s));
-    SJ_CHECK(hipMemsetAsync(flag1.p + g.n_rows, 0, 8, s));
-    SJ_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, flag1.p, roff.p, (int)g.n_rows + 1, s));
-    int64_t total = 0;
-    SJ_CHECK(hipMemcpyAsync(&total, roff.p + g.n_rows, 8, hipMemcpyDeviceToHost, s));
-    SJ_CHECK(hipStreamSynchronize(s));
-    ri.n_rasters = total;
-    if (total > 0) {
-      ri.raster.alloc((size_t)total * kRasterWords, s);
-      Ctx c;
-      c.B = g.view();
-      c.R = ri.view();
-      hipLaunchKernelGGL(k_raster_build, blocks_for((uint64_t)g.n_rows * 64), kBlock, 0, s, c, ri.desc.p, flag.p,
-                         roff.p, ri.raster.p);
-      SJ_CHECK(hipGetLastError());
+    // rows flagged by `flags` get consecutive raster slots (roff); returns
+    // their count
+    DevArray<int64_t> flag(g.n_rows + 1, s), roff(g.n_rows + 1, s);
+    auto slots = [&](auto flags) {
+      SJ_CHECK(hipMemsetAsync(flag.p + g.n_rows, 0, 8, s));
+      hipLaunchKernelGGL(flags, blocks_for(g.n_rows), kBlock, 0, s, g.view(), ri.desc.p, flag.p);
+      // the scan runs over n_rows + 1 entries so that roff[n_rows] is the total
+      size_t tb = 0;
+      SJ_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, flag.p, roff.p, (int)g.n_rows + 1, s));
+      DevArray<uint8_t> tmp(tb + 16, s);
+      SJ_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, flag.p, roff.p, (int)g.n_rows + 1, s));
+      int64_t total = 0;
+      SJ_CHECK(hipMemcpyAsync(&total, roff.p + g.n_rows, 8, hipMemcpyDeviceToHost, s));
+      SJ_CHECK(hipStreamSynchronize(s));
+      return total;
+    };
+    const uint32_t polys = (1u << kPolygon) | (1u << kMultiPolygon), lines = (1u << kLine) | (1u << kMultiLine);
+    if (g.type_mask & polys) {
+      ri.n_rasters = slots(k_raster_flags);
+      if (ri.n_rasters > 0) {
+        ri.raster.alloc((size_t)ri.n_rasters * kRasterWords, s);
+        Ctx c;
+        c.B = g.view();
+        c.R = ri.view();
+        hipLaunchKernelGGL(k_raster_build, blocks_for((uint64_t)g.n_rows * 64), kBlock, 0, s, c, ri.desc.p, flag.p,
+                           roff.p, ri.raster.p);
+        SJ_CHECK(hipGetLastError());
+      }
+    }
+    if (g.type_mask & lines) {
+      ri.n_line_rasters = slots(k_line_raster_flags);
+      if (ri.n_line_rasters > 0) {
+        ri.line_raster.alloc((size_t)ri.n_line_rasters * kLineCells, s);
+        hipLaunchKernelGGL(k_line_raster_build, blocks_for((uint64_t)g.n_rows * 64), kBlock, 0, s, g.view(),
+                           ri.desc.p, flag.p, roff.p, ri.line_raster.p);
+        SJ_CHECK(hipGetLastError());
+      }
     }
   }
   SJ_CHECK(hipStreamSynchronize(s));
@@ -1716,73 +1845,95 @@ bool pip_refine_applies(const GeomColumn& build, const RingIndex& ri, const Geom
          (predicate == kIntersects || predicate == kCovers || predicate == kContains);
 }
 
+// Runs a raster pass over n candidates (launch(grid, mask, tile_counts))
+// and lists the candidates it marked, in candidate order, into todo (mask:
+// the pass's marks); *count (device memory, in count_store) receives their
+// number.
+template <class Launch>
+void list_marked(uint64_t n, Launch launch, DevArray<uint32_t>& todo, DevArray<uint64_t>& mask,
+                 DevArray<unsigned long long>& count_store, const unsigned long long** count, hipStream_t s) {
+  const uint64_t n_tiles = (n + kRasterTile - 1) / kRasterTile;
+  fit(mask, (n + 63) / 64, s);
+  DevArray<unsigned long long> cnt(n_tiles + 1, s);
+  count_store.alloc(n_tiles + 1, s);
+  fit(todo, n, s);
+  SJ_CHECK(hipMemsetAsync(cnt.p + n_tiles, 0, 8, s));
+  const unsigned int g = (unsigned int)std::min<uint64_t>(n_tiles, 4096);
+  launch(g, mask.p, cnt.p);
+  size_t tb = 0;
+  SJ_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt.p, count_store.p, (int)(n_tiles + 1), s));
+  DevArray<uint8_t> tmp(tb, s);
+  SJ_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, cnt.p, count_store.p, (int)(n_tiles + 1), s));
+  hipLaunchKernelGGL(k_todo_scatter, g, kBlock, 0, s, mask.p, count_store.p, n, todo.p);
+  SJ_CHECK(hipGetLastError());
+  // the scan's last entry is the number of marked candidates
+  *count = count_store.p + n_tiles;
+}
+
 uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, int predicate,
                          const uint32_t* cb, const uint32_t* cp, uint64_t n, int precision, PairBuf& out,
                          RefineStats* stats, hipStream_t s, KernelTimer* timer, const uint32_t* probe_ids,
-                         int pip_waves) {
+                         int pip_waves, Workspace* ws) {
   out.counter.reserve(1, s);
-  out.b.reserve(n > 0 ? n : 1, s);
-  out.p.reserve(n > 0 ? n : 1, s);
+  fit(out.b, n > 0 ? n : 1, s);
+  fit(out.p, n > 0 ? n : 1, s);
   if (n == 0) return 0;
   if (build.points_only || ri.ring_rect.p == nullptr) precision = 1;
   Ctx c;
   c.B = build.view();
   c.R = ri.view();
   c.P = probe.view();
-  DevArray<unsigned long long> aux(5, s);  // counter of long list, stats[2], bad flag, raster todo count
+  DevArray<unsigned long long> aux(4, s);  // counter of long list, stats[2], bad flag
   DevArray<uint32_t> long_list;
   bool pip = pip_refine_applies(build, ri, probe, predicate);
   if (probe_ids && !pip) throw Error("probe id mapping needs the point-in-polygon refine");
   SJ_CHECK(hipMemsetAsync(out.counter.p, 0, 8, s));
-  SJ_CHECK(hipMemsetAsync(aux.p, 0, 40, s));
+  SJ_CHECK(hipMemsetAsync(aux.p, 0, 32, s));
   if (pip) {
     int allow = predicate == kContains ? 0 : 1;
     if (timer) timer->record_start(s);
     if (precision == 0 && n <= 0xffffffffull) {
       // certified FP32 pass; candidates with an uncertain test (aux[0] of
       // them) are refined again in FP64 by the second launch
-      DevArray<uint32_t> redo(n, s);
+      DevArray<uint32_t> redo_own, todo_own;
+      DevArray<uint64_t> mask_own;
+      DevArray<uint32_t>& redo = ws ? ws->redo : redo_own;
+      DevArray<uint32_t>& todo = ws ? ws->todo : todo_own;
+      DevArray<uint64_t>& mask = ws ? ws->mask : mask_own;
+      fit(redo, n, s);
       // candidates in clean raster cells are decided first; the walk takes
-      // the rest from a list (count in aux[4])
-      DevArray<uint32_t> todo;
-      if (ri.n_rasters > 0) {
-        const uint64_t n_tiles = (n + kRasterTile - 1) / kRasterTile;
-        DevArray<uint64_t> mask((n + 63) / 64, s);
-        DevArray<unsigned long long> cnt(n_tiles + 1, s), off(n_tiles + 1, s);
-        todo.alloc(n, s);
-        SJ_CHECK(hipMemsetAsync(cnt.p + n_tiles, 0, 8, s));
-        const unsigned int g = (unsigned int)std::min<uint64_t>(n_tiles, 4096);
-        hipLaunchKernelGGL(k_raster_pass, g, kBlock, 0, s, c, cb, cp, n, probe_ids, out.b.p, out.p.p,
-                           (uint64_t)out.b.n, out.counter.p, mask.p, cnt.p);
-        size_t tb = 0;
-        SJ_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt.p, off.p, (int)(n_tiles + 1), s));
-        DevArray<uint8_t> tmp(tb, s);
-        SJ_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, cnt.p, off.p, (int)(n_tiles + 1), s));
-        hipLaunchKernelGGL(k_todo_scatter, g, kBlock, 0, s, mask.p, off.p, n, todo.p);
-        // the walk's candidate count: the scan's last entry
-        SJ_CHECK(hipMemcpyAsync(aux.p + 4, off.p + n_tiles, 8, hipMemcpyDeviceToDevice, s));
-      }
+      // the rest from a list
+      DevArray<unsigned long long> todo_count;
+      const unsigned long long* n_todo = nullptr;
+      if (ri.n_rasters > 0)
+        list_marked(
+            n,
+            [&](unsigned int g, uint64_t* marks, unsigned long long* tile_cnt) {
+              hipLaunchKernelGGL(k_raster_pass, g, kBlock, 0, s, c, cb, cp, n, probe_ids, out.b.p, out.p.p,
+                                 out.cap(), out.counter.p, marks, tile_cnt);
+            },
+            todo, mask, todo_count, &n_todo, s);
+      const uint32_t* list = n_todo ? todo.p : nullptr;
       auto fp32_pass = [&](auto kernel) {
-        hipLaunchKernelGGL(kernel, grid_for(n), kBlock, 0, s, c, allow, cb, cp, n, todo.p, out.b.p, out.p.p,
-                           (uint64_t)out.b.n, out.counter.p, aux.p + 1, probe_ids, redo.p, aux.p,
-                           todo.p ? aux.p + 4 : nullptr);
+        hipLaunchKernelGGL(kernel, grid_for(n), kBlock, 0, s, c, allow, cb, cp, n, list, out.b.p, out.p.p,
+                           out.cap(), out.counter.p, aux.p + 1, probe_ids, redo.p, aux.p, n_todo);
       };
       // resident waves per SIMD (an autotuned choice; registers spill beyond 4)
       if (pip_waves == 4) fp32_pass(k_refine_pip<kPrecDeferred, 4>);
       else if (pip_waves == 6) fp32_pass(k_refine_pip<kPrecDeferred, 6>);
       else fp32_pass(k_refine_pip<kPrecDeferred, 8>);
-      // the timer brackets the FP32 pass alone (the kernel the roofline
-      // figures describe)
+      // the timer brackets the raster pass, the listing and the FP32 walk
+      // (the launches the roofline figures describe)
       if (timer) timer->record_stop(s);
       timer = nullptr;
       hipLaunchKernelGGL(k_refine_pip<1>, 1024, kBlock, 0, s, c, allow, cb, cp, n, redo.p, out.b.p, out.p.p,
-                         (uint64_t)out.b.n, out.counter.p, nullptr, probe_ids, nullptr, nullptr, aux.p);
+                         out.cap(), out.counter.p, nullptr, probe_ids, nullptr, nullptr, aux.p);
     } else if (precision == 0) {
       hipLaunchKernelGGL(k_refine_pip<0>, grid_for(n), kBlock, 0, s, c, allow, cb, cp, n, nullptr, out.b.p,
-                         out.p.p, (uint64_t)out.b.n, out.counter.p, aux.p + 1, probe_ids, nullptr, nullptr, nullptr);
+                         out.p.p, out.cap(), out.counter.p, aux.p + 1, probe_ids, nullptr, nullptr, nullptr);
     } else {
       hipLaunchKernelGGL(k_refine_pip<1>, grid_for(n), kBlock, 0, s, c, allow, cb, cp, n, nullptr, out.b.p,
-                         out.p.p, (uint64_t)out.b.n, out.counter.p, aux.p + 1, probe_ids, nullptr, nullptr, nullptr);
+                         out.p.p, out.cap(), out.counter.p, aux.p + 1, probe_ids, nullptr, nullptr, nullptr);
     }
     SJ_CHECK(hipGetLastError());
     if (timer) timer->record_stop(s);
@@ -1810,7 +1961,7 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
     if (timer) timer->record_start(s);
     const uint64_t waves = std::min<uint64_t>(n, 1u << 18);
     hipLaunchKernelGGL(k_refine_polypoly, (unsigned int)((waves * 64 + kBlock - 1) / kBlock), kBlock, 0, s, c, cb, cp,
-                       n, out.b.p, out.p.p, (uint64_t)out.b.n, out.counter.p);
+                       n, out.b.p, out.p.p, out.cap(), out.counter.p);
     SJ_CHECK(hipGetLastError());
     if (timer) timer->record_stop(s);
     unsigned long long cnt = 0;
@@ -1823,13 +1974,13 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
   if (use_long) long_list.alloc(n, s);
   if (timer) timer->record_start(s);
   hipLaunchKernelGGL(k_refine_lane, grid_for(n), kBlock, 0, s, c, predicate, precision, cb, cp, n, out.b.p, out.p.p,
-                     (uint64_t)out.b.n, out.counter.p, use_long ? long_list.p : nullptr, aux.p, aux.p + 1,
+                     out.cap(), out.counter.p, use_long ? long_list.p : nullptr, aux.p, aux.p + 1,
                      (int*)(aux.p + 3));
   SJ_CHECK(hipGetLastError());
   if (use_long) {
     // enough waves to fill the chip several times over
     hipLaunchKernelGGL(k_refine_wave, 256 * 16, kBlock, 0, s, c, predicate, precision, cb, cp, long_list.p, aux.p,
-                       out.b.p, out.p.p, (uint64_t)out.b.n, out.counter.p, aux.p + 1);
+                       out.b.p, out.p.p, out.cap(), out.counter.p, aux.p + 1);
     SJ_CHECK(hipGetLastError());
   }
   if (timer) timer->record_stop(s);
@@ -1847,10 +1998,10 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
 
 uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
                          const uint32_t* cb, const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s,
-                         KernelTimer* timer, int precision, RefineStats* stats) {
+                         KernelTimer* timer, int precision, RefineStats* stats, Workspace* ws) {
   out.counter.reserve(1, s);
-  out.b.reserve(n > 0 ? n : 1, s);
-  out.p.reserve(n > 0 ? n : 1, s);
+  fit(out.b, n > 0 ? n : 1, s);
+  fit(out.p, n > 0 ? n : 1, s);
   if (n == 0) return 0;
   if (!probe.points_only && (probe.type_mask & ~(1u << kPoint)))
     throw Error("distance refine: probe rows must be points");
@@ -1863,18 +2014,36 @@ uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const Geo
   DevArray<unsigned long long> aux(2, s);
   SJ_CHECK(hipMemsetAsync(aux.p, 0, 16, s));
   const int plain = precision == 1 ? 1 : 0;
-  switch (plain ? kDistAny : dist_kind(build.type_mask, build.points_only, ri)) {
+  const int kind = plain ? kDistAny : dist_kind(build.type_mask, build.points_only, ri);
+  // lines: candidates whose point's square [p - d, p + d] meets no raster
+  // cell of the line are dropped first; the walk takes the rest from a list
+  DevArray<uint32_t> todo_own;
+  DevArray<uint64_t> mask_own;
+  DevArray<uint32_t>& todo = ws ? ws->todo : todo_own;
+  DevArray<uint64_t>& mask = ws ? ws->mask : mask_own;
+  DevArray<unsigned long long> todo_count;
+  const unsigned long long* n_todo = nullptr;
+  if (kind == kDistLines && ri.n_line_rasters > 0 && probe.points_only && d >= 0 && d < INFINITY &&
+      n <= 0xffffffffull)
+    list_marked(
+        n,
+        [&](unsigned int g, uint64_t* marks, unsigned long long* tile_cnt) {
+          hipLaunchKernelGGL(k_dist_raster_pass, g, kBlock, 0, s, c, d, cb, cp, n, marks, tile_cnt);
+        },
+        todo, mask, todo_count, &n_todo, s);
+  const uint32_t* list = n_todo ? todo.p : nullptr;
+  switch (kind) {
     case kDistLines:
-      hipLaunchKernelGGL(k_refine_distance<kDistLines>, grid_for(n), kBlock, 0, s, c, d, 0, cb, cp, n, out.b.p,
-                         out.p.p, (uint64_t)out.b.n, out.counter.p, aux.p, nullptr);
+      hipLaunchKernelGGL(k_refine_distance<kDistLines>, grid_for(n), kBlock, 0, s, c, d, 0, cb, cp, n, list,
+                         out.b.p, out.p.p, out.cap(), out.counter.p, aux.p, n_todo);
       break;
     case kDistPolygons:
-      hipLaunchKernelGGL(k_refine_distance<kDistPolygons>, grid_for(n), kBlock, 0, s, c, d, 0, cb, cp, n, out.b.p,
-                         out.p.p, (uint64_t)out.b.n, out.counter.p, aux.p, nullptr);
+      hipLaunchKernelGGL(k_refine_distance<kDistPolygons>, grid_for(n), kBlock, 0, s, c, d, 0, cb, cp, n, list,
+                         out.b.p, out.p.p, out.cap(), out.counter.p, aux.p, n_todo);
       break;
     default:
-      hipLaunchKernelGGL(k_refine_distance<kDistAny>, grid_for(n), kBlock, 0, s, c, d, plain, cb, cp, n, out.b.p,
-                         out.p.p, (uint64_t)out.b.n, out.counter.p, aux.p, nullptr);
+      hipLaunchKernelGGL(k_refine_distance<kDistAny>, grid_for(n), kBlock, 0, s, c, d, plain, cb, cp, n,
+                         nullptr, out.b.p, out.p.p, out.cap(), out.counter.p, aux.p, nullptr);
   }
   SJ_CHECK(hipGetLastError());
   if (timer) timer->record_stop(s);
@@ -1924,16 +2093,16 @@ void launch_distance_refine_dev(const GeomColumn& build, const RingIndex& ri, co
   c.P = probe.view();
   switch (dist_kind(build.type_mask, build.points_only, ri)) {
     case kDistLines:
-      hipLaunchKernelGGL(k_refine_distance<kDistLines>, grid_for(n_cap), kBlock, 0, s, c, d, 0, cb, cp, n_cap, ob, op,
-                         out_cap, out_n, stats, n_dev);
-      break;
-    case kDistPolygons:
-      hipLaunchKernelGGL(k_refine_distance<kDistPolygons>, grid_for(n_cap), kBlock, 0, s, c, d, 0, cb, cp, n_cap, ob,
+      hipLaunchKernelGGL(k_refine_distance<kDistLines>, grid_for(n_cap), kBlock, 0, s, c, d, 0, cb, cp, n_cap, nullptr, ob,
                          op, out_cap, out_n, stats, n_dev);
       break;
+    case kDistPolygons:
+      hipLaunchKernelGGL(k_refine_distance<kDistPolygons>, grid_for(n_cap), kBlock, 0, s, c, d, 0, cb, cp, n_cap, nullptr,
+                         ob, op, out_cap, out_n, stats, n_dev);
+      break;
     default:
-      hipLaunchKernelGGL(k_refine_distance<kDistAny>, grid_for(n_cap), kBlock, 0, s, c, d, 0, cb, cp, n_cap, ob, op,
-                         out_cap, out_n, stats, n_dev);
+      hipLaunchKernelGGL(k_refine_distance<kDistAny>, grid_for(n_cap), kBlock, 0, s, c, d, 0, cb, cp, n_cap, nullptr, ob,
+                         op, out_cap, out_n, stats, n_dev);
   }
   SJ_CHECK(hipGetLastError());
 }
@@ -1965,13 +2134,13 @@ uint64_t filter_refine_distance(const RectTree& t, const float4* q, int64_t nq, 
     kw.stats = aux.p;
     if (t.leaf == 4)
       hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 4, decltype(kw)>), blocks_for(nq), kFilterBlock, 0, s, v, q, nq,
-                         qorder, 0, kw, out.b.p, out.p.p, (uint64_t)out.b.n, out.counter.p);
+                         qorder, 0, kw, out.b.p, out.p.p, out.cap(), out.counter.p);
     else if (t.leaf == 16)
       hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 16, decltype(kw)>), blocks_for(nq), kFilterBlock, 0, s, v, q, nq,
-                         qorder, 0, kw, out.b.p, out.p.p, (uint64_t)out.b.n, out.counter.p);
+                         qorder, 0, kw, out.b.p, out.p.p, out.cap(), out.counter.p);
     else
       hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 8, decltype(kw)>), blocks_for(nq), kFilterBlock, 0, s, v, q, nq,
-                         qorder, 0, kw, out.b.p, out.p.p, (uint64_t)out.b.n, out.counter.p);
+                         qorder, 0, kw, out.b.p, out.p.p, out.cap(), out.counter.p);
   };
   for (int attempt = 0; attempt < 2; attempt++) {
     SJ_CHECK(hipMemsetAsync(out.counter.p, 0, 8, s));
@@ -1987,7 +2156,7 @@ uint64_t filter_refine_distance(const RectTree& t, const float4* q, int64_t nq, 
     SJ_CHECK(hipMemcpyAsync(h, aux.p, 24, hipMemcpyDeviceToHost, s));
     SJ_CHECK(hipStreamSynchronize(s));
     t.kept_per_query_x16.store((uint64_t)((total * 16) / (uint64_t)nq + 1), std::memory_order_relaxed);
-    if (total <= out.b.n) {
+    if (total <= out.cap()) {
       if (candidates) *candidates = h[0];
       if (stats) {
         stats->segment_tests += h[1];

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <atomic>
 #include <mutex>
 #include <stdexcept>
@@ -61,6 +62,20 @@ inline hipStream_t detached_stream() {
   return streams[dev];
 }
 
+// Bytes requested from the stream-ordered pool for `bytes`: 256-byte
+// granules, and above 1 MiB the next of eight size classes per power of
+// two (at most 12.5 % more).  Buffers whose sizes vary a little from one
+// join to the next (candidates, pairs) then fall into blocks the pool
+// already holds; a block of a new size makes the driver map (and clear)
+// fresh memory while the host waits, about 1 s per 10 GB.
+inline size_t pool_bytes(size_t bytes) {
+  bytes = (bytes + 255) & ~size_t(255);
+  if (bytes <= (size_t(1) << 20)) return bytes;
+  size_t top = size_t(1) << (63 - __builtin_clzll((unsigned long long)bytes));
+  const size_t step = top >> 3;
+  return (bytes + step - 1) / step * step;
+}
+
 template <typename T>
 struct DevArray {
   T* p = nullptr;
@@ -80,7 +95,7 @@ struct DevArray {
     release();
     s = stream;
     n = count;
-    if (count) SJ_CHECK(hipMallocAsync((void**)&p, (count * sizeof(T) + 255) & ~size_t(255), stream));
+    if (count) SJ_CHECK(hipMallocAsync((void**)&p, pool_bytes(count * sizeof(T)), stream));
   }
   // grows (without preserving contents) when smaller than count
   void reserve(size_t count, hipStream_t stream) {
@@ -119,8 +134,9 @@ constexpr int kSegMaxLevels = 5;
 
 // Record of a build row that is one polygon with one ring, or one line
 // (npts >= 0): its ring's rect, first vertex, vertex count, FP32 bound,
-// segment-tree offset (-1: none) and cell-raster offset (-1: none).
-// npts = -1 marks every other row.
+// segment-tree offset (-1: none) and cell-raster offset (-1: none; a
+// polygon's offset indexes RingIndex::raster, a line's
+// RingIndex::line_raster).  npts = -1 marks every other row.
 struct alignas(32) PolyDesc {
   double4 rect;
   int64_t v0;
@@ -136,6 +152,11 @@ struct alignas(32) PolyDesc {
 constexpr int kRasterCells = 32;
 constexpr int kRasterWords = kRasterCells * kRasterCells / 16;
 
+// Cell raster of a line: kLineCells x kLineCells cells over the line's
+// rect, one bit per cell (set: some segment may pass through the cell), one
+// 64-bit word per row of cells.
+constexpr int kLineCells = 64;
+
 // Device view of the ring refine index (build side).
 struct RingIndexView {
   const double4* ring_rect = nullptr;   // xmin, ymin, xmax, ymax (double)
@@ -146,6 +167,7 @@ struct RingIndexView {
   const float4* seg_tree = nullptr;       // boxes (xmin, ymin, xmax, ymax), ring-local FP32
   const PolyDesc* desc = nullptr;         // per build row
   const uint32_t* raster = nullptr;       // cell rasters (see PolyDesc::raster_off)
+  const uint64_t* line_raster = nullptr;  // line cell rasters (see PolyDesc::raster_off)
 };
 
 // Owned geometry column on the device.
@@ -178,8 +200,10 @@ struct RingIndex {
   DevArray<float4> seg_tree;
   DevArray<PolyDesc> desc;
   DevArray<uint32_t> raster;
+  DevArray<uint64_t> line_raster;
   int64_t n_boxes = 0;
   int64_t n_rasters = 0;
+  int64_t n_line_rasters = 0;
   RingIndexView view() const {
     RingIndexView v;
     v.ring_rect = ring_rect.p;
@@ -190,6 +214,7 @@ struct RingIndex {
     v.seg_tree = seg_tree.p;
     v.desc = desc.p;
     v.raster = raster.p;
+    v.line_raster = line_raster.p;
     return v;
   }
 };
@@ -284,6 +309,34 @@ struct KernelTimer {
 struct PairBuf {
   DevArray<uint32_t> b, p;
   DevArray<unsigned long long> counter;
+  // pairs both arrays can hold (their sizes can differ once one of them has
+  // been replaced, e.g. by a sort's output array)
+  uint64_t cap() const { return (uint64_t)std::min(b.n, p.n); }
+};
+
+// Sizes a for at least count elements: kept when large enough, else
+// reallocated with a quarter of headroom (contents not preserved).
+template <class T>
+void fit(DevArray<T>& a, size_t count, hipStream_t s) {
+  if (a.p && a.n >= count) return;
+  a.alloc(count + count / 4 + 256, s);
+}
+
+// Device scratch kept between joins: the filter's candidate pairs, the
+// refine's lists and marks, and its output before the exact-size copy
+// into the join's result.  Reused (and grown with headroom), so that joins
+// of similar size take no fresh blocks from the stream-ordered pool: a
+// block of a new size makes the driver map and clear memory while the host
+// waits.  A workspace serves one join at a time; set_stream names the
+// stream its buffers are freed on (the join's, then the detached stream).
+struct Workspace {
+  PairBuf cand, out;
+  DevArray<uint32_t> todo, redo;
+  DevArray<uint64_t> mask;
+  void set_stream(hipStream_t s) {
+    cand.b.s = cand.p.s = cand.counter.s = out.b.s = out.p.s = out.counter.s = s;
+    todo.s = redo.s = mask.s = s;
+  }
 };
 
 // Runs the filter of queries against the tree into pb; returns the count.
@@ -319,14 +372,15 @@ bool pip_refine_applies(const GeomColumn& build, const RingIndex& ri, const Geom
 uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, int predicate,
                          const uint32_t* cb, const uint32_t* cp, uint64_t n, int precision, PairBuf& out,
                          RefineStats* stats, hipStream_t s, KernelTimer* timer = nullptr,
-                         const uint32_t* probe_ids = nullptr, int pip_waves = 8);
+                         const uint32_t* probe_ids = nullptr, int pip_waves = 8, Workspace* ws = nullptr);
 
 // Distance refine (build.distance(probe) <= d); probe must be points.
 // precision 1: the all-FP64 form over every segment.  stats: FP64 segment
 // distance tests, and exact ring scans (uncertain certified crossing counts).
 uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
                          const uint32_t* cb, const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s,
-                         KernelTimer* timer = nullptr, int precision = 0, RefineStats* stats = nullptr);
+                         KernelTimer* timer = nullptr, int precision = 0, RefineStats* stats = nullptr,
+                         Workspace* ws = nullptr);
 
 // ST_DWithin join of point probes in one kernel: the filter walk decides
 // the distance predicate for each rect hit, so only kept pairs are

@@ -36,10 +36,13 @@ PREDICATE_BY_NAME = {p.name.lower(): p for p in GpuSpatialRelationPredicate}
 
 
 class GpuSpatialOptions:
-    """options.rs:19-32"""
+    """c/sedona-libgpuspatial/src/options.rs:19-32.  The pool percentage
+    defaults to the operator's 50 (rust/sedona-spatial-join-gpu/src/options.rs:39):
+    the library keeps that share of HBM cached in its stream-ordered pool
+    between joins instead of returning it to the driver."""
 
     def __init__(self, concurrency=1, device_id=0, compress_bvh=False, pipeline_batches=1,
-                 cuda_use_memory_pool=True, cuda_memory_pool_init_percent=10):
+                 cuda_use_memory_pool=True, cuda_memory_pool_init_percent=50):
         self.concurrency = concurrency
         self.device_id = device_id
         self.compress_bvh = compress_bvh

@@ -231,17 +231,67 @@ def test_cell_raster_boundaries_vs_oracle(pred, sort_probes, opts):
     assert np.array_equal(gb, ob) and np.array_equal(gp, op), (len(gb), len(ob))
 
 
-@pytest.mark.parametrize("refine_order", [1, 2])
-def test_dwithin_random_walks_vs_oracle(refine_order, opts):
-    # ~160 candidate boxes per point at this line density (as at full size)
+@pytest.mark.parametrize("refine_order,d", [(0, 0.05), (0, 0.0), (0, 3.0), (1, 0.05), (2, 0.05)])
+def test_dwithin_random_walks_vs_oracle(refine_order, d, opts):
+    # ~160 candidate boxes per point at this line density (as at full size);
+    # d = 3 spans more raster rows than the line raster pass reads (walked)
     b, p = D.workload_dwithin_walks(10_000, 2_000, seed=13)
-    say("dwithin walks", len(b), len(p), refine_order)
+    say("dwithin walks", len(b), len(p), refine_order, d)
     ix, rf = build_pair(b.to_arrow(), opts)
-    r = G.join_device(ix, rf, p.to_arrow(), "dwithin", distance=0.05, refine_order=refine_order)
+    r = G.join_device(ix, rf, p.to_arrow(), "dwithin", distance=d, refine_order=refine_order)
     gb, gp = r.to_host()
-    ob, op = O.join(O.Store(b.as_tuple()), O.Store(p.as_tuple()), "dwithin", distance=0.05, nthreads=8)
+    ob, op = O.join(O.Store(b.as_tuple()), O.Store(p.as_tuple()), "dwithin", distance=d, nthreads=8)
     assert np.array_equal(gb, ob) and np.array_equal(gp, op), (len(gb), len(ob))
-    assert len(gb) > 1000
+    if d > 0:
+        assert len(gb) > 1000
+
+
+def _line_raster_probe_build():
+    """Lines with cell rasters and edge cases of the raster: a staircase on
+    the cell lines of its rect ([0, 64]^2, cell width 1), a line with
+    repeated vertices (zero-length segments), a two-vertex diagonal, a
+    3000-vertex random walk, and a horizontal line (zero height: no raster).
+    Probes: cell corners and centres (one cell beyond the rect), vertices,
+    edge midpoints, points at distance 0.01 from the vertices, random points."""
+    rng = np.random.default_rng(41)
+    lines = []
+    st = [(0.0, 0.0)]
+    for j in range(32):
+        st += [(2.0 * j + 2.0, 2.0 * j), (2.0 * j + 2.0, 2.0 * j + 2.0)]
+    lines.append(np.array(st) + [100.0, 0.0])
+    rep = np.cumsum(rng.normal(0, 1.0, size=(40, 2)), axis=0) + [200.0, 50.0]
+    lines.append(np.repeat(rep, 2, axis=0))
+    lines.append(np.array([[300.0, 10.0], [330.0, 47.0]]))
+    lines.append(np.cumsum(rng.normal(0, 0.5, size=(3000, 2)), axis=0) + [400.0, 50.0])
+    lines.append(np.array([[500.0, 20.0], [510.0, 20.0], [520.0, 20.0]]))
+    counts = [len(l) for l in lines]
+    b = D.rings_wkb(counts, np.vstack(lines), 2)
+    pts = []
+    for ln in lines:
+        x0, y0 = ln.min(axis=0)
+        x1, y1 = ln.max(axis=0)
+        cw, ch = max(x1 - x0, 1.0) / 64, max(y1 - y0, 1.0) / 64
+        for t in (np.arange(-1, 66, dtype=np.float64), np.arange(-1, 65, dtype=np.float64) + 0.5):
+            gx, gy = np.meshgrid(x0 + t * cw, y0 + t * ch)
+            pts.append(np.stack([gx.ravel(), gy.ravel()], axis=1))
+        pts.append(ln)
+        pts.append(0.5 * (ln[:-1] + ln[1:]))
+        pts.append(ln + [0.01, 0.0])
+        pts.append(np.stack([x0 + (x1 - x0 + 2) * rng.random(3000) - 1, y0 + (y1 - y0 + 2) * rng.random(3000) - 1], axis=1))
+    xy = np.vstack(pts)
+    return b, D.points_wkb(xy[:, 0].copy(), xy[:, 1].copy())
+
+
+@pytest.mark.parametrize("d", [0.0, 0.01, 0.3])
+def test_line_raster_edges_vs_oracle(d, opts):
+    b, p = _line_raster_probe_build()
+    say("line raster edges", d, "probes", len(p.offsets) - 1)
+    ix, rf = build_pair(b.to_arrow(), opts)
+    r = G.join_device(ix, rf, p.to_arrow(), "dwithin", distance=d, refine_order=0)
+    gb, gp = r.to_host()
+    ob, op = O.join(O.Store(b.as_tuple()), O.Store(p.as_tuple()), "dwithin", distance=d, nthreads=8)
+    assert len(ob) > 1000
+    assert np.array_equal(gb, ob) and np.array_equal(gp, op), (len(gb), len(ob))
 
 
 def test_corpus_dwithin_all_pairs_vs_oracle(opts):
@@ -367,6 +417,26 @@ def test_knn_device_outputs_and_resident_tree(opts):
         assert ms > 0
     hi, hd = rf.knn(qcol.to_arrow(), 8)
     assert np.array_equal(hi, oid)
+
+
+def test_workspace_reuse_across_joins_of_growing_size(opts):
+    """Joins on one refiner reuse its device workspace.  A polygon-major
+    join (whose regrouping sort leaves the candidate arrays with different
+    sizes) is followed by joins with more candidates and other predicates;
+    each gives the oracle's pairs."""
+    b, p = D.workload_pip_zipf(120_000, 1_000, seed=17)
+    ix, rf = build_pair(b.to_arrow(), opts)
+    store_b = O.Store(b.as_tuple())
+    plan = [(2_000, "contains", 0.0, 1), (40_000, "intersects", 0.0, 0), (10_000, "contains", 0.0, 1),
+            (120_000, "dwithin", 0.2, 0), (60_000, "contains", 0.0, 1), (120_000, "intersects", 0.0, 1)]
+    for n, pred, d, order in plan:
+        sub = D.take_rows(p, np.arange(n))
+        r = G.join_device(ix, rf, sub.to_arrow(), pred, distance=d, refine_order=order, sort_output=True)
+        gb, gp = r.to_host()
+        r.release()
+        ob, op = O.join(store_b, O.Store(sub.as_tuple()), pred, distance=d, nthreads=8)
+        say("workspace reuse", n, pred, order, len(gb))
+        assert np.array_equal(gb, ob) and np.array_equal(gp, op), (n, pred, len(gb), len(ob))
 
 
 def test_decode_errors_name_the_row(opts):
