@@ -1045,13 +1045,23 @@ __global__ __launch_bounds__(kBlock) void k_dist_raster_pass(Ctx c, double dist,
             const double fy0 = floor((pt.y - dist - d.rect.y) / ch) - 1.0, fy1 = floor((pt.y + dist - d.rect.y) / ch) + 1.0;
             if (fx1 < 0.0 || fy1 < 0.0 || fx0 > kLineCells - 1.0 || fy0 > kLineCells - 1.0) {
               walk = false;
-            } else if (fy1 - fy0 < 16.0) {
+            } else if (fy1 - fy0 < 24.0) {
               const int i0 = (int)fmax(fx0, 0.0), i1 = (int)fmin(fx1, kLineCells - 1.0);
               const int j0 = (int)fmax(fy0, 0.0), j1 = (int)fmin(fy1, kLineCells - 1.0);
               if (i0 <= i1 && j0 <= j1) {
-                const uint64_t m = (~0ull >> (63 - (i1 - i0))) << i0;
+                // bits i0..i1 of a row, split over its words
+                uint64_t m[kLineRowWords];
+#pragma unroll
+                for (int w = 0; w < kLineRowWords; w++) {
+                  const int lo = max(i0 - 64 * w, 0), hi = min(i1 - 64 * w, 63);
+                  m[w] = lo <= hi ? (~0ull >> (63 - (hi - lo))) << lo : 0ull;
+                }
                 walk = false;
-                for (int r = j0; r <= j1 && !walk; r++) walk = (c.R.line_raster[d.raster_off + r] & m) != 0;
+                for (int r = j0; r <= j1 && !walk; r++) {
+                  const uint64_t* rw = c.R.line_raster + d.raster_off + (int64_t)r * kLineRowWords;
+#pragma unroll
+                  for (int w = 0; w < kLineRowWords; w++) walk |= (rw[w] & m[w]) != 0;
+                }
               }
             }
           }
@@ -1696,13 +1706,12 @@ __device__ __forceinline__ int line_cell_hi(double v) { return (int)fmin(fmax(fl
 // with a NaN coordinate set no cell; they are never within a distance.
 __global__ __launch_bounds__(kBlock) void k_line_raster_build(GeomView g, PolyDesc* desc, const int64_t* flag,
                                                              const int64_t* roff, uint64_t* raster) {
-  static_assert(kLineCells == 64, "one row of cells per lane");
-  __shared__ unsigned long long s_rows[kBlock / 64][kLineCells];
+  __shared__ unsigned long long s_words[kBlock / 64][kLineWords];
   const int lane = threadIdx.x & 63;
   const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (row >= g.n_rows || !flag[row]) return;  // wave-uniform
-  unsigned long long* rows = s_rows[threadIdx.x >> 6];
-  rows[lane] = 0;
+  unsigned long long* words = s_words[threadIdx.x >> 6];
+  for (int k = lane; k < kLineWords; k += 64) words[k] = 0;
   wave_lds_sync();
   const PolyDesc d = desc[row];
   const double4 rb = d.rect;
@@ -1717,18 +1726,21 @@ __global__ __launch_bounds__(kBlock) void k_line_raster_build(GeomView g, PolyDe
     const double len = sqrt(ex * ex + ey * ey);
     for (int cy = j0; cy <= j1; cy++) {
       const double yc = rb.y + (cy + 0.5) * ch;
-      unsigned long long bits = 0;
+      unsigned long long bits[kLineRowWords] = {};
       for (int cx = i0; cx <= i1; cx++) {
         const double xc = rb.x + (cx + 0.5) * cw;
-        const double dl = len > 0 ? fabs(ex * (yc - a.y) - ey * (xc - a.x)) / len : sqrt((xc - a.x) * (xc - a.x) + (yc - a.y) * (yc - a.y));
-        if (dl <= r) bits |= 1ull << cx;
+        const double dl = len > 0 ? fabs(ex * (yc - a.y) - ey * (xc - a.x)) / len
+                                  : sqrt((xc - a.x) * (xc - a.x) + (yc - a.y) * (yc - a.y));
+        if (dl <= r) bits[cx >> 6] |= 1ull << (cx & 63);
       }
-      if (bits) atomicOr(&rows[cy], bits);
+#pragma unroll
+      for (int w = 0; w < kLineRowWords; w++)
+        if (bits[w]) atomicOr(&words[cy * kLineRowWords + w], bits[w]);
     }
   }
   wave_lds_sync();
-  raster[roff[row] * kLineCells + lane] = rows[lane];
-  if (lane == 0) desc[row].raster_off = roff[row] * kLineCells;
+  for (int k = lane; k < kLineWords; k += 64) raster[roff[row] * kLineWords + k] = words[k];
+  if (lane == 0) desc[row].raster_off = roff[row] * kLineWords;
 }
 
 }  // namespace
@@ -1806,7 +1818,7 @@ void build_ring_index(const GeomColumn& g, RingIndex& ri, const RingIndexParams&
     if (g.type_mask & lines) {
       ri.n_line_rasters = slots(k_line_raster_flags);
       if (ri.n_line_rasters > 0) {
-        ri.line_raster.alloc((size_t)ri.n_line_rasters * kLineCells, s);
+        ri.line_raster.alloc((size_t)ri.n_line_rasters * kLineWords, s);
         hipLaunchKernelGGL(k_line_raster_build, blocks_for((uint64_t)g.n_rows * 64), kBlock, 0, s, g.view(),
                            ri.desc.p, flag.p, roff.p, ri.line_raster.p);
         SJ_CHECK(hipGetLastError());

@@ -153,9 +153,12 @@ constexpr int kRasterCells = 32;
 constexpr int kRasterWords = kRasterCells * kRasterCells / 16;
 
 // Cell raster of a line: kLineCells x kLineCells cells over the line's
-// rect, one bit per cell (set: some segment may pass through the cell), one
-// 64-bit word per row of cells.
-constexpr int kLineCells = 64;
+// rect, one bit per cell (set: some segment may pass through the cell),
+// kLineRowWords 64-bit words per row of cells (cell x of row y: bit x % 64
+// of word y * kLineRowWords + x / 64).
+constexpr int kLineCells = 128;
+constexpr int kLineRowWords = kLineCells / 64;
+constexpr int kLineWords = kLineCells * kLineRowWords;
 
 // Device view of the ring refine index (build side).
 struct RingIndexView {
