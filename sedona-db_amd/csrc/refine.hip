@@ -921,10 +921,10 @@ __global__ __launch_bounds__(kBlock) void k_refine_polypoly(Ctx c, const uint32_
 // falls in a clean cell of a single-ring polygon are decided here (kept or
 // dropped); the others are marked for the walk in per-wavefront bit masks
 // (mask[i / 64], bit i % 64) and counted per tile of kRasterTile
-// consecutive candidates (tile_cnt).  k_todo_scatter then lists them in
+// consecutive candidates (tile_cnt).  k_open_scatter then copies them in
 // candidate order, so that the walk keeps the polygon-major order of its
-// input and its wavefronts are not held back by candidates that need no
-// walk.
+// input, reads its candidates with coalesced loads, and its wavefronts are
+// not held back by candidates that need no walk.
 constexpr int kRasterTile = 4096;
 
 __global__ __launch_bounds__(kBlock) void k_raster_pass(Ctx c, const uint32_t* cb, const uint32_t* cp, uint64_t n,
@@ -979,10 +979,12 @@ __global__ __launch_bounds__(kBlock) void k_raster_pass(Ctx c, const uint32_t* c
   wave_flush(wo, ob, op, cap, counter);
 }
 
-// Lists the candidates marked by k_raster_pass: tile t's entries start at
-// tile_off[t] (exclusive scan of the tile counts) and keep candidate order.
-__global__ __launch_bounds__(kBlock) void k_todo_scatter(const uint64_t* mask, const unsigned long long* tile_off,
-                                                         uint64_t n, uint32_t* todo) {
+// Copies the candidates marked by a raster pass to (ob, op): tile t's
+// entries start at tile_off[t] (exclusive scan of the tile counts) and keep
+// candidate order.
+__global__ __launch_bounds__(kBlock) void k_open_scatter(const uint64_t* mask, const unsigned long long* tile_off,
+                                                         uint64_t n, const uint32_t* cb, const uint32_t* cp,
+                                                         uint32_t* ob, uint32_t* op) {
   constexpr int kWords = kRasterTile / 64;
   __shared__ unsigned int word_off[kWords];
   const uint64_t n_tiles = (n + kRasterTile - 1) / kRasterTile;
@@ -1009,18 +1011,53 @@ __global__ __launch_bounds__(kBlock) void k_todo_scatter(const uint64_t* mask, c
       const uint64_t w = t * kWords + j;
       if (w >= n_words) break;
       const uint64_t m = mask[w];
-      if ((m >> lane) & 1)
-        todo[base + word_off[j] + __popcll(m & ((1ull << lane) - 1))] = (uint32_t)(w * 64 + lane);
+      if ((m >> lane) & 1) {
+        const uint64_t i = w * 64 + lane, o = base + word_off[j] + __popcll(m & ((1ull << lane) - 1));
+        ob[o] = cb[i];
+        op[o] = cp[i];
+      }
     }
   }
 }
 
+// Line-raster test of one candidate (single-line build row b, point pt,
+// distance dist >= 0 and finite): false when no segment of the row can be
+// within dist of the point -- no raster cell that the square
+// [pt - dist, pt + dist] meets (widened by one cell) may hold a segment,
+// or the square misses the line's rect, or the point has no coordinates.
+// Rows without a raster answer true.
+__device__ __forceinline__ bool line_may_be_within(const Ctx& c, uint32_t b, double2 pt, double dist) {
+  if (!(pt.x == pt.x || pt.y == pt.y)) return false;
+  const PolyDesc d = c.R.desc[b];
+  if (d.raster_off < 0) return true;
+  const double cw = (d.rect.z - d.rect.x) / kLineCells, ch = (d.rect.w - d.rect.y) / kLineCells;
+  const double fx0 = floor((pt.x - dist - d.rect.x) / cw) - 1.0, fx1 = floor((pt.x + dist - d.rect.x) / cw) + 1.0;
+  const double fy0 = floor((pt.y - dist - d.rect.y) / ch) - 1.0, fy1 = floor((pt.y + dist - d.rect.y) / ch) + 1.0;
+  if (fx1 < 0.0 || fy1 < 0.0 || fx0 > kLineCells - 1.0 || fy0 > kLineCells - 1.0) return false;
+  if (!(fy1 - fy0 < 24.0)) return true;  // a window of many rows: left to the walk
+  const int i0 = (int)fmax(fx0, 0.0), i1 = (int)fmin(fx1, kLineCells - 1.0);
+  const int j0 = (int)fmax(fy0, 0.0), j1 = (int)fmin(fy1, kLineCells - 1.0);
+  if (i0 > i1 || j0 > j1) return true;
+  // bits i0..i1 of a row, split over its words
+  uint64_t m[kLineRowWords];
+#pragma unroll
+  for (int w = 0; w < kLineRowWords; w++) {
+    const int lo = max(i0 - 64 * w, 0), hi = min(i1 - 64 * w, 63);
+    m[w] = lo <= hi ? (~0ull >> (63 - (hi - lo))) << lo : 0ull;
+  }
+  for (int r = j0; r <= j1; r++) {
+    const uint64_t* rw = c.R.line_raster + d.raster_off + (int64_t)r * kLineRowWords;
+    bool any = false;
+#pragma unroll
+    for (int w = 0; w < kLineRowWords; w++) any |= (rw[w] & m[w]) != 0;
+    if (any) return true;
+  }
+  return false;
+}
+
 // Cell-raster pass of the distance refine over single-line build rows and
-// point probes: a candidate is dropped when no raster cell that the square
-// [p - d, p + d] meets (widened by one cell) may hold a segment, since then
-// every segment is farther than d on one axis; a point outside the line's
-// rect by more than d, or with no coordinates, is dropped as well.  The
-// others are marked for the walk as in k_raster_pass.
+// point probes: candidates that line_may_be_within rules out are dropped,
+// the others marked for the walk as in k_raster_pass.
 __global__ __launch_bounds__(kBlock) void k_dist_raster_pass(Ctx c, double dist, const uint32_t* cb,
                                                              const uint32_t* cp, uint64_t n, uint64_t* mask,
                                                              unsigned long long* tile_cnt) {
@@ -1034,39 +1071,7 @@ __global__ __launch_bounds__(kBlock) void k_dist_raster_pass(Ctx c, double dist,
     for (int it = 0; it < kRasterTile / kBlock; it++) {
       const uint64_t i = t * kRasterTile + (uint64_t)it * kBlock + threadIdx.x;
       bool walk = false;
-      if (i < n) {
-        const double2 pt = c.P.xy[cp[i]];
-        if (pt.x == pt.x || pt.y == pt.y) {
-          const PolyDesc d = c.R.desc[cb[i]];
-          walk = true;
-          if (d.raster_off >= 0) {
-            const double cw = (d.rect.z - d.rect.x) / kLineCells, ch = (d.rect.w - d.rect.y) / kLineCells;
-            const double fx0 = floor((pt.x - dist - d.rect.x) / cw) - 1.0, fx1 = floor((pt.x + dist - d.rect.x) / cw) + 1.0;
-            const double fy0 = floor((pt.y - dist - d.rect.y) / ch) - 1.0, fy1 = floor((pt.y + dist - d.rect.y) / ch) + 1.0;
-            if (fx1 < 0.0 || fy1 < 0.0 || fx0 > kLineCells - 1.0 || fy0 > kLineCells - 1.0) {
-              walk = false;
-            } else if (fy1 - fy0 < 24.0) {
-              const int i0 = (int)fmax(fx0, 0.0), i1 = (int)fmin(fx1, kLineCells - 1.0);
-              const int j0 = (int)fmax(fy0, 0.0), j1 = (int)fmin(fy1, kLineCells - 1.0);
-              if (i0 <= i1 && j0 <= j1) {
-                // bits i0..i1 of a row, split over its words
-                uint64_t m[kLineRowWords];
-#pragma unroll
-                for (int w = 0; w < kLineRowWords; w++) {
-                  const int lo = max(i0 - 64 * w, 0), hi = min(i1 - 64 * w, 63);
-                  m[w] = lo <= hi ? (~0ull >> (63 - (hi - lo))) << lo : 0ull;
-                }
-                walk = false;
-                for (int r = j0; r <= j1 && !walk; r++) {
-                  const uint64_t* rw = c.R.line_raster + d.raster_off + (int64_t)r * kLineRowWords;
-#pragma unroll
-                  for (int w = 0; w < kLineRowWords; w++) walk |= (rw[w] & m[w]) != 0;
-                }
-              }
-            }
-          }
-        }
-      }
+      if (i < n) walk = line_may_be_within(c, cb[i], c.P.xy[cp[i]], dist);
       // i - lane is a multiple of 64: one mask word per wavefront step
       const uint64_t m = __ballot(walk);
       if (lane == 0 && i < n) mask[i >> 6] = m;
@@ -1857,18 +1862,20 @@ bool pip_refine_applies(const GeomColumn& build, const RingIndex& ri, const Geom
          (predicate == kIntersects || predicate == kCovers || predicate == kContains);
 }
 
-// Runs a raster pass over n candidates (launch(grid, mask, tile_counts))
-// and lists the candidates it marked, in candidate order, into todo (mask:
-// the pass's marks); *count (device memory, in count_store) receives their
-// number.
+// Runs a raster pass over the n candidates (cb, cp) (launch(grid, mask,
+// tile_counts)) and copies the candidates it marked, in candidate order, to
+// open (mask: the pass's marks); *count (device memory, in count_store)
+// receives their number.
 template <class Launch>
-void list_marked(uint64_t n, Launch launch, DevArray<uint32_t>& todo, DevArray<uint64_t>& mask,
-                 DevArray<unsigned long long>& count_store, const unsigned long long** count, hipStream_t s) {
+void list_marked(uint64_t n, Launch launch, const uint32_t* cb, const uint32_t* cp, PairBuf& open,
+                 DevArray<uint64_t>& mask, DevArray<unsigned long long>& count_store,
+                 const unsigned long long** count, hipStream_t s) {
   const uint64_t n_tiles = (n + kRasterTile - 1) / kRasterTile;
   fit(mask, (n + 63) / 64, s);
   DevArray<unsigned long long> cnt(n_tiles + 1, s);
   count_store.alloc(n_tiles + 1, s);
-  fit(todo, n, s);
+  fit(open.b, n, s);
+  fit(open.p, n, s);
   SJ_CHECK(hipMemsetAsync(cnt.p + n_tiles, 0, 8, s));
   const unsigned int g = (unsigned int)std::min<uint64_t>(n_tiles, 4096);
   launch(g, mask.p, cnt.p);
@@ -1876,7 +1883,7 @@ void list_marked(uint64_t n, Launch launch, DevArray<uint32_t>& todo, DevArray<u
   SJ_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt.p, count_store.p, (int)(n_tiles + 1), s));
   DevArray<uint8_t> tmp(tb, s);
   SJ_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, cnt.p, count_store.p, (int)(n_tiles + 1), s));
-  hipLaunchKernelGGL(k_todo_scatter, g, kBlock, 0, s, mask.p, count_store.p, n, todo.p);
+  hipLaunchKernelGGL(k_open_scatter, g, kBlock, 0, s, mask.p, count_store.p, n, cb, cp, open.b.p, open.p.p);
   SJ_CHECK(hipGetLastError());
   // the scan's last entry is the number of marked candidates
   *count = count_store.p + n_tiles;
@@ -1907,16 +1914,17 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
     if (precision == 0 && n <= 0xffffffffull) {
       // certified FP32 pass; candidates with an uncertain test (aux[0] of
       // them) are refined again in FP64 by the second launch
-      DevArray<uint32_t> redo_own, todo_own;
+      DevArray<uint32_t> redo_own;
+      PairBuf open_own;
       DevArray<uint64_t> mask_own;
       DevArray<uint32_t>& redo = ws ? ws->redo : redo_own;
-      DevArray<uint32_t>& todo = ws ? ws->todo : todo_own;
+      PairBuf& open = ws ? ws->open : open_own;
       DevArray<uint64_t>& mask = ws ? ws->mask : mask_own;
       fit(redo, n, s);
-      // candidates in clean raster cells are decided first; the walk takes
-      // the rest from a list
-      DevArray<unsigned long long> todo_count;
-      const unsigned long long* n_todo = nullptr;
+      // candidates in clean raster cells are decided first; the walk (and
+      // the FP64 pass after it) take the rest, copied in order to `open`
+      DevArray<unsigned long long> open_count;
+      const unsigned long long* n_open = nullptr;
       if (ri.n_rasters > 0)
         list_marked(
             n,
@@ -1924,11 +1932,12 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
               hipLaunchKernelGGL(k_raster_pass, g, kBlock, 0, s, c, cb, cp, n, probe_ids, out.b.p, out.p.p,
                                  out.cap(), out.counter.p, marks, tile_cnt);
             },
-            todo, mask, todo_count, &n_todo, s);
-      const uint32_t* list = n_todo ? todo.p : nullptr;
+            cb, cp, open, mask, open_count, &n_open, s);
+      const uint32_t* wb = n_open ? open.b.p : cb;
+      const uint32_t* wp = n_open ? open.p.p : cp;
       auto fp32_pass = [&](auto kernel) {
-        hipLaunchKernelGGL(kernel, grid_for(n), kBlock, 0, s, c, allow, cb, cp, n, list, out.b.p, out.p.p,
-                           out.cap(), out.counter.p, aux.p + 1, probe_ids, redo.p, aux.p, n_todo);
+        hipLaunchKernelGGL(kernel, grid_for(n), kBlock, 0, s, c, allow, wb, wp, n, nullptr, out.b.p, out.p.p,
+                           out.cap(), out.counter.p, aux.p + 1, probe_ids, redo.p, aux.p, n_open);
       };
       // resident waves per SIMD (an autotuned choice; registers spill beyond 4)
       if (pip_waves == 4) fp32_pass(k_refine_pip<kPrecDeferred, 4>);
@@ -1938,7 +1947,7 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
       // (the launches the roofline figures describe)
       if (timer) timer->record_stop(s);
       timer = nullptr;
-      hipLaunchKernelGGL(k_refine_pip<1>, 1024, kBlock, 0, s, c, allow, cb, cp, n, redo.p, out.b.p, out.p.p,
+      hipLaunchKernelGGL(k_refine_pip<1>, 1024, kBlock, 0, s, c, allow, wb, wp, n, redo.p, out.b.p, out.p.p,
                          out.cap(), out.counter.p, nullptr, probe_ids, nullptr, nullptr, aux.p);
     } else if (precision == 0) {
       hipLaunchKernelGGL(k_refine_pip<0>, grid_for(n), kBlock, 0, s, c, allow, cb, cp, n, nullptr, out.b.p,
@@ -2029,12 +2038,12 @@ uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const Geo
   const int kind = plain ? kDistAny : dist_kind(build.type_mask, build.points_only, ri);
   // lines: candidates whose point's square [p - d, p + d] meets no raster
   // cell of the line are dropped first; the walk takes the rest from a list
-  DevArray<uint32_t> todo_own;
+  PairBuf open_own;
   DevArray<uint64_t> mask_own;
-  DevArray<uint32_t>& todo = ws ? ws->todo : todo_own;
+  PairBuf& open = ws ? ws->open : open_own;
   DevArray<uint64_t>& mask = ws ? ws->mask : mask_own;
-  DevArray<unsigned long long> todo_count;
-  const unsigned long long* n_todo = nullptr;
+  DevArray<unsigned long long> open_count;
+  const unsigned long long* n_open = nullptr;
   if (kind == kDistLines && ri.n_line_rasters > 0 && probe.points_only && d >= 0 && d < INFINITY &&
       n <= 0xffffffffull)
     list_marked(
@@ -2042,16 +2051,17 @@ uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const Geo
         [&](unsigned int g, uint64_t* marks, unsigned long long* tile_cnt) {
           hipLaunchKernelGGL(k_dist_raster_pass, g, kBlock, 0, s, c, d, cb, cp, n, marks, tile_cnt);
         },
-        todo, mask, todo_count, &n_todo, s);
-  const uint32_t* list = n_todo ? todo.p : nullptr;
+        cb, cp, open, mask, open_count, &n_open, s);
+  const uint32_t* wb = n_open ? open.b.p : cb;
+  const uint32_t* wp = n_open ? open.p.p : cp;
   switch (kind) {
     case kDistLines:
-      hipLaunchKernelGGL(k_refine_distance<kDistLines>, grid_for(n), kBlock, 0, s, c, d, 0, cb, cp, n, list,
-                         out.b.p, out.p.p, out.cap(), out.counter.p, aux.p, n_todo);
+      hipLaunchKernelGGL(k_refine_distance<kDistLines>, grid_for(n), kBlock, 0, s, c, d, 0, wb, wp, n, nullptr,
+                         out.b.p, out.p.p, out.cap(), out.counter.p, aux.p, n_open);
       break;
     case kDistPolygons:
-      hipLaunchKernelGGL(k_refine_distance<kDistPolygons>, grid_for(n), kBlock, 0, s, c, d, 0, cb, cp, n, list,
-                         out.b.p, out.p.p, out.cap(), out.counter.p, aux.p, n_todo);
+      hipLaunchKernelGGL(k_refine_distance<kDistPolygons>, grid_for(n), kBlock, 0, s, c, d, 0, wb, wp, n, nullptr,
+                         out.b.p, out.p.p, out.cap(), out.counter.p, aux.p, n_open);
       break;
     default:
       hipLaunchKernelGGL(k_refine_distance<kDistAny>, grid_for(n), kBlock, 0, s, c, d, plain, cb, cp, n,

@@ -334,11 +334,12 @@ void fit(DevArray<T>& a, size_t count, hipStream_t s) {
 // stream its buffers are freed on (the join's, then the detached stream).
 struct Workspace {
   PairBuf cand, out;
-  DevArray<uint32_t> todo, redo;
+  PairBuf open;  // the candidates a raster pass leaves to the walk
+  DevArray<uint32_t> redo;
   DevArray<uint64_t> mask;
   void set_stream(hipStream_t s) {
     cand.b.s = cand.p.s = cand.counter.s = out.b.s = out.p.s = out.counter.s = s;
-    todo.s = redo.s = mask.s = s;
+    open.b.s = open.p.s = open.counter.s = redo.s = mask.s = s;
   }
 };
 
