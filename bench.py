@@ -140,9 +140,19 @@ def main():
     tuned = None
     if args.autotune:
         ta = time.time()
-        tuned = G.autotune(ix, rf, pdev.slice(0, min(n_points, 10_000_000)), "contains")
+        # the sample is a vertical strip holding about a tenth of the points:
+        # at the workload's own density, the candidates per polygon (which
+        # decide between filter order and polygon-major order) are those of
+        # the full join, unlike in a random tenth of the points
+        x = wl["probe_xy"][:, 0]
+        lo, hi = np.quantile(x[:: max(1, len(x) // 1_000_000)], [0.45, 0.55])
+        srows = np.nonzero((x >= lo) & (x < hi))[0]
+        sample = G.DeviceWkb.from_host(D.take_rows(probe, srows))
+        tuned = G.autotune(ix, rf, sample, "contains")
+        del sample
         tuned["tune_s"] = time.time() - ta
-        log(f"autotune (10M-point sample): {tuned}")
+        tuned["sample_points"] = int(len(srows))
+        log(f"autotune (strip of {len(srows)} points): {tuned}")
 
     def step(precision, profile=True):
         return G.join_device(ix, rf, pdev, "contains", precision=precision, sort_output=False, sort_probes=True,
@@ -159,13 +169,13 @@ def main():
     for i in range(args.steps):
         r = step(args.precision)
         stage += np.array(list(r.res.stage_ms))
-        sizes.append((r.size, r.candidates, r.segment_tests, r.fallback_tests))
+        sizes.append((r.size, r.candidates, r.segment_tests, r.fallback_tests, r.refine_candidates))
         if i < args.steps - 1:
             r.release()
     elapsed = time.time() - t
     last = r
     ms_per_step = elapsed * 1000 / args.steps
-    pairs, cands, seg_tests, fb = sizes[-1]
+    pairs, cands, seg_tests, fb, walked = sizes[-1]
     assert all(s[0] == pairs for s in sizes), "pair count changed between steps"
     stage /= args.steps
     value = pairs / (ms_per_step / 1000)
@@ -190,19 +200,23 @@ def main():
 
     # Rooflines.  Algorithmic bytes per launch:
     #   filter: 16 B per probe rect read + 8 B per candidate pair written;
-    #   refine: 8 B per candidate read + 8 B per kept pair written + the
-    #           FP32 vertex runs of the build polygons (8 B per vertex, each
-    #           distinct polygon counted once).
-    # The refine stage is three launches on one stream, timed together: the
-    # cell-raster pass (k_raster_pass), the listing of the candidates it left
-    # open (k_todo_scatter) and the certified FP32 walk over them
-    # (k_refine_pip<2>); its measured traffic is the sum of theirs.
-    # The refine's arithmetic roof (FP32 VALU, 8 flops per segment test) is
+    #   refine: 8 B per candidate it reads + 8 B per kept pair (every pair of
+    #           the join: an upper bound where a raster split wrote part of
+    #           them before) + the FP32 vertex runs of the build polygons
+    #           (8 B per vertex, each distinct polygon counted once).
+    # With polygon-major order (1) the cell-raster split runs before the
+    # regrouping (in its stage), and the refine stage is the certified FP32
+    # walk alone (k_refine_pip<2>) over the candidates the split left open.
+    # In filter order (0) the refine stage is three launches on one stream,
+    # timed together: the raster pass (k_raster_pass), the copy of the open
+    # candidates (k_open_scatter) and the walk (no profiled traffic).  The
+    # refine's arithmetic roof (FP32 VALU, 8 flops per segment test) is
     # reported beside its HBM roof.
+    order_used = tuned["refine_order"] if tuned else args.refine_order
     filt_ms, ref_ms = stage[2], stage[3]
     filter_bytes = 16 * n_points + 8 * cands
     verts32 = 8 * (int(wl["counts"].sum()) + n_polys)
-    refine_bytes = 8 * cands + 8 * pairs + verts32
+    refine_bytes = 8 * (walked if order_used == 1 else cands) + 8 * pairs + verts32
     refine_flops = FLOPS_PER_SEGMENT_TEST * seg_tests
     traffic = read_traffic(os.path.join(ROOT, "profiles", "traffic_pip_zipf_100Mx1M_contains.md"))
     full = args.scale == 1.0 and args.precision == 0
@@ -212,8 +226,9 @@ def main():
     roof_filter["frac"] = roof_filter["achieved"] / HBM_PEAK_GBS
     ach_tf = refine_flops / (ref_ms / 1e3) / 1e12
     roof_refine = dict(bound="hbm", achieved=refine_bytes / (ref_ms / 1e3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
-                       traffic=traffic.get("refine_stage") if full else None,
-                       kernel="k_raster_pass+k_todo_scatter+k_refine_pip<2>", ms=ref_ms,
+                       traffic=traffic.get("k_refine_pip") if full and order_used == 1 else None,
+                       kernel="k_refine_pip<2>" if order_used == 1 else "k_raster_pass+k_open_scatter+k_refine_pip<2>",
+                       ms=ref_ms,
                        algorithmic_bytes=refine_bytes,
                        valu_fp32={"achieved": ach_tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                                   "frac": ach_tf / FP32_PEAK_TFLOPS, "flops": refine_flops})
@@ -253,9 +268,10 @@ def main():
         "cpu_baseline": cpu,
         "pairs_per_step": int(pairs),
         "candidates_per_step": int(cands),
+        "walked_candidates_per_step": int(walked),
         "probe_points_per_s": n_points / (ms_per_step / 1000),
         "stage_ms": {"decode_rects": stage[0], "morton": stage[1], "filter": stage[2],
-                     "group_by_polygon": stage[5], "refine": stage[3]},
+                     "raster_split_and_group": stage[5], "refine": stage[3]},
         "segment_tests": int(seg_tests),
         "fallback_rate": fb / max(1, seg_tests),
         "build_ms": build_ms,
@@ -645,7 +661,7 @@ def run_poly_poly(args):
         "config": {"workload": "poly_poly", "polygons_each_side": na, "scale_of_5M": args.scale,
                    "predicate": "ST_Intersects", "parallelism": "single-gpu"},
         "pairs_per_step": pairs, "candidates_per_step": cands,
-        "stage_ms": {"decode_rects": stage[0], "morton": stage[1], "filter": stage[2], "group_by_polygon": stage[5],
+        "stage_ms": {"decode_rects": stage[0], "morton": stage[1], "filter": stage[2], "raster_split_and_group": stage[5],
                      "refine": stage[3]},
         "cpu_baseline": cpu,
     }

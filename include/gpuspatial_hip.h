@@ -239,11 +239,14 @@ struct SjhipJoinResult {
                                in FP64 (their candidate re-refined in FP64
                                by the point-in-polygon kernel) */
   uint64_t segment_tests;   /* segment tests performed */
+  uint64_t refine_candidates; /* candidates the refine's walk decided: those
+                                 the cell-raster passes left open */
   /* device time of the stages (ms, HIP events on the join's stream) when
    * options.profile is set: [0] probe decode + rects, [1] Morton order of
    * probes, [2] filter kernel, [3] refine kernels, [4] copy of the kept
    * pairs into the result (exact size) and output sort, [5] candidate
-   * regrouping (refine_order 1) */
+   * regrouping (refine_order 1; for the point-in-polygon refine with
+   * cell rasters, the raster split before it) */
   double stage_ms[6];
   void (*release)(struct SjhipJoinResult* self);
   void* private_data;

@@ -643,22 +643,30 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
     // result gets the kept pairs at their exact size
     WorkspaceLease lease(rf, s);
     Workspace* ws = lease.ws.get();
-    PairBuf& cand = ws->cand;
-    nc = filter_pairs(ix->tree, q.p, nq, ord, cand, s, ev.on ? &ft : nullptr, ranked);
+    nc = filter_pairs(ix->tree, q.p, nq, ord, ws->cand, s, ev.on ? &ft : nullptr, ranked);
     mark(3);
+    // point in polygon, polygon-major order: candidates in clean raster
+    // cells are decided first, so that the regrouping sorts only the rest
+    uint64_t n_open = 0;
+    const bool split = !dwithin && order_used == 1 &&
+                       raster_split(rf->build, rf->ring_index, rp, o.predicate, o.precision, ws->cand.b.p,
+                                    ws->cand.p.p, nc, ranked ? ord : nullptr, ws->out, ws->open, ws->mask, &n_open,
+                                    s);
+    PairBuf& cand = split ? ws->open : ws->cand;
+    const uint64_t n_cand = split ? n_open : nc;
     // polygon-major order: one wavefront then walks few distinct rings
     // (height slabs only for the point-in-polygon refine)
     if (order_used == 1) {
       const float4* br = !dwithin && ix->rects.n >= (size_t)rf->build.n_rows ? ix->rects.p : nullptr;
-      sort_pairs_by_build(cand, nc, rf->build.n_rows, br, rp, s);
+      sort_pairs_by_build(cand, n_cand, rf->build.n_rows, br, rp, s);
     }
     if (dwithin)
-      kept = refine_distance(rf->build, rf->ring_index, probe, o.distance, cand.b.p, cand.p.p, nc, ws->out, s,
+      kept = refine_distance(rf->build, rf->ring_index, probe, o.distance, cand.b.p, cand.p.p, n_cand, ws->out, s,
                              ev.on ? &rt : nullptr, o.precision, &st, ws);
     else
-      kept = refine_relation(rf->build, rf->ring_index, rp, o.predicate, cand.b.p, cand.p.p, nc, o.precision,
+      kept = refine_relation(rf->build, rf->ring_index, rp, o.predicate, cand.b.p, cand.p.p, n_cand, o.precision,
                              ws->out, &st, s, ev.on ? &rt : nullptr, ranked ? ord : nullptr, ix->tuning.pip_waves,
-                             ws);
+                             ws, split);
     mark(8);
     jr->pairs.b.alloc(kept > 0 ? kept : 1, s);
     jr->pairs.p.alloc(kept > 0 ? kept : 1, s);
@@ -689,6 +697,7 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
   out->candidates = nc;
   out->segment_tests = st.segment_tests;
   out->fallback_tests = st.fallback_tests;
+  out->refine_candidates = st.walked;
   out->release = jr_release;
   out->private_data = jr.release();
 }
@@ -859,6 +868,7 @@ bool run_plan(GraphPlan& g, const WkbInput& in, SjhipJoinResult* out, hipStream_
   out->candidates = h[1];
   out->segment_tests = h[4];
   out->fallback_tests = h[5];
+  out->refine_candidates = h[1];  // the captured sequence walks every candidate
   out->release = jr_release;
   out->private_data = jr.release();
   return true;

@@ -917,6 +917,20 @@ __global__ __launch_bounds__(kBlock) void k_refine_polypoly(Ctx c, const uint32_
   wave_flush(wo, ob, op, cap, counter);
 }
 
+// Cell-raster class of a candidate's point for a single-ring polygon row:
+// 1 inside, 2 outside (a clean cell, or outside the ring's rect), 0 for
+// the walk (a cell near the boundary, or no raster).
+__device__ __forceinline__ int pip_raster_class(const Ctx& c, uint32_t b, double2 pt) {
+  const PolyDesc d = c.R.desc[b];
+  if (d.raster_off < 0) return 0;
+  if (!rect_covers(d.rect, pt.x, pt.y)) return 2;
+  const double cw = (d.rect.z - d.rect.x) / kRasterCells, ch = (d.rect.w - d.rect.y) / kRasterCells;
+  const int ci = min(max((int)floor((pt.x - d.rect.x) / cw), 0), kRasterCells - 1);
+  const int cj = min(max((int)floor((pt.y - d.rect.y) / ch), 0), kRasterCells - 1);
+  const int cell = cj * kRasterCells + ci;
+  return (c.R.raster[d.raster_off + (cell >> 4)] >> (2 * (cell & 15))) & 3;
+}
+
 // Cell-raster pass of the point-in-polygon refine: candidates whose point
 // falls in a clean cell of a single-ring polygon are decided here (kept or
 // dropped); the others are marked for the walk in per-wavefront bit masks
@@ -948,19 +962,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_pass(Ctx c, const uint32_t* c
         p = cp[i];
         const double2 pt = c.P.xy[p];
         if (pt.x == pt.x || pt.y == pt.y) {
-          const PolyDesc d = c.R.desc[b];
-          int cls = 0;
-          if (d.raster_off >= 0) {
-            if (!rect_covers(d.rect, pt.x, pt.y)) {
-              cls = 2;
-            } else {
-              const double cw = (d.rect.z - d.rect.x) / kRasterCells, ch = (d.rect.w - d.rect.y) / kRasterCells;
-              const int ci = min(max((int)floor((pt.x - d.rect.x) / cw), 0), kRasterCells - 1);
-              const int cj = min(max((int)floor((pt.y - d.rect.y) / ch), 0), kRasterCells - 1);
-              const int cell = cj * kRasterCells + ci;
-              cls = (c.R.raster[d.raster_off + (cell >> 4)] >> (2 * (cell & 15))) & 3;
-            }
-          }
+          const int cls = pip_raster_class(c, b, pt);
           keep = cls == 1;
           walk = cls == 0;
         }
@@ -1889,10 +1891,42 @@ void list_marked(uint64_t n, Launch launch, const uint32_t* cb, const uint32_t* 
   *count = count_store.p + n_tiles;
 }
 
+bool raster_split(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, int predicate,
+                  int precision, const uint32_t* cb, const uint32_t* cp, uint64_t n, const uint32_t* probe_ids,
+                  PairBuf& out, PairBuf& open, DevArray<uint64_t>& mask, uint64_t* n_open, hipStream_t s) {
+  if (ri.n_rasters == 0 || precision != 0 || n > 0xffffffffull || build.points_only || ri.ring_rect.p == nullptr ||
+      !pip_refine_applies(build, ri, probe, predicate))
+    return false;
+  out.counter.reserve(1, s);
+  fit(out.b, n > 0 ? n : 1, s);
+  fit(out.p, n > 0 ? n : 1, s);
+  SJ_CHECK(hipMemsetAsync(out.counter.p, 0, 8, s));
+  *n_open = 0;
+  if (n == 0) return true;
+  Ctx c;
+  c.B = build.view();
+  c.R = ri.view();
+  c.P = probe.view();
+  DevArray<unsigned long long> open_count;
+  const unsigned long long* count = nullptr;
+  list_marked(
+      n,
+      [&](unsigned int g, uint64_t* marks, unsigned long long* tile_cnt) {
+        hipLaunchKernelGGL(k_raster_pass, g, kBlock, 0, s, c, cb, cp, n, probe_ids, out.b.p, out.p.p, out.cap(),
+                           out.counter.p, marks, tile_cnt);
+      },
+      cb, cp, open, mask, open_count, &count, s);
+  unsigned long long h = 0;
+  SJ_CHECK(hipMemcpyAsync(&h, count, 8, hipMemcpyDeviceToHost, s));
+  SJ_CHECK(hipStreamSynchronize(s));
+  *n_open = h;
+  return true;
+}
+
 uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, int predicate,
                          const uint32_t* cb, const uint32_t* cp, uint64_t n, int precision, PairBuf& out,
                          RefineStats* stats, hipStream_t s, KernelTimer* timer, const uint32_t* probe_ids,
-                         int pip_waves, Workspace* ws) {
+                         int pip_waves, Workspace* ws, bool raster_done) {
   out.counter.reserve(1, s);
   fit(out.b, n > 0 ? n : 1, s);
   fit(out.p, n > 0 ? n : 1, s);
@@ -1906,10 +1940,13 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
   DevArray<uint32_t> long_list;
   bool pip = pip_refine_applies(build, ri, probe, predicate);
   if (probe_ids && !pip) throw Error("probe id mapping needs the point-in-polygon refine");
-  SJ_CHECK(hipMemsetAsync(out.counter.p, 0, 8, s));
+  if (raster_done && !(pip && precision == 0)) throw Error("raster split without the certified point-in-polygon refine");
+  // after raster_split, out already holds the decided pairs
+  if (!raster_done) SJ_CHECK(hipMemsetAsync(out.counter.p, 0, 8, s));
   SJ_CHECK(hipMemsetAsync(aux.p, 0, 32, s));
   if (pip) {
     int allow = predicate == kContains ? 0 : 1;
+    unsigned long long walked = n;  // candidates the walk decides (host copy)
     if (timer) timer->record_start(s);
     if (precision == 0 && n <= 0xffffffffull) {
       // certified FP32 pass; candidates with an uncertain test (aux[0] of
@@ -1925,7 +1962,7 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
       // the FP64 pass after it) take the rest, copied in order to `open`
       DevArray<unsigned long long> open_count;
       const unsigned long long* n_open = nullptr;
-      if (ri.n_rasters > 0)
+      if (ri.n_rasters > 0 && !raster_done)
         list_marked(
             n,
             [&](unsigned int g, uint64_t* marks, unsigned long long* tile_cnt) {
@@ -1949,6 +1986,8 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
       timer = nullptr;
       hipLaunchKernelGGL(k_refine_pip<1>, 1024, kBlock, 0, s, c, allow, wb, wp, n, redo.p, out.b.p, out.p.p,
                          out.cap(), out.counter.p, nullptr, probe_ids, nullptr, nullptr, aux.p);
+      // (read before open_count is released on the stream)
+      if (n_open) SJ_CHECK(hipMemcpyAsync(&walked, n_open, 8, hipMemcpyDeviceToHost, s));
     } else if (precision == 0) {
       hipLaunchKernelGGL(k_refine_pip<0>, grid_for(n), kBlock, 0, s, c, allow, cb, cp, n, nullptr, out.b.p,
                          out.p.p, out.cap(), out.counter.p, aux.p + 1, probe_ids, nullptr, nullptr, nullptr);
@@ -1965,9 +2004,11 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
     if (stats) {
       stats->segment_tests += h[1];
       stats->fallback_tests += h[2];
+      stats->walked += walked;
     }
     return cnt;
   }
+  if (stats) stats->walked += n;
   // polygon x polygon: the probe's rings get segment trees too
   const uint32_t polys = (1u << kPolygon) | (1u << kMultiPolygon);
   RingIndex probe_ri;
@@ -2069,13 +2110,15 @@ uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const Geo
   }
   SJ_CHECK(hipGetLastError());
   if (timer) timer->record_stop(s);
-  unsigned long long cnt = 0, h[2];
+  unsigned long long cnt = 0, h[2], walked = n;
   SJ_CHECK(hipMemcpyAsync(&cnt, out.counter.p, 8, hipMemcpyDeviceToHost, s));
   SJ_CHECK(hipMemcpyAsync(h, aux.p, 16, hipMemcpyDeviceToHost, s));
+  if (n_open) SJ_CHECK(hipMemcpyAsync(&walked, n_open, 8, hipMemcpyDeviceToHost, s));
   SJ_CHECK(hipStreamSynchronize(s));
   if (stats) {
     stats->segment_tests += h[0];
     stats->fallback_tests += h[1];
+    stats->walked += walked;
   }
   return cnt;
 }

@@ -363,6 +363,7 @@ int64_t morton_order(const float4* rects, int64_t n, DevArray<uint32_t>& order, 
 struct RefineStats {
   uint64_t segment_tests = 0;
   uint64_t fallback_tests = 0;
+  uint64_t walked = 0;  // candidates the refine's walk decided (after cell-raster passes)
 };
 
 // Whether refine_relation takes its point-in-polygon kernel (polygon
@@ -376,7 +377,21 @@ bool pip_refine_applies(const GeomColumn& build, const RingIndex& ri, const Geom
 uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, int predicate,
                          const uint32_t* cb, const uint32_t* cp, uint64_t n, int precision, PairBuf& out,
                          RefineStats* stats, hipStream_t s, KernelTimer* timer = nullptr,
-                         const uint32_t* probe_ids = nullptr, int pip_waves = 8, Workspace* ws = nullptr);
+                         const uint32_t* probe_ids = nullptr, int pip_waves = 8, Workspace* ws = nullptr,
+                         bool raster_done = false);
+
+// Cell-raster split of point-in-polygon candidates, before they are
+// regrouped: of the n candidates (cb, cp), those whose point lies in a
+// clean raster cell are decided (kept pairs written to out, probe
+// positions mapped through probe_ids; out.counter set to their number) and
+// the others copied, in candidate order, to open (mask: scratch for the
+// marks).  Returns false, doing nothing, where it does not apply (no
+// rasters, not the certified point-in-polygon refine); else *n_open = the
+// open candidates, which refine_relation(..., raster_done = true) then
+// refines, appending to out.
+bool raster_split(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, int predicate,
+                  int precision, const uint32_t* cb, const uint32_t* cp, uint64_t n, const uint32_t* probe_ids,
+                  PairBuf& out, PairBuf& open, DevArray<uint64_t>& mask, uint64_t* n_open, hipStream_t s);
 
 // Distance refine (build.distance(probe) <= d); probe must be points.
 // precision 1: the all-FP64 form over every segment.  stats: FP64 segment

@@ -154,6 +154,7 @@ SjhipJoinResult._fields_ = [
     ("candidates", ctypes.c_uint64),
     ("fallback_tests", ctypes.c_uint64),
     ("segment_tests", ctypes.c_uint64),
+    ("refine_candidates", ctypes.c_uint64),
     ("stage_ms", ctypes.c_double * 6),
     ("release", ctypes.CFUNCTYPE(None, ctypes.POINTER(SjhipJoinResult))),
     ("private_data", ctypes.c_void_p),

@@ -462,6 +462,10 @@ class JoinResult:
         return self.res.segment_tests
 
     @property
+    def refine_candidates(self):
+        return self.res.refine_candidates
+
+    @property
     def fallback_tests(self):
         return self.res.fallback_tests
 
