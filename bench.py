@@ -448,13 +448,13 @@ def run_dwithin_walks(args):
     bsz = args.probe_batch
 
     def step(precision, keep_pairs=False):
-        tot = np.zeros(4, np.int64)
+        tot = np.zeros(5, np.int64)
         stage = np.zeros(6)
         kept = []
         for a in range(0, n_points, bsz):
             r = G.join_device(ix, rf, pdev.slice(a, min(n_points, a + bsz)), "dwithin", distance=dist,
                               precision=precision, sort_output=False, refine_order=args.refine_order, profile=True)
-            tot += (r.size, r.candidates, r.segment_tests, r.fallback_tests)
+            tot += (r.size, r.candidates, r.segment_tests, r.fallback_tests, r.refine_candidates)
             stage += np.array(list(r.res.stage_ms))
             if keep_pairs:
                 gb, gp = r.to_host()
@@ -469,7 +469,7 @@ def run_dwithin_walks(args):
     for i in range(args.steps):
         tot, stage, _ = step(args.precision)
     ms_per_step = (time.time() - t) * 1000 / args.steps
-    pairs, cands, tests, scans = (int(x) for x in tot)
+    pairs, cands, tests, scans, walked = (int(x) for x in tot)
     log(f"timed: {ms_per_step:.1f} ms/step, {pairs} pairs, {cands} candidates, stages(ms) {list(stage)}")
     plain = None
     if not args.no_plain:
@@ -504,7 +504,8 @@ def run_dwithin_walks(args):
         "config": {"workload": "dwithin_walks", "probe_points": n_points, "build_lines": n_lines,
                    "build_vertices": int(wl["counts"].sum()), "predicate": "ST_DWithin", "distance": dist,
                    "probe_batch": bsz, "parallelism": "single-gpu"},
-        "pairs_per_step": pairs, "candidates_per_step": cands, "probe_points_per_s": n_points / (ms_per_step / 1000),
+        "pairs_per_step": pairs, "candidates_per_step": cands, "walked_candidates_per_step": walked,
+        "probe_points_per_s": n_points / (ms_per_step / 1000),
         "stage_ms": {"decode_rects": stage[0], "morton": stage[1], "filter": stage[2], "group_by_line": stage[5],
                      "refine": stage[3]},
         "roofline_filter": {"bound": "hbm", "achieved": filter_bytes / (stage[2] / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
@@ -647,7 +648,9 @@ def run_poly_poly(args):
             r.release()
     ms_per_step = (time.time() - t) * 1000 / args.steps
     pairs, cands, stage = int(r.size), int(r.candidates), list(r.res.stage_ms)
-    log(f"timed: {ms_per_step:.1f} ms/step, {pairs} pairs, {cands} candidates, stages {stage}")
+    walked = int(r.refine_candidates)
+    log(f"timed: {ms_per_step:.1f} ms/step, {pairs} pairs, {cands} candidates ({walked} left to the wavefront "
+        f"kernel by the raster pass), stages {stage}")
     cpu = None
     if strip is not None:
         gb, gp = r.to_host()
@@ -660,8 +663,8 @@ def run_poly_poly(args):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded, generated on the host)",
         "config": {"workload": "poly_poly", "polygons_each_side": na, "scale_of_5M": args.scale,
                    "predicate": "ST_Intersects", "parallelism": "single-gpu"},
-        "pairs_per_step": pairs, "candidates_per_step": cands,
-        "stage_ms": {"decode_rects": stage[0], "morton": stage[1], "filter": stage[2], "raster_split_and_group": stage[5],
+        "pairs_per_step": pairs, "candidates_per_step": cands, "walked_candidates_per_step": walked,
+        "stage_ms": {"decode_rects": stage[0], "morton": stage[1], "filter": stage[2], "group_by_polygon": stage[5],
                      "refine": stage[3]},
         "cpu_baseline": cpu,
     }

@@ -892,8 +892,10 @@ __device__ bool wave_iring_intersects(IRing ring, IRing other, int lane) {
 // parts with holes decided by every lane alike).
 __global__ __launch_bounds__(kBlock) void k_refine_polypoly(Ctx c, const uint32_t* cb, const uint32_t* cp,
                                                             uint64_t n, uint32_t* ob, uint32_t* op, uint64_t cap,
-                                                            unsigned long long* counter) {
+                                                            unsigned long long* counter,
+                                                            const unsigned long long* n_dev) {
   SJ_WAVE_OUT(wo);
+  if (n_dev) n = min(n, (uint64_t)*n_dev);  // candidate count produced on the device (n: its bound)
   const int lane = threadIdx.x & 63;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
@@ -917,19 +919,55 @@ __global__ __launch_bounds__(kBlock) void k_refine_polypoly(Ctx c, const uint32_
   wave_flush(wo, ob, op, cap, counter);
 }
 
-// Cell-raster class of a candidate's point for a single-ring polygon row:
-// 1 inside, 2 outside (a clean cell, or outside the ring's rect), 0 for
-// the walk (a cell near the boundary, or no raster).
-__device__ __forceinline__ int pip_raster_class(const Ctx& c, uint32_t b, double2 pt) {
-  const PolyDesc d = c.R.desc[b];
+// Cell-raster class of a point for a single-ring polygon (record d, its
+// ring index R): 1 inside, 2 outside (a clean cell, or outside the ring's
+// rect), 0 undecided (a cell near the boundary, or no raster).
+__device__ __forceinline__ int raster_class(const RingIndexView& R, const PolyDesc& d, double2 pt) {
   if (d.raster_off < 0) return 0;
   if (!rect_covers(d.rect, pt.x, pt.y)) return 2;
   const double cw = (d.rect.z - d.rect.x) / kRasterCells, ch = (d.rect.w - d.rect.y) / kRasterCells;
   const int ci = min(max((int)floor((pt.x - d.rect.x) / cw), 0), kRasterCells - 1);
   const int cj = min(max((int)floor((pt.y - d.rect.y) / ch), 0), kRasterCells - 1);
   const int cell = cj * kRasterCells + ci;
-  return (c.R.raster[d.raster_off + (cell >> 4)] >> (2 * (cell & 15))) & 3;
+  return (R.raster[d.raster_off + (cell >> 4)] >> (2 * (cell & 15))) & 3;
 }
+
+// Whether the raster shows no point of the polygon (record d) in rect o,
+// a part of its rect: every cell that o meets, widened by one cell, is
+// clean outside (such a cell holds no boundary and lies outside).  Windows
+// of more than 64 cells are not read (false).
+__device__ __forceinline__ bool raster_outside(const RingIndexView& R, const PolyDesc& d, double4 o) {
+  if (d.raster_off < 0) return false;
+  const double cw = (d.rect.z - d.rect.x) / kRasterCells, ch = (d.rect.w - d.rect.y) / kRasterCells;
+  const int i0 = (int)fmax(floor((o.x - d.rect.x) / cw) - 1.0, 0.0);
+  const int i1 = (int)fmin(floor((o.z - d.rect.x) / cw) + 1.0, kRasterCells - 1.0);
+  const int j0 = (int)fmax(floor((o.y - d.rect.y) / ch) - 1.0, 0.0);
+  const int j1 = (int)fmin(floor((o.w - d.rect.y) / ch) + 1.0, kRasterCells - 1.0);
+  if (i0 > i1 || j0 > j1 || (i1 - i0 + 1) * (j1 - j0 + 1) > 64) return false;
+  for (int j = j0; j <= j1; j++)
+    for (int i = i0; i <= i1; i++) {
+      const int cell = j * kRasterCells + i;
+      if (((R.raster[d.raster_off + (cell >> 4)] >> (2 * (cell & 15))) & 3) != 2) return false;
+    }
+  return true;
+}
+
+// The class of a candidate's point for build row b.
+__device__ __forceinline__ int pip_raster_class(const Ctx& c, uint32_t b, double2 pt) {
+  return raster_class(c.R, c.R.desc[b], pt);
+}
+
+// Cell-raster pass of the polygon x polygon refine (ST_Intersects): a
+// candidate is kept when it finds a point of both polygons in clean inside
+// cells of both rasters (the centre of the rects' overlap, or one of four
+// spread vertices of either ring inside the other), dropped when the
+// rings' rects do not meet or one raster shows no point of its polygon in
+// the rects' overlap, and else marked (as in k_raster_pass) for
+// k_refine_polypoly.
+__global__ __launch_bounds__(kBlock) void k_pp_raster_pass(Ctx c, const uint32_t* cb, const uint32_t* cp, uint64_t n,
+                                                           uint32_t* ob, uint32_t* op, uint64_t cap,
+                                                           unsigned long long* counter, uint64_t* mask,
+                                                           unsigned long long* tile_cnt);
 
 // Cell-raster pass of the point-in-polygon refine: candidates whose point
 // falls in a clean cell of a single-ring polygon are decided here (kept or
@@ -971,6 +1009,59 @@ __global__ __launch_bounds__(kBlock) void k_raster_pass(Ctx c, const uint32_t* c
       emit(wo, keep, b, p, ob, op, cap, counter);
       // i - lane is a multiple of 64: one mask word per wavefront step
       const uint64_t m = __ballot(walk);
+      if (lane == 0 && i < n) mask[i >> 6] = m;
+      wave_cnt += __popcll(m);
+    }
+    if (lane == 0 && wave_cnt) atomicAdd(&block_cnt, wave_cnt);
+    __syncthreads();
+    if (threadIdx.x == 0) tile_cnt[t] = block_cnt;
+  }
+  wave_flush(wo, ob, op, cap, counter);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pp_raster_pass(Ctx c, const uint32_t* cb, const uint32_t* cp, uint64_t n,
+                                                           uint32_t* ob, uint32_t* op, uint64_t cap,
+                                                           unsigned long long* counter, uint64_t* mask,
+                                                           unsigned long long* tile_cnt) {
+  SJ_WAVE_OUT(wo);
+  __shared__ unsigned int block_cnt;
+  const uint64_t n_tiles = (n + kRasterTile - 1) / kRasterTile;
+  const int lane = threadIdx.x & 63;
+  for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    if (threadIdx.x == 0) block_cnt = 0;
+    __syncthreads();
+    unsigned int wave_cnt = 0;
+    for (int it = 0; it < kRasterTile / kBlock; it++) {
+      const uint64_t i = t * kRasterTile + (uint64_t)it * kBlock + threadIdx.x;
+      bool keep = false, open = false;
+      uint32_t b = 0, p = 0;
+      if (i < n) {
+        b = cb[i];
+        p = cp[i];
+        open = true;
+        const PolyDesc db = c.R.desc[b], dp = c.PR.desc[p];
+        if (db.npts >= 3 && dp.npts >= 3) {  // one ring each
+          const double4 ra = db.rect, rb = dp.rect;
+          if (rb.x > ra.z || rb.z < ra.x || rb.y > ra.w || rb.w < ra.y) {
+            open = false;
+          } else {
+            // a point of both: the centre of the rects' overlap inside both,
+            // or one of four spread vertices of either ring inside the other
+            const double2 m = make_double2(0.5 * (fmax(ra.x, rb.x) + fmin(ra.z, rb.z)),
+                                           0.5 * (fmax(ra.y, rb.y) + fmin(ra.w, rb.w)));
+            keep = raster_class(c.R, db, m) == 1 && raster_class(c.PR, dp, m) == 1;
+            for (int k = 0; k < 4 && !keep; k++) {
+              keep = raster_class(c.R, db, c.P.xy[dp.v0 + (int64_t)k * (dp.npts - 1) / 4]) == 1 ||
+                     raster_class(c.PR, dp, c.B.xy[db.v0 + (int64_t)k * (db.npts - 1) / 4]) == 1;
+            }
+            // no point of both: one polygon has none in the rects' overlap
+            const double4 ov = make_double4(fmax(ra.x, rb.x), fmax(ra.y, rb.y), fmin(ra.z, rb.z), fmin(ra.w, rb.w));
+            open = !keep && !raster_outside(c.R, db, ov) && !raster_outside(c.PR, dp, ov);
+          }
+        }
+      }
+      emit(wo, keep, b, p, ob, op, cap, counter);
+      const uint64_t m = __ballot(open);
       if (lane == 0 && i < n) mask[i >> 6] = m;
       wave_cnt += __popcll(m);
     }
@@ -2008,27 +2099,45 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
     }
     return cnt;
   }
-  if (stats) stats->walked += n;
-  // polygon x polygon: the probe's rings get segment trees too
+  // polygon x polygon: the probe's rings get segment trees (and rasters) too
   const uint32_t polys = (1u << kPolygon) | (1u << kMultiPolygon);
   RingIndex probe_ri;
   if (predicate == kIntersects && (build.type_mask & polys) && (probe.type_mask & polys) && !probe.points_only &&
       c.R.ring_rect != nullptr) {
     RingIndexParams prm;
-    prm.raster = false;  // rasters serve point probes only
     build_ring_index(probe, probe_ri, prm, s);
     c.PR = probe_ri.view();
   }
   if (c.PR.ring_rect != nullptr && (build.type_mask & ~polys) == 0 && (probe.type_mask & ~polys) == 0) {
     if (timer) timer->record_start(s);
+    // candidates decided by the rings' rects and cell rasters first; the
+    // wavefront-per-candidate kernel takes the rest, copied in order
+    PairBuf open_own;
+    DevArray<uint64_t> mask_own;
+    PairBuf& open = ws ? ws->open : open_own;
+    DevArray<uint64_t>& mask = ws ? ws->mask : mask_own;
+    DevArray<unsigned long long> open_count;
+    const unsigned long long* n_open = nullptr;
+    if ((ri.n_rasters > 0 || probe_ri.n_rasters > 0) && n <= 0xffffffffull)
+      list_marked(
+          n,
+          [&](unsigned int g, uint64_t* marks, unsigned long long* tile_cnt) {
+            hipLaunchKernelGGL(k_pp_raster_pass, g, kBlock, 0, s, c, cb, cp, n, out.b.p, out.p.p, out.cap(),
+                               out.counter.p, marks, tile_cnt);
+          },
+          cb, cp, open, mask, open_count, &n_open, s);
+    const uint32_t* wb = n_open ? open.b.p : cb;
+    const uint32_t* wp = n_open ? open.p.p : cp;
     const uint64_t waves = std::min<uint64_t>(n, 1u << 18);
-    hipLaunchKernelGGL(k_refine_polypoly, (unsigned int)((waves * 64 + kBlock - 1) / kBlock), kBlock, 0, s, c, cb, cp,
-                       n, out.b.p, out.p.p, out.cap(), out.counter.p);
+    hipLaunchKernelGGL(k_refine_polypoly, (unsigned int)((waves * 64 + kBlock - 1) / kBlock), kBlock, 0, s, c, wb, wp,
+                       n, out.b.p, out.p.p, out.cap(), out.counter.p, n_open);
     SJ_CHECK(hipGetLastError());
     if (timer) timer->record_stop(s);
-    unsigned long long cnt = 0;
+    unsigned long long cnt = 0, walked = n;
     SJ_CHECK(hipMemcpyAsync(&cnt, out.counter.p, 8, hipMemcpyDeviceToHost, s));
+    if (n_open) SJ_CHECK(hipMemcpyAsync(&walked, n_open, 8, hipMemcpyDeviceToHost, s));
     SJ_CHECK(hipStreamSynchronize(s));
+    if (stats) stats->walked += walked;
     return cnt;
   }
   bool use_long = !build.points_only && probe.points_only && c.R.ring_rect != nullptr &&
@@ -2054,6 +2163,7 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
   if (stats) {
     stats->segment_tests += h[1];
     stats->fallback_tests += h[2];
+    stats->walked += n;
   }
   return cnt;
 }
