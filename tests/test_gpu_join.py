@@ -363,6 +363,43 @@ def test_poly_poly_large_rings_vs_oracle(opts):
     assert len(gb) > 1000
 
 
+def _star(cx, cy, r_in, r_out, n, phase=0.0):
+    ang = phase + np.arange(n) * (2 * np.pi / n)
+    r = np.where(np.arange(n) % 2 == 0, r_out, r_in)
+    ring = np.stack([cx + r * np.cos(ang), cy + r * np.sin(ang)], axis=1)
+    return np.vstack([ring, ring[:1]])
+
+
+def test_poly_poly_raster_cases_vs_oracle(opts):
+    """Polygon pairs that the raster pass decides or leaves to the exact
+    kernel: nested stars (a common point at the centre of the rects'
+    overlap), stars apart whose rects overlap (a raster with no point in
+    the overlap), stars touching at a vertex, squares sharing an edge and a
+    corner (many vertices on their sides), and crossing stars."""
+    side = np.linspace(0.0, 10.0, 41)[:-1]
+    sq = lambda x0, y0: np.vstack([np.stack([x0 + side, y0 + 0 * side], 1), np.stack([x0 + 10 + 0 * side, y0 + side], 1),
+                                   np.stack([x0 + 10 - side, y0 + 10 + 0 * side], 1),
+                                   np.stack([x0 + 0 * side, y0 + 10 - side], 1), [[x0, y0]]])
+    build = [_star(0, 0, 6, 10, 200), _star(40, 0, 2, 10, 64), sq(100, 0), _star(200, 0, 7, 10, 300)]
+    touch = _star(60, 0, 7, 10, 128)
+    touch[64] = (50.0, 0.0)  # build 1's first vertex: the two stars touch there
+    probe = [_star(0, 0, 1, 2, 50),  # nested in build 0
+             _star(47.5, 7.5, 0.5, 2, 40),  # in build 1's rect corner, outside the star
+             touch,
+             sq(110, 0), sq(110, 10), sq(95, 5),  # shares an edge / a corner / overlaps build 2
+             _star(212, 3, 6, 9, 256, phase=0.3),  # crosses build 3
+             _star(0, 30, 6, 10, 200)]  # disjoint rects
+    b = D.rings_wkb([len(r) for r in build], np.vstack(build), 3)
+    p = D.rings_wkb([len(r) for r in probe], np.vstack(probe), 3)
+    ix, rf = build_pair(b.to_arrow(), opts)
+    r = G.join_device(ix, rf, p.to_arrow(), "intersects", sort_output=True)
+    gb, gp = r.to_host()
+    ob, op = O.join(O.Store(b.as_tuple()), O.Store(p.as_tuple()), "intersects", nthreads=8)
+    say("poly raster cases", list(zip(ob.tolist(), op.tolist())))
+    assert np.array_equal(gb, ob) and np.array_equal(gp, op), (list(zip(gb, gp)), list(zip(ob, op)))
+    assert (1, 2) in set(zip(ob.tolist(), op.tolist())) and len(ob) >= 6
+
+
 def test_dwithin_polygons_and_refine_distance_abi(opts):
     b, p = D.workload_pip_uniform(20_000, 200, seed=17, extent=100.0 * np.sqrt(200 / 10_000))
     ix, rf = build_pair(b.to_arrow(), opts)
