@@ -926,6 +926,47 @@ static int point_covers_polyish(const ora_store* s, int64_t g, double px, double
   return 1;
 }
 
+/* tg.c:1171-1186 tg_point_touches_poly: the point lies on a segment of the
+ * exterior ring or of a hole (tg_line_covers_point, tg.c:3417-3443). */
+static int poly_boundary_covers_point(const poly_t* p, double px, double py) {
+  for (int64_t k = 0; k < poly_nrings(p); k++) {
+    ring_t r = poly_ring(p, k);
+    for (int64_t i = 0; i < ring_nsegs(&r); i++) {
+      double sg[4];
+      ring_seg(&r, i, sg);
+      if (segment_covers_point(sg, px, py)) return 1;
+    }
+  }
+  return 0;
+}
+
+/* tg_geom_touches(poly or multipolygon, point).  Polygon: tg.c:6682-6690
+ * -> tg_poly_touches_point (tg.c:3957-3959).  Multipolygon: tg.c:6735-6763,
+ * some part touches and no part that does not touch intersects. */
+static int polyish_touches_point(const ora_store* s, int64_t g, double px, double py) {
+  if (s->type[g] == ORA_POLYGON) {
+    poly_t p = {s, s->part_off[g]};
+    return poly_boundary_covers_point(&p, px, py);
+  }
+  int touches = 0;
+  for (int64_t q = s->part_off[g]; q < s->part_off[g + 1]; q++) {
+    poly_t p = {s, q};
+    if (poly_boundary_covers_point(&p, px, py)) touches = 1;
+    else if (poly_contains_point(&p, px, py, 1)) return 0;
+  }
+  return touches;
+}
+
+/* tg_geom_touches(point, poly or multipolygon): tg.c:6578-6604, the point
+ * touches some part (no test of the other parts' interiors). */
+static int point_touches_polyish(const ora_store* s, int64_t g, double px, double py) {
+  for (int64_t q = s->part_off[g]; q < s->part_off[g + 1]; q++) {
+    poly_t p = {s, q};
+    if (poly_boundary_covers_point(&p, px, py)) return 1;
+  }
+  return 0;
+}
+
 static int polyish_polyish_intersects(const ora_store* A, int64_t a, const ora_store* B, int64_t b) {
   for (int64_t q = A->part_off[a]; q < A->part_off[a + 1]; q++)
     for (int64_t r = B->part_off[b]; r < B->part_off[b + 1]; r++) {
@@ -954,6 +995,8 @@ static int eval_pair(const ora_store* B, int64_t b, const ora_store* P, int64_t 
       case ORA_PRED_CONTAINS: return polyish_point(B, b, px, py, 0);
       case ORA_PRED_WITHIN: return 0;
       case ORA_PRED_COVEREDBY: return point_covers_polyish(B, b, px, py);
+      case ORA_PRED_TOUCHES: return polyish_touches_point(B, b, px, py);
+      case ORA_PRED_EQUALS: return 0; /* tg.c:15299-15301: within && contains; within(poly, point) is false */
       default: return -1;
     }
   }
@@ -965,6 +1008,8 @@ static int eval_pair(const ora_store* B, int64_t b, const ora_store* P, int64_t 
       case ORA_PRED_WITHIN: return polyish_point(P, p, bx, by, 0);
       case ORA_PRED_CONTAINS: return 0;
       case ORA_PRED_COVERS: return point_covers_polyish(P, p, bx, by);
+      case ORA_PRED_TOUCHES: return point_touches_polyish(P, p, bx, by);
+      case ORA_PRED_EQUALS: return 0; /* tg.c:15299-15301: contains(point, poly) is false */
       default: return -1;
     }
   }
@@ -975,6 +1020,7 @@ static int eval_pair(const ora_store* B, int64_t b, const ora_store* P, int64_t 
       case ORA_PRED_INTERSECTS: case ORA_PRED_CONTAINS: case ORA_PRED_WITHIN:
       case ORA_PRED_COVERS: case ORA_PRED_COVEREDBY: case ORA_PRED_EQUALS:
         return feq(bx, px) && feq(by, py);
+      case ORA_PRED_TOUCHES: return 0; /* tg.c:1102-1106: points have no boundary */
       default: return -1;
     }
   }

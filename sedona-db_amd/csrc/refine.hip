@@ -512,6 +512,45 @@ __device__ __forceinline__ bool point_of(const GeomView& g, int64_t row, double&
   return p.x == p.x || p.y == p.y;
 }
 
+// tg_point_touches_poly (tg.c:1171-1186) for one polygon part: the point
+// lies on a segment of the exterior ring or of a hole (point_on_segment,
+// tg.c:1192-1197: the segment's rect covers the point and the three points
+// are collinear).
+__device__ bool part_boundary_covers(const GeomView& g, int64_t part, double px, double py) {
+  for (int64_t r = g.ring_off[part]; r < g.ring_off[part + 1]; r++) {
+    const RingRef ring = ring_at(g, r);
+    const int64_t ns = ring.npts >= 3 ? ring.npts - 1 : 0;
+    for (int64_t j = 0; j < ns; j++) {
+      const double2 a = ring.xy[j], b = ring.xy[j + 1];
+      if (px < fmin0(a.x, b.x) || py < fmin0(a.y, b.y) || px > fmax0(a.x, b.x) || py > fmax0(a.y, b.y)) continue;
+      if (collinear(a.x, a.y, b.x, b.y, px, py)) return true;
+    }
+  }
+  return false;
+}
+
+// tg_geom_touches(polygon or multipolygon row, point): a polygon touches
+// the point on its boundary (tg.c:6682-6690, 3957-3959); a multipolygon
+// when some part does and no part that does not holds the point
+// (tg.c:6735-6763, where a non-touching part that intersects ends it).
+__device__ bool polyish_touches_point(const GeomView& g, int64_t row, double px, double py) {
+  if (g.type[row] == kPolygon) return part_boundary_covers(g, g.part_off[row], px, py);
+  bool touches = false;
+  for (int64_t q = g.part_off[row]; q < g.part_off[row + 1]; q++) {
+    if (part_boundary_covers(g, q, px, py)) touches = true;
+    else if (any_poly_contains(g, q, px, py, true)) return false;
+  }
+  return touches;
+}
+
+// tg_geom_touches(point, polygon or multipolygon row): the point touches
+// some part (tg.c:6578-6604; the other parts' interiors are not tested).
+__device__ bool point_touches_polyish(const GeomView& g, int64_t row, double px, double py) {
+  for (int64_t q = g.part_off[row]; q < g.part_off[row + 1]; q++)
+    if (part_boundary_covers(g, q, px, py)) return true;
+  return false;
+}
+
 // tg_point_covers_poly (tg.c:1150-1154) for every part of a polyish row.
 __device__ bool point_covers_polyish(const GeomView& g, int64_t row, double px, double py) {
   int64_t p0 = g.part_off[row], p1 = g.part_off[row + 1];
@@ -716,6 +755,8 @@ __device__ int eval_relation(const Ctx& c, int64_t b, int64_t p, int pred, int p
       case kContains: allow = false; break;
       case kWithin: return 0;
       case kCoveredBy: return point_covers_polyish(c.B, b, px, py) ? 1 : 0;
+      case kTouches: return polyish_touches_point(c.B, b, px, py) ? 1 : 0;
+      case kEquals: return 0;  // within && contains (tg.c:15299-15301): within(polygon, point) is false
       default: return -1;
     }
     for (int64_t q = c.B.part_off[b]; q < c.B.part_off[b + 1]; q++)
@@ -733,6 +774,8 @@ __device__ int eval_relation(const Ctx& c, int64_t b, int64_t p, int pred, int p
       case kWithin: allow = false; break;
       case kContains: return 0;
       case kCovers: return point_covers_polyish(c.P, p, bx, by) ? 1 : 0;
+      case kTouches: return point_touches_polyish(c.P, p, bx, by) ? 1 : 0;
+      case kEquals: return 0;  // contains(point, polygon) is false
       default: return -1;
     }
     for (int64_t q = c.P.part_off[p]; q < c.P.part_off[p + 1]; q++)
@@ -744,6 +787,7 @@ __device__ int eval_relation(const Ctx& c, int64_t b, int64_t p, int pred, int p
     switch (pred) {
       case kIntersects: case kContains: case kWithin: case kCovers: case kCoveredBy: case kEquals:
         return (feq(bx, px) && feq(by, py)) ? 1 : 0;
+      case kTouches: return 0;  // points have no boundary (tg.c:1102-1106)
       default: return -1;
     }
   }
@@ -1934,11 +1978,8 @@ void check_relation_types(uint32_t bm, uint32_t pm, int pred) {
     case kIntersects:
       ok = only(bm, poly | pt) && only(pm, poly | pt);
       break;
-    case kContains: case kWithin: case kCovers: case kCoveredBy:
+    case kContains: case kWithin: case kCovers: case kCoveredBy: case kTouches: case kEquals:
       ok = only(bm, poly | pt) && only(pm, poly | pt) && !((bm & poly) && (pm & poly));
-      break;
-    case kEquals:
-      ok = only(bm, pt) && only(pm, pt);
       break;
     default:
       ok = false;

@@ -38,11 +38,12 @@ from sjhip import datagen as D  # noqa: E402
 FIXTURE_JOINS = [
     # (name, build file, probe file, predicates)
     ("pip", "synthetic_pip_polygons.parquet", "synthetic_pip_points.parquet",
-     ["intersects", "contains", "covers", "within", "coveredby"]),
+     ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"]),
     ("countries", "countries.parquet", "countries_points.parquet",
-     ["intersects", "contains", "covers", "within", "coveredby"]),
+     ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"]),
     ("polypoly", "synthetic_pip_polygons.parquet", "synthetic_poly_polygons2.parquet", ["intersects"]),
-    ("points_in_countries_rev", "countries_points.parquet", "countries.parquet", ["within", "intersects", "coveredby"]),
+    ("points_in_countries_rev", "countries_points.parquet", "countries.parquet",
+     ["within", "intersects", "coveredby", "touches"]),
 ]
 
 
@@ -81,7 +82,7 @@ def main():
     bi = np.repeat(np.arange(nb), np_).astype(np.uint32)
     pi = np.tile(np.arange(np_), nb).astype(np.uint32)
     tb, tp = O.TgColumn(cb, index=1), O.TgColumn(cp, index=0)
-    for pred in ["intersects", "contains", "covers", "within", "coveredby"]:
+    for pred in ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"]:
         m = O.ref_refine(tb, tp, pred, bi, pi)
         res[f"corpus/{pred}/mask"] = np.packbits(m)
         m2 = O.ref_refine(tp, tb, pred, pi, bi)

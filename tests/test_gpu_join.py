@@ -10,6 +10,7 @@ import pytest
 
 import corpus as C
 import oracle as O
+from touch_cases import touch_cases
 from helpers import expected, pairs_equal, read_fixture
 from sjhip import datagen as D
 from sjhip import gpuspatial as G
@@ -56,10 +57,11 @@ def corpus_arrow(cols):
 
 FIXTURES = [
     ("pip", "synthetic_pip_polygons.parquet", "synthetic_pip_points.parquet",
-     ["intersects", "contains", "covers", "within", "coveredby"]),
+     ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"]),
     ("countries", "countries.parquet", "countries_points.parquet",
-     ["intersects", "contains", "covers", "within", "coveredby"]),
-    ("points_in_countries_rev", "countries_points.parquet", "countries.parquet", ["within", "intersects", "coveredby"]),
+     ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"]),
+    ("points_in_countries_rev", "countries_points.parquet", "countries.parquet",
+     ["within", "intersects", "coveredby", "touches"]),
     ("polypoly", "synthetic_pip_polygons.parquet", "synthetic_poly_polygons2.parquet", ["intersects"]),
 ]
 
@@ -83,6 +85,39 @@ def test_c_abi_join_matches_reference_goldens(name, bf, pf, preds, opts):
         # sorted by (probe, build) like the reference's refiner output
         key = (op.astype(np.uint64) << 32) | ob
         assert np.all(np.diff(key.astype(np.int64)) > 0) or len(key) < 2
+
+
+@pytest.mark.parametrize("pred", ["touches", "equals", "intersects", "covers"])
+def test_touches_polygons_multipolygons_both_roles_vs_oracle(pred, opts):
+    """tg_geom_touches: a polygon's boundary (holes included); a multipolygon
+    (as the first argument) only when no other part holds the point inside;
+    a point (as the first argument) touching any part."""
+    bcol, pcol = touch_cases()
+    B, P = O.Store(bcol), O.Store(pcol)
+    bi = np.repeat(np.arange(B.n), P.n).astype(np.uint32)
+    pi = np.tile(np.arange(P.n), B.n).astype(np.uint32)
+    for roles in ("poly_build", "point_build"):
+        if roles == "poly_build":
+            rf = G.GpuSpatialRefiner(opts)
+            rf.init_build_schema(corpus_arrow(bcol))
+            rf.push_build(corpus_arrow(bcol))
+            rf.finish_building()
+            ob, op = rf.refine(corpus_arrow(pcol), pred, bi, pi)
+            want = O.refine(B, P, pred, bi, pi)
+            exp = (bi[want], pi[want])
+        else:
+            rf = G.GpuSpatialRefiner(opts)
+            rf.init_build_schema(corpus_arrow(pcol))
+            rf.push_build(corpus_arrow(pcol))
+            rf.finish_building()
+            ob, op = rf.refine(corpus_arrow(bcol), pred, pi, bi)
+            want = O.refine(P, B, pred, pi, bi)
+            exp = (pi[want], bi[want])
+        say(pred, roles, int(want.sum()))
+        eq, miss, extra = pairs_equal((ob, op), exp)
+        assert eq, (pred, roles, miss, extra)
+        if pred == "touches":
+            assert want.sum() >= 8
 
 
 def test_filter_matches_oracle_candidates(opts):
@@ -127,7 +162,7 @@ def test_corpus_all_pairs_match_reference(precision, opts):
     rf.init_build_schema(b)
     rf.push_build(b)
     rf.finish_building()
-    for pred in ["intersects", "contains", "covers", "within", "coveredby"]:
+    for pred in ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"]:
         say("corpus", pred)
         want = np.unpackbits(E[f"corpus/{pred}/mask"])[: len(bi)].astype(bool)
         ob, op = rf.refine(p, pred, bi, pi)
@@ -137,7 +172,7 @@ def test_corpus_all_pairs_match_reference(precision, opts):
     rrf.init_build_schema(p)
     rrf.push_build(p)
     rrf.finish_building()
-    for pred in ["intersects", "within", "coveredby"]:
+    for pred in ["intersects", "within", "coveredby", "touches", "equals"]:
         want = np.unpackbits(E[f"corpus_rev/{pred}/mask"])[: len(bi)].astype(bool)
         ob, op = rrf.refine(b, pred, pi, bi)
         eq, miss, extra = pairs_equal((ob, op), (pi[want], bi[want]))

@@ -13,11 +13,12 @@ from helpers import expected, read_fixture
 
 FIXTURES = [
     ("pip", "synthetic_pip_polygons.parquet", "synthetic_pip_points.parquet",
-     ["intersects", "contains", "covers", "within", "coveredby"]),
+     ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"]),
     ("countries", "countries.parquet", "countries_points.parquet",
-     ["intersects", "contains", "covers", "within", "coveredby"]),
+     ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"]),
     ("polypoly", "synthetic_pip_polygons.parquet", "synthetic_poly_polygons2.parquet", ["intersects"]),
-    ("points_in_countries_rev", "countries_points.parquet", "countries.parquet", ["within", "intersects", "coveredby"]),
+    ("points_in_countries_rev", "countries_points.parquet", "countries.parquet",
+     ["within", "intersects", "coveredby", "touches"]),
 ]
 
 
@@ -38,7 +39,7 @@ def test_oracle_matches_reference_on_adversarial_corpus():
     bi = np.repeat(np.arange(nb), np_).astype(np.uint32)
     pi = np.tile(np.arange(np_), nb).astype(np.uint32)
     E = expected()
-    for pred in ["intersects", "contains", "covers", "within", "coveredby"]:
+    for pred in ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"]:
         want = np.unpackbits(E[f"corpus/{pred}/mask"])[: len(bi)].astype(bool)
         assert np.array_equal(O.refine(B, P, pred, bi, pi), want), pred
         want2 = np.unpackbits(E[f"corpus_rev/{pred}/mask"])[: len(bi)].astype(bool)
@@ -91,6 +92,25 @@ def test_oracle_matches_reference_tg_on_fresh_workloads(seed):
         rb, rp = O.ref_join(tb, tp, B.rects(True), P.rects(True), pred)
         ob, op = O.join(B, P, pred)
         assert np.array_equal(ob, rb) and np.array_equal(op, rp)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference TG library not built here")
+@pytest.mark.parametrize("pred", ["touches", "equals", "intersects", "covers", "contains", "within", "coveredby"])
+def test_oracle_touches_multipolygons_match_reference_tg(pred):
+    """The multipolygon rules of tg_geom_touches (as the first argument: no
+    other part may hold the point inside; as the second: any part's
+    boundary), polygons with holes, both roles, against TG itself."""
+    from touch_cases import touch_cases
+    bcol, pcol = touch_cases()
+    B, P = O.Store(bcol), O.Store(pcol)
+    bi = np.repeat(np.arange(B.n), P.n).astype(np.uint32)
+    pi = np.tile(np.arange(P.n), B.n).astype(np.uint32)
+    tb, tp = O.TgColumn(bcol, index=1), O.TgColumn(pcol, index=0)
+    assert np.array_equal(O.refine(B, P, pred, bi, pi), O.ref_refine(tb, tp, pred, bi, pi))
+    assert np.array_equal(O.refine(P, B, pred, pi, bi), O.ref_refine(tp, tb, pred, pi, bi))
+    if pred == "touches":
+        m1, m2 = O.ref_refine(tb, tp, pred, bi, pi), O.ref_refine(tp, tb, pred, pi, bi)
+        assert m1.sum() >= 8 and not np.array_equal(m1, m2)  # the multipolygon asymmetry is exercised
 
 
 def _poly(rings):
