@@ -264,7 +264,10 @@ struct SjhipJoinOptions {
                            them by build row first (polygon-major), 2 =
                            (DWithin, point probes) decide the distance inside
                            the filter walk, writing no candidates; -1 = the
-                           order SjhipAutotune chose (else 1, DWithin 0) */
+                           order SjhipAutotune chose (else 1; DWithin 0
+                           unless the build side is single lines, whose
+                           cell rasters drop candidates before the
+                           grouping) */
   void* stream;        /* hipStream_t, or NULL for a stream of the call */
 };
 

@@ -628,7 +628,9 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
   const int kind = dwithin ? 1 : ranked ? 0 : 2;
   int order_used = o.refine_order;
   if (order_used < 0) order_used = ix->tuning.refine_order[kind];
-  if (order_used < 0) order_used = dwithin ? 0 : 1;
+  // default: polygon-major; DWithin in filter order unless the build side
+  // has line rasters (whose split leaves few enough candidates to sort)
+  if (order_used < 0) order_used = dwithin && rf->ring_index.n_line_rasters == 0 ? 0 : 1;
   // refine_order 2 (DWithin over point probes): filter and distance refine
   // in one kernel, candidates never written out
   const bool fused = dwithin && probe.points_only && o.precision != 1 && order_used == 2;
@@ -647,11 +649,14 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
     mark(3);
     // point in polygon, polygon-major order: candidates in clean raster
     // cells are decided first, so that the regrouping sorts only the rest
+    // (for DWithin over lines, the line rasters drop candidates first)
     uint64_t n_open = 0;
-    const bool split = !dwithin && order_used == 1 &&
-                       raster_split(rf->build, rf->ring_index, rp, o.predicate, o.precision, ws->cand.b.p,
-                                    ws->cand.p.p, nc, ranked ? ord : nullptr, ws->out, ws->open, ws->mask, &n_open,
-                                    s);
+    const bool split =
+        order_used == 1 &&
+        (dwithin ? raster_split_distance(rf->build, rf->ring_index, probe, o.distance, o.precision, ws->cand.b.p,
+                                         ws->cand.p.p, nc, ws->open, ws->mask, &n_open, s)
+                 : raster_split(rf->build, rf->ring_index, rp, o.predicate, o.precision, ws->cand.b.p, ws->cand.p.p,
+                                nc, ranked ? ord : nullptr, ws->out, ws->open, ws->mask, &n_open, s));
     PairBuf& cand = split ? ws->open : ws->cand;
     const uint64_t n_cand = split ? n_open : nc;
     // polygon-major order: one wavefront then walks few distinct rings
@@ -662,7 +667,7 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
     }
     if (dwithin)
       kept = refine_distance(rf->build, rf->ring_index, probe, o.distance, cand.b.p, cand.p.p, n_cand, ws->out, s,
-                             ev.on ? &rt : nullptr, o.precision, &st, ws);
+                             ev.on ? &rt : nullptr, o.precision, &st, ws, split);
     else
       kept = refine_relation(rf->build, rf->ring_index, rp, o.predicate, cand.b.p, cand.p.p, n_cand, o.precision,
                              ws->out, &st, s, ev.on ? &rt : nullptr, ranked ? ord : nullptr, ix->tuning.pip_waves,
@@ -1241,7 +1246,8 @@ int SjhipAutotune(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner*
       decode_wkb(in, probe_types, ss.s);
       kind = pip_refine_applies(rf->build, rf->ring_index, probe_types, o.predicate) ? 0 : 2;
     }
-    o.refine_order = ix->tuning.refine_order[kind] >= 0 ? ix->tuning.refine_order[kind] : (dwithin ? 0 : 1);
+    o.refine_order = ix->tuning.refine_order[kind] >= 0 ? ix->tuning.refine_order[kind]
+                                                        : (dwithin && rf->ring_index.n_line_rasters == 0 ? 0 : 1);
     (void)time_join();  // warm-up: code objects, memory pool, output-size hints
     // 1. rect tree leaf size
     double best = 1e300;

@@ -2055,6 +2055,33 @@ bool raster_split(const GeomColumn& build, const RingIndex& ri, const GeomColumn
   return true;
 }
 
+bool raster_split_distance(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
+                           int precision, const uint32_t* cb, const uint32_t* cp, uint64_t n, PairBuf& open,
+                           DevArray<uint64_t>& mask, uint64_t* n_open, hipStream_t s) {
+  if (precision == 1 || dist_kind(build.type_mask, build.points_only, ri) != kDistLines || ri.n_line_rasters == 0 ||
+      !probe.points_only || !(d >= 0 && d < INFINITY) || n > 0xffffffffull)
+    return false;
+  *n_open = 0;
+  if (n == 0) return true;
+  Ctx c;
+  c.B = build.view();
+  c.R = ri.view();
+  c.P = probe.view();
+  DevArray<unsigned long long> open_count;
+  const unsigned long long* count = nullptr;
+  list_marked(
+      n,
+      [&](unsigned int g, uint64_t* marks, unsigned long long* tile_cnt) {
+        hipLaunchKernelGGL(k_dist_raster_pass, g, kBlock, 0, s, c, d, cb, cp, n, marks, tile_cnt);
+      },
+      cb, cp, open, mask, open_count, &count, s);
+  unsigned long long h = 0;
+  SJ_CHECK(hipMemcpyAsync(&h, count, 8, hipMemcpyDeviceToHost, s));
+  SJ_CHECK(hipStreamSynchronize(s));
+  *n_open = h;
+  return true;
+}
+
 uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, int predicate,
                          const uint32_t* cb, const uint32_t* cp, uint64_t n, int precision, PairBuf& out,
                          RefineStats* stats, hipStream_t s, KernelTimer* timer, const uint32_t* probe_ids,
@@ -2211,7 +2238,7 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
 
 uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
                          const uint32_t* cb, const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s,
-                         KernelTimer* timer, int precision, RefineStats* stats, Workspace* ws) {
+                         KernelTimer* timer, int precision, RefineStats* stats, Workspace* ws, bool raster_done) {
   out.counter.reserve(1, s);
   fit(out.b, n > 0 ? n : 1, s);
   fit(out.p, n > 0 ? n : 1, s);
@@ -2236,7 +2263,7 @@ uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const Geo
   DevArray<uint64_t>& mask = ws ? ws->mask : mask_own;
   DevArray<unsigned long long> open_count;
   const unsigned long long* n_open = nullptr;
-  if (kind == kDistLines && ri.n_line_rasters > 0 && probe.points_only && d >= 0 && d < INFINITY &&
+  if (!raster_done && kind == kDistLines && ri.n_line_rasters > 0 && probe.points_only && d >= 0 && d < INFINITY &&
       n <= 0xffffffffull)
     list_marked(
         n,

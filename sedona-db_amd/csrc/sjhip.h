@@ -393,13 +393,22 @@ bool raster_split(const GeomColumn& build, const RingIndex& ri, const GeomColumn
                   int precision, const uint32_t* cb, const uint32_t* cp, uint64_t n, const uint32_t* probe_ids,
                   PairBuf& out, PairBuf& open, DevArray<uint64_t>& mask, uint64_t* n_open, hipStream_t s);
 
+// The same for ST_DWithin over single-line build rows and point probes
+// (the line rasters only drop candidates): the candidates they cannot rule
+// out are copied, in candidate order, to open, for
+// refine_distance(..., raster_done = true).  Returns false, doing nothing,
+// where the line rasters do not apply.
+bool raster_split_distance(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
+                           int precision, const uint32_t* cb, const uint32_t* cp, uint64_t n, PairBuf& open,
+                           DevArray<uint64_t>& mask, uint64_t* n_open, hipStream_t s);
+
 // Distance refine (build.distance(probe) <= d); probe must be points.
 // precision 1: the all-FP64 form over every segment.  stats: FP64 segment
 // distance tests, and exact ring scans (uncertain certified crossing counts).
 uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
                          const uint32_t* cb, const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s,
                          KernelTimer* timer = nullptr, int precision = 0, RefineStats* stats = nullptr,
-                         Workspace* ws = nullptr);
+                         Workspace* ws = nullptr, bool raster_done = false);
 
 // ST_DWithin join of point probes in one kernel: the filter walk decides
 // the distance predicate for each rect hit, so only kept pairs are
