@@ -660,11 +660,7 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
     PairBuf& cand = split ? ws->open : ws->cand;
     const uint64_t n_cand = split ? n_open : nc;
     // polygon-major order: one wavefront then walks few distinct rings
-    // (height slabs only for the point-in-polygon refine)
-    if (order_used == 1) {
-      const float4* br = !dwithin && ix->rects.n >= (size_t)rf->build.n_rows ? ix->rects.p : nullptr;
-      sort_pairs_by_build(cand, n_cand, rf->build.n_rows, br, rp, s);
-    }
+    if (order_used == 1) sort_pairs_by_build(cand, n_cand, rf->build.n_rows, s);
     if (dwithin)
       kept = refine_distance(rf->build, rf->ring_index, probe, o.distance, cand.b.p, cand.p.p, n_cand, ws->out, s,
                              ev.on ? &rt : nullptr, o.precision, &st, ws, split);

@@ -2429,69 +2429,17 @@ __global__ void k_unpack(const uint64_t* k, uint64_t n, uint32_t* b, uint32_t* p
 }
 }  // namespace
 
-namespace {
-// Key of a candidate: build id in the high bits, the probe point's height
-// within the build rect quantised to ybits in the low bits.
-template <typename K>
-__global__ void k_group_keys(const uint32_t* cb, const uint32_t* cp, uint64_t n, const float4* brect,
-                             const double2* pxy, int ybits, K* keys) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint32_t b = cb[i];
-  float4 r = brect[b];
-  double y = pxy[cp[i]].y;
-  float h = r.w - r.y;
-  float f = h > 0.f ? ((float)y - r.y) / h : 0.f;
-  float q = f * (float)(1u << ybits);
-  uint32_t yq = q <= 0.f || q != q ? 0u : (q >= (float)((1u << ybits) - 1) ? (1u << ybits) - 1 : (uint32_t)q);
-  keys[i] = ((K)b << ybits) | (K)yq;
-}
-
-template <typename K>
-__global__ void k_group_unkey(const K* keys, uint64_t n, int ybits, uint32_t* cb) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) cb[i] = (uint32_t)(keys[i] >> ybits);
-}
-
-template <typename K>
-void group_sort(PairBuf& pb, uint64_t n, int bbits, int ybits, const float4* brect, const double2* pxy,
-                hipStream_t s) {
-  DevArray<K> k1(n, s), k2(n, s);
-  DevArray<uint32_t> vp(n, s);
-  hipLaunchKernelGGL(k_group_keys<K>, blocks_for(n), kBlock, 0, s, pb.b.p, pb.p.p, n, brect, pxy, ybits, k1.p);
-  size_t tmp = 0;
-  const int end = bbits + ybits;
-  SJ_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, k1.p, k2.p, pb.p.p, vp.p, (int)n, 0, end, s));
-  DevArray<uint8_t> t(tmp + 16, s);
-  SJ_CHECK(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, k1.p, k2.p, pb.p.p, vp.p, (int)n, 0, end, s));
-  hipLaunchKernelGGL(k_group_unkey<K>, blocks_for(n), kBlock, 0, s, k2.p, n, ybits, pb.b.p);
-  pb.p = std::move(vp);
-  SJ_CHECK(hipGetLastError());
-}
-}  // namespace
-
-void sort_pairs_by_build(PairBuf& pb, uint64_t n, int64_t n_build, const float4* build_rects,
-                         const GeomColumn& probe, hipStream_t s) {
+void sort_pairs_by_build(PairBuf& pb, uint64_t n, int64_t n_build, hipStream_t s) {
   if (n < 2) return;
   int bbits = 1;
   while (bbits < 32 && (1ll << bbits) < n_build) bbits++;
-  // height slabs only for point probes with build rects available
-  int ybits = (build_rects && probe.points_only) ? 12 : 0;
-  if (bbits + ybits <= 32) {
-    if (ybits == 0) {
-      DevArray<uint32_t> kb(n, s), vp(n, s);
-      size_t tmp = 0;
-      SJ_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, pb.b.p, kb.p, pb.p.p, vp.p, (int)n, 0, bbits, s));
-      DevArray<uint8_t> t(tmp + 16, s);
-      SJ_CHECK(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, pb.b.p, kb.p, pb.p.p, vp.p, (int)n, 0, bbits, s));
-      pb.b = std::move(kb);
-      pb.p = std::move(vp);
-    } else {
-      group_sort<uint32_t>(pb, n, bbits, ybits, build_rects, probe.xy.p, s);
-    }
-  } else {
-    group_sort<uint64_t>(pb, n, bbits, ybits, build_rects, probe.xy.p, s);
-  }
+  DevArray<uint32_t> kb(n, s), vp(n, s);
+  size_t tmp = 0;
+  SJ_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, pb.b.p, kb.p, pb.p.p, vp.p, (int)n, 0, bbits, s));
+  DevArray<uint8_t> t(tmp + 16, s);
+  SJ_CHECK(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, pb.b.p, kb.p, pb.p.p, vp.p, (int)n, 0, bbits, s));
+  pb.b = std::move(kb);
+  pb.p = std::move(vp);
 }
 
 void sort_pairs(uint32_t* b, uint32_t* p, uint64_t n, hipStream_t s) {

@@ -419,11 +419,10 @@ uint64_t filter_refine_distance(const RectTree& t, const float4* q, int64_t nq, 
                                 PairBuf& out, uint64_t* candidates, RefineStats* stats, hipStream_t s,
                                 KernelTimer* timer = nullptr);
 
-// Regroups candidate pairs by build row and, for point probes, by the
-// point's height within the build rect (radix sort), so that a wavefront
-// refines points of one polygon lying in one horizontal slab.
-void sort_pairs_by_build(PairBuf& pb, uint64_t n, int64_t n_build, const float4* build_rects,
-                         const GeomColumn& probe, hipStream_t s);
+// Regroups candidate pairs by build row with a stable radix sort: within a
+// row they keep their order (the probes' Morton order), so that a
+// wavefront refines nearby points of one polygon.
+void sort_pairs_by_build(PairBuf& pb, uint64_t n, int64_t n_build, hipStream_t s);
 
 // Sorts pairs by (probe, build) in place.
 void sort_pairs(uint32_t* b, uint32_t* p, uint64_t n, hipStream_t s);
