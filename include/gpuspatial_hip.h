@@ -289,19 +289,28 @@ struct SjhipTuneResult {
   int32_t pip_waves;    /* resident waves per SIMD of the point-in-polygon refine */
   int32_t evaluations;  /* configurations timed */
   double best_ms;       /* wall time of the chosen configuration on the sample */
+  int32_t seg_leaf;     /* segments per leaf of the build rings' trees (16 or 32) */
 };
 
 /* Autotuner (no counterpart in the reference): times joins of a probe
  * sample against the built index and refiner under the options' predicate
  * and distance, choosing in turn the rect tree's leaf size (the tree is
  * rebuilt), the candidate order before the refine (per kind of refine:
- * point in polygon, distance, other) and the point-in-polygon kernel's
- * resident waves.  The choices stay with the index: joins whose
- * options.refine_order is -1 use the tuned order.  Must not run
- * concurrently with joins on the same index. */
+ * point in polygon, distance, other), the point-in-polygon kernel's
+ * resident waves and, for point-in-polygon samples, the segments per leaf
+ * of the build rings' trees (the ring index is rebuilt).  The choices stay
+ * with the index and refiner: joins whose options.refine_order is -1 use
+ * the tuned order.  Must not run concurrently with joins on the same index
+ * or refiner. */
 int SjhipAutotune(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner* refiner,
                   const struct ArrowSchema* probe_schema, const struct ArrowDeviceArray* probe_sample,
                   const struct SjhipJoinOptions* options, struct SjhipTuneResult* out);
+
+/* Rebuilds the built refiner's ring index with `segments_per_leaf` (16 or
+ * 32) segments per leaf of its rings' segment trees (the choice the
+ * autotuner makes; no counterpart in the reference).  Must not run
+ * concurrently with joins on the refiner. */
+int SjhipRefinerSetSegmentLeaf(struct SedonaSpatialRefiner* refiner, int32_t segments_per_leaf);
 
 /* Request loop over a resident index and refiner (the serving path of the
  * GPU operator: rust/sedona-spatial-join-gpu/src/index/gpu_spatial_index.rs:

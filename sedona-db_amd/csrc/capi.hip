@@ -133,6 +133,7 @@ struct IndexImpl {
   hipStream_t stream = nullptr;
   DevArray<float4> rects;
   RectTree tree;
+  uint64_t tree_gen = 0;  // bumped whenever the tree is rebuilt (captured plans compare it)
   Tuning tuning;
   bool built = false;
   std::mutex mu;
@@ -154,6 +155,7 @@ int ix_clear(SedonaFloatIndex2D* self) {
     use_device(ix->rt);
     ix->host_rects.clear();
     ix->tree = RectTree();
+    ix->tree_gen++;
     ix->rects.release();
     ix->built = false;
   });
@@ -182,6 +184,7 @@ void index_finish(IndexImpl* ix) {
     SJ_CHECK(hipMemcpyAsync(ix->rects.p, ix->host_rects.data(), n * sizeof(float4), hipMemcpyHostToDevice,
                             ix->stream));
   build_rect_tree(ix->rects.p, n, ix->tree, ix->stream);
+  ix->tree_gen++;
   SJ_CHECK(hipStreamSynchronize(ix->stream));
   ix->built = true;
 }
@@ -240,6 +243,7 @@ void ix_release(SedonaFloatIndex2D* self) {
   try {
     use_device(ix->rt);
     ix->tree = RectTree();
+    ix->tree_gen++;
     ix->rects.release();
     if (ix->stream) {
       (void)hipStreamSynchronize(ix->stream);
@@ -271,6 +275,7 @@ struct RefinerImpl {
   // join running at a time, kept between joins
   std::mutex ws_mu;
   std::vector<std::unique_ptr<Workspace>> ws_free;
+  uint64_t ring_gen = 0;  // bumped whenever the ring index is rebuilt (captured plans compare it)
 };
 
 // A join's workspace: taken from the refiner's free list (or new) for the
@@ -313,6 +318,7 @@ int rf_clear(SedonaSpatialRefiner* self) {
     rf->batches.clear();
     rf->build = GeomColumn();
     rf->ring_index = RingIndex();
+    rf->ring_gen++;
     rf->knn = KnnIndex();
     rf->built = false;
   });
@@ -447,6 +453,7 @@ void refiner_finish(RefinerImpl* rf) {
   RingIndexParams prm;
   rf->knn = KnnIndex();
   build_ring_index(rf->build, rf->ring_index, prm, rf->stream);
+  rf->ring_gen++;
   SJ_CHECK(hipStreamSynchronize(rf->stream));
   rf->built = true;
 }
@@ -519,6 +526,7 @@ void rf_release(SedonaSpatialRefiner* self) {
     rf->batches.clear();
     rf->build = GeomColumn();
     rf->ring_index = RingIndex();
+    rf->ring_gen++;
     rf->knn = KnnIndex();
     if (rf->stream) {
       (void)hipStreamSynchronize(rf->stream);
@@ -730,7 +738,8 @@ struct GraphPlan {
   double distance = 0;
   int pip_waves = 8;
   int leaf = 8;
-  const float4* tree_nodes = nullptr;  // the index tree the plan was captured over
+  uint64_t tree_gen = 0;  // generations of the index tree and of the ring index the plan was captured over
+  uint64_t ring_gen = 0;
   int device_id = 0;
   int64_t B = 0;
   uint64_t C = 0;
@@ -764,7 +773,8 @@ std::unique_ptr<GraphPlan> capture_plan(IndexImpl* ix, RefinerImpl* rf, int kind
   g->distance = o.distance;
   g->pip_waves = ix->tuning.pip_waves;
   g->leaf = ix->tree.leaf;
-  g->tree_nodes = ix->tree.nodes.p;
+  g->tree_gen = ix->tree_gen;
+  g->ring_gen = rf->ring_gen;
   g->device_id = rf->rt->device_id;
   g->B = B;
   g->C = C;
@@ -952,7 +962,8 @@ struct Server {
       auto key = std::make_tuple(kind, (int)o.predicate, B);
       auto& plan = plans[key];
       if (plan && (plan->distance != o.distance || plan->leaf != ix->tree.leaf ||
-                   plan->tree_nodes != ix->tree.nodes.p || plan->pip_waves != ix->tuning.pip_waves))
+                   plan->tree_gen != ix->tree_gen || plan->ring_gen != rf->ring_gen ||
+                   plan->pip_waves != ix->tuning.pip_waves))
         plan.reset();
       if (!plan) {
         uint64_t per16 = ix->tree.cand_per_query_x16.load(std::memory_order_relaxed);
@@ -1188,6 +1199,25 @@ int SjhipJoinDevice(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefine
   });
 }
 
+int SjhipRefinerSetSegmentLeaf(struct SedonaSpatialRefiner* refiner, int32_t segments_per_leaf) {
+  return guarded(nullptr, [&] {
+    if (!refiner || !refiner->private_data) throw Error("null refiner");
+    RefinerImpl* rf = (RefinerImpl*)refiner->private_data;
+    if (!rf->built) throw Error("refiner must be built");
+    if (segments_per_leaf != 16 && segments_per_leaf != 32) throw Error("segments_per_leaf must be 16 or 32");
+    use_device(rf->rt);
+    std::lock_guard<std::mutex> g(rf->mu);
+    RingIndexParams prm;
+    prm.seg_leaf_shift = segments_per_leaf == 32 ? 5 : 4;
+    if (rf->ring_index.ring_rect.p == nullptr || rf->ring_index.seg_leaf_shift == prm.seg_leaf_shift) return;
+    RingIndex ri;
+    build_ring_index(rf->build, ri, prm, rf->stream);
+    SJ_CHECK(hipStreamSynchronize(rf->stream));
+    rf->ring_index = std::move(ri);
+    rf->ring_gen++;
+  });
+}
+
 int SjhipAutotune(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner* refiner,
                   const struct ArrowSchema* probe_schema, const struct ArrowDeviceArray* probe_sample,
                   const struct SjhipJoinOptions* options, struct SjhipTuneResult* out) {
@@ -1232,6 +1262,7 @@ int SjhipAutotune(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner*
       t.cand_per_query_x16.store(ix->tree.cand_per_query_x16.load());
       t.kept_per_query_x16.store(ix->tree.kept_per_query_x16.load());
       ix->tree = std::move(t);
+      ix->tree_gen++;
     };
     // the kind of refine this probe takes (as JoinEngine::run decides); the
     // decoded column is released before its stream
@@ -1277,6 +1308,31 @@ int SjhipAutotune(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner*
       }
       ix->tuning.pip_waves = best_w;
     }
+    // 4. segments per leaf of the build rings' trees (16 or 32; the ring
+    //    index is rebuilt for the other size and kept when faster).  The
+    //    trees serve every refine against this build side; the point-in-
+    //    polygon walk is the one that prefers larger leaves.
+    if (kind == 0) {
+      best = time_join();
+      for (int ls : {4, 5}) {
+        if (ls == rf->ring_index.seg_leaf_shift) continue;
+        RingIndexParams prm;
+        prm.seg_leaf_shift = ls;
+        RingIndex other;
+        build_ring_index(rf->build, other, prm, ix->stream);
+        SJ_CHECK(hipStreamSynchronize(ix->stream));
+        other.detach_streams();
+        std::swap(rf->ring_index, other);
+        rf->ring_gen++;
+        double ms = time_join();
+        if (ms < best) {
+          best = ms;
+        } else {
+          std::swap(rf->ring_index, other);
+          rf->ring_gen++;
+        }
+      }
+    }
     // give back the pool memory the sample joins left cached, so that the
     // first full-size joins do not find it fragmented
     SJ_CHECK(hipStreamSynchronize(ix->stream));
@@ -1286,6 +1342,7 @@ int SjhipAutotune(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner*
     out->leaf = ix->tree.leaf;
     out->refine_order = best_order;
     out->pip_waves = ix->tuning.pip_waves;
+    out->seg_leaf = 1 << rf->ring_index.seg_leaf_shift;
     out->evaluations = evals;
     out->best_ms = best;
   });
@@ -1493,6 +1550,7 @@ int SjhipIndexBuildFromRefiner(struct SedonaFloatIndex2D* index, struct SedonaSp
     use_device(ix->rt);
     compute_rects(rf->build, &rf->ring_index, true, NAN, ix->rects, ix->stream);
     build_rect_tree(ix->rects.p, rf->build.n_rows, ix->tree, ix->stream);
+    ix->tree_gen++;
     SJ_CHECK(hipStreamSynchronize(ix->stream));
     ix->host_rects.clear();
     ix->built = true;

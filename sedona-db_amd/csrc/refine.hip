@@ -163,9 +163,9 @@ struct TreeShape {
 
 __host__ __device__ __forceinline__ int up_level(int c) { return (c + kSegFanout - 1) / kSegFanout; }
 
-__host__ __device__ __forceinline__ TreeShape seg_tree_shape(int ns) {
+__host__ __device__ __forceinline__ TreeShape seg_tree_shape(int ns, int ls) {
   TreeShape t;
-  t.c0 = (ns + kSegLeaf - 1) / kSegLeaf;
+  t.c0 = (ns + (1 << ls) - 1) >> ls;
   t.c1 = t.c2 = t.c3 = t.c4 = 0;
   t.levels = 1;
   if (t.c0 > kSegFanout) {
@@ -196,6 +196,7 @@ struct TreeWalk {
   float E;
   double px, py;
   int ns;
+  int ls;  // segments per leaf: 1 << ls
   int on, in;
   uint32_t tests, fallbacks;
 };
@@ -218,8 +219,8 @@ constexpr int kLeafBatch = SJ_LEAF_BATCH;
 // A leaf's (up to) 16 segments, walking the vertex run once.
 template <int PREC>
 __device__ __forceinline__ void scan_leaf(TreeWalk& w, int leaf) {
-  const int s0 = leaf * kSegLeaf;
-  const int s1 = min(s0 + kSegLeaf, w.ns);
+  const int s0 = leaf << w.ls;
+  const int s1 = min(s0 + (1 << w.ls), w.ns);
   if constexpr (PREC != 1) {
     // the vertices of kLeafBatch segments are requested before the first
     // test, so that their latencies overlap (the walk is latency-bound)
@@ -291,8 +292,8 @@ __device__ __forceinline__ int level_off(const TreeShape& t, int l) {
 
 // Offset (in boxes) and node count of level l of the tree over ns
 // segments, recomputed instead of held in registers.
-__device__ __forceinline__ void level_geom(int ns, int l, int& off, int& cnt) {
-  int c = (ns + kSegLeaf - 1) / kSegLeaf, o = 0;
+__device__ __forceinline__ void level_geom(int ns, int ls, int l, int& off, int& cnt) {
+  int c = (ns + (1 << ls) - 1) >> ls, o = 0;
   for (int k = 0; k < l; k++) {
     o += c;
     c = (c + kSegFanout - 1) / kSegFanout;
@@ -309,7 +310,7 @@ template <int PREC>
 __device__ __forceinline__ void walk_tree(TreeWalk& w) {
   const int top = w.levels - 1;
   int otop, ntop;
-  level_geom(w.ns, top, otop, ntop);
+  level_geom(w.ns, w.ls, top, otop, ntop);
   const float4* btop = w.box + otop;
   for (int c0 = 0; c0 < ntop; c0 += kSegFanout) {
     uint64_t masks = (uint64_t)children_mask(w, btop + c0, min(kSegFanout, ntop - c0)) << (8 * top);
@@ -330,7 +331,7 @@ __device__ __forceinline__ void walk_tree(TreeWalk& w) {
       }
       int cb = node * kSegFanout;
       int loff, lcnt;
-      level_geom(w.ns, lvl - 1, loff, lcnt);
+      level_geom(w.ns, w.ls, lvl - 1, loff, lcnt);
       uint32_t cm = children_mask(w, w.box + loff + cb, min(kSegFanout, lcnt - cb));
       if (cm) {
         lvl--;
@@ -390,7 +391,7 @@ __device__ __forceinline__ bool walk_any(const float4* box, const TreeShape& sha
 // and FP32 ring-local q32, both starting at the ring's first vertex), FP32
 // frame (origin, bound E) and segment tree (nullptr: none).
 template <int PREC>
-__device__ __forceinline__ bool ring_contains_run(const double2* xy, const float2* q32, const float4* tree,
+__device__ __forceinline__ bool ring_contains_run(const double2* xy, const float2* q32, const float4* tree, int ls,
                                                   int64_t npts, const double4& rb, double2 o, float E, double px,
                                                   double py, bool allow_on_edge, uint64_t& tests,
                                                   uint64_t& fallbacks) {
@@ -405,7 +406,7 @@ __device__ __forceinline__ bool ring_contains_run(const double2* xy, const float
   if (tree) {
     TreeWalk w;
     w.box = tree;
-    w.levels = seg_tree_shape((int)ns).levels;
+    w.levels = seg_tree_shape((int)ns, ls).levels;
     w.pxl = pxl;
     w.pyl = pyl;
     w.m = 2.0f * E;
@@ -415,6 +416,7 @@ __device__ __forceinline__ bool ring_contains_run(const double2* xy, const float
     w.px = px;
     w.py = py;
     w.ns = (int)ns;
+    w.ls = ls;
     w.on = 0;
     w.in = 0;
     w.tests = 0;
@@ -438,8 +440,8 @@ template <int PREC>
 __device__ __forceinline__ bool ring_contains_at(const Ctx& c, const double4& rb, int64_t v0, int64_t npts, double2 o,
                                                  float E, int64_t toff, double px, double py, bool allow_on_edge,
                                                  uint64_t& tests, uint64_t& fallbacks) {
-  return ring_contains_run<PREC>(c.B.xy + v0, c.R.xy32 + v0, toff >= 0 ? c.R.seg_tree + toff : nullptr, npts, rb, o,
-                                 E, px, py, allow_on_edge, tests, fallbacks);
+  return ring_contains_run<PREC>(c.B.xy + v0, c.R.xy32 + v0, toff >= 0 ? c.R.seg_tree + toff : nullptr,
+                                 c.R.seg_leaf_shift, npts, rb, o, E, px, py, allow_on_edge, tests, fallbacks);
 }
 
 template <int PREC>
@@ -613,6 +615,7 @@ struct IRing {
   const double2* xy;
   const float2* q32;
   const float4* tree;  // nullptr: none
+  int ls;              // segments per tree leaf: 1 << ls
   int64_t npts;
   double4 rect;
   double2 o;
@@ -627,6 +630,7 @@ __device__ __forceinline__ IRing iring(const GeomView& g, const RingIndexView& R
   r.npts = g.pt_off[ring + 1] - v0;
   const int64_t toff = R.seg_tree_off ? R.seg_tree_off[ring] : -1;
   r.tree = toff >= 0 ? R.seg_tree + toff : nullptr;
+  r.ls = R.seg_leaf_shift;
   r.rect = R.ring_rect[ring];
   r.o = R.origin[ring];
   r.E = R.err[ring];
@@ -637,8 +641,8 @@ __device__ __forceinline__ IRing iring(const GeomView& g, const RingIndexView& R
 // reference's segment x segment semantics (its exact bounding-box test
 // comes first, so the tree's boxes, within 2E of the exact ones, prune
 // conservatively with a margin of 4E).
-__device__ __forceinline__ bool ring_two_crossings(const double2* xy, int64_t npts, const float4* tree, float E,
-                                                   double2 o, double ax, double ay, double bx, double by) {
+__device__ __forceinline__ bool ring_two_crossings(const double2* xy, int64_t npts, const float4* tree, int ls,
+                                                   float E, double2 o, double ax, double ay, double bx, double by) {
   const int64_t ns = npts >= 3 ? npts - 1 : 0;
   int count = 0;
   if (!tree || !(E < INFINITY)) {
@@ -653,8 +657,8 @@ __device__ __forceinline__ bool ring_two_crossings(const double2* xy, int64_t np
   const float y0 = (float)(fmin(ay, by) - o.y) - m, y1 = (float)(fmax(ay, by) - o.y) + m;
   const int nsi = (int)ns;
   auto pass = [=](const float4& b) { return b.x <= x1 && b.z >= x0 && b.y <= y1 && b.w >= y0; };
-  auto leaf = [&count, xy, nsi, ax, ay, bx, by](int lf) {
-    const int s0 = lf * kSegLeaf, s1 = min(s0 + kSegLeaf, nsi);
+  auto leaf = [&count, xy, nsi, ls, ax, ay, bx, by](int lf) {
+    const int s0 = lf << ls, s1 = min(s0 + (1 << ls), nsi);
     for (int j = s0; j < s1; j++) {
       double2 c = xy[j], d = xy[j + 1];
       if (segment_intersects(ax, ay, bx, by, c.x, c.y, d.x, d.y)) {
@@ -664,7 +668,7 @@ __device__ __forceinline__ bool ring_two_crossings(const double2* xy, int64_t np
     }
     return false;
   };
-  return walk_any(tree, seg_tree_shape(nsi), pass, leaf);
+  return walk_any(tree, seg_tree_shape(nsi, ls), pass, leaf);
 }
 
 // tg_ring_intersects_ring (tg.c:3166-3190) over indexed rings: some
@@ -687,12 +691,12 @@ __device__ bool iring_intersects_iring(IRing ring, IRing other) {
     double2 p = other.xy[i], q = other.xy[i + 1];
     double sx0 = fmin0(p.x, q.x), sy0 = fmin0(p.y, q.y), sx1 = fmax0(p.x, q.x), sy1 = fmax0(p.y, q.y);
     if (a.x > sx1 || a.z < sx0 || a.y > sy1 || a.w < sy0) continue;
-    if (ring_contains_run<0>(ring.xy, ring.q32, ring.tree, ring.npts, a, ring.o, ring.E, p.x, p.y, true, tests,
+    if (ring_contains_run<0>(ring.xy, ring.q32, ring.tree, ring.ls, ring.npts, a, ring.o, ring.E, p.x, p.y, true, tests,
                              fallbacks) ||
-        ring_contains_run<0>(ring.xy, ring.q32, ring.tree, ring.npts, a, ring.o, ring.E, q.x, q.y, true, tests,
+        ring_contains_run<0>(ring.xy, ring.q32, ring.tree, ring.ls, ring.npts, a, ring.o, ring.E, q.x, q.y, true, tests,
                              fallbacks))
       return true;
-    if (ring_two_crossings(ring.xy, ring.npts, ring.tree, ring.E, ring.o, p.x, p.y, q.x, q.y)) return true;
+    if (ring_two_crossings(ring.xy, ring.npts, ring.tree, ring.ls, ring.E, ring.o, p.x, p.y, q.x, q.y)) return true;
   }
   return false;
 }
@@ -920,11 +924,11 @@ __device__ bool wave_iring_intersects(IRing ring, IRing other, int lane) {
       double2 p = other.xy[i], q = other.xy[i + 1];
       double sx0 = fmin0(p.x, q.x), sy0 = fmin0(p.y, q.y), sx1 = fmax0(p.x, q.x), sy1 = fmax0(p.y, q.y);
       if (!(a.x > sx1 || a.z < sx0 || a.y > sy1 || a.w < sy0))
-        hit = ring_contains_run<0>(ring.xy, ring.q32, ring.tree, ring.npts, a, ring.o, ring.E, p.x, p.y, true, tests,
+        hit = ring_contains_run<0>(ring.xy, ring.q32, ring.tree, ring.ls, ring.npts, a, ring.o, ring.E, p.x, p.y, true, tests,
                                    fallbacks) ||
-              ring_contains_run<0>(ring.xy, ring.q32, ring.tree, ring.npts, a, ring.o, ring.E, q.x, q.y, true, tests,
+              ring_contains_run<0>(ring.xy, ring.q32, ring.tree, ring.ls, ring.npts, a, ring.o, ring.E, q.x, q.y, true, tests,
                                    fallbacks) ||
-              ring_two_crossings(ring.xy, ring.npts, ring.tree, ring.E, ring.o, p.x, p.y, q.x, q.y);
+              ring_two_crossings(ring.xy, ring.npts, ring.tree, ring.ls, ring.E, ring.o, p.x, p.y, q.x, q.y);
     }
     if (__ballot(hit)) return true;
   }
@@ -1415,6 +1419,7 @@ __device__ bool ring_any_within_at(const Ctx& c, int64_t v0, int64_t npts, int64
                                    double px, double py, double d, bool check_on, uint64_t& tests) {
   const double2* xy = c.B.xy + v0;
   const int64_t ns = npts - 1;
+  const int ls = c.R.seg_leaf_shift;
   const double dm = E < INFINITY ? d * 1.0001 + 4.0 * (double)E : INFINITY;
   if (toff < 0 || ns < 1 || !(E < INFINITY)) {
     for (int64_t j = 0; j < ns; j++) {
@@ -1429,14 +1434,14 @@ __device__ bool ring_any_within_at(const Ctx& c, int64_t v0, int64_t npts, int64
   const int nsi = (int)ns;
   auto pass = [&](const float4& b) { return b.x <= qx1 && b.z >= qx0 && b.y <= qy1 && b.w >= qy0; };
   auto leaf = [&](int lf) {
-    const int s0 = lf * kSegLeaf, s1 = min(s0 + kSegLeaf, nsi);
+    const int s0 = lf << ls, s1 = min(s0 + (1 << ls), nsi);
     for (int j = s0; j < s1; j++) {
       tests++;
       if (segment_within(xy, j, px, py, d, dm, check_on)) return true;
     }
     return false;
   };
-  return walk_any(c.R.seg_tree + toff, seg_tree_shape(nsi), pass, leaf);
+  return walk_any(c.R.seg_tree + toff, seg_tree_shape(nsi, ls), pass, leaf);
 }
 
 __device__ __forceinline__ bool ring_any_within(const Ctx& c, int64_t ring, double px, double py, double d,
@@ -1655,13 +1660,13 @@ __global__ void k_local32(GeomView g, int64_t n_rings, const double2* origin, fl
 }
 
 // Boxes per ring (0 for rings scanned whole).
-__global__ void k_seg_counts(GeomView g, int64_t n_rings, int min_segs, int64_t* cnt) {
+__global__ void k_seg_counts(GeomView g, int64_t n_rings, int min_segs, int ls, int64_t* cnt) {
   int64_t ring = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (ring >= n_rings) return;
   int64_t npts = g.pt_off[ring + 1] - g.pt_off[ring];
   int64_t ns = npts >= 3 ? npts - 1 : 0;
   int64_t total = 0;
-  if (ns >= min_segs && ns < (1ll << 30)) total = seg_tree_shape((int)ns).total();
+  if (ns >= min_segs && ns < (1ll << 30)) total = seg_tree_shape((int)ns, ls).total();
   cnt[ring] = total;
 }
 
@@ -1672,7 +1677,8 @@ __global__ void k_seg_offsets(const int64_t* cnt, const int64_t* scan, int64_t n
 }
 
 // Level 0: one wavefront per ring, lanes over leaves.
-__global__ void k_seg_leaves(GeomView g, int64_t n_rings, const float2* xy32, const int64_t* off, float4* tree) {
+__global__ void k_seg_leaves(GeomView g, int64_t n_rings, const float2* xy32, const int64_t* off, int ls,
+                             float4* tree) {
   int64_t ring = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   int lane = threadIdx.x & 63;
   if (ring >= n_rings) return;
@@ -1680,9 +1686,9 @@ __global__ void k_seg_leaves(GeomView g, int64_t n_rings, const float2* xy32, co
   if (o < 0) return;
   int64_t v0 = g.pt_off[ring];
   int64_t ns = g.pt_off[ring + 1] - v0 - 1;
-  int64_t nleaf = (ns + kSegLeaf - 1) / kSegLeaf;
+  int64_t nleaf = (ns + (1 << ls) - 1) >> ls;
   for (int64_t k = lane; k < nleaf; k += 64) {
-    int64_t a = k * kSegLeaf, e = min(a + kSegLeaf, ns);  // vertices a..e
+    int64_t a = k << ls, e = min(a + (1 << ls), ns);  // vertices a..e
     float2 p = xy32[v0 + a];
     float4 b = make_float4(p.x, p.y, p.x, p.y);
     for (int64_t v = a + 1; v <= e; v++) {
@@ -1695,13 +1701,13 @@ __global__ void k_seg_leaves(GeomView g, int64_t n_rings, const float2* xy32, co
 }
 
 // Upper levels: one thread per ring.
-__global__ void k_seg_upper(GeomView g, int64_t n_rings, const int64_t* off, float4* tree) {
+__global__ void k_seg_upper(GeomView g, int64_t n_rings, const int64_t* off, int ls, float4* tree) {
   int64_t ring = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (ring >= n_rings) return;
   int64_t o = off[ring];
   if (o < 0) return;
   int64_t ns = g.pt_off[ring + 1] - g.pt_off[ring] - 1;
-  TreeShape sh = seg_tree_shape((int)ns);
+  TreeShape sh = seg_tree_shape((int)ns, ls);
   int64_t c[kSegMaxLevels] = {sh.c0, sh.c1, sh.c2, sh.c3, sh.c4};
   int64_t lo[kSegMaxLevels] = {sh.off<0>(), sh.off<1>(), sh.off<2>(), sh.off<3>(), sh.off<4>()};
   int L = sh.levels;
@@ -1816,7 +1822,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_build(Ctx c, PolyDesc* desc, 
       const int cell = lane * 16 + k;
       if ((touch[cell >> 5] >> (cell & 31)) & 1u) continue;
       const double px = rb.x + ((cell % kRasterCells) + 0.5) * cw, py = rb.y + ((cell / kRasterCells) + 0.5) * ch;
-      const bool in = ring_contains_run<0>(xy, q32, tree, d.npts, rb, o, d.err, px, py, true, t, f);
+      const bool in = ring_contains_run<0>(xy, q32, tree, c.R.seg_leaf_shift, d.npts, rb, o, d.err, px, py, true, t, f);
       word |= (in ? 1u : 2u) << (2 * k);
     }
     raster[roff[row] * kRasterWords + lane] = word;
@@ -1900,8 +1906,10 @@ void build_ring_index(const GeomColumn& g, RingIndex& ri, const RingIndexParams&
   hipLaunchKernelGGL(k_local32, blocks_for(threads), kBlock, 0, s, g.view(), g.n_rings, ri.origin.p, ri.xy32.p);
   // segment trees
   DevArray<int64_t> cnt(g.n_rings, s);
+  if (prm.seg_leaf_shift < 3 || prm.seg_leaf_shift > 6) throw Error("segment tree leaf shift must be 3..6");
+  ri.seg_leaf_shift = prm.seg_leaf_shift;
   hipLaunchKernelGGL(k_seg_counts, blocks_for(g.n_rings), kBlock, 0, s, g.view(), g.n_rings, prm.min_segs_for_tree,
-                     cnt.p);
+                     ri.seg_leaf_shift, cnt.p);
   DevArray<int64_t> scan(g.n_rings + 1, s);
   size_t tmp = 0;
   SJ_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, cnt.p, scan.p, (int)g.n_rings, s));
@@ -1918,9 +1926,9 @@ void build_ring_index(const GeomColumn& g, RingIndex& ri, const RingIndexParams&
   ri.seg_tree.alloc(ri.n_boxes > 0 ? ri.n_boxes : 1, s);
   if (ri.n_boxes > 0) {
     hipLaunchKernelGGL(k_seg_leaves, blocks_for(threads), kBlock, 0, s, g.view(), g.n_rings, ri.xy32.p,
-                       ri.seg_tree_off.p, ri.seg_tree.p);
+                       ri.seg_tree_off.p, ri.seg_leaf_shift, ri.seg_tree.p);
     hipLaunchKernelGGL(k_seg_upper, blocks_for(g.n_rings), kBlock, 0, s, g.view(), g.n_rings, ri.seg_tree_off.p,
-                       ri.seg_tree.p);
+                       ri.seg_leaf_shift, ri.seg_tree.p);
   }
   ri.desc.alloc(g.n_rows > 0 ? g.n_rows : 1, s);
   if (g.n_rows > 0) hipLaunchKernelGGL(k_poly_desc, blocks_for(g.n_rows), kBlock, 0, s, g.view(), ri.view(), ri.desc.p);

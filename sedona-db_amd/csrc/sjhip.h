@@ -126,9 +126,10 @@ struct GeomView {
 };
 
 // Segment tree of one ring: level 0 holds one FP32 ring-local box per
-// kSegLeaf consecutive segments, level l+1 one box per kSegFanout nodes of
-// level l, up to a top level of at most kSegFanout nodes.
-constexpr int kSegLeaf = 16;
+// 2^seg_leaf_shift consecutive segments (an index-wide choice, 16 unless
+// the autotuner picks 32), level l+1 one box per kSegFanout nodes of level
+// l, up to a top level of at most kSegFanout nodes.
+constexpr int kSegLeafShift = 4;
 constexpr int kSegFanout = 8;
 constexpr int kSegMaxLevels = 5;
 
@@ -171,6 +172,7 @@ struct RingIndexView {
   const PolyDesc* desc = nullptr;         // per build row
   const uint32_t* raster = nullptr;       // cell rasters (see PolyDesc::raster_off)
   const uint64_t* line_raster = nullptr;  // line cell rasters (see PolyDesc::raster_off)
+  int seg_leaf_shift = kSegLeafShift;     // segments per tree leaf: 1 << seg_leaf_shift
 };
 
 // Owned geometry column on the device.
@@ -207,6 +209,14 @@ struct RingIndex {
   int64_t n_boxes = 0;
   int64_t n_rasters = 0;
   int64_t n_line_rasters = 0;
+  int seg_leaf_shift = kSegLeafShift;
+  // For an index built on a stream that may end before it (after that
+  // stream has been synchronised): its buffers are then freed on the
+  // detached stream.
+  void detach_streams() {
+    hipStream_t d = detached_stream();
+    ring_rect.s = xy32.s = origin.s = err.s = seg_tree_off.s = seg_tree.s = desc.s = raster.s = line_raster.s = d;
+  }
   RingIndexView view() const {
     RingIndexView v;
     v.ring_rect = ring_rect.p;
@@ -218,6 +228,7 @@ struct RingIndex {
     v.desc = desc.p;
     v.raster = raster.p;
     v.line_raster = line_raster.p;
+    v.seg_leaf_shift = seg_leaf_shift;
     return v;
   }
 };
@@ -254,6 +265,7 @@ void compute_rects(const GeomColumn& g, const RingIndex* ri, bool point_nearest,
 struct RingIndexParams {
   int min_segs_for_tree = 32;  // rings with fewer segments are scanned whole
   bool raster = true;          // cell rasters for single-ring polygon rows with trees
+  int seg_leaf_shift = kSegLeafShift;  // segments per tree leaf: 1 << seg_leaf_shift (4 or 5)
 };
 void build_ring_index(const GeomColumn& g, RingIndex& ri, const RingIndexParams& p, hipStream_t s);
 

@@ -176,7 +176,7 @@ class SjhipJoinOptions(ctypes.Structure):
 
 class SjhipTuneResult(ctypes.Structure):
     _fields_ = [("leaf", ctypes.c_int32), ("refine_order", ctypes.c_int32), ("pip_waves", ctypes.c_int32),
-                ("evaluations", ctypes.c_int32), ("best_ms", ctypes.c_double)]
+                ("evaluations", ctypes.c_int32), ("best_ms", ctypes.c_double), ("seg_leaf", ctypes.c_int32)]
 
 
 class SjhipServerConfig(ctypes.Structure):
@@ -196,7 +196,7 @@ EXPORTED = [
     "SjhipComputeRects", "SjhipDeviceAlloc", "SjhipDeviceFree", "SjhipCopyToDevice", "SjhipCopyToHost",
     "SjhipSetDevice", "SjhipCopyKernel",
     "SjhipServerCreate", "SjhipServerSubmit", "SjhipServerWait", "SjhipServerDestroy", "SjhipRefinerKnnDevice",
-    "SjhipAutotune",
+    "SjhipAutotune", "SjhipRefinerSetSegmentLeaf",
     "SjhipHostRaycast", "SjhipHostRaycastF32", "SjhipHostCollinear", "SjhipHostSegmentIntersects",
     "SjhipHostOrient2d", "SjhipHostHypot", "SjhipHostCoordToLine", "SjhipHostRaycastBatch", "SjhipHostHypotBatch",
 ]
@@ -232,6 +232,7 @@ def lib():
         L.SjhipServerDestroy.argtypes = [ctypes.c_void_p]
         L.SjhipAutotune.argtypes = [_IX, _RF, _SCH, ctypes.POINTER(ArrowDeviceArray), ctypes.POINTER(SjhipJoinOptions),
                                     ctypes.POINTER(SjhipTuneResult)]
+        L.SjhipRefinerSetSegmentLeaf.argtypes = [_RF, ctypes.c_int32]
         L.SjhipRefinerKnnDevice.argtypes = [_RF, _SCH, ctypes.POINTER(ArrowDeviceArray), ctypes.c_uint32,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.SjhipIndexBuildFromRefiner.argtypes = [_IX, _RF]

@@ -387,6 +387,13 @@ class GpuSpatialRefiner:
     def finish_building(self):
         _check(self.rf.finish_building(ctypes.byref(self.rf)), self._err)
 
+    def set_segment_leaf(self, segments_per_leaf):
+        """Rebuilds the ring index with 16 or 32 segments per tree leaf."""
+        lib = self.lib
+        rc = lib.SjhipRefinerSetSegmentLeaf(ctypes.byref(self.rf), int(segments_per_leaf))
+        if rc:
+            raise GpuSpatialError(lib.SjhipLastError().decode())
+
     def refine(self, probe_array, predicate, build_indices, probe_indices):
         """Filters the candidate pairs in place (libgpuspatial.rs:453-486):
         returns the kept (build, probe) arrays sorted by (probe, build)."""
@@ -534,7 +541,7 @@ def autotune(index, refiner, probe_sample, predicate, distance=0.0, precision=0)
     if rc:
         raise GpuSpatialError(L.lib().SjhipLastError().decode())
     return {"leaf": out.leaf, "refine_order": out.refine_order, "pip_waves": out.pip_waves,
-            "evaluations": out.evaluations, "best_ms": out.best_ms}
+            "seg_leaf": out.seg_leaf, "evaluations": out.evaluations, "best_ms": out.best_ms}
 
 
 class GpuSpatialServer:

@@ -384,6 +384,48 @@ def test_autotuned_join_gives_the_same_pairs(pred, d, opts):
     assert np.array_equal(gb, ob) and np.array_equal(gp, op), (len(gb), len(ob))
 
 
+def test_segment_leaf_32_gives_the_oracle_pairs(opts):
+    """Every walk over the build rings' segment trees with 32 segments per
+    leaf (the autotuner's other choice): point in polygon (contains,
+    intersects, the certified and the FP64 path), distance to polygons,
+    polygon x polygon, and the adversarial corpus."""
+    b, p = D.workload_pip_zipf(100_000, 1_000, seed=11)
+    ix, rf = build_pair(b.to_arrow(), opts)
+    rf.set_segment_leaf(32)
+    OB, OP = O.Store(b.as_tuple()), O.Store(p.as_tuple())
+    for pred, d, precision in [("contains", 0.0, 0), ("intersects", 0.0, 1), ("dwithin", 0.05, 0)]:
+        r = G.join_device(ix, rf, p.to_arrow(), pred, distance=d, precision=precision)
+        gb, gp = r.to_host()
+        ob, op = O.join(OB, OP, pred, distance=d, nthreads=8)
+        say("leaf 32", pred, precision, len(gb))
+        assert np.array_equal(gb, ob) and np.array_equal(gp, op), (pred, len(gb), len(ob))
+    a, q = D.workload_poly_poly(3_000, 3_000, seed=29)
+    ix2, rf2 = build_pair(a.to_arrow(), opts)
+    rf2.set_segment_leaf(32)
+    r = G.join_device(ix2, rf2, G.DeviceWkb.from_host(q), "intersects")
+    gb, gp = r.to_host()
+    ob, op = O.join(O.Store(a.as_tuple()), O.Store(q.as_tuple()), "intersects", nthreads=8)
+    assert np.array_equal(gb, ob) and np.array_equal(gp, op), ("polypoly", len(gb), len(ob))
+    cb, cp = C.corpus()
+    bcol, pcol = corpus_arrow(cb), corpus_arrow(cp)
+    rf3 = G.GpuSpatialRefiner(opts)
+    rf3.init_build_schema(bcol)
+    rf3.push_build(bcol)
+    rf3.finish_building()
+    rf3.set_segment_leaf(32)
+    nb, np_ = len(bcol), len(pcol)
+    bi = np.repeat(np.arange(nb), np_).astype(np.uint32)
+    pi = np.tile(np.arange(np_), nb).astype(np.uint32)
+    E = expected()
+    for pred in ["intersects", "contains", "covers", "touches"]:
+        want = np.unpackbits(E[f"corpus/{pred}/mask"])[: len(bi)].astype(bool)
+        ob, op = rf3.refine(pcol, pred, bi, pi)
+        eq, miss, extra = pairs_equal((ob, op), (bi[want], pi[want]))
+        assert eq, ("corpus", pred, miss, extra)
+    with pytest.raises(G.GpuSpatialError, match="16 or 32"):
+        rf3.set_segment_leaf(8)
+
+
 def test_poly_poly_large_rings_vs_oracle(opts):
     """Polygon x polygon intersects with rings of up to 1024 vertices (both
     sides get segment trees): the indexed crossing counts and point tests
