@@ -67,6 +67,10 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=96,
                     help="requests in the serving measurement (0 = skip it)")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--no-host", dest="host_inclusive", action="store_false",
+                    help="pip_zipf_contains: skip the host-inclusive (PCIe) measurement")
+    ap.add_argument("--abi-batch", type=int, default=1_000_000,
+                    help="pip_zipf_contains: probe rows per batch in the reference-ABI host flow")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only time the plain copy kernel (HBM calibration for the counter summaries)")
     args = ap.parse_args()
@@ -198,6 +202,10 @@ def main():
         r1.release()
         r1b.release()
 
+    host = None
+    if args.host_inclusive:
+        host = host_inclusive_measurement(G, ix, rf, probe, pairs, args)
+
     # Rooflines.  Algorithmic bytes per launch:
     #   filter: 16 B per probe rect read + 8 B per candidate pair written;
     #   refine: 8 B per candidate it reads + 8 B per kept pair (every pair of
@@ -278,12 +286,65 @@ def main():
         "plain_fp64": plain,
         "autotune": tuned,
         "serving": serving,
+        "host_inclusive": host,
     }
     if plain and args.precision == 0:
         out["speedup_vs_plain_fp64"] = plain["ms_per_step"] / ms_per_step
         out["refine_speedup_vs_plain_fp64"] = plain["stage_ms"][3] / stage[3]
     print(json.dumps(out), flush=True)
     D.release_shared(wl["keep"])
+
+
+def pair_checksum(b, p, p0=0):
+    """Order-independent checksum of (build, probe + p0) pairs (sum mod 2^64
+    of a mix of both ids)."""
+    k = (b.astype(np.uint64) << np.uint64(32)) | (p.astype(np.uint64) + np.uint64(p0))
+    k *= np.uint64(0x9E3779B97F4A7C15)
+    k ^= k >> np.uint64(29)
+    return int(np.sum(k, dtype=np.uint64))
+
+
+def host_inclusive_measurement(G, ix, rf, probe, pairs, args):
+    """The same join with the probe WKB in host memory and the pairs copied
+    back to host memory (PCIe included), two ways:
+      fused: one SjhipJoinDevice call over the host column, then the copy
+             of its pairs;
+      reference ABI: per batch of rows (the operator's probe batches,
+             stream.rs:1166-1269), compute the rects, probe the index with
+             the host rects (pairs come back through the callback,
+             gpu_spatial_index.rs:163-229), then refine the host candidates
+             against the host WKB batch (gpuspatial_c.h:154-211)."""
+    parr = probe.to_arrow(large=True)
+    n = len(parr)
+    t = time.time()
+    r = G.join_device(ix, rf, parr, "contains", sort_output=False, refine_order=-1 if args.autotune else args.refine_order)
+    gb, gp = r.to_host()
+    fused_ms = (time.time() - t) * 1000
+    r.release()
+    fused_sum = pair_checksum(gb, gp)
+    fused = dict(ms=fused_ms, pairs=int(len(gb)), pairs_per_s=len(gb) / (fused_ms / 1000),
+                 h2d_bytes=int(probe.offsets[-1] + 8 * (n + 1)), d2h_bytes=int(8 * len(gb)))
+    del gb, gp
+    log(f"host-inclusive fused join: {fused_ms:.1f} ms, {fused['pairs']} pairs")
+
+    bsz = args.abi_batch
+    kept = 0
+    cands = 0
+    abi_sum = 0
+    t = time.time()
+    for a in range(0, n, bsz):
+        part = parr.slice(a, min(bsz, n - a))
+        rects = G.compute_rects(part)
+        cb, cp = ix.probe(rects)
+        cands += len(cb)
+        kb, kp = rf.refine(part, "contains", cb, cp)
+        kept += len(kb)
+        abi_sum = (abi_sum + pair_checksum(kb, kp, a)) % (1 << 64)
+    abi_ms = (time.time() - t) * 1000
+    abi = dict(batch_rows=bsz, batches=(n + bsz - 1) // bsz, ms=abi_ms, pairs=int(kept), candidates=int(cands),
+               pairs_per_s=kept / (abi_ms / 1000))
+    log(f"host-inclusive reference-ABI flow: {abi_ms:.1f} ms, {kept} pairs in {abi['batches']} batches")
+    return dict(fused=fused, reference_abi=abi, same_pairs=bool(fused["pairs"] == pairs == kept and fused_sum == abi_sum))
 
 
 SERVE_MIX = (("intersects", 0.0), ("dwithin", 0.05))

@@ -216,15 +216,19 @@ constexpr int kPrecDeferred = 2;
 #endif
 constexpr int kLeafBatch = SJ_LEAF_BATCH;
 
-// A leaf's (up to) 16 segments, walking the vertex run once.
-template <int PREC>
-__device__ __forceinline__ void scan_leaf(TreeWalk& w, int leaf) {
-  const int s0 = leaf << w.ls;
-  const int s1 = min(s0 + (1 << w.ls), w.ns);
+// A leaf's (up to) LEAF segments from s0, walking the vertex run once.
+// LEAF is a compile-time size so that the batched loads below unroll
+// (the FP64 mode reads the leaf size from the walk instead).
+template <int PREC, int LEAF>
+__device__ __forceinline__ void scan_leaf_n(TreeWalk& w, const int s0) {
+  const int s1 = min(s0 + LEAF, w.ns);
   if constexpr (PREC != 1) {
     // the vertices of kLeafBatch segments are requested before the first
     // test, so that their latencies overlap (the walk is latency-bound)
-    for (int c0 = s0; c0 < s1; c0 += kLeafBatch) {
+#pragma unroll
+    for (int cb = 0; cb < LEAF; cb += kLeafBatch) {
+      const int c0 = s0 + cb;
+      if (c0 >= s1) break;
       float2 v[kLeafBatch + 1];
 #pragma unroll
       for (int k = 0; k <= kLeafBatch; k++) v[k] = c0 + k <= s1 ? w.q32[c0 + k] : make_float2(0.f, 0.f);
@@ -244,8 +248,9 @@ __device__ __forceinline__ void scan_leaf(TreeWalk& w, int leaf) {
       }
     }
   } else {
+    const int e = min(s0 + (1 << w.ls), w.ns);
     double2 a = w.xy[s0];
-    for (int j = s0; j < s1; j++) {
+    for (int j = s0; j < e; j++) {
       double2 b = w.xy[j + 1];
       double2 aa = a;
       a = b;
@@ -256,6 +261,16 @@ __device__ __forceinline__ void scan_leaf(TreeWalk& w, int leaf) {
       w.in ^= rc == kRayIn;
     }
   }
+}
+
+// The index's leaf size is one of two (kSegLeafShift or kSegLeafShift + 1)
+// and the same for every lane of a launch, so the branch is uniform.
+// The FP64 mode (a plain loop) keeps the runtime size.
+template <int PREC>
+__device__ __forceinline__ void scan_leaf(TreeWalk& w, int leaf) {
+  if constexpr (PREC == 1) scan_leaf_n<1, 0>(w, leaf << w.ls);
+  else if (w.ls == kSegLeafShift + 1) scan_leaf_n<PREC, 2 << kSegLeafShift>(w, leaf << (kSegLeafShift + 1));
+  else scan_leaf_n<PREC, 1 << kSegLeafShift>(w, leaf << kSegLeafShift);
 }
 
 #ifndef SJ_BOX_BATCH
@@ -1906,7 +1921,8 @@ void build_ring_index(const GeomColumn& g, RingIndex& ri, const RingIndexParams&
   hipLaunchKernelGGL(k_local32, blocks_for(threads), kBlock, 0, s, g.view(), g.n_rings, ri.origin.p, ri.xy32.p);
   // segment trees
   DevArray<int64_t> cnt(g.n_rings, s);
-  if (prm.seg_leaf_shift < 3 || prm.seg_leaf_shift > 6) throw Error("segment tree leaf shift must be 3..6");
+  if (prm.seg_leaf_shift != kSegLeafShift && prm.seg_leaf_shift != kSegLeafShift + 1)
+    throw Error("segment tree leaves must hold 16 or 32 segments");
   ri.seg_leaf_shift = prm.seg_leaf_shift;
   hipLaunchKernelGGL(k_seg_counts, blocks_for(g.n_rings), kBlock, 0, s, g.view(), g.n_rings, prm.min_segs_for_tree,
                      ri.seg_leaf_shift, cnt.p);
