@@ -63,6 +63,8 @@ def main():
     ap.add_argument("--knn-sample", type=int, default=200_000, help="knn: queries checked against the oracle")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="pip_zipf_contains: skip the autotuner (use --refine-order and the default kernels)")
+    ap.add_argument("--seg-leaf", type=int, default=0,
+                    help="pip_zipf_contains: segments per ring-tree leaf (16 or 32) set before the joins (0 = keep)")
     ap.add_argument("--probe-batch", type=int, default=5_000_000, help="dwithin_walks: probe points per batch")
     ap.add_argument("--serve-requests", type=int, default=96,
                     help="requests in the serving measurement (0 = skip it)")
@@ -141,6 +143,8 @@ def main():
     build_ms = (time.time() - tb) * 1000
     log(f"build side resident: {build_ms:.0f} ms (includes host->device copy)")
     pdev = G.DeviceWkb.from_host(probe)
+    if args.seg_leaf:
+        rf.set_segment_leaf(args.seg_leaf)
     tuned = None
     if args.autotune:
         ta = time.time()
@@ -245,7 +249,14 @@ def main():
 
     serving = None
     if serve is not None:
-        serving = serving_measurement(G, ix, rf, serve, args)
+        try:
+            serving = serving_measurement(G, ix, rf, serve, args)
+        except G.GpuSpatialError as e:
+            # reported in the result line; the headline above stands
+            import traceback
+            traceback.print_exc()
+            log(f"serving measurement failed: {e}")
+            serving = {"error": str(e)}
         D.release_shared(serve["keep"])
 
     cpu = None
@@ -379,29 +390,34 @@ def serving_measurement(G, ix, rf, serve, args):
         rng = np.random.default_rng(seed)
         gaps = rng.exponential(1.0 / rate, size=len(reqs))
         sv = G.GpuSpatialServer(ix, rf, slots=slots, graphs=graphs)
-        if graphs:
-            # capture every (slot, predicate, size bucket) graph before timing
-            for q, pred, d in reqs:
-                for _ in range(slots):
-                    sv.wait(sv.submit(q, pred, distance=d, sort_output=False))[0].release()
-        ids = []
-        t0 = time.perf_counter()
-        due = 0.0
-        for (q, pred, d), g in zip(reqs, gaps):
-            due += g
-            wait = due - (time.perf_counter() - t0)
-            if wait > 0:
-                time.sleep(wait)
-            ids.append(sv.submit(q, pred, distance=d, sort_output=False))
-        lat, first, last, npairs = [], None, None, 0
-        for rid in ids:
-            res, tm = sv.wait(rid)
-            npairs += res.size
-            res.release()
-            lat.append(tm["done_ms"] - tm["submit_ms"])
-            first = tm["submit_ms"] if first is None else min(first, tm["submit_ms"])
-            last = tm["done_ms"] if last is None else max(last, tm["done_ms"])
-        sv.close()
+        # the server is closed (its queue drained, its workers joined) before
+        # anything leaves this function, an exception included: the request
+        # columns must outlive every join still running on the device
+        try:
+            if graphs:
+                # capture every (slot, predicate, size bucket) graph before timing
+                for q, pred, d in reqs:
+                    for _ in range(slots):
+                        sv.wait(sv.submit(q, pred, distance=d, sort_output=False))[0].release()
+            ids = []
+            t0 = time.perf_counter()
+            due = 0.0
+            for (q, pred, d), g in zip(reqs, gaps):
+                due += g
+                wait = due - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(wait)
+                ids.append(sv.submit(q, pred, distance=d, sort_output=False))
+            lat, first, last, npairs = [], None, None, 0
+            for rid in ids:
+                res, tm = sv.wait(rid)
+                npairs += res.size
+                res.release()
+                lat.append(tm["done_ms"] - tm["submit_ms"])
+                first = tm["submit_ms"] if first is None else min(first, tm["submit_ms"])
+                last = tm["done_ms"] if last is None else max(last, tm["done_ms"])
+        finally:
+            sv.close()
         span = (last - first) / 1000
         lat = np.array(lat)
         return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),

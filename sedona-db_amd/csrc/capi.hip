@@ -911,6 +911,11 @@ struct Server {
   std::deque<ServerReq*> queue;
   std::map<uint64_t, std::unique_ptr<ServerReq>> reqs;
   std::vector<std::thread> workers;
+  // Plans that are no longer replayed.  They are destroyed only once every
+  // worker has joined: their pinned host buffers are freed with
+  // hipHostFree, which synchronises the device and must not run while
+  // another worker is capturing a graph on its stream.
+  std::vector<std::unique_ptr<GraphPlan>> retired;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
   double now_ms() const {
@@ -954,6 +959,7 @@ struct Server {
     SJ_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     // this worker's captured sequences, by (kind, predicate, bucket rows)
     std::map<std::tuple<int, int, int64_t>, std::unique_ptr<GraphPlan>> plans;
+    std::vector<std::unique_ptr<GraphPlan>> old;  // replaced plans (see `retired`)
     auto run_graph = [&](const WkbInput& in, const SjhipJoinOptions& o, SjhipJoinResult* res) {
       const int kind = graph_kind(in, o);
       if (kind < 0) return false;
@@ -964,7 +970,7 @@ struct Server {
       if (plan && (plan->distance != o.distance || plan->leaf != ix->tree.leaf ||
                    plan->tree_gen != ix->tree_gen || plan->ring_gen != rf->ring_gen ||
                    plan->pip_waves != ix->tuning.pip_waves))
-        plan.reset();
+        old.push_back(std::move(plan));
       if (!plan) {
         uint64_t per16 = ix->tree.cand_per_query_x16.load(std::memory_order_relaxed);
         double cpq = per16 ? per16 / 16.0 : 8.0;
@@ -981,7 +987,7 @@ struct Server {
       // too many candidates for the plan: capture a larger one next time
       if (plan->host_cnt[0] == ~0ull && plan->host_cnt[1] > plan->C) {
         uint64_t C = plan->host_cnt[1] + plan->host_cnt[1] / 4 + 65536;
-        plan.reset();
+        old.push_back(std::move(plan));
         plan = capture_plan(ix, rf, kind, o, B, C, s);
         if (run_plan(*plan, in, res, s)) {
           if (o.sort_output && res->size) {
@@ -1029,8 +1035,13 @@ struct Server {
       }
       cv.notify_all();
     }
-    plans.clear();
     (void)hipStreamSynchronize(s);
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      for (auto& kv : plans)
+        if (kv.second) retired.push_back(std::move(kv.second));
+      for (auto& g : old) retired.push_back(std::move(g));
+    }
     (void)hipStreamDestroy(s);
   }
 };
@@ -1254,6 +1265,22 @@ int SjhipAutotune(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner*
       evals++;
       return best;
     };
+    // best of three, HIP-event time of the refine stage alone (for choices
+    // that change only the refine, which the join's jitter would hide)
+    auto time_refine = [&]() {
+      double best = 1e300;
+      o.profile = 1;
+      for (int r = 0; r < 3; r++) {
+        SjhipJoinResult res{};
+        JoinEngine::run(ix, rf, in, o, &res);
+        double ms = res.stage_ms[3];
+        if (res.release) res.release(&res);
+        best = std::min(best, ms);
+      }
+      o.profile = 0;
+      evals++;
+      return best;
+    };
     auto rebuild = [&](int leaf) {
       if (ix->tree.leaf == leaf) return;
       RectTree t;
@@ -1313,7 +1340,7 @@ int SjhipAutotune(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner*
     //    trees serve every refine against this build side; the point-in-
     //    polygon walk is the one that prefers larger leaves.
     if (kind == 0) {
-      best = time_join();
+      double best_refine = time_refine();
       for (int ls : {4, 5}) {
         if (ls == rf->ring_index.seg_leaf_shift) continue;
         RingIndexParams prm;
@@ -1324,14 +1351,15 @@ int SjhipAutotune(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner*
         other.detach_streams();
         std::swap(rf->ring_index, other);
         rf->ring_gen++;
-        double ms = time_join();
-        if (ms < best) {
-          best = ms;
+        double ms = time_refine();
+        if (ms < best_refine) {
+          best_refine = ms;
         } else {
           std::swap(rf->ring_index, other);
           rf->ring_gen++;
         }
       }
+      best = time_join();
     }
     // give back the pool memory the sample joins left cached, so that the
     // first full-size joins do not find it fragmented
@@ -1452,6 +1480,7 @@ int SjhipServerDestroy(void* server) {
     }
     sv->cv.notify_all();
     for (auto& t : sv->workers) t.join();
+    sv->retired.clear();  // no worker is capturing any more
     for (auto& kv : sv->reqs)
       if (kv.second->status == 2 && kv.second->res.release) kv.second->res.release(&kv.second->res);
     delete sv;
