@@ -998,7 +998,8 @@ __device__ __forceinline__ int raster_class(const RingIndexView& R, const PolyDe
 // Whether the raster shows no point of the polygon (record d) in rect o,
 // a part of its rect: every cell that o meets, widened by one cell, is
 // clean outside (such a cell holds no boundary and lies outside).  Windows
-// of more than 64 cells are not read (false).
+// of more than 64 cells at 32 x 32 (the same share of the rect at other
+// resolutions) are not read (false).
 __device__ __forceinline__ bool raster_outside(const RingIndexView& R, const PolyDesc& d, double4 o) {
   if (d.raster_off < 0) return false;
   const double cw = (d.rect.z - d.rect.x) / kRasterCells, ch = (d.rect.w - d.rect.y) / kRasterCells;
@@ -1006,7 +1007,8 @@ __device__ __forceinline__ bool raster_outside(const RingIndexView& R, const Pol
   const int i1 = (int)fmin(floor((o.z - d.rect.x) / cw) + 1.0, kRasterCells - 1.0);
   const int j0 = (int)fmax(floor((o.y - d.rect.y) / ch) - 1.0, 0.0);
   const int j1 = (int)fmin(floor((o.w - d.rect.y) / ch) + 1.0, kRasterCells - 1.0);
-  if (i0 > i1 || j0 > j1 || (i1 - i0 + 1) * (j1 - j0 + 1) > 64) return false;
+  constexpr int kMaxWindow = 64 * (kRasterCells / 32) * (kRasterCells / 32);
+  if (i0 > i1 || j0 > j1 || (i1 - i0 + 1) * (j1 - j0 + 1) > kMaxWindow) return false;
   for (int j = j0; j <= j1; j++)
     for (int i = i0; i <= i1; i++) {
       const int cell = j * kRasterCells + i;
@@ -1826,21 +1828,21 @@ __global__ __launch_bounds__(kBlock) void k_raster_build(Ctx c, PolyDesc* desc, 
       }
   }
   wave_lds_sync();
-  // lane k < kRasterWords writes word k: 16 cells
-  if (lane < kRasterWords) {
+  // lane k writes words k, k + 64, ...: 16 cells each
+  const double2 o = ring_origin(rb);
+  const float4* tree = c.R.seg_tree + d.tree_off;
+  const float2* q32 = c.R.xy32 + d.v0;
+  for (int wi = lane; wi < kRasterWords; wi += 64) {
     uint32_t word = 0;
     uint64_t t = 0, f = 0;
-    const double2 o = ring_origin(rb);
-    const float4* tree = c.R.seg_tree + d.tree_off;
-    const float2* q32 = c.R.xy32 + d.v0;
     for (int k = 0; k < 16; k++) {
-      const int cell = lane * 16 + k;
+      const int cell = wi * 16 + k;
       if ((touch[cell >> 5] >> (cell & 31)) & 1u) continue;
       const double px = rb.x + ((cell % kRasterCells) + 0.5) * cw, py = rb.y + ((cell / kRasterCells) + 0.5) * ch;
       const bool in = ring_contains_run<0>(xy, q32, tree, c.R.seg_leaf_shift, d.npts, rb, o, d.err, px, py, true, t, f);
       word |= (in ? 1u : 2u) << (2 * k);
     }
-    raster[roff[row] * kRasterWords + lane] = word;
+    raster[roff[row] * kRasterWords + wi] = word;
   }
   if (lane == 0) desc[row].raster_off = roff[row] * kRasterWords;
 }

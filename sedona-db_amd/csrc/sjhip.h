@@ -150,7 +150,11 @@ struct alignas(32) PolyDesc {
 // Cell raster of a polygon ring: kRasterCells x kRasterCells cells over
 // the ring's rect, 2 bits each (0: near the boundary, 1: inside, 2:
 // outside), kRasterWords 32-bit words per ring.
-constexpr int kRasterCells = 32;
+#ifndef SJ_RASTER_CELLS
+#define SJ_RASTER_CELLS 64
+#endif
+constexpr int kRasterCells = SJ_RASTER_CELLS;
+static_assert(kRasterCells == 32 || kRasterCells == 64 || kRasterCells == 128, "raster cells per side: 32, 64 or 128");
 constexpr int kRasterWords = kRasterCells * kRasterCells / 16;
 
 // Cell raster of a line: kLineCells x kLineCells cells over the line's
