@@ -206,6 +206,38 @@ void build_rect_tree(const float4* rects, int64_t n, RectTree& t, hipStream_t s,
   SJ_CHECK(hipStreamSynchronize(s));
 }
 
+#ifndef SJ_FILTER_WAVE
+#define SJ_FILTER_WAVE 1
+#endif
+
+// The plain filter over the tree: wavefront-cooperative (k_filter_wave)
+// when the queries come in a locality order, else one walk per lane.
+void launch_plain_filter(const RectTree& t, const TreeView& v, const float4* q, int64_t nq, const uint32_t* qorder,
+                         int er, uint32_t* ob, uint32_t* op, uint64_t cap, unsigned long long* counter,
+                         hipStream_t s) {
+  if (SJ_FILTER_WAVE && qorder) {
+    const unsigned grid = (unsigned)((nq + kFilterWaveBlock - 1) / kFilterWaveBlock);
+    if (t.leaf == 4)
+      hipLaunchKernelGGL((k_filter_wave<RectTree::FANOUT, 4>), grid, kFilterWaveBlock, 0, s, v, q, nq, qorder, er,
+                         ob, op, cap, counter);
+    else if (t.leaf == 16)
+      hipLaunchKernelGGL((k_filter_wave<RectTree::FANOUT, 16>), grid, kFilterWaveBlock, 0, s, v, q, nq, qorder, er,
+                         ob, op, cap, counter);
+    else
+      hipLaunchKernelGGL((k_filter_wave<RectTree::FANOUT, 8>), grid, kFilterWaveBlock, 0, s, v, q, nq, qorder, er,
+                         ob, op, cap, counter);
+  } else if (t.leaf == 4) {
+    hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 4, KeepAll>), blocks_for(nq), kBlock, 0, s, v, q, nq, qorder, er,
+                       KeepAll{}, ob, op, cap, counter);
+  } else if (t.leaf == 16) {
+    hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 16, KeepAll>), blocks_for(nq), kBlock, 0, s, v, q, nq, qorder, er,
+                       KeepAll{}, ob, op, cap, counter);
+  } else {
+    hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 8, KeepAll>), blocks_for(nq), kBlock, 0, s, v, q, nq, qorder, er,
+                       KeepAll{}, ob, op, cap, counter);
+  }
+}
+
 uint64_t filter_pairs(const RectTree& t, const float4* q, int64_t nq, const uint32_t* qorder, PairBuf& pb,
                       hipStream_t s, KernelTimer* timer, bool emit_rank) {
   if (pb.counter.n < 1) pb.counter.alloc(1, s);
@@ -222,15 +254,7 @@ uint64_t filter_pairs(const RectTree& t, const float4* q, int64_t nq, const uint
   for (int attempt = 0; attempt < 2; attempt++) {
     SJ_CHECK(hipMemsetAsync(pb.counter.p, 0, 8, s));
     if (timer) timer->record_start(s);
-    if (t.leaf == 4)
-      hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 4, KeepAll>), blocks_for(nq), kBlock, 0, s, v, q, nq, qorder,
-                         emit_rank ? 1 : 0, KeepAll{}, pb.b.p, pb.p.p, pb.cap(), pb.counter.p);
-    else if (t.leaf == 16)
-      hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 16, KeepAll>), blocks_for(nq), kBlock, 0, s, v, q, nq, qorder,
-                         emit_rank ? 1 : 0, KeepAll{}, pb.b.p, pb.p.p, pb.cap(), pb.counter.p);
-    else
-      hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 8, KeepAll>), blocks_for(nq), kBlock, 0, s, v, q, nq, qorder,
-                         emit_rank ? 1 : 0, KeepAll{}, pb.b.p, pb.p.p, pb.cap(), pb.counter.p);
+    launch_plain_filter(t, v, q, nq, qorder, emit_rank ? 1 : 0, pb.b.p, pb.p.p, pb.cap(), pb.counter.p, s);
     SJ_CHECK(hipGetLastError());
     if (timer) timer->record_stop(s);
     unsigned long long total = 0;
@@ -295,16 +319,7 @@ void launch_filter(const RectTree& t, const float4* q, int64_t nq, const uint32_
                    uint32_t* ob, uint32_t* op, uint64_t cap, unsigned long long* counter, hipStream_t s) {
   if (nq <= 0 || t.n_levels == 0) return;
   TreeView v = tree_view(t);
-  const int er = emit_rank ? 1 : 0;
-  if (t.leaf == 4)
-    hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 4, KeepAll>), blocks_for(nq), kBlock, 0, s, v, q, nq, qorder, er,
-                       KeepAll{}, ob, op, cap, counter);
-  else if (t.leaf == 16)
-    hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 16, KeepAll>), blocks_for(nq), kBlock, 0, s, v, q, nq, qorder, er,
-                       KeepAll{}, ob, op, cap, counter);
-  else
-    hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 8, KeepAll>), blocks_for(nq), kBlock, 0, s, v, q, nq, qorder, er,
-                       KeepAll{}, ob, op, cap, counter);
+  launch_plain_filter(t, v, q, nq, qorder, emit_rank ? 1 : 0, ob, op, cap, counter, s);
   SJ_CHECK(hipGetLastError());
 }
 

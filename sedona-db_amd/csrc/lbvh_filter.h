@@ -132,6 +132,133 @@ __global__ __launch_bounds__(kFilterBlock) void k_filter(TreeView t, const float
   kp.finish();
 }
 
+// The walk of k_filter_wave for one wavefront with at least one active
+// lane; the pairs go to wo (flushed by the caller at the end).
+template <int F, int LEAF>
+__device__ __forceinline__ void filter_wave_walk(const TreeView& t, bool active, const float4& box, uint32_t pid,
+                                                 WaveOut& wo, uint32_t* ob, uint32_t* op, uint64_t cap,
+                                                 unsigned long long* counter) {
+  const int lane = threadIdx.x & 63;
+  // union of the active lanes' rects
+  float ux0 = active ? box.x : INFINITY, uy0 = active ? box.y : INFINITY;
+  float ux1 = active ? box.z : -INFINITY, uy1 = active ? box.w : -INFINITY;
+  for (int o = 32; o > 0; o >>= 1) {
+    ux0 = fminf(ux0, __shfl_xor(ux0, o));
+    uy0 = fminf(uy0, __shfl_xor(uy0, o));
+    ux1 = fmaxf(ux1, __shfl_xor(ux1, o));
+    uy1 = fmaxf(uy1, __shfl_xor(uy1, o));
+  }
+  const float4 ub = make_float4(ux0, uy0, ux1, uy1);
+  // children still to visit, 8 bits per level: levels 0..7 in mlo, 8..15 in mhi
+  uint64_t mlo = 0, mhi = 0;
+  auto get_mask = [&](int l) -> uint32_t {
+    return (uint32_t)((l < 8 ? mlo >> (8 * l) : mhi >> (8 * (l - 8))) & 0xffu);
+  };
+  auto set_mask = [&](int l, uint32_t m) {
+    if (l < 8) mlo = (mlo & ~(0xffull << (8 * l))) | ((uint64_t)m << (8 * l));
+    else mhi = (mhi & ~(0xffull << (8 * (l - 8)))) | ((uint64_t)m << (8 * (l - 8)));
+  };
+  const int top = t.n_levels;  // level top: the virtual root over the <= F nodes of level top-1
+  {
+    const int64_t nc = t.cnt[top - 1];
+    const bool h = lane < F && lane < nc && hit(t.nodes[t.off[top - 1] + lane], ub);
+    set_mask(top, (uint32_t)__ballot(h) & ((1u << F) - 1));
+  }
+  int lvl = top;
+  int64_t cur = 0;  // node at level lvl
+  for (;;) {
+    while (lvl <= top && get_mask(lvl) == 0) {
+      lvl++;
+      cur /= F;
+    }
+    if (lvl > top) break;
+    const uint32_t m = __builtin_amdgcn_readfirstlane(get_mask(lvl));
+    const int c = __ffs(m) - 1;
+    set_mask(lvl, m & (m - 1));
+    const int64_t child = cur * F + c;  // index at level lvl-1
+    if (lvl == 1) {
+      const int64_t hbase = child * LEAF;
+      const int64_t e = min(hbase + LEAF, t.n_valid);
+      const float4* lr = t.leaf_rects + hbase;
+      uint32_t hits = 0;
+#pragma unroll
+      for (int k = 0; k < LEAF; k++)
+        if (hbase + k < e) {
+          const float4 r = lr[k];
+          if (active && hit(r, box)) hits |= 1u << k;
+        }
+      if (__ballot(hits != 0)) wave_append_mask(wo, hits, t.leaf_ids, hbase, pid, ob, op, cap, counter);
+    } else {
+      const int cl = lvl - 2;  // level of the child's children
+      const int64_t c0 = child * F;
+      const int64_t nc = t.cnt[cl] - c0;
+      const bool h = lane < F && lane < nc && hit(t.nodes[t.off[cl] + c0 + lane], ub);
+      lvl--;
+      cur = child;
+      set_mask(lvl, (uint32_t)__ballot(h) & ((1u << F) - 1));
+    }
+  }
+}
+
+// Wavefront-cooperative filter for queries visited in a locality order
+// (qorder: Morton order of the query rects).  The wavefront walks the
+// tree once with the union of its (up to) 64 query rects: the children of
+// a node are tested by F lanes at once and the walk state is one ballot
+// mask per level, uniform across the wavefront.  At each leaf reached,
+// every lane tests its own rect against the leaf's rects (uniform loads).
+// A rect that meets a lane's query also meets the union, so the pairs are
+// exactly those of k_filter (one walk per lane); they come out leaf by
+// leaf, lanes in order.
+//
+// The pairs a wavefront still holds at the end are written with one
+// atomic per block (kFilterWaveBlock threads): a single output counter
+// takes one same-address atomic per flush, and one per wavefront would be
+// about one per 64 queries.
+constexpr int kFilterWaveBlock = 512;
+
+template <int F, int LEAF>
+__global__ __launch_bounds__(kFilterWaveBlock) void k_filter_wave(TreeView t, const float4* q, int64_t nq,
+                                                                  const uint32_t* qorder, int emit_rank,
+                                                                  uint32_t* ob, uint32_t* op, uint64_t cap,
+                                                                  unsigned long long* counter) {
+  static_assert(F <= 8 && LEAF <= 32, "fan-out of at most 8 (8-bit masks), leaves of at most 32 rects");
+  constexpr int kWaves = kFilterWaveBlock / 64;
+  __shared__ uint32_t s_ob[kWaves][kWaveOutCap], s_op[kWaves][kWaveOutCap];
+  __shared__ uint32_t s_n[kWaves];
+  __shared__ unsigned long long s_base;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  WaveOut wo{s_ob[wave], s_op[wave], 0};
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + tid;
+  bool active = i < nq && t.n_levels > 0;
+  uint32_t pid = 0;
+  float4 box = make_float4(0, 0, 0, 0);
+  if (active) {
+    const uint32_t qid = qorder ? qorder[i] : (uint32_t)i;
+    pid = emit_rank ? (uint32_t)i : qid;
+    box = q[qid];
+    if (!rect_ok(box)) active = false;
+  }
+  if (__any(active)) filter_wave_walk<F, LEAF>(t, active, box, pid, wo, ob, op, cap, counter);
+  // block flush: every thread of the block reaches the barriers
+  if (lane == 0) s_n[wave] = wo.n;
+  __syncthreads();
+  if (tid == 0) {
+    unsigned long long tot = 0;
+    for (int k = 0; k < kWaves; k++) tot += s_n[k];
+    s_base = tot ? atomicAdd(counter, tot) : 0ull;
+  }
+  __syncthreads();
+  unsigned long long base = s_base;
+  for (int k = 0; k < wave; k++) base += s_n[k];
+  for (uint32_t k = lane; k < wo.n; k += 64) {
+    const unsigned long long d = base + k;
+    if (d < cap) {
+      ob[d] = wo.sb[k];
+      op[d] = wo.sp[k];
+    }
+  }
+}
+
 inline TreeView tree_view(const RectTree& t) {
   TreeView v{};
   v.nodes = t.nodes.p;
