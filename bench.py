@@ -233,7 +233,7 @@ def main():
     traffic = read_traffic(os.path.join(ROOT, "profiles", "traffic_pip_zipf_100Mx1M_contains.md"))
     full = args.scale == 1.0 and args.precision == 0
     roof_filter = dict(bound="hbm", achieved=filter_bytes / (filt_ms / 1e3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
-                       traffic=traffic.get("k_filter") if full else None, kernel="k_filter", ms=filt_ms,
+                       traffic=traffic.get("k_filter") if full else None, kernel="k_filter_wave", ms=filt_ms,
                        algorithmic_bytes=filter_bytes)
     roof_filter["frac"] = roof_filter["achieved"] / HBM_PEAK_GBS
     ach_tf = refine_flops / (ref_ms / 1e3) / 1e12

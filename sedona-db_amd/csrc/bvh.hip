@@ -23,6 +23,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "lbvh_filter.h"
+#include <algorithm>
 #include "sjhip.h"
 #include "wave_out.h"
 
@@ -113,6 +114,19 @@ __global__ void k_gather_leaves(const float4* r, const uint32_t* sorted_ids, int
   out[i] = r[sorted_ids[i]];
 }
 
+// Extent (max of width, height) of up to n_out leaf nodes, every stride-th.
+__global__ void k_leaf_extents(const float4* nodes, int64_t n_nodes, int64_t stride, int n_out, float* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_out) return;
+  const int64_t k = (int64_t)i * stride;
+  if (k >= n_nodes) {
+    out[i] = 0.f;
+    return;
+  }
+  const float4 b = nodes[k];
+  out[i] = fmaxf(b.z - b.x, b.w - b.y);
+}
+
 __global__ void k_merge(const float4* child, int64_t n_child, int group, float4* parent, int64_t n_parent) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_parent) return;
@@ -157,6 +171,7 @@ void build_rect_tree(const float4* rects, int64_t n, RectTree& t, hipStream_t s,
   t.level_off.clear();
   t.level_cnt.clear();
   t.n_levels = 0;
+  t.coop_extent = 0.f;
   float bb[4];
   center_bounds(rects, n, bb, &t.n_valid, s);
   if (t.n_valid == 0) {
@@ -203,7 +218,18 @@ void build_rect_tree(const float4* rects, int64_t n, RectTree& t, hipStream_t s,
     hipLaunchKernelGGL(k_merge, blocks_for(cnt[l]), kBlock, 0, s, t.nodes.p + t.level_off[l - 1], cnt[l - 1],
                        RectTree::FANOUT, t.nodes.p + t.level_off[l], cnt[l]);
   SJ_CHECK(hipGetLastError());
+  // the median extent of (a strided sample of) the leaf nodes
+  const int n_sample = (int)std::min<int64_t>(cnt[0], 4096);
+  const int64_t stride = std::max<int64_t>(1, cnt[0] / n_sample);
+  DevArray<float> ext(n_sample, s);
+  hipLaunchKernelGGL(k_leaf_extents, blocks_for(n_sample), kBlock, 0, s, t.nodes.p + t.level_off[0], cnt[0], stride,
+                     n_sample, ext.p);
+  SJ_CHECK(hipGetLastError());
+  std::vector<float> h(n_sample);
+  SJ_CHECK(hipMemcpyAsync(h.data(), ext.p, n_sample * sizeof(float), hipMemcpyDeviceToHost, s));
   SJ_CHECK(hipStreamSynchronize(s));
+  std::nth_element(h.begin(), h.begin() + n_sample / 2, h.end());
+  t.coop_extent = 2.0f * h[n_sample / 2];
 }
 
 #ifndef SJ_FILTER_WAVE

@@ -34,6 +34,7 @@ struct TreeView {
   const uint32_t* leaf_ids;
   int64_t n_valid;
   int n_levels;
+  float coop_extent;  // see RectTree::coop_extent
   int64_t off[kMaxLevels];
   int64_t cnt[kMaxLevels];
 };
@@ -135,20 +136,10 @@ __global__ __launch_bounds__(kFilterBlock) void k_filter(TreeView t, const float
 // The walk of k_filter_wave for one wavefront with at least one active
 // lane; the pairs go to wo (flushed by the caller at the end).
 template <int F, int LEAF>
-__device__ __forceinline__ void filter_wave_walk(const TreeView& t, bool active, const float4& box, uint32_t pid,
-                                                 WaveOut& wo, uint32_t* ob, uint32_t* op, uint64_t cap,
+__device__ __forceinline__ void filter_wave_walk(const TreeView& t, bool active, const float4& box, const float4& ub,
+                                                 uint32_t pid, WaveOut& wo, uint32_t* ob, uint32_t* op, uint64_t cap,
                                                  unsigned long long* counter) {
   const int lane = threadIdx.x & 63;
-  // union of the active lanes' rects
-  float ux0 = active ? box.x : INFINITY, uy0 = active ? box.y : INFINITY;
-  float ux1 = active ? box.z : -INFINITY, uy1 = active ? box.w : -INFINITY;
-  for (int o = 32; o > 0; o >>= 1) {
-    ux0 = fminf(ux0, __shfl_xor(ux0, o));
-    uy0 = fminf(uy0, __shfl_xor(uy0, o));
-    ux1 = fmaxf(ux1, __shfl_xor(ux1, o));
-    uy1 = fmaxf(uy1, __shfl_xor(uy1, o));
-  }
-  const float4 ub = make_float4(ux0, uy0, ux1, uy1);
   // children still to visit, 8 bits per level: levels 0..7 in mlo, 8..15 in mhi
   uint64_t mlo = 0, mhi = 0;
   auto get_mask = [&](int l) -> uint32_t {
@@ -200,6 +191,66 @@ __device__ __forceinline__ void filter_wave_walk(const TreeView& t, bool active,
   }
 }
 
+// One walk per lane (the loop of k_filter with every hit kept), for a
+// wavefront whose queries lie too far apart to share a walk.  mask: the
+// block's per-level child masks, mask[l * blockDim.x + threadIdx.x].
+template <int F, int LEAF>
+__device__ __forceinline__ void filter_lane_walk(const TreeView& t, bool active, const float4& box, uint32_t pid,
+                                                 uint8_t* mask, WaveOut& wo, uint32_t* ob, uint32_t* op,
+                                                 uint64_t cap, unsigned long long* counter) {
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const int top = t.n_levels;
+  int lvl = top;
+  int64_t cur = 0;
+  if (active) {
+    uint32_t m = 0;
+    const float4* nd = t.nodes + t.off[top - 1];
+    const int64_t nc = t.cnt[top - 1];
+    for (int c = 0; c < F; c++)
+      if (c < nc && hit(nd[c], box)) m |= 1u << c;
+    mask[top * bs + tid] = (uint8_t)m;
+  }
+  while (__any(active)) {
+    uint32_t hits = 0;
+    int64_t hbase = 0;
+    if (active) {
+      while (lvl <= top && mask[lvl * bs + tid] == 0) {
+        lvl++;
+        cur /= F;
+      }
+      if (lvl > top) {
+        active = false;
+      } else {
+        const uint32_t m = mask[lvl * bs + tid];
+        const int c = __ffs(m) - 1;
+        mask[lvl * bs + tid] = (uint8_t)(m & (m - 1));
+        const int64_t child = cur * F + c;
+        if (lvl == 1) {
+          hbase = child * LEAF;
+          const int64_t e = min(hbase + LEAF, t.n_valid);
+          const float4* lr = t.leaf_rects + hbase;
+#pragma unroll
+          for (int k = 0; k < LEAF; k++)
+            if (hbase + k < e && hit(lr[k], box)) hits |= 1u << k;
+        } else {
+          const int cl = lvl - 2;
+          const float4* nd = t.nodes + t.off[cl];
+          const int64_t c0 = child * F;
+          const int64_t nc = t.cnt[cl] - c0;
+          uint32_t mm = 0;
+#pragma unroll
+          for (int k = 0; k < F; k++)
+            if (k < nc && hit(nd[c0 + k], box)) mm |= 1u << k;
+          lvl--;
+          cur = child;
+          mask[lvl * bs + tid] = (uint8_t)mm;
+        }
+      }
+    }
+    if (__ballot(hits != 0)) wave_append_mask(wo, hits, t.leaf_ids, hbase, pid, ob, op, cap, counter);
+  }
+}
+
 // Wavefront-cooperative filter for queries visited in a locality order
 // (qorder: Morton order of the query rects).  The wavefront walks the
 // tree once with the union of its (up to) 64 query rects: the children of
@@ -208,7 +259,9 @@ __device__ __forceinline__ void filter_wave_walk(const TreeView& t, bool active,
 // every lane tests its own rect against the leaf's rects (uniform loads).
 // A rect that meets a lane's query also meets the union, so the pairs are
 // exactly those of k_filter (one walk per lane); they come out leaf by
-// leaf, lanes in order.
+// leaf, lanes in order.  A wavefront whose union is wider than
+// t.coop_extent (its queries lie far apart, as in a sparse probe) walks
+// one lane per query instead.
 //
 // The pairs a wavefront still holds at the end are written with one
 // atomic per block (kFilterWaveBlock threads): a single output counter
@@ -226,6 +279,7 @@ __global__ __launch_bounds__(kFilterWaveBlock) void k_filter_wave(TreeView t, co
   __shared__ uint32_t s_ob[kWaves][kWaveOutCap], s_op[kWaves][kWaveOutCap];
   __shared__ uint32_t s_n[kWaves];
   __shared__ unsigned long long s_base;
+  __shared__ uint8_t s_mask[(kMaxLevels + 1) * kFilterWaveBlock];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   WaveOut wo{s_ob[wave], s_op[wave], 0};
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + tid;
@@ -238,7 +292,22 @@ __global__ __launch_bounds__(kFilterWaveBlock) void k_filter_wave(TreeView t, co
     box = q[qid];
     if (!rect_ok(box)) active = false;
   }
-  if (__any(active)) filter_wave_walk<F, LEAF>(t, active, box, pid, wo, ob, op, cap, counter);
+  if (__any(active)) {
+    // union of the active lanes' rects
+    float ux0 = active ? box.x : INFINITY, uy0 = active ? box.y : INFINITY;
+    float ux1 = active ? box.z : -INFINITY, uy1 = active ? box.w : -INFINITY;
+    for (int o = 32; o > 0; o >>= 1) {
+      ux0 = fminf(ux0, __shfl_xor(ux0, o));
+      uy0 = fminf(uy0, __shfl_xor(uy0, o));
+      ux1 = fmaxf(ux1, __shfl_xor(ux1, o));
+      uy1 = fmaxf(uy1, __shfl_xor(uy1, o));
+    }
+    const float4 ub = make_float4(ux0, uy0, ux1, uy1);
+    if (fmaxf(ux1 - ux0, uy1 - uy0) <= t.coop_extent)  // wavefront-uniform
+      filter_wave_walk<F, LEAF>(t, active, box, ub, pid, wo, ob, op, cap, counter);
+    else
+      filter_lane_walk<F, LEAF>(t, active, box, pid, s_mask, wo, ob, op, cap, counter);
+  }
   // block flush: every thread of the block reaches the barriers
   if (lane == 0) s_n[wave] = wo.n;
   __syncthreads();
@@ -266,6 +335,7 @@ inline TreeView tree_view(const RectTree& t) {
   v.leaf_ids = t.leaf_ids.p;
   v.n_valid = t.n_valid;
   v.n_levels = t.n_levels;
+  v.coop_extent = t.coop_extent;
   for (int l = 0; l < t.n_levels; l++) {
     v.off[l] = t.level_off[l];
     v.cnt[l] = t.level_cnt[l];

@@ -287,6 +287,9 @@ struct RectTree {
   DevArray<uint32_t> leaf_ids;      // their build ids
   DevArray<uint32_t> leaf_keys;     // their Morton keys (sorted)
   float bb[4] = {0, 0, 0, 0};       // bounds of rect centres (Morton frame)
+  // query unions up to this extent (max of width, height) are walked by
+  // a whole wavefront at once: twice the median extent of a leaf node
+  float coop_extent = 0.f;
   // observed candidates (and fused-join kept pairs) per query x 16: sizes
   // the next probe's output
   mutable std::atomic<uint64_t> cand_per_query_x16{0};
@@ -310,6 +313,7 @@ struct RectTree {
     leaf_ids = std::move(o.leaf_ids);
     leaf_keys = std::move(o.leaf_keys);
     for (int k = 0; k < 4; k++) bb[k] = o.bb[k];
+    coop_extent = o.coop_extent;
     cand_per_query_x16.store(o.cand_per_query_x16.load());
     kept_per_query_x16.store(o.kept_per_query_x16.load());
     return *this;
