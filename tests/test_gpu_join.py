@@ -133,6 +133,59 @@ def test_filter_matches_oracle_candidates(opts):
     assert eq, (miss, extra)
 
 
+def brute_force_candidates(br, qr):
+    """(build, query) pairs whose FP32 rects meet (inclusive), valid rects only."""
+    okb = (br[:, 0] <= br[:, 2]) & (br[:, 1] <= br[:, 3])
+    okq = (qr[:, 0] <= qr[:, 2]) & (qr[:, 1] <= qr[:, 3])
+    bs, ps = [], []
+    for a in range(0, len(qr), 2048):
+        q = qr[a:a + 2048]
+        h = ((br[None, :, 0] <= q[:, None, 2]) & (br[None, :, 2] >= q[:, None, 0]) &
+             (br[None, :, 1] <= q[:, None, 3]) & (br[None, :, 3] >= q[:, None, 1]) &
+             okb[None, :] & okq[a:a + 2048, None])
+        qi, bi = np.nonzero(h)
+        bs.append(bi.astype(np.uint32))
+        ps.append((qi + a).astype(np.uint32))
+    return np.concatenate(bs), np.concatenate(ps)
+
+
+@pytest.mark.parametrize("layout", ["dense", "sparse", "mixed", "boxes"])
+def test_filter_both_walk_modes_vs_brute_force(layout, opts):
+    """The filter walks a wavefront of Morton-ordered queries once with the
+    union of their rects when they lie close together (dense), and one lane
+    per query when they lie far apart (sparse); both must give exactly the
+    rect hits, edges included."""
+    rng = np.random.default_rng(17)
+    nb = 3000
+    c = rng.uniform(0, 100, size=(nb, 2)).astype(np.float32)
+    h = rng.uniform(0.05, 1.5, size=(nb, 2)).astype(np.float32)
+    br = np.concatenate([c - h, c + h], axis=1).astype(np.float32)
+    if layout == "dense":
+        xy = rng.uniform(40, 42, size=(30000, 2)).astype(np.float32)
+    elif layout == "sparse":
+        xy = rng.uniform(0, 100, size=(3000, 2)).astype(np.float32)
+    else:
+        xy = np.concatenate([rng.uniform(40, 42, size=(20000, 2)), rng.uniform(0, 100, size=(3000, 2))]).astype(
+            np.float32)
+    # points on build rect edges and corners (the compare is inclusive)
+    e = rng.integers(0, nb, size=500)
+    xy = np.concatenate([xy, np.stack([br[e, 0], br[e, 1]], 1), np.stack([br[e, 2], rng.uniform(br[e, 1], br[e, 3])], 1)
+                         ]).astype(np.float32)
+    qr = np.concatenate([xy, xy], axis=1)
+    if layout == "boxes":
+        qr[:, 2:] += rng.uniform(0, 0.3, size=(len(qr), 2)).astype(np.float32)
+    qr[::97] = np.nan  # null rows
+    qr[5::101, 0] = qr[5::101, 2] + 1  # inverted (empty) rects
+    ix = G.GpuSpatialIndex(opts)
+    ix.push_build(br)
+    ix.finish_building()
+    gb, gp = ix.probe(qr)
+    ob, op = brute_force_candidates(br, qr)
+    say(layout, len(qr), len(ob))
+    eq, miss, extra = pairs_equal((gb, gp), (ob, op))
+    assert eq, (layout, miss, extra)
+
+
 def test_device_rects_match_reference_rounding():
     """bounds.rs:147-162 outward rounding and the GPU operator's nearest
     rounding for points, as restated by the oracle."""
