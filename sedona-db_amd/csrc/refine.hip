@@ -196,13 +196,14 @@ struct TreeWalk {
   float E;
   double px, py;
   int ns;
-  int ls;  // segments per leaf: 1 << ls
+  int ls;    // segments per leaf: 1 << ls
+  float sx;  // ray direction: +1 toward +x, -1 toward -x (x mirrored)
   int on, in;
   uint32_t tests, fallbacks;
 };
 
 __device__ __forceinline__ bool box_pass(const TreeWalk& w, const float4& b) {
-  return w.pyl >= b.y - w.m && w.pyl <= b.w + w.m && w.pxl <= b.z + w.m;
+  return w.pyl >= b.y - w.m && w.pyl <= b.w + w.m && (w.sx > 0.f ? w.pxl <= b.z + w.m : w.pxl >= b.x - w.m);
 }
 
 // Ring-walk precision modes (template parameter PREC): 0 = certified FP32,
@@ -235,7 +236,8 @@ __device__ __forceinline__ void scan_leaf_n(TreeWalk& w, const int s0) {
 #pragma unroll
       for (int k = 0; k < kLeafBatch; k++) {
         if (c0 + k >= s1) break;
-        int rc = raycast_f32_certified(v[k].x, v[k].y, v[k + 1].x, v[k + 1].y, w.pxl, w.pyl, w.E);
+        int rc = raycast_f32_certified(w.sx * v[k].x, v[k].y, w.sx * v[k + 1].x, v[k + 1].y, w.sx * w.pxl, w.pyl,
+                                       w.E);
         if (rc == kRayOut) continue;
         w.tests++;
         if (rc < 0) {
@@ -432,6 +434,16 @@ __device__ __forceinline__ bool ring_contains_run(const double2* xy, const float
     w.py = py;
     w.ns = (int)ns;
     w.ls = ls;
+    // The certified pass of the deferred mode casts its ray toward the
+    // nearer side of the ring's rect, so that fewer segments and tree nodes
+    // lie on it.  Mirroring x is exact, every ring is closed (the decoder
+    // rejects open rings) and the crossing parity of a closed ring is the
+    // same for every ray from a point off its boundary; a point on the
+    // boundary or level with a vertex leaves a test uncertain, and the
+    // candidate is then re-refined along +x in FP64 (the reference's
+    // raycast).  The other modes decide uncertain tests segment by segment
+    // along +x, so they keep +x throughout.
+    w.sx = (PREC == kPrecDeferred && px - rb.x < rb.z - px) ? -1.0f : 1.0f;
     w.on = 0;
     w.in = 0;
     w.tests = 0;
