@@ -197,13 +197,20 @@ struct TreeWalk {
   double px, py;
   int ns;
   int ls;    // segments per leaf: 1 << ls
-  float sx;  // ray direction: +1 toward +x, -1 toward -x (x mirrored)
+  // ray direction: along the x axis (swap = 0) or the y axis (swap = 1),
+  // toward + (sx = 1) or - (sx = -1); the tests read (along, across)
+  // coordinates (sx * x, y) or (sx * y, x)
+  float sx;
+  int swap;
   int on, in;
   uint32_t tests, fallbacks;
 };
 
 __device__ __forceinline__ bool box_pass(const TreeWalk& w, const float4& b) {
-  return w.pyl >= b.y - w.m && w.pyl <= b.w + w.m && (w.sx > 0.f ? w.pxl <= b.z + w.m : w.pxl >= b.x - w.m);
+  const float c0 = w.swap ? b.x : b.y, c1 = w.swap ? b.z : b.w;  // across the ray
+  const float a0 = w.swap ? b.y : b.x, a1 = w.swap ? b.w : b.z;  // along the ray
+  const float pa = w.swap ? w.pyl : w.pxl, pc = w.swap ? w.pxl : w.pyl;
+  return pc >= c0 - w.m && pc <= c1 + w.m && (w.sx > 0.f ? pa <= a1 + w.m : pa >= a0 - w.m);
 }
 
 // Ring-walk precision modes (template parameter PREC): 0 = certified FP32,
@@ -236,8 +243,9 @@ __device__ __forceinline__ void scan_leaf_n(TreeWalk& w, const int s0) {
 #pragma unroll
       for (int k = 0; k < kLeafBatch; k++) {
         if (c0 + k >= s1) break;
-        int rc = raycast_f32_certified(w.sx * v[k].x, v[k].y, w.sx * v[k + 1].x, v[k + 1].y, w.sx * w.pxl, w.pyl,
-                                       w.E);
+        const float2 a = v[k], b = v[k + 1];
+        int rc = w.swap ? raycast_f32_certified(w.sx * a.y, a.x, w.sx * b.y, b.x, w.sx * w.pyl, w.pxl, w.E)
+                        : raycast_f32_certified(w.sx * a.x, a.y, w.sx * b.x, b.y, w.sx * w.pxl, w.pyl, w.E);
         if (rc == kRayOut) continue;
         w.tests++;
         if (rc < 0) {
@@ -435,15 +443,27 @@ __device__ __forceinline__ bool ring_contains_run(const double2* xy, const float
     w.ns = (int)ns;
     w.ls = ls;
     // The certified pass of the deferred mode casts its ray toward the
-    // nearer side of the ring's rect, so that fewer segments and tree nodes
-    // lie on it.  Mirroring x is exact, every ring is closed (the decoder
-    // rejects open rings) and the crossing parity of a closed ring is the
-    // same for every ray from a point off its boundary; a point on the
-    // boundary or level with a vertex leaves a test uncertain, and the
-    // candidate is then re-refined along +x in FP64 (the reference's
-    // raycast).  The other modes decide uncertain tests segment by segment
-    // along +x, so they keep +x throughout.
-    w.sx = (PREC == kPrecDeferred && px - rb.x < rb.z - px) ? -1.0f : 1.0f;
+    // nearest side of the ring's rect, so that fewer segments and tree
+    // nodes lie on it.  Mirroring and transposing the coordinates are
+    // exact, every ring is closed (the decoder rejects open rings) and the
+    // crossing parity of a closed ring is the same for every ray from a
+    // point off its boundary; a point on the boundary, or level with a
+    // vertex across the ray, leaves a test uncertain, and the candidate is
+    // then re-refined along +x in FP64 (the reference's raycast).  The
+    // other modes decide uncertain tests segment by segment along +x, so
+    // they keep +x throughout.
+    w.sx = 1.0f;
+    w.swap = 0;
+    if constexpr (PREC == kPrecDeferred) {
+      const double dl = px - rb.x, dr = rb.z - px, db = py - rb.y, dt = rb.w - py;
+      const double dx = fmin(dl, dr), dy = fmin(db, dt);
+      if (dy < dx) {
+        w.swap = 1;
+        w.sx = db < dt ? -1.0f : 1.0f;
+      } else {
+        w.sx = dl < dr ? -1.0f : 1.0f;
+      }
+    }
     w.on = 0;
     w.in = 0;
     w.tests = 0;
