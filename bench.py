@@ -429,10 +429,8 @@ def serving_measurement(G, ix, rf, serve, args):
         rate = rel * capacity
         row = {"rel_to_one_at_a_time_capacity": rel, "rate_rps": rate,
                "batched": run(rate, args.serve_slots, 7 + int(rel * 10)),
-               "batched_graphs": run(rate, args.serve_slots, 7 + int(rel * 10), graphs=True),
                "one_at_a_time": run(rate, 1, 7 + int(rel * 10))}
         log(f"serving rate {rate:.1f}/s: batched p50 {row['batched']['p50_ms']:.2f} p99 {row['batched']['p99_ms']:.2f} ms;"
-            f" with graphs p50 {row['batched_graphs']['p50_ms']:.2f} p99 {row['batched_graphs']['p99_ms']:.2f} ms;"
             f" one-at-a-time p50 {row['one_at_a_time']['p50_ms']:.2f} p99 {row['one_at_a_time']['p99_ms']:.2f} ms")
         rates.append(row)
     del reqs, dev
@@ -440,8 +438,7 @@ def serving_measurement(G, ix, rf, serve, args):
             "mix": "ST_Intersects / ST_DWithin(0.05) alternating", "slots": args.serve_slots,
             "memory_budget": "90% of free HBM", "one_at_a_time_service_ms": service_s * 1000,
             "one_at_a_time_capacity_rps": capacity, "rates": rates,
-            "graphs": "batched_graphs replays one captured HIP graph per slot, predicate and power-of-two size "
-                      "bucket for requests of up to 65536 points; larger requests run the eager pipeline",
+            "graphs": "not measured: captured-graph replay is switched off in the library",
             "p50_ms": rates[1]["batched"]["p50_ms"], "p99_ms": rates[1]["batched"]["p99_ms"]}
 
 

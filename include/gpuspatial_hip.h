@@ -327,7 +327,9 @@ struct SjhipServerConfig {
                               point probes in device memory (up to 65536
                               rows; point-in-polygon predicates and DWithin):
                               one graph per slot, predicate and power-of-two
-                              size bucket, captured on first use */
+                              size bucket, captured on first use.  Switched
+                              off in this build: SjhipServerCreate fails
+                              with EINVAL unless graphs is 0 */
 };
 
 /* Timing of one request, ms on a steady clock since the server's

@@ -1382,6 +1382,11 @@ int SjhipServerCreate(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefi
     if (!index || !index->private_data || !refiner || !refiner->private_data) throw Error("null index or refiner");
     if (!config || !out_server) throw Error("null argument");
     if (config->slots < 1 || config->slots > 64) throw Error("slots must be in 1..64");
+    // Graph replay is switched off: one serving run with it (4 slots under
+    // load) met an illegal memory access whose cause has not been found.
+    if (config->graphs != 0)
+      throw Error("captured-graph replay in the request loop is switched off (graphs must be 0): a loaded serving "
+                  "run with it faulted and the cause is not found");
     *out_server = nullptr;
     std::unique_ptr<Server> sv(new Server());
     sv->ix = (IndexImpl*)index->private_data;

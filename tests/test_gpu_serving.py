@@ -135,6 +135,8 @@ def test_request_error_is_reported_by_wait(resident):
         sv.close()
 
 
+@pytest.mark.skip(reason="captured-graph replay is switched off in the library: a loaded 4-slot serving run with "
+                         "it met an illegal memory access whose cause is not found")
 def test_graph_replays_equal_eager_joins(resident):
     """With captured graphs, every replay (three rounds over the same size
     buckets, with eager joins in between) returns each request's pairs as
