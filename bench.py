@@ -65,6 +65,9 @@ def main():
                     help="pip_zipf_contains: skip the autotuner (use --refine-order and the default kernels)")
     ap.add_argument("--seg-leaf", type=int, default=0,
                     help="pip_zipf_contains: segments per ring-tree leaf (16 or 32) set before the joins (0 = keep)")
+    ap.add_argument("--pip-waves", type=int, default=0,
+                    help="pip_zipf_contains: resident waves of the point-in-polygon kernel (4, 6, 8) set before the "
+                         "joins (0 = keep)")
     ap.add_argument("--probe-batch", type=int, default=5_000_000, help="dwithin_walks: probe points per batch")
     ap.add_argument("--serve-requests", type=int, default=96,
                     help="requests in the serving measurement (0 = skip it)")
@@ -145,22 +148,20 @@ def main():
     pdev = G.DeviceWkb.from_host(probe)
     if args.seg_leaf:
         rf.set_segment_leaf(args.seg_leaf)
+    if args.pip_waves:
+        ix.set_tuning(kind=0, refine_order=-1, pip_waves=args.pip_waves)
     tuned = None
     if args.autotune:
         ta = time.time()
-        # the sample is a vertical strip holding about a tenth of the points:
-        # at the workload's own density, the candidates per polygon (which
-        # decide between filter order and polygon-major order) are those of
-        # the full join, unlike in a random tenth of the points
-        x = wl["probe_xy"][:, 0]
-        lo, hi = np.quantile(x[:: max(1, len(x) // 1_000_000)], [0.45, 0.55])
-        srows = np.nonzero((x >= lo) & (x < hi))[0]
-        sample = G.DeviceWkb.from_host(D.take_rows(probe, srows))
-        tuned = G.autotune(ix, rf, sample, "contains")
-        del sample
+        # the tuner times joins of the whole resident probe column (about
+        # 2 x 12 joins, a few seconds): a strip of a tenth of the points had
+        # the workload's density but no longer ranked the candidate orders
+        # as the full join does (filter order won on the strip and lost by
+        # 16 % at full size)
+        tuned = G.autotune(ix, rf, pdev, "contains")
         tuned["tune_s"] = time.time() - ta
-        tuned["sample_points"] = int(len(srows))
-        log(f"autotune (strip of {len(srows)} points): {tuned}")
+        tuned["sample_points"] = int(n_points)
+        log(f"autotune (all {n_points} points): {tuned}")
 
     def step(precision, profile=True):
         return G.join_device(ix, rf, pdev, "contains", precision=precision, sort_output=False, sort_probes=True,

@@ -312,6 +312,15 @@ int SjhipAutotune(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner*
  * concurrently with joins on the refiner. */
 int SjhipRefinerSetSegmentLeaf(struct SedonaSpatialRefiner* refiner, int32_t segments_per_leaf);
 
+/* Sets the join parameters the autotuner would choose for the index, so
+ * that a run can reproduce a tuned configuration without tuning: the
+ * candidate order for joins whose refine is `kind` (0 point in polygon,
+ * 1 distance, 2 other relations; refine_order 0, 1 or 2, or -1 to clear)
+ * and the point-in-polygon kernel's resident waves (4, 6 or 8; 0 keeps the
+ * current value).  No counterpart in the reference.  Must not run
+ * concurrently with joins on the index. */
+int SjhipIndexSetTuning(struct SedonaFloatIndex2D* index, int32_t kind, int32_t refine_order, int32_t pip_waves);
+
 /* Request loop over a resident index and refiner (the serving path of the
  * GPU operator: rust/sedona-spatial-join-gpu/src/index/gpu_spatial_index.rs:
  * 163-229 called for many probe batches at once).  Requests are joined

@@ -1229,6 +1229,20 @@ int SjhipRefinerSetSegmentLeaf(struct SedonaSpatialRefiner* refiner, int32_t seg
   });
 }
 
+int SjhipIndexSetTuning(struct SedonaFloatIndex2D* index, int32_t kind, int32_t refine_order, int32_t pip_waves) {
+  return guarded(nullptr, [&] {
+    if (!index || !index->private_data) throw Error("null index");
+    IndexImpl* ix = (IndexImpl*)index->private_data;
+    if (kind < 0 || kind > 2) throw Error("kind must be 0, 1 or 2");
+    if (refine_order < -1 || refine_order > 2) throw Error("refine_order must be -1, 0, 1 or 2");
+    if (pip_waves != 0 && pip_waves != 4 && pip_waves != 6 && pip_waves != 8)
+      throw Error("pip_waves must be 0, 4, 6 or 8");
+    std::lock_guard<std::mutex> g(ix->mu);
+    ix->tuning.refine_order[kind] = refine_order;
+    if (pip_waves) ix->tuning.pip_waves = pip_waves;
+  });
+}
+
 int SjhipAutotune(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner* refiner,
                   const struct ArrowSchema* probe_schema, const struct ArrowDeviceArray* probe_sample,
                   const struct SjhipJoinOptions* options, struct SjhipTuneResult* out) {
@@ -1313,15 +1327,24 @@ int SjhipAutotune(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner*
     }
     rebuild(best_leaf);
     // 2. candidate order before the refine
+    //    The engine's default order is kept unless another is faster by
+    //    more than 5 % on the sample: near a tie the sample's timing noise
+    //    would decide, and at full size the default has been the faster.
     std::vector<int> orders = {0, 1};
     if (dwithin) orders.push_back(2);
-    int best_order = o.refine_order;
-    best = 1e300;
+    const int default_order = dwithin && rf->ring_index.n_line_rasters == 0 ? 0 : 1;
+    std::vector<double> order_ms;
     for (int ro : orders) {
       o.refine_order = ro;
-      double ms = time_join();
-      if (ms < best) { best = ms; best_order = ro; }
+      order_ms.push_back(time_join());
     }
+    int best_order = default_order;
+    best = order_ms[default_order];
+    for (size_t k = 0; k < orders.size(); k++)
+      if (order_ms[k] < 0.95 * order_ms[default_order] && order_ms[k] < best) {
+        best = order_ms[k];
+        best_order = orders[k];
+      }
     o.refine_order = best_order;
     ix->tuning.refine_order[kind] = best_order;
     // 3. resident waves of the point-in-polygon refine

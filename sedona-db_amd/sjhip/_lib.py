@@ -196,7 +196,7 @@ EXPORTED = [
     "SjhipComputeRects", "SjhipDeviceAlloc", "SjhipDeviceFree", "SjhipCopyToDevice", "SjhipCopyToHost",
     "SjhipSetDevice", "SjhipCopyKernel",
     "SjhipServerCreate", "SjhipServerSubmit", "SjhipServerWait", "SjhipServerDestroy", "SjhipRefinerKnnDevice",
-    "SjhipAutotune", "SjhipRefinerSetSegmentLeaf",
+    "SjhipAutotune", "SjhipRefinerSetSegmentLeaf", "SjhipIndexSetTuning",
     "SjhipHostRaycast", "SjhipHostRaycastF32", "SjhipHostCollinear", "SjhipHostSegmentIntersects",
     "SjhipHostOrient2d", "SjhipHostHypot", "SjhipHostCoordToLine", "SjhipHostRaycastBatch", "SjhipHostHypotBatch",
 ]
@@ -233,6 +233,7 @@ def lib():
         L.SjhipAutotune.argtypes = [_IX, _RF, _SCH, ctypes.POINTER(ArrowDeviceArray), ctypes.POINTER(SjhipJoinOptions),
                                     ctypes.POINTER(SjhipTuneResult)]
         L.SjhipRefinerSetSegmentLeaf.argtypes = [_RF, ctypes.c_int32]
+        L.SjhipIndexSetTuning.argtypes = [_IX, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
         L.SjhipRefinerKnnDevice.argtypes = [_RF, _SCH, ctypes.POINTER(ArrowDeviceArray), ctypes.c_uint32,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.SjhipIndexBuildFromRefiner.argtypes = [_IX, _RF]

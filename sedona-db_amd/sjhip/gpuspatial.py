@@ -329,6 +329,13 @@ class GpuSpatialIndex:
             return np.zeros(0, np.uint32), np.zeros(0, np.uint32)
         return np.concatenate(outs_b), np.concatenate(outs_p)
 
+    def set_tuning(self, kind=0, refine_order=-1, pip_waves=0):
+        """Sets the tuned candidate order for joins of refine `kind` and the
+        point-in-polygon kernel's resident waves (0 keeps them)."""
+        rc = self.lib.SjhipIndexSetTuning(ctypes.byref(self.ix), int(kind), int(refine_order), int(pip_waves))
+        if rc:
+            raise GpuSpatialError(self.lib.SjhipLastError().decode())
+
     def release(self):
         if self.ix.private_data:
             self.ix.release(ctypes.byref(self.ix))
