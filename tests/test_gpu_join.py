@@ -437,6 +437,24 @@ def test_autotuned_join_gives_the_same_pairs(pred, d, opts):
     assert np.array_equal(gb, ob) and np.array_equal(gp, op), (len(gb), len(ob))
 
 
+def test_set_tuning_orders_and_waves_give_the_oracle_pairs(opts):
+    """SjhipIndexSetTuning: every candidate order and point-in-polygon wave
+    count it accepts gives the oracle's pairs through refine_order=-1;
+    values outside its range are rejected."""
+    b, p = D.workload_pip_zipf(100_000, 1_000, seed=13)
+    ix, rf = build_pair(b.to_arrow(), opts)
+    ob, op = O.join(O.Store(b.as_tuple()), O.Store(p.as_tuple()), "contains", nthreads=8)
+    for order, waves in [(0, 4), (1, 6), (1, 8), (-1, 0)]:
+        ix.set_tuning(kind=0, refine_order=order, pip_waves=waves)
+        r = G.join_device(ix, rf, p.to_arrow(), "contains", refine_order=-1)
+        gb, gp = r.to_host()
+        say("tuning", order, waves, len(gb))
+        assert np.array_equal(gb, ob) and np.array_equal(gp, op), (order, waves, len(gb), len(ob))
+    for kind, order, waves in [(3, 0, 0), (0, 3, 0), (0, 1, 5)]:
+        with pytest.raises(G.GpuSpatialError):
+            ix.set_tuning(kind=kind, refine_order=order, pip_waves=waves)
+
+
 def test_segment_leaf_32_gives_the_oracle_pairs(opts):
     """Every walk over the build rings' segment trees with 32 segments per
     leaf (the autotuner's other choice): point in polygon (contains,
