@@ -450,24 +450,38 @@ __device__ __forceinline__ bool ring_contains_run(const double2* xy, const float
     // point off its boundary; a point on the boundary, or level with a
     // vertex across the ray, leaves a test uncertain, and the candidate is
     // then re-refined along +x in FP64 (the reference's raycast).  The
-    // other modes decide uncertain tests segment by segment along +x, so
-    // they keep +x throughout.
+    // inline mode (PREC 0) decides uncertain tests segment by segment along
+    // +x, which cannot be mixed with another ray: it walks along the
+    // nearest side certified first and keeps that answer only when no test
+    // was uncertain, else walks the ring again along +x.  The FP64 mode
+    // keeps +x throughout.
     w.sx = 1.0f;
     w.swap = 0;
-    if constexpr (PREC == kPrecDeferred) {
-      const double dl = px - rb.x, dr = rb.z - px, db = py - rb.y, dt = rb.w - py;
-      const double dx = fmin(dl, dr), dy = fmin(db, dt);
-      if (dy < dx) {
-        w.swap = 1;
-        w.sx = db < dt ? -1.0f : 1.0f;
-      } else {
-        w.sx = dl < dr ? -1.0f : 1.0f;
-      }
-    }
     w.on = 0;
     w.in = 0;
     w.tests = 0;
     w.fallbacks = 0;
+    if constexpr (PREC != 1) {
+      TreeWalk v = w;
+      const double dl = px - rb.x, dr = rb.z - px, db = py - rb.y, dt = rb.w - py;
+      const double dx = fmin(dl, dr), dy = fmin(db, dt);
+      if (dy < dx) {
+        v.swap = 1;
+        v.sx = db < dt ? -1.0f : 1.0f;
+      } else {
+        v.sx = dl < dr ? -1.0f : 1.0f;
+      }
+      if constexpr (PREC == kPrecDeferred) {
+        w = v;
+      } else {
+        // inline mode: a certified walk along the nearest side decides when
+        // none of its tests is uncertain; else the ring is walked again
+        // along +x with uncertain segments decided in FP64
+        walk_tree<kPrecDeferred>(v);
+        tests += v.tests;
+        if (v.fallbacks == 0) return v.in != 0;  // (no test is ON when none is uncertain)
+      }
+    }
     walk_tree<PREC>(w);
     tests += w.tests;
     fallbacks += w.fallbacks;
