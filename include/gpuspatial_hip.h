@@ -69,6 +69,9 @@ struct ArrowArray {
 typedef int32_t ArrowDeviceType;
 #define ARROW_DEVICE_CPU 1
 #define ARROW_DEVICE_ROCM 10
+/* sync_event: NULL, or (for ROCm arrays) a pointer to the producer's
+ * hipEvent_t.  Every entry point taking an ArrowDeviceArray waits on the
+ * host for that event before it reads or queues the buffers. */
 struct ArrowDeviceArray {
   struct ArrowArray array;
   int64_t device_id;
@@ -320,6 +323,13 @@ int SjhipRefinerSetSegmentLeaf(struct SedonaSpatialRefiner* refiner, int32_t seg
  * current value).  No counterpart in the reference.  Must not run
  * concurrently with joins on the index. */
 int SjhipIndexSetTuning(struct SedonaFloatIndex2D* index, int32_t kind, int32_t refine_order, int32_t pip_waves);
+
+/* Precision of the refiner's refine and SjhipRefinerRefineDistance calls:
+ * 0 (the default) runs the certified FP32 tests with FP64 re-evaluation
+ * where a sign is uncertain, 1 the all-FP64 plain kernels.  Both give the
+ * same pair set; 1 exists for comparisons.  No counterpart in the
+ * reference. */
+int SjhipRefinerSetPrecision(struct SedonaSpatialRefiner* refiner, int32_t precision);
 
 /* Request loop over a resident index and refiner (the serving path of the
  * GPU operator: rust/sedona-spatial-join-gpu/src/index/gpu_spatial_index.rs:

@@ -29,6 +29,13 @@ double SjhipHostCoordToLine(double px, double py, double sx, double sy, double e
 void SjhipHostRaycastBatch(const double* seg_pts, int64_t n, int32_t* out);
 void SjhipHostHypotBatch(const double* xy, int64_t n, double* out);
 
+/* Device test hook: the kernels-only stable radix sort that captured
+ * request sequences use, over device buffers of n (u32 key, u32 value)
+ * pairs on device `device_id` (keys ordered by bits [begin_bit, end_bit)).
+ * Returns 0, or EINVAL with the message in SjhipLastError(). */
+int SjhipTestingGraphSortPairs(const uint32_t* keys, const uint32_t* values, uint32_t* keys_out,
+                               uint32_t* values_out, int64_t n, int begin_bit, int end_bit, int device_id);
+
 #ifdef __cplusplus
 }
 #endif

@@ -324,12 +324,7 @@ void fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_
   SJ_CHECK(hipGetLastError());
 }
 
-size_t morton_sort_temp_bytes(int64_t n) {
-  size_t tmp = 0;
-  SJ_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                              (uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 32));
-  return tmp;
-}
+size_t morton_sort_temp_bytes(int64_t n) { return graph_sort_temp_bytes(n); }
 
 void morton_order_fixed(const RectTree& t, const float4* rects, int64_t n, uint32_t* keys, uint32_t* ids,
                         uint32_t* keys_sorted, uint32_t* order, void* temp, size_t temp_bytes, hipStream_t s) {
@@ -338,7 +333,11 @@ void morton_order_fixed(const RectTree& t, const float4* rects, int64_t n, uint3
   float sx = bb[2] > bb[0] ? 65535.f / (bb[2] - bb[0]) : 0.f;
   float sy = bb[3] > bb[1] ? 65535.f / (bb[3] - bb[1]) : 0.f;
   hipLaunchKernelGGL(k_query_keys, blocks_for(n), kBlock, 0, s, rects, n, bb[0], bb[1], sx, sy, keys, ids);
-  SJ_CHECK(hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys, keys_sorted, ids, order, (int)n, 0, 32, s));
+  SJ_CHECK(hipGetLastError());
+  // not rocPRIM's onesweep sort: it clears its digit counters and lookback
+  // states with hipMemsetAsync, and a memset node in a replayed graph does
+  // not reliably clear them again
+  graph_sort_pairs(keys, ids, keys_sorted, order, n, 0, 32, temp, temp_bytes, s);
 }
 
 void launch_filter(const RectTree& t, const float4* q, int64_t nq, const uint32_t* qorder, bool emit_rank,

@@ -20,6 +20,7 @@
 #include <thread>
 
 #include "../../include/gpuspatial_hip.h"
+#include "../../include/gpuspatial_hip_testing.h"
 #include "sjhip.h"
 
 namespace sjhip {
@@ -62,6 +63,14 @@ Runtime* runtime_of(const GpuSpatialRuntime* rt) {
 }
 
 void use_device(const Runtime* r) { SJ_CHECK(hipSetDevice(r->device_id)); }
+
+// The Arrow C device interface's sync_event: for a ROCm array, a pointer to
+// the producer's hipEvent_t, which must complete before the buffers are
+// read.  The host waits for it before the call reads or queues the array.
+void wait_sync_event(const ArrowDeviceArray* a) {
+  if (a && a->device_type == ARROW_DEVICE_ROCM && a->sync_event)
+    SJ_CHECK(hipEventSynchronize(*(hipEvent_t*)a->sync_event));
+}
 
 // A stream owned for the duration of one call.
 struct ScopedStream {
@@ -276,6 +285,9 @@ struct RefinerImpl {
   std::mutex ws_mu;
   std::vector<std::unique_ptr<Workspace>> ws_free;
   uint64_t ring_gen = 0;  // bumped whenever the ring index is rebuilt (captured plans compare it)
+  // refine precision of the ABI's refine calls: 0 certified FP32 with FP64
+  // fallback, 1 the all-FP64 plain kernels
+  std::atomic<int> precision{0};
 };
 
 // A join's workspace: taken from the refiner's free list (or new) for the
@@ -489,10 +501,11 @@ void refine_host(RefinerImpl* rf, const ArrowSchema* probe_schema, const ArrowAr
   SJ_CHECK(hipMemcpyAsync(cp.p, pidx, (size_t)n * 4, hipMemcpyHostToDevice, s));
   PairBuf out;
   uint64_t kept;
+  const int precision = rf->precision.load();
   if (pred == 100)
-    kept = refine_distance(rf->build, rf->ring_index, probe, distance, cb.p, cp.p, n, out, s);
+    kept = refine_distance(rf->build, rf->ring_index, probe, distance, cb.p, cp.p, n, out, s, nullptr, precision);
   else
-    kept = refine_relation(rf->build, rf->ring_index, probe, pred, cb.p, cp.p, n, 0, out, nullptr, s);
+    kept = refine_relation(rf->build, rf->ring_index, probe, pred, cb.p, cp.p, n, precision, out, nullptr, s);
   sort_pairs(out.b.p, out.p.p, kept, s);
   if (kept) {
     SJ_CHECK(hipMemcpyAsync(bidx, out.b.p, kept * 4, hipMemcpyDeviceToHost, s));
@@ -763,10 +776,36 @@ struct GraphPlan {
   }
 };
 
+// Captures are made one at a time in the process: a plan's pinned host
+// buffers are allocated, and its sequence captured and instantiated, while
+// no other thread captures.
+std::mutex& capture_mutex() {
+  static std::mutex mu;
+  return mu;
+}
+
+// Throws unless every node of a captured graph is a kernel or a copy.  A
+// memset node does not reliably clear its memory again when a graph is
+// replayed, and an allocation node would make replays allocate.
+void check_graph_nodes(hipGraph_t graph) {
+  size_t nn = 0;
+  SJ_CHECK(hipGraphGetNodes(graph, nullptr, &nn));
+  std::vector<hipGraphNode_t> nodes(nn);
+  if (nn) SJ_CHECK(hipGraphGetNodes(graph, nodes.data(), &nn));
+  for (size_t i = 0; i < nn; i++) {
+    hipGraphNodeType ty;
+    SJ_CHECK(hipGraphNodeGetType(nodes[i], &ty));
+    if (ty != hipGraphNodeTypeKernel && ty != hipGraphNodeTypeMemcpy && ty != hipGraphNodeTypeEmpty)
+      throw Error("captured request sequence holds a node of type " + std::to_string((int)ty) +
+                  " (only kernels and copies may be replayed)");
+  }
+}
+
 // Builds and captures a plan on stream s (the stream's work is complete on
-// return; the plan's buffers are freed on the null stream).
+// return; the plan's buffers are freed on the detached stream).
 std::unique_ptr<GraphPlan> capture_plan(IndexImpl* ix, RefinerImpl* rf, int kind, const SjhipJoinOptions& o,
                                         int64_t B, uint64_t C, hipStream_t s) {
+  std::lock_guard<std::mutex> capture_lock(capture_mutex());
   std::unique_ptr<GraphPlan> g(new GraphPlan());
   g->kind = kind;
   g->predicate = o.predicate;
@@ -834,6 +873,7 @@ std::unique_ptr<GraphPlan> capture_plan(IndexImpl* ix, RefinerImpl* rf, int kind
     throw;
   }
   SJ_CHECK(hipStreamEndCapture(s, &g->graph));
+  check_graph_nodes(g->graph);
   SJ_CHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
   // the plan outlives the capturing stream's use: free on the detached stream
   hipStream_t ds = detached_stream();
@@ -852,7 +892,8 @@ std::unique_ptr<GraphPlan> capture_plan(IndexImpl* ix, RefinerImpl* rf, int kind
 // the request must take the eager path instead.
 bool run_plan(GraphPlan& g, const WkbInput& in, SjhipJoinResult* out, hipStream_t s) {
   g.host_desc->data = in.data;
-  g.host_desc->offsets = in.offsets;
+  // an Arrow array's row offset moves the start of its offsets buffer
+  g.host_desc->offsets = (const uint8_t*)in.offsets + in.offset * (in.large ? 8 : 4);
   g.host_desc->n = in.length;
   g.host_desc->large = in.large ? 1 : 0;
   g.host_desc->pad = 0;
@@ -1103,6 +1144,20 @@ int GpuSpatialRefinerCreate(struct SedonaSpatialRefiner* refiner, const struct G
 
 int SjhipExtVersion(void) { return SJHIP_EXT_VERSION; }
 
+int SjhipTestingGraphSortPairs(const uint32_t* keys, const uint32_t* values, uint32_t* keys_out,
+                               uint32_t* values_out, int64_t n, int begin_bit, int end_bit, int device_id) {
+  return guarded(nullptr, [&] {
+    if (n < 0 || (n > 0 && (!keys || !values || !keys_out || !values_out))) throw Error("null buffer");
+    SJ_CHECK(hipSetDevice(device_id));
+    ScopedStream ss;
+    {
+      DevArray<uint8_t> temp(graph_sort_temp_bytes(n), ss.s);
+      graph_sort_pairs(keys, values, keys_out, values_out, n, begin_bit, end_bit, temp.p, temp.n, ss.s);
+    }
+    SJ_CHECK(hipStreamSynchronize(ss.s));
+  });
+}
+
 const char* SjhipLastError(void) { return t_last_error.c_str(); }
 
 int SjhipRefinerRefineDistance(struct SedonaSpatialRefiner* self, const struct ArrowSchema* probe_schema,
@@ -1168,6 +1223,7 @@ int SjhipRefinerKnnDevice(struct SedonaSpatialRefiner* self, const struct ArrowS
     if (!dev && probe_array->device_type != ARROW_DEVICE_CPU)
       throw Error("probe array must be in ROCm device memory or CPU memory");
     use_device(rf->rt);
+    wait_sync_event(probe_array);
     ScopedStream ss;
     GeomColumn probe;
     decode_wkb(wkb_input_from_arrow(probe_schema, &probe_array->array, dev), probe, ss.s);
@@ -1206,6 +1262,7 @@ int SjhipJoinDevice(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefine
     if (!dev && probe_array->device_type != ARROW_DEVICE_CPU)
       throw Error("probe array must be in ROCm device memory or CPU memory");
     WkbInput in = wkb_input_from_arrow(probe_schema, &probe_array->array, dev);
+    wait_sync_event(probe_array);
     JoinEngine::run((IndexImpl*)index->private_data, (RefinerImpl*)refiner->private_data, in, *options, out);
   });
 }
@@ -1226,6 +1283,14 @@ int SjhipRefinerSetSegmentLeaf(struct SedonaSpatialRefiner* refiner, int32_t seg
     SJ_CHECK(hipStreamSynchronize(rf->stream));
     rf->ring_index = std::move(ri);
     rf->ring_gen++;
+  });
+}
+
+int SjhipRefinerSetPrecision(struct SedonaSpatialRefiner* refiner, int32_t precision) {
+  return guarded(nullptr, [&] {
+    if (!refiner || !refiner->private_data) throw Error("null refiner");
+    if (precision != 0 && precision != 1) throw Error("precision must be 0 or 1");
+    ((RefinerImpl*)refiner->private_data)->precision.store(precision);
   });
 }
 
@@ -1258,6 +1323,7 @@ int SjhipAutotune(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner*
       throw Error("probe sample must be in ROCm device memory or CPU memory");
     use_device(rf->rt);
     WkbInput in = wkb_input_from_arrow(probe_schema, &probe_sample->array, dev);
+    wait_sync_event(probe_sample);
     const bool dwithin = options->predicate == 100;
     const int64_t n_rects = (int64_t)(ix->rects.n);
     SjhipJoinOptions o = *options;
@@ -1405,11 +1471,6 @@ int SjhipServerCreate(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefi
     if (!index || !index->private_data || !refiner || !refiner->private_data) throw Error("null index or refiner");
     if (!config || !out_server) throw Error("null argument");
     if (config->slots < 1 || config->slots > 64) throw Error("slots must be in 1..64");
-    // Graph replay is switched off: one serving run with it (4 slots under
-    // load) met an illegal memory access whose cause has not been found.
-    if (config->graphs != 0)
-      throw Error("captured-graph replay in the request loop is switched off (graphs must be 0): a loaded serving "
-                  "run with it faulted and the cause is not found");
     *out_server = nullptr;
     std::unique_ptr<Server> sv(new Server());
     sv->ix = (IndexImpl*)index->private_data;
@@ -1445,6 +1506,7 @@ int SjhipServerSubmit(void* server, const struct ArrowSchema* probe_schema, cons
       throw Error("probe array must be in ROCm device memory or CPU memory");
     std::unique_ptr<ServerReq> r(new ServerReq());
     r->in = wkb_input_from_arrow(probe_schema, &probe_array->array, dev);
+    wait_sync_event(probe_array);
     r->opt = *options;
     r->estimate = sv->estimate(r->in, r->opt);
     {
@@ -1622,6 +1684,7 @@ int SjhipRefinerPushBuildDevice(struct SedonaSpatialRefiner* refiner, const stru
     RefinerImpl* rf = refiner_of(refiner);
     if (!array) throw Error("null array");
     bool dev = array->device_type == ARROW_DEVICE_ROCM;
+    wait_sync_event(array);
     refiner_push(rf, wkb_input_from_arrow(schema, &array->array, dev));
   });
 }

@@ -107,13 +107,14 @@ SJ_HD int raycast(double ax, double ay, double bx, double by, double px, double 
 // and the FP64 rounding the reference's own arithmetic can commit).
 // Returns kRayOut / kRayIn when the reference's FP64 decision is certain,
 // -1 when it is not (the caller then runs raycast() in FP64).
-__device__ __forceinline__ int raycast_f32_certified(float ax, float ay, float bx, float by, float px, float py,
-                                                     float E) {
-  float lo = fminf(ay, by), hi = fmaxf(ay, by);
+// Host and device compile this one definition (the kernels call it on
+// mirrored and transposed coordinates for rays toward -x, +y and -y).
+SJ_HD int raycast_f32_certified(float ax, float ay, float bx, float by, float px, float py, float E) {
+  float lo = ay < by ? ay : by, hi = ay > by ? ay : by;
   float e2 = 2.0f * E;
   if (py < lo - e2 || py > hi + e2) return kRayOut;
   if (!(py > lo + e2 && py < hi - e2)) return -1;  // near a vertex height
-  float xl = fminf(ax, bx), xh = fmaxf(ax, bx);
+  float xl = ax < bx ? ax : bx, xh = ax > bx ? ax : bx;
   if (px < xl - e2) return kRayIn;
   if (px > xh + e2) return kRayOut;
   // a below b

@@ -7,38 +7,13 @@
 
 using namespace sjhip;
 
-namespace {
-// host copy of the certified FP32 test (the device one is __device__ only
-// because it uses device fminf/fmaxf overloads)
-int raycast_f32_host(float ax, float ay, float bx, float by, float px, float py, float E) {
-  float lo = ay < by ? ay : by, hi = ay > by ? ay : by;
-  float e2 = 2.0f * E;
-  if (py < lo - e2 || py > hi + e2) return kRayOut;
-  if (!(py > lo + e2 && py < hi - e2)) return -1;
-  float xl = ax < bx ? ax : bx, xh = ax > bx ? ax : bx;
-  if (px < xl - e2) return kRayIn;
-  if (px > xh + e2) return kRayOut;
-  if (by < ay) {
-    float t = ax; ax = bx; bx = t;
-    t = ay; ay = by; by = t;
-  }
-  float d1 = py - ay, d2 = bx - ax, d3 = by - ay, d4 = px - ax;
-  float det = d1 * d2 - d3 * d4;
-  float a1 = fabsf(d1), a2 = fabsf(d2), a3 = fabsf(d3), a4 = fabsf(d4);
-  const float u = 5.9604645e-08f;
-  float bound = 3.0f * E * (a1 + a2 + a3 + a4) + 8.0f * u * (a1 * a2 + a3 * a4) + 8.0f * E * E;
-  if (det > bound) return kRayIn;
-  if (det < -bound) return kRayOut;
-  return -1;
-}
-}  // namespace
 
 extern "C" {
 int SjhipHostRaycast(double ax, double ay, double bx, double by, double px, double py) {
   return raycast(ax, ay, bx, by, px, py);
 }
 int SjhipHostRaycastF32(float ax, float ay, float bx, float by, float px, float py, float e) {
-  return raycast_f32_host(ax, ay, bx, by, px, py, e);
+  return raycast_f32_certified(ax, ay, bx, by, px, py, e);
 }
 int SjhipHostCollinear(double x1, double y1, double x2, double y2, double x3, double y3) {
   return collinear(x1, y1, x2, y2, x3, y3) ? 1 : 0;

@@ -480,8 +480,17 @@ void rects_of_points_into(const double2* xy, const uint8_t* type, int64_t n, boo
                           float4* out, hipStream_t s);
 void gather_points_into(const double2* xy, const uint32_t* order, int64_t n, double2* out, hipStream_t s);
 
+// Stable LSD radix sort of (u32 key, u32 value) pairs over bits
+// [begin_bit, end_bit), n <= 2^24, made of kernels only (no memset,
+// allocation or host synchronisation): a captured graph may replay it.
+// `temp` holds graph_sort_temp_bytes(n) bytes; inputs are not modified.
+size_t graph_sort_temp_bytes(int64_t n);
+void graph_sort_pairs(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout, int64_t n,
+                      int begin_bit, int end_bit, void* temp, size_t temp_bytes, hipStream_t s);
+
 // Morton order of n query rects in the tree's frame (invalid rects last):
-// keys/ids and their sorted copies, with `temp` of sort_temp_bytes(n).
+// keys/ids and their sorted copies, with `temp` of
+// morton_sort_temp_bytes(n).  Kernels only (graph_sort_pairs).
 size_t morton_sort_temp_bytes(int64_t n);
 void morton_order_fixed(const RectTree& t, const float4* rects, int64_t n, uint32_t* keys, uint32_t* ids,
                         uint32_t* keys_sorted, uint32_t* order, void* temp, size_t temp_bytes, hipStream_t s);

@@ -196,9 +196,10 @@ EXPORTED = [
     "SjhipComputeRects", "SjhipDeviceAlloc", "SjhipDeviceFree", "SjhipCopyToDevice", "SjhipCopyToHost",
     "SjhipSetDevice", "SjhipCopyKernel",
     "SjhipServerCreate", "SjhipServerSubmit", "SjhipServerWait", "SjhipServerDestroy", "SjhipRefinerKnnDevice",
-    "SjhipAutotune", "SjhipRefinerSetSegmentLeaf", "SjhipIndexSetTuning",
+    "SjhipAutotune", "SjhipRefinerSetSegmentLeaf", "SjhipIndexSetTuning", "SjhipRefinerSetPrecision",
     "SjhipHostRaycast", "SjhipHostRaycastF32", "SjhipHostCollinear", "SjhipHostSegmentIntersects",
     "SjhipHostOrient2d", "SjhipHostHypot", "SjhipHostCoordToLine", "SjhipHostRaycastBatch", "SjhipHostHypotBatch",
+    "SjhipTestingGraphSortPairs",
 ]
 
 _lib = None
@@ -234,6 +235,7 @@ def lib():
                                     ctypes.POINTER(SjhipTuneResult)]
         L.SjhipRefinerSetSegmentLeaf.argtypes = [_RF, ctypes.c_int32]
         L.SjhipIndexSetTuning.argtypes = [_IX, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+        L.SjhipRefinerSetPrecision.argtypes = [_RF, ctypes.c_int32]
         L.SjhipRefinerKnnDevice.argtypes = [_RF, _SCH, ctypes.POINTER(ArrowDeviceArray), ctypes.c_uint32,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.SjhipIndexBuildFromRefiner.argtypes = [_IX, _RF]
@@ -259,5 +261,7 @@ def lib():
         L.SjhipHostCoordToLine.restype = ctypes.c_double
         L.SjhipHostRaycastBatch.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
         L.SjhipHostHypotBatch.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+        L.SjhipTestingGraphSortPairs.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                                                         ctypes.c_int]
         _lib = L
     return _lib

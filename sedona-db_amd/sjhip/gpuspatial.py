@@ -172,6 +172,16 @@ class DeviceWkb:
         return DeviceWkb(self.bufs[1] + 8 * start, self.bufs[2], stop - start, device_id=self.dev.device_id,
                          keepalive=self)
 
+    def arrow_slice(self, start, stop):
+        """Rows [start, stop) as an Arrow slice: the same buffers with the
+        array's `offset` set to start (the form pyarrow's slice exports)."""
+        start, stop = int(start), int(stop)
+        if not (0 <= start <= stop <= self.dev.array.length):
+            raise ValueError("slice out of range")
+        out = DeviceWkb(self.bufs[1], self.bufs[2], stop - start, device_id=self.dev.device_id, keepalive=self)
+        out.dev.array.offset = self.dev.array.offset + start
+        return out
+
     @classmethod
     def from_host(cls, col, device_id=0):
         """Copies a datagen.WkbColumn (int64 offsets + data) into device
@@ -398,6 +408,14 @@ class GpuSpatialRefiner:
         """Rebuilds the ring index with 16 or 32 segments per tree leaf."""
         lib = self.lib
         rc = lib.SjhipRefinerSetSegmentLeaf(ctypes.byref(self.rf), int(segments_per_leaf))
+        if rc:
+            raise GpuSpatialError(lib.SjhipLastError().decode())
+
+    def set_precision(self, precision):
+        """0: certified FP32 with FP64 fallback (default); 1: all-FP64 plain
+        kernels, for refine and refine_distance."""
+        lib = self.lib
+        rc = lib.SjhipRefinerSetPrecision(ctypes.byref(self.rf), int(precision))
         if rc:
             raise GpuSpatialError(lib.SjhipLastError().decode())
 

@@ -215,8 +215,9 @@ def test_corpus_all_pairs_match_reference(precision, opts):
     rf.init_build_schema(b)
     rf.push_build(b)
     rf.finish_building()
+    rf.set_precision(precision)
     for pred in ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"]:
-        say("corpus", pred)
+        say("corpus", pred, "precision", precision)
         want = np.unpackbits(E[f"corpus/{pred}/mask"])[: len(bi)].astype(bool)
         ob, op = rf.refine(p, pred, bi, pi)
         eq, miss, extra = pairs_equal((ob, op), (bi[want], pi[want]))
@@ -225,6 +226,7 @@ def test_corpus_all_pairs_match_reference(precision, opts):
     rrf.init_build_schema(p)
     rrf.push_build(p)
     rrf.finish_building()
+    rrf.set_precision(precision)
     for pred in ["intersects", "within", "coveredby", "touches", "equals"]:
         want = np.unpackbits(E[f"corpus_rev/{pred}/mask"])[: len(bi)].astype(bool)
         ob, op = rrf.refine(b, pred, pi, bi)

@@ -135,8 +135,6 @@ def test_request_error_is_reported_by_wait(resident):
         sv.close()
 
 
-@pytest.mark.skip(reason="captured-graph replay is switched off in the library: a loaded 4-slot serving run with "
-                         "it met an illegal memory access whose cause is not found")
 def test_graph_replays_equal_eager_joins(resident):
     """With captured graphs, every replay (three rounds over the same size
     buckets, with eager joins in between) returns each request's pairs as
@@ -158,5 +156,26 @@ def test_graph_replays_equal_eager_joins(resident):
             gb, gp = alone(R, q)
             assert np.array_equal(gb, want[rnd][0]) and np.array_equal(gp, want[rnd][1])
             say("graph round", rnd, "ok")
+    finally:
+        sv.close()
+
+
+def test_graph_replays_honour_arrow_offsets(resident):
+    """Requests given as Arrow slices (array offset != 0) replay the captured
+    sequence over rows [offset, offset + length)."""
+    R = resident
+    picks = [0, 4, 8, 2]
+    want = [alone(R, R["reqs"][k]) for k in picks]
+    sv = G.GpuSpatialServer(R["ix"], R["rf"], slots=2, graphs=True)
+    try:
+        for rnd in range(2):
+            ids = [sv.submit(R["dev"].arrow_slice(R["reqs"][k]["lo"], R["reqs"][k]["hi"]), R["reqs"][k]["pred"],
+                             distance=R["reqs"][k]["d"], sort_output=True) for k in picks]
+            for j, rid in enumerate(ids):
+                res, _ = sv.wait(rid)
+                gb, gp = res.to_host()
+                res.release()
+                assert np.array_equal(gb, want[j][0]) and np.array_equal(gp, want[j][1]), (rnd, picks[j], len(gb))
+            say("offset round", rnd, "ok")
     finally:
         sv.close()

@@ -146,10 +146,22 @@ def test_fp32_certified_raycast_never_contradicts_fp64():
         if rng.random() < 0.1:
             p[1] = a[1]
         E = np.float32(H * 2.384185791015625e-07) * np.float32(1.0001)
-        loc = [np.float32(v) for v in (a[0] - o[0], a[1] - o[1], b[0] - o[0], b[1] - o[1], p[0] - o[0], p[1] - o[1])]
+        # the kernels also cast the ray toward -x (x mirrored), +y and -y
+        # (x and y swapped, then mirrored): both maps are exact
+        sx = 1.0 if rng.random() < 0.5 else -1.0
+        swap = rng.random() < 0.5
+        pts = [(a[0] - o[0], a[1] - o[1]), (b[0] - o[0], b[1] - o[1]), (p[0] - o[0], p[1] - o[1])]
+        if swap:
+            pts = [(y, x) for x, y in pts]
+        pts = [(sx * x, y) for x, y in pts]
+        loc = [np.float32(v) for q in pts for v in q]
         f = lib.SjhipHostRaycastF32(*[float(x) for x in loc], float(E))
         if f < 0:
             continue
         decided += 1
-        assert f == lib.SjhipHostRaycast(a[0], a[1], b[0], b[1], p[0], p[1])
+        if swap:
+            want = lib.SjhipHostRaycast(sx * a[1], a[0], sx * b[1], b[0], sx * p[1], p[0])
+        else:
+            want = lib.SjhipHostRaycast(sx * a[0], a[1], sx * b[0], b[1], sx * p[0], p[1])
+        assert f == want
     assert decided > 20000
