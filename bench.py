@@ -13,9 +13,9 @@ index, rect tree) is built once before the timed region and reported as
 build_ms.  Inputs are resident in HBM when the timed region starts.
 
 The CPU baseline is the reference's own TG refine (oracle/_ref, built from
-the reference sources) behind a grid filter, timed on this host's cores over
-a bounded vertical strip of the same workload; the GPU's full-size pair set
-restricted to that strip must equal the CPU pairs exactly.
+the reference sources) behind a grid filter, run over the whole workload on
+this host's usable CPUs (at most 16); its pair set must equal the GPU's pair
+for pair.  A one-thread rate is timed on a narrow vertical strip.
 """
 import argparse
 import json
@@ -46,8 +46,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the full workload (both sides)")
     ap.add_argument("--precision", type=int, default=0, help="0 = FP32 certified + FP64 fallback, 1 = all FP64")
-    ap.add_argument("--cpu-strip", type=float, default=0.12, help="CPU sample: strip width / extent")
+    ap.add_argument("--cpu-strip", type=float, default=0.12,
+                    help="CPU sample of the other workloads (all threads): strip width / extent")
+    ap.add_argument("--cpu-strip-1t", type=float, default=0.01,
+                    help="pip_zipf_contains: one-thread CPU sample, strip width / extent")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu-full", dest="cpu_full", action="store_false",
+                    help="pip_zipf_contains: skip the reference CPU join of the whole workload (all threads)")
     ap.add_argument("--no-plain", action="store_true", help="skip the all-FP64 comparison step")
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--seed", type=int, default=2)
@@ -115,7 +120,7 @@ def main():
     if not args.no_cpu:
         ext = wl["extent"]
         x0 = 0.5 * ext
-        w = args.cpu_strip * ext
+        w = args.cpu_strip_1t * ext
         brows = np.nonzero((wl["centers"][:, 0] >= x0) & (wl["centers"][:, 0] < x0 + w))[0]
         prows = np.nonzero((wl["probe_xy"][:, 0] >= x0) & (wl["probe_xy"][:, 0] < x0 + w))[0]
         strip = dict(brows=brows, prows=prows, bcol=D.take_rows(build, brows), pcol=D.take_rows(probe, prows),
@@ -163,9 +168,9 @@ def main():
         tuned["sample_points"] = int(n_points)
         log(f"autotune (all {n_points} points): {tuned}")
 
-    def step(precision, profile=True):
-        return G.join_device(ix, rf, pdev, "contains", precision=precision, sort_output=False, sort_probes=True,
-                             refine_order=-1 if args.autotune else args.refine_order,
+    def step(precision, profile=True, sort_output=False):
+        return G.join_device(ix, rf, pdev, "contains", precision=precision, sort_output=sort_output,
+                             sort_probes=True, refine_order=-1 if args.autotune else args.refine_order,
                              profile=profile)
 
     for i in range(args.warmup):
@@ -191,10 +196,15 @@ def main():
     log(f"timed: {ms_per_step:.2f} ms/step, {pairs} pairs, stages(ms) decode {stage[0]:.2f} morton {stage[1]:.2f} "
         f"filter {stage[2]:.2f} refine {stage[3]:.2f}")
 
-    gb_pairs = gp_pairs = None
-    if strip is not None:
-        gb_pairs, gp_pairs = last.to_host()
     last.release()
+    # untimed: the same join with its pairs sorted by (probe, build) on the
+    # device, for the exact pair-set comparisons below
+    t_sorted = time.time()
+    r = step(args.precision, profile=False, sort_output=True)
+    sorted_ms = (time.time() - t_sorted) * 1000
+    gb_pairs, gp_pairs = r.to_host()
+    r.release()
+    assert len(gb_pairs) == pairs
 
     plain = None
     if not args.no_plain:
@@ -203,9 +213,15 @@ def main():
         r1b = step(1 - args.precision)
         plain_ms = (time.time() - t1) * 1000
         plain = dict(precision=1 - args.precision, ms_per_step=plain_ms, pairs=int(r1b.size),
-                     stage_ms=list(r1b.res.stage_ms), same_pair_count=bool(r1b.size == pairs))
+                     stage_ms=list(r1b.res.stage_ms))
         r1.release()
         r1b.release()
+        r1c = step(1 - args.precision, profile=False, sort_output=True)
+        pb, pp = r1c.to_host()
+        r1c.release()
+        plain["pair_set_diff"] = pair_set_diff(gb_pairs, gp_pairs, pb, pp)
+        del pb, pp
+        log(f"all-FP64 plain: {plain_ms:.1f} ms/step, pair-set diff {plain['pair_set_diff']}")
 
     host = None
     if args.host_inclusive:
@@ -231,7 +247,7 @@ def main():
     verts32 = 8 * (int(wl["counts"].sum()) + n_polys)
     refine_bytes = 8 * (walked if order_used == 1 else cands) + 8 * pairs + verts32
     refine_flops = FLOPS_PER_SEGMENT_TEST * seg_tests
-    traffic = read_traffic(os.path.join(ROOT, "profiles", "traffic_pip_zipf_100Mx1M_contains.md"))
+    traffic, traffic_source = read_traffic(os.path.join(ROOT, "profiles", "traffic_pip_zipf_100Mx1M_contains.md"))
     full = args.scale == 1.0 and args.precision == 0
     roof_filter = dict(bound="hbm", achieved=filter_bytes / (filt_ms / 1e3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
                        traffic=traffic.get("k_filter") if full else None, kernel="k_filter_wave", ms=filt_ms,
@@ -262,7 +278,8 @@ def main():
 
     cpu = None
     if strip is not None:
-        cpu = cpu_baseline(strip, gb_pairs, gp_pairs, args)
+        cpu = cpu_baseline_c2(build, probe, strip, gb_pairs, gp_pairs, args)
+    del gb_pairs, gp_pairs
 
     out = {
         "metric": "refined join pairs/s (ST_Contains, 100M points x 1M Zipf star polygons, device-resident)",
@@ -280,9 +297,13 @@ def main():
         "config": {"workload": "pip_zipf_contains", "probe_points": n_points, "build_polygons": n_polys,
                    "build_vertices": int(wl["counts"].sum()) + n_polys, "predicate": "ST_Contains",
                    "precision": "fp32-certified+fp64-fallback" if args.precision == 0 else "fp64-plain",
+                   "output": "pairs in refine order (an inner join needs no order; the reference refiner "
+                             "sorts by probe: see sorted_output_ms_per_step)",
                    "parallelism": "single-gpu"},
+        "sorted_output_ms_per_step": sorted_ms,
         "roofline": {k: dominant[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
         "roofline_kernel": dominant["kernel"],
+        "roofline_traffic_source": traffic_source,
         "roofline_valu_fp32": dominant.get("valu_fp32"),
         "roofline_stages": {"filter": roof_filter, "refine": roof_refine},
         "cpu_baseline": cpu,
@@ -305,6 +326,107 @@ def main():
         out["refine_speedup_vs_plain_fp64"] = plain["stage_ms"][3] / stage[3]
     print(json.dumps(out), flush=True)
     D.release_shared(wl["keep"])
+
+
+def pair_set_diff(b1, p1, b2, p2):
+    """Size of the symmetric difference of two pair sets, each sorted by
+    (probe, build) without repeats."""
+    if len(b1) == len(b2) and np.array_equal(p1, p2) and np.array_equal(b1, b2):
+        return 0
+    k1 = (p1.astype(np.uint64) << np.uint64(32)) | b1.astype(np.uint64)
+    k2 = (p2.astype(np.uint64) << np.uint64(32)) | b2.astype(np.uint64)
+    return int(len(np.setxor1d(k1, k2, assume_unique=True)))
+
+
+def host_cpu_info():
+    """CPU model, logical CPUs and the CPUs this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "cpus_available": avail}
+
+
+def cpu_threads():
+    """Worker threads of the CPU baseline: the CPUs this process may use, at
+    most 16 (one GPU's share of the host)."""
+    info = host_cpu_info()
+    return max(1, min(16, info["cpus_available"] or 1))
+
+
+def cpu_reference_join(bcol, pcol, predicate, threads, cap_hint=0, what="polygons"):
+    """The reference CPU join on host columns: WKB parse (the oracle's SoA
+    for the FP32 rects, the reference TG library for the geometries), the
+    grid filter and the reference TG refine.  (pairs, seconds, kind)."""
+    import oracle as O
+    kind = "reference" if O.ref_available() else "port"
+    t = time.time()
+    if kind == "reference":
+        st_b, st_p = O.Store(bcol.as_tuple()), O.Store(pcol.as_tuple())
+        br, pr = st_b.rects(True), st_p.rects(True)
+        del st_b, st_p
+        tb = O.TgColumn(bcol.as_tuple(), index=1, nthreads=threads)
+        tp = O.TgColumn(pcol.as_tuple(), index=1 if what == "polygons" and predicate == "intersects" else 0,
+                        nthreads=threads)
+        cb, cp = O.ref_join(tb, tp, br, pr, predicate, nthreads=threads, cap_hint=cap_hint)
+        secs = time.time() - t
+        del tb, tp
+    else:
+        st_b, st_p = O.Store(bcol.as_tuple()), O.Store(pcol.as_tuple())
+        cb, cp = O.join(st_b, st_p, predicate, nthreads=threads)
+        secs = time.time() - t
+    return cb, cp, secs, kind
+
+
+def cpu_baseline_c2(build, probe, strip, gb, gp, args):
+    """Reference CPU join of the whole workload on all usable host CPUs (its
+    pair set against the GPU's, pair for pair), and a one-thread rate on a
+    narrow vertical strip of the same workload."""
+    threads = cpu_threads()
+    res = host_cpu_info()
+    if args.cpu_full:
+        cb, cp, secs, kind = cpu_reference_join(build, probe, "contains", threads, cap_hint=len(gb) + (1 << 20))
+        res.update({"value": len(cb) / secs, "unit": "pairs/s", "cores": threads, "kind": kind,
+                    "sample": f"the whole workload: {len(build)} polygons x {len(probe)} points, {len(cb)} pairs, "
+                              f"{secs:.2f} s (WKB parse + filter + refine)",
+                    "cpu_pairs": int(len(cb)), "gpu_pairs": int(len(gb)),
+                    "pair_set_diff": pair_set_diff(gb, gp, cb, cp)})
+        log(f"cpu reference, whole workload, {threads} threads: {len(cb)} pairs in {secs:.1f} s, "
+            f"pair-set diff {res['pair_set_diff']}")
+        del cb, cp
+    cb, cp, secs, kind = cpu_reference_join(strip["bcol"], strip["pcol"], "contains", 1)
+    one = {"value": len(cb) / secs, "unit": "pairs/s", "cores": 1, "kind": kind,
+           "sample": f"vertical strip x in [{strip['x0']:.1f}, {strip['x0'] + strip['w']:.1f}): "
+                     f"{len(strip['brows'])} polygons x {len(strip['prows'])} points, {len(cb)} pairs, {secs:.2f} s"}
+    one["pair_set_diff"] = strip_diff(strip, gb, gp, cb, cp)
+    log(f"cpu reference, strip, 1 thread: {len(cb)} pairs in {secs:.1f} s")
+    res["one_thread"] = one
+    if not args.cpu_full:
+        res.update({k: one[k] for k in ("value", "unit", "cores", "kind", "sample", "pair_set_diff")})
+    return res
+
+
+def strip_diff(strip, gb, gp, cb, cp):
+    """Pair-set difference between the GPU's full pairs restricted to the
+    strip's rows and the CPU pairs of the strip (local row numbers)."""
+    bmap = np.full(int(gb.max()) + 1 if len(gb) else 1, -1, np.int64)
+    sel_b = strip["brows"]
+    bmap[sel_b[sel_b < len(bmap)]] = np.nonzero(sel_b < len(bmap))[0]
+    pset = np.full(int(gp.max()) + 1 if len(gp) else 1, -1, np.int64)
+    sel_p = strip["prows"]
+    pset[sel_p[sel_p < len(pset)]] = np.nonzero(sel_p < len(pset))[0]
+    lb = bmap[gb]
+    lp = pset[gp]
+    keep = (lb >= 0) & (lp >= 0)
+    gk = np.sort((lp[keep].astype(np.uint64) << 32) | lb[keep].astype(np.uint64))
+    ck = np.sort((cp.astype(np.uint64) << 32) | cb.astype(np.uint64))
+    return int(len(np.setxor1d(gk, ck)))
 
 
 def pair_checksum(b, p, p0=0):
@@ -450,17 +572,39 @@ def serving_measurement(G, ix, rf, serve, args):
             "p50_ms": rates[1]["batched"]["p50_ms"], "p99_ms": rates[1]["batched"]["p99_ms"]}
 
 
+def sources_digest():
+    """First 16 hex digits of the SHA-256 over the library's HIP sources
+    (sedona-db_amd/csrc, files in name order): the counter summaries record
+    the digest of the sources they were measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    d = os.path.join(ROOT, "sedona-db_amd", "csrc")
+    for name in sorted(os.listdir(d)):
+        h.update(name.encode())
+        with open(os.path.join(d, name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def read_traffic(path):
     """HBM bytes per launch of each profiled kernel: lines 'traffic <kernel> <bytes>'
     of the committed counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the
-    copy-kernel calibration)."""
-    out = {}
+    copy-kernel calibration), and a note on where they come from.  The
+    summary's 'sources <digest>' line must match the sources being run, else
+    no traffic is reported (the counters describe other kernels)."""
+    out, digest = {}, None
     if os.path.exists(path):
         for line in open(path):
             f = line.split()
             if len(f) == 3 and f[0] == "traffic":
                 out[f[1]] = int(f[2])
-    return out
+            elif len(f) == 2 and f[0] == "sources":
+                digest = f[1]
+    rel = os.path.relpath(path, ROOT)
+    now = sources_digest()
+    if digest != now:
+        return {}, f"none: {rel} was measured on sources {digest}, these are {now}"
+    return out, f"{rel} (rocprofv3 FETCH_SIZE + WRITE_SIZE, sources {now})"
 
 
 def calibrate_copy(nbytes=1 << 30, reps=20):

@@ -250,12 +250,20 @@ def ref_refine(build, probe, pred, bidx, pidx, nthreads=0):
     return mask[:len(bidx)].astype(bool)
 
 
-def ref_join(build, probe, brects, prects, pred, nthreads=0):
+def ref_join(build, probe, brects, prects, pred, nthreads=0, cap_hint=0):
+    """Pairs sorted by (probe, build).  With cap_hint >= the pair count the
+    join runs once; otherwise a counting run comes first."""
     br = np.ascontiguousarray(brects, np.float32)
     pr = np.ascontiguousarray(prects, np.float32)
     R = reflib()
-    n = R.tgref_join(build.h, probe.h, _ptr(br), _ptr(pr), PRED[pred], None, None, 0, nthreads)
-    ob = np.zeros(max(n, 1), np.uint32)
-    op = np.zeros(max(n, 1), np.uint32)
-    R.tgref_join(build.h, probe.h, _ptr(br), _ptr(pr), PRED[pred], _ptr(ob), _ptr(op), n, nthreads)
+    cap = int(cap_hint)
+    if cap <= 0:
+        cap = R.tgref_join(build.h, probe.h, _ptr(br), _ptr(pr), PRED[pred], None, None, 0, nthreads)
+    ob = np.zeros(max(cap, 1), np.uint32)
+    op = np.zeros(max(cap, 1), np.uint32)
+    n = R.tgref_join(build.h, probe.h, _ptr(br), _ptr(pr), PRED[pred], _ptr(ob), _ptr(op), cap, nthreads)
+    if n < 0:
+        raise ValueError("unsupported predicate for TG")
+    if n > cap:
+        return ref_join(build, probe, brects, prects, pred, nthreads, cap_hint=n)
     return ob[:n], op[:n]
