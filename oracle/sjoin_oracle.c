@@ -657,9 +657,9 @@ static int ring_intersects_ring(const ring_t* ring, const ring_t* other) {
   return 0;
 }
 
-/* tg.c:3119-3151 tg_ring_contains_ring restricted to the convex/concave
- * dispatch used by poly x poly intersects; only needed when holes exist. */
-static int ring_contains_ring_simple(const ring_t* a, const ring_t* b);
+/* tg.c:3119-3151 tg_ring_contains_ring(a, b, allow_on_edge = false), the
+ * hole test of poly x poly intersects. */
+static int ring_contains_ring_strict(const ring_t* a, const ring_t* b);
 
 /* tg.c:4100-4136 */
 static int poly_intersects_poly(const poly_t* p, const poly_t* o) {
@@ -671,37 +671,82 @@ static int poly_intersects_poly(const poly_t* p, const poly_t* o) {
   if (!ring_intersects_ring(&oe, &pe)) return 0;
   for (int64_t k = 1; k < np; k++) {
     ring_t h = poly_ring(p, k);
-    if (ring_contains_ring_simple(&h, &oe)) return 0;
+    if (ring_contains_ring_strict(&h, &oe)) return 0;
   }
   for (int64_t k = 1; k < no; k++) {
     ring_t h = poly_ring(o, k);
-    if (ring_contains_ring_simple(&h, &pe)) return 0;
+    if (ring_contains_ring_strict(&h, &pe)) return 0;
   }
   return 1;
 }
 
-/* Holes inside polygon x polygon: this restatement supports the hole test
- * only through the exact-vertex route (every vertex of b strictly inside a
- * and no crossings), which is what tg_ring_contains_ring(.., false) reduces
- * to for the non-degenerate inputs of the test corpus.  It is marked parity
- * unpinned where documented. */
-static int ring_contains_ring_simple(const ring_t* a, const ring_t* b) {
+/* tg.c:1910-2120 (process_points): the convex flag TG keeps per ring.  Every
+ * point takes part, the closing one included, with its two successors taken
+ * modulo npoints; the first turn's sign sets the direction (a zero turn
+ * counts as counter-clockwise), later zero turns are ignored, and the first
+ * turn against the direction makes the ring concave. */
+static int ring_convex_tg(const ring_t* r) {
+  int64_t n = r->npts;
+  if (n < 2) return 0;
+  int dir = 0;
+  for (int64_t i = 0; i < n; i++) {
+    int64_t j = i + 1 < n ? i + 1 : i + 1 - n;
+    int64_t k = i + 2 < n ? i + 2 : i + 2 - n;
+    double ax = r->xy[2 * i], ay = r->xy[2 * i + 1];
+    double bx = r->xy[2 * j], by = r->xy[2 * j + 1];
+    double cx = r->xy[2 * k], cy = r->xy[2 * k + 1];
+    double z = (bx - ax) * (cy - by) - (by - ay) * (cx - bx);
+    if (dir == 0) {
+      dir = z < 0 ? -1 : 1;
+    } else if (z < 0) {
+      if (dir == 1) return 0;
+    } else if (z > 0) {
+      if (dir == -1) return 0;
+    }
+  }
+  return 1;
+}
+
+/* tg.c:2910-3016 tg_ring_contains_segment with allow_on_edge = false: the
+ * ring's rect covers the segment's, both endpoints lie strictly inside (a
+ * zero-length segment needs one), and, for a concave ring, no ring segment
+ * meets it (contsegiter3, tg.c:2899-2907; the rect search of tg_ring_search
+ * drops only segments that tg_segment_intersects_segment rejects anyway). */
+static int ring_contains_segment_strict(const ring_t* r, const double* rr, const double* s, int convex) {
+  double sr[4] = {fmin0(s[0], s[2]), fmin0(s[1], s[3]), fmax0(s[0], s[2]), fmax0(s[1], s[3])};
+  if (sr[0] < rr[0] || sr[2] > rr[2] || sr[1] < rr[1] || sr[3] > rr[3]) return 0;
+  if (!ring_contains_point(r, s[0], s[1], 0)) return 0;
+  if (feq(s[2], s[0]) && feq(s[3], s[1])) return 1;
+  if (!ring_contains_point(r, s[2], s[3], 0)) return 0;
+  if (convex) return 1;
+  int64_t ns = ring_nsegs(r);
+  for (int64_t i = 0; i < ns; i++) {
+    double t[4];
+    ring_seg(r, i, t);
+    if (seg_intersects_seg(s, t)) return 0;
+  }
+  return 1;
+}
+
+/* tg.c:3119-3151: a convex ring contains b when every point of b lies
+ * strictly inside it, a concave one when every segment of b does
+ * (ring_contains_segment_strict). */
+static int ring_contains_ring_strict(const ring_t* a, const ring_t* b) {
   if (ring_empty(a) || ring_empty(b)) return 0;
   double ra[4], rb[4];
   ring_rect(a, ra);
   ring_rect(b, rb);
   if (rb[0] < ra[0] || rb[2] > ra[2] || rb[1] < ra[1] || rb[3] > ra[3]) return 0;
-  for (int64_t i = 0; i < b->npts; i++)
-    if (!ring_contains_point(a, b->xy[2 * i], b->xy[2 * i + 1], 0)) return 0;
-  int64_t na = ring_nsegs(a), nb = ring_nsegs(b);
+  if (ring_convex_tg(a)) {
+    for (int64_t i = 0; i < b->npts; i++)
+      if (!ring_contains_point(a, b->xy[2 * i], b->xy[2 * i + 1], 0)) return 0;
+    return 1;
+  }
+  int64_t nb = ring_nsegs(b);
   for (int64_t i = 0; i < nb; i++) {
     double s[4];
     ring_seg(b, i, s);
-    for (int64_t j = 0; j < na; j++) {
-      double t[4];
-      ring_seg(a, j, t);
-      if (seg_intersects_seg(s, t)) return 0;
-    }
+    if (!ring_contains_segment_strict(a, ra, s, 0)) return 0;
   }
   return 1;
 }

@@ -762,24 +762,66 @@ __device__ bool iring_intersects_iring(IRing ring, IRing other) {
   return false;
 }
 
-// tg_ring_contains_ring(hole, ring, false) as used by poly x poly
-// intersects: every vertex strictly inside and no segment crossing.
+// tg.c:1910-2120 (process_points): the convex flag TG keeps per ring.
+// Every point takes part, the closing one included, with its two successors
+// taken modulo npoints; the first turn's sign sets the direction (a zero
+// turn counts as counter-clockwise), later zero turns are ignored, and the
+// first turn against the direction makes the ring concave.
+__device__ bool ring_convex_tg(const RingRef& r) {
+  const int64_t n = r.npts;
+  if (n < 2) return false;
+  int dir = 0;
+  for (int64_t i = 0; i < n; i++) {
+    const int64_t j = i + 1 < n ? i + 1 : i + 1 - n;
+    const int64_t k = i + 2 < n ? i + 2 : i + 2 - n;
+    const double2 a = r.xy[i], b = r.xy[j], c = r.xy[k];
+    const double z = (b.x - a.x) * (c.y - b.y) - (b.y - a.y) * (c.x - b.x);
+    if (dir == 0) {
+      dir = z < 0 ? -1 : 1;
+    } else if (z < 0) {
+      if (dir == 1) return false;
+    } else if (z > 0) {
+      if (dir == -1) return false;
+    }
+  }
+  return true;
+}
+
+// Point strictly inside ring a (tg_ring_contains_point(a, p, false).hit).
+__device__ __forceinline__ bool ring_holds_point_strict(const RingRef& a, const double4& ra, double px, double py) {
+  if (!rect_covers(ra, px, py)) return false;
+  int on = 0, in = 0;
+  uint64_t t = 0;
+  ring_segments_f64(a, 0, a.npts - 1, 1, px, py, on, in, t);
+  return !on && in;
+}
+
+// tg_ring_contains_ring(hole, ring, false) as poly x poly intersects uses
+// it (tg.c:3119-3151): a convex hole contains the ring when every point of
+// the ring lies strictly inside it; a concave one when every segment does
+// (tg_ring_contains_segment(.., false), tg.c:2910-3016: the hole's rect
+// covers the segment's, both endpoints strictly inside, a zero-length
+// segment needing one, and no hole segment meets it).
 __device__ bool ring_contains_ring_strict(const RingRef& a, const RingRef& b) {
   if (a.npts < 3 || b.npts < 3) return false;
   double4 ra = ring_rect_scan(a), rb = ring_rect_scan(b);
   if (rb.x < ra.x || rb.z > ra.z || rb.y < ra.y || rb.w > ra.w) return false;
-  for (int64_t i = 0; i < b.npts; i++) {
-    double2 p = b.xy[i];
-    int on = 0, in = 0;
-    uint64_t t = 0;
-    if (!rect_covers(ra, p.x, p.y)) return false;
-    ring_segments_f64(a, 0, a.npts - 1, 1, p.x, p.y, on, in, t);
-    if (on || !in) return false;
+  if (ring_convex_tg(a)) {
+    for (int64_t i = 0; i < b.npts; i++) {
+      const double2 p = b.xy[i];
+      if (!ring_holds_point_strict(a, ra, p.x, p.y)) return false;
+    }
+    return true;
   }
   for (int64_t i = 0; i + 1 < b.npts; i++) {
-    double2 p = b.xy[i], q = b.xy[i + 1];
+    const double2 p = b.xy[i], q = b.xy[i + 1];
+    const double sx0 = fmin0(p.x, q.x), sy0 = fmin0(p.y, q.y), sx1 = fmax0(p.x, q.x), sy1 = fmax0(p.y, q.y);
+    if (sx0 < ra.x || sx1 > ra.z || sy0 < ra.y || sy1 > ra.w) return false;
+    if (!ring_holds_point_strict(a, ra, p.x, p.y)) return false;
+    if (feq(q.x, p.x) && feq(q.y, p.y)) continue;
+    if (!ring_holds_point_strict(a, ra, q.x, q.y)) return false;
     for (int64_t j = 0; j + 1 < a.npts; j++) {
-      double2 c = a.xy[j], d = a.xy[j + 1];
+      const double2 c = a.xy[j], d = a.xy[j + 1];
       if (segment_intersects(p.x, p.y, q.x, q.y, c.x, c.y, d.x, d.y)) return false;
     }
   }

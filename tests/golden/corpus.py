@@ -147,3 +147,123 @@ def corpus(seed=11):
     b = build_polygons(seed)
     p = probe_points(b, seed + 1)
     return column(b), column(p)
+
+
+def _star(rng, cx, cy, n, rmin, rmax, grid=0.5, ccw=True):
+    ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+    r = rng.uniform(rmin, rmax, n)
+    pts = [(cx + np.round(ri * np.cos(a) / grid) * grid, cy + np.round(ri * np.sin(a) / grid) * grid)
+           for ri, a in zip(r, ang)]
+    pts = [(float(x), float(y)) for x, y in pts]
+    dedup = [pts[0]]
+    for q in pts[1:]:
+        if q != dedup[-1]:
+            dedup.append(q)
+    if not ccw:
+        dedup = dedup[::-1]
+    dedup.append(dedup[0])
+    return dedup
+
+
+def _square(x0, y0, x1, y1, ccw=True):
+    pts = [(x0, y0), (x1, y0), (x1, y1), (x0, y1), (x0, y0)]
+    return pts if ccw else pts[::-1]
+
+
+def _pentagram(cx, cy, r, ccw=True):
+    """The star polygon {5/2}: the five points of a circle joined every
+    second one (a self-crossing ring)."""
+    pts = []
+    for k in range(5):
+        a = np.pi / 2 + (2 if ccw else -2) * k * 2 * np.pi / 5
+        pts.append((cx + r * np.cos(a), cy + r * np.sin(a)))
+    return pts + [pts[0]]
+
+
+def hole_polygons(seed=17):
+    """Polygons with holes over [0, 16]^2 (vertices on a 0.5 grid): convex
+    holes in both orientations, concave holes (L shape, stars), a hole whose
+    first three points are collinear (TG then treats the convex hole as
+    concave), two holes, holes inside multipolygon parts, and small
+    polygons without holes."""
+    rng = np.random.default_rng(seed)
+    polys = []
+    outer = _square(0, 0, 16, 16)
+    polys.append(polygon_wkb([outer, _square(4, 4, 12, 12)]))
+    polys.append(polygon_wkb([outer, _square(4, 4, 12, 12, ccw=False)]))
+    polys.append(polygon_wkb([outer, [(4, 4), (12, 4), (12, 8), (8, 8), (8, 12), (4, 12), (4, 4)]]))
+    polys.append(polygon_wkb([outer, [(4, 4), (4, 8), (4, 12), (12, 12), (12, 4), (4, 4)]]))
+    polys.append(polygon_wkb([outer, [(4, 4), (8, 4), (12, 4), (12, 12), (4, 12), (4, 4)]]))
+    polys.append(polygon_wkb([outer, [(8, 3), (13, 8), (8, 13), (3, 8), (8, 3)]]))
+    polys.append(polygon_wkb([outer, _square(2, 2, 7, 7), [(9, 9), (14, 9), (14, 14), (11.5, 11.5), (9, 14), (9, 9)]]))
+    for k in range(6):
+        polys.append(polygon_wkb([outer, _star(rng, 8, 8, int(rng.integers(5, 12)), 2.0, 6.5, ccw=bool(k % 2))]))
+    polys.append(multipolygon_wkb([[_square(0, 0, 7, 7), _square(2, 2, 5, 5)],
+                                   [_square(9, 9, 16, 16), [(10, 10), (15, 10), (12.5, 12.5), (15, 15), (10, 15), (10, 10)]]]))
+    for _ in range(8):
+        cx, cy = (float(v) for v in rng.integers(2, 15, 2))
+        polys.append(polygon_wkb([_star(rng, cx, cy, int(rng.integers(3, 9)), 0.5, 3.0)]))
+    # pentagram holes: every turn has the same sign, so TG flags them convex
+    # (vertex test only) although their edges cross; a bowtie hole (concave)
+    for ccw in (True, False):
+        polys.append(polygon_wkb([outer, _pentagram(8.0, 8.0, 6.0, ccw)]))
+    polys.append(polygon_wkb([outer, [(4, 4), (12, 12), (12, 4), (4, 12), (4, 4)]]))
+    # general position: an irrational star exterior with a star hole
+    for _ in range(4):
+        cx, cy = rng.uniform(6, 10, 2)
+        n = int(rng.integers(8, 30))
+        ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+        ext = [(cx + r * np.cos(a), cy + r * np.sin(a)) for r, a in zip(rng.uniform(6, 8, n), ang)]
+        hole = [(cx + r * np.cos(a), cy + r * np.sin(a)) for r, a in zip(rng.uniform(1.5, 4.5, n), ang)]
+        polys.append(polygon_wkb([ext + [ext[0]], hole + [hole[0]]]))
+    return polys
+
+
+def hole_probe_polygons(seed=18, n_random=260):
+    """Small polygons around the holes of hole_polygons(): inside, touching
+    at a vertex or along an edge, equal to a hole, crossing a hole's edge
+    or notch, covering a hole; polygons with holes of their own; random
+    grid stars."""
+    rng = np.random.default_rng(seed)
+    polys = []
+    polys.append(polygon_wkb([_square(5, 5, 11, 11)]))
+    polys.append(polygon_wkb([_square(4, 4, 12, 12)]))
+    polys.append(polygon_wkb([_square(4, 4, 12, 12, ccw=False)]))
+    polys.append(polygon_wkb([_square(4, 5, 6, 7)]))
+    polys.append(polygon_wkb([_square(3, 5, 6, 7)]))
+    polys.append(polygon_wkb([_square(3.5, 3.5, 12.5, 12.5)]))
+    polys.append(polygon_wkb([[(5, 5), (11, 5), (5, 11), (5, 5)]]))
+    polys.append(polygon_wkb([[(5, 5), (11, 6), (6, 11), (5, 5)]]))
+    polys.append(polygon_wkb([[(9, 9), (11, 9), (11, 11), (9, 11), (9, 9)]]))
+    polys.append(polygon_wkb([[(5, 9), (9, 5), (7, 11), (5, 9)]]))
+    polys.append(polygon_wkb([[(8, 4), (12, 8), (8, 12), (4, 8), (8, 4)]]))
+    polys.append(polygon_wkb([_square(-1, -1, 17, 17), _square(-0.5, -0.5, 16.5, 16.5)]))
+    polys.append(polygon_wkb([_square(-1, -1, 17, 17), _square(1, 1, 15, 15)]))
+    polys.append(polygon_wkb([_square(1, 1, 15, 15), _square(2, 2, 14, 14)]))
+    polys.append(polygon_wkb([_square(6, 6, 10, 10), _square(7, 7, 9, 9)]))
+    # triangles and quads joining points in the tips of the pentagram holes
+    # (each tip has crossing parity 1, the centre 0): every vertex is inside
+    # the hole while the edges cross it
+    tips = [(8.0 + 4.8 * np.cos(np.pi / 2 + k * 2 * np.pi / 5), 8.0 + 4.8 * np.sin(np.pi / 2 + k * 2 * np.pi / 5))
+            for k in range(5)]
+    for ks in ((0, 2, 3), (0, 1, 2), (1, 3, 4), (0, 1, 2, 3), (4, 2, 0)):
+        ring = [tips[k] for k in ks]
+        polys.append(polygon_wkb([ring + [ring[0]]]))
+    small = [(tips[0][0] + dx, tips[0][1] + dy) for dx, dy in ((-0.2, -0.3), (0.2, -0.3), (0.0, 0.2))]
+    polys.append(polygon_wkb([small + [small[0]]]))
+    for _ in range(n_random):
+        cx, cy = (float(v) for v in rng.integers(0, 33, 2) * 0.5)
+        polys.append(polygon_wkb([_star(rng, cx, cy, int(rng.integers(3, 8)), 0.4, 3.5,
+                                        ccw=bool(rng.integers(0, 2)))]))
+    for _ in range(20):
+        cx, cy = rng.uniform(1, 15, 2)
+        n = int(rng.integers(3, 12))
+        ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+        r = rng.uniform(0.3, 3.0, n)
+        pts = [(cx + ri * np.cos(a), cy + ri * np.sin(a)) for ri, a in zip(r, ang)]
+        polys.append(polygon_wkb([pts + [pts[0]]]))
+    return polys
+
+
+def hole_corpus(seed=17):
+    return column(hole_polygons(seed)), column(hole_probe_polygons(seed + 1))

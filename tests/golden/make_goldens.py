@@ -10,7 +10,8 @@ Inputs (all committed as data):
                      byte-identical to synthetic_pip/polygons.parquet),
                      countries/{natural-earth_countries_geo,
                      generated_points}.parquet)
-  corpus.py       -- the adversarial corpus (seeded)
+  corpus.py       -- the adversarial corpus and the polygon-with-holes
+                     corpus (seeded)
   a small uniform star-polygon workload (seeded, sedona-db_amd/sjhip/datagen.py)
 
 Outputs: for every (build, probe, predicate) the refined (build, probe)
@@ -91,6 +92,22 @@ def main():
     bb = np.repeat(np.arange(nb), nb).astype(np.uint32)
     pp = np.tile(np.arange(nb), nb).astype(np.uint32)
     res["corpus_polypoly/intersects/mask"] = np.packbits(O.ref_refine(tb, tb, "intersects", bb, pp))
+    # polygons with holes x small polygons, every pair, both roles
+    hb, hp = C.hole_corpus()
+    nhb, nhp = len(hb[0]) - 1, len(hp[0]) - 1
+    hbi = np.repeat(np.arange(nhb), nhp).astype(np.uint32)
+    hpi = np.tile(np.arange(nhp), nhb).astype(np.uint32)
+    for ix in (0, 1, 2):
+        thb, thp = O.TgColumn(hb, index=ix), O.TgColumn(hp, index=ix)
+        m = O.ref_refine(thb, thp, "intersects", hbi, hpi)
+        m2 = O.ref_refine(thp, thb, "intersects", hpi, hbi)
+        if ix == 0:
+            res["holes_polypoly/intersects/mask"] = np.packbits(m)
+            res["holes_polypoly_rev/intersects/mask"] = np.packbits(m2)
+        else:
+            assert np.array_equal(np.packbits(m), res["holes_polypoly/intersects/mask"]), "TG index modes disagree"
+            assert np.array_equal(np.packbits(m2), res["holes_polypoly_rev/intersects/mask"])
+    print("holes_polypoly", int(m.sum()), int(m2.sum()))
     # small uniform star workload (1/50 of the 1M x 10k case, same densities)
     build, probe = D.workload_pip_uniform(20_000, 200, seed=7, extent=100.0 * np.sqrt(200 / 10_000))
     ob, op = ref_pairs(build.as_tuple(), probe.as_tuple(), "intersects")

@@ -550,6 +550,59 @@ def test_poly_poly_raster_cases_vs_oracle(opts):
     assert (1, 2) in set(zip(ob.tolist(), op.tolist())) and len(ob) >= 6
 
 
+@pytest.mark.parametrize("rev", [False, True])
+def test_hole_corpus_polypoly_vs_reference_goldens(rev, opts):
+    """Polygons with convex, concave and pentagram holes x small polygons,
+    every pair through the refiner ABI and the fused join, against the
+    reference TG goldens (the holes decided as tg_ring_contains_ring does)."""
+    hb, hp = C.hole_corpus()
+    b, p = corpus_arrow(hb), corpus_arrow(hp)
+    if rev:
+        b, p = p, b
+    nb, np_ = len(b), len(p)
+    bi = np.repeat(np.arange(nb), np_).astype(np.uint32)
+    pi = np.tile(np.arange(np_), nb).astype(np.uint32)
+    E = expected()
+    key = "holes_polypoly_rev" if rev else "holes_polypoly"
+    # the goldens index (hole polygon, small polygon) pairs in build-major order
+    want = np.unpackbits(E[f"{key}/intersects/mask"])[: len(bi)].astype(bool)
+    if rev:
+        want = want.reshape(np_, nb).T.reshape(-1)
+    say("hole corpus", rev, int(want.sum()))
+    rf = G.GpuSpatialRefiner(opts)
+    rf.init_build_schema(b)
+    rf.push_build(b)
+    rf.finish_building()
+    ob, op = rf.refine(p, "intersects", bi, pi)
+    eq, miss, extra = pairs_equal((ob, op), (bi[want], pi[want]))
+    assert eq, (rev, miss, extra)
+    ix = G.GpuSpatialIndex(opts)
+    ix.build_from_refiner(rf)
+    r = G.join_device(ix, rf, p, "intersects", sort_output=True)
+    gb, gp = r.to_host()
+    assert pairs_equal((gb, gp), (bi[want], pi[want]))[0]
+
+
+@pytest.mark.parametrize("d", [0.0, 0.3, 1.5])
+def test_dwithin_hole_polygons_vs_oracle(d, opts):
+    """ST_DWithin of points against polygons with holes and multipolygons:
+    points inside holes, on hole edges and deep inside solid parts."""
+    hb, _ = C.hole_corpus()
+    b = corpus_arrow(hb)
+    g = np.arange(-1.0, 17.01, 0.25)
+    xs, ys = np.meshgrid(g, g)
+    rng = np.random.default_rng(41)
+    px = np.concatenate([xs.ravel(), rng.uniform(-1, 17, 4000)])
+    py = np.concatenate([ys.ravel(), rng.uniform(-1, 17, 4000)])
+    p = D.points_wkb(px, py)
+    ix, rf = build_pair(b, opts)
+    r = G.join_device(ix, rf, p.to_arrow(), "dwithin", distance=d, sort_output=True)
+    gb, gp = r.to_host()
+    ob, op = O.join(O.Store(hb), O.Store(p.as_tuple()), "dwithin", distance=d, nthreads=8)
+    say("hole dwithin", d, len(ob))
+    assert np.array_equal(gb, ob) and np.array_equal(gp, op), (d, len(gb), len(ob))
+
+
 def test_dwithin_polygons_and_refine_distance_abi(opts):
     b, p = D.workload_pip_uniform(20_000, 200, seed=17, extent=100.0 * np.sqrt(200 / 10_000))
     ix, rf = build_pair(b.to_arrow(), opts)

@@ -50,6 +50,23 @@ def test_oracle_matches_reference_on_adversarial_corpus():
     assert np.array_equal(O.refine(B, B, "intersects", bb, pp), want)
 
 
+def test_oracle_hole_containment_matches_reference_on_hole_corpus():
+    """Polygons with convex, concave and self-crossing (pentagram) holes x
+    small polygons, every pair, both roles: TG decides holes in polygon x
+    polygon intersects with tg_ring_contains_ring's convex/concave dispatch
+    (tg.c:3119-3151).  Vertices inside a pentagram hole with edges crossing
+    it are where a vertex-and-crossing test would differ."""
+    hb, hp = C.hole_corpus()
+    B, P = O.Store(hb), O.Store(hp)
+    bi = np.repeat(np.arange(B.n), P.n).astype(np.uint32)
+    pi = np.tile(np.arange(P.n), B.n).astype(np.uint32)
+    E = expected()
+    want = np.unpackbits(E["holes_polypoly/intersects/mask"])[: len(bi)].astype(bool)
+    assert np.array_equal(O.refine(B, P, "intersects", bi, pi), want)
+    want2 = np.unpackbits(E["holes_polypoly_rev/intersects/mask"])[: len(bi)].astype(bool)
+    assert np.array_equal(O.refine(P, B, "intersects", pi, bi), want2)
+
+
 def test_corpus_reaches_every_ray_branch():
     """The corpus must exercise ON results and vertex-height nudges, or the
     pins above would not constrain those branches."""
@@ -228,6 +245,99 @@ def test_distance_against_exact_rationals():
         ex2 = min(_exact_seg_dist2(p[0], p[1], *pts[i], *pts[i + 1]) for i in range(len(pts) - 1))
         ex = float(ex2) ** 0.5
         assert abs(got - ex) <= 1e-12 * max(ex, 1e-300)
+
+
+def _exact_ring_position(ring, px, py):
+    """2 on the boundary, 1 inside, 0 outside, in exact rational arithmetic
+    (crossing parity of the +x ray)."""
+    inside = False
+    for (ax, ay), (bx, by) in zip(ring, ring[1:]):
+        ax, ay, bx, by = (Fraction(v) for v in (ax, ay, bx, by))
+        cross = (bx - ax) * (py - ay) - (by - ay) * (px - ax)
+        if cross == 0 and min(ax, bx) <= px <= max(ax, bx) and min(ay, by) <= py <= max(ay, by):
+            return 2
+        if (ay > py) != (by > py) and px < ax + (py - ay) * (bx - ax) / (by - ay):
+            inside = not inside
+    return 1 if inside else 0
+
+
+def _exact_polygon_dist2(polys, x, y):
+    """Squared distance from (x, y) to a (multi)polygon: 0 on a boundary or
+    inside an exterior and no hole, else the least squared distance to any
+    ring's segments (holes included)."""
+    px, py = Fraction(x), Fraction(y)
+    best = None
+    for rings in polys:
+        st = _exact_ring_position(rings[0], px, py)
+        if st == 2:
+            return Fraction(0)
+        if st == 1:
+            hs = [_exact_ring_position(h, px, py) for h in rings[1:]]
+            if 2 in hs or 1 not in hs:
+                return Fraction(0)
+        for ring in rings:
+            for (ax, ay), (bx, by) in zip(ring, ring[1:]):
+                d = _exact_seg_dist2(x, y, ax, ay, bx, by)
+                best = d if best is None or d < best else best
+    return best
+
+
+def _polygon_distance_cases():
+    rng = np.random.default_rng(23)
+    sq_hole = [[[(0, 0), (8, 0), (8, 8), (0, 8), (0, 0)], [(2, 2), (2, 6), (6, 6), (6, 2), (2, 2)]]]
+    two_holes = [[[(0, 0), (12, 0), (12, 6), (0, 6), (0, 0)], [(1, 1), (5, 1), (5, 5), (1, 5), (1, 1)],
+                  [(7, 1), (11, 1), (9, 5), (7, 1)]]]
+    multi = [[[(0, 0), (4, 0), (4, 4), (0, 4), (0, 0)], [(1, 1), (3, 1), (3, 3), (1, 3), (1, 1)]],
+             [[(6, 0), (10, 0), (8, 4), (6, 0)]]]
+    n = 24
+    ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+    ext = [(5 + r * np.cos(a), 5 + r * np.sin(a)) for r, a in zip(rng.uniform(4, 5, n), ang)]
+    hole = [(5 + r * np.cos(a), 5 + r * np.sin(a)) for r, a in zip(rng.uniform(1, 2.5, n), ang)]
+    star = [[ext + [ext[0]], hole + [hole[0]]]]
+    cases = []
+    for polys in (sq_hole, two_holes, multi, star):
+        pts = [(1.0, 1.0), (7.0, 4.0), (4.0, 4.0), (3.0, 3.0), (2.0, 4.0), (4.0, 2.0), (2.5, 5.5), (6.0, 6.0),
+               (9.0, 2.0), (9.0, 4.5), (3.0, 0.5), (13.0, 3.0), (-1.0, 7.0), (5.0, 5.0), (2.0, 3.0), (8.0, 1.0)]
+        for rings in polys:
+            for ring in rings:
+                for (ax, ay), (bx, by) in zip(ring, ring[1:]):
+                    pts.append(((ax + bx) / 2, (ay + by) / 2))      # on an edge (hole edges included)
+                    pts.append((ax - 0.3, ay))                       # at a vertex's height
+                    pts.append((ax + 0.01, ay + 0.02))
+        pts += [tuple(rng.uniform(-2, 14, 2)) for _ in range(150)]
+        cases.append((polys, pts))
+    return cases
+
+
+def test_polygon_distance_against_exact_rationals():
+    """Point-to-(multi)polygon distance (the DWithin refine over polygon
+    build rows, geo's utils.rs:231-271 semantics): exactly 0 inside an
+    exterior and outside every hole, and on any boundary; otherwise within
+    1e-12 relative of the exact distance to the nearest ring, hole rings
+    included.  Points lie deep inside solid parts, inside holes, on hole
+    edges and at vertex heights, so dropping the interior rule, the holes
+    or the hole rings' segments each fails.  Tolerance: 1e-12 relative
+    plus a few ulps of the coordinates absolute."""
+    zero = inside_hole = 0
+    for polys, pts in _polygon_distance_cases():
+        wkb = (C.polygon_wkb(polys[0]) if len(polys) == 1 else C.multipolygon_wkb(polys))
+        A = O.Store(C.column([wkb]))
+        B = O.Store(C.column([C.point_wkb(float(x), float(y)) for x, y in pts]))
+        for i, (x, y) in enumerate(pts):
+            got = O.distance(A, 0, B, i)
+            ex2 = _exact_polygon_dist2(polys, float(x), float(y))
+            if ex2 == 0:
+                zero += 1
+                assert got == 0.0, (x, y, got)
+                continue
+            ex = float(ex2) ** 0.5
+            if any(_exact_ring_position(rings[0], Fraction(x), Fraction(y)) == 1 for rings in polys):
+                inside_hole += 1
+            # geo's formula divides a cross product by the segment length,
+            # so points a few ulps off an edge carry an absolute error of a
+            # few ulps of the coordinates (here below 16)
+            assert abs(got - ex) <= 1e-12 * ex + 16 * 2.0**-52 * 16, (x, y, got, ex)
+    assert zero > 50 and inside_hole > 20, (zero, inside_hole)
 
 
 # ---------------------------------------------------------------------------
