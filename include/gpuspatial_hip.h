@@ -285,6 +285,32 @@ int SjhipJoinDevice(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefine
                     const struct ArrowDeviceArray* probe_array,
                     const struct SjhipJoinOptions* options, struct SjhipJoinResult* out);
 
+/* The join of SjhipJoinDevice without allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  It allocates nothing: its buffers live in a library-owned
+ * static device array of SJHIP_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends, and the pairs go to the caller's device arrays
+ * out_build_indices / out_probe_indices of out_capacity entries, in refine
+ * order.  Limits: a device-resident Binary or LargeBinary column of at most
+ * SJHIP_NOALLOC_MAX_ROWS 21-byte WKB points without nulls; at most
+ * SJHIP_NOALLOC_MAX_CANDIDATES candidate pairs; options.precision 0 and
+ * options.sort_output 0; ST_Intersects, ST_Contains or ST_Covers over a
+ * (multi)polygon build side, or ST_DWithin (predicate 100).  Beyond a limit
+ * the call fails with EINVAL and a message; when only the output capacity
+ * is short, *out_size holds the pair count.  options.stream (or the null
+ * stream) runs the kernels; the call returns when they are done.  Every
+ * call on one GPU uses that GPU's one array, whose contents the next call
+ * overwrites: calls from threads of this process are serialised per GPU,
+ * and calls from other processes have arrays of their own. */
+#define SJHIP_NOALLOC_MAX_ROWS 65536
+#define SJHIP_NOALLOC_MAX_CANDIDATES (1u << 20)
+#define SJHIP_NOALLOC_SCRATCH_BYTES 17384192ull
+int SjhipJoinDeviceNoAlloc(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner* refiner,
+                           const struct ArrowSchema* probe_schema, const struct ArrowDeviceArray* probe_array,
+                           const struct SjhipJoinOptions* options, uint32_t* out_build_indices,
+                           uint32_t* out_probe_indices, uint64_t out_capacity, uint64_t* out_size);
+
 /* Parameters chosen by SjhipAutotune. */
 struct SjhipTuneResult {
   int32_t leaf;         /* rect tree leaf size (4, 8 or 16 rects) */

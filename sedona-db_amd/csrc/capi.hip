@@ -64,6 +64,8 @@ Runtime* runtime_of(const GpuSpatialRuntime* rt) {
 
 void use_device(const Runtime* r) { SJ_CHECK(hipSetDevice(r->device_id)); }
 
+unsigned char* noalloc_scratch(int dev);
+
 // The Arrow C device interface's sync_event: for a ROCm array, a pointer to
 // the producer's hipEvent_t, which must complete before the buffers are
 // read.  The host waits for it before the call reads or queues the array.
@@ -102,6 +104,7 @@ int rt_init(GpuSpatialRuntime* self, GpuSpatialRuntimeConfig* cfg) {
       throw Error("device_id " + std::to_string(cfg->device_id) + " out of range (" + std::to_string(n) + " devices)");
     r->device_id = cfg->device_id;
     SJ_CHECK(hipSetDevice(r->device_id));
+    (void)noalloc_scratch(r->device_id);
     if (cfg->use_cuda_memory_pool) {
       hipMemPool_t pool;
       SJ_CHECK(hipDeviceGetDefaultMemPool(&pool, r->device_id));
@@ -776,6 +779,23 @@ struct GraphPlan {
   }
 };
 
+// The kind of fixed launch sequence (kernels over fixed-capacity buffers,
+// no host synchronisation) that can join a probe batch: 0 point in polygon,
+// 1 distance, or -1 when the batch needs the general engine (host or
+// BinaryView buffers, nulls, the all-FP64 kernels, other predicates or
+// build types).
+int fixed_sequence_kind(const RefinerImpl* rf, const WkbInput& in, const SjhipJoinOptions& o) {
+  if (!in.on_device || in.view || in.validity || in.length <= 0 || o.precision != 0) return -1;
+  const GeomColumn& b = rf->build;
+  if (o.predicate == 100) return b.n_rows > 0 ? 1 : -1;
+  const uint32_t polys = (1u << kPolygon) | (1u << kMultiPolygon);
+  const bool pip = !b.points_only && rf->ring_index.ring_rect.p && rf->ring_index.desc.p &&
+                   (b.type_mask & ~polys) == 0 &&
+                   (o.predicate == SedonaSpatialPredicateIntersects || o.predicate == SedonaSpatialPredicateCovers ||
+                    o.predicate == SedonaSpatialPredicateContains);
+  return pip ? 0 : -1;
+}
+
 // Captures are made one at a time in the process: a plan's pinned host
 // buffers are allocated, and its sequence captured and instantiated, while
 // no other thread captures.
@@ -926,6 +946,191 @@ bool run_plan(GraphPlan& g, const WkbInput& in, SjhipJoinResult* out, hipStream_
   return true;
 }
 
+// ---------------------------------------------------------------------------
+// The join without allocation (SjhipJoinDeviceNoAlloc)
+//
+// The launch sequence of a captured request plan, over a file-scope device
+// array instead of allocated buffers.  Each GPU holds its own copy of the
+// array for the life of the process; host code reaches it only through the
+// address hipGetSymbolAddress returns on that device.
+
+constexpr int64_t kNoAllocMaxRows = SJHIP_NOALLOC_MAX_ROWS;
+constexpr uint64_t kNoAllocMaxCandidates = SJHIP_NOALLOC_MAX_CANDIDATES;
+
+constexpr size_t na_align(size_t b) { return (b + 255) / 256 * 256; }
+// bytes of the radix sort's temporary storage for n keys (graph_sort_temp_bytes)
+constexpr size_t na_sort_bytes(int64_t n) {
+  return 2 * na_align((size_t)n * 4) + (size_t)((n + 4095) / 4096) * 256 * 4 + 256;
+}
+
+// Scratch layout, in order, each part 256-byte aligned: the probe
+// descriptor, 8 counters, decoded points (double2) and types, rects,
+// Morton keys, ids, sorted keys and order, the sort's storage, the points
+// in Morton order, then per candidate pair its build id, probe position
+// and deferred-list slot.
+struct NoAllocLayout {
+  static constexpr size_t desc = 0;
+  static constexpr size_t cnt = desc + na_align(sizeof(PointsDesc));
+  static constexpr size_t xy = cnt + na_align(8 * 8);
+  static constexpr size_t type = xy + na_align(16 * kNoAllocMaxRows);
+  static constexpr size_t rects = type + na_align(kNoAllocMaxRows);
+  static constexpr size_t keys = rects + na_align(16 * kNoAllocMaxRows);
+  static constexpr size_t ids = keys + na_align(4 * kNoAllocMaxRows);
+  static constexpr size_t keys_sorted = ids + na_align(4 * kNoAllocMaxRows);
+  static constexpr size_t order = keys_sorted + na_align(4 * kNoAllocMaxRows);
+  static constexpr size_t sort = order + na_align(4 * kNoAllocMaxRows);
+  static constexpr size_t rxy = sort + na_align(na_sort_bytes(kNoAllocMaxRows));
+  static constexpr size_t cb = rxy + na_align(16 * kNoAllocMaxRows);
+  static constexpr size_t cp = cb + na_align(4 * kNoAllocMaxCandidates);
+  static constexpr size_t redo = cp + na_align(4 * kNoAllocMaxCandidates);
+  static constexpr size_t total = redo + na_align(4 * kNoAllocMaxCandidates);
+};
+static_assert(NoAllocLayout::total == SJHIP_NOALLOC_SCRATCH_BYTES, "scratch size declared in the header");
+
+}  // namespace sjhip
+
+// The scratch array (file scope, one copy per GPU).  Host code must not
+// pass its name to a kernel: it is a host-side stand-in, and the device
+// address comes from hipGetSymbolAddress (noalloc_scratch below).
+static __device__ __align__(256) unsigned char scratch[sjhip::NoAllocLayout::total];
+
+namespace sjhip {
+namespace {
+
+struct NoAllocDevice {
+  std::mutex mu;  // calls on one device run one at a time (they share the array)
+  unsigned char* ptr = nullptr;
+};
+
+NoAllocDevice& noalloc_device(int dev) {
+  static NoAllocDevice devs[64];
+  if (dev < 0 || dev >= 64) throw Error("device id out of range for the no-allocation join");
+  return devs[dev];
+}
+
+// The device address of `scratch` on device `dev`, resolved once per device
+// with that device current (the runtime's init does it; a first call on
+// another device does it then).
+unsigned char* noalloc_scratch(int dev) {
+  NoAllocDevice& d = noalloc_device(dev);
+  static std::mutex resolve_mu;
+  std::lock_guard<std::mutex> g(resolve_mu);
+  if (!d.ptr) {
+    int cur = -1;
+    SJ_CHECK(hipGetDevice(&cur));
+    if (cur != dev) throw Error("no-allocation scratch resolved with another device current");
+    void* p = nullptr;
+    SJ_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error("no-allocation scratch has no device address");
+    d.ptr = (unsigned char*)p;
+  }
+  return d.ptr;
+}
+
+// A geometry column over borrowed device memory: its buffers are detached
+// before the column is destroyed, so nothing is freed.
+struct BorrowedPoints {
+  GeomColumn col;
+  BorrowedPoints(double2* xy, uint8_t* type, int64_t n) {
+    col.n_rows = col.n_parts = col.n_rings = col.n_pts = n;
+    col.points_only = true;
+    col.type_mask = 1u << kPoint;
+    col.xy.p = xy;
+    col.xy.n = (size_t)n;
+    col.type.p = type;
+    col.type.n = type ? (size_t)n : 0;
+  }
+  ~BorrowedPoints() {
+    col.xy.p = nullptr;
+    col.xy.n = 0;
+    col.type.p = nullptr;
+    col.type.n = 0;
+  }
+};
+
+void join_no_alloc(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const SjhipJoinOptions& o, uint32_t* ob,
+                   uint32_t* op, uint64_t out_cap, uint64_t* out_size) {
+  if (!ix->built || !rf->built) throw Error("index and refiner must be built");
+  if (o.sort_output) throw Error("the no-allocation join returns pairs in refine order: sort_output must be 0");
+  if (in.length > kNoAllocMaxRows)
+    throw Error("the no-allocation join takes at most " + std::to_string(kNoAllocMaxRows) + " probe rows, not " +
+                std::to_string(in.length));
+  if (in.length == 0) {
+    *out_size = 0;
+    return;
+  }
+  const int kind = fixed_sequence_kind(rf, in, o);
+  if (kind < 0)
+    throw Error("the no-allocation join takes device-resident Binary/LargeBinary point columns without nulls, "
+                "certified precision (0), and ST_Intersects / ST_Contains / ST_Covers over a polygon build side or "
+                "ST_DWithin");
+  if (out_cap > 0 && (!ob || !op)) throw Error("null output arrays");
+  NoAllocDevice& dev = noalloc_device(rf->rt->device_id);
+  std::lock_guard<std::mutex> call_lock(dev.mu);
+  unsigned char* sp = noalloc_scratch(rf->rt->device_id);
+  using L = NoAllocLayout;
+  PointsDesc* desc = (PointsDesc*)(sp + L::desc);
+  unsigned long long* cnt = (unsigned long long*)(sp + L::cnt);
+  double2* xy = (double2*)(sp + L::xy);
+  uint8_t* type = sp + L::type;
+  float4* rects = (float4*)(sp + L::rects);
+  uint32_t* keys = (uint32_t*)(sp + L::keys);
+  uint32_t* ids = (uint32_t*)(sp + L::ids);
+  uint32_t* keys_sorted = (uint32_t*)(sp + L::keys_sorted);
+  uint32_t* order = (uint32_t*)(sp + L::order);
+  void* sort_tmp = sp + L::sort;
+  double2* rxy = (double2*)(sp + L::rxy);
+  uint32_t* cb = (uint32_t*)(sp + L::cb);
+  uint32_t* cp = (uint32_t*)(sp + L::cp);
+  uint32_t* redo = (uint32_t*)(sp + L::redo);
+  const int64_t n = in.length;
+  const uint64_t C = kNoAllocMaxCandidates;
+  if (graph_sort_temp_bytes(n) > na_sort_bytes(kNoAllocMaxRows)) throw Error("no-allocation sort storage too small");
+  hipStream_t s = (hipStream_t)o.stream;
+  PointsDesc d;
+  d.data = in.data;
+  d.offsets = (const uint8_t*)in.offsets + in.offset * (in.large ? 8 : 4);
+  d.n = n;
+  d.large = in.large ? 1 : 0;
+  d.pad = 0;
+  const bool dwithin = kind == 1;
+  // every counter the sequence reads is set here, by kernels: the array
+  // keeps the previous call's contents
+  store_points_desc(d, desc, s);
+  fill_u64(cnt, 1, ~0ull, s);
+  fill_u64(cnt + 1, 7, 0ull, s);
+  decode_points_desc(desc, n, xy, type, cnt, s);
+  rects_of_points_into(xy, type, n, !dwithin, dwithin ? o.distance : NAN, rects, s);
+  morton_order_fixed(ix->tree, rects, n, keys, ids, keys_sorted, order, sort_tmp, na_sort_bytes(kNoAllocMaxRows), s);
+  if (!dwithin) {
+    BorrowedPoints rcol(rxy, nullptr, n);
+    gather_points_into(xy, order, n, rxy, s);
+    launch_filter(ix->tree, rects, n, order, true, cb, cp, C, cnt + 1, s);
+    launch_pip_refine_dev(rf->build, rf->ring_index, rcol.col, o.predicate, cb, cp, cnt + 1, C, order, redo, cnt + 2,
+                          ob, op, out_cap, cnt + 3, cnt + 4, ix->tuning.pip_waves, s);
+  } else {
+    BorrowedPoints pcol(xy, type, n);
+    launch_filter(ix->tree, rects, n, order, false, cb, cp, C, cnt + 1, s);
+    launch_distance_refine_dev(rf->build, rf->ring_index, pcol.col, o.distance, cb, cp, cnt + 1, C, ob, op, out_cap,
+                               cnt + 3, cnt + 4, s);
+  }
+  unsigned long long h[8];
+  SJ_CHECK(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, s));
+  SJ_CHECK(hipStreamSynchronize(s));
+  if (h[0] != ~0ull)
+    throw Error("no-allocation join: probe row " + std::to_string(h[0] & 0xffffffffull) +
+                " is not a 21-byte WKB point record");
+  if (h[1] > C)
+    throw Error("no-allocation join: " + std::to_string(h[1]) + " candidate pairs exceed the scratch limit of " +
+                std::to_string(C) + " (split the batch)");
+  *out_size = h[3];
+  if (h[3] > out_cap)
+    throw Error("no-allocation join: " + std::to_string(h[3]) + " pairs exceed the output capacity of " +
+                std::to_string(out_cap) + " (*out_size holds the count)");
+}
+
+}  // namespace
+
 struct ServerReq {
   uint64_t id = 0;
   WkbInput in;
@@ -981,17 +1186,8 @@ struct Server {
   // The kind of captured sequence a request can replay (0 point in
   // polygon, 1 distance), or -1: eager path.
   int graph_kind(const WkbInput& in, const SjhipJoinOptions& o) const {
-    if (!graphs || !in.on_device || in.view || in.validity || in.length <= 0 || in.length > GraphPlan::kMaxRows ||
-        o.precision != 0)
-      return -1;
-    const GeomColumn& b = rf->build;
-    if (o.predicate == 100) return b.n_rows > 0 ? 1 : -1;
-    const uint32_t polys = (1u << kPolygon) | (1u << kMultiPolygon);
-    const bool pip = !b.points_only && rf->ring_index.ring_rect.p && rf->ring_index.desc.p &&
-                     (b.type_mask & ~polys) == 0 &&
-                     (o.predicate == SedonaSpatialPredicateIntersects || o.predicate == SedonaSpatialPredicateCovers ||
-                      o.predicate == SedonaSpatialPredicateContains);
-    return pip ? 0 : -1;
+    if (!graphs || in.length > GraphPlan::kMaxRows) return -1;
+    return fixed_sequence_kind(rf, in, o);
   }
 
   void work() {
@@ -1264,6 +1460,25 @@ int SjhipJoinDevice(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefine
     WkbInput in = wkb_input_from_arrow(probe_schema, &probe_array->array, dev);
     wait_sync_event(probe_array);
     JoinEngine::run((IndexImpl*)index->private_data, (RefinerImpl*)refiner->private_data, in, *options, out);
+  });
+}
+
+int SjhipJoinDeviceNoAlloc(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner* refiner,
+                           const struct ArrowSchema* probe_schema, const struct ArrowDeviceArray* probe_array,
+                           const struct SjhipJoinOptions* options, uint32_t* out_build_indices,
+                           uint32_t* out_probe_indices, uint64_t out_capacity, uint64_t* out_size) {
+  return guarded(nullptr, [&] {
+    if (!index || !index->private_data || !refiner || !refiner->private_data) throw Error("null index or refiner");
+    if (!probe_array || !options || !out_size) throw Error("null argument");
+    *out_size = 0;
+    if (probe_array->device_type != ARROW_DEVICE_ROCM)
+      throw Error("the no-allocation join needs the probe column in ROCm device memory");
+    RefinerImpl* rf = (RefinerImpl*)refiner->private_data;
+    use_device(rf->rt);
+    WkbInput in = wkb_input_from_arrow(probe_schema, &probe_array->array, true);
+    wait_sync_event(probe_array);
+    join_no_alloc((IndexImpl*)index->private_data, rf, in, *options, out_build_indices, out_probe_indices,
+                  out_capacity, out_size);
   });
 }
 

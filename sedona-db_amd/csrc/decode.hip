@@ -658,6 +658,15 @@ __global__ void k_gather_points(const double2* xy, const uint32_t* order, int64_
 
 }  // namespace
 
+namespace {
+__global__ void k_store_points_desc(PointsDesc d, PointsDesc* dst) { *dst = d; }
+}  // namespace
+
+void store_points_desc(const PointsDesc& d, PointsDesc* dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_store_points_desc, 1, 1, 0, s, d, dst);
+  SJ_CHECK(hipGetLastError());
+}
+
 void decode_points_desc(const PointsDesc* desc_dev, int64_t cap, double2* xy, uint8_t* type, unsigned long long* err,
                         hipStream_t s) {
   if (cap > 0) hipLaunchKernelGGL(k_points_desc, blocks_for(cap), kBlock, 0, s, desc_dev, cap, xy, type, err);

@@ -472,6 +472,10 @@ struct PointsDesc {
   int32_t pad;
 };
 
+// Writes a descriptor to device memory with a kernel (no copy from host
+// memory, no allocation).
+void store_points_desc(const PointsDesc& d, PointsDesc* dst, hipStream_t s);
+
 // Decodes rows [0, d->n) of the described column into xy/type, rows up to
 // cap as empty; rows that are not point records set *err.
 void decode_points_desc(const PointsDesc* desc_dev, int64_t cap, double2* xy, uint8_t* type, unsigned long long* err,

@@ -199,7 +199,7 @@ EXPORTED = [
     "SjhipAutotune", "SjhipRefinerSetSegmentLeaf", "SjhipIndexSetTuning", "SjhipRefinerSetPrecision",
     "SjhipHostRaycast", "SjhipHostRaycastF32", "SjhipHostCollinear", "SjhipHostSegmentIntersects",
     "SjhipHostOrient2d", "SjhipHostHypot", "SjhipHostCoordToLine", "SjhipHostRaycastBatch", "SjhipHostHypotBatch",
-    "SjhipTestingGraphSortPairs",
+    "SjhipTestingGraphSortPairs", "SjhipJoinDeviceNoAlloc",
 ]
 
 _lib = None
@@ -225,6 +225,9 @@ def lib():
         L.SjhipJoinDevice.argtypes = [_IX, _RF, _SCH, ctypes.POINTER(ArrowDeviceArray),
                                       ctypes.POINTER(SjhipJoinOptions), ctypes.POINTER(SjhipJoinResult)]
         L.SjhipJoinResultCopyToHost.argtypes = [ctypes.POINTER(SjhipJoinResult), c_u32p, c_u32p]
+        L.SjhipJoinDeviceNoAlloc.argtypes = [_IX, _RF, _SCH, ctypes.POINTER(ArrowDeviceArray),
+                                             ctypes.POINTER(SjhipJoinOptions), ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
         L.SjhipServerCreate.argtypes = [_IX, _RF, ctypes.POINTER(SjhipServerConfig), ctypes.POINTER(ctypes.c_void_p)]
         L.SjhipServerSubmit.argtypes = [ctypes.c_void_p, _SCH, ctypes.POINTER(ArrowDeviceArray),
                                         ctypes.POINTER(SjhipJoinOptions), ctypes.POINTER(ctypes.c_uint64)]

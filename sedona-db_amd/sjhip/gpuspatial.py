@@ -550,6 +550,30 @@ def join_device(index, refiner, probe, predicate, distance=0.0, precision=0, sor
     return JoinResult(res)
 
 
+# SJHIP_NOALLOC_* of include/gpuspatial_hip.h
+NOALLOC_MAX_ROWS = 65536
+NOALLOC_MAX_CANDIDATES = 1 << 20
+
+
+def join_device_no_alloc(index, refiner, probe, predicate, out_build, out_probe, capacity, distance=0.0):
+    """The fused join without allocation (SjhipJoinDeviceNoAlloc): `probe`
+    is a DeviceWkb point column, out_build / out_probe device buffers (for
+    example DeviceBuffer) of `capacity` u32 entries.  Returns the number of
+    pairs written, in refine order."""
+    opts = _join_options(predicate, distance, 0, False, True, False, 1, None)
+    n = ctypes.c_uint64(0)
+    ob = out_build.ptr if isinstance(out_build, DeviceBuffer) else out_build
+    op = out_probe.ptr if isinstance(out_probe, DeviceBuffer) else out_probe
+    rc = L.lib().SjhipJoinDeviceNoAlloc(ctypes.byref(index.ix), ctypes.byref(refiner.rf), ctypes.byref(probe.schema),
+                                        ctypes.byref(probe.dev), ctypes.byref(opts), ob, op, int(capacity),
+                                        ctypes.byref(n))
+    if rc:
+        err = GpuSpatialError(L.lib().SjhipLastError().decode())
+        err.pairs = int(n.value)  # the pair count when only the capacity was short
+        raise err
+    return int(n.value)
+
+
 def autotune(index, refiner, probe_sample, predicate, distance=0.0, precision=0):
     """Tunes the index for joins like join_device(index, refiner, probe,
     predicate, refine_order=-1): returns the chosen parameters."""
