@@ -752,6 +752,529 @@ static int ring_contains_ring_strict(const ring_t* a, const ring_t* b) {
 }
 
 /* ------------------------------------------------------------------ */
+/* TG ring / line relations with boundaries (polygon x polygon covers, */
+/* contains, touches, equals; intersects with lines)                   */
+/* ------------------------------------------------------------------ */
+
+static int pteq2(double ax, double ay, double bx, double by) { return feq(ax, bx) && feq(ay, by); }
+
+/* tg.c:1910-2120 (process_points): TG's clockwise flag, the shoelace sum
+ * over every point (closing one included, successor modulo npoints) in
+ * index order, clockwise when positive. */
+static int ring_clockwise_tg(const ring_t* r) {
+  int64_t n = r->npts;
+  if (n < 2) return 0;
+  double cwc = 0;
+  for (int64_t i = 0; i < n; i++) {
+    int64_t j = i + 1 < n ? i + 1 : i + 1 - n;
+    double ax = r->xy[2 * i], ay = r->xy[2 * i + 1], bx = r->xy[2 * j], by = r->xy[2 * j + 1];
+    cwc += (bx - ax) * (by + ay);
+  }
+  return cwc > 0;
+}
+
+/* tg.c:2669-2822: point in ring, with the index of the segment the point
+ * is on (-1 when on none).  A segment the point is on sets the result to
+ * allow_on_edge and its index; IN toggles count only before the first
+ * such segment; segments are visited in index order, so the index is the
+ * last one. */
+typedef struct { int hit; int64_t idx; } ring_hit;
+
+static ring_hit ring_contains_point_ix(const ring_t* r, double px, double py, int allow_on_edge) {
+  ring_hit out = {0, -1};
+  double rc[4];
+  ring_rect(r, rc);
+  if (px < rc[0] || py < rc[1] || px > rc[2] || py > rc[3]) return out;
+  int in = 0;
+  int64_t idx = -1;
+  int64_t ns = ring_nsegs(r);
+  for (int64_t i = 0; i < ns; i++) {
+    double s[4];
+    ring_seg(r, i, s);
+    int rcst = ora_raycast(s[0], s[1], s[2], s[3], px, py);
+    if (rcst == RC_ON) {
+      in = allow_on_edge;
+      idx = i;
+    } else if (rcst == RC_IN && idx == -1) {
+      in = !in;
+    }
+  }
+  out.hit = in;
+  out.idx = idx;
+  return out;
+}
+
+static void seg_rect4(const double* s, double* r) {
+  r[0] = fmin0(s[0], s[2]); r[1] = fmin0(s[1], s[3]);
+  r[2] = fmax0(s[0], s[2]); r[3] = fmax0(s[1], s[3]);
+}
+static int rect_covers_rect4(const double* a, const double* b) {
+  return !(b[0] < a[0] || b[2] > a[2] || b[1] < a[1] || b[3] > a[3]);
+}
+static int rect_isect4(const double* a, const double* b) {
+  return !(b[0] > a[2] || b[2] < a[0] || b[1] > a[3] || b[3] < a[1]);
+}
+
+/* tg.c:2910-3016 tg_ring_contains_segment.  The rect searches of
+ * tg_ring_search (tg.c:2468-2485) visit every segment whose rect meets the
+ * query, and each iterator (tg.c:2847-2907) first asks
+ * tg_segment_intersects_segment, which rejects disjoint rects: visiting
+ * every segment decides the same. */
+static int ring_contains_segment_tg(const ring_t* r, const double* s, int allow_on_edge) {
+  double rr[4], sr[4];
+  ring_rect(r, rr);
+  seg_rect4(s, sr);
+  if (!rect_covers_rect4(rr, sr)) return 0;
+  ring_hit a = ring_contains_point_ix(r, s[0], s[1], allow_on_edge);
+  if (!a.hit) return 0;
+  if (pteq2(s[2], s[3], s[0], s[1])) return 1;
+  ring_hit b = ring_contains_point_ix(r, s[2], s[3], allow_on_edge);
+  if (!b.hit) return 0;
+  if (ring_convex_tg(r)) return 1;
+  int64_t ns = ring_nsegs(r);
+  if (allow_on_edge) {
+    int mode;
+    if (a.idx != -1 && b.idx != -1) {
+      if (b.idx == a.idx) return 1;
+      double sa[4], sb[4];
+      ring_seg(r, a.idx, sa);
+      ring_seg(r, b.idx, sb);
+      if (pteq2(sa[0], sa[1], s[0], s[1]) || pteq2(sa[2], sa[3], s[0], s[1]) || pteq2(sb[0], sb[1], s[0], s[1]) ||
+          pteq2(sb[2], sb[3], s[0], s[1]) || pteq2(sa[0], sa[1], s[2], s[3]) || pteq2(sa[2], sa[3], s[2], s[3]) ||
+          pteq2(sb[0], sb[1], s[2], s[3]) || pteq2(sb[2], sb[3], s[2], s[3]))
+        return 1;
+      if (b.idx < a.idx) {
+        double t[4];
+        memcpy(t, sa, sizeof t); memcpy(sa, sb, sizeof t); memcpy(sb, t, sizeof t);
+      }
+      double pts[10] = {sa[0], sa[1], sa[2], sa[3], sb[0], sb[1], sb[2], sb[3], sa[0], sa[1]};
+      double cwc = 0.0;
+      for (int i = 0; i < 4; i++) cwc += (pts[2 * i + 2] - pts[2 * i]) * (pts[2 * i + 3] + pts[2 * i + 1]);
+      if ((cwc > 0) != ring_clockwise_tg(r)) return 0;
+      mode = 1; /* contsegiter1 */
+    } else if (a.idx != -1) {
+      mode = 4; /* contsegiter4 */
+    } else if (b.idx != -1) {
+      mode = 2; /* contsegiter2 */
+    } else {
+      mode = 5; /* contsegiter5 */
+    }
+    for (int64_t i = 0; i < ns; i++) {
+      double t[4];
+      ring_seg(r, i, t);
+      if (!seg_intersects_seg(s, t)) continue;
+      int hit;
+      switch (mode) {
+        case 1: hit = ora_raycast(t[0], t[1], t[2], t[3], s[0], s[1]) != RC_ON &&
+                      ora_raycast(t[0], t[1], t[2], t[3], s[2], s[3]) != RC_ON; break;
+        case 4: hit = ora_raycast(t[0], t[1], t[2], t[3], s[0], s[1]) != RC_ON; break;
+        case 2: hit = ora_raycast(t[0], t[1], t[2], t[3], s[2], s[3]) != RC_ON; break;
+        default: hit = ora_raycast(s[0], s[1], s[2], s[3], t[0], t[1]) != RC_ON &&
+                       ora_raycast(s[0], s[1], s[2], s[3], t[2], t[3]) != RC_ON;
+      }
+      if (hit) return 0;
+    }
+    return 1;
+  }
+  for (int64_t i = 0; i < ns; i++) { /* contsegiter3 */
+    double t[4];
+    ring_seg(r, i, t);
+    if (seg_intersects_seg(s, t)) return 0;
+  }
+  return 1;
+}
+
+/* tg.c:3119-3151 with allow_on_edge. */
+static int ring_contains_ring_tg(const ring_t* a, const ring_t* b, int allow_on_edge) {
+  if (ring_empty(a) || ring_empty(b)) return 0;
+  double ra[4], rb[4];
+  ring_rect(a, ra);
+  ring_rect(b, rb);
+  if (!rect_covers_rect4(ra, rb)) return 0;
+  if (ring_convex_tg(a)) {
+    for (int64_t i = 0; i < b->npts; i++)
+      if (!ring_contains_point_ix(a, b->xy[2 * i], b->xy[2 * i + 1], allow_on_edge).hit) return 0;
+    return 1;
+  }
+  int64_t nb = ring_nsegs(b);
+  for (int64_t i = 0; i < nb; i++) {
+    double s[4];
+    ring_seg(b, i, s);
+    if (!ring_contains_segment_tg(a, s, allow_on_edge)) return 0;
+  }
+  return 1;
+}
+
+/* tg.c:3019-3111 tg_ring_intersects_segment with intersegiter: segments
+ * visited in index order (tg_ring_search), the count reset by a collinear
+ * overlap and the visit stopped at two crossings. */
+static int ring_intersects_segment_tg(const ring_t* r, const double* s, int allow_on_edge) {
+  double rr[4], sr[4];
+  ring_rect(r, rr);
+  seg_rect4(s, sr);
+  if (!rect_isect4(sr, rr)) return 0;
+  if (ring_contains_point_ix(r, s[0], s[1], allow_on_edge).hit ||
+      ring_contains_point_ix(r, s[2], s[3], allow_on_edge).hit)
+    return 1;
+  int count = 0, a_on = 0, b_on = 0;
+  int64_t ns = ring_nsegs(r);
+  for (int64_t i = 0; i < ns; i++) {
+    double t[4], tr[4];
+    ring_seg(r, i, t);
+    seg_rect4(t, tr);
+    if (!rect_isect4(sr, tr)) continue;
+    if (!seg_intersects_seg(s, t)) continue;
+    if (allow_on_edge) {
+      count++;
+      if (count >= 2) break;
+      continue;
+    }
+    int ccol = ora_collinear(s[0], s[1], s[2], s[3], t[0], t[1]);
+    int dcol = ora_collinear(s[0], s[1], s[2], s[3], t[2], t[3]);
+    if (ccol && dcol) {
+      count = 0;
+    } else {
+      if (!a_on && (pteq2(s[0], s[1], t[0], t[1]) || pteq2(s[0], s[1], t[2], t[3]))) {
+        a_on = 1;
+        continue;
+      }
+      if (!b_on && (pteq2(s[2], s[3], t[0], t[1]) || pteq2(s[2], s[3], t[2], t[3]))) {
+        b_on = 1;
+        continue;
+      }
+      count++;
+    }
+    if (count >= 2) break;
+  }
+  return count >= 2;
+}
+
+/* tg.c:3166-3190 with allow_on_edge. */
+static int ring_intersects_ring_tg(const ring_t* ring, const ring_t* other, int allow_on_edge) {
+  if (ring_empty(ring) || ring_empty(other)) return 0;
+  double a[4], b[4];
+  ring_rect(ring, a);
+  ring_rect(other, b);
+  if (!rect_isect4(a, b)) return 0;
+  if (rect_area(b) > rect_area(a)) {
+    const ring_t* t = ring;
+    ring = other;
+    other = t;
+  }
+  int64_t ns = ring_nsegs(other);
+  for (int64_t i = 0; i < ns; i++) {
+    double s[4];
+    ring_seg(other, i, s);
+    if (ring_intersects_segment_tg(ring, s, allow_on_edge)) return 1;
+  }
+  return 0;
+}
+
+/* tg.c:3436-3443 tg_line_covers_point over a ring's segments. */
+static int ring_boundary_covers_point(const ring_t* r, double px, double py) {
+  int64_t ns = ring_nsegs(r);
+  for (int64_t i = 0; i < ns; i++) {
+    double s[4];
+    ring_seg(r, i, s);
+    if (segment_covers_point(s, px, py)) return 1;
+  }
+  return 0;
+}
+
+/* tg.c:4040-4092 tg_poly_covers_poly (tg_poly_contains_poly is the same,
+ * tg.c:4094-4096). */
+static int poly_covers_poly_tg(const poly_t* a, const poly_t* b) {
+  int64_t na = poly_nrings(a), nb = poly_nrings(b);
+  if (na == 0 || nb == 0) return 0;
+  ring_t ae = poly_ring(a, 0), be = poly_ring(b, 0);
+  if (ring_empty(&ae) || ring_empty(&be)) return 0;
+  if (!ring_contains_ring_tg(&ae, &be, 1)) return 0;
+  for (int64_t i = 1; i < na; i++) {
+    ring_t h = poly_ring(a, i);
+    if (ring_contains_ring_tg(&h, &be, 1) || ring_intersects_ring_tg(&h, &be, 0)) {
+      int covers = 0;
+      for (int64_t j = 1; j < nb; j++) {
+        ring_t bh = poly_ring(b, j);
+        if (ring_contains_ring_tg(&bh, &h, 1)) {
+          covers = 1;
+          break;
+        }
+      }
+      if (!covers) return 0;
+    }
+  }
+  return 1;
+}
+
+/* tg.c:4184-4249 tg_poly_touches_poly. */
+static int poly_touches_poly_tg(const poly_t* a, const poly_t* b) {
+  if (poly_nrings(a) == 0 || poly_nrings(b) == 0) return 0;
+  ring_t aext = poly_ring(a, 0), bext = poly_ring(b, 0);
+  double ra[4], rb[4];
+  ring_rect(&aext, ra);
+  ring_rect(&bext, rb);
+  if (!rect_isect4(ra, rb)) return 0;
+  for (int ii = 0; ii < 2; ii++) {
+    const poly_t* poly = ii == 0 ? a : b;
+    const ring_t* ring = ii == 0 ? &bext : &aext;
+    for (int64_t i = 1; i < poly_nrings(poly); i++) {
+      ring_t hole = poly_ring(poly, i);
+      if (ring_contains_ring_tg(&hole, ring, 1)) {
+        for (int64_t j = 0; j < ring->npts; j++)
+          if (ring_boundary_covers_point(&hole, ring->xy[2 * j], ring->xy[2 * j + 1])) return 1;
+        return 0;
+      }
+    }
+  }
+  int64_t ansegs = ring_nsegs(&aext), bnsegs = ring_nsegs(&bext);
+  int64_t atouches = 0, btouches = 0;
+  for (int ii = 0; ii < 2; ii++) {
+    const ring_t* ra_ = ii == 0 ? &aext : &bext;
+    const ring_t* rb_ = ii == 0 ? &bext : &aext;
+    int64_t nsegs = ii == 0 ? bnsegs : ansegs;
+    int64_t touches = 0;
+    for (int64_t i = 0; i < nsegs; i++) {
+      double s[4];
+      ring_seg(rb_, i, s);
+      int isects0 = ring_intersects_segment_tg(ra_, s, 1);
+      int isects1 = ring_intersects_segment_tg(ra_, s, 0);
+      if (isects0 && !isects1) touches++;
+      else if (isects0 || isects1) return 0;
+    }
+    if (ii == 0) btouches = touches;
+    else atouches = touches;
+  }
+  if (atouches > 0 || btouches > 0) return !(atouches == ansegs && btouches == bnsegs);
+  return 0;
+}
+
+/* A (multi)polygon row's part q is a polygon; an empty part (no exterior,
+ * or an exterior of fewer than 3 points) is TG's empty polygon. */
+static int part_empty(const ora_store* s, int64_t q) {
+  poly_t p = {s, q};
+  if (poly_nrings(&p) == 0) return 1;
+  ring_t e = poly_ring(&p, 0);
+  return ring_empty(&e);
+}
+
+/* tg_geom_covers(A, B) for (multi)polygon rows (tg.c:5925-6045):
+ * polygon A covers B when it covers B's polygon, or every part of a
+ * multipolygon B (none empty, at least one); a multipolygon A covers B
+ * when every non-empty part of B (at least one) is covered by some part of
+ * A. */
+static int polyish_covers_polyish(const ora_store* A, int64_t a, const ora_store* B, int64_t b) {
+  int64_t a0 = A->part_off[a], a1 = A->part_off[a + 1], b0 = B->part_off[b], b1 = B->part_off[b + 1];
+  if (a0 == a1 || b0 == b1) return 0;
+  if (A->type[a] == ORA_POLYGON) {
+    poly_t pa = {A, a0};
+    for (int64_t q = b0; q < b1; q++) {
+      poly_t pb = {B, q};
+      if (!poly_covers_poly_tg(&pa, &pb)) return 0;
+    }
+    return 1;
+  }
+  int any = 0;
+  for (int64_t q = b0; q < b1; q++) {
+    if (part_empty(B, q)) continue;
+    poly_t pb = {B, q};
+    int found = 0;
+    for (int64_t r = a0; r < a1 && !found; r++) {
+      poly_t pa = {A, r};
+      found = poly_covers_poly_tg(&pa, &pb);
+    }
+    if (!found) return 0;
+    any = 1;
+  }
+  return any;
+}
+
+/* tg_geom_contains(A, B) for (multi)polygon rows (tg.c:6060-6106,
+ * 6418-6520): a polygon A contains exactly what it covers; a multipolygon A
+ * must cover B and some non-empty part of B must meet some part of A
+ * without merely touching it. */
+static int polyish_contains_polyish(const ora_store* A, int64_t a, const ora_store* B, int64_t b) {
+  if (A->type[a] == ORA_POLYGON) return polyish_covers_polyish(A, a, B, b);
+  if (!polyish_covers_polyish(A, a, B, b)) return 0;
+  for (int64_t q = B->part_off[b]; q < B->part_off[b + 1]; q++) {
+    if (part_empty(B, q)) continue;
+    poly_t pb = {B, q};
+    for (int64_t r = A->part_off[a]; r < A->part_off[a + 1]; r++) {
+      if (part_empty(A, r)) continue;
+      poly_t pa = {A, r};
+      if (poly_intersects_poly(&pa, &pb) && !poly_touches_poly_tg(&pa, &pb)) return 1;
+    }
+  }
+  return 0;
+}
+
+/* poly_touches_geom(part, B) (tg.c:6660-6700): B a polygon, or every part
+ * of a multipolygon B touching or missing the polygon and one touching. */
+static int part_touches_polyish(const poly_t* pa, const ora_store* B, int64_t b) {
+  int64_t b0 = B->part_off[b], b1 = B->part_off[b + 1];
+  if (b0 == b1) return 0;
+  if (B->type[b] == ORA_POLYGON) {
+    poly_t pb = {B, b0};
+    return poly_touches_poly_tg(pa, &pb);
+  }
+  int touches = 0;
+  for (int64_t q = b0; q < b1; q++) {
+    poly_t pb = {B, q};
+    if (poly_touches_poly_tg(pa, &pb)) touches = 1;
+    else if (poly_intersects_poly(pa, &pb)) return 0;
+  }
+  return touches;
+}
+
+static int part_intersects_polyish(const poly_t* pa, const ora_store* B, int64_t b) {
+  for (int64_t q = B->part_off[b]; q < B->part_off[b + 1]; q++) {
+    poly_t pb = {B, q};
+    if (poly_intersects_poly(pa, &pb)) return 1;
+  }
+  return 0;
+}
+
+/* tg_geom_touches(A, B) for (multi)polygon rows (tg.c:6660-6770). */
+static int polyish_touches_polyish(const ora_store* A, int64_t a, const ora_store* B, int64_t b) {
+  int64_t a0 = A->part_off[a], a1 = A->part_off[a + 1];
+  if (a0 == a1) return 0;
+  if (A->type[a] == ORA_POLYGON) {
+    poly_t pa = {A, a0};
+    return part_touches_polyish(&pa, B, b);
+  }
+  int touches = 0;
+  for (int64_t r = a0; r < a1; r++) {
+    poly_t pa = {A, r};
+    if (part_touches_polyish(&pa, B, b)) touches = 1;
+    else if (part_intersects_polyish(&pa, B, b)) return 0;
+  }
+  return touches;
+}
+
+/* Lines: a row's part holds one run of points; TG lines are open series
+ * (tg.c:2126-2135: npoints - 1 segments, empty below two points). */
+static int64_t line_nsegs(const ring_t* l) { return l->npts < 2 ? 0 : l->npts - 1; }
+static void line_seg(const ring_t* l, int64_t i, double* a) {
+  a[0] = l->xy[2 * i]; a[1] = l->xy[2 * i + 1];
+  a[2] = l->xy[2 * i + 2]; a[3] = l->xy[2 * i + 3];
+}
+static void line_rect(const ring_t* l, double* rc) { ring_rect(l, rc); }
+
+/* tg.c:3252-3275 tg_ring_intersects_line. */
+static int ring_intersects_line_tg(const ring_t* r, const ring_t* l, int allow_on_edge) {
+  if (ring_empty(r) || l->npts < 2) return 0;
+  double rr[4], lr[4];
+  ring_rect(r, rr);
+  line_rect(l, lr);
+  if (!rect_isect4(rr, lr)) return 0;
+  for (int64_t i = 0; i < line_nsegs(l); i++) {
+    double s[4];
+    line_seg(l, i, s);
+    if (ring_intersects_segment_tg(r, s, allow_on_edge)) return 1;
+  }
+  return 0;
+}
+
+/* tg.c:3194-3248 tg_ring_contains_line. */
+static int ring_contains_line_tg(const ring_t* a, const ring_t* l, int allow_on_edge, int respect_boundaries) {
+  if (ring_empty(a) || l->npts < 2) return 0;
+  double ra[4], lr[4];
+  ring_rect(a, ra);
+  line_rect(l, lr);
+  if (!rect_covers_rect4(ra, lr)) return 0;
+  for (int64_t i = 0; i < line_nsegs(l); i++) {
+    double s[4];
+    line_seg(l, i, s);
+    if (!allow_on_edge && respect_boundaries) {
+      if (!ring_contains_segment_tg(a, s, 1)) return 0;
+      if (!ring_intersects_segment_tg(a, s, 0)) return 0;
+    } else if (!ring_contains_segment_tg(a, s, allow_on_edge)) {
+      return 0;
+    }
+  }
+  return 1;
+}
+
+/* tg.c:4016-4038 tg_poly_intersects_line. */
+static int poly_intersects_line_tg(const poly_t* p, const ring_t* l) {
+  int64_t nr = poly_nrings(p);
+  if (nr == 0) return 0;
+  ring_t e = poly_ring(p, 0);
+  if (ring_empty(&e) || l->npts < 2) return 0;
+  if (!ring_intersects_line_tg(&e, l, 1)) return 0;
+  for (int64_t k = 1; k < nr; k++) {
+    ring_t h = poly_ring(p, k);
+    if (ring_contains_line_tg(&h, l, 0, 0)) return 0;
+  }
+  return 1;
+}
+
+/* tg.c:3595-3611 line_intersects_line (SI_INTERSECTS) through
+ * tg_ring_ring_search (tg.c:2596-2647): both non-empty, rects meeting,
+ * some pair of segments intersecting. */
+static int line_intersects_line_tg(const ring_t* a, const ring_t* b) {
+  if (a->npts < 2 || b->npts < 2) return 0;
+  double ra[4], rb[4];
+  line_rect(a, ra);
+  line_rect(b, rb);
+  if (!rect_isect4(ra, rb)) return 0;
+  for (int64_t i = 0; i < line_nsegs(a); i++) {
+    double s[4];
+    line_seg(a, i, s);
+    for (int64_t j = 0; j < line_nsegs(b); j++) {
+      double t[4];
+      line_seg(b, j, t);
+      if (seg_intersects_seg(s, t)) return 1;
+    }
+  }
+  return 0;
+}
+
+/* tg.c:3436-3462 tg_line_covers_point (= intersects). */
+static int line_covers_point_tg(const ring_t* l, double px, double py) {
+  for (int64_t i = 0; i < line_nsegs(l); i++) {
+    double s[4];
+    line_seg(l, i, s);
+    if (segment_covers_point(s, px, py)) return 1;
+  }
+  return 0;
+}
+
+static ring_t part_line(const ora_store* s, int64_t q) {
+  int64_t r = s->ring_off[q];
+  ring_t o;
+  o.xy = s->xy + 2 * s->pt_off[r];
+  o.npts = s->pt_off[r + 1] - s->pt_off[r];
+  return o;
+}
+
+static int is_lineish(int t) { return t == ORA_LINESTRING || t == ORA_MULTILINESTRING; }
+
+/* tg_geom_intersects between rows where one side is a (multi)linestring
+ * and the other a point, (multi)linestring or (multi)polygon
+ * (tg.c:5544-5740: some pair of parts intersects). */
+static int lineish_intersects(const ora_store* L, int64_t l, const ora_store* O, int64_t o) {
+  int to = O->type[o];
+  for (int64_t q = L->part_off[l]; q < L->part_off[l + 1]; q++) {
+    ring_t ln = part_line(L, q);
+    for (int64_t r = O->part_off[o]; r < O->part_off[o + 1]; r++) {
+      int hit;
+      if (to == ORA_POINT || to == ORA_MULTIPOINT) {
+        ring_t pt = part_line(O, r);
+        hit = line_covers_point_tg(&ln, pt.xy[0], pt.xy[1]);
+      } else if (is_lineish(to)) {
+        ring_t ol = part_line(O, r);
+        hit = line_intersects_line_tg(&ln, &ol);
+      } else {
+        poly_t pp = {O, r};
+        hit = poly_intersects_line_tg(&pp, &ln);
+      }
+      if (hit) return 1;
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
 /* Exact orientation (Shewchuk-style expansion arithmetic)            */
 /* ------------------------------------------------------------------ */
 
@@ -1069,8 +1592,23 @@ static int eval_pair(const ora_store* B, int64_t b, const ora_store* P, int64_t 
       default: return -1;
     }
   }
-  if (is_polyish(tb) && is_polyish(tp) && pred == ORA_PRED_INTERSECTS)
-    return polyish_polyish_intersects(B, b, P, p);
+  if (is_polyish(tb) && is_polyish(tp)) {
+    switch (pred) {
+      case ORA_PRED_INTERSECTS: return polyish_polyish_intersects(B, b, P, p);
+      case ORA_PRED_COVERS: return polyish_covers_polyish(B, b, P, p);
+      case ORA_PRED_COVEREDBY: return polyish_covers_polyish(P, p, B, b);
+      case ORA_PRED_CONTAINS: return polyish_contains_polyish(B, b, P, p);
+      case ORA_PRED_WITHIN: return polyish_contains_polyish(P, p, B, b);
+      case ORA_PRED_TOUCHES: return polyish_touches_polyish(B, b, P, p);
+      /* tg.c:15299-15301: within(a, b) && contains(a, b) */
+      case ORA_PRED_EQUALS: return polyish_contains_polyish(P, p, B, b) && polyish_contains_polyish(B, b, P, p);
+      default: return -1;
+    }
+  }
+  if (pred == ORA_PRED_INTERSECTS && (is_lineish(tb) || is_lineish(tp))) {
+    if (is_lineish(tb)) return lineish_intersects(B, b, P, p);
+    return lineish_intersects(P, p, B, b);
+  }
   return -1;
 }
 

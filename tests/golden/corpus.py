@@ -267,3 +267,109 @@ def hole_probe_polygons(seed=18, n_random=260):
 
 def hole_corpus(seed=17):
     return column(hole_polygons(seed)), column(hole_probe_polygons(seed + 1))
+
+
+def relate_polygons(seed=19, n_random=90):
+    """(Multi)polygons on a 1.0 grid over [0, 12]^2 for the polygon x polygon
+    relations: tiles sharing edges and corners, nested squares (touching the
+    outer boundary from inside or not), equal rings in both orientations
+    and with a rotated start, squares with holes holding other squares,
+    multipolygons of tiles, L shapes, and random grid stars."""
+    rng = np.random.default_rng(seed)
+    polys = []
+    for x0, y0 in ((0, 0), (2, 0), (0, 2), (2, 2), (4, 4), (1, 1)):
+        polys.append(polygon_wkb([_square(x0, y0, x0 + 2, y0 + 2)]))
+    polys.append(polygon_wkb([_square(0, 0, 4, 4)]))
+    polys.append(polygon_wkb([_square(0, 0, 4, 4, ccw=False)]))
+    polys.append(polygon_wkb([[(4, 0), (4, 4), (0, 4), (0, 0), (4, 0)]]))
+    polys.append(polygon_wkb([[(0, 0), (2, 0), (4, 0), (4, 4), (0, 4), (0, 0)]]))
+    polys.append(polygon_wkb([_square(1, 1, 3, 3)]))
+    polys.append(polygon_wkb([_square(0, 1, 2, 3)]))
+    polys.append(polygon_wkb([_square(0, 0, 4, 4), _square(1, 1, 3, 3)]))
+    polys.append(polygon_wkb([_square(0, 0, 4, 4), _square(1, 1, 2, 2)]))
+    polys.append(polygon_wkb([_square(-1, -1, 5, 5), _square(0, 0, 4, 4)]))
+    polys.append(polygon_wkb([[(0, 0), (4, 0), (4, 2), (2, 2), (2, 4), (0, 4), (0, 0)]]))
+    polys.append(polygon_wkb([[(2, 2), (4, 2), (4, 4), (2, 4), (2, 2)]]))
+    polys.append(polygon_wkb([[(0, 0), (4, 0), (2, 2), (0, 0)]]))
+    polys.append(polygon_wkb([[(0, 0), (2, 2), (0, 4), (0, 0)]]))
+    polys.append(polygon_wkb([[(2, 2), (6, 2), (6, 6), (2, 6), (2, 2)]]))
+    # triangles with every vertex on or inside the L above and one edge
+    # across its notch (endpoints on two different edges of the L)
+    polys.append(polygon_wkb([[(3, 2), (2, 3), (1, 1), (3, 2)]]))
+    polys.append(polygon_wkb([[(3, 2), (1, 1), (2, 3), (3, 2)]]))
+    polys.append(polygon_wkb([[(0, 0), (0, 4), (2, 4), (2, 2), (4, 2), (4, 0), (0, 0)]]))
+    polys.append(multipolygon_wkb([[_square(0, 0, 2, 2)], [_square(2, 2, 4, 4)]]))
+    polys.append(multipolygon_wkb([[_square(0, 0, 2, 2)], [_square(2, 0, 4, 2)]]))
+    polys.append(multipolygon_wkb([[_square(0, 0, 2, 2)], [_square(6, 6, 8, 8)]]))
+    polys.append(multipolygon_wkb([[_square(0, 0, 4, 4), _square(1, 1, 3, 3)], [_square(1, 1, 3, 3)]]))
+    polys.append(multipolygon_wkb([[_square(0, 0, 2, 2)]]))
+    polys.append(multipolygon_wkb([[_square(4, 4, 6, 6)], [_square(0, 0, 1, 1)], [_square(8, 0, 9, 1)]]))
+    # concave stars and polygons whose vertices sit on the star's vertices
+    # and edge midpoints (segments from one boundary edge to another, inside
+    # or across the star)
+    for _ in range(12):
+        cx, cy = (float(v) for v in rng.integers(3, 10, 2))
+        star = _star(rng, cx, cy, int(rng.integers(5, 10)), 1.0, 4.0, grid=1.0, ccw=bool(rng.integers(0, 2)))
+        polys.append(polygon_wkb([star]))
+        cand = star[:-1] + [((a[0] + b[0]) / 2, (a[1] + b[1]) / 2) for a, b in zip(star, star[1:])]
+        for _ in range(3):
+            k = rng.choice(len(cand), size=int(rng.integers(3, 5)), replace=False)
+            ring = [cand[i] for i in k]
+            polys.append(polygon_wkb([ring + [ring[0]]]))
+    for _ in range(n_random):
+        cx, cy = (float(v) for v in rng.integers(0, 13, 2))
+        kind = rng.integers(0, 4)
+        if kind == 0:
+            w, h = (float(v) for v in rng.integers(1, 5, 2))
+            polys.append(polygon_wkb([_square(cx, cy, cx + w, cy + h, ccw=bool(rng.integers(0, 2)))]))
+        elif kind == 1:
+            polys.append(polygon_wkb([_star(rng, cx, cy, int(rng.integers(3, 9)), 1.0, 4.0, grid=1.0,
+                                            ccw=bool(rng.integers(0, 2)))]))
+        elif kind == 2:
+            w = float(rng.integers(3, 6))
+            polys.append(polygon_wkb([_square(cx, cy, cx + w, cy + w), _square(cx + 1, cy + 1, cx + w - 1, cy + w - 1)]))
+        else:
+            parts = []
+            for _ in range(int(rng.integers(2, 4))):
+                x, y = (float(v) for v in rng.integers(0, 12, 2))
+                parts.append([_square(x, y, x + 1, y + 1)])
+            polys.append(multipolygon_wkb(parts))
+    return polys
+
+
+def multilinestring_wkb(lines):
+    return b"\x01" + struct.pack("<I", 5) + struct.pack("<I", len(lines)) + b"".join(linestring_wkb(l) for l in lines)
+
+
+def relate_lines(seed=20, n_random=80):
+    """(Multi)linestrings on a 0.5 grid over [0, 12]^2: along polygon
+    edges, through vertices, inside and across holes, ending on a
+    boundary, closed, zero-length segments, and random grid walks."""
+    rng = np.random.default_rng(seed)
+    lines = [
+        [(0, 0), (4, 0)], [(1, 0), (3, 0)], [(-1, 0), (5, 0)], [(0, 0), (4, 4)], [(2, -1), (2, 5)],
+        [(1.5, 1.5), (2.5, 2.5)], [(1.5, 1.5), (1.5, 2.5), (2.5, 2.5)], [(5, 5), (6, 6)],
+        [(4, 4), (5, 6)], [(0, 4), (4, 4), (4, 0)], [(1, 1), (3, 1), (3, 3), (1, 3), (1, 1)],
+        [(0.5, 0.5), (0.5, 0.5), (3.5, 0.5)], [(2, 2), (2, 2)], [(6, 0), (8, 0), (8, 2)],
+        [(-2, 2), (-1, 2)], [(4, 2), (6, 2)], [(3, 2), (5, 2)],
+    ]
+    out = [linestring_wkb(l) for l in lines]
+    out.append(multilinestring_wkb([[(0, 0), (1, 0)], [(10, 10), (11, 11)]]))
+    out.append(multilinestring_wkb([[(-3, -3), (-2, -2)], [(20, 20), (21, 21)]]))
+    for _ in range(n_random):
+        n = int(rng.integers(2, 7))
+        p = rng.integers(0, 25, 2) * 0.5
+        pts = [tuple(float(v) for v in p)]
+        for _ in range(n - 1):
+            p = p + rng.integers(-4, 5, 2) * 0.5
+            pts.append(tuple(float(v) for v in p))
+        out.append(linestring_wkb(pts))
+    return out
+
+
+def relate_points(seed=21, n=150):
+    """Points on a 0.5 grid over [-1, 13]^2 (on vertices, edges and
+    interiors of the relate corpora)."""
+    rng = np.random.default_rng(seed)
+    pts = rng.integers(-2, 27, size=(n, 2)) * 0.5
+    return [point_wkb(float(x), float(y)) for x, y in pts]

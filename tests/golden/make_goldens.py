@@ -10,8 +10,9 @@ Inputs (all committed as data):
                      byte-identical to synthetic_pip/polygons.parquet),
                      countries/{natural-earth_countries_geo,
                      generated_points}.parquet)
-  corpus.py       -- the adversarial corpus and the polygon-with-holes
-                     corpus (seeded)
+  corpus.py       -- the adversarial corpus, the polygon-with-holes corpus
+                     and the relate corpus of polygons, lines and points
+                     (seeded)
   a small uniform star-polygon workload (seeded, sedona-db_amd/sjhip/datagen.py)
 
 Outputs: for every (build, probe, predicate) the refined (build, probe)
@@ -42,7 +43,10 @@ FIXTURE_JOINS = [
      ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"]),
     ("countries", "countries.parquet", "countries_points.parquet",
      ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"]),
-    ("polypoly", "synthetic_pip_polygons.parquet", "synthetic_poly_polygons2.parquet", ["intersects"]),
+    ("polypoly", "synthetic_pip_polygons.parquet", "synthetic_poly_polygons2.parquet",
+     ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"]),
+    ("countries_countries", "countries.parquet", "countries.parquet",
+     ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"]),
     ("points_in_countries_rev", "countries_points.parquet", "countries.parquet",
      ["within", "intersects", "coveredby", "touches"]),
 ]
@@ -108,6 +112,25 @@ def main():
             assert np.array_equal(np.packbits(m), res["holes_polypoly/intersects/mask"]), "TG index modes disagree"
             assert np.array_equal(np.packbits(m2), res["holes_polypoly_rev/intersects/mask"])
     print("holes_polypoly", int(m.sum()), int(m2.sum()))
+    # relate corpus: (multi)polygons x (multi)polygons for every predicate,
+    # and intersects between (multi)linestrings and polygons, lines, points
+    cols = {"p": C.column(C.relate_polygons()), "l": C.column(C.relate_lines()), "q": C.column(C.relate_points())}
+    cases = [("pp", "p", "p", ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"])]
+    cases += [(a + b, a, b, ["intersects"]) for a, b in (("l", "p"), ("p", "l"), ("l", "l"), ("q", "l"), ("l", "q"))]
+    for name, a, b, preds in cases:
+        na, nb = len(cols[a][0]) - 1, len(cols[b][0]) - 1
+        ai = np.repeat(np.arange(na), nb).astype(np.uint32)
+        bi_ = np.tile(np.arange(nb), na).astype(np.uint32)
+        for pred in preds:
+            key = f"relate_{name}/{pred}/mask"
+            for ix in (0, 1, 2):
+                ta, tb_ = O.TgColumn(cols[a], index=ix), O.TgColumn(cols[b], index=ix)
+                m = np.packbits(O.ref_refine(ta, tb_, pred, ai, bi_))
+                if ix == 0:
+                    res[key] = m
+                else:
+                    assert np.array_equal(m, res[key]), ("TG index modes disagree", key)
+            print(key, int(np.unpackbits(res[key]).sum()))
     # small uniform star workload (1/50 of the 1M x 10k case, same densities)
     build, probe = D.workload_pip_uniform(20_000, 200, seed=7, extent=100.0 * np.sqrt(200 / 10_000))
     ob, op = ref_pairs(build.as_tuple(), probe.as_tuple(), "intersects")

@@ -62,7 +62,10 @@ FIXTURES = [
      ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"]),
     ("points_in_countries_rev", "countries_points.parquet", "countries.parquet",
      ["within", "intersects", "coveredby", "touches"]),
-    ("polypoly", "synthetic_pip_polygons.parquet", "synthetic_poly_polygons2.parquet", ["intersects"]),
+    ("polypoly", "synthetic_pip_polygons.parquet", "synthetic_poly_polygons2.parquet",
+     ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"]),
+    ("countries_countries", "countries.parquet", "countries.parquet",
+     ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"]),
 ]
 
 

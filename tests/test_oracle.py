@@ -16,7 +16,10 @@ FIXTURES = [
      ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"]),
     ("countries", "countries.parquet", "countries_points.parquet",
      ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"]),
-    ("polypoly", "synthetic_pip_polygons.parquet", "synthetic_poly_polygons2.parquet", ["intersects"]),
+    ("polypoly", "synthetic_pip_polygons.parquet", "synthetic_poly_polygons2.parquet",
+     ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"]),
+    ("countries_countries", "countries.parquet", "countries.parquet",
+     ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"]),
     ("points_in_countries_rev", "countries_points.parquet", "countries.parquet",
      ["within", "intersects", "coveredby", "touches"]),
 ]
@@ -65,6 +68,32 @@ def test_oracle_hole_containment_matches_reference_on_hole_corpus():
     assert np.array_equal(O.refine(B, P, "intersects", bi, pi), want)
     want2 = np.unpackbits(E["holes_polypoly_rev/intersects/mask"])[: len(bi)].astype(bool)
     assert np.array_equal(O.refine(P, B, "intersects", pi, bi), want2)
+
+
+RELATE = [("pp", "p", "p", ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"])] + \
+    [(a + b, a, b, ["intersects"]) for a, b in (("l", "p"), ("p", "l"), ("l", "l"), ("q", "l"), ("l", "q"))]
+
+
+def relate_cols():
+    return {"p": C.column(C.relate_polygons()), "l": C.column(C.relate_lines()), "q": C.column(C.relate_points())}
+
+
+@pytest.mark.parametrize("name,a,b,preds", RELATE)
+def test_oracle_relate_corpus_matches_reference(name, a, b, preds):
+    """(Multi)polygon x (multi)polygon under every predicate (TG's
+    tg_poly_covers_poly, tg_poly_touches_poly and the multi-geometry
+    dispatch, tg.c:4040-4249, 5925-6800, 15299-15330) and intersects with
+    (multi)linestrings (tg.c:3194-3275, 3595-3611, 4016-4038), every pair of
+    the relate corpus against the reference TG goldens."""
+    cols = relate_cols()
+    A, B = O.Store(cols[a]), O.Store(cols[b])
+    ai = np.repeat(np.arange(A.n), B.n).astype(np.uint32)
+    bi = np.tile(np.arange(B.n), A.n).astype(np.uint32)
+    E = expected()
+    for pred in preds:
+        want = np.unpackbits(E[f"relate_{name}/{pred}/mask"])[: len(ai)].astype(bool)
+        got = O.refine(A, B, pred, ai, bi)
+        assert np.array_equal(got, want), (name, pred, int((got != want).sum()))
 
 
 def test_corpus_reaches_every_ray_branch():
