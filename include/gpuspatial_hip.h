@@ -169,10 +169,12 @@ enum SedonaSpatialRelationPredicate {
  * LargeBinary ("Z") or BinaryView ("vz") arrays of 2-D WKB.  refine
  * evaluates predicate(build, probe) for each candidate pair, rewrites the
  * caller's arrays in place with the pairs that hold, sorted by (probe,
- * build), and sets *new_indices_size.  Supported: Intersects, Contains,
- * Within, Covers, CoveredBy for (Multi)Polygon x Point, Point x
- * (Multi)Polygon and Point x Point; Intersects for (Multi)Polygon x
- * (Multi)Polygon.  Other combinations fail with EINVAL. */
+ * build), and sets *new_indices_size.  Supported, with TG's semantics:
+ * Intersects, Contains, Within, Covers, CoveredBy, Touches and Equals for
+ * Point x Point, Point x (Multi)Polygon in both roles and (Multi)Polygon x
+ * (Multi)Polygon; Intersects between (Multi)LineString and Point,
+ * (Multi)LineString or (Multi)Polygon in either role.  Other combinations
+ * fail with EINVAL. */
 struct SedonaSpatialRefiner {
   int (*clear)(struct SedonaSpatialRefiner* self);
   int (*init_build_schema)(struct SedonaSpatialRefiner* self, const struct ArrowSchema* build_schema);

@@ -847,6 +847,410 @@ __device__ bool poly_intersects_poly(const GeomView& A, int64_t pa, const GeomVi
   return true;
 }
 
+// ---------------------------------------------------------------------------
+// TG's relations with boundaries between polygons, and intersects with
+// lines (tg.c:2669-3275, 3595-3611, 4016-4249, 5925-6800, 15299-15330), in
+// FP64 with the reference's expression sequences, one lane per candidate.
+// Rings are closed (the decoder rejects open ones): npts - 1 segments.
+
+__device__ __forceinline__ bool pteq2(double ax, double ay, double bx, double by) { return feq(ax, bx) && feq(ay, by); }
+
+// tg.c:1910-2120 (process_points): the clockwise flag, the shoelace sum
+// over every point (closing one included, successor modulo npoints) in
+// index order, clockwise when positive.
+__device__ bool ring_clockwise_tg(const RingRef& r) {
+  const int64_t n = r.npts;
+  if (n < 2) return false;
+  double cwc = 0;
+  for (int64_t i = 0; i < n; i++) {
+    const int64_t j = i + 1 < n ? i + 1 : i + 1 - n;
+    const double2 a = r.xy[i], b = r.xy[j];
+    cwc += (b.x - a.x) * (b.y + a.y);
+  }
+  return cwc > 0;
+}
+
+// tg.c:2669-2822: point in ring with the index of the last segment (in
+// index order) the point is on, -1 when none.  A segment the point is on
+// sets the result to allow_on_edge; IN toggles count only before the first
+// such segment.
+__device__ bool ring_contains_point_ix(const RingRef& r, const double4& rr, double px, double py, bool allow_on_edge,
+                                       int64_t& idx) {
+  idx = -1;
+  if (!rect_covers(rr, px, py)) return false;
+  bool in = false;
+  const int64_t ns = r.npts - 1;
+  for (int64_t i = 0; i < ns; i++) {
+    const double2 a = r.xy[i], b = r.xy[i + 1];
+    if (py < fmin0(a.y, b.y) || py > fmax0(a.y, b.y)) continue;  // OUT in raycast's first test
+    const int rc = raycast(a.x, a.y, b.x, b.y, px, py);
+    if (rc == kRayOn) {
+      in = allow_on_edge;
+      idx = i;
+    } else if (rc == kRayIn && idx == -1) {
+      in = !in;
+    }
+  }
+  return in;
+}
+
+__device__ __forceinline__ bool rect_covers_rect4(const double4& a, double x0, double y0, double x1, double y1) {
+  return !(x0 < a.x || x1 > a.z || y0 < a.y || y1 > a.w);
+}
+__device__ __forceinline__ bool rect_isect4(const double4& a, double x0, double y0, double x1, double y1) {
+  return !(x0 > a.z || x1 < a.x || y0 > a.w || y1 < a.y);
+}
+
+// tg.c:2910-3016 tg_ring_contains_segment (the iterators' own segment test
+// rejects disjoint rects, so visiting every segment decides as the rect
+// search does).
+__device__ bool ring_contains_segment_tg(const RingRef& r, const double4& rr, bool convex, double sax, double say,
+                                         double sbx, double sby, bool allow_on_edge) {
+  if (!rect_covers_rect4(rr, fmin0(sax, sbx), fmin0(say, sby), fmax0(sax, sbx), fmax0(say, sby))) return false;
+  int64_t ia, ib;
+  if (!ring_contains_point_ix(r, rr, sax, say, allow_on_edge, ia)) return false;
+  if (pteq2(sbx, sby, sax, say)) return true;
+  if (!ring_contains_point_ix(r, rr, sbx, sby, allow_on_edge, ib)) return false;
+  if (convex) return true;
+  const int64_t ns = r.npts - 1;
+  int mode = 3;
+  if (allow_on_edge) {
+    if (ia != -1 && ib != -1) {
+      if (ib == ia) return true;
+      double2 a0 = r.xy[ia], a1 = r.xy[ia + 1], b0 = r.xy[ib], b1 = r.xy[ib + 1];
+      if (pteq2(a0.x, a0.y, sax, say) || pteq2(a1.x, a1.y, sax, say) || pteq2(b0.x, b0.y, sax, say) ||
+          pteq2(b1.x, b1.y, sax, say) || pteq2(a0.x, a0.y, sbx, sby) || pteq2(a1.x, a1.y, sbx, sby) ||
+          pteq2(b0.x, b0.y, sbx, sby) || pteq2(b1.x, b1.y, sbx, sby))
+        return true;
+      if (ib < ia) {
+        double2 t0 = a0, t1 = a1;
+        a0 = b0; a1 = b1; b0 = t0; b1 = t1;
+      }
+      const double2 pts[5] = {a0, a1, b0, b1, a0};
+      double cwc = 0.0;
+      for (int i = 0; i < 4; i++) cwc += (pts[i + 1].x - pts[i].x) * (pts[i + 1].y + pts[i].y);
+      if ((cwc > 0) != ring_clockwise_tg(r)) return false;
+      mode = 1;
+    } else if (ia != -1) {
+      mode = 4;
+    } else if (ib != -1) {
+      mode = 2;
+    } else {
+      mode = 5;
+    }
+  }
+  for (int64_t i = 0; i < ns; i++) {
+    const double2 c = r.xy[i], d = r.xy[i + 1];
+    if (!segment_intersects(sax, say, sbx, sby, c.x, c.y, d.x, d.y)) continue;
+    bool hit;
+    switch (mode) {
+      case 1: hit = raycast(c.x, c.y, d.x, d.y, sax, say) != kRayOn && raycast(c.x, c.y, d.x, d.y, sbx, sby) != kRayOn; break;
+      case 4: hit = raycast(c.x, c.y, d.x, d.y, sax, say) != kRayOn; break;
+      case 2: hit = raycast(c.x, c.y, d.x, d.y, sbx, sby) != kRayOn; break;
+      case 5: hit = raycast(sax, say, sbx, sby, c.x, c.y) != kRayOn && raycast(sax, say, sbx, sby, d.x, d.y) != kRayOn; break;
+      default: hit = true;
+    }
+    if (hit) return false;
+  }
+  return true;
+}
+
+// tg.c:3119-3151 tg_ring_contains_ring with allow_on_edge.
+__device__ bool ring_contains_ring_tg(const RingRef& a, const RingRef& b, bool allow_on_edge) {
+  if (a.npts < 3 || b.npts < 3) return false;
+  const double4 ra = ring_rect_scan(a), rb = ring_rect_scan(b);
+  if (!rect_covers_rect4(ra, rb.x, rb.y, rb.z, rb.w)) return false;
+  int64_t idx;
+  if (ring_convex_tg(a)) {
+    for (int64_t i = 0; i < b.npts; i++) {
+      const double2 p = b.xy[i];
+      if (!ring_contains_point_ix(a, ra, p.x, p.y, allow_on_edge, idx)) return false;
+    }
+    return true;
+  }
+  for (int64_t i = 0; i + 1 < b.npts; i++) {
+    const double2 p = b.xy[i], q = b.xy[i + 1];
+    if (!ring_contains_segment_tg(a, ra, false, p.x, p.y, q.x, q.y, allow_on_edge)) return false;
+  }
+  return true;
+}
+
+// tg.c:3019-3111 tg_ring_intersects_segment with intersegiter: segments in
+// index order, the count reset by a collinear overlap, stopped at two.
+__device__ bool ring_intersects_segment_tg(const RingRef& r, const double4& rr, double sax, double say, double sbx,
+                                           double sby, bool allow_on_edge) {
+  const double x0 = fmin0(sax, sbx), y0 = fmin0(say, sby), x1 = fmax0(sax, sbx), y1 = fmax0(say, sby);
+  if (!rect_isect4(rr, x0, y0, x1, y1)) return false;
+  int64_t idx;
+  if (ring_contains_point_ix(r, rr, sax, say, allow_on_edge, idx) ||
+      ring_contains_point_ix(r, rr, sbx, sby, allow_on_edge, idx))
+    return true;
+  int count = 0;
+  bool a_on = false, b_on = false;
+  const int64_t ns = r.npts - 1;
+  for (int64_t i = 0; i < ns; i++) {
+    const double2 c = r.xy[i], d = r.xy[i + 1];
+    if (!segment_intersects(sax, say, sbx, sby, c.x, c.y, d.x, d.y)) continue;
+    if (allow_on_edge) {
+      if (++count >= 2) break;
+      continue;
+    }
+    const bool ccol = collinear(sax, say, sbx, sby, c.x, c.y);
+    const bool dcol = collinear(sax, say, sbx, sby, d.x, d.y);
+    if (ccol && dcol) {
+      count = 0;
+    } else {
+      if (!a_on && (pteq2(sax, say, c.x, c.y) || pteq2(sax, say, d.x, d.y))) {
+        a_on = true;
+        continue;
+      }
+      if (!b_on && (pteq2(sbx, sby, c.x, c.y) || pteq2(sbx, sby, d.x, d.y))) {
+        b_on = true;
+        continue;
+      }
+      count++;
+    }
+    if (count >= 2) break;
+  }
+  return count >= 2;
+}
+
+// tg.c:3166-3190 tg_ring_intersects_ring with allow_on_edge.
+__device__ bool ring_intersects_ring_tg(RingRef ring, RingRef other, bool allow_on_edge) {
+  if (ring.npts < 3 || other.npts < 3) return false;
+  double4 a = ring_rect_scan(ring), b = ring_rect_scan(other);
+  if (!rect_isect4(a, b.x, b.y, b.z, b.w)) return false;
+  if ((b.z - b.x) * (b.w - b.y) > (a.z - a.x) * (a.w - a.y)) {
+    RingRef t = ring; ring = other; other = t;
+    a = b;
+  }
+  for (int64_t i = 0; i + 1 < other.npts; i++) {
+    const double2 p = other.xy[i], q = other.xy[i + 1];
+    if (ring_intersects_segment_tg(ring, a, p.x, p.y, q.x, q.y, allow_on_edge)) return true;
+  }
+  return false;
+}
+
+// tg.c:3436-3443 tg_line_covers_point over a run of segments.
+__device__ bool run_covers_point(const RingRef& r, double px, double py) {
+  for (int64_t i = 0; i + 1 < r.npts; i++) {
+    const double2 a = r.xy[i], b = r.xy[i + 1];
+    if (segment_covers_point(a.x, a.y, b.x, b.y, px, py)) return true;
+  }
+  return false;
+}
+
+// tg.c:4040-4092 tg_poly_covers_poly (also tg_poly_contains_poly,
+// tg.c:4094-4096); polygons are parts pa of A and pb of B.
+__device__ bool poly_covers_poly_tg(const GeomView& A, int64_t pa, const GeomView& B, int64_t pb) {
+  const int64_t a0 = A.ring_off[pa], a1 = A.ring_off[pa + 1], b0 = B.ring_off[pb], b1 = B.ring_off[pb + 1];
+  if (a0 == a1 || b0 == b1) return false;
+  const RingRef ae = ring_at(A, a0), be = ring_at(B, b0);
+  if (ae.npts < 3 || be.npts < 3) return false;
+  if (!ring_contains_ring_tg(ae, be, true)) return false;
+  for (int64_t h = a0 + 1; h < a1; h++) {
+    const RingRef hr = ring_at(A, h);
+    if (ring_contains_ring_tg(hr, be, true) || ring_intersects_ring_tg(hr, be, false)) {
+      bool covers = false;
+      for (int64_t j = b0 + 1; j < b1 && !covers; j++) covers = ring_contains_ring_tg(ring_at(B, j), hr, true);
+      if (!covers) return false;
+    }
+  }
+  return true;
+}
+
+// tg.c:4184-4249 tg_poly_touches_poly.
+__device__ bool poly_touches_poly_tg(const GeomView& A, int64_t pa, const GeomView& B, int64_t pb) {
+  const int64_t a0 = A.ring_off[pa], a1 = A.ring_off[pa + 1], b0 = B.ring_off[pb], b1 = B.ring_off[pb + 1];
+  if (a0 == a1 || b0 == b1) return false;
+  const RingRef aext = ring_at(A, a0), bext = ring_at(B, b0);
+  const double4 ra = ring_rect_scan(aext), rb = ring_rect_scan(bext);
+  if (!rect_isect4(ra, rb.x, rb.y, rb.z, rb.w)) return false;
+  for (int ii = 0; ii < 2; ii++) {
+    const GeomView& P = ii == 0 ? A : B;
+    const int64_t h0 = (ii == 0 ? a0 : b0) + 1, h1 = ii == 0 ? a1 : b1;
+    const RingRef& ring = ii == 0 ? bext : aext;
+    for (int64_t h = h0; h < h1; h++) {
+      const RingRef hole = ring_at(P, h);
+      if (ring_contains_ring_tg(hole, ring, true)) {
+        for (int64_t j = 0; j < ring.npts; j++)
+          if (run_covers_point(hole, ring.xy[j].x, ring.xy[j].y)) return true;
+        return false;
+      }
+    }
+  }
+  const int64_t ansegs = aext.npts - 1, bnsegs = bext.npts - 1;
+  int64_t atouches = 0, btouches = 0;
+  for (int ii = 0; ii < 2; ii++) {
+    const RingRef& x = ii == 0 ? aext : bext;
+    const RingRef& y = ii == 0 ? bext : aext;
+    const double4 rx = ii == 0 ? ra : rb;
+    int64_t touches = 0;
+    for (int64_t i = 0; i + 1 < y.npts; i++) {
+      const double2 p = y.xy[i], q = y.xy[i + 1];
+      const bool isects0 = ring_intersects_segment_tg(x, rx, p.x, p.y, q.x, q.y, true);
+      const bool isects1 = ring_intersects_segment_tg(x, rx, p.x, p.y, q.x, q.y, false);
+      if (isects0 && !isects1) touches++;
+      else if (isects0 || isects1) return false;
+    }
+    if (ii == 0) btouches = touches;
+    else atouches = touches;
+  }
+  if (atouches > 0 || btouches > 0) return !(atouches == ansegs && btouches == bnsegs);
+  return false;
+}
+
+__device__ __forceinline__ bool part_empty(const GeomView& g, int64_t q) {
+  const int64_t r0 = g.ring_off[q], r1 = g.ring_off[q + 1];
+  return r0 == r1 || g.pt_off[r0 + 1] - g.pt_off[r0] < 3;
+}
+
+// tg_geom_covers(A row a, B row b) for (multi)polygon rows (tg.c:5925-6045).
+__device__ bool polyish_covers_polyish(const GeomView& A, int64_t a, const GeomView& B, int64_t b) {
+  const int64_t a0 = A.part_off[a], a1 = A.part_off[a + 1], b0 = B.part_off[b], b1 = B.part_off[b + 1];
+  if (a0 == a1 || b0 == b1) return false;
+  if (A.type[a] == kPolygon) {
+    for (int64_t q = b0; q < b1; q++)
+      if (!poly_covers_poly_tg(A, a0, B, q)) return false;
+    return true;
+  }
+  bool any = false;
+  for (int64_t q = b0; q < b1; q++) {
+    if (part_empty(B, q)) continue;
+    bool found = false;
+    for (int64_t r = a0; r < a1 && !found; r++) found = poly_covers_poly_tg(A, r, B, q);
+    if (!found) return false;
+    any = true;
+  }
+  return any;
+}
+
+// tg_geom_contains for (multi)polygon rows (tg.c:6060-6106, 6418-6520).
+__device__ bool polyish_contains_polyish(const GeomView& A, int64_t a, const GeomView& B, int64_t b) {
+  if (A.type[a] == kPolygon) return polyish_covers_polyish(A, a, B, b);
+  if (!polyish_covers_polyish(A, a, B, b)) return false;
+  for (int64_t q = B.part_off[b]; q < B.part_off[b + 1]; q++) {
+    if (part_empty(B, q)) continue;
+    for (int64_t r = A.part_off[a]; r < A.part_off[a + 1]; r++) {
+      if (part_empty(A, r)) continue;
+      if (poly_intersects_poly(A, r, B, q) && !poly_touches_poly_tg(A, r, B, q)) return true;
+    }
+  }
+  return false;
+}
+
+// poly_touches_geom(part pa of A, B row b) (tg.c:6660-6700).
+__device__ bool part_touches_polyish(const GeomView& A, int64_t pa, const GeomView& B, int64_t b) {
+  const int64_t b0 = B.part_off[b], b1 = B.part_off[b + 1];
+  if (b0 == b1) return false;
+  if (B.type[b] == kPolygon) return poly_touches_poly_tg(A, pa, B, b0);
+  bool touches = false;
+  for (int64_t q = b0; q < b1; q++) {
+    if (poly_touches_poly_tg(A, pa, B, q)) touches = true;
+    else if (poly_intersects_poly(A, pa, B, q)) return false;
+  }
+  return touches;
+}
+
+// tg_geom_touches for (multi)polygon rows (tg.c:6660-6770).
+__device__ bool polyish_touches_polyish(const GeomView& A, int64_t a, const GeomView& B, int64_t b) {
+  const int64_t a0 = A.part_off[a], a1 = A.part_off[a + 1];
+  if (a0 == a1) return false;
+  if (A.type[a] == kPolygon) return part_touches_polyish(A, a0, B, b);
+  bool touches = false;
+  for (int64_t r = a0; r < a1; r++) {
+    if (part_touches_polyish(A, r, B, b)) {
+      touches = true;
+    } else {
+      for (int64_t q = B.part_off[b]; q < B.part_off[b + 1]; q++)
+        if (poly_intersects_poly(A, r, B, q)) return false;
+    }
+  }
+  return touches;
+}
+
+// Lines are open runs (tg.c:2126-2135): npts - 1 segments.
+
+// tg.c:3252-3275 tg_ring_intersects_line.
+__device__ bool ring_intersects_run_tg(const RingRef& r, const RingRef& l, bool allow_on_edge) {
+  if (r.npts < 3 || l.npts < 2) return false;
+  const double4 rr = ring_rect_scan(r), lr = ring_rect_scan(l);
+  if (!rect_isect4(rr, lr.x, lr.y, lr.z, lr.w)) return false;
+  for (int64_t i = 0; i + 1 < l.npts; i++) {
+    const double2 p = l.xy[i], q = l.xy[i + 1];
+    if (ring_intersects_segment_tg(r, rr, p.x, p.y, q.x, q.y, allow_on_edge)) return true;
+  }
+  return false;
+}
+
+// tg.c:3194-3248 tg_ring_contains_line.
+__device__ bool ring_contains_run_tg(const RingRef& a, const RingRef& l, bool allow_on_edge, bool respect_boundaries) {
+  if (a.npts < 3 || l.npts < 2) return false;
+  const double4 ra = ring_rect_scan(a), lr = ring_rect_scan(l);
+  if (!rect_covers_rect4(ra, lr.x, lr.y, lr.z, lr.w)) return false;
+  const bool convex = ring_convex_tg(a);
+  for (int64_t i = 0; i + 1 < l.npts; i++) {
+    const double2 p = l.xy[i], q = l.xy[i + 1];
+    if (!allow_on_edge && respect_boundaries) {
+      if (!ring_contains_segment_tg(a, ra, convex, p.x, p.y, q.x, q.y, true)) return false;
+      if (!ring_intersects_segment_tg(a, ra, p.x, p.y, q.x, q.y, false)) return false;
+    } else if (!ring_contains_segment_tg(a, ra, convex, p.x, p.y, q.x, q.y, allow_on_edge)) {
+      return false;
+    }
+  }
+  return true;
+}
+
+// tg.c:4016-4038 tg_poly_intersects_line (part pa of A, line run l).
+__device__ bool poly_intersects_run_tg(const GeomView& A, int64_t pa, const RingRef& l) {
+  const int64_t a0 = A.ring_off[pa], a1 = A.ring_off[pa + 1];
+  if (a0 == a1) return false;
+  const RingRef e = ring_at(A, a0);
+  if (e.npts < 3 || l.npts < 2) return false;
+  if (!ring_intersects_run_tg(e, l, true)) return false;
+  for (int64_t h = a0 + 1; h < a1; h++)
+    if (ring_contains_run_tg(ring_at(A, h), l, false, false)) return false;
+  return true;
+}
+
+// tg.c:3595-3611 (SI_INTERSECTS) through tg_ring_ring_search
+// (tg.c:2596-2647): rects meeting and some pair of segments intersecting.
+__device__ bool run_intersects_run_tg(const RingRef& a, const RingRef& b) {
+  if (a.npts < 2 || b.npts < 2) return false;
+  const double4 ra = ring_rect_scan(a), rb = ring_rect_scan(b);
+  if (!rect_isect4(ra, rb.x, rb.y, rb.z, rb.w)) return false;
+  for (int64_t i = 0; i + 1 < a.npts; i++) {
+    const double2 p = a.xy[i], q = a.xy[i + 1];
+    for (int64_t j = 0; j + 1 < b.npts; j++) {
+      const double2 c = b.xy[j], d = b.xy[j + 1];
+      if (segment_intersects(p.x, p.y, q.x, q.y, c.x, c.y, d.x, d.y)) return true;
+    }
+  }
+  return false;
+}
+
+__device__ __forceinline__ bool is_lineish(int t) { return t == kLine || t == kMultiLine; }
+
+// tg_geom_intersects(L row l, O row o) where L is a (multi)linestring and O a
+// (multi)linestring or (multi)polygon row (tg.c:5544-5740: some pair of
+// parts intersects).
+__device__ bool lineish_intersects(const GeomView& L, int64_t l, const GeomView& O, int64_t o) {
+  const int to = O.type[o];
+  for (int64_t q = L.part_off[l]; q < L.part_off[l + 1]; q++) {
+    const RingRef ln = ring_at(L, L.ring_off[q]);
+    for (int64_t r = O.part_off[o]; r < O.part_off[o + 1]; r++) {
+      bool hit;
+      if (is_lineish(to)) {
+        hit = run_intersects_run_tg(ln, ring_at(O, O.ring_off[r]));
+      } else {
+        hit = poly_intersects_run_tg(O, r, ln);
+      }
+      if (hit) return true;
+    }
+  }
+  return false;
+}
+
 // Evaluates pred(build b, probe p).  Returns 0/1, or -1 when unsupported.
 __device__ int eval_relation(const Ctx& c, int64_t b, int64_t p, int pred, int precision, uint64_t& tests,
                              uint64_t& fallbacks) {
@@ -905,6 +1309,45 @@ __device__ int eval_relation(const Ctx& c, int64_t b, int64_t p, int pred, int p
     return 0;
   }
   return -1;
+}
+
+// The relations eval_relation leaves out: (multi)polygon x (multi)polygon
+// under predicates other than Intersects, and Intersects with
+// (multi)linestrings.  Returns 0/1, -1 when unsupported, -2 when the pair
+// is eval_relation's.
+__device__ int eval_relate_extended(const Ctx& c, int64_t b, int64_t p, int pred) {
+  const int tb = c.B.type[b], tp = c.P.type[p];
+  if (tb == kNull || tp == kNull) return 0;
+  if (is_polyish(tb) && is_polyish(tp) && pred != kIntersects) {
+    switch (pred) {
+      case kCovers: return polyish_covers_polyish(c.B, b, c.P, p) ? 1 : 0;
+      case kCoveredBy: return polyish_covers_polyish(c.P, p, c.B, b) ? 1 : 0;
+      case kContains: return polyish_contains_polyish(c.B, b, c.P, p) ? 1 : 0;
+      case kWithin: return polyish_contains_polyish(c.P, p, c.B, b) ? 1 : 0;
+      case kTouches: return polyish_touches_polyish(c.B, b, c.P, p) ? 1 : 0;
+      // within(a, b) && contains(a, b) (tg.c:15299-15301)
+      case kEquals:
+        return (polyish_contains_polyish(c.P, p, c.B, b) && polyish_contains_polyish(c.B, b, c.P, p)) ? 1 : 0;
+      default: return -1;
+    }
+  }
+  if (pred == kIntersects && (is_lineish(tb) || is_lineish(tp))) {
+    if (tb == kPoint || tp == kPoint) {
+      // a point row (points-only column or not) and a line row:
+      // tg_line_covers_point over the line's parts (tg.c:3436-3462)
+      const bool bpt = tb == kPoint;
+      double x, y;
+      if (!point_of(bpt ? c.B : c.P, bpt ? b : p, x, y)) return 0;
+      const GeomView& L = bpt ? c.P : c.B;
+      const int64_t l = bpt ? p : b;
+      for (int64_t q = L.part_off[l]; q < L.part_off[l + 1]; q++)
+        if (run_covers_point(ring_at(L, L.ring_off[q]), x, y)) return 1;
+      return 0;
+    }
+    if (is_lineish(tb)) return lineish_intersects(c.B, b, c.P, p) ? 1 : 0;
+    return lineish_intersects(c.P, p, c.B, b) ? 1 : 0;
+  }
+  return -2;
 }
 
 // Whether candidate i is handled by the wavefront-cooperative kernel.
@@ -1358,6 +1801,41 @@ __global__ __launch_bounds__(kBlock) void k_refine_lane(Ctx c, int pred, int pre
   }
   wave_flush(wo, ob, op, cap, counter);
   // per-wave statistics
+  for (int o = 32; o > 0; o >>= 1) {
+    tests += __shfl_xor(tests, o);
+    fallbacks += __shfl_xor(fallbacks, o);
+  }
+  if ((threadIdx.x & 63) == 0 && stats) {
+    atomicAdd(&stats[0], (unsigned long long)tests);
+    atomicAdd(&stats[1], (unsigned long long)fallbacks);
+  }
+}
+
+// The general relation refine: one lane per candidate, every supported
+// pair of geometry types (the polygon x polygon relations with boundaries
+// and the line intersections included).
+__global__ __launch_bounds__(kBlock) void k_refine_relate(Ctx c, int pred, int precision, const uint32_t* cb,
+                                                          const uint32_t* cp, uint64_t n, uint32_t* ob, uint32_t* op,
+                                                          uint64_t cap, unsigned long long* counter,
+                                                          unsigned long long* stats, int* bad) {
+  SJ_WAVE_OUT(wo);
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t tests = 0, fallbacks = 0;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+    uint64_t i = base + threadIdx.x;
+    bool keep = false;
+    uint32_t b = 0, p = 0;
+    if (i < n) {
+      b = cb[i];
+      p = cp[i];
+      int r = eval_relate_extended(c, b, p, pred);
+      if (r == -2) r = eval_relation(c, b, p, pred, precision, tests, fallbacks);
+      if (r < 0) atomicOr(bad, 1);
+      keep = r > 0;
+    }
+    emit(wo, keep, b, p, ob, op, cap, counter);
+  }
+  wave_flush(wo, ob, op, cap, counter);
   for (int o = 32; o > 0; o >>= 1) {
     tests += __shfl_xor(tests, o);
     fallbacks += __shfl_xor(fallbacks, o);
@@ -2086,14 +2564,17 @@ void build_ring_index(const GeomColumn& g, RingIndex& ri, const RingIndexParams&
 
 void check_relation_types(uint32_t bm, uint32_t pm, int pred) {
   const uint32_t poly = (1u << kPolygon) | (1u << kMultiPolygon), pt = 1u << kPoint;
+  const uint32_t line = (1u << kLine) | (1u << kMultiLine);
   bool ok = true;
   auto only = [](uint32_t m, uint32_t allowed) { return (m & ~allowed) == 0; };
   switch (pred) {
     case kIntersects:
-      ok = only(bm, poly | pt) && only(pm, poly | pt);
+      // points, (multi)linestrings and (multi)polygons in any combination
+      ok = only(bm, poly | pt | line) && only(pm, poly | pt | line);
       break;
     case kContains: case kWithin: case kCovers: case kCoveredBy: case kTouches: case kEquals:
-      ok = only(bm, poly | pt) && only(pm, poly | pt) && !((bm & poly) && (pm & poly));
+      // point x point, point x (multi)polygon in both roles, and (multi)polygon x (multi)polygon
+      ok = only(bm, poly | pt) && only(pm, poly | pt);
       break;
     default:
       ok = false;
@@ -2322,13 +2803,20 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
     if (stats) stats->walked += walked;
     return cnt;
   }
-  bool use_long = !build.points_only && probe.points_only && c.R.ring_rect != nullptr &&
+  const uint32_t lines = (1u << kLine) | (1u << kMultiLine);
+  const bool general = ((build.type_mask | probe.type_mask) & lines) ||
+                       (predicate != kIntersects && (build.type_mask & polys) && (probe.type_mask & polys));
+  bool use_long = !general && !build.points_only && probe.points_only && c.R.ring_rect != nullptr &&
                   (predicate == kIntersects || predicate == kCovers || predicate == kContains);
   if (use_long) long_list.alloc(n, s);
   if (timer) timer->record_start(s);
-  hipLaunchKernelGGL(k_refine_lane, grid_for(n), kBlock, 0, s, c, predicate, precision, cb, cp, n, out.b.p, out.p.p,
-                     out.cap(), out.counter.p, use_long ? long_list.p : nullptr, aux.p, aux.p + 1,
-                     (int*)(aux.p + 3));
+  if (general)
+    hipLaunchKernelGGL(k_refine_relate, grid_for(n), kBlock, 0, s, c, predicate, precision, cb, cp, n, out.b.p,
+                       out.p.p, out.cap(), out.counter.p, aux.p + 1, (int*)(aux.p + 3));
+  else
+    hipLaunchKernelGGL(k_refine_lane, grid_for(n), kBlock, 0, s, c, predicate, precision, cb, cp, n, out.b.p,
+                       out.p.p, out.cap(), out.counter.p, use_long ? long_list.p : nullptr, aux.p, aux.p + 1,
+                       (int*)(aux.p + 3));
   SJ_CHECK(hipGetLastError());
   if (use_long) {
     // enough waves to fill the chip several times over

@@ -586,6 +586,35 @@ def test_hole_corpus_polypoly_vs_reference_goldens(rev, opts):
     assert pairs_equal((gb, gp), (bi[want], pi[want]))[0]
 
 
+RELATE = [("pp", "p", "p", ["intersects", "contains", "covers", "within", "coveredby", "touches", "equals"])] + \
+    [(a + b, a, b, ["intersects"]) for a, b in (("l", "p"), ("p", "l"), ("l", "l"), ("q", "l"), ("l", "q"))]
+
+
+@pytest.mark.parametrize("name,a,b,preds", RELATE)
+def test_relate_corpus_vs_reference_goldens(name, a, b, preds, opts):
+    """(Multi)polygon x (multi)polygon under every predicate and intersects
+    with (multi)linestrings, every pair of the relate corpus through the
+    refiner ABI, and the rect-filtered pairs through the fused join, against
+    the reference TG goldens."""
+    cols = {"p": C.relate_polygons(), "l": C.relate_lines(), "q": C.relate_points()}
+    A, B = corpus_arrow(C.column(cols[a])), corpus_arrow(C.column(cols[b]))
+    na, nb = len(A), len(B)
+    ai = np.repeat(np.arange(na), nb).astype(np.uint32)
+    bi = np.tile(np.arange(nb), na).astype(np.uint32)
+    E = expected()
+    ix, rf = build_pair(A, opts)
+    for pred in preds:
+        say("relate", name, pred)
+        want = np.unpackbits(E[f"relate_{name}/{pred}/mask"])[: len(ai)].astype(bool)
+        ob, op = rf.refine(B, pred, ai, bi)
+        eq, miss, extra = pairs_equal((ob, op), (ai[want], bi[want]))
+        assert eq, (name, pred, miss, extra)
+        r = G.join_device(ix, rf, B, pred, sort_output=True)
+        gb, gp = r.to_host()
+        r.release()
+        assert pairs_equal((gb, gp), (ai[want], bi[want]))[0], (name, pred, "fused")
+
+
 @pytest.mark.parametrize("d", [0.0, 0.3, 1.5])
 def test_dwithin_hole_polygons_vs_oracle(d, opts):
     """ST_DWithin of points against polygons with holes and multipolygons:
