@@ -212,14 +212,34 @@ int SjhipRefinerRefineDistance(struct SedonaSpatialRefiner* self,
                                uint32_t* build_indices, uint32_t* probe_indices,
                                uint32_t indices_size, uint32_t* new_indices_size);
 
-/* KNN join: for every probe point the k nearest build points by Euclidean
- * distance (hypot), ties broken by build id.  Replaces the CPU
- * SpatialIndex::query_knn (rust/sedona-spatial-join/src/index/
+/* KNN join: for every probe row the k nearest build rows by Euclidean
+ * distance (geo's, FP64: hypot between points, point to (multi)linestring
+ * or (multi)polygon), ties broken by build id, k in 1..32.  Replaces the
+ * CPU SpatialIndex::query_knn (rust/sedona-spatial-join/src/index/
  * default_spatial_index.rs:306-502) that the GPU index leaves
  * unimplemented (rust/sedona-spatial-join-gpu/src/index/
- * gpu_spatial_index.rs:252-264).  Outputs are row-major n_probe x k;
- * a probe with fewer than k neighbours gets build id 0xFFFFFFFF and
- * distance +inf in the unused slots.  Build rows must be points. */
+ * gpu_spatial_index.rs:252-264).  Outputs are row-major n_probe x k; a
+ * probe with fewer than k neighbours gets build id 0xFFFFFFFF and distance
+ * +inf in the unused slots.  One side must be points: point probes against
+ * build rows of any geometry, or probes of any geometry against point
+ * build rows. */
+/* KNN with the reference's include_tie_breakers option
+ * (default_spatial_index.rs:379-472): as SjhipRefinerKnn, and with
+ * include_tie_breakers != 0 every further build row whose distance is
+ * within DISTANCE_TOLERANCE = 1e-9 of the k-th distance is kept as well
+ * (index/spatial_index.rs:32).  Rows have variable length: probe row q's
+ * neighbours are entries [out_offsets[q], out_offsets[q+1]) of
+ * out_build_indices / out_distances, sorted by (distance, build id); a
+ * null or empty probe has none.  out_offsets holds n_probe + 1 entries,
+ * the other two `capacity`; when the total exceeds capacity the call
+ * fails with EINVAL and *out_total holds it (retry with larger arrays).
+ * The order among ties at the k-th distance is by build id, as in the
+ * reference's test ground truth. */
+int SjhipRefinerKnnTies(struct SedonaSpatialRefiner* self, const struct ArrowSchema* probe_schema,
+                        const struct ArrowArray* probe_array, uint32_t k, int32_t include_tie_breakers,
+                        uint64_t* out_offsets, uint32_t* out_build_indices, double* out_distances, uint64_t capacity,
+                        uint64_t* out_total);
+
 int SjhipRefinerKnn(struct SedonaSpatialRefiner* self, const struct ArrowSchema* probe_schema,
                     const struct ArrowArray* probe_array, uint32_t k, uint32_t* out_build_indices,
                     double* out_distances);

@@ -41,6 +41,8 @@ def _load(path):
     lib.ora_distance.argtypes = [_P, _I64, _P, _I64]
     lib.ora_distance.restype = ctypes.c_double
     lib.ora_knn.argtypes = [_P, _P, ctypes.c_int, _P, _P, ctypes.c_int]
+    lib.ora_knn_brute.argtypes = [_P, _P, ctypes.c_int, ctypes.c_double, _P, _P, _P, _I64, ctypes.c_int]
+    lib.ora_knn_brute.restype = _I64
     d6 = [ctypes.c_double] * 6
     lib.ora_raycast.argtypes = d6
     lib.ora_collinear.argtypes = d6
@@ -267,3 +269,19 @@ def ref_join(build, probe, brects, prects, pred, nthreads=0, cap_hint=0):
     if n > cap:
         return ref_join(build, probe, brects, prects, pred, nthreads, cap_hint=n)
     return ob[:n], op[:n]
+
+
+def knn_brute(build, probe, k, include_tie_breakers=False, nthreads=0):
+    """KNN over any geometry (one side points) by brute force, rows of
+    variable length: (offsets, ids, distances); with include_tie_breakers
+    every object within 1e-9 of the k-th distance is kept as well."""
+    tol = 1e-9 if include_tie_breakers else -1.0
+    L = lib()
+    off = np.zeros(probe.n + 1, np.int64)
+    n = L.ora_knn_brute(build.h, probe.h, int(k), tol, _ptr(off), None, None, 0, nthreads)
+    if n < 0:
+        raise ValueError("knn: one side must be points")
+    ids = np.zeros(max(n, 1), np.uint32)
+    d = np.zeros(max(n, 1), np.float64)
+    L.ora_knn_brute(build.h, probe.h, int(k), tol, _ptr(off), _ptr(ids), _ptr(d), n, nthreads)
+    return off, ids[:n], d[:n]

@@ -1838,3 +1838,74 @@ int ora_knn(const ora_store* B, const ora_store* P, int k, uint32_t* out_ids, do
   free(off); free(items); free(bx); free(ok);
   return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* KNN by brute force over any geometry (one side points)              */
+/* ------------------------------------------------------------------ */
+
+static int knn_ent_cmp(const void* a, const void* b) {
+  const knn_ent* x = (const knn_ent*)a;
+  const knn_ent* y = (const knn_ent*)b;
+  if (x->d < y->d) return -1;
+  if (x->d > y->d) return 1;
+  return x->id < y->id ? -1 : (x->id > y->id ? 1 : 0);
+}
+
+/* The reference's KNN (default_spatial_index.rs:306-502, ground truth of
+ * spatial_join_integration.rs:1373-1414): per query every object's geo
+ * distance (ora_distance; symmetric when the query is not a point and the
+ * objects are), sorted by (distance, id), the first k kept, and with
+ * tol >= 0 every further object within tol of the k-th distance
+ * (include_tie_breakers, DISTANCE_TOLERANCE).  Rows of variable length:
+ * out_off[q] .. out_off[q+1]; returns the total (outputs written when it
+ * fits cap), or -1 when neither side is points. */
+int64_t ora_knn_brute(const ora_store* B, const ora_store* P, int k, double tol, int64_t* out_off,
+                      uint32_t* out_ids, double* out_d, int64_t cap, int nthreads) {
+  int64_t nb = B->n_geoms, np = P->n_geoms;
+  int bpts = 1, qpts = 1;
+  for (int64_t b = 0; b < nb; b++) if (B->type[b] != ORA_NULL && B->type[b] != ORA_POINT) bpts = 0;
+  for (int64_t q = 0; q < np; q++) if (P->type[q] != ORA_NULL && P->type[q] != ORA_POINT) qpts = 0;
+  if (!bpts && !qpts) return -1;
+  int64_t* cnt = (int64_t*)calloc((size_t)np + 1, 8);
+  knn_ent** rows = (knn_ent**)calloc((size_t)(np > 0 ? np : 1), sizeof(knn_ent*));
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+  for (int64_t q = 0; q < np; q++) {
+    knn_ent* all = (knn_ent*)malloc((size_t)(nb > 0 ? nb : 1) * sizeof(knn_ent));
+    int64_t m = 0;
+    double qx, qy;
+    int qpoint = geom_point(P, q, &qx, &qy);
+    for (int64_t b = 0; b < nb; b++) {
+      double d;
+      if (qpoint) d = ora_distance(B, b, P, q);
+      else if (P->type[q] != ORA_NULL && P->part_off[q] != P->part_off[q + 1]) d = ora_distance(P, q, B, b);
+      else d = NAN;
+      if (d == d) { all[m].d = d; all[m].id = (uint32_t)b; m++; }
+    }
+    qsort(all, (size_t)m, sizeof(knn_ent), knn_ent_cmp);
+    int64_t keep = m < k ? m : k;
+    if (tol >= 0 && m >= k && k > 0) {
+      double thr = all[k - 1].d + tol;
+      while (keep < m && all[keep].d <= thr) keep++;
+    }
+    cnt[q + 1] = keep;
+    rows[q] = all;
+  }
+  for (int64_t q = 0; q < np; q++) cnt[q + 1] += cnt[q];
+  int64_t total = cnt[np];
+  if (out_off) memcpy(out_off, cnt, (size_t)(np + 1) * 8);
+  for (int64_t q = 0; q < np; q++) {
+    if (total <= cap && out_ids && out_d)
+      for (int64_t j = cnt[q]; j < cnt[q + 1]; j++) {
+        out_ids[j] = rows[q][j - cnt[q]].id;
+        out_d[j] = rows[q][j - cnt[q]].d;
+      }
+    free(rows[q]);
+  }
+  free(rows);
+  free(cnt);
+  (void)nthreads;
+  return total;
+}

@@ -112,6 +112,11 @@ int ora_orient2d_sign(double ax, double ay, double bx, double by, double cx, dou
 int ora_segment_intersects(double ax, double ay, double bx, double by,
                            double cx, double cy, double dx, double dy);
 
+/* KNN by brute force over any geometry, one side points, rows of variable
+ * length (tol >= 0: tie breakers within tol of the k-th distance). */
+int64_t ora_knn_brute(const ora_store* B, const ora_store* P, int k, double tol, int64_t* out_off,
+                      uint32_t* out_ids, double* out_d, int64_t cap, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

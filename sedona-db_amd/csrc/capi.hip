@@ -1406,6 +1406,49 @@ int SjhipRefinerKnn(struct SedonaSpatialRefiner* self, const struct ArrowSchema*
   return rc;
 }
 
+int SjhipRefinerKnnTies(struct SedonaSpatialRefiner* self, const struct ArrowSchema* probe_schema,
+                        const struct ArrowArray* probe_array, uint32_t k, int32_t include_tie_breakers,
+                        uint64_t* out_offsets, uint32_t* out_build_indices, double* out_distances, uint64_t capacity,
+                        uint64_t* out_total) {
+  RefinerImpl* rf = self ? (RefinerImpl*)self->private_data : nullptr;
+  std::string err;
+  int rc = guarded(&err, [&] {
+    if (!rf) throw Error("refiner is null or released");
+    if (!rf->built) throw Error("knn before finish_building");
+    if (!out_offsets || !out_total) throw Error("null knn output arrays");
+    *out_total = 0;
+    use_device(rf->rt);
+    ScopedStream ss;
+    GeomColumn probe;
+    decode_wkb(wkb_input_from_arrow(probe_schema, probe_array, false), probe, ss.s);
+    DevArray<unsigned long long> off;
+    DevArray<uint32_t> ids;
+    DevArray<double> d;
+    uint64_t total;
+    {
+      std::lock_guard<std::mutex> g(rf->knn_mu);
+      // DISTANCE_TOLERANCE of the reference (index/spatial_index.rs:32)
+      total = knn_with_ties(rf->build, rf->knn, probe, (int)k, include_tie_breakers ? 1e-9 : -1.0, off, ids, d, ss.s);
+    }
+    *out_total = total;
+    if (total > capacity)
+      throw Error("knn: " + std::to_string(total) + " neighbours exceed the output capacity of " +
+                  std::to_string(capacity) + " (*out_total holds the count)");
+    if (total > 0 && (!out_build_indices || !out_distances)) throw Error("null knn output arrays");
+    SJ_CHECK(hipMemcpyAsync(out_offsets, off.p, (probe.n_rows + 1) * 8, hipMemcpyDeviceToHost, ss.s));
+    if (total) {
+      SJ_CHECK(hipMemcpyAsync(out_build_indices, ids.p, total * 4, hipMemcpyDeviceToHost, ss.s));
+      SJ_CHECK(hipMemcpyAsync(out_distances, d.p, total * 8, hipMemcpyDeviceToHost, ss.s));
+    }
+    SJ_CHECK(hipStreamSynchronize(ss.s));
+  });
+  if (rc && rf) {
+    std::lock_guard<std::mutex> g(rf->mu);
+    rf->last_error = err;
+  }
+  return rc;
+}
+
 int SjhipRefinerKnnDevice(struct SedonaSpatialRefiner* self, const struct ArrowSchema* probe_schema,
                           const struct ArrowDeviceArray* probe_array, uint32_t k, uint32_t* out_build_indices,
                           double* out_distances, double* kernel_ms) {

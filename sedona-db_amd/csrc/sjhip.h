@@ -453,8 +453,17 @@ struct KnnIndex {
   RectTree tree;
   bool built = false;
 };
+// k nearest objects per query into fixed rows of k (id 0xffffffff and
+// distance +inf where fewer exist).  One side must be points.
 void knn_points(const GeomColumn& build, KnnIndex& index, const GeomColumn& probe, int k, uint32_t* out_ids_dev,
                 double* out_d_dev, hipStream_t s, KernelTimer* timer = nullptr);
+// The same with variable-length rows: query q's neighbours are
+// [offsets[q], offsets[q+1]) of out_ids/out_d, sorted by (distance, id);
+// with tol >= 0 every object within tol of the k-th distance is kept too
+// (tie breakers), with tol < 0 at most k.  Returns the total.
+uint64_t knn_with_ties(const GeomColumn& build, KnnIndex& index, const GeomColumn& probe, int k, double tol,
+                       DevArray<unsigned long long>& offsets, DevArray<uint32_t>& out_ids, DevArray<double>& out_d,
+                       hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // Launch sequence without host synchronisation or allocation, for capture

@@ -199,7 +199,7 @@ EXPORTED = [
     "SjhipAutotune", "SjhipRefinerSetSegmentLeaf", "SjhipIndexSetTuning", "SjhipRefinerSetPrecision",
     "SjhipHostRaycast", "SjhipHostRaycastF32", "SjhipHostCollinear", "SjhipHostSegmentIntersects",
     "SjhipHostOrient2d", "SjhipHostHypot", "SjhipHostCoordToLine", "SjhipHostRaycastBatch", "SjhipHostHypotBatch",
-    "SjhipTestingGraphSortPairs", "SjhipJoinDeviceNoAlloc",
+    "SjhipTestingGraphSortPairs", "SjhipJoinDeviceNoAlloc", "SjhipRefinerKnnTies",
 ]
 
 _lib = None
@@ -222,6 +222,9 @@ def lib():
         L.SjhipRefinerRefineDistance.argtypes = [_RF, _SCH, _ARR, ctypes.c_double, c_u32p, c_u32p, ctypes.c_uint32,
                                                  c_u32p]
         L.SjhipRefinerKnn.argtypes = [_RF, _SCH, _ARR, ctypes.c_uint32, c_u32p, ctypes.POINTER(ctypes.c_double)]
+        L.SjhipRefinerKnnTies.argtypes = [_RF, _SCH, _ARR, ctypes.c_uint32, ctypes.c_int32, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                                          ctypes.POINTER(ctypes.c_uint64)]
         L.SjhipJoinDevice.argtypes = [_IX, _RF, _SCH, ctypes.POINTER(ArrowDeviceArray),
                                       ctypes.POINTER(SjhipJoinOptions), ctypes.POINTER(SjhipJoinResult)]
         L.SjhipJoinResultCopyToHost.argtypes = [ctypes.POINTER(SjhipJoinResult), c_u32p, c_u32p]

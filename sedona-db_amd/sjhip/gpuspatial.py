@@ -450,6 +450,28 @@ class GpuSpatialRefiner:
                                         _ptr32(ids), d.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), self._err)
         return ids, d
 
+    def knn_ties(self, probe_array, k, include_tie_breakers=True):
+        """KNN with variable-length rows (SjhipRefinerKnnTies): (offsets,
+        ids, distances); probe q's neighbours are ids[offsets[q]:offsets[q+1]]."""
+        ex = ExportedArrow(probe_array)
+        n = ex.array.length
+        cap = max(1, n * k + 1024)
+        for _ in range(2):
+            off = np.zeros(n + 1, np.uint64)
+            ids = np.zeros(cap, np.uint32)
+            d = np.zeros(cap, np.float64)
+            total = ctypes.c_uint64()
+            rc = self.lib.SjhipRefinerKnnTies(ctypes.byref(self.rf), ctypes.byref(ex.schema), ctypes.byref(ex.array),
+                                              int(k), 1 if include_tie_breakers else 0, off.ctypes.data,
+                                              ids.ctypes.data, d.ctypes.data, cap, ctypes.byref(total))
+            if rc == 0:
+                t = int(total.value)
+                return off, ids[:t], d[:t]
+            if int(total.value) <= cap:
+                break
+            cap = int(total.value)
+        raise GpuSpatialError(self._err())
+
     def knn_device(self, probe, k):
         """KNN of a DeviceWkb probe column with device-resident outputs:
         (ids DeviceBuffer, distances DeviceBuffer, kernel ms)."""

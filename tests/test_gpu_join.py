@@ -667,6 +667,90 @@ def test_knn_exact_with_id_ties(k, opts):
     assert np.array_equal(d.view(np.uint64), od.view(np.uint64))
 
 
+def _knn_rows(ids, d):
+    """Fixed rows (id 0xffffffff / inf padding) as variable-length rows."""
+    keep = ids != 0xFFFFFFFF
+    off = np.concatenate([[0], np.cumsum(keep.sum(axis=1))]).astype(np.uint64)
+    return off, ids[keep], d[keep]
+
+
+@pytest.mark.parametrize("k", [1, 6])
+def test_knn_point_queries_against_polygons_and_lines(k, opts):
+    """Point queries against (multi)polygons with holes and linestrings:
+    geo distances (0 inside a polygon or on a line), ids and FP64 distances
+    identical to the brute-force oracle."""
+    hb, _ = C.hole_corpus()
+    lines = C.column(C.relate_lines())
+    for name, col in (("polygons", hb), ("lines", lines)):
+        b = corpus_arrow(col)
+        rng = np.random.default_rng(43 + k)
+        q = np.round(rng.uniform(-2, 18, size=(3000, 2)) * 8) / 8
+        p = D.points_wkb(q[:, 0], q[:, 1])
+        rf = G.GpuSpatialRefiner(opts)
+        rf.init_build_schema(b)
+        rf.push_build(b)
+        rf.finish_building()
+        ids, d = rf.knn(p.to_arrow(), k)
+        off, oi, od = O.knn_brute(O.Store(col), O.Store(p.as_tuple()), k, nthreads=8)
+        go, gi, gd = _knn_rows(ids, d)
+        say("knn", name, k, len(oi))
+        assert np.array_equal(go, off.astype(np.uint64)) and np.array_equal(gi, oi) and np.array_equal(gd, od), name
+
+
+def test_knn_polygon_queries_against_points(opts):
+    """Queries of any geometry against point objects (the distance of a
+    pair is symmetric)."""
+    hb, hp = C.hole_corpus()
+    rng = np.random.default_rng(47)
+    obj = np.round(rng.uniform(-1, 17, size=(5000, 2)) * 4) / 4
+    b = D.points_wkb(obj[:, 0], obj[:, 1])
+    rf = G.GpuSpatialRefiner(opts)
+    rf.init_build_schema(b.to_arrow())
+    rf.push_build(b.to_arrow())
+    rf.finish_building()
+    for col in (hb, hp):
+        ids, d = rf.knn(corpus_arrow(col), 5)
+        off, oi, od = O.knn_brute(O.Store(b.as_tuple()), O.Store(col), 5, nthreads=8)
+        go, gi, gd = _knn_rows(ids, d)
+        assert np.array_equal(go, off.astype(np.uint64)) and np.array_equal(gi, oi) and np.array_equal(gd, od)
+
+
+@pytest.mark.parametrize("ties", [False, True])
+def test_knn_tie_breakers_vs_oracle(ties, opts):
+    """include_tie_breakers: objects on a grid (exact ties at the k-th
+    distance, some a few ulps apart) and line objects, rows of variable
+    length identical to the brute-force oracle's."""
+    rng = np.random.default_rng(53)
+    obj = np.round(rng.uniform(0, 30, size=(40_000, 2)) * 4) / 4
+    obj[:500] += rng.choice([-1e-12, 0.0, 1e-12], size=(500, 2))
+    q = np.round(rng.uniform(0, 30, size=(4000, 2)) * 4) / 4
+    b, p = D.points_wkb(obj[:, 0], obj[:, 1]), D.points_wkb(q[:, 0], q[:, 1])
+    rf = G.GpuSpatialRefiner(opts)
+    rf.init_build_schema(b.to_arrow())
+    rf.push_build(b.to_arrow())
+    rf.finish_building()
+    for k in (1, 8):
+        goff, gi, gd = rf.knn_ties(p.to_arrow(), k, include_tie_breakers=ties)
+        off, oi, od = O.knn_brute(O.Store(b.as_tuple()), O.Store(p.as_tuple()), k, include_tie_breakers=ties,
+                                  nthreads=8)
+        say("knn ties", ties, k, len(oi))
+        assert np.array_equal(goff, off.astype(np.uint64)) and np.array_equal(gi, oi) and np.array_equal(gd, od)
+        if ties:
+            assert np.any(np.diff(off) > k)
+    # lines as objects, point queries
+    lines = C.column(C.relate_lines())
+    lb = corpus_arrow(lines)
+    rl = G.GpuSpatialRefiner(opts)
+    rl.init_build_schema(lb)
+    rl.push_build(lb)
+    rl.finish_building()
+    q2 = np.round(rng.uniform(-1, 13, size=(2000, 2)) * 2) / 2
+    p2 = D.points_wkb(q2[:, 0], q2[:, 1])
+    goff, gi, gd = rl.knn_ties(p2.to_arrow(), 3, include_tie_breakers=ties)
+    off, oi, od = O.knn_brute(O.Store(lines), O.Store(p2.as_tuple()), 3, include_tie_breakers=ties, nthreads=8)
+    assert np.array_equal(goff, off.astype(np.uint64)) and np.array_equal(gi, oi) and np.array_equal(gd, od)
+
+
 def test_knn_device_outputs_and_resident_tree(opts):
     """The device-output KNN entry point, and repeated calls over the
     object tree kept by the first call, give the oracle's exact lists."""

@@ -388,6 +388,35 @@ def test_knn_matches_brute_force_with_id_ties():
         assert np.array_equal(d, np.take_along_axis(dd, order, axis=1))
 
 
+def test_knn_brute_tie_breakers_match_numpy():
+    """include_tie_breakers (default_spatial_index.rs:379-472): the k nearest
+    by (distance, id), then every further object within 1e-9 of the k-th
+    distance; without it at most k.  Grid points make many exact ties and
+    near-ties a few ulps apart."""
+    rng = np.random.default_rng(29)
+    obj = np.round(rng.uniform(0, 10, size=(2000, 2)) * 4) / 4
+    obj[:50] += rng.choice([-1e-12, 0.0, 1e-12], size=(50, 2))  # ties within the tolerance
+    q = np.round(rng.uniform(0, 10, size=(300, 2)) * 4) / 4
+    from sjhip import datagen as D
+    B = O.Store(D.points_wkb(obj[:, 0], obj[:, 1]).as_tuple())
+    P = O.Store(D.points_wkb(q[:, 0], q[:, 1]).as_tuple())
+    dd = np.hypot(q[:, None, 0] - obj[None, :, 0], q[:, None, 1] - obj[None, :, 1])
+    for k in (1, 4):
+        for ties in (False, True):
+            off, ids, d = O.knn_brute(B, P, k, include_tie_breakers=ties)
+            for i in range(len(q)):
+                order = np.lexsort((np.arange(len(obj)), dd[i]))
+                keep = k
+                if ties:
+                    thr = dd[i, order[k - 1]] + 1e-9
+                    keep = int(np.sum(dd[i] <= thr))
+                assert off[i + 1] - off[i] == keep
+                assert np.array_equal(ids[off[i]:off[i + 1]], order[:keep].astype(np.uint32))
+                assert np.array_equal(d[off[i]:off[i + 1]], dd[i, order[:keep]])
+    off, _, _ = O.knn_brute(B, P, 4, include_tie_breakers=True)
+    assert np.any(np.diff(off) > 4)
+
+
 def test_rect_rounding_is_outward_and_tight():
     """bounds.rs:147-162: lo rounds down, hi rounds up, at most one ulp."""
     rng = np.random.default_rng(3)
