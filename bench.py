@@ -675,18 +675,21 @@ def run_dwithin_walks(args):
     log(f"build side resident: {build_ms:.0f} ms")
     bsz = args.probe_batch
 
-    def step(precision, keep_pairs=False):
+    def step(precision, keep_pairs=False, per_batch=None):
         tot = np.zeros(5, np.int64)
         stage = np.zeros(6)
         kept = []
         for a in range(0, n_points, bsz):
             r = G.join_device(ix, rf, pdev.slice(a, min(n_points, a + bsz)), "dwithin", distance=dist,
-                              precision=precision, sort_output=False, refine_order=args.refine_order, profile=True)
+                              precision=precision, sort_output=per_batch is not None,
+                              refine_order=args.refine_order, profile=True)
             tot += (r.size, r.candidates, r.segment_tests, r.fallback_tests, r.refine_candidates)
             stage += np.array(list(r.res.stage_ms))
             if keep_pairs:
                 gb, gp = r.to_host()
                 kept.append(strip_pairs_mask(strip, gb, (gp + a).astype(np.int64)))
+            if per_batch is not None:
+                per_batch(a, r)
             r.release()
         return tot, stage, kept
 
@@ -703,8 +706,19 @@ def run_dwithin_walks(args):
     if not args.no_plain:
         t1 = time.time()
         ptot, pstage, _ = step(1)
-        plain = dict(ms_per_step=(time.time() - t1) * 1000, pairs=int(ptot[0]), same_pair_count=bool(ptot[0] == pairs),
-                     refine_ms=float(pstage[3]))
+        plain = dict(ms_per_step=(time.time() - t1) * 1000, pairs=int(ptot[0]), refine_ms=float(pstage[3]))
+        # the pair sets, batch by batch, both sorted by (probe, build) on the device
+        mine = {}
+        step(args.precision, per_batch=lambda a, r: mine.__setitem__(a, r.to_host()))
+        diff = [0]
+
+        def compare(a, r):
+            pb, pp = r.to_host()
+            gb, gp = mine.pop(a)
+            diff[0] += pair_set_diff(gb, gp, pb, pp)
+
+        step(1, per_batch=compare)
+        plain["pair_set_diff"] = diff[0]
         log(f"plain fp64: {plain}")
     cpu = None
     if strip is not None:
@@ -718,7 +732,14 @@ def run_dwithin_walks(args):
                         distance=dist, nthreads=threads)
         secs = time.time() - tc
         ck = np.sort((cp.astype(np.uint64) << 32) | cb.astype(np.uint64))
+        # the distances behind the decisions: the library's FP64 distance of
+        # every sampled pair against the oracle's
+        gdist = rf.pair_distances(strip["pcol"].to_arrow(large=True), strip["brows"][cb], cp)
+        odist = O.pair_distances(O.Store(strip["bcol"].as_tuple()), O.Store(strip["pcol"].as_tuple()), cb, cp,
+                                 nthreads=threads)
+        rel = np.abs(gdist - odist) / np.maximum(np.abs(odist), 1e-300)
         cpu = {"value": len(cb) / secs, "unit": "pairs/s", "cores": threads, "kind": "port",
+               "max_distance_rel_error": float(rel.max()) if len(rel) else 0.0,
                "sample": f"vertical strip x in [{strip['x0']:.1f}, {strip['x0'] + strip['w']:.1f}) of line starts "
                          f"and points: {len(strip['brows'])} lines x {len(strip['prows'])} points, {len(cb)} pairs, "
                          f"{secs:.2f} s (WKB parse + grid filter + geo-semantics distance refine)",
@@ -739,7 +760,9 @@ def run_dwithin_walks(args):
                      "refine": stage[3]},
         "roofline_filter": {"bound": "hbm", "achieved": filter_bytes / (stage[2] / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
                             "unit": "GB/s", "frac": filter_bytes / (stage[2] / 1e3) / 1e9 / HBM_PEAK_GBS},
-        "fp64_segment_tests": tests, "exact_ring_scans": scans,
+        "precision": "fp32-certified+fp64-fallback" if args.precision == 0 else "fp64-plain",
+        "segment_tests": tests, "fp64_reevaluations": scans,
+        "fallback_rate": scans / max(1, tests),
         "build_ms": build_ms, "plain_fp64": plain, "cpu_baseline": cpu,
     }
     if plain:

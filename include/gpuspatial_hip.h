@@ -223,6 +223,16 @@ int SjhipRefinerRefineDistance(struct SedonaSpatialRefiner* self,
  * +inf in the unused slots.  One side must be points: point probes against
  * build rows of any geometry, or probes of any geometry against point
  * build rows. */
+/* The distance of each (build row, probe point) pair, FP64, as the
+ * reference's distance refine computes it (geo's Euclidean distance,
+ * rust/sedona-geo-generic-alg/src/algorithm/line_measures/metric_spaces/
+ * euclidean/utils.rs:96-271); NaN for a null or empty row.  The ST_Distance
+ * values behind an ST_DWithin decision; no counterpart in the reference's
+ * C ABI.  Probe rows must be points. */
+int SjhipRefinerPairDistances(struct SedonaSpatialRefiner* self, const struct ArrowSchema* probe_schema,
+                              const struct ArrowArray* probe_array, const uint32_t* build_indices,
+                              const uint32_t* probe_indices, uint64_t n, double* out_distances);
+
 /* KNN with the reference's include_tie_breakers option
  * (default_spatial_index.rs:379-472): as SjhipRefinerKnn, and with
  * include_tie_breakers != 0 every further build row whose distance is

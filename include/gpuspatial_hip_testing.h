@@ -25,6 +25,11 @@ int SjhipHostOrient2d(double ax, double ay, double bx, double by, double cx, dou
 double SjhipHostHypot(double x, double y);
 /* Point to segment distance of the reference's geo refine. */
 double SjhipHostCoordToLine(double px, double py, double sx, double sy, double ex, double ey);
+/* Certified FP32 DWithin term of one segment in ring-local coordinates
+ * (inputs within e_ab / e_p of exact per coordinate): 1 certainly within
+ * d, 0 certainly not, -1 undecided (FP64 fallback needed). */
+int SjhipHostSegmentWithinF32(float ax, float ay, float bx, float by, float px, float py, float e_ab, float e_p,
+                              double d);
 /* Batched forms over arrays (n entries). */
 void SjhipHostRaycastBatch(const double* seg_pts, int64_t n, int32_t* out);
 void SjhipHostHypotBatch(const double* xy, int64_t n, double* out);

@@ -43,6 +43,7 @@ def _load(path):
     lib.ora_knn.argtypes = [_P, _P, ctypes.c_int, _P, _P, ctypes.c_int]
     lib.ora_knn_brute.argtypes = [_P, _P, ctypes.c_int, ctypes.c_double, _P, _P, _P, _I64, ctypes.c_int]
     lib.ora_knn_brute.restype = _I64
+    lib.ora_pair_distances.argtypes = [_P, _P, _P, _P, _I64, _P, ctypes.c_int]
     d6 = [ctypes.c_double] * 6
     lib.ora_raycast.argtypes = d6
     lib.ora_collinear.argtypes = d6
@@ -285,3 +286,12 @@ def knn_brute(build, probe, k, include_tie_breakers=False, nthreads=0):
     d = np.zeros(max(n, 1), np.float64)
     L.ora_knn_brute(build.h, probe.h, int(k), tol, _ptr(off), _ptr(ids), _ptr(d), n, nthreads)
     return off, ids[:n], d[:n]
+
+
+def pair_distances(build, probe, bidx, pidx, nthreads=0):
+    """ora_distance over (build, probe) pairs."""
+    b = np.ascontiguousarray(bidx, np.uint32)
+    p = np.ascontiguousarray(pidx, np.uint32)
+    out = np.zeros(max(len(b), 1), np.float64)
+    lib().ora_pair_distances(build.h, probe.h, _ptr(b), _ptr(p), len(b), _ptr(out), nthreads)
+    return out[:len(b)]

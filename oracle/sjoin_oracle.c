@@ -1909,3 +1909,14 @@ int64_t ora_knn_brute(const ora_store* B, const ora_store* P, int k, double tol,
   (void)nthreads;
   return total;
 }
+
+/* ora_distance over n (build, probe) pairs. */
+void ora_pair_distances(const ora_store* B, const ora_store* P, const uint32_t* bidx, const uint32_t* pidx,
+                        int64_t n, double* out, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 4096)
+#endif
+  for (int64_t i = 0; i < n; i++) out[i] = ora_distance(B, bidx[i], P, pidx[i]);
+  (void)nthreads;
+}

@@ -117,6 +117,10 @@ int ora_segment_intersects(double ax, double ay, double bx, double by,
 int64_t ora_knn_brute(const ora_store* B, const ora_store* P, int k, double tol, int64_t* out_off,
                       uint32_t* out_ids, double* out_d, int64_t cap, int nthreads);
 
+/* ora_distance over n (build, probe) pairs. */
+void ora_pair_distances(const ora_store* B, const ora_store* P, const uint32_t* bidx, const uint32_t* pidx,
+                        int64_t n, double* out, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

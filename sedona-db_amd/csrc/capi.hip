@@ -1406,6 +1406,41 @@ int SjhipRefinerKnn(struct SedonaSpatialRefiner* self, const struct ArrowSchema*
   return rc;
 }
 
+int SjhipRefinerPairDistances(struct SedonaSpatialRefiner* self, const struct ArrowSchema* probe_schema,
+                              const struct ArrowArray* probe_array, const uint32_t* build_indices,
+                              const uint32_t* probe_indices, uint64_t n, double* out_distances) {
+  RefinerImpl* rf = self ? (RefinerImpl*)self->private_data : nullptr;
+  std::string err;
+  int rc = guarded(&err, [&] {
+    if (!rf) throw Error("refiner is null or released");
+    if (!rf->built) throw Error("distances before finish_building");
+    if (n > 0 && (!build_indices || !probe_indices || !out_distances)) throw Error("null distance arrays");
+    use_device(rf->rt);
+    ScopedStream ss;
+    GeomColumn probe;
+    decode_wkb(wkb_input_from_arrow(probe_schema, probe_array, false), probe, ss.s);
+    for (uint64_t i = 0; i < n; i++) {
+      if (build_indices[i] >= (uint64_t)rf->build.n_rows)
+        throw Error("build index out of range: " + std::to_string(build_indices[i]));
+      if (probe_indices[i] >= (uint64_t)probe.n_rows)
+        throw Error("probe index out of range: " + std::to_string(probe_indices[i]));
+    }
+    if (n == 0) return;
+    DevArray<uint32_t> cb(n, ss.s), cp(n, ss.s);
+    DevArray<double> d(n, ss.s);
+    SJ_CHECK(hipMemcpyAsync(cb.p, build_indices, n * 4, hipMemcpyHostToDevice, ss.s));
+    SJ_CHECK(hipMemcpyAsync(cp.p, probe_indices, n * 4, hipMemcpyHostToDevice, ss.s));
+    pair_distances(rf->build, probe, cb.p, cp.p, n, d.p, ss.s);
+    SJ_CHECK(hipMemcpyAsync(out_distances, d.p, n * 8, hipMemcpyDeviceToHost, ss.s));
+    SJ_CHECK(hipStreamSynchronize(ss.s));
+  });
+  if (rc && rf) {
+    std::lock_guard<std::mutex> g(rf->mu);
+    rf->last_error = err;
+  }
+  return rc;
+}
+
 int SjhipRefinerKnnTies(struct SedonaSpatialRefiner* self, const struct ArrowSchema* probe_schema,
                         const struct ArrowArray* probe_array, uint32_t k, int32_t include_tie_breakers,
                         uint64_t* out_offsets, uint32_t* out_build_indices, double* out_distances, uint64_t capacity,

@@ -321,4 +321,37 @@ SJ_HD double rmin(double a, double b) {
   return a < b ? a : b;
 }
 
+// Certified FP32 form of the DWithin term of one segment, over ring-local
+// FP32 coordinates (a, b the segment, p the point):
+// 1 when the reference's FP64 term is certainly <= d, 0 when certainly
+// > d, -1 when it is not certain (the caller then evaluates the FP64 term).
+//   Inputs: a, b within E of the exact local vertices per coordinate and the
+//   point within Ep; the distance from a point to a segment moves by at most
+//   the displacement of the point plus that of the farther endpoint, so the
+//   rounded inputs change it by at most sqrt(2) (E + Ep).
+//   Arithmetic: the projection branch (endpoint a, endpoint b, or the
+//   perpendicular |w x v| / |v|) is evaluated in FP32; each form errs by at
+//   most 12 u (|w| + |w2|) (u = 2^-24; w = p - a, w2 = p - b), and near a
+//   branch boundary the two forms agree to the same order.
+//   The reference's FP64 term (geo's distance_coord_to_line) errs by at most
+//   16 u64 (|w| + |w2|) for the same reason; 1e-14 (|w| + |w2|) covers it.
+// The FP32 distance must exceed d by more than the sum of these bounds (x
+// 1.001) to decide 0, and fall short of it by as much to decide 1.
+SJ_HD int segment_within_f32(float ax, float ay, float bx, float by, float pxl, float pyl, float Eab, float Ep,
+                             double d) {
+  const float vx = bx - ax, vy = by - ay;
+  const float wx = pxl - ax, wy = pyl - ay;
+  const float w2x = pxl - bx, w2y = pyl - by;
+  const float nw = sqrtf(wx * wx + wy * wy), nw2 = sqrtf(w2x * w2x + w2y * w2y);
+  const float c1 = wx * vx + wy * vy, c2 = vx * vx + vy * vy;
+  float dist;
+  if (c1 <= 0.f) dist = nw;
+  else if (c1 >= c2) dist = nw2;
+  else dist = fabsf(wx * vy - wy * vx) / sqrtf(c2);
+  const float m = (1.4142136f * (Eab + Ep) + (12.f * 5.9604645e-08f + 1e-14f) * (nw + nw2)) * 1.001f;
+  if ((double)dist + (double)m <= d) return 1;
+  if ((double)dist - (double)m > d) return 0;
+  return -1;
+}
+
 }  // namespace sjhip

@@ -32,6 +32,10 @@ double SjhipHostCoordToLine(double px, double py, double sx, double sy, double e
 void SjhipHostRaycastBatch(const double* s, int64_t n, int32_t* out) {
   for (int64_t i = 0; i < n; i++) out[i] = raycast(s[6 * i], s[6 * i + 1], s[6 * i + 2], s[6 * i + 3], s[6 * i + 4], s[6 * i + 5]);
 }
+int SjhipHostSegmentWithinF32(float ax, float ay, float bx, float by, float px, float py, float e_ab, float e_p,
+                              double d) {
+  return segment_within_f32(ax, ay, bx, by, px, py, e_ab, e_p, d);
+}
 void SjhipHostHypotBatch(const double* xy, int64_t n, double* out) {
   for (int64_t i = 0; i < n; i++) out[i] = hypot_ref(xy[2 * i], xy[2 * i + 1]);
 }

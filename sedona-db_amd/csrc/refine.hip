@@ -1908,9 +1908,11 @@ __device__ __forceinline__ bool segment_within(const double2* xy, int64_t j, dou
 // Whether some segment of a build ring (or line) is within d of the point.
 // Segments whose FP32 tree boxes lie farther than d from the point, plus
 // the boxes' and the point's FP32 error bounds (4E) and the rounding of d,
-// cannot be within d and are skipped.
+// cannot be within d and are skipped.  With the ring-local FP32 vertices,
+// each remaining segment is decided by segment_within_f32, and only the
+// uncertain ones (counted in fb) by the FP64 term.
 __device__ bool ring_any_within_at(const Ctx& c, int64_t v0, int64_t npts, int64_t toff, float E, double2 o,
-                                   double px, double py, double d, bool check_on, uint64_t& tests) {
+                                   double px, double py, double d, bool check_on, uint64_t& tests, uint64_t& fb) {
   const double2* xy = c.B.xy + v0;
   const int64_t ns = npts - 1;
   const int ls = c.R.seg_leaf_shift;
@@ -1926,11 +1928,20 @@ __device__ bool ring_any_within_at(const Ctx& c, int64_t v0, int64_t npts, int64
   const float pxl = (float)(px - o.x), pyl = (float)(py - o.y);
   const float qx0 = pxl - D, qx1 = pxl + D, qy0 = pyl - D, qy1 = pyl + D;
   const int nsi = (int)ns;
+  // the point's own rounding to local FP32 (its distance from the origin
+  // can exceed the ring's half-extent by d)
+  const float Ep = fmaxf(fabsf(pxl), fabsf(pyl)) * 5.9604645e-08f * 1.0001f;
+  const float2* q32 = c.R.xy32 + v0;
   auto pass = [&](const float4& b) { return b.x <= qx1 && b.z >= qx0 && b.y <= qy1 && b.w >= qy0; };
   auto leaf = [&](int lf) {
     const int s0 = lf << ls, s1 = min(s0 + (1 << ls), nsi);
     for (int j = s0; j < s1; j++) {
       tests++;
+      const float2 qa = q32[j], qb = q32[j + 1];
+      const int r = segment_within_f32(qa.x, qa.y, qb.x, qb.y, pxl, pyl, E, Ep, d);
+      if (r == 1) return true;
+      if (r == 0) continue;
+      fb++;
       if (segment_within(xy, j, px, py, d, dm, check_on)) return true;
     }
     return false;
@@ -1939,10 +1950,10 @@ __device__ bool ring_any_within_at(const Ctx& c, int64_t v0, int64_t npts, int64
 }
 
 __device__ __forceinline__ bool ring_any_within(const Ctx& c, int64_t ring, double px, double py, double d,
-                                                bool check_on, uint64_t& tests) {
+                                                bool check_on, uint64_t& tests, uint64_t& fb) {
   const int64_t v0 = c.B.pt_off[ring];
   return ring_any_within_at(c, v0, c.B.pt_off[ring + 1] - v0, c.R.seg_tree_off ? c.R.seg_tree_off[ring] : -1,
-                            c.R.err[ring], c.R.origin[ring], px, py, d, check_on, tests);
+                            c.R.err[ring], c.R.origin[ring], px, py, d, check_on, tests, fb);
 }
 
 // Position of the point w.r.t. a closed build ring, exactly: 2 boundary,
@@ -1979,7 +1990,7 @@ enum DistKind : int { kDistAny = 0, kDistLines = 1, kDistPolygons = 2 };
 
 template <int KIND>
 __device__ __forceinline__ bool within_distance(const Ctx& c, int64_t row, double px, double py, double d,
-                                                uint64_t& tests, uint64_t& exact_scans) {
+                                                uint64_t& tests, uint64_t& exact_scans, uint64_t& fb) {
   if (!(d >= 0)) return false;  // every distance is >= 0 or NaN
   const GeomView& g = c.B;
   const int t = g.type[row];
@@ -1992,14 +2003,14 @@ __device__ __forceinline__ bool within_distance(const Ctx& c, int64_t row, doubl
     const PolyDesc r = c.R.desc[row];
     if (r.npts >= 0) {
       if (r.npts < 2) return true;  // point_to_run: 0
-      return ring_any_within_at(c, r.v0, r.npts, r.tree_off, r.err, ring_origin(r.rect), px, py, d, true, tests);
+      return ring_any_within_at(c, r.v0, r.npts, r.tree_off, r.err, ring_origin(r.rect), px, py, d, true, tests, fb);
     }
   }
   for (int64_t q = g.part_off[row]; q < g.part_off[row + 1]; q++) {
     const int64_t r0 = g.ring_off[q], r1 = g.ring_off[q + 1];
     if (KIND == kDistLines || (KIND == kDistAny && (t == kLine || t == kMultiLine))) {
       if (g.pt_off[r0 + 1] - g.pt_off[r0] < 2) return true;  // point_to_run: 0
-      if (ring_any_within(c, r0, px, py, d, true, tests)) return true;
+      if (ring_any_within(c, r0, px, py, d, true, tests, fb)) return true;
       continue;
     }
     if constexpr (KIND == kDistLines) continue;
@@ -2016,7 +2027,7 @@ __device__ __forceinline__ bool within_distance(const Ctx& c, int64_t row, doubl
       if (!in_hole) return true;
     }
     for (int64_t k = r0; k < r1; k++)
-      if (ring_any_within(c, k, px, py, d, false, tests)) return true;
+      if (ring_any_within(c, k, px, py, d, false, tests, fb)) return true;
   }
   return false;
 }
@@ -2043,7 +2054,7 @@ __global__ __launch_bounds__(kBlock) void k_refine_distance(Ctx c, double dist, 
   const GeomView& P = c.P;
   SJ_WAVE_OUT(wo);
   if (n_dev) n = min(n, (uint64_t)*n_dev);  // candidate count produced on the device (n: its bound)
-  uint64_t tests = 0, exact_scans = 0;
+  uint64_t tests = 0, exact_scans = 0, fb = 0;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
     uint64_t i = base + threadIdx.x;
@@ -2057,7 +2068,7 @@ __global__ __launch_bounds__(kBlock) void k_refine_distance(Ctx c, double dist, 
       bool bok = B.type[b] != kNull && (B.points_only || B.part_off[b] != B.part_off[b + 1]);
       if (bok && point_of(P, p, px, py))
         keep = plain ? distance_to_point(B, b, px, py) <= dist
-                     : within_distance<KIND>(c, b, px, py, dist, tests, exact_scans);
+                     : within_distance<KIND>(c, b, px, py, dist, tests, exact_scans, fb);
     }
     emit(wo, keep, b, p, ob, op, cap, counter);
   }
@@ -2065,11 +2076,26 @@ __global__ __launch_bounds__(kBlock) void k_refine_distance(Ctx c, double dist, 
   for (int o = 32; o > 0; o >>= 1) {
     tests += __shfl_xor(tests, o);
     exact_scans += __shfl_xor(exact_scans, o);
+    fb += __shfl_xor(fb, o);
   }
   if ((threadIdx.x & 63) == 0 && stats) {
     atomicAdd(&stats[0], (unsigned long long)tests);
     atomicAdd(&stats[1], (unsigned long long)exact_scans);
+    atomicAdd(&stats[2], (unsigned long long)fb);
   }
+}
+
+// The reference distance of each (build row, probe point) pair, FP64
+// (NaN for a null or empty row).
+__global__ __launch_bounds__(kBlock) void k_pair_distance(GeomView B, GeomView P, const uint32_t* cb,
+                                                          const uint32_t* cp, uint64_t n, double* out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t b = cb[i], p = cp[i];
+  double px, py, d = NAN;
+  const bool bok = B.type[b] != kNull && (B.points_only || B.part_off[b] != B.part_off[b + 1]);
+  if (bok && point_of(P, p, px, py)) d = distance_to_point(B, b, px, py);
+  out[i] = d;
 }
 
 // Fused filter + distance refine: the filter walk over the rect tree keeps
@@ -2079,8 +2105,8 @@ template <int KIND>
 struct KeepWithin {
   Ctx c;
   double d;
-  unsigned long long* stats;  // [0] rect hits, [1] FP64 segment tests, [2] exact ring scans
-  uint64_t hits_seen, tests, scans;
+  unsigned long long* stats;  // [0] rect hits, [1] segment tests, [2] exact ring scans, [3] FP64 segment terms
+  uint64_t hits_seen, tests, scans, fb;
   __device__ __forceinline__ uint32_t operator()(uint32_t hits, const uint32_t* ids, uint32_t qid) {
     hits_seen += __popc(hits);
     const double2 pt = c.P.xy[qid];  // a point column: null and empty rows hold NaN
@@ -2090,7 +2116,7 @@ struct KeepWithin {
       const int k = __ffs(m) - 1;
       const uint32_t b = ids[k];
       const bool bok = c.B.type[b] != kNull && (c.B.points_only || c.B.part_off[b] != c.B.part_off[b + 1]);
-      if (bok && within_distance<KIND>(c, b, pt.x, pt.y, d, tests, scans)) kept |= 1u << k;
+      if (bok && within_distance<KIND>(c, b, pt.x, pt.y, d, tests, scans, fb)) kept |= 1u << k;
     }
     return kept;
   }
@@ -2099,11 +2125,13 @@ struct KeepWithin {
       hits_seen += __shfl_xor(hits_seen, o);
       tests += __shfl_xor(tests, o);
       scans += __shfl_xor(scans, o);
+      fb += __shfl_xor(fb, o);
     }
     if ((threadIdx.x & 63) == 0 && stats) {
       atomicAdd(&stats[0], (unsigned long long)hits_seen);
       atomicAdd(&stats[1], (unsigned long long)tests);
       atomicAdd(&stats[2], (unsigned long long)scans);
+      atomicAdd(&stats[3], (unsigned long long)fb);
     }
   }
 };
@@ -2749,6 +2777,15 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
   return cnt;
 }
 
+void pair_distances(const GeomColumn& build, const GeomColumn& probe, const uint32_t* cb, const uint32_t* cp,
+                    uint64_t n, double* out, hipStream_t s) {
+  if (n == 0) return;
+  if (!probe.points_only && (probe.type_mask & ~(1u << kPoint)))
+    throw Error("pair distances: probe rows must be points");
+  hipLaunchKernelGGL(k_pair_distance, grid_for(n), kBlock, 0, s, build.view(), probe.view(), cb, cp, n, out);
+  SJ_CHECK(hipGetLastError());
+}
+
 uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
                          const uint32_t* cb, const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s,
                          KernelTimer* timer, int precision, RefineStats* stats, Workspace* ws, bool raster_done) {
@@ -2764,8 +2801,8 @@ uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const Geo
   c.B = build.view();
   c.R = ri.view();
   c.P = probe.view();
-  DevArray<unsigned long long> aux(2, s);
-  SJ_CHECK(hipMemsetAsync(aux.p, 0, 16, s));
+  DevArray<unsigned long long> aux(3, s);  // segment tests, exact ring scans, FP64 segment terms
+  SJ_CHECK(hipMemsetAsync(aux.p, 0, 24, s));
   const int plain = precision == 1 ? 1 : 0;
   const int kind = plain ? kDistAny : dist_kind(build.type_mask, build.points_only, ri);
   // lines: candidates whose point's square [p - d, p + d] meets no raster
@@ -2801,14 +2838,14 @@ uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const Geo
   }
   SJ_CHECK(hipGetLastError());
   if (timer) timer->record_stop(s);
-  unsigned long long cnt = 0, h[2], walked = n;
+  unsigned long long cnt = 0, h[3], walked = n;
   SJ_CHECK(hipMemcpyAsync(&cnt, out.counter.p, 8, hipMemcpyDeviceToHost, s));
-  SJ_CHECK(hipMemcpyAsync(h, aux.p, 16, hipMemcpyDeviceToHost, s));
+  SJ_CHECK(hipMemcpyAsync(h, aux.p, 24, hipMemcpyDeviceToHost, s));
   if (n_open) SJ_CHECK(hipMemcpyAsync(&walked, n_open, 8, hipMemcpyDeviceToHost, s));
   SJ_CHECK(hipStreamSynchronize(s));
   if (stats) {
     stats->segment_tests += h[0];
-    stats->fallback_tests += h[1];
+    stats->fallback_tests += h[1] + h[2];  // FP64 re-evaluations: exact ring scans and segment terms
     stats->walked += walked;
   }
   return cnt;
@@ -2880,13 +2917,13 @@ uint64_t filter_refine_distance(const RectTree& t, const float4* q, int64_t nq, 
   c.B = build.view();
   c.R = ri.view();
   c.P = probe.view();
-  DevArray<unsigned long long> aux(3, s);
+  DevArray<unsigned long long> aux(4, s);
   TreeView v = tree_view(t);
   const int kind = dist_kind(build.type_mask, build.points_only, ri);
   auto launch = [&](auto kw) {
     kw.c = c;
     kw.d = d;
-    kw.hits_seen = kw.tests = kw.scans = 0;
+    kw.hits_seen = kw.tests = kw.scans = kw.fb = 0;
     kw.stats = aux.p;
     if (t.leaf == 4)
       hipLaunchKernelGGL((k_filter<RectTree::FANOUT, 4, decltype(kw)>), blocks_for(nq), kFilterBlock, 0, s, v, q, nq,
@@ -2900,23 +2937,23 @@ uint64_t filter_refine_distance(const RectTree& t, const float4* q, int64_t nq, 
   };
   for (int attempt = 0; attempt < 2; attempt++) {
     SJ_CHECK(hipMemsetAsync(out.counter.p, 0, 8, s));
-    SJ_CHECK(hipMemsetAsync(aux.p, 0, 24, s));
+    SJ_CHECK(hipMemsetAsync(aux.p, 0, 32, s));
     if (timer) timer->record_start(s);
     if (kind == kDistLines) launch(KeepWithin<kDistLines>{});
     else if (kind == kDistPolygons) launch(KeepWithin<kDistPolygons>{});
     else launch(KeepWithin<kDistAny>{});
     SJ_CHECK(hipGetLastError());
     if (timer) timer->record_stop(s);
-    unsigned long long total = 0, h[3];
+    unsigned long long total = 0, h[4];
     SJ_CHECK(hipMemcpyAsync(&total, out.counter.p, 8, hipMemcpyDeviceToHost, s));
-    SJ_CHECK(hipMemcpyAsync(h, aux.p, 24, hipMemcpyDeviceToHost, s));
+    SJ_CHECK(hipMemcpyAsync(h, aux.p, 32, hipMemcpyDeviceToHost, s));
     SJ_CHECK(hipStreamSynchronize(s));
     t.kept_per_query_x16.store((uint64_t)((total * 16) / (uint64_t)nq + 1), std::memory_order_relaxed);
     if (total <= out.cap()) {
       if (candidates) *candidates = h[0];
       if (stats) {
         stats->segment_tests += h[1];
-        stats->fallback_tests += h[2];
+        stats->fallback_tests += h[2] + h[3];
       }
       return total;
     }

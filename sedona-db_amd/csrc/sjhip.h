@@ -425,6 +425,9 @@ bool raster_split_distance(const GeomColumn& build, const RingIndex& ri, const G
 // Distance refine (build.distance(probe) <= d); probe must be points.
 // precision 1: the all-FP64 form over every segment.  stats: FP64 segment
 // distance tests, and exact ring scans (uncertain certified crossing counts).
+// FP64 reference distance (build row cb[i], probe point cp[i]) into out[i].
+void pair_distances(const GeomColumn& build, const GeomColumn& probe, const uint32_t* cb, const uint32_t* cp,
+                    uint64_t n, double* out, hipStream_t s);
 uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
                          const uint32_t* cb, const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s,
                          KernelTimer* timer = nullptr, int precision = 0, RefineStats* stats = nullptr,

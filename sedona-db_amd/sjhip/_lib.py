@@ -199,7 +199,8 @@ EXPORTED = [
     "SjhipAutotune", "SjhipRefinerSetSegmentLeaf", "SjhipIndexSetTuning", "SjhipRefinerSetPrecision",
     "SjhipHostRaycast", "SjhipHostRaycastF32", "SjhipHostCollinear", "SjhipHostSegmentIntersects",
     "SjhipHostOrient2d", "SjhipHostHypot", "SjhipHostCoordToLine", "SjhipHostRaycastBatch", "SjhipHostHypotBatch",
-    "SjhipTestingGraphSortPairs", "SjhipJoinDeviceNoAlloc", "SjhipRefinerKnnTies",
+    "SjhipTestingGraphSortPairs", "SjhipJoinDeviceNoAlloc", "SjhipRefinerKnnTies", "SjhipHostSegmentWithinF32",
+    "SjhipRefinerPairDistances",
 ]
 
 _lib = None
@@ -222,6 +223,8 @@ def lib():
         L.SjhipRefinerRefineDistance.argtypes = [_RF, _SCH, _ARR, ctypes.c_double, c_u32p, c_u32p, ctypes.c_uint32,
                                                  c_u32p]
         L.SjhipRefinerKnn.argtypes = [_RF, _SCH, _ARR, ctypes.c_uint32, c_u32p, ctypes.POINTER(ctypes.c_double)]
+        L.SjhipRefinerPairDistances.argtypes = [_RF, _SCH, _ARR, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                                                ctypes.c_void_p]
         L.SjhipRefinerKnnTies.argtypes = [_RF, _SCH, _ARR, ctypes.c_uint32, ctypes.c_int32, ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
                                           ctypes.POINTER(ctypes.c_uint64)]
@@ -267,6 +270,7 @@ def lib():
         L.SjhipHostCoordToLine.restype = ctypes.c_double
         L.SjhipHostRaycastBatch.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
         L.SjhipHostHypotBatch.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+        L.SjhipHostSegmentWithinF32.argtypes = [ctypes.c_float] * 8 + [ctypes.c_double]
         L.SjhipTestingGraphSortPairs.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                                                          ctypes.c_int]
         _lib = L

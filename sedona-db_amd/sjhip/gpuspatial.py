@@ -450,6 +450,16 @@ class GpuSpatialRefiner:
                                         _ptr32(ids), d.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), self._err)
         return ids, d
 
+    def pair_distances(self, probe_array, build_indices, probe_indices):
+        """FP64 reference distances of (build row, probe point) pairs."""
+        ex = ExportedArrow(probe_array)
+        b = np.ascontiguousarray(build_indices, np.uint32)
+        p = np.ascontiguousarray(probe_indices, np.uint32)
+        out = np.zeros(max(len(b), 1), np.float64)
+        _check(self.lib.SjhipRefinerPairDistances(ctypes.byref(self.rf), ctypes.byref(ex.schema), ctypes.byref(ex.array),
+                                                  b.ctypes.data, p.ctypes.data, len(b), out.ctypes.data), self._err)
+        return out[:len(b)]
+
     def knn_ties(self, probe_array, k, include_tie_breakers=True):
         """KNN with variable-length rows (SjhipRefinerKnnTies): (offsets,
         ids, distances); probe q's neighbours are ids[offsets[q]:offsets[q+1]]."""

@@ -635,6 +635,45 @@ def test_dwithin_hole_polygons_vs_oracle(d, opts):
     assert np.array_equal(gb, ob) and np.array_equal(gp, op), (d, len(gb), len(ob))
 
 
+def test_pair_distances_match_oracle(opts):
+    """SjhipRefinerPairDistances: the FP64 distance of every (polygon or
+    line, point) pair of the hole and relate corpora equals the oracle's bit
+    for bit (0 inside polygons and on lines, holes included)."""
+    hb, _ = C.hole_corpus()
+    rng = np.random.default_rng(59)
+    q = np.concatenate([np.round(rng.uniform(-2, 18, size=(600, 2)) * 4) / 4, rng.uniform(-2, 18, size=(600, 2))])
+    p = D.points_wkb(q[:, 0], q[:, 1])
+    for col in (hb, C.column(C.relate_lines()), C.column(C.relate_polygons())):
+        b = corpus_arrow(col)
+        rf = G.GpuSpatialRefiner(opts)
+        rf.init_build_schema(b)
+        rf.push_build(b)
+        rf.finish_building()
+        nb = len(b)
+        bi = np.repeat(np.arange(nb), len(q)).astype(np.uint32)
+        pi = np.tile(np.arange(len(q)), nb).astype(np.uint32)
+        got = rf.pair_distances(p.to_arrow(), bi, pi)
+        want = O.pair_distances(O.Store(col), O.Store(p.as_tuple()), bi, pi, nthreads=8)
+        assert np.array_equal(got, want, equal_nan=True), int(np.sum(~((got == want) | (np.isnan(got) & np.isnan(want)))))
+        assert np.sum(got == 0) > 100
+
+
+@pytest.mark.parametrize("d", [0.0, 0.05, 0.4])
+def test_dwithin_certified_equals_plain_fp64(d, opts):
+    """The certified FP32 distance terms with FP64 re-evaluation give the
+    pair set of the all-FP64 plain kernel, on random walks (line rasters and
+    segment trees) and on Zipf polygons."""
+    for b, p in (D.workload_dwithin_walks(400_000, 20_000, seed=61), D.workload_pip_zipf(200_000, 2_000, seed=62)):
+        ix, rf = build_pair(b.to_arrow(), opts)
+        dev = G.DeviceWkb.from_host(p)
+        r0 = G.join_device(ix, rf, dev, "dwithin", distance=d, precision=0, sort_output=True)
+        r1 = G.join_device(ix, rf, dev, "dwithin", distance=d, precision=1, sort_output=True)
+        a, b_ = r0.to_host(), r1.to_host()
+        say("certified dwithin", d, len(a[0]), r0.segment_tests, r0.fallback_tests)
+        assert np.array_equal(a[0], b_[0]) and np.array_equal(a[1], b_[1]), (d, len(a[0]), len(b_[0]))
+        assert len(a[0]) > 0 or d == 0.0
+
+
 def test_dwithin_polygons_and_refine_distance_abi(opts):
     b, p = D.workload_pip_uniform(20_000, 200, seed=17, extent=100.0 * np.sqrt(200 / 10_000))
     ix, rf = build_pair(b.to_arrow(), opts)

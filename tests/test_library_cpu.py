@@ -165,3 +165,46 @@ def test_fp32_certified_raycast_never_contradicts_fp64():
             want = lib.SjhipHostRaycast(sx * a[0], a[1], sx * b[0], b[1], sx * p[0], p[1])
         assert f == want
     assert decided > 20000
+
+
+def test_fp32_certified_distance_never_contradicts_fp64():
+    """The certified FP32 DWithin term (ring-local FP32 vertices with the
+    library's bound E = 2^-22 H and the point's own rounding) never decides
+    otherwise than the reference's FP64 term (on the segment, or
+    distance_coord_to_line <= d): points placed at distance d plus or minus
+    tiny offsets, on the segment, past its ends; short and long segments;
+    rings of extent 1e-3 .. 1e3 at coordinates up to 1e4."""
+    rng = np.random.default_rng(17)
+    lib = L.lib()
+    decided = total = 0
+    seen = {0: 0, 1: 0}
+    for _ in range(40000):
+        o = rng.uniform(-1e4, 1e4, 2)
+        H = 10.0 ** rng.uniform(-3, 3)
+        a = o + rng.uniform(-H, H, 2)
+        b = a + rng.normal(0, H * 10.0 ** rng.uniform(-4, 0), 2)
+        d = H * 10.0 ** rng.uniform(-3, 0.5) if rng.random() < 0.9 else 0.0
+        t = rng.uniform(-0.3, 1.3)
+        v = b - a
+        nv = np.hypot(*v) or 1.0
+        nrm = np.array([-v[1], v[0]]) / nv
+        off = d * (1 + rng.choice([0.0, 1e-12, -1e-12, 1e-7, -1e-7, 1e-4, -1e-4, 0.3, -0.3]))
+        if rng.random() < 0.1:
+            off = 0.0  # on the segment's line
+        p = a + t * v + off * nrm
+        E = np.float32(H * 2.384185791015625e-07) * np.float32(1.0001)
+        loc = [np.float32(x) for x in (a[0] - o[0], a[1] - o[1], b[0] - o[0], b[1] - o[1], p[0] - o[0], p[1] - o[1])]
+        Ep = np.float32(max(abs(loc[4]), abs(loc[5]))) * np.float32(5.9604645e-08) * np.float32(1.0001)
+        f = lib.SjhipHostSegmentWithinF32(*[float(x) for x in loc], float(E), float(Ep), float(d))
+        total += 1
+        if f < 0:
+            continue
+        decided += 1
+        seen[f] += 1
+        on = (lib.SjhipHostOrient2d(a[0], a[1], b[0], b[1], p[0], p[1]) == 0 and
+              min(a[0], b[0]) <= p[0] <= max(a[0], b[0]) and min(a[1], b[1]) <= p[1] <= max(a[1], b[1]))
+        want = on or lib.SjhipHostCoordToLine(p[0], p[1], a[0], a[1], b[0], b[1]) <= d
+        assert bool(f) == want, (a, b, p, d, f)
+    # most samples sit within 1e-7 of the threshold on purpose; those far
+    # from it are decided both ways
+    assert decided > 0.4 * total and min(seen.values()) > 5000, (decided, total, seen)
