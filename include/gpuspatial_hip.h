@@ -382,6 +382,13 @@ int SjhipRefinerSetSegmentLeaf(struct SedonaSpatialRefiner* refiner, int32_t seg
  * concurrently with joins on the index. */
 int SjhipIndexSetTuning(struct SedonaFloatIndex2D* index, int32_t kind, int32_t refine_order, int32_t pip_waves);
 
+/* Filter of the index's probes: 0 (the default) walks the cell grid for
+ * point probes and the rect tree for others, 1 walks the tree always, 2
+ * the grid whenever the index has one (any probe rects).  Every mode gives
+ * the same candidates; for tests and measurements.  No counterpart in the
+ * reference.  Must not run concurrently with probes of the index. */
+int SjhipIndexSetFilterMode(struct SedonaFloatIndex2D* index, int32_t mode);
+
 /* Precision of the refiner's refine and SjhipRefinerRefineDistance calls:
  * 0 (the default) runs the certified FP32 tests with FP64 re-evaluation
  * where a sign is uncertain, 1 the all-FP64 plain kernels.  Both give the

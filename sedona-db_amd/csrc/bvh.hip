@@ -24,6 +24,7 @@
 
 #include "lbvh_filter.h"
 #include <algorithm>
+#include <cmath>
 #include "sjhip.h"
 #include "wave_out.h"
 
@@ -167,6 +168,201 @@ void sort_keys(DevArray<uint32_t>& keys, DevArray<uint32_t>& ids, int64_t n, hip
 
 }  // namespace
 
+// ---- cell grid (RectTree::grid_*) ----------------------------------------
+
+namespace {
+
+// bounds of the valid rects as ordered uints: min x0, min y0, max x1, max y1
+__global__ void k_rect_bounds(const float4* r, int64_t n, unsigned int* acc) {
+  __shared__ unsigned int s[4][kBlock];
+  const int t = threadIdx.x;
+  unsigned int a = 0xffffffffu, b = 0xffffffffu, c = 0, d = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + t; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float4 q = r[i];
+    if (!rect_ok(q)) continue;
+    a = min(a, f2o(q.x)); b = min(b, f2o(q.y));
+    c = max(c, f2o(q.z)); d = max(d, f2o(q.w));
+  }
+  s[0][t] = a; s[1][t] = b; s[2][t] = c; s[3][t] = d;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (t < o) {
+      s[0][t] = min(s[0][t], s[0][t + o]);
+      s[1][t] = min(s[1][t], s[1][t + o]);
+      s[2][t] = max(s[2][t], s[2][t + o]);
+      s[3][t] = max(s[3][t], s[3][t + o]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    atomicMin(&acc[0], s[0][0]);
+    atomicMin(&acc[1], s[1][0]);
+    atomicMax(&acc[2], s[2][0]);
+    atomicMax(&acc[3], s[3][0]);
+  }
+}
+
+// max(width, height) of every stride-th rect (-1 for an invalid one)
+__global__ void k_rect_extents(const float4* r, int64_t n, int64_t stride, int n_out, float* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_out) return;
+  const int64_t k = (int64_t)i * stride;
+  if (k >= n) {
+    out[i] = -1.f;
+    return;
+  }
+  const float4 b = r[k];
+  out[i] = rect_ok(b) ? fmaxf(b.z - b.x, b.w - b.y) : -1.f;
+}
+
+__global__ void k_grid_counts(const float4* r, int64_t n, GridView g, uint32_t* counts) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 b = r[i];
+  uint32_t c = 0;
+  if (rect_ok(b)) {
+    const int cx0 = grid_cell(b.x, g.x0, g.inv_x, g.nx), cx1 = grid_cell(b.z, g.x0, g.inv_x, g.nx);
+    const int cy0 = grid_cell(b.y, g.y0, g.inv_y, g.ny), cy1 = grid_cell(b.w, g.y0, g.inv_y, g.ny);
+    const uint64_t cc = (uint64_t)(cx1 - cx0 + 1) * (uint64_t)(cy1 - cy0 + 1);
+    c = cc > 0xffffffffull ? 0xffffffffu : (uint32_t)cc;
+  }
+  counts[i] = c;
+}
+
+__global__ void k_grid_fill(const float4* r, int64_t n, GridView g, const uint64_t* start, uint32_t* keys,
+                            uint32_t* vals) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 b = r[i];
+  if (!rect_ok(b)) return;
+  const int cx0 = grid_cell(b.x, g.x0, g.inv_x, g.nx), cx1 = grid_cell(b.z, g.x0, g.inv_x, g.nx);
+  const int cy0 = grid_cell(b.y, g.y0, g.inv_y, g.ny), cy1 = grid_cell(b.w, g.y0, g.inv_y, g.ny);
+  uint64_t w = start[i];
+  for (int cy = cy0; cy <= cy1; cy++)
+    for (int cx = cx0; cx <= cx1; cx++) {
+      keys[w] = (uint32_t)cy * (uint32_t)g.nx + (uint32_t)cx;
+      vals[w] = (uint32_t)i;
+      w++;
+    }
+}
+
+// off[c] = first entry of cell c (entries sorted by cell), off[ncells] = T
+__global__ void k_grid_starts(const uint32_t* keys, int64_t T, int64_t ncells, uint32_t* off) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e > T) return;
+  const int64_t lo = e == 0 ? 0 : (int64_t)keys[e - 1] + 1;
+  const int64_t hi = e == T ? ncells : (int64_t)keys[e];
+  for (int64_t c = lo; c <= hi; c++) off[c] = (uint32_t)e;
+}
+
+__global__ void k_grid_gather(const float4* r, const uint32_t* vals, int64_t T, float4* out_r, uint32_t* out_id) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= T) return;
+  const uint32_t v = vals[e];
+  out_r[e] = r[v];
+  out_id[e] = v;
+}
+
+}  // namespace
+
+// Cell width as a multiple of the median rect extent: narrower cells list
+// each rect in more cells but leave fewer rects to test per query.
+#ifndef SJ_GRID_CELL_SCALE
+#define SJ_GRID_CELL_SCALE 1.0
+#endif
+constexpr double kGridCellScale = SJ_GRID_CELL_SCALE;
+
+// Builds t's cell grid over the n rects.  Cells are kGridCellScale times
+// the median rect extent wide (coarsened while the rects would be listed
+// more than 16 times each on average); no grid when the rects' bounds are
+// degenerate or too many entries remain.
+void build_rect_grid(const float4* rects, int64_t n, RectTree& t, hipStream_t s) {
+  t.grid_nx = t.grid_ny = 0;
+  t.grid_off.release();
+  t.grid_rects.release();
+  t.grid_ids.release();
+  if (n <= 0 || t.n_valid <= 0) return;
+  DevArray<unsigned int> acc(4, s);
+  const unsigned int init[4] = {0xffffffffu, 0xffffffffu, 0u, 0u};
+  SJ_CHECK(hipMemcpyAsync(acc.p, init, sizeof(init), hipMemcpyHostToDevice, s));
+  const unsigned grid = (unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, blocks_for(n)));
+  hipLaunchKernelGGL(k_rect_bounds, grid, kBlock, 0, s, rects, n, acc.p);
+  const int n_sample = (int)std::min<int64_t>(n, 4096);
+  const int64_t stride = std::max<int64_t>(1, n / n_sample);
+  DevArray<float> ext(n_sample, s);
+  hipLaunchKernelGGL(k_rect_extents, blocks_for(n_sample), kBlock, 0, s, rects, n, stride, n_sample, ext.p);
+  SJ_CHECK(hipGetLastError());
+  unsigned int hb[4];
+  std::vector<float> he(n_sample);
+  SJ_CHECK(hipMemcpyAsync(hb, acc.p, sizeof(hb), hipMemcpyDeviceToHost, s));
+  SJ_CHECK(hipMemcpyAsync(he.data(), ext.p, n_sample * sizeof(float), hipMemcpyDeviceToHost, s));
+  SJ_CHECK(hipStreamSynchronize(s));
+  const float X0 = o2f(hb[0]), Y0 = o2f(hb[1]), X1 = o2f(hb[2]), Y1 = o2f(hb[3]);
+  if (!(X1 > X0) || !(Y1 > Y0) || !std::isfinite(X1 - X0) || !std::isfinite(Y1 - Y0)) return;
+  std::vector<float> valid;
+  for (float v : he)
+    if (v >= 0.f) valid.push_back(v);
+  if (valid.empty()) return;
+  std::nth_element(valid.begin(), valid.begin() + valid.size() / 2, valid.end());
+  double cs = std::max<double>(valid[valid.size() / 2] * kGridCellScale, std::max(X1 - X0, Y1 - Y0) / 4096.0);
+  DevArray<uint32_t> counts(n, s);
+  DevArray<uint64_t> start(n + 1, s);
+  size_t tb = 0;
+  SJ_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, counts.p, start.p, (int)(n + 1), s));
+  DevArray<uint8_t> tmp(tb + 16, s);
+  GridView g{};
+  uint64_t total = 0;
+  for (int attempt = 0;; attempt++) {
+    const double nxd = std::ceil((X1 - X0) / cs), nyd = std::ceil((Y1 - Y0) / cs);
+    if (nxd * nyd > double(1 << 24)) {
+      cs *= 2;
+      continue;
+    }
+    g.nx = std::max(1, (int)nxd);
+    g.ny = std::max(1, (int)nyd);
+    g.x0 = X0;
+    g.y0 = Y0;
+    g.inv_x = (float)(g.nx / (double(X1) - X0));
+    g.inv_y = (float)(g.ny / (double(Y1) - Y0));
+    hipLaunchKernelGGL(k_grid_counts, blocks_for(n), kBlock, 0, s, rects, n, g, counts.p);
+    // start[i] = entries before rect i (start[n] = the total), in 64 bits
+    SJ_CHECK(hipMemsetAsync(start.p + n, 0, 8, s));
+    SJ_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, counts.p, start.p, (int)n, s));
+    uint64_t last_start = 0;
+    uint32_t last_cnt = 0;
+    SJ_CHECK(hipMemcpyAsync(&last_start, start.p + n - 1, 8, hipMemcpyDeviceToHost, s));
+    SJ_CHECK(hipMemcpyAsync(&last_cnt, counts.p + n - 1, 4, hipMemcpyDeviceToHost, s));
+    SJ_CHECK(hipStreamSynchronize(s));
+    total = last_start + last_cnt;
+    if (total <= (uint64_t)16 * (uint64_t)t.n_valid + 1024 && total < 0xffffffffull) break;
+    if (attempt >= 8) return;  // no grid: the tree serves these rects
+    cs *= 2;
+  }
+  const int64_t T = (int64_t)total;
+  const int64_t ncells = (int64_t)g.nx * g.ny;
+  DevArray<uint32_t> keys(T, s), vals(T, s), k2(T, s), v2(T, s);
+  hipLaunchKernelGGL(k_grid_fill, blocks_for(n), kBlock, 0, s, rects, n, g, start.p, keys.p, vals.p);
+  int bits = 1;
+  while ((int64_t(1) << bits) < ncells) bits++;
+  size_t sb = 0;
+  SJ_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, sb, keys.p, k2.p, vals.p, v2.p, (int)T, 0, bits, s));
+  DevArray<uint8_t> st(sb + 16, s);
+  SJ_CHECK(hipcub::DeviceRadixSort::SortPairs(st.p, sb, keys.p, k2.p, vals.p, v2.p, (int)T, 0, bits, s));
+  t.grid_off.alloc(ncells + 1, s);
+  hipLaunchKernelGGL(k_grid_starts, blocks_for(T + 1), kBlock, 0, s, k2.p, T, ncells, t.grid_off.p);
+  t.grid_rects.alloc(T > 0 ? T : 1, s);
+  t.grid_ids.alloc(T > 0 ? T : 1, s);
+  if (T > 0) hipLaunchKernelGGL(k_grid_gather, blocks_for(T), kBlock, 0, s, rects, v2.p, T, t.grid_rects.p, t.grid_ids.p);
+  SJ_CHECK(hipGetLastError());
+  SJ_CHECK(hipStreamSynchronize(s));
+  t.grid_nx = g.nx;
+  t.grid_ny = g.ny;
+  t.grid_x0 = g.x0;
+  t.grid_y0 = g.y0;
+  t.grid_inv_x = g.inv_x;
+  t.grid_inv_y = g.inv_y;
+}
+
 void build_rect_tree(const float4* rects, int64_t n, RectTree& t, hipStream_t s, int leaf) {
   t.level_off.clear();
   t.level_cnt.clear();
@@ -230,6 +426,7 @@ void build_rect_tree(const float4* rects, int64_t n, RectTree& t, hipStream_t s,
   SJ_CHECK(hipStreamSynchronize(s));
   std::nth_element(h.begin(), h.begin() + n_sample / 2, h.end());
   t.coop_extent = 2.0f * h[n_sample / 2];
+  build_rect_grid(rects, n, t, s);
 }
 
 #ifndef SJ_FILTER_WAVE
@@ -264,8 +461,27 @@ void launch_plain_filter(const RectTree& t, const TreeView& v, const float4* q, 
   }
 }
 
+// The grid filter for point queries when the tree has a grid (the tree
+// walk otherwise).
+bool use_grid(const RectTree& t, bool point_probes) {
+#ifdef SJ_NO_GRID_FILTER
+  return false;
+#else
+  if (t.grid_nx <= 0 || t.filter_mode == 1) return false;
+  return point_probes || t.filter_mode == 2;
+#endif
+}
+
+void launch_grid_filter(const RectTree& t, const float4* q, const double2* xy, int64_t nq, const uint32_t* qorder,
+                        int er, uint32_t* ob, uint32_t* op, uint64_t cap, unsigned long long* counter, hipStream_t s) {
+  const unsigned grid = (unsigned)((nq + kFilterGridBlock - 1) / kFilterGridBlock);
+  hipLaunchKernelGGL(k_filter_grid, grid, kFilterGridBlock, 0, s, grid_view(t), q, xy, nq, qorder, er, ob, op, cap,
+                     counter);
+}
+
 uint64_t filter_pairs(const RectTree& t, const float4* q, int64_t nq, const uint32_t* qorder, PairBuf& pb,
-                      hipStream_t s, KernelTimer* timer, bool emit_rank) {
+                      hipStream_t s, KernelTimer* timer, bool emit_rank, bool point_probes,
+                      const double2* ordered_points) {
   if (pb.counter.n < 1) pb.counter.alloc(1, s);
   // first capacity: the tree's observed candidates per query (16ths) from
   // earlier probes, else 4 per query; one resize-and-rerun on overflow
@@ -280,7 +496,11 @@ uint64_t filter_pairs(const RectTree& t, const float4* q, int64_t nq, const uint
   for (int attempt = 0; attempt < 2; attempt++) {
     SJ_CHECK(hipMemsetAsync(pb.counter.p, 0, 8, s));
     if (timer) timer->record_start(s);
-    launch_plain_filter(t, v, q, nq, qorder, emit_rank ? 1 : 0, pb.b.p, pb.p.p, pb.cap(), pb.counter.p, s);
+    if (use_grid(t, point_probes))
+      launch_grid_filter(t, q, emit_rank ? ordered_points : nullptr, nq, qorder, emit_rank ? 1 : 0, pb.b.p, pb.p.p,
+                         pb.cap(), pb.counter.p, s);
+    else
+      launch_plain_filter(t, v, q, nq, qorder, emit_rank ? 1 : 0, pb.b.p, pb.p.p, pb.cap(), pb.counter.p, s);
     SJ_CHECK(hipGetLastError());
     if (timer) timer->record_stop(s);
     unsigned long long total = 0;
@@ -341,10 +561,15 @@ void morton_order_fixed(const RectTree& t, const float4* rects, int64_t n, uint3
 }
 
 void launch_filter(const RectTree& t, const float4* q, int64_t nq, const uint32_t* qorder, bool emit_rank,
-                   uint32_t* ob, uint32_t* op, uint64_t cap, unsigned long long* counter, hipStream_t s) {
+                   uint32_t* ob, uint32_t* op, uint64_t cap, unsigned long long* counter, hipStream_t s,
+                   bool point_probes) {
   if (nq <= 0 || t.n_levels == 0) return;
-  TreeView v = tree_view(t);
-  launch_plain_filter(t, v, q, nq, qorder, emit_rank ? 1 : 0, ob, op, cap, counter, s);
+  if (use_grid(t, point_probes)) {
+    launch_grid_filter(t, q, nullptr, nq, qorder, emit_rank ? 1 : 0, ob, op, cap, counter, s);
+  } else {
+    TreeView v = tree_view(t);
+    launch_plain_filter(t, v, q, nq, qorder, emit_rank ? 1 : 0, ob, op, cap, counter, s);
+  }
   SJ_CHECK(hipGetLastError());
 }
 

@@ -222,8 +222,15 @@ int ix_probe(SedonaFloatIndex2D* self, SedonaSpatialIndexContext* context, const
     SJ_CHECK(hipMemcpyAsync(q.p, buf, (size_t)n * sizeof(float4), hipMemcpyHostToDevice, ss.s));
     DevArray<uint32_t> order;
     int64_t nv = morton_order(q.p, n, order, ss.s);
+    // point rects (the operator's rects of point rows, nulls as NaN) take
+    // the grid filter
+    bool points = true;
+    for (uint32_t i = 0; i < n && points; i++) {
+      const float* r = buf + 4 * (size_t)i;
+      points = (r[0] == r[2] && r[1] == r[3]) || !(r[0] <= r[2] && r[1] <= r[3]);
+    }
     PairBuf pb;
-    uint64_t cnt = filter_pairs(ix->tree, q.p, nv, order.p, pb, ss.s);
+    uint64_t cnt = filter_pairs(ix->tree, q.p, nv, order.p, pb, ss.s, nullptr, false, points);
     std::vector<uint32_t> hb(cnt), hp(cnt);
     if (cnt) {
       SJ_CHECK(hipMemcpyAsync(hb.data(), pb.b.p, cnt * 4, hipMemcpyDeviceToHost, ss.s));
@@ -669,7 +676,8 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
     // result gets the kept pairs at their exact size
     WorkspaceLease lease(rf, s);
     Workspace* ws = lease.ws.get();
-    nc = filter_pairs(ix->tree, q.p, nq, ord, ws->cand, s, ev.on ? &ft : nullptr, ranked);
+    nc = filter_pairs(ix->tree, q.p, nq, ord, ws->cand, s, ev.on ? &ft : nullptr, ranked,
+                      probe.points_only && !dwithin, ranked ? rprobe.xy.p : nullptr);
     mark(3);
     // point in polygon, polygon-major order: candidates in clean raster
     // cells are decided first, so that the regrouping sorts only the rest
@@ -876,7 +884,7 @@ std::unique_ptr<GraphPlan> capture_plan(IndexImpl* ix, RefinerImpl* rf, int kind
                        g->sort_bytes, s);
     if (!dwithin) {
       gather_points_into(g->pcol.xy.p, g->order.p, B, g->rcol.xy.p, s);
-      launch_filter(ix->tree, g->rects.p, B, g->order.p, true, g->cb.p, g->cp.p, C, g->cnt.p + 1, s);
+      launch_filter(ix->tree, g->rects.p, B, g->order.p, true, g->cb.p, g->cp.p, C, g->cnt.p + 1, s, true);
       launch_pip_refine_dev(rf->build, rf->ring_index, g->rcol, o.predicate, g->cb.p, g->cp.p, g->cnt.p + 1, C,
                             g->order.p, g->redo.p, g->cnt.p + 2, g->ob.p, g->op.p, C, g->cnt.p + 3, g->cnt.p + 4,
                             g->pip_waves, s);
@@ -1105,7 +1113,7 @@ void join_no_alloc(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const Sjh
   if (!dwithin) {
     BorrowedPoints rcol(rxy, nullptr, n);
     gather_points_into(xy, order, n, rxy, s);
-    launch_filter(ix->tree, rects, n, order, true, cb, cp, C, cnt + 1, s);
+    launch_filter(ix->tree, rects, n, order, true, cb, cp, C, cnt + 1, s, true);
     launch_pip_refine_dev(rf->build, rf->ring_index, rcol.col, o.predicate, cb, cp, cnt + 1, C, order, redo, cnt + 2,
                           ob, op, out_cap, cnt + 3, cnt + 4, ix->tuning.pip_waves, s);
   } else {
@@ -1576,6 +1584,16 @@ int SjhipRefinerSetSegmentLeaf(struct SedonaSpatialRefiner* refiner, int32_t seg
     SJ_CHECK(hipStreamSynchronize(rf->stream));
     rf->ring_index = std::move(ri);
     rf->ring_gen++;
+  });
+}
+
+int SjhipIndexSetFilterMode(struct SedonaFloatIndex2D* index, int32_t mode) {
+  return guarded(nullptr, [&] {
+    if (!index || !index->private_data) throw Error("null index");
+    if (mode < 0 || mode > 2) throw Error("filter mode must be 0, 1 or 2");
+    IndexImpl* ix = (IndexImpl*)index->private_data;
+    std::lock_guard<std::mutex> g(ix->mu);
+    ix->tree.filter_mode = mode;
   });
 }
 

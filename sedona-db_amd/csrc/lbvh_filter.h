@@ -328,6 +328,156 @@ __global__ __launch_bounds__(kFilterWaveBlock) void k_filter_wave(TreeView t, co
   }
 }
 
+// ---------------------------------------------------------------------------
+// Grid filter (RectTree's cell grid): one lane per query in the given
+// order; a query visits the cells its rect covers and tests their rects.
+// A pair found in several cells is kept in one: the cell holding the
+// corner (max of the two min x, max of the two min y) of the two rects'
+// overlap, which lies in both rects and so in a cell listing the build
+// rect.  (A point query covers one cell.)  Each lane counts its pairs,
+// the block scans the counts and takes one output range with one atomic,
+// then each lane writes its pairs contiguously, so they come out in query
+// order.  The same inclusive FP32 rect test as the tree walk decides a
+// hit.
+
+struct GridView {
+  const uint32_t* off;
+  const float4* rects;
+  const uint32_t* ids;
+  int nx, ny;
+  float x0, y0, inv_x, inv_y;
+};
+
+__device__ __forceinline__ int grid_cell(float v, float v0, float inv, int n) {
+  float f = (v - v0) * inv;
+  f = fminf(fmaxf(f, 0.f), (float)(n - 1));
+  return (int)f;
+}
+
+// Counts (WRITE = false) or writes the pairs of one query.  For a query
+// in one cell the counting pass records which of the cell's first 64
+// rects it hits in *mask, and the writing pass then reads only their ids.
+template <bool WRITE>
+__device__ __forceinline__ uint32_t grid_visit(const GridView& g, const float4& box, uint32_t pid, uint32_t* ob,
+                                               uint32_t* op, uint64_t base, uint64_t cap, uint64_t* mask) {
+  const int cx0 = grid_cell(box.x, g.x0, g.inv_x, g.nx), cx1 = grid_cell(box.z, g.x0, g.inv_x, g.nx);
+  const int cy0 = grid_cell(box.y, g.y0, g.inv_y, g.ny), cy1 = grid_cell(box.w, g.y0, g.inv_y, g.ny);
+  const bool one = cx0 == cx1 && cy0 == cy1;
+  uint32_t n = 0;
+  if (one) {
+    const int64_t cell = (int64_t)cy0 * g.nx + cx0;
+    const uint32_t k0 = g.off[cell], k1 = g.off[cell + 1];
+    if (k1 - k0 <= 64) {
+      if (!WRITE) {
+        uint64_t m = 0;
+        for (uint32_t k = k0; k < k1; k++)
+          if (hit(g.rects[k], box)) m |= 1ull << (k - k0);
+        *mask = m;
+        return (uint32_t)__popcll(m);
+      }
+      for (uint64_t m = *mask; m; m &= m - 1) {
+        const uint32_t k = k0 + (uint32_t)(__ffsll((unsigned long long)m) - 1);
+        const uint64_t d = base + n;
+        if (d < cap) {
+          ob[d] = g.ids[k];
+          op[d] = pid;
+        }
+        n++;
+      }
+      return n;
+    }
+  }
+  for (int cy = cy0; cy <= cy1; cy++)
+    for (int cx = cx0; cx <= cx1; cx++) {
+      const int64_t cell = (int64_t)cy * g.nx + cx;
+      const uint32_t k0 = g.off[cell], k1 = g.off[cell + 1];
+      for (uint32_t k = k0; k < k1; k++) {
+        const float4 r = g.rects[k];
+        if (!hit(r, box)) continue;
+        if (!one && (grid_cell(fmaxf(r.x, box.x), g.x0, g.inv_x, g.nx) != cx ||
+                     grid_cell(fmaxf(r.y, box.y), g.y0, g.inv_y, g.ny) != cy))
+          continue;
+        if (WRITE) {
+          const uint64_t d = base + n;
+          if (d < cap) {
+            ob[d] = g.ids[k];
+            op[d] = pid;
+          }
+        }
+        n++;
+      }
+    }
+  return n;
+}
+
+constexpr int kFilterGridBlock = 512;
+
+// Queries are FP32 rects q[qorder[i]] (pid: i or the row), or, with
+// xy, FP64 points already in visiting order whose rects are the points
+// rounded to nearest FP32 (the operator's point rects; pid = i).
+static __global__ __launch_bounds__(kFilterGridBlock) void k_filter_grid(GridView g, const float4* q,
+                                                                         const double2* xy, int64_t nq,
+                                                                         const uint32_t* qorder, int emit_rank,
+                                                                         uint32_t* ob, uint32_t* op, uint64_t cap,
+                                                                         unsigned long long* counter) {
+  constexpr int kWaves = kFilterGridBlock / 64;
+  __shared__ uint32_t s_wave[kWaves];
+  __shared__ unsigned long long s_base;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + tid;
+  bool active = i < nq;
+  uint32_t pid = 0;
+  float4 box = make_float4(0, 0, 0, 0);
+  if (active) {
+    if (xy) {
+      const double2 p = xy[i];
+      const float fx = (float)p.x, fy = (float)p.y;
+      box = make_float4(fx, fy, fx, fy);
+      pid = (uint32_t)i;
+    } else {
+      const uint32_t qid = qorder ? qorder[i] : (uint32_t)i;
+      pid = emit_rank ? (uint32_t)i : qid;
+      box = q[qid];
+    }
+    active = rect_ok(box);
+  }
+  uint64_t mask = 0;
+  const uint32_t n = active ? grid_visit<false>(g, box, pid, ob, op, 0, 0, &mask) : 0u;
+  // exclusive prefix of n within the wavefront, then across wavefronts
+  uint32_t incl = n;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t v = __shfl_up(incl, o);
+    if (lane >= o) incl += v;
+  }
+  if (lane == 63) s_wave[wave] = incl;
+  __syncthreads();
+  if (tid == 0) {
+    unsigned long long tot = 0;
+    for (int w = 0; w < kWaves; w++) {
+      const uint32_t c = s_wave[w];
+      s_wave[w] = (uint32_t)tot;
+      tot += c;
+    }
+    s_base = tot ? atomicAdd(counter, tot) : 0ull;
+  }
+  __syncthreads();
+  if (active && n) grid_visit<true>(g, box, pid, ob, op, s_base + s_wave[wave] + (incl - n), cap, &mask);
+}
+
+inline GridView grid_view(const RectTree& t) {
+  GridView g{};
+  g.off = t.grid_off.p;
+  g.rects = t.grid_rects.p;
+  g.ids = t.grid_ids.p;
+  g.nx = t.grid_nx;
+  g.ny = t.grid_ny;
+  g.x0 = t.grid_x0;
+  g.y0 = t.grid_y0;
+  g.inv_x = t.grid_inv_x;
+  g.inv_y = t.grid_inv_y;
+  return g;
+}
+
 inline TreeView tree_view(const RectTree& t) {
   TreeView v{};
   v.nodes = t.nodes.p;

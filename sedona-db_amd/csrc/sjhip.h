@@ -294,6 +294,21 @@ struct RectTree {
   // the next probe's output
   mutable std::atomic<uint64_t> cand_per_query_x16{0};
   mutable std::atomic<uint64_t> kept_per_query_x16{0};
+  // Uniform cell grid over the valid rects, built with the tree for the
+  // filter of point probes.  The cell of a coordinate is
+  // clamp(floor((v - grid_x0) * grid_inv_x), 0, grid_nx - 1) in FP32 (and
+  // likewise in y), the same monotonic map for rects and probes, so a rect
+  // is listed in every cell that can hold a point of it.  Cell c's rects
+  // are grid_rects / grid_ids[grid_off[c] .. grid_off[c + 1]).  grid_nx = 0:
+  // no grid.
+  int grid_nx = 0, grid_ny = 0;
+  float grid_x0 = 0.f, grid_y0 = 0.f, grid_inv_x = 0.f, grid_inv_y = 0.f;
+  // 0: the grid filter for point queries, the tree walk otherwise; 1: the
+  // tree walk always; 2: the grid filter whenever there is a grid
+  int filter_mode = 0;
+  DevArray<uint32_t> grid_off;
+  DevArray<float4> grid_rects;
+  DevArray<uint32_t> grid_ids;
   RectTree() = default;
   RectTree(RectTree&& o) noexcept { *this = std::move(o); }
   // For a tree that outlives the stream it was built on (after that
@@ -301,6 +316,7 @@ struct RectTree {
   // detached stream instead of the destroyed one.
   void detach_streams() {
     nodes.s = leaf_rects.s = leaf_ids.s = leaf_keys.s = detached_stream();
+    grid_off.s = grid_rects.s = grid_ids.s = detached_stream();
   }
   RectTree& operator=(RectTree&& o) noexcept {
     leaf = o.leaf;
@@ -314,6 +330,16 @@ struct RectTree {
     leaf_keys = std::move(o.leaf_keys);
     for (int k = 0; k < 4; k++) bb[k] = o.bb[k];
     coop_extent = o.coop_extent;
+    filter_mode = o.filter_mode;
+    grid_nx = o.grid_nx;
+    grid_ny = o.grid_ny;
+    grid_x0 = o.grid_x0;
+    grid_y0 = o.grid_y0;
+    grid_inv_x = o.grid_inv_x;
+    grid_inv_y = o.grid_inv_y;
+    grid_off = std::move(o.grid_off);
+    grid_rects = std::move(o.grid_rects);
+    grid_ids = std::move(o.grid_ids);
     cand_per_query_x16.store(o.cand_per_query_x16.load());
     kept_per_query_x16.store(o.kept_per_query_x16.load());
     return *this;
@@ -367,8 +393,13 @@ struct Workspace {
 // qorder (optional) gives the visiting order of the queries; with
 // emit_rank the pairs carry the query's position i in qorder instead of
 // its row qorder[i].
+// point_probes: every query rect is a point (the grid filter applies);
+// ordered_points (with emit_rank): the query points already gathered in
+// qorder's order, which the grid filter then reads in sequence instead of
+// the rects through qorder.
 uint64_t filter_pairs(const RectTree& t, const float4* q, int64_t nq, const uint32_t* qorder, PairBuf& pb,
-                      hipStream_t s, KernelTimer* timer = nullptr, bool emit_rank = false);
+                      hipStream_t s, KernelTimer* timer = nullptr, bool emit_rank = false, bool point_probes = false,
+                      const double2* ordered_points = nullptr);
 
 // Gathers the points of a point column in the given row order
 // (out.xy[i] = probe.xy[order[i]]), producing a point column of n rows
@@ -514,7 +545,8 @@ void morton_order_fixed(const RectTree& t, const float4* rects, int64_t n, uint3
 // One filter launch into fixed buffers (pairs beyond cap are counted, not
 // written).
 void launch_filter(const RectTree& t, const float4* q, int64_t nq, const uint32_t* qorder, bool emit_rank,
-                   uint32_t* ob, uint32_t* op, uint64_t cap, unsigned long long* counter, hipStream_t s);
+                   uint32_t* ob, uint32_t* op, uint64_t cap, unsigned long long* counter, hipStream_t s,
+                   bool point_probes = false);
 
 // Fills n 64-bit words with v (a kernel: captured sequences record no
 // memset nodes).

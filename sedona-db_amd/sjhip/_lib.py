@@ -200,7 +200,7 @@ EXPORTED = [
     "SjhipHostRaycast", "SjhipHostRaycastF32", "SjhipHostCollinear", "SjhipHostSegmentIntersects",
     "SjhipHostOrient2d", "SjhipHostHypot", "SjhipHostCoordToLine", "SjhipHostRaycastBatch", "SjhipHostHypotBatch",
     "SjhipTestingGraphSortPairs", "SjhipJoinDeviceNoAlloc", "SjhipRefinerKnnTies", "SjhipHostSegmentWithinF32",
-    "SjhipRefinerPairDistances",
+    "SjhipRefinerPairDistances", "SjhipIndexSetFilterMode",
 ]
 
 _lib = None
@@ -245,6 +245,7 @@ def lib():
         L.SjhipRefinerSetSegmentLeaf.argtypes = [_RF, ctypes.c_int32]
         L.SjhipIndexSetTuning.argtypes = [_IX, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
         L.SjhipRefinerSetPrecision.argtypes = [_RF, ctypes.c_int32]
+        L.SjhipIndexSetFilterMode.argtypes = [_IX, ctypes.c_int32]
         L.SjhipRefinerKnnDevice.argtypes = [_RF, _SCH, ctypes.POINTER(ArrowDeviceArray), ctypes.c_uint32,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.SjhipIndexBuildFromRefiner.argtypes = [_IX, _RF]

@@ -346,6 +346,12 @@ class GpuSpatialIndex:
         if rc:
             raise GpuSpatialError(self.lib.SjhipLastError().decode())
 
+    def set_filter_mode(self, mode):
+        """0: grid for point probes, tree otherwise; 1: tree; 2: grid."""
+        rc = self.lib.SjhipIndexSetFilterMode(ctypes.byref(self.ix), int(mode))
+        if rc:
+            raise GpuSpatialError(self.lib.SjhipLastError().decode())
+
     def release(self):
         if self.ix.private_data:
             self.ix.release(ctypes.byref(self.ix))

@@ -152,12 +152,15 @@ def brute_force_candidates(br, qr):
     return np.concatenate(bs), np.concatenate(ps)
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
 @pytest.mark.parametrize("layout", ["dense", "sparse", "mixed", "boxes"])
-def test_filter_both_walk_modes_vs_brute_force(layout, opts):
-    """The filter walks a wavefront of Morton-ordered queries once with the
-    union of their rects when they lie close together (dense), and one lane
-    per query when they lie far apart (sparse); both must give exactly the
-    rect hits, edges included."""
+def test_filter_both_walk_modes_vs_brute_force(layout, mode, opts):
+    """The tree filter walks a wavefront of Morton-ordered queries once with
+    the union of their rects when they lie close together (dense), and one
+    lane per query when they lie far apart (sparse); the grid filter visits
+    the cells of each query (mode 0: grid for point queries, 1: tree only,
+    2: grid for every query, boxes spanning cells included); every form must
+    give exactly the rect hits, edges included, each pair once."""
     rng = np.random.default_rng(17)
     nb = 3000
     c = rng.uniform(0, 100, size=(nb, 2)).astype(np.float32)
@@ -182,9 +185,10 @@ def test_filter_both_walk_modes_vs_brute_force(layout, opts):
     ix = G.GpuSpatialIndex(opts)
     ix.push_build(br)
     ix.finish_building()
+    ix.set_filter_mode(mode)
     gb, gp = ix.probe(qr)
     ob, op = brute_force_candidates(br, qr)
-    say(layout, len(qr), len(ob))
+    say(layout, mode, len(qr), len(ob))
     eq, miss, extra = pairs_equal((gb, gp), (ob, op))
     assert eq, (layout, miss, extra)
 
