@@ -1,0 +1,273 @@
+// pip_walk.hip -- the certified FP32 walk of point-in-polygon candidates
+// whose build row is one polygon with one ring (a PolyDesc record).
+//
+// The classification is tg_ring_contains_point (c/sedona-tg/src/tg/
+// tg.c:2809-2822) as restated in ring_walk.h, in the deferred mode: a
+// candidate with any uncertain FP32 test is not decided here but listed
+// for the FP64 pass along +x, the reference's own ray.
+//
+// One lane per candidate.  Any ray from a point off the boundary of a
+// closed ring crosses it an odd number of times exactly when the point is
+// inside (mirroring and transposing the coordinates are exact, so the
+// certified test is the same test on other axes): each lane casts its ray
+// toward the nearest side of its ring's rect, where fewer segments and
+// tree nodes lie on it.  (A direction shared by the wavefront, or one walk
+// per direction present, gave longer rays or serialised walks and was
+// slower on the 100M x 1M star join.)
+//
+// The kernel is kept lean for occupancy: the walk state is a few
+// registers, vertex loads are batched (the walk waits on memory more than
+// on arithmetic), and the cell-raster read is compiled out when a raster
+// pass has already decided the candidates in clean cells.
+//
+// Rows that are not single rings (multipolygons, polygons with holes) are
+// listed in `rest` for the general kernel (or in `redo` when no rest list
+// is given).
+#include "ring_walk.h"
+#include "wave_out.h"
+
+namespace sjhip {
+
+namespace {
+
+constexpr int kWalkBlock = 256;
+constexpr int kWalkBoxBatch = 4;  // box loads in flight (more raise the register count)
+#ifndef SJ_SEG_BATCH
+#define SJ_SEG_BATCH 6
+#endif
+constexpr int kSegBatch = SJ_SEG_BATCH;  // vertex loads in flight
+
+// A lane's walk: ray direction dir (bit 1 set: along the y axis, bit 0
+// set: toward minus), the point's coordinates along and across the ray
+// (ring-local FP32, the along one mirrored so that the ray runs toward
+// + along), and the result st (bit 0: crossing parity, bits 1..: the
+// number of uncertain tests).
+struct LaneWalk {
+  const float4* box;  // the ring's tree, level 0 first (nullptr: none)
+  const float2* q32;  // the ring's FP32 ring-local vertices
+  float pa, pc;
+  float E;            // certification bound (boxes are widened by 2E)
+  int ns;
+  int dir;
+  uint32_t st;
+};
+
+__device__ __forceinline__ float along(int dir, float x, float y) {
+  const float a = (dir & 2) ? y : x;
+  return (dir & 1) ? -a : a;
+}
+__device__ __forceinline__ float across(int dir, float x, float y) { return (dir & 2) ? x : y; }
+
+// Whether a segment in box b can meet the ray: the point's across
+// coordinate within the box's across range and the box's along range not
+// entirely behind the point (both widened by the margin m).
+__device__ __forceinline__ bool box_hit(const LaneWalk& w, float m, const float4& b) {
+  const bool sw = (w.dir & 2) != 0;
+  const float c0 = sw ? b.x : b.y, c1 = sw ? b.z : b.w;
+  const float a1 = (w.dir & 1) ? -(sw ? b.y : b.x) : (sw ? b.w : b.z);
+  return w.pc >= c0 - m && w.pc <= c1 + m && w.pa <= a1 + m;
+}
+
+__device__ __forceinline__ void segment_test(LaneWalk& w, uint32_t& tests, float2 a, float2 b) {
+  const int rc = raycast_f32_certified(along(w.dir, a.x, a.y), across(w.dir, a.x, a.y), along(w.dir, b.x, b.y),
+                                       across(w.dir, b.x, b.y), w.pa, w.pc, w.E);
+  if (rc == kRayOut) return;
+  tests++;
+  w.st ^= rc < 0 ? 0u : 1u;  // certified: kRayIn
+  w.st += rc < 0 ? 2u : 0u;
+}
+
+// Segments [s0, s1), kSegBatch vertex loads issued before the first test.
+// The batch loop itself is not unrolled, which keeps the code small.
+__device__ __forceinline__ void seg_scan(LaneWalk& w, uint32_t& tests, const int s0, const int s1) {
+  float2 a = w.q32[s0];
+  for (int c0 = s0; c0 < s1; c0 += kSegBatch) {
+    float2 v[kSegBatch];
+#pragma unroll
+    for (int k = 0; k < kSegBatch; k++) v[k] = c0 + k + 1 <= s1 ? w.q32[c0 + k + 1] : a;
+    segment_test(w, tests, a, v[0]);
+#pragma unroll
+    for (int k = 1; k < kSegBatch; k++)
+      if (c0 + k < s1) segment_test(w, tests, v[k - 1], v[k]);
+    a = v[kSegBatch - 1];
+  }
+}
+
+__device__ __forceinline__ uint32_t hit_mask(const LaneWalk& w, const float4* b, int n) {
+  const float m = 2.0f * w.E;
+  uint32_t mask = 0;
+#pragma unroll
+  for (int c0 = 0; c0 < kSegFanout; c0 += kWalkBoxBatch) {
+    float4 bx[kWalkBoxBatch];
+#pragma unroll
+    for (int k = 0; k < kWalkBoxBatch; k++) bx[k] = c0 + k < n ? b[c0 + k] : make_float4(1.f, 1.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < kWalkBoxBatch; k++)
+      if (c0 + k < n && box_hit(w, m, bx[k])) mask |= 1u << (c0 + k);
+  }
+  return mask;
+}
+
+// Depth-first walk over the ring's segment tree (leaves of 1 << ls
+// segments), the layout walk_tree uses: byte l of `masks` holds the
+// children of level l still to visit.
+__device__ __forceinline__ void walk_ring(LaneWalk& w, int ls, uint32_t& tests) {
+  if (!w.box) {
+    seg_scan(w, tests, 0, w.ns);
+    return;
+  }
+  const int top = seg_tree_shape(w.ns, ls).levels - 1;
+  int otop, ntop;
+  level_geom(w.ns, ls, top, otop, ntop);
+  const float4* btop = w.box + otop;
+  for (int c0 = 0; c0 < ntop; c0 += kSegFanout) {
+    uint64_t masks = (uint64_t)hit_mask(w, btop + c0, min(kSegFanout, ntop - c0)) << (8 * top);
+    int lvl = top, base = c0;
+    while (masks) {
+      const uint32_t m = (uint32_t)(masks >> (8 * lvl)) & 0xffu;
+      if (m == 0) {
+        lvl++;
+        base = (base >> 3) & ~(kSegFanout - 1);
+        continue;
+      }
+      const int k = __ffs(m) - 1;
+      masks &= ~(1ull << (8 * lvl + k));
+      const int node = base + k;
+      if (lvl == 0) {
+        const int s0 = node << ls;
+        seg_scan(w, tests, s0, min(s0 + (1 << ls), w.ns));
+        continue;
+      }
+      const int cb = node * kSegFanout;
+      int loff, lcnt;
+      level_geom(w.ns, ls, lvl - 1, loff, lcnt);
+      const uint32_t cm = hit_mask(w, w.box + loff + cb, min(kSegFanout, lcnt - cb));
+      if (cm) {
+        lvl--;
+        base = cb;
+        masks |= (uint64_t)cm << (8 * lvl);
+      }
+    }
+  }
+}
+
+// Appends v to list (count *cnt) for every lane with f set: one atomic
+// per wavefront.  Called by all lanes of the wavefront.
+__device__ __forceinline__ void wave_push(bool f, uint32_t v, uint32_t* list, unsigned long long* cnt) {
+  const uint64_t m = __ballot(f);
+  if (!m) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(cnt, (unsigned long long)__popcll(m));
+  base = __shfl(base, leader);
+  if (f) list[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
+}
+
+// RASTER: whether to read the candidates' cell rasters (false when a raster
+// pass has already decided every candidate in a clean cell).
+template <int WAVES, bool RASTER>
+__global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_walk_rings(
+    RingIndexView R, const double2* __restrict__ pts, const uint32_t* __restrict__ cb,
+    const uint32_t* __restrict__ cp, uint64_t n, const unsigned long long* n_dev, const uint32_t* probe_ids,
+    uint32_t* ob, uint32_t* op, uint64_t cap, unsigned long long* counter,
+    unsigned long long* stats, uint32_t* redo, unsigned long long* n_redo, uint32_t* rest,
+    unsigned long long* n_rest) {
+  __shared__ uint32_t wo_b[kWalkBlock / 64][kWaveOutCap], wo_p[kWalkBlock / 64][kWaveOutCap];
+  WaveOut wo{wo_b[threadIdx.x >> 6], wo_p[threadIdx.x >> 6], 0};
+  if (n_dev) n = min(n, (uint64_t)*n_dev);  // candidate count produced on the device (n: its bound)
+  const int ls = R.seg_leaf_shift;
+  const uint32_t n32 = (uint32_t)n;  // (the launcher admits at most 2^32 - 1 candidates)
+  const uint32_t stride = gridDim.x * kWalkBlock;
+  uint32_t tests = 0, unsure = 0;
+  for (uint32_t base = blockIdx.x * kWalkBlock; base < n32; base += stride) {
+    const uint32_t i = base + threadIdx.x;
+    bool keep = false, walk = false, general = false;
+    LaneWalk w;
+    if (i < n32) {
+      const uint32_t b = cb[i];
+      // the probe is a point column: null and empty rows hold NaN
+      const double2 pt = pts[cp[i]];
+      if (pt.x == pt.x || pt.y == pt.y) {
+        const PolyDesc d = R.desc[b];
+        if (d.npts < 0) {
+          general = true;
+        } else {
+          const int cls = RASTER ? raster_class(R, d, pt) : (rect_covers(d.rect, pt.x, pt.y) ? 0 : 2);
+          keep = cls == 1;
+          if (cls == 0 && d.npts >= 3) {
+            walk = true;
+            const double2 o = ring_origin(d.rect);
+            const float xl = (float)(pt.x - o.x), yl = (float)(pt.y - o.y);
+            const double dl = pt.x - d.rect.x, dr = d.rect.z - pt.x, db = pt.y - d.rect.y, dt = d.rect.w - pt.y;
+            w.dir = fmin(db, dt) < fmin(dl, dr) ? (db < dt ? 3 : 2) : (dl < dr ? 1 : 0);
+            w.box = d.tree_off >= 0 ? R.seg_tree + d.tree_off : nullptr;
+            w.q32 = R.xy32 + d.v0;
+            w.E = d.err;
+            w.ns = d.npts - 1;
+            w.st = 0;
+            w.pa = along(w.dir, xl, yl);
+            w.pc = across(w.dir, xl, yl);
+          }
+        }
+      }
+    }
+    if (walk) {
+      walk_ring(w, ls, tests);
+      keep = w.st == 1u;  // odd crossings, no uncertain test
+      unsure += w.st >> 1;
+    }
+    const bool deferred = walk && w.st > 1u;
+    wave_push(deferred || (general && !rest), i, redo, n_redo);
+    if (rest) wave_push(general, i, rest, n_rest);
+    // (the ids are read again here rather than held through the walk)
+    uint32_t b = 0, p = 0;
+    if (keep) {
+      b = cb[i];
+      p = cp[i];
+      if (probe_ids) p = probe_ids[p];
+    }
+    wave_append(wo, keep, b, p, ob, op, cap, counter);
+  }
+  wave_flush(wo, ob, op, cap, counter);
+  for (int o = 32; o > 0; o >>= 1) {
+    tests += __shfl_xor(tests, o);
+    unsure += __shfl_xor(unsure, o);
+  }
+  if ((threadIdx.x & 63) == 0 && stats && (tests | unsure)) {
+    atomicAdd(&stats[0], (unsigned long long)tests);
+    atomicAdd(&stats[1], (unsigned long long)unsure);
+  }
+}
+
+}  // namespace
+
+void launch_ring_walk(const RingIndex& ri, const GeomColumn& probe, const uint32_t* cb, const uint32_t* cp,
+                      uint64_t n, const unsigned long long* n_dev, const uint32_t* probe_ids, bool raster_checked,
+                      uint32_t* ob, uint32_t* op, uint64_t cap, unsigned long long* counter,
+                      unsigned long long* stats, uint32_t* redo, unsigned long long* n_redo, uint32_t* rest,
+                      unsigned long long* n_rest, int waves, hipStream_t s) {
+  if (n == 0) return;
+  if (!probe.points_only) throw Error("launch_ring_walk: the probe column must hold points only");
+  if (ri.desc.p == nullptr) throw Error("launch_ring_walk: the ring index has no row records");
+  if (n > 0xffffffffull) throw Error("launch_ring_walk: more than 2^32 candidates");
+  const uint64_t blocks = (n + kWalkBlock - 1) / kWalkBlock;
+  const unsigned g = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, 1u << 16));
+  const RingIndexView R = ri.view();
+  auto go = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, g, kWalkBlock, 0, s, R, probe.xy.p, cb, cp, n, n_dev, probe_ids, ob, op, cap, counter,
+                       stats, redo, n_redo, rest, n_rest);
+  };
+  if (raster_checked) {
+    if (waves == 4) go(k_walk_rings<4, false>);
+    else if (waves == 6) go(k_walk_rings<6, false>);
+    else go(k_walk_rings<8, false>);
+  } else {
+    if (waves == 4) go(k_walk_rings<4, true>);
+    else if (waves == 6) go(k_walk_rings<6, true>);
+    else go(k_walk_rings<8, true>);
+  }
+  SJ_CHECK(hipGetLastError());
+}
+
+}  // namespace sjhip

@@ -382,10 +382,11 @@ struct Workspace {
   PairBuf cand, out;
   PairBuf open;  // the candidates a raster pass leaves to the walk
   DevArray<uint32_t> redo;
+  DevArray<uint32_t> rest;  // candidates of rows the single-ring walk leaves to the general kernel
   DevArray<uint64_t> mask;
   void set_stream(hipStream_t s) {
     cand.b.s = cand.p.s = cand.counter.s = out.b.s = out.p.s = out.counter.s = s;
-    open.b.s = open.p.s = open.counter.s = redo.s = mask.s = s;
+    open.b.s = open.p.s = open.counter.s = redo.s = rest.s = mask.s = s;
   }
 };
 
@@ -551,6 +552,20 @@ void launch_filter(const RectTree& t, const float4* q, int64_t nq, const uint32_
 // Fills n 64-bit words with v (a kernel: captured sequences record no
 // memset nodes).
 void fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
+
+// Certified FP32 point-in-polygon walk of candidates [0, n) (count
+// min(n, *n_dev) when n_dev is given) over single-ring build rows, one ray
+// direction per wavefront.  Kept pairs go to (ob, op) (probe ids mapped
+// through probe_ids when given); candidates with an uncertain test are
+// listed in redo (count *n_redo) for the FP64 pass; candidates of other
+// rows in rest (count *n_rest), or in redo when rest is nullptr.
+// raster_checked: a raster pass has already decided the candidates in
+// clean cells.  stats[0..1]: tests and uncertain tests.
+void launch_ring_walk(const RingIndex& ri, const GeomColumn& probe, const uint32_t* cb, const uint32_t* cp,
+                      uint64_t n, const unsigned long long* n_dev, const uint32_t* probe_ids, bool raster_checked,
+                      uint32_t* ob, uint32_t* op, uint64_t cap, unsigned long long* counter,
+                      unsigned long long* stats, uint32_t* redo, unsigned long long* n_redo, uint32_t* rest,
+                      unsigned long long* n_rest, int waves, hipStream_t s);
 
 // Refine launches over min(*n_dev, n_cap) candidates.  Point in polygon:
 // certified FP32 pass (uncertain candidates to redo/*redo_n) and the FP64
