@@ -31,6 +31,9 @@ namespace sjhip {
 namespace {
 
 constexpr int kWalkBlock = 256;
+// candidates a wavefront takes from the queue at a time (64 per step): one
+// counter shared by the grid takes one atomic per chunk
+constexpr uint32_t kQueueChunk = 1024;
 constexpr int kWalkBoxBatch = 4;  // box loads in flight (more raise the register count)
 #ifndef SJ_SEG_BATCH
 #define SJ_SEG_BATCH 6
@@ -172,16 +175,31 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(WAVE
     const uint32_t* __restrict__ cp, uint64_t n, const unsigned long long* n_dev, const uint32_t* probe_ids,
     uint32_t* ob, uint32_t* op, uint64_t cap, unsigned long long* counter,
     unsigned long long* stats, uint32_t* redo, unsigned long long* n_redo, uint32_t* rest,
-    unsigned long long* n_rest) {
+    unsigned long long* n_rest, unsigned long long* queue) {
   __shared__ uint32_t wo_b[kWalkBlock / 64][kWaveOutCap], wo_p[kWalkBlock / 64][kWaveOutCap];
   WaveOut wo{wo_b[threadIdx.x >> 6], wo_p[threadIdx.x >> 6], 0};
   if (n_dev) n = min(n, (uint64_t)*n_dev);  // candidate count produced on the device (n: its bound)
   const int ls = R.seg_leaf_shift;
   const uint32_t n32 = (uint32_t)n;  // (the launcher admits at most 2^32 - 1 candidates)
-  const uint32_t stride = gridDim.x * kWalkBlock;
+  const int lane = threadIdx.x & 63;
+  // Work distribution: with a queue, each wavefront takes the next 64
+  // candidates from the shared counter whenever it finishes its last ones
+  // (walks differ in length by orders of magnitude); without, a grid-stride
+  // loop over the candidates.
+  uint32_t chunk_end = 0;  // end of the wavefront's current chunk (queue mode)
+  auto next = [&](uint32_t cur) -> uint32_t {
+    if (!queue) return cur + gridDim.x * kWalkBlock;
+    if (cur + 64 < chunk_end) return cur + 64;
+    unsigned long long v = 0;
+    if (lane == 0) v = atomicAdd(queue, (unsigned long long)kQueueChunk);
+    v = __shfl(v, 0);
+    chunk_end = v + kQueueChunk < n32 ? (uint32_t)(v + kQueueChunk) : n32;
+    return v < n32 ? (uint32_t)v : n32;
+  };
   uint32_t tests = 0, unsure = 0;
-  for (uint32_t base = blockIdx.x * kWalkBlock; base < n32; base += stride) {
-    const uint32_t i = base + threadIdx.x;
+  for (uint32_t wbase = queue ? next(0) : blockIdx.x * kWalkBlock + (threadIdx.x & ~63u); wbase < n32;
+       wbase = next(wbase)) {
+    const uint32_t i = wbase + lane;
     bool keep = false, walk = false, general = false;
     LaneWalk w;
     if (i < n32) {
@@ -246,17 +264,25 @@ void launch_ring_walk(const RingIndex& ri, const GeomColumn& probe, const uint32
                       uint64_t n, const unsigned long long* n_dev, const uint32_t* probe_ids, bool raster_checked,
                       uint32_t* ob, uint32_t* op, uint64_t cap, unsigned long long* counter,
                       unsigned long long* stats, uint32_t* redo, unsigned long long* n_redo, uint32_t* rest,
-                      unsigned long long* n_rest, int waves, hipStream_t s) {
+                      unsigned long long* n_rest, int waves, hipStream_t s, unsigned long long* queue) {
   if (n == 0) return;
   if (!probe.points_only) throw Error("launch_ring_walk: the probe column must hold points only");
   if (ri.desc.p == nullptr) throw Error("launch_ring_walk: the ring index has no row records");
-  if (n > 0xffffffffull) throw Error("launch_ring_walk: more than 2^32 candidates");
+  if (n >= 0xffffffffull - 64) throw Error("launch_ring_walk: 2^32 - 64 candidates or more");
   const uint64_t blocks = (n + kWalkBlock - 1) / kWalkBlock;
-  const unsigned g = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, 1u << 16));
+  unsigned g = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, 1u << 16));
+  if (queue) {
+    // a queue launch needs only the blocks that can be resident at once
+    int dev = 0, cus = 0;
+    SJ_CHECK(hipGetDevice(&dev));
+    SJ_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int per_cu = 4 * (waves == 4 ? 4 : waves == 6 ? 6 : 8) / (kWalkBlock / 64);
+    g = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, (uint64_t)cus * per_cu));
+  }
   const RingIndexView R = ri.view();
   auto go = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, g, kWalkBlock, 0, s, R, probe.xy.p, cb, cp, n, n_dev, probe_ids, ob, op, cap, counter,
-                       stats, redo, n_redo, rest, n_rest);
+                       stats, redo, n_redo, rest, n_rest, queue);
   };
   if (raster_checked) {
     if (waves == 4) go(k_walk_rings<4, false>);

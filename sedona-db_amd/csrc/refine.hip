@@ -2198,14 +2198,14 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
   c.B = build.view();
   c.R = ri.view();
   c.P = probe.view();
-  DevArray<unsigned long long> aux(5, s);  // counter of long list / deferred, stats[2], bad flag, rest count
+  DevArray<unsigned long long> aux(6, s);  // counter of long list / deferred, stats[2], bad flag, rest count, queue
   DevArray<uint32_t> long_list;
   bool pip = pip_refine_applies(build, ri, probe, predicate);
   if (probe_ids && !pip) throw Error("probe id mapping needs the point-in-polygon refine");
   if (raster_done && !(pip && precision == 0)) throw Error("raster split without the certified point-in-polygon refine");
   // after raster_split, out already holds the decided pairs
   if (!raster_done) SJ_CHECK(hipMemsetAsync(out.counter.p, 0, 8, s));
-  SJ_CHECK(hipMemsetAsync(aux.p, 0, 40, s));
+  SJ_CHECK(hipMemsetAsync(aux.p, 0, 48, s));
   if (pip) {
     int allow = predicate == kContains ? 0 : 1;
     unsigned long long walked = n;  // candidates the walk decides (host copy)
@@ -2240,7 +2240,8 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
       // the other rows (listed in `rest`, count aux[4]): the general kernel.
       // Both list the candidates they defer in redo (count aux[0]).
       launch_ring_walk(ri, probe, wb, wp, n, n_open, probe_ids, raster_done || n_open != nullptr, out.b.p, out.p.p,
-                       out.cap(), out.counter.p, aux.p + 1, redo.p, aux.p, rest.p, aux.p + 4, pip_waves, s);
+                       out.cap(), out.counter.p, aux.p + 1, redo.p, aux.p, rest.p, aux.p + 4, pip_waves, s,
+                       aux.p + 5);
       // (a grid-stride launch of 2048 blocks: the list is often empty)
       auto fp32_pass = [&](auto kernel) {
         hipLaunchKernelGGL(kernel, std::min(grid_for(n), 2048u), kBlock, 0, s, c, allow, wb, wp, n, rest.p, out.b.p, out.p.p,

@@ -560,12 +560,14 @@ void fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_
 // listed in redo (count *n_redo) for the FP64 pass; candidates of other
 // rows in rest (count *n_rest), or in redo when rest is nullptr.
 // raster_checked: a raster pass has already decided the candidates in
-// clean cells.  stats[0..1]: tests and uncertain tests.
+// clean cells.  stats[0..1]: tests and uncertain tests.  queue: a
+// counter that is zero at launch, for wavefronts that take 64 candidates
+// at a time (nullptr: a grid-stride loop).
 void launch_ring_walk(const RingIndex& ri, const GeomColumn& probe, const uint32_t* cb, const uint32_t* cp,
                       uint64_t n, const unsigned long long* n_dev, const uint32_t* probe_ids, bool raster_checked,
                       uint32_t* ob, uint32_t* op, uint64_t cap, unsigned long long* counter,
                       unsigned long long* stats, uint32_t* redo, unsigned long long* n_redo, uint32_t* rest,
-                      unsigned long long* n_rest, int waves, hipStream_t s);
+                      unsigned long long* n_rest, int waves, hipStream_t s, unsigned long long* queue = nullptr);
 
 // Refine launches over min(*n_dev, n_cap) candidates.  Point in polygon:
 // certified FP32 pass (uncertain candidates to redo/*redo_n) and the FP64
