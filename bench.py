@@ -81,6 +81,8 @@ def main():
                     help="pip_zipf_contains: skip the host-inclusive (PCIe) measurement")
     ap.add_argument("--abi-batch", type=int, default=1_000_000,
                     help="pip_zipf_contains: probe rows per batch in the reference-ABI host flow")
+    ap.add_argument("--abi-threads", type=int, default=4,
+                    help="pip_zipf_contains: threads of the concurrent reference-ABI host flow")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only time the plain copy kernel (HBM calibration for the counter summaries)")
     args = ap.parse_args()
@@ -461,24 +463,51 @@ def host_inclusive_measurement(G, ix, rf, probe, pairs, args):
     del gb, gp
     log(f"host-inclusive fused join: {fused_ms:.1f} ms, {fused['pairs']} pairs")
 
+    import threading
+
     bsz = args.abi_batch
-    kept = 0
-    cands = 0
-    abi_sum = 0
-    t = time.time()
-    for a in range(0, n, bsz):
-        part = parr.slice(a, min(bsz, n - a))
-        rects = G.compute_rects(part)
-        cb, cp = ix.probe(rects)
-        cands += len(cb)
-        kb, kp = rf.refine(part, "contains", cb, cp)
-        kept += len(kb)
-        abi_sum = (abi_sum + pair_checksum(kb, kp, a)) % (1 << 64)
-    abi_ms = (time.time() - t) * 1000
-    abi = dict(batch_rows=bsz, batches=(n + bsz - 1) // bsz, ms=abi_ms, pairs=int(kept), candidates=int(cands),
-               pairs_per_s=kept / (abi_ms / 1000))
-    log(f"host-inclusive reference-ABI flow: {abi_ms:.1f} ms, {kept} pairs in {abi['batches']} batches")
-    return dict(fused=fused, reference_abi=abi, same_pairs=bool(fused["pairs"] == pairs == kept and fused_sum == abi_sum))
+    starts = list(range(0, n, bsz))
+
+    def abi_flow(threads):
+        """Batches dealt round-robin to `threads` threads, each with its own
+        index context per probe call (the operator's partitions run
+        concurrently; the ABI is thread-safe with one context per thread)."""
+        acc = [[0, 0, 0] for _ in range(threads)]  # kept, candidates, checksum
+        errors = []
+
+        def work(k):
+            try:
+                for a in starts[k::threads]:
+                    part = parr.slice(a, min(bsz, n - a))
+                    rects = G.compute_rects(part)
+                    cb, cp = ix.probe(rects)
+                    acc[k][1] += len(cb)
+                    kb, kp = rf.refine(part, "contains", cb, cp)
+                    acc[k][0] += len(kb)
+                    acc[k][2] = (acc[k][2] + pair_checksum(kb, kp, a)) % (1 << 64)
+            except BaseException as e:
+                errors.append(e)
+
+        th = [threading.Thread(target=work, args=(k,)) for k in range(threads)]
+        t = time.time()
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        ms = (time.time() - t) * 1000
+        if errors:
+            raise errors[0]
+        kept = sum(a[0] for a in acc)
+        res = dict(threads=threads, batch_rows=bsz, batches=len(starts), ms=ms, pairs=int(kept),
+                   candidates=int(sum(a[1] for a in acc)), pairs_per_s=kept / (ms / 1000))
+        log(f"host-inclusive reference-ABI flow, {threads} thread(s): {ms:.1f} ms, {kept} pairs in "
+            f"{len(starts)} batches")
+        return res, kept, sum(a[2] for a in acc) % (1 << 64)
+
+    abi, kept1, sum1 = abi_flow(1)
+    abi_mt, kept_t, sum_t = abi_flow(args.abi_threads)
+    same = fused["pairs"] == pairs == kept1 == kept_t and fused_sum == sum1 == sum_t
+    return dict(fused=fused, reference_abi=abi, reference_abi_threads=abi_mt, same_pairs=bool(same))
 
 
 SERVE_MIX = (("intersects", 0.0), ("dwithin", 0.05))

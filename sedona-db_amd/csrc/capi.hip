@@ -74,6 +74,102 @@ void wait_sync_event(const ArrowDeviceArray* a) {
     SJ_CHECK(hipEventSynchronize(*(hipEvent_t*)a->sync_event));
 }
 
+// Pinned host blocks for the host-facing calls (probe results handed to
+// the callback, staging of caller arrays).  A call leases a block at least
+// as large as it needs; the block returns to the pool when the lease ends
+// and is kept for later calls, so that steady-state calls allocate no
+// pinned memory (and free none: hipHostFree synchronises the device).
+// Blocks are held until the process ends.
+class PinnedPool {
+ public:
+  static PinnedPool& get() {
+    static PinnedPool* p = new PinnedPool();  // never destroyed: blocks live until exit
+    return *p;
+  }
+  void* take(size_t bytes, size_t* got) {
+    std::lock_guard<std::mutex> g(mu_);
+    // the smallest free block that fits
+    Block* best = nullptr;
+    for (auto& b : blocks_)
+      if (!b.used && b.bytes >= bytes && (!best || b.bytes < best->bytes)) best = &b;
+    if (!best) {
+      size_t sz = std::max<size_t>(bytes, (size_t)1 << 20);
+      sz = (sz + ((size_t)1 << 20) - 1) & ~(((size_t)1 << 20) - 1);
+      void* p = nullptr;
+      SJ_CHECK(hipHostMalloc(&p, sz, hipHostMallocDefault));
+      blocks_.push_back(Block{p, sz, false});
+      best = &blocks_.back();
+    }
+    best->used = true;
+    *got = best->bytes;
+    return best->p;
+  }
+  void give(void* p) {
+    std::lock_guard<std::mutex> g(mu_);
+    for (auto& b : blocks_)
+      if (b.p == p) b.used = false;
+  }
+
+ private:
+  struct Block {
+    void* p;
+    size_t bytes;
+    bool used;
+  };
+  std::mutex mu_;
+  std::deque<Block> blocks_;  // (a deque keeps element addresses stable)
+};
+
+struct PinnedLease {
+  void* p = nullptr;
+  size_t bytes = 0;
+  explicit PinnedLease(size_t n) { p = PinnedPool::get().take(n, &bytes); }
+  ~PinnedLease() { PinnedPool::get().give(p); }
+  PinnedLease(const PinnedLease&) = delete;
+  PinnedLease& operator=(const PinnedLease&) = delete;
+};
+
+// Copies n bytes of pageable host memory to the device through pinned
+// blocks: the CPU fills one chunk while the DMA of the previous one runs.
+// Returns when every chunk has been transferred (the caller may then reuse
+// or free src).
+void upload_staged(void* dst, const void* src, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  constexpr size_t kChunk = (size_t)8 << 20;
+  if (n <= kChunk) {
+    PinnedLease l(n);
+    std::memcpy(l.p, src, n);
+    SJ_CHECK(hipMemcpyAsync(dst, l.p, n, hipMemcpyHostToDevice, s));
+    SJ_CHECK(hipStreamSynchronize(s));
+    return;
+  }
+  PinnedLease a(kChunk), b(kChunk);
+  void* buf[2] = {a.p, b.p};
+  hipEvent_t done[2];
+  SJ_CHECK(hipEventCreateWithFlags(&done[0], hipEventDisableTiming));
+  SJ_CHECK(hipEventCreateWithFlags(&done[1], hipEventDisableTiming));
+  bool pending[2] = {false, false};
+  try {
+    int k = 0;
+    for (size_t off = 0; off < n; off += kChunk, k ^= 1) {
+      const size_t len = std::min(kChunk, n - off);
+      if (pending[k]) SJ_CHECK(hipEventSynchronize(done[k]));
+      std::memcpy(buf[k], (const uint8_t*)src + off, len);
+      SJ_CHECK(hipMemcpyAsync((uint8_t*)dst + off, buf[k], len, hipMemcpyHostToDevice, s));
+      SJ_CHECK(hipEventRecord(done[k], s));
+      pending[k] = true;
+    }
+    SJ_CHECK(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    (void)hipEventDestroy(done[0]);
+    (void)hipEventDestroy(done[1]);
+    throw;
+  }
+  SJ_CHECK(hipEventDestroy(done[0]));
+  SJ_CHECK(hipEventDestroy(done[1]));
+}
+
 // A stream owned for the duration of one call.
 struct ScopedStream {
   hipStream_t s = nullptr;
@@ -219,7 +315,7 @@ int ix_probe(SedonaFloatIndex2D* self, SedonaSpatialIndexContext* context, const
     if (n == 0) return;
     ScopedStream ss;
     DevArray<float4> q(n, ss.s);
-    SJ_CHECK(hipMemcpyAsync(q.p, buf, (size_t)n * sizeof(float4), hipMemcpyHostToDevice, ss.s));
+    upload_staged(q.p, buf, (size_t)n * sizeof(float4), ss.s);
     DevArray<uint32_t> order;
     int64_t nv = morton_order(q.p, n, order, ss.s);
     // point rects (the operator's rects of point rows, nulls as NaN) take
@@ -231,17 +327,20 @@ int ix_probe(SedonaFloatIndex2D* self, SedonaSpatialIndexContext* context, const
     }
     PairBuf pb;
     uint64_t cnt = filter_pairs(ix->tree, q.p, nv, order.p, pb, ss.s, nullptr, false, points);
-    std::vector<uint32_t> hb(cnt), hp(cnt);
+    // the pairs come back into pinned blocks that the callback reads in place
+    PinnedLease lb(cnt * 4 + 4), lp(cnt * 4 + 4);
+    uint32_t* hb = (uint32_t*)lb.p;
+    uint32_t* hp = (uint32_t*)lp.p;
     if (cnt) {
-      SJ_CHECK(hipMemcpyAsync(hb.data(), pb.b.p, cnt * 4, hipMemcpyDeviceToHost, ss.s));
-      SJ_CHECK(hipMemcpyAsync(hp.data(), pb.p.p, cnt * 4, hipMemcpyDeviceToHost, ss.s));
+      SJ_CHECK(hipMemcpyAsync(hb, pb.b.p, cnt * 4, hipMemcpyDeviceToHost, ss.s));
+      SJ_CHECK(hipMemcpyAsync(hp, pb.p.p, cnt * 4, hipMemcpyDeviceToHost, ss.s));
     }
     SJ_CHECK(hipStreamSynchronize(ss.s));
     // deliver in chunks that fit the callback's u32 length
     const uint64_t chunk = 1ull << 30;
     for (uint64_t off = 0; off < cnt || (cnt == 0 && off == 0); off += chunk) {
       uint32_t len = (uint32_t)std::min<uint64_t>(chunk, cnt - off);
-      cb_rc = cb(hb.data() + off, hp.data() + off, len, ud);
+      cb_rc = cb(hb + off, hp + off, len, ud);
       if (cb_rc != 0 || cnt == 0) break;
     }
   });
@@ -496,19 +595,38 @@ void refine_host(RefinerImpl* rf, const ArrowSchema* probe_schema, const ArrowAr
     return;
   }
   if (!bidx || !pidx) throw Error("null candidate index arrays");
-  ScopedStream ss;
+  ScopedStream ss, ss2;
   hipStream_t s = ss.s;
+  // two slots: a helper thread uploads the candidate arrays on its own
+  // stream while this thread decodes the probe batch
+  DevArray<uint32_t> cb, cp;
+  std::exception_ptr up_err;
+  std::thread up([&] {
+    try {
+      use_device(rf->rt);
+      cb.alloc(n, ss2.s);
+      cp.alloc(n, ss2.s);
+      upload_staged(cb.p, bidx, (size_t)n * 4, ss2.s);  // (returns with ss2's work complete)
+      upload_staged(cp.p, pidx, (size_t)n * 4, ss2.s);
+    } catch (...) {
+      up_err = std::current_exception();
+    }
+  });
   GeomColumn probe;
-  decode_wkb(wkb_input_from_arrow(probe_schema, probe_array, false), probe, s);
-  // range checks on the host side (cheap, and they keep kernels in bounds)
-  for (uint32_t i = 0; i < n; i++) {
-    if (bidx[i] >= (uint64_t)rf->build.n_rows) throw Error("build index out of range: " + std::to_string(bidx[i]));
-    if (pidx[i] >= (uint64_t)probe.n_rows) throw Error("probe index out of range: " + std::to_string(pidx[i]));
+  std::exception_ptr dec_err;
+  try {
+    decode_wkb(wkb_input_from_arrow(probe_schema, probe_array, false), probe, s);
+  } catch (...) {
+    dec_err = std::current_exception();
   }
+  up.join();
+  // the arrays are released on the call's stream (ss2 ends first)
+  cb.s = cp.s = s;
+  if (dec_err) std::rethrow_exception(dec_err);
+  if (up_err) std::rethrow_exception(up_err);
   if (pred != 100) check_relation_types(rf->build.type_mask, probe.type_mask, pred);
-  DevArray<uint32_t> cb(n, s), cp(n, s);
-  SJ_CHECK(hipMemcpyAsync(cb.p, bidx, (size_t)n * 4, hipMemcpyHostToDevice, s));
-  SJ_CHECK(hipMemcpyAsync(cp.p, pidx, (size_t)n * 4, hipMemcpyHostToDevice, s));
+  // range checks on the device, before any kernel indexes with them
+  check_pair_ranges(cb.p, cp.p, n, (uint64_t)rf->build.n_rows, (uint64_t)probe.n_rows, s);
   PairBuf out;
   uint64_t kept;
   const int precision = rf->precision.load();
@@ -518,8 +636,14 @@ void refine_host(RefinerImpl* rf, const ArrowSchema* probe_schema, const ArrowAr
     kept = refine_relation(rf->build, rf->ring_index, probe, pred, cb.p, cp.p, n, precision, out, nullptr, s);
   sort_pairs(out.b.p, out.p.p, kept, s);
   if (kept) {
-    SJ_CHECK(hipMemcpyAsync(bidx, out.b.p, kept * 4, hipMemcpyDeviceToHost, s));
-    SJ_CHECK(hipMemcpyAsync(pidx, out.p.p, kept * 4, hipMemcpyDeviceToHost, s));
+    // through pinned memory, then into the caller's arrays
+    PinnedLease l(kept * 8);
+    uint32_t* h = (uint32_t*)l.p;
+    SJ_CHECK(hipMemcpyAsync(h, out.b.p, kept * 4, hipMemcpyDeviceToHost, s));
+    SJ_CHECK(hipMemcpyAsync(h + kept, out.p.p, kept * 4, hipMemcpyDeviceToHost, s));
+    SJ_CHECK(hipStreamSynchronize(s));
+    std::memcpy(bidx, h, kept * 4);
+    std::memcpy(pidx, h + kept, kept * 4);
   }
   SJ_CHECK(hipStreamSynchronize(s));
   *new_n = (uint32_t)kept;

@@ -2553,6 +2553,37 @@ __global__ void k_unpack(const uint64_t* k, uint64_t n, uint32_t* b, uint32_t* p
 }
 }  // namespace
 
+namespace {
+// bad[0] / bad[1]: the first position whose build / probe index is out of
+// range (initially ~0)
+__global__ void k_check_ranges(const uint32_t* cb, const uint32_t* cp, uint64_t n, uint64_t nb, uint64_t np,
+                               unsigned long long* bad) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    if (cb[i] >= nb) atomicMin(&bad[0], (unsigned long long)i);
+    if (cp[i] >= np) atomicMin(&bad[1], (unsigned long long)i);
+  }
+}
+}  // namespace
+
+void check_pair_ranges(const uint32_t* cb, const uint32_t* cp, uint64_t n, uint64_t nb, uint64_t np,
+                       hipStream_t s) {
+  if (n == 0) return;
+  DevArray<unsigned long long> bad(2, s);
+  SJ_CHECK(hipMemsetAsync(bad.p, 0xff, 16, s));
+  hipLaunchKernelGGL(k_check_ranges, std::min(grid_for(n), 4096u), kBlock, 0, s, cb, cp, n, nb, np, bad.p);
+  SJ_CHECK(hipGetLastError());
+  unsigned long long h[2];
+  SJ_CHECK(hipMemcpyAsync(h, bad.p, 16, hipMemcpyDeviceToHost, s));
+  SJ_CHECK(hipStreamSynchronize(s));
+  if (h[0] == ~0ull && h[1] == ~0ull) return;
+  // the first bad position, its build index checked before its probe index
+  const bool build_bad = h[0] <= h[1];
+  uint32_t v = 0;
+  SJ_CHECK(hipMemcpyAsync(&v, (build_bad ? cb : cp) + (build_bad ? h[0] : h[1]), 4, hipMemcpyDeviceToHost, s));
+  SJ_CHECK(hipStreamSynchronize(s));
+  throw Error(std::string(build_bad ? "build" : "probe") + " index out of range: " + std::to_string(v));
+}
+
 void sort_pairs_by_build(PairBuf& pb, uint64_t n, int64_t n_build, hipStream_t s) {
   if (n < 2) return;
   int bbits = 1;

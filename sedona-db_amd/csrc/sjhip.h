@@ -569,6 +569,11 @@ void launch_ring_walk(const RingIndex& ri, const GeomColumn& probe, const uint32
                       unsigned long long* stats, uint32_t* redo, unsigned long long* n_redo, uint32_t* rest,
                       unsigned long long* n_rest, int waves, hipStream_t s, unsigned long long* queue = nullptr);
 
+// Throws "build index out of range: v" / "probe index out of range: v" for
+// the first position (build index first) with cb[i] >= nb or cp[i] >= np.
+void check_pair_ranges(const uint32_t* cb, const uint32_t* cp, uint64_t n, uint64_t nb, uint64_t np,
+                       hipStream_t s);
+
 // Refine launches over min(*n_dev, n_cap) candidates.  Point in polygon:
 // certified FP32 pass (uncertain candidates to redo/*redo_n) and the FP64
 // pass over them; stats[0..1] segment tests and uncertain tests.
