@@ -81,6 +81,10 @@ def main():
                     help="pip_zipf_contains: skip the host-inclusive (PCIe) measurement")
     ap.add_argument("--abi-batch", type=int, default=1_000_000,
                     help="pip_zipf_contains: probe rows per batch in the reference-ABI host flow")
+    ap.add_argument("--stream-batch", type=int, default=10_000_000,
+                    help="pip_zipf_contains: probe rows per batch of the pipelined host-column join")
+    ap.add_argument("--stream-slots", type=int, default=2,
+                    help="pip_zipf_contains: batches in flight in the pipelined host-column join")
     ap.add_argument("--abi-threads", type=int, default=4,
                     help="pip_zipf_contains: threads of the concurrent reference-ABI host flow")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -506,8 +510,19 @@ def host_inclusive_measurement(G, ix, rf, probe, pairs, args):
 
     abi, kept1, sum1 = abi_flow(1)
     abi_mt, kept_t, sum_t = abi_flow(args.abi_threads)
-    same = fused["pairs"] == pairs == kept1 == kept_t and fused_sum == sum1 == sum_t
-    return dict(fused=fused, reference_abi=abi, reference_abi_threads=abi_mt, same_pairs=bool(same))
+    # the pipelined host-column join: batches copied in through pinned
+    # blocks while other batches join (SjhipJoinHostStream)
+    t = time.time()
+    sb, sp = G.join_host_stream(ix, rf, parr, "contains", batch_rows=args.stream_batch, slots=args.stream_slots,
+                                refine_order=-1 if args.autotune else args.refine_order)
+    stream_ms = (time.time() - t) * 1000
+    stream_sum = pair_checksum(sb, sp)
+    stream = dict(batch_rows=args.stream_batch, slots=args.stream_slots, ms=stream_ms, pairs=int(len(sb)),
+                  pairs_per_s=len(sb) / (stream_ms / 1000))
+    log(f"host-inclusive pipelined join ({args.stream_slots} slots of {args.stream_batch} rows): {stream_ms:.1f} ms")
+    same = (fused["pairs"] == pairs == kept1 == kept_t == len(sb) and fused_sum == sum1 == sum_t == stream_sum)
+    del sb, sp
+    return dict(fused=fused, pipelined=stream, reference_abi=abi, reference_abi_threads=abi_mt, same_pairs=bool(same))
 
 
 SERVE_MIX = (("intersects", 0.0), ("dwithin", 0.05))

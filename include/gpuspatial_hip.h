@@ -317,6 +317,24 @@ int SjhipJoinDevice(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefine
                     const struct ArrowDeviceArray* probe_array,
                     const struct SjhipJoinOptions* options, struct SjhipJoinResult* out);
 
+/* The join of a host-resident probe column (Binary or LargeBinary WKB in
+ * CPU memory) in batches of batch_rows rows with `slots` (1..8) batches in
+ * flight: each slot copies its batch in through pinned blocks, joins it on
+ * its own stream as SjhipJoinDevice does, and copies the pairs back, so
+ * that the copies of one batch overlap the join of another (the
+ * reference's two-slot refine pipeline, c/sedona-libgpuspatial/
+ * libgpuspatial/src/rt_spatial_refiner.cu:244-436, and pinned staging,
+ * parallel_wkb_loader.hpp:505-534).  callback receives each batch's pairs
+ * in batch order, probe indices global to the column, from pinned memory
+ * valid during the call (one call with n = 0 for an empty column); its
+ * nonzero return stops the join and is returned.  options.stream must be
+ * NULL; with options.sort_output each batch (and so the whole output) is
+ * sorted by (probe, build). */
+int SjhipJoinHostStream(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner* refiner,
+                        const struct ArrowSchema* probe_schema, const struct ArrowArray* probe_array,
+                        const struct SjhipJoinOptions* options, int64_t batch_rows, int32_t slots,
+                        int (*callback)(const uint32_t*, const uint32_t*, uint32_t, void*), void* user_data);
+
 /* The join of SjhipJoinDevice without allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid

@@ -1415,6 +1415,129 @@ struct Server {
   }
 };
 
+namespace {
+
+__global__ void k_add_u32(uint32_t* p, uint64_t n, uint32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] += v;
+}
+
+}  // namespace
+
+// The join of a host-resident probe column in batches of batch_rows rows,
+// `slots` batches in flight: a worker thread per slot takes every
+// slots-th batch, copies its rows in through pinned blocks (the CPU fills
+// one chunk while the DMA of the previous one runs), joins it on its own
+// stream and copies the pairs back into a pinned block, so that one
+// batch's copies overlap another batch's join.  The callback gets each
+// batch's pairs, probe indices global to the column, in batch order.
+// Follows the reference's two-slot refine pipeline
+// (c/sedona-libgpuspatial/libgpuspatial/src/rt_spatial_refiner.cu:244-436)
+// and its pinned staging (parallel_wkb_loader.hpp:505-534).
+int join_host_stream(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const SjhipJoinOptions& opts,
+                     int64_t batch_rows, int slots, int (*cb)(const uint32_t*, const uint32_t*, uint32_t, void*),
+                     void* ud) {
+  if (in.on_device) throw Error("the streaming join reads a host column (use SjhipJoinDevice for device arrays)");
+  if (in.view) throw Error("the streaming join takes Binary or LargeBinary columns (BinaryView: SjhipJoinDevice)");
+  if (batch_rows <= 0 || batch_rows > (int64_t)1 << 31) throw Error("batch_rows must be in 1 .. 2^31");
+  if (slots < 1 || slots > 8) throw Error("slots must be in 1 .. 8");
+  if (!cb) throw Error("null callback");
+  if (opts.stream) throw Error("the streaming join runs on streams of its own (options.stream must be NULL)");
+  const int64_t n = in.length;
+  const int64_t nb = (n + batch_rows - 1) / batch_rows;
+  const size_t ow = in.large ? 8 : 4;
+  auto off_at = [&](int64_t row) -> int64_t {  // (row counts the array offset in)
+    return in.large ? ((const int64_t*)in.offsets)[row] : (int64_t)((const int32_t*)in.offsets)[row];
+  };
+  std::mutex mu;
+  std::condition_variable turn;
+  int64_t next = 0;  // the batch whose pairs go to the callback next
+  bool stop = false;
+  int cb_rc = 0;
+  std::exception_ptr err;
+  auto worker = [&](int w) {
+    try {
+      use_device(rf->rt);
+      ScopedStream ss;
+      DevArray<uint8_t> d_off, d_data, d_valid;
+      for (int64_t k = w; k < nb; k += slots) {
+        {
+          std::lock_guard<std::mutex> g(mu);
+          if (stop) break;
+        }
+        const int64_t r0 = k * batch_rows, cnt = std::min(batch_rows, n - r0);
+        // rows [first, first + lead + cnt) of the array, first a multiple of
+        // 8 so that the validity bytes start at a byte boundary
+        const int64_t abs0 = in.offset + r0;
+        const int64_t lead = abs0 & 7, first = abs0 - lead;
+        const int64_t o0 = off_at(first), o1 = off_at(abs0 + cnt);
+        d_off.reserve((size_t)(lead + cnt + 1) * ow, ss.s);
+        d_data.reserve((size_t)(o1 - o0) + 32, ss.s);
+        upload_staged(d_off.p, (const uint8_t*)in.offsets + first * ow, (size_t)(lead + cnt + 1) * ow, ss.s);
+        upload_staged(d_data.p, in.data + o0, (size_t)(o1 - o0), ss.s);
+        SJ_CHECK(hipMemsetAsync(d_data.p + (o1 - o0), 0, 32, ss.s));  // (the host decode's zero padding)
+        WkbInput bin;
+        bin.length = cnt;
+        bin.offset = lead;
+        bin.offsets = d_off.p;
+        bin.large = in.large;
+        // the offsets keep their values: the data pointer is moved back by o0
+        bin.data = d_data.p - o0;
+        bin.on_device = true;
+        if (in.validity) {
+          const size_t vb = (size_t)((lead + cnt + 7) / 8);
+          d_valid.reserve(vb, ss.s);
+          upload_staged(d_valid.p, in.validity + first / 8, vb, ss.s);
+          bin.validity = d_valid.p;
+        }
+        SjhipJoinOptions o = opts;
+        o.stream = ss.s;
+        SjhipJoinResult res;
+        std::memset(&res, 0, sizeof(res));
+        JoinEngine::run(ix, rf, bin, o, &res);
+        std::unique_ptr<SjhipJoinResult, void (*)(SjhipJoinResult*)> hold(&res, [](SjhipJoinResult* r) {
+          if (r->release) r->release(r);
+        });
+        const uint64_t m = res.size;
+        if (m > 0xffffffffull) throw Error("a batch gave more than 2^32 - 1 pairs: use a smaller batch_rows");
+        PinnedLease l(m * 8 + 8);
+        uint32_t* hb = (uint32_t*)l.p;
+        uint32_t* hp = hb + m;
+        if (m) {
+          if (r0) hipLaunchKernelGGL(k_add_u32, (unsigned)std::min<uint64_t>((m + 255) / 256, 4096), 256, 0, ss.s,
+                                     res.probe_indices, m, (uint32_t)r0);
+          SJ_CHECK(hipGetLastError());
+          SJ_CHECK(hipMemcpyAsync(hb, res.build_indices, m * 4, hipMemcpyDeviceToHost, ss.s));
+          SJ_CHECK(hipMemcpyAsync(hp, res.probe_indices, m * 4, hipMemcpyDeviceToHost, ss.s));
+        }
+        SJ_CHECK(hipStreamSynchronize(ss.s));
+        std::unique_lock<std::mutex> lk(mu);
+        turn.wait(lk, [&] { return next == k || stop; });
+        if (stop) break;
+        const int rc = cb(hb, hp, (uint32_t)m, ud);
+        if (rc) {
+          cb_rc = rc;
+          stop = true;
+        }
+        next++;
+        turn.notify_all();
+      }
+    } catch (...) {
+      std::lock_guard<std::mutex> g(mu);
+      if (!err) err = std::current_exception();
+      stop = true;
+      turn.notify_all();
+    }
+  };
+  const int nw = (int)std::min<int64_t>(slots, std::max<int64_t>(nb, 1));
+  std::vector<std::thread> th;
+  for (int w = 0; w < nw; w++) th.emplace_back(worker, w);
+  for (auto& t : th) t.join();
+  if (err) std::rethrow_exception(err);
+  if (nb == 0 && !cb_rc) cb_rc = cb(nullptr, nullptr, 0, ud);
+  return cb_rc;
+}
+
 }  // namespace sjhip
 
 using namespace sjhip;
@@ -1671,6 +1794,21 @@ int SjhipJoinDevice(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefine
     wait_sync_event(probe_array);
     JoinEngine::run((IndexImpl*)index->private_data, (RefinerImpl*)refiner->private_data, in, *options, out);
   });
+}
+
+int SjhipJoinHostStream(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner* refiner,
+                        const struct ArrowSchema* probe_schema, const struct ArrowArray* probe_array,
+                        const struct SjhipJoinOptions* options, int64_t batch_rows, int32_t slots,
+                        int (*callback)(const uint32_t*, const uint32_t*, uint32_t, void*), void* user_data) {
+  int cb_rc = 0;
+  int rc = guarded(nullptr, [&] {
+    if (!index || !index->private_data || !refiner || !refiner->private_data) throw Error("null index or refiner");
+    if (!probe_array || !options) throw Error("null argument");
+    WkbInput in = wkb_input_from_arrow(probe_schema, probe_array, false);
+    cb_rc = join_host_stream((IndexImpl*)index->private_data, (RefinerImpl*)refiner->private_data, in, *options,
+                             batch_rows, slots, callback, user_data);
+  });
+  return rc ? rc : cb_rc;
 }
 
 int SjhipJoinDeviceNoAlloc(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner* refiner,

@@ -200,7 +200,7 @@ EXPORTED = [
     "SjhipHostRaycast", "SjhipHostRaycastF32", "SjhipHostCollinear", "SjhipHostSegmentIntersects",
     "SjhipHostOrient2d", "SjhipHostHypot", "SjhipHostCoordToLine", "SjhipHostRaycastBatch", "SjhipHostHypotBatch",
     "SjhipTestingGraphSortPairs", "SjhipJoinDeviceNoAlloc", "SjhipRefinerKnnTies", "SjhipHostSegmentWithinF32",
-    "SjhipRefinerPairDistances", "SjhipIndexSetFilterMode",
+    "SjhipRefinerPairDistances", "SjhipIndexSetFilterMode", "SjhipJoinHostStream",
 ]
 
 _lib = None
@@ -231,6 +231,8 @@ def lib():
         L.SjhipJoinDevice.argtypes = [_IX, _RF, _SCH, ctypes.POINTER(ArrowDeviceArray),
                                       ctypes.POINTER(SjhipJoinOptions), ctypes.POINTER(SjhipJoinResult)]
         L.SjhipJoinResultCopyToHost.argtypes = [ctypes.POINTER(SjhipJoinResult), c_u32p, c_u32p]
+        L.SjhipJoinHostStream.argtypes = [_IX, _RF, _SCH, _ARR, ctypes.POINTER(SjhipJoinOptions), ctypes.c_int64,
+                                          ctypes.c_int32, PROBE_CB, ctypes.c_void_p]
         L.SjhipJoinDeviceNoAlloc.argtypes = [_IX, _RF, _SCH, ctypes.POINTER(ArrowDeviceArray),
                                              ctypes.POINTER(SjhipJoinOptions), ctypes.c_void_p, ctypes.c_void_p,
                                              ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]

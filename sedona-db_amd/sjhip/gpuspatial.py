@@ -588,6 +588,33 @@ def join_device(index, refiner, probe, predicate, distance=0.0, precision=0, sor
     return JoinResult(res)
 
 
+def join_host_stream(index, refiner, probe, predicate, batch_rows=1_000_000, slots=2, distance=0.0, precision=0,
+                     sort_output=False, sort_probes=True, refine_order=-1, callback_rc=0):
+    """The pipelined join of a host column (SjhipJoinHostStream): batches of
+    batch_rows rows, `slots` in flight.  Returns (build, probe) host arrays,
+    probe indices global, batches in order."""
+    opts = _join_options(predicate, distance, precision, sort_output, sort_probes, False, refine_order, None)
+    ex = ExportedArrow(probe)
+    outs_b, outs_p = [], []
+
+    def cb(b, p, n, ud):
+        if n:
+            outs_b.append(np.ctypeslib.as_array(b, shape=(n,)).copy())
+            outs_p.append(np.ctypeslib.as_array(p, shape=(n,)).copy())
+        return callback_rc
+
+    ccb = L.PROBE_CB(cb)
+    rc = L.lib().SjhipJoinHostStream(ctypes.byref(index.ix), ctypes.byref(refiner.rf), ctypes.byref(ex.schema),
+                                     ctypes.byref(ex.array), ctypes.byref(opts), int(batch_rows), int(slots), ccb,
+                                     None)
+    del ex
+    if rc:
+        raise GpuSpatialError(f"streaming join failed (rc={rc}): {L.lib().SjhipLastError().decode()}")
+    if not outs_b:
+        return np.zeros(0, np.uint32), np.zeros(0, np.uint32)
+    return np.concatenate(outs_b), np.concatenate(outs_p)
+
+
 # SJHIP_NOALLOC_* of include/gpuspatial_hip.h
 NOALLOC_MAX_ROWS = 65536
 NOALLOC_MAX_CANDIDATES = 1 << 20

@@ -5,7 +5,8 @@ against one index and one refiner
 c/sedona-libgpuspatial/libgpuspatial/include/gpuspatial/index/spatial_index.hpp:51-53).
 Each thread runs the whole batched join of the reference's flow (rects,
 probe, refine per batch of rows) and its pairs are checked against the
-oracle."""
+oracle.  Also the pipelined join of a host column (SjhipJoinHostStream),
+whose slots are threads of the library."""
 import sys
 import threading
 
@@ -138,3 +139,43 @@ def test_probe_contexts_keep_their_own_errors(workload):
     for k in range(3):
         assert np.array_equal(np.sort(out[k][0].astype(np.uint64) << 32 | out[k][1]),
                               np.sort(ref_b.astype(np.uint64) << 32 | ref_p))
+
+
+def _with_nulls(col, every=97):
+    v = np.ones(len(col), bool)
+    v[::every] = False
+    return D.WkbColumn(col.offsets, col.data, v)
+
+
+@pytest.mark.parametrize("large", [False, True])
+@pytest.mark.parametrize("slots,batch", [(1, 50_000), (2, 30_000), (3, 17_001)])
+def test_host_stream_join_matches_device_join(workload, slots, batch, large):
+    """SjhipJoinHostStream over a host column with nulls and a nonzero array
+    offset (not a multiple of 8) gives the pairs of one SjhipJoinDevice call
+    over the same rows, in batch order, probe indices global."""
+    b, p, ix, rf = workload
+    parr = _with_nulls(p).to_arrow(large=large).slice(5, len(p) - 11)
+    say("stream", slots, batch, large, len(parr))
+    gb, gp = G.join_host_stream(ix, rf, parr, "contains", batch_rows=batch, slots=slots, sort_output=True)
+    r = G.join_device(ix, rf, parr, "contains", sort_output=True)
+    wb, wp = r.to_host()
+    r.release()
+    assert len(gb) > 0
+    # batches in order and each sorted: the whole output sorted by (probe, build)
+    assert np.array_equal(gb, wb) and np.array_equal(gp, wp)
+    B, P = O.Store(b.as_tuple()), O.Store(_with_nulls(p).slice(5, len(p) - 6).as_tuple())
+    ob, op = O.join(B, P, "contains", nthreads=8)
+    assert np.array_equal(gb, ob) and np.array_equal(gp, op)
+
+
+def test_host_stream_join_callback_stops_and_errors(workload):
+    b, p, ix, rf = workload
+    parr = p.to_arrow()
+    with pytest.raises(G.GpuSpatialError, match="rc=9"):
+        G.join_host_stream(ix, rf, parr, "contains", batch_rows=20_000, slots=2, callback_rc=9)
+    with pytest.raises(G.GpuSpatialError, match="slots"):
+        G.join_host_stream(ix, rf, parr, "contains", batch_rows=20_000, slots=0)
+    with pytest.raises(G.GpuSpatialError, match="batch_rows"):
+        G.join_host_stream(ix, rf, parr, "contains", batch_rows=0)
+    e_b, e_p = G.join_host_stream(ix, rf, parr.slice(0, 0), "contains")
+    assert len(e_b) == len(e_p) == 0
