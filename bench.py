@@ -85,6 +85,9 @@ def main():
                     help="pip_zipf_contains: probe rows per batch of the pipelined host-column join")
     ap.add_argument("--stream-slots", type=int, default=2,
                     help="pip_zipf_contains: batches in flight in the pipelined host-column join")
+    ap.add_argument("--partition-budget-gb", type=float, default=0.0,
+                    help="pip_zipf_contains: also run the partitioned join with this device budget for the build side "
+                         "(0 = skip)")
     ap.add_argument("--abi-threads", type=int, default=4,
                     help="pip_zipf_contains: threads of the concurrent reference-ABI host flow")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -233,6 +236,22 @@ def main():
     if args.host_inclusive:
         host = host_inclusive_measurement(G, ix, rf, probe, pairs, args)
 
+    # a build side split under a device-memory budget (SjhipPartitionedJoin):
+    # K-D-B partitions built and joined one at a time against the resident
+    # probe column
+    partitioned = None
+    if args.partition_budget_gb > 0:
+        t = time.time()
+        xb, xp, st = G.partitioned_join(build.to_arrow(large=True), pdev, "contains",
+                                        int(args.partition_budget_gb * (1 << 30)))
+        part_ms = (time.time() - t) * 1000
+        partitioned = dict(budget_bytes=int(args.partition_budget_gb * (1 << 30)), ms=part_ms, stats=st,
+                           pairs=int(len(xb)), same_pairs=bool(len(xb) == pairs and pair_checksum(xb, xp) ==
+                                                                pair_checksum(gb_pairs, gp_pairs)))
+        log(f"partitioned join: {st['partitions']} partitions under {args.partition_budget_gb} GiB, {part_ms:.0f} ms "
+            f"(build {st['build_ms']:.0f} ms, joins {st['join_ms']:.0f} ms), same pairs {partitioned['same_pairs']}")
+        del xb, xp
+
     # Rooflines.  Algorithmic bytes per launch:
     #   filter: 16 B per probe rect read + 8 B per candidate pair written;
     #   refine: 8 B per candidate it reads + 8 B per kept pair (every pair of
@@ -326,6 +345,7 @@ def main():
         "autotune": tuned,
         "serving": serving,
         "host_inclusive": host,
+        "partitioned": partitioned,
     }
     if plain and args.precision == 0:
         out["speedup_vs_plain_fp64"] = plain["ms_per_step"] / ms_per_step

@@ -317,6 +317,42 @@ int SjhipJoinDevice(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefine
                     const struct ArrowDeviceArray* probe_array,
                     const struct SjhipJoinOptions* options, struct SjhipJoinResult* out);
 
+/* Statistics of a partitioned join. */
+struct SjhipPartitionStats {
+  uint32_t partitions;          /* build partitions joined */
+  uint64_t max_partition_rows;  /* rows of the largest partition */
+  uint64_t max_partition_bytes; /* its estimated resident device bytes */
+  uint64_t pairs;               /* pairs delivered */
+  double build_ms;              /* building the partitions' refiners and indexes (host gather included) */
+  double join_ms;               /* the partitions' joins, pairs copied back */
+  double total_ms;              /* the whole call */
+};
+
+/* A join whose build side need not fit in device memory: the host build
+ * column (Binary or LargeBinary WKB) is split into partitions whose
+ * estimated resident size is at most device_budget_bytes, K-D-B style
+ * (rust/sedona-spatial-join/src/partitioning/kdb.rs:36-40: along the
+ * longer axis, at the median by bytes), every row in exactly one
+ * partition; rows with an empty rect (null or empty) meet nothing and
+ * are left out.  Partitions are built and joined one at a time against
+ * the whole probe column (ROCm device memory or CPU memory) as
+ * SjhipJoinDevice does, then released: the device holds one partition at
+ * a time, the reference's memory-quota batching of the build side
+ * (c/sedona-libgpuspatial/libgpuspatial/src/relate_engine.cu:468-479).
+ * Each build row is in one partition, so every pair comes from exactly one
+ * partition and no pair is repeated.  callback receives each partition's
+ * pairs (build indices of the whole column, probe indices of the probe
+ * column), from pinned memory valid during the call; its nonzero return
+ * stops the join and is returned (one call with n = 0 when no build row
+ * has a rect).  options.stream must be NULL; options.sort_output sorts
+ * within a partition.  stats may be NULL. */
+int SjhipPartitionedJoin(struct GpuSpatialRuntime* runtime, const struct ArrowSchema* build_schema,
+                         const struct ArrowArray* build_array, const struct ArrowSchema* probe_schema,
+                         const struct ArrowDeviceArray* probe_array, const struct SjhipJoinOptions* options,
+                         uint64_t device_budget_bytes,
+                         int (*callback)(const uint32_t*, const uint32_t*, uint32_t, void*), void* user_data,
+                         struct SjhipPartitionStats* stats);
+
 /* The join of a host-resident probe column (Binary or LargeBinary WKB in
  * CPU memory) in batches of batch_rows rows with `slots` (1..8) batches in
  * flight: each slot copies its batch in through pinned blocks, joins it on

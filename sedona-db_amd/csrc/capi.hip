@@ -1538,6 +1538,59 @@ int join_host_stream(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
   return cb_rc;
 }
 
+namespace {
+
+__global__ void k_map_u32(uint32_t* p, uint64_t n, const uint32_t* table) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = table[p[i]];
+}
+
+}  // namespace
+
+// Estimated device bytes of a build row once resident (decoded FP64
+// vertices, FP32 ring-local copies, segment trees, cell raster, row
+// records and offsets) from its WKB size: a polygon's WKB holds 16 bytes
+// per vertex, its resident form about 30 (16 FP64 + 8 FP32 + tree boxes
+// and raster), plus a fixed share per row.
+uint64_t resident_bytes_of_row(int64_t wkb_bytes) { return 2 * (uint64_t)wkb_bytes + 256; }
+
+// Splits the rows ids[b, e) into partitions of at most `budget` estimated
+// resident bytes, K-D-B style (rust/sedona-spatial-join/src/partitioning/
+// kdb.rs:36-40, 302-360): along the longer axis of the rows' rect minima,
+// ordered by (min on that axis, min on the other, row id), at the row
+// where the bytes before it reach half (at least one row on each side).
+// Every row lands in exactly one partition.
+void kdb_split(std::vector<uint32_t>& ids, size_t b, size_t e, const std::vector<float4>& rects,
+               const std::vector<uint64_t>& bytes, uint64_t budget, std::vector<std::vector<uint32_t>>& out) {
+  uint64_t total = 0;
+  for (size_t i = b; i < e; i++) total += bytes[ids[i]];
+  if (total <= budget || e - b <= 1) {
+    out.emplace_back(ids.begin() + b, ids.begin() + e);
+    return;
+  }
+  float x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+  for (size_t i = b; i < e; i++) {
+    const float4 r = rects[ids[i]];
+    x0 = std::min(x0, r.x); x1 = std::max(x1, r.x);
+    y0 = std::min(y0, r.y); y1 = std::max(y1, r.y);
+  }
+  const bool by_x = (x1 - x0) >= (y1 - y0);
+  std::sort(ids.begin() + b, ids.begin() + e, [&](uint32_t u, uint32_t v) {
+    const float4 a = rects[u], c = rects[v];
+    const float a0 = by_x ? a.x : a.y, c0 = by_x ? c.x : c.y;
+    if (a0 != c0) return a0 < c0;
+    const float a1 = by_x ? a.y : a.x, c1 = by_x ? c.y : c.x;
+    if (a1 != c1) return a1 < c1;
+    return u < v;
+  });
+  size_t k = b;
+  uint64_t acc = 0;
+  while (k < e - 1 && (acc + bytes[ids[k]]) * 2 <= total) acc += bytes[ids[k++]];
+  if (k == b) k = b + 1;
+  kdb_split(ids, b, k, rects, bytes, budget, out);
+  kdb_split(ids, k, e, rects, bytes, budget, out);
+}
+
 }  // namespace sjhip
 
 using namespace sjhip;
@@ -1794,6 +1847,157 @@ int SjhipJoinDevice(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefine
     wait_sync_event(probe_array);
     JoinEngine::run((IndexImpl*)index->private_data, (RefinerImpl*)refiner->private_data, in, *options, out);
   });
+}
+
+int SjhipPartitionedJoin(struct GpuSpatialRuntime* runtime, const struct ArrowSchema* build_schema,
+                         const struct ArrowArray* build_array, const struct ArrowSchema* probe_schema,
+                         const struct ArrowDeviceArray* probe_array, const struct SjhipJoinOptions* options,
+                         uint64_t device_budget_bytes,
+                         int (*callback)(const uint32_t*, const uint32_t*, uint32_t, void*), void* user_data,
+                         struct SjhipPartitionStats* stats) {
+  int cb_rc = 0;
+  int rc = guarded(nullptr, [&] {
+    if (!build_array || !probe_array || !options || !callback) throw Error("null argument");
+    if (device_budget_bytes < ((uint64_t)1 << 20)) throw Error("device_budget_bytes must be at least 1 MiB");
+    if (options->stream) throw Error("options.stream must be NULL (each partition's join runs on its own stream)");
+    Runtime* rt = runtime_of(runtime);
+    use_device(rt);
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    const auto t_start = std::chrono::steady_clock::now();
+    WkbInput in = wkb_input_from_arrow(build_schema, build_array, false);
+    if (in.view) throw Error("the partitioned join takes a Binary or LargeBinary build column");
+    const int64_t n = in.length;
+    if (n >= 0xffffffffll) throw Error("at most 2^32 - 2 build rows");
+    auto off_at = [&](int64_t row) -> int64_t {
+      return in.large ? ((const int64_t*)in.offsets)[row] : (int64_t)((const int32_t*)in.offsets)[row];
+    };
+    // the rows' rects (the index's semantics), decoded on the device in
+    // chunks of at most a quarter of the budget of WKB
+    std::vector<float4> rects((size_t)std::max<int64_t>(n, 1));
+    std::vector<uint64_t> bytes((size_t)std::max<int64_t>(n, 1));
+    {
+      ScopedStream ss;
+      const uint64_t chunk_bytes = std::max<uint64_t>(device_budget_bytes / 4, 1);
+      int64_t r0 = 0;
+      while (r0 < n) {
+        int64_t r1 = r0 + 1;
+        const int64_t base = off_at(in.offset + r0);
+        while (r1 < n && (uint64_t)(off_at(in.offset + r1 + 1) - base) <= chunk_bytes) r1++;
+        WkbInput sub = in;
+        sub.offset = in.offset + r0;
+        sub.length = r1 - r0;
+        GeomColumn g;
+        decode_wkb(sub, g, ss.s);
+        DevArray<float4> r;
+        compute_rects(g, nullptr, true, NAN, r, ss.s);
+        SJ_CHECK(hipMemcpyAsync(rects.data() + r0, r.p, (size_t)(r1 - r0) * sizeof(float4), hipMemcpyDeviceToHost,
+                                ss.s));
+        SJ_CHECK(hipStreamSynchronize(ss.s));
+        r0 = r1;
+      }
+    }
+    // rows with an empty rect (null or empty geometry) meet nothing
+    std::vector<uint32_t> ids;
+    for (int64_t i = 0; i < n; i++) {
+      bytes[i] = resident_bytes_of_row(off_at(in.offset + i + 1) - off_at(in.offset + i));
+      const float4 r = rects[i];
+      if (r.x <= r.z && r.y <= r.w) ids.push_back((uint32_t)i);
+    }
+    std::vector<std::vector<uint32_t>> parts;
+    if (!ids.empty()) kdb_split(ids, 0, ids.size(), rects, bytes, device_budget_bytes, parts);
+    if (stats) stats->partitions = (uint32_t)parts.size();
+    // one partition at a time: its rows gathered into a LargeBinary column,
+    // built (refiner + index), joined with the whole probe column, released
+    const char* fmt = "Z";
+    for (const auto& rows : parts) {
+      const auto t0 = std::chrono::steady_clock::now();
+      std::vector<int64_t> offs(rows.size() + 1, 0);
+      uint64_t est = 0;
+      for (size_t k = 0; k < rows.size(); k++) {
+        const int64_t a = off_at(in.offset + rows[k]), e = off_at(in.offset + rows[k] + 1);
+        offs[k + 1] = offs[k] + (e - a);
+        est += bytes[rows[k]];
+      }
+      std::vector<uint8_t> data((size_t)std::max<int64_t>(offs.back(), 1));
+      for (size_t k = 0; k < rows.size(); k++) {
+        const int64_t a = off_at(in.offset + rows[k]);
+        std::memcpy(data.data() + offs[k], in.data + a, (size_t)(offs[k + 1] - offs[k]));
+      }
+      const void* bufs[3] = {nullptr, offs.data(), data.data()};
+      ArrowArray arr;
+      std::memset(&arr, 0, sizeof(arr));
+      arr.length = (int64_t)rows.size();
+      arr.n_buffers = 3;
+      arr.buffers = bufs;
+      arr.release = [](ArrowArray* a) { a->release = nullptr; };
+      ArrowSchema sch;
+      std::memset(&sch, 0, sizeof(sch));
+      sch.format = fmt;
+      sch.release = [](ArrowSchema* x) { x->release = nullptr; };
+      SedonaSpatialRefiner rf;
+      SedonaFloatIndex2D ix;
+      std::memset(&rf, 0, sizeof(rf));
+      std::memset(&ix, 0, sizeof(ix));
+      struct Holder {
+        SedonaSpatialRefiner* rf;
+        SedonaFloatIndex2D* ix;
+        ~Holder() {
+          if (ix->release) ix->release(ix);
+          if (rf->release) rf->release(rf);
+        }
+      } hold{&rf, &ix};
+      GpuSpatialRefinerConfig rcfg{runtime, 1, false, 1};
+      GpuSpatialIndexConfig icfg{runtime, 1};
+      if (GpuSpatialRefinerCreate(&rf, &rcfg)) throw Error(t_last_error);
+      auto rf_check = [&](int r) {
+        if (r) throw Error(std::string("partition build: ") + rf.get_last_error(&rf));
+      };
+      rf_check(rf.init_build_schema(&rf, &sch));
+      rf_check(rf.push_build(&rf, &arr));
+      rf_check(rf.finish_building(&rf));
+      data.clear();
+      data.shrink_to_fit();
+      if (GpuSpatialIndexFloat2DCreate(&ix, &icfg)) throw Error(t_last_error);
+      if (SjhipIndexBuildFromRefiner(&ix, &rf)) throw Error(t_last_error);
+      const auto t1 = std::chrono::steady_clock::now();
+      SjhipJoinResult res;
+      std::memset(&res, 0, sizeof(res));
+      if (SjhipJoinDevice(&ix, &rf, probe_schema, probe_array, options, &res)) throw Error(t_last_error);
+      std::unique_ptr<SjhipJoinResult, void (*)(SjhipJoinResult*)> hres(&res, [](SjhipJoinResult* r) {
+        if (r->release) r->release(r);
+      });
+      const uint64_t m = res.size;
+      if (m > 0xffffffffull) throw Error("a partition gave more than 2^32 - 1 pairs");
+      PinnedLease l(m * 8 + 8);
+      uint32_t* hb = (uint32_t*)l.p;
+      uint32_t* hp = hb + m;
+      if (m) {
+        // partition rows -> build rows of the whole column
+        ScopedStream ss;
+        DevArray<uint32_t> table(rows.size(), ss.s);
+        SJ_CHECK(hipMemcpyAsync(table.p, rows.data(), rows.size() * 4, hipMemcpyHostToDevice, ss.s));
+        hipLaunchKernelGGL(k_map_u32, (unsigned)std::min<uint64_t>((m + 255) / 256, 4096), 256, 0, ss.s,
+                           res.build_indices, m, table.p);
+        SJ_CHECK(hipGetLastError());
+        SJ_CHECK(hipMemcpyAsync(hb, res.build_indices, m * 4, hipMemcpyDeviceToHost, ss.s));
+        SJ_CHECK(hipMemcpyAsync(hp, res.probe_indices, m * 4, hipMemcpyDeviceToHost, ss.s));
+        SJ_CHECK(hipStreamSynchronize(ss.s));
+      }
+      const auto t2 = std::chrono::steady_clock::now();
+      if (stats) {
+        stats->pairs += m;
+        stats->max_partition_rows = std::max<uint64_t>(stats->max_partition_rows, rows.size());
+        stats->max_partition_bytes = std::max<uint64_t>(stats->max_partition_bytes, est);
+        stats->build_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+        stats->join_ms += std::chrono::duration<double, std::milli>(t2 - t1).count();
+      }
+      cb_rc = callback(hb, hp, (uint32_t)m, user_data);
+      if (cb_rc) return;
+    }
+    if (parts.empty()) cb_rc = callback(nullptr, nullptr, 0, user_data);
+    if (stats) stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  });
+  return rc ? rc : cb_rc;
 }
 
 int SjhipJoinHostStream(struct SedonaFloatIndex2D* index, struct SedonaSpatialRefiner* refiner,

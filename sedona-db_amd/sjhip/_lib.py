@@ -183,6 +183,12 @@ class SjhipServerConfig(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_uint32), ("memory_budget", ctypes.c_uint64), ("graphs", ctypes.c_int32)]
 
 
+class SjhipPartitionStats(ctypes.Structure):
+    _fields_ = [("partitions", ctypes.c_uint32), ("max_partition_rows", ctypes.c_uint64),
+                ("max_partition_bytes", ctypes.c_uint64), ("pairs", ctypes.c_uint64), ("build_ms", ctypes.c_double),
+                ("join_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+
+
 class SjhipRequestTiming(ctypes.Structure):
     _fields_ = [("submit_ms", ctypes.c_double), ("admit_ms", ctypes.c_double), ("done_ms", ctypes.c_double),
                 ("estimate_bytes", ctypes.c_uint64)]
@@ -200,7 +206,7 @@ EXPORTED = [
     "SjhipHostRaycast", "SjhipHostRaycastF32", "SjhipHostCollinear", "SjhipHostSegmentIntersects",
     "SjhipHostOrient2d", "SjhipHostHypot", "SjhipHostCoordToLine", "SjhipHostRaycastBatch", "SjhipHostHypotBatch",
     "SjhipTestingGraphSortPairs", "SjhipJoinDeviceNoAlloc", "SjhipRefinerKnnTies", "SjhipHostSegmentWithinF32",
-    "SjhipRefinerPairDistances", "SjhipIndexSetFilterMode", "SjhipJoinHostStream",
+    "SjhipRefinerPairDistances", "SjhipIndexSetFilterMode", "SjhipJoinHostStream", "SjhipPartitionedJoin",
 ]
 
 _lib = None
@@ -231,6 +237,10 @@ def lib():
         L.SjhipJoinDevice.argtypes = [_IX, _RF, _SCH, ctypes.POINTER(ArrowDeviceArray),
                                       ctypes.POINTER(SjhipJoinOptions), ctypes.POINTER(SjhipJoinResult)]
         L.SjhipJoinResultCopyToHost.argtypes = [ctypes.POINTER(SjhipJoinResult), c_u32p, c_u32p]
+        L.SjhipPartitionedJoin.argtypes = [ctypes.POINTER(GpuSpatialRuntime), _SCH, _ARR, _SCH,
+                                           ctypes.POINTER(ArrowDeviceArray), ctypes.POINTER(SjhipJoinOptions),
+                                           ctypes.c_uint64, PROBE_CB, ctypes.c_void_p,
+                                           ctypes.POINTER(SjhipPartitionStats)]
         L.SjhipJoinHostStream.argtypes = [_IX, _RF, _SCH, _ARR, ctypes.POINTER(SjhipJoinOptions), ctypes.c_int64,
                                           ctypes.c_int32, PROBE_CB, ctypes.c_void_p]
         L.SjhipJoinDeviceNoAlloc.argtypes = [_IX, _RF, _SCH, ctypes.POINTER(ArrowDeviceArray),

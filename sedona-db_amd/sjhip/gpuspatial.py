@@ -615,6 +615,43 @@ def join_host_stream(index, refiner, probe, predicate, batch_rows=1_000_000, slo
     return np.concatenate(outs_b), np.concatenate(outs_p)
 
 
+def partitioned_join(build, probe, predicate, device_budget_bytes, distance=0.0, precision=0, sort_output=False,
+                     refine_order=-1, options=None, callback_rc=0):
+    """A join whose build side is split into partitions of at most
+    device_budget_bytes resident bytes, built and joined one at a time
+    (SjhipPartitionedJoin).  `build` is an Arrow-exportable host column,
+    `probe` a DeviceWkb or a host column.  Returns (build, probe) host
+    arrays in partition order and the call's statistics."""
+    opts = _join_options(predicate, distance, precision, sort_output, True, False, refine_order, None)
+    rt = global_runtime(options or GpuSpatialOptions())
+    bex = ExportedArrow(build)
+    if isinstance(probe, DeviceWkb):
+        pkeep, psch, pdev = None, probe.schema, probe.dev
+    else:
+        pkeep = ExportedArrow(probe)
+        psch, pdev = pkeep.schema, host_device_array(pkeep)
+    outs_b, outs_p = [], []
+
+    def cb(b, p, n, ud):
+        if n:
+            outs_b.append(np.ctypeslib.as_array(b, shape=(n,)).copy())
+            outs_p.append(np.ctypeslib.as_array(p, shape=(n,)).copy())
+        return callback_rc
+
+    ccb = L.PROBE_CB(cb)
+    st = L.SjhipPartitionStats()
+    rc = L.lib().SjhipPartitionedJoin(ctypes.byref(rt), ctypes.byref(bex.schema), ctypes.byref(bex.array),
+                                      ctypes.byref(psch), ctypes.byref(pdev), ctypes.byref(opts),
+                                      int(device_budget_bytes), ccb, None, ctypes.byref(st))
+    del bex, pkeep
+    if rc:
+        raise GpuSpatialError(f"partitioned join failed (rc={rc}): {L.lib().SjhipLastError().decode()}")
+    stats = {k: getattr(st, k) for k, _ in L.SjhipPartitionStats._fields_}
+    if not outs_b:
+        return np.zeros(0, np.uint32), np.zeros(0, np.uint32), stats
+    return np.concatenate(outs_b), np.concatenate(outs_p), stats
+
+
 # SJHIP_NOALLOC_* of include/gpuspatial_hip.h
 NOALLOC_MAX_ROWS = 65536
 NOALLOC_MAX_CANDIDATES = 1 << 20
