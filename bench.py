@@ -260,7 +260,8 @@ def main():
     #           (8 B per vertex, each distinct polygon counted once).
     # With polygon-major order (1) the cell-raster split runs before the
     # regrouping (in its stage), and the refine stage is the certified FP32
-    # walk alone (k_refine_pip<2>) over the candidates the split left open.
+    # walk alone over the candidates the split left open: k_walk_rings for
+    # rows of one ring (all of this workload's), k_refine_pip for the rest.
     # In filter order (0) the refine stage is three launches on one stream,
     # timed together: the raster pass (k_raster_pass), the copy of the open
     # candidates (k_open_scatter) and the walk (no profiled traffic).  The
@@ -275,13 +276,13 @@ def main():
     traffic, traffic_source = read_traffic(os.path.join(ROOT, "profiles", "traffic_pip_zipf_100Mx1M_contains.md"))
     full = args.scale == 1.0 and args.precision == 0
     roof_filter = dict(bound="hbm", achieved=filter_bytes / (filt_ms / 1e3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
-                       traffic=traffic.get("k_filter") if full else None, kernel="k_filter_wave", ms=filt_ms,
+                       traffic=traffic.get("k_filter_grid") if full else None, kernel="k_filter_grid", ms=filt_ms,
                        algorithmic_bytes=filter_bytes)
     roof_filter["frac"] = roof_filter["achieved"] / HBM_PEAK_GBS
     ach_tf = refine_flops / (ref_ms / 1e3) / 1e12
     roof_refine = dict(bound="hbm", achieved=refine_bytes / (ref_ms / 1e3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
-                       traffic=traffic.get("k_refine_pip") if full and order_used == 1 else None,
-                       kernel="k_refine_pip<2>" if order_used == 1 else "k_raster_pass+k_open_scatter+k_refine_pip<2>",
+                       traffic=traffic.get("k_walk_rings") if full and order_used == 1 else None,
+                       kernel="k_walk_rings" if order_used == 1 else "k_raster_pass+k_open_scatter+k_walk_rings",
                        ms=ref_ms,
                        algorithmic_bytes=refine_bytes,
                        valu_fp32={"achieved": ach_tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",

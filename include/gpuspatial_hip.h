@@ -437,10 +437,13 @@ int SjhipRefinerSetSegmentLeaf(struct SedonaSpatialRefiner* refiner, int32_t seg
 int SjhipIndexSetTuning(struct SedonaFloatIndex2D* index, int32_t kind, int32_t refine_order, int32_t pip_waves);
 
 /* Filter of the index's probes: 0 (the default) walks the cell grid for
- * point probes and the rect tree for others, 1 walks the tree always, 2
- * the grid whenever the index has one (any probe rects).  Every mode gives
- * the same candidates; for tests and measurements.  No counterpart in the
- * reference.  Must not run concurrently with probes of the index. */
+ * point probes and the rect tree for others, and in the fused point-in-
+ * polygon join over Morton-ordered points (polygon-major order) lets each
+ * build rect test the points of the Morton cells it covers; 1 walks the
+ * tree always; 2 the grid whenever the index has one (any probe rects); 3
+ * is 0 without the build-rect pass.  Every mode gives the same candidates
+ * and pairs; for tests and measurements.  No counterpart in the reference.
+ * Must not run concurrently with probes of the index. */
 int SjhipIndexSetFilterMode(struct SedonaFloatIndex2D* index, int32_t mode);
 
 /* Precision of the refiner's refine and SjhipRefinerRefineDistance calls:

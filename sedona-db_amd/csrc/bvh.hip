@@ -573,7 +573,8 @@ void launch_filter(const RectTree& t, const float4* q, int64_t nq, const uint32_
   SJ_CHECK(hipGetLastError());
 }
 
-int64_t morton_order(const float4* rects, int64_t n, DevArray<uint32_t>& order, hipStream_t s) {
+int64_t morton_order(const float4* rects, int64_t n, DevArray<uint32_t>& order, hipStream_t s, MortonFrame* frame,
+                     DevArray<uint32_t>* sorted_keys) {
   float bb[4];
   int64_t nv;
   center_bounds(rects, n, bb, &nv, s);
@@ -587,6 +588,13 @@ int64_t morton_order(const float4* rects, int64_t n, DevArray<uint32_t>& order, 
   hipLaunchKernelGGL(k_query_keys, blocks_for(n), kBlock, 0, s, rects, n, bb[0], bb[1], sx, sy, keys.p, ids.p);
   sort_keys(keys, ids, n, s);
   order = std::move(ids);
+  if (frame) {
+    frame->x0 = bb[0];
+    frame->y0 = bb[1];
+    frame->sx = sx;
+    frame->sy = sy;
+  }
+  if (sorted_keys) *sorted_keys = std::move(keys);
   return nv;
 }
 

@@ -758,11 +758,12 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
   // predicates; outward + distance expansion for DWithin
   compute_rects(probe, nullptr, !dwithin, dwithin ? o.distance : NAN, q, s);
   mark(1);
-  DevArray<uint32_t> order;
+  DevArray<uint32_t> order, mkeys;
+  MortonFrame frame;
   int64_t nq = probe.n_rows;
   const uint32_t* ord = nullptr;
   if (o.sort_probes) {
-    nq = morton_order(q.p, probe.n_rows, order, s);
+    nq = morton_order(q.p, probe.n_rows, order, s, &frame, &mkeys);
     ord = order.p;
   }
   // Point probes refined by the point-in-polygon kernel are gathered into
@@ -800,23 +801,44 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
     // result gets the kept pairs at their exact size
     WorkspaceLease lease(rf, s);
     Workspace* ws = lease.ws.get();
-    nc = filter_pairs(ix->tree, q.p, nq, ord, ws->cand, s, ev.on ? &ft : nullptr, ranked,
-                      probe.points_only && !dwithin, ranked ? rprobe.xy.p : nullptr);
+    uint64_t n_open = 0;
+    bool split = false, grouped = false;
+    // point in polygon over Morton-ordered points, polygon-major order:
+    // each build rect tests the points of the Morton cells it covers, so
+    // the candidates come out grouped by build row, and the cell-raster
+    // split is decided in the same pass (no regrouping sort)
+    if (order_used == 1 && ranked && ix->tree.filter_mode == 0) {
+      const RingIndex& ri = rf->ring_index;
+      const bool pm_raster = ri.n_rasters > 0 && o.precision == 0 && !rf->build.points_only && ri.ring_rect.p;
+      PmSplitResult pr;
+      if (ev.on) ft.record_start(s);
+      grouped = pm_split(ix->tree, ri, rprobe.xy.p, mkeys.p, nq, frame, ord, pm_raster, ws->out,
+                         pm_raster ? ws->open : ws->cand, ws->pm, &pr, s);
+      if (ev.on) ft.record_stop(s);
+      if (grouped) {
+        nc = pr.candidates;
+        split = pm_raster;
+        n_open = pr.open;
+      }
+    }
+    if (!grouped)
+      nc = filter_pairs(ix->tree, q.p, nq, ord, ws->cand, s, ev.on ? &ft : nullptr, ranked,
+                        probe.points_only && !dwithin, ranked ? rprobe.xy.p : nullptr);
     mark(3);
     // point in polygon, polygon-major order: candidates in clean raster
     // cells are decided first, so that the regrouping sorts only the rest
     // (for DWithin over lines, the line rasters drop candidates first)
-    uint64_t n_open = 0;
-    const bool split =
-        order_used == 1 &&
-        (dwithin ? raster_split_distance(rf->build, rf->ring_index, probe, o.distance, o.precision, ws->cand.b.p,
-                                         ws->cand.p.p, nc, ws->open, ws->mask, &n_open, s)
-                 : raster_split(rf->build, rf->ring_index, rp, o.predicate, o.precision, ws->cand.b.p, ws->cand.p.p,
-                                nc, ranked ? ord : nullptr, ws->out, ws->open, ws->mask, &n_open, s));
+    if (!grouped)
+      split = order_used == 1 &&
+              (dwithin ? raster_split_distance(rf->build, rf->ring_index, probe, o.distance, o.precision,
+                                               ws->cand.b.p, ws->cand.p.p, nc, ws->open, ws->mask, &n_open, s)
+                       : raster_split(rf->build, rf->ring_index, rp, o.predicate, o.precision, ws->cand.b.p,
+                                      ws->cand.p.p, nc, ranked ? ord : nullptr, ws->out, ws->open, ws->mask, &n_open,
+                                      s));
     PairBuf& cand = split ? ws->open : ws->cand;
     const uint64_t n_cand = split ? n_open : nc;
     // polygon-major order: one wavefront then walks few distinct rings
-    if (order_used == 1) sort_pairs_by_build(cand, n_cand, rf->build.n_rows, s);
+    if (order_used == 1 && !grouped) sort_pairs_by_build(cand, n_cand, rf->build.n_rows, s);
     if (dwithin)
       kept = refine_distance(rf->build, rf->ring_index, probe, o.distance, cand.b.p, cand.p.p, n_cand, ws->out, s,
                              ev.on ? &rt : nullptr, o.precision, &st, ws, split);
@@ -2056,7 +2078,7 @@ int SjhipRefinerSetSegmentLeaf(struct SedonaSpatialRefiner* refiner, int32_t seg
 int SjhipIndexSetFilterMode(struct SedonaFloatIndex2D* index, int32_t mode) {
   return guarded(nullptr, [&] {
     if (!index || !index->private_data) throw Error("null index");
-    if (mode < 0 || mode > 2) throw Error("filter mode must be 0, 1 or 2");
+    if (mode < 0 || mode > 3) throw Error("filter mode must be 0, 1, 2 or 3");
     IndexImpl* ix = (IndexImpl*)index->private_data;
     std::lock_guard<std::mutex> g(ix->mu);
     ix->tree.filter_mode = mode;

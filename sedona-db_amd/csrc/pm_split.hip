@@ -1,0 +1,405 @@
+// pm_split.hip -- polygon-major filter of Morton-ordered point probes, with
+// the cell-raster split of the point-in-polygon refine folded in.
+//
+// The probe-major filter (the cell grid of build rects, one lane per probe)
+// emits candidates in probe order, and the walk wants them grouped by
+// polygon: that took a stable radix sort of every open candidate by build
+// row.  Here the roles are swapped.  The probe points are already sorted by
+// their 32-bit Morton keys (16 bits per axis in the probe frame), so every
+// aligned square of 2^s x 2^s Morton units -- a cell -- holds one
+// contiguous range of them.  Each build rect (in the rect tree's Morton
+// order) visits the cells its rect covers and tests their points, 64 at a
+// time, so its candidates come out together, in Morton order within a
+// cell.
+//
+// Filter semantics are those of the rect filter (bvh.hip): a pair is a
+// candidate when the FP32 build rect and the probe's FP32 point (rounded
+// to nearest) meet, bounds inclusive.  The cells a rect covers are found
+// with the same monotone FP32 map that made the probes' keys
+// (k_query_keys: q(v) = clamp(((0.5v + 0.5v) - v0) * scale, 0, 65535)),
+// so every point inside the rect lies in one of them.
+//
+// Passes (one join):
+//   k_pm_ends   one past the last sorted position of each occupied cell
+//               (cells are key >> 2s), then a max scan: each cell's start;
+//   k_pm_count  per build rect, the 64-point steps of its covered cells;
+//   scan        step offsets per rect;
+//   k_pm_fill   one record (rect, first point, points <= 64) per step;
+//   k_pm_test   per step, the hit lanes, split by the cell raster of the
+//               build row into kept (clean inside cell), open (near the
+//               boundary, or no raster) and dropped (clean outside cell),
+//               as two 64-bit masks per step;
+//   scan        kept / open offsets per step (packed in one 64-bit word);
+//   k_pm_emit   the kept pairs into the join's output and the open ones
+//               into the walk's list, each step's at its scanned offset.
+// The output order is deterministic: build rects in Morton order, then
+// cells row by row, then points in Morton order.
+#include <hipcub/hipcub.hpp>
+
+#include "ring_walk.h"
+#include "sjhip.h"
+
+namespace sjhip {
+
+namespace {
+
+constexpr int kPmBlock = 256;
+// rects covering more cells than this take the probe-major path
+constexpr int64_t kPmMaxCellsPerRect = 1 << 14;
+
+struct PmGrid {
+  float x0, y0, sx, sy;  // Morton frame of the probe keys
+  int s;                 // cell = 2^s x 2^s Morton units
+  int L;                 // cells per axis = 2^L (L = 16 - s)
+};
+
+__device__ __forceinline__ uint32_t pm_spread16(uint32_t x) {
+  x &= 0xffff;
+  x = (x | (x << 8)) & 0x00ff00ff;
+  x = (x | (x << 4)) & 0x0f0f0f0f;
+  x = (x | (x << 2)) & 0x33333333;
+  x = (x | (x << 1)) & 0x55555555;
+  return x;
+}
+
+// The probe keys' coordinate map (k_query_keys applied to a point rect,
+// whose centre 0.5v + 0.5v is v itself for normal floats), monotone in v.
+__device__ __forceinline__ uint32_t pm_q(float v, float v0, float sc) {
+  const float c = 0.5f * v + 0.5f * v;
+  return (uint32_t)fminf(fmaxf((c - v0) * sc, 0.f), 65535.f);
+}
+
+__device__ __forceinline__ uint32_t rl32(uint32_t v, int j) { return (uint32_t)__builtin_amdgcn_readlane((int)v, j); }
+__device__ __forceinline__ uint64_t rl64(uint64_t v, int j) {
+  return ((uint64_t)rl32((uint32_t)(v >> 32), j) << 32) | rl32((uint32_t)v, j);
+}
+
+// ends[c] = one past the last sorted position of cell c (cells with no
+// point keep 0); an inclusive max scan then gives each cell's start.
+__global__ void k_pm_ends(const uint32_t* __restrict__ keys, int64_t nq, int sh, uint32_t* __restrict__ ends) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nq) return;
+  const uint32_t c = keys[e] >> sh;
+  if (e == nq - 1 || (keys[e + 1] >> sh) != c) ends[c] = (uint32_t)(e + 1);
+}
+
+__global__ void k_pm_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// The cell range a build rect covers.
+__device__ __forceinline__ void pm_range(const PmGrid& g, const float4& r, uint32_t& cx0, uint32_t& cx1,
+                                         uint32_t& cy0, uint32_t& cy1) {
+  cx0 = pm_q(r.x, g.x0, g.sx) >> g.s;
+  cx1 = pm_q(r.z, g.x0, g.sx) >> g.s;
+  cy0 = pm_q(r.y, g.y0, g.sy) >> g.s;
+  cy1 = pm_q(r.w, g.y0, g.sy) >> g.s;
+}
+
+// cnt[i] = 64-point steps over the cells of leaf rect i (cnt[n] = 0), one
+// wavefront per rect, lanes over its cells; *flag set when a rect covers
+// too many cells or has too many steps.
+__global__ __launch_bounds__(kPmBlock) void k_pm_count(PmGrid g, const float4* __restrict__ rects, int64_t n,
+                                                       const uint32_t* __restrict__ start, uint32_t* __restrict__ cnt,
+                                                       unsigned int* flag) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)(((uint64_t)blockIdx.x * kPmBlock + threadIdx.x) >> 6);
+  if (i > n) return;
+  if (i == n) {
+    if (lane == 0) cnt[n] = 0;
+    return;
+  }
+  const float4 r = rects[i];
+  uint64_t steps = 0;
+  if (r.x <= r.z && r.y <= r.w) {
+    uint32_t cx0, cx1, cy0, cy1;
+    pm_range(g, r, cx0, cx1, cy0, cy1);
+    const uint32_t w = cx1 - cx0 + 1;
+    const int64_t cells = (int64_t)w * (int64_t)(cy1 - cy0 + 1);
+    if (cells > kPmMaxCellsPerRect) {
+      if (lane == 0) atomicOr(flag, 1u);
+    } else {
+      for (int64_t idx = lane; idx < cells; idx += 64) {
+        const uint32_t cx = cx0 + (uint32_t)(idx % w), cy = cy0 + (uint32_t)(idx / w);
+        const uint32_t c = pm_spread16(cx) | (pm_spread16(cy) << 1);
+        steps += (start[c + 1] - start[c] + 63u) >> 6;
+      }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) steps += __shfl_xor(steps, o);
+  if (lane == 0) {
+    if (steps > 0x7fffffffull) {
+      atomicOr(flag, 2u);
+      steps = 0;
+    }
+    cnt[i] = (uint32_t)steps;
+  }
+}
+
+// One record per step: leaf rect index, first point, points in the step.
+// One wavefront per rect: cells in row order, lanes over a cell's steps.
+__global__ __launch_bounds__(kPmBlock) void k_pm_fill(PmGrid g, const float4* __restrict__ rects, int64_t n,
+                                                      const uint32_t* __restrict__ start,
+                                                      const uint64_t* __restrict__ off, uint32_t* __restrict__ rec_i,
+                                                      uint32_t* __restrict__ rec_s, uint8_t* __restrict__ rec_n) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)(((uint64_t)blockIdx.x * kPmBlock + threadIdx.x) >> 6);
+  if (i >= n) return;
+  uint64_t k = off[i];
+  const uint64_t end = off[i + 1];
+  if (k == end) return;
+  const float4 r = rects[i];
+  uint32_t cx0, cx1, cy0, cy1;
+  pm_range(g, r, cx0, cx1, cy0, cy1);
+  for (uint32_t cy = cy0; cy <= cy1; cy++)
+    for (uint32_t cx = cx0; cx <= cx1; cx++) {
+      const uint32_t c = pm_spread16(cx) | (pm_spread16(cy) << 1);
+      const uint32_t a = start[c], b = start[c + 1];
+      const uint32_t ns = (b - a + 63u) >> 6;
+      for (uint32_t j = lane; j < ns; j += 64) {
+        const uint64_t kk = k + j;
+        if (kk < end) {
+          rec_i[kk] = (uint32_t)i;
+          rec_s[kk] = a + 64u * j;
+          rec_n[kk] = (uint8_t)min(64u, b - a - 64u * j);
+        }
+      }
+      k += ns;
+    }
+}
+
+// Per step (64 per wavefront): the hit lanes split into kept / open masks
+// (RASTER), or all hits open; packed counts (kept << 32 | open) for the
+// scan; the wavefront's hits in whits.
+template <bool RASTER>
+__global__ __launch_bounds__(kPmBlock) void k_pm_test(const float4* __restrict__ rects,
+                                                      const uint32_t* __restrict__ ids, RingIndexView R,
+                                                      const double2* __restrict__ xy, const uint32_t* __restrict__ rec_i,
+                                                      const uint32_t* __restrict__ rec_s,
+                                                      const uint8_t* __restrict__ rec_n, uint64_t S,
+                                                      ulonglong2* __restrict__ masks, uint64_t* __restrict__ cnt2,
+                                                      uint64_t* __restrict__ whits) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t w = ((uint64_t)blockIdx.x * kPmBlock + threadIdx.x) >> 6;
+  const uint64_t k0 = w * 64;
+  if (k0 >= S) return;
+  const uint64_t k = k0 + lane;
+  const bool have = k < S;
+  const uint32_t my_i = have ? rec_i[k] : 0u, my_s = have ? rec_s[k] : 0u, my_n = have ? rec_n[k] : 0u;
+  const int steps = (int)min<uint64_t>(64, S - k0);
+  uint64_t my_mk = 0, my_mo = 0;
+  uint32_t hits = 0;
+  uint32_t cur = 0xffffffffu;
+  float4 r = make_float4(0, 0, 0, 0);
+  PolyDesc d{};
+  for (int j = 0; j < steps; j++) {
+    const uint32_t i = rl32(my_i, j), st = rl32(my_s, j), n = rl32(my_n, j);
+    if (i != cur) {
+      cur = i;
+      r = rects[i];
+      if (RASTER) d = R.desc[ids[i]];
+    }
+    const bool in = (uint32_t)lane < n;
+    double2 pt = make_double2(0.0, 0.0);
+    if (in) pt = xy[st + lane];
+    const float fx = (float)pt.x, fy = (float)pt.y;
+    const bool h = in && !(fx > r.z || fx < r.x || fy > r.w || fy < r.y);
+    bool keep = false, open = h;
+    if (RASTER && h) {
+      const int cls = raster_class(R, d, pt);
+      keep = cls == 1;
+      open = cls == 0;
+    }
+    const uint64_t mk = __ballot(keep), mo = __ballot(open);
+    hits += (uint32_t)__popcll(__ballot(h));
+    if (lane == j) {
+      my_mk = mk;
+      my_mo = mo;
+    }
+  }
+  if (have) {
+    masks[k] = make_ulonglong2(my_mk, my_mo);
+    cnt2[k] = ((uint64_t)__popcll(my_mk) << 32) | (uint64_t)__popcll(my_mo);
+  }
+  if (lane == 0) whits[w] = hits;  // (one 64-bit word per wavefront: the sum needs no wider accumulator)
+}
+
+// Writes each step's kept pairs (build id, probe_ids[position]) at its
+// scanned kept offset and its open pairs (build id, position) at its open
+// offset.
+template <bool RASTER>
+__global__ __launch_bounds__(kPmBlock) void k_pm_emit(const uint32_t* __restrict__ ids,
+                                                      const uint32_t* __restrict__ rec_i,
+                                                      const uint32_t* __restrict__ rec_s, uint64_t S,
+                                                      const ulonglong2* __restrict__ masks,
+                                                      const uint64_t* __restrict__ off2,
+                                                      const uint32_t* __restrict__ probe_ids, uint32_t* __restrict__ kb,
+                                                      uint32_t* __restrict__ kp, uint32_t* __restrict__ ob,
+                                                      uint32_t* __restrict__ op) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t w = ((uint64_t)blockIdx.x * kPmBlock + threadIdx.x) >> 6;
+  const uint64_t k0 = w * 64;
+  if (k0 >= S) return;
+  const uint64_t k = k0 + lane;
+  const bool have = k < S;
+  ulonglong2 m = make_ulonglong2(0, 0);
+  uint64_t o = 0;
+  uint32_t my_i = 0, my_s = 0;
+  if (have) {
+    m = masks[k];
+    o = off2[k];
+    my_i = rec_i[k];
+    my_s = rec_s[k];
+  }
+  const int steps = (int)min<uint64_t>(64, S - k0);
+  const uint64_t lt = (1ull << lane) - 1ull;
+  uint32_t cur = 0xffffffffu, b = 0;
+  for (int j = 0; j < steps; j++) {
+    const uint64_t mk = rl64(m.x, j), mo = rl64(m.y, j);
+    if ((mk | mo) == 0) continue;
+    const uint32_t i = rl32(my_i, j), st = rl32(my_s, j);
+    const uint64_t oj = rl64(o, j);
+    if (i != cur) {
+      cur = i;
+      b = ids[i];
+    }
+    if (RASTER && ((mk >> lane) & 1)) {
+      const uint64_t pos = (oj >> 32) + (uint64_t)__popcll(mk & lt);
+      kb[pos] = b;
+      kp[pos] = probe_ids[st + lane];
+    }
+    if ((mo >> lane) & 1) {
+      const uint64_t pos = (oj & 0xffffffffull) + (uint64_t)__popcll(mo & lt);
+      ob[pos] = b;
+      op[pos] = st + lane;
+    }
+  }
+}
+
+__global__ void k_pm_set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+
+}  // namespace
+
+bool pm_split(const RectTree& t, const RingIndex& ri, const double2* xy, const uint32_t* keys, int64_t nq,
+              const MortonFrame& f, const uint32_t* probe_ids, bool raster, PairBuf& out, PairBuf& open,
+              PmScratch& sc, PmSplitResult* res, hipStream_t s) {
+  *res = PmSplitResult{};
+  if (t.grid_nx <= 0 || t.n_valid <= 0 || nq <= 0 || nq >= (int64_t)0xffffffffll || !(t.grid_inv_x > 0.f))
+    return false;
+  if (raster && (ri.desc.p == nullptr || ri.n_rasters == 0)) return false;
+  // cells about half the median build rect wide: the rect grid's cells are
+  // the median extent (coarser where the rects are large)
+  const double cell_world = 1.0 / (double)t.grid_inv_x;
+  const double scale = f.sx > 0.f ? (double)f.sx : (double)f.sy;
+  int sh = 15;
+  if (scale > 0.0) {
+    const double units = 0.5 * cell_world * scale;
+    sh = units <= 1.0 ? 0 : (int)std::lround(std::log2(units));
+  }
+  sh = std::min(15, std::max(4, sh));  // 4^(16 - sh) + 1 cell starts: at most 2^24 + 1
+  PmGrid g{f.x0, f.y0, f.sx, f.sy, sh, 16 - sh};
+  const int64_t ncell = (int64_t)1 << (2 * g.L);
+  const int64_t n = t.n_valid;
+  fit(sc.start, (size_t)(ncell + 1), s);
+  fit(sc.cnt, (size_t)(n + 1), s);
+  fit(sc.off, (size_t)(n + 1), s);
+  fit(sc.flag, 1, s);
+  // cell starts: ends of the occupied cells, then an inclusive max scan
+  // (start[0] = 0; start[c + 1] = the largest end of cells 0..c)
+  hipLaunchKernelGGL(k_pm_fill_u32, (unsigned)((ncell + 1 + kPmBlock - 1) / kPmBlock), kPmBlock, 0, s, sc.start.p,
+                     ncell + 1, 0u);
+  fit(sc.ends, (size_t)ncell, s);
+  hipLaunchKernelGGL(k_pm_fill_u32, (unsigned)((ncell + kPmBlock - 1) / kPmBlock), kPmBlock, 0, s, sc.ends.p, ncell,
+                     0u);
+  hipLaunchKernelGGL(k_pm_ends, (unsigned)((nq + kPmBlock - 1) / kPmBlock), kPmBlock, 0, s, keys, nq, 2 * sh,
+                     sc.ends.p);
+  {
+    size_t tb = 0;
+    SJ_CHECK(hipcub::DeviceScan::InclusiveScan(nullptr, tb, sc.ends.p, sc.start.p + 1, hipcub::Max(), (int)ncell, s));
+    fit(sc.tmp, tb + 16, s);
+    SJ_CHECK(hipcub::DeviceScan::InclusiveScan(sc.tmp.p, tb, sc.ends.p, sc.start.p + 1, hipcub::Max(), (int)ncell, s));
+  }
+  hipLaunchKernelGGL(k_pm_set_u64, 1, 1, 0, s, (unsigned long long*)sc.flag.p, 0ull);
+  const unsigned bw = (unsigned)(((n + 1) * 64 + kPmBlock - 1) / kPmBlock);  // one wavefront per rect
+  hipLaunchKernelGGL(k_pm_count, bw, kPmBlock, 0, s, g, t.leaf_rects.p, n, sc.start.p, sc.cnt.p,
+                     (unsigned int*)sc.flag.p);
+  SJ_CHECK(hipGetLastError());
+  {
+    size_t tb = 0;
+    SJ_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, sc.cnt.p, sc.off.p, (int)(n + 1), s));
+    fit(sc.tmp, tb + 16, s);
+    SJ_CHECK(hipcub::DeviceScan::ExclusiveSum(sc.tmp.p, tb, sc.cnt.p, sc.off.p, (int)(n + 1), s));
+  }
+  uint64_t h[2] = {0, 0};
+  SJ_CHECK(hipMemcpyAsync(&h[0], sc.off.p + n, 8, hipMemcpyDeviceToHost, s));
+  SJ_CHECK(hipMemcpyAsync(&h[1], sc.flag.p, 8, hipMemcpyDeviceToHost, s));
+  SJ_CHECK(hipStreamSynchronize(s));
+  const uint64_t S = h[0];
+  if (h[1] != 0 || S >= 0x7fffffffull) return false;
+  if (S == 0) {
+    out.counter.reserve(1, s);
+    hipLaunchKernelGGL(k_pm_set_u64, 1, 1, 0, s, out.counter.p, 0ull);
+    SJ_CHECK(hipGetLastError());
+    return true;
+  }
+  fit(sc.rec_i, S, s);
+  fit(sc.rec_s, S, s);
+  fit(sc.rec_n, S, s);
+  fit(sc.masks, S, s);
+  fit(sc.cnt2, S + 1, s);
+  fit(sc.off2, S + 1, s);
+  const uint64_t nw = (S + 63) / 64;
+  fit(sc.whits, nw, s);
+  fit(sc.hits, 1, s);
+  hipLaunchKernelGGL(k_pm_fill, (unsigned)((n * 64 + kPmBlock - 1) / kPmBlock), kPmBlock, 0, s, g, t.leaf_rects.p, n,
+                     sc.start.p, sc.off.p, sc.rec_i.p, sc.rec_s.p, sc.rec_n.p);
+  const unsigned bs = (unsigned)((S + kPmBlock - 1) / kPmBlock);
+  const RingIndexView R = ri.view();
+  if (raster)
+    hipLaunchKernelGGL(k_pm_test<true>, bs, kPmBlock, 0, s, t.leaf_rects.p, t.leaf_ids.p, R, xy, sc.rec_i.p,
+                       sc.rec_s.p, sc.rec_n.p, S, sc.masks.p, sc.cnt2.p, sc.whits.p);
+  else
+    hipLaunchKernelGGL(k_pm_test<false>, bs, kPmBlock, 0, s, t.leaf_rects.p, t.leaf_ids.p, R, xy, sc.rec_i.p,
+                       sc.rec_s.p, sc.rec_n.p, S, sc.masks.p, sc.cnt2.p, sc.whits.p);
+  SJ_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_pm_set_u64, 1, 1, 0, s, (unsigned long long*)(sc.cnt2.p + S), 0ull);
+  {
+    size_t tb = 0;
+    SJ_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, sc.cnt2.p, sc.off2.p, (int)(S + 1), s));
+    size_t tr = 0;
+    SJ_CHECK(hipcub::DeviceReduce::Sum(nullptr, tr, sc.whits.p, sc.hits.p, (int)nw, s));
+    fit(sc.tmp, std::max(tb, tr) + 16, s);
+    SJ_CHECK(hipcub::DeviceScan::ExclusiveSum(sc.tmp.p, tb, sc.cnt2.p, sc.off2.p, (int)(S + 1), s));
+    SJ_CHECK(hipcub::DeviceReduce::Sum(sc.tmp.p, tr, sc.whits.p, sc.hits.p, (int)nw, s));
+  }
+  uint64_t tot = 0, hits = 0;
+  SJ_CHECK(hipMemcpyAsync(&tot, sc.off2.p + S, 8, hipMemcpyDeviceToHost, s));
+  SJ_CHECK(hipMemcpyAsync(&hits, sc.hits.p, 8, hipMemcpyDeviceToHost, s));
+  SJ_CHECK(hipStreamSynchronize(s));
+  if (hits >= 0xffffffffull) throw Error("pm_split: 2^32 - 1 candidates or more");
+  const uint64_t kept = tot >> 32, n_open = tot & 0xffffffffull;
+  res->candidates = hits;
+  res->kept = kept;
+  res->open = n_open;
+  // the walk appends its kept pairs after these (capacity: every candidate)
+  if (raster) {
+    out.counter.reserve(1, s);
+    fit(out.b, hits > 0 ? hits : 1, s);
+    fit(out.p, hits > 0 ? hits : 1, s);
+    hipLaunchKernelGGL(k_pm_set_u64, 1, 1, 0, s, out.counter.p, (unsigned long long)kept);
+  }
+  fit(open.b, n_open > 0 ? n_open : 1, s);
+  fit(open.p, n_open > 0 ? n_open : 1, s);
+  if (raster)
+    hipLaunchKernelGGL(k_pm_emit<true>, bs, kPmBlock, 0, s, t.leaf_ids.p, sc.rec_i.p, sc.rec_s.p, S, sc.masks.p,
+                       sc.off2.p, probe_ids, out.b.p, out.p.p, open.b.p, open.p.p);
+  else
+    hipLaunchKernelGGL(k_pm_emit<false>, bs, kPmBlock, 0, s, t.leaf_ids.p, sc.rec_i.p, sc.rec_s.p, S, sc.masks.p,
+                       sc.off2.p, probe_ids, nullptr, nullptr, open.b.p, open.p.p);
+  SJ_CHECK(hipGetLastError());
+  if (nq > 0) t.cand_per_query_x16.store((uint64_t)((hits * 16) / (uint64_t)nq + 1), std::memory_order_relaxed);
+  return true;
+}
+
+}  // namespace sjhip

@@ -2192,7 +2192,14 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
   out.counter.reserve(1, s);
   fit(out.b, n > 0 ? n : 1, s);
   fit(out.p, n > 0 ? n : 1, s);
-  if (n == 0) return 0;
+  if (n == 0) {
+    if (!raster_done) return 0;
+    // a raster split decided every candidate: out holds the kept ones
+    unsigned long long cnt = 0;
+    SJ_CHECK(hipMemcpyAsync(&cnt, out.counter.p, 8, hipMemcpyDeviceToHost, s));
+    SJ_CHECK(hipStreamSynchronize(s));
+    return cnt;
+  }
   if (build.points_only || ri.ring_rect.p == nullptr) precision = 1;
   Ctx c;
   c.B = build.view();

@@ -371,6 +371,19 @@ void fit(DevArray<T>& a, size_t count, hipStream_t s) {
   a.alloc(count + count / 4 + 256, s);
 }
 
+// Scratch of the polygon-major split, kept in a workspace between joins.
+struct PmScratch {
+  DevArray<uint32_t> start, ends, cnt, rec_i, rec_s;
+  DevArray<uint64_t> off, flag, cnt2, off2, whits, hits;
+  DevArray<uint8_t> rec_n, tmp;
+  DevArray<ulonglong2> masks;
+  void set_stream(hipStream_t s) {
+    start.s = ends.s = cnt.s = rec_i.s = rec_s.s = s;
+    off.s = flag.s = cnt2.s = off2.s = whits.s = hits.s = s;
+    rec_n.s = tmp.s = masks.s = s;
+  }
+};
+
 // Device scratch kept between joins: the filter's candidate pairs, the
 // refine's lists and marks, and its output before the exact-size copy
 // into the join's result.  Reused (and grown with headroom), so that joins
@@ -384,9 +397,11 @@ struct Workspace {
   DevArray<uint32_t> redo;
   DevArray<uint32_t> rest;  // candidates of rows the single-ring walk leaves to the general kernel
   DevArray<uint64_t> mask;
+  PmScratch pm;
   void set_stream(hipStream_t s) {
     cand.b.s = cand.p.s = cand.counter.s = out.b.s = out.p.s = out.counter.s = s;
     open.b.s = open.p.s = open.counter.s = redo.s = rest.s = mask.s = s;
+    pm.set_stream(s);
   }
 };
 
@@ -407,9 +422,41 @@ uint64_t filter_pairs(const RectTree& t, const float4* q, int64_t nq, const uint
 // without a type array (its rows are valid points or NaN).
 void gather_points(const GeomColumn& probe, const uint32_t* order, int64_t n, GeomColumn& out, hipStream_t s);
 
+// Morton frame of a set of query rects: the key of a rect with centre
+// (cx, cy) interleaves qx = clamp((cx - x0) * sx, 0, 65535) (x in the even
+// bits) and qy likewise, in FP32.
+struct MortonFrame {
+  float x0 = 0.f, y0 = 0.f, sx = 0.f, sy = 0.f;
+};
+
 // Morton order of query rects (a permutation of valid queries; invalid ones
-// are dropped).  Returns the number of valid queries.
-int64_t morton_order(const float4* rects, int64_t n, DevArray<uint32_t>& order, hipStream_t s);
+// are dropped).  Returns the number of valid queries.  Optionally returns
+// the frame and the sorted keys (the first n_valid of them are the valid
+// queries').
+int64_t morton_order(const float4* rects, int64_t n, DevArray<uint32_t>& order, hipStream_t s,
+                     MortonFrame* frame = nullptr, DevArray<uint32_t>* sorted_keys = nullptr);
+
+struct PmSplitResult {
+  uint64_t candidates = 0;  // (build, probe) rect hits
+  uint64_t kept = 0;        // decided inside by a cell raster
+  uint64_t open = 0;        // left to the walk
+};
+
+// Polygon-major filter of Morton-ordered point probes: xy[0, nq) are the
+// valid probe points in Morton order (keys[0, nq) their sorted keys in
+// frame f), probe_ids maps a position to its probe row.  Every build rect
+// of t tests the points of the Morton cells it covers; the candidates come
+// out grouped by build rect (rect-tree order), in Morton order within a
+// cell.  With raster (certified point-in-polygon refine, cell rasters
+// present) the candidates in clean raster cells are decided: kept pairs
+// (build, probe row) go to out (out.counter = their number) and the rest
+// to open as (build, position); without, every candidate goes to open.
+// Returns false, having written nothing the caller reads, where it does
+// not apply (no cell grid, a rect over too many cells); the caller then
+// takes the probe-major filter.
+bool pm_split(const RectTree& t, const RingIndex& ri, const double2* xy, const uint32_t* keys, int64_t nq,
+              const MortonFrame& f, const uint32_t* probe_ids, bool raster, PairBuf& out, PairBuf& open,
+              PmScratch& sc, PmSplitResult* res, hipStream_t s);
 
 // Refine statistics
 struct RefineStats {

@@ -347,7 +347,9 @@ class GpuSpatialIndex:
             raise GpuSpatialError(self.lib.SjhipLastError().decode())
 
     def set_filter_mode(self, mode):
-        """0: grid for point probes, tree otherwise; 1: tree; 2: grid."""
+        """0: grid for point probes, tree otherwise, and the build-rect pass
+        (polygon-major) in fused point-in-polygon joins; 1: tree; 2: grid;
+        3: as 0 without the build-rect pass."""
         rc = self.lib.SjhipIndexSetFilterMode(ctypes.byref(self.ix), int(mode))
         if rc:
             raise GpuSpatialError(self.lib.SjhipLastError().decode())
