@@ -534,6 +534,57 @@ __global__ void k_query_keys(const float4* r, int64_t n, float x0, float y0, flo
 }
 }  // namespace
 
+namespace {
+// k_query_keys of the point rects (fx, fy, fx, fy), fx = x rounded to
+// FP32, read from the points themselves; rows that are empty, null or NaN
+// get the invalid key
+__global__ void k_point_keys(const double2* xy, const uint8_t* type, int64_t n, float x0, float y0, float sx,
+                             float sy, uint32_t* keys, uint32_t* ids) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double2 p = xy[i];
+  const float fx = (float)p.x, fy = (float)p.y;
+  uint32_t k = 0xffffffffu;
+  if (type[i] != 0 && fx == fx && fy == fy) {
+    const float cx = 0.5f * fx + 0.5f * fx, cy = 0.5f * fy + 0.5f * fy;
+    const uint32_t qx = (uint32_t)fminf(fmaxf((cx - x0) * sx, 0.f), 65535.f);
+    const uint32_t qy = (uint32_t)fminf(fmaxf((cy - y0) * sy, 0.f), 65535.f);
+    k = spread16(qx) | (spread16(qy) << 1);
+    if (k == 0xffffffffu) k = 0xfffffffeu;
+  }
+  keys[i] = k;
+  ids[i] = (uint32_t)i;
+}
+}  // namespace
+
+int64_t morton_order_points(const GeomColumn& g, DevArray<uint32_t>& order, hipStream_t s, MortonFrame* frame,
+                            DevArray<uint32_t>* sorted_keys) {
+  if (!g.points_only || g.pt_valid < 0 || (g.n_rows > 0 && g.type.p == nullptr))
+    throw Error("morton_order_points needs a decoded point column");
+  const int64_t n = g.n_rows, nv = g.pt_valid;
+  if (nv == 0) {
+    order.alloc(1, s);
+    return 0;
+  }
+  const float* bb = g.pt_bounds;
+  const float sx = bb[2] > bb[0] ? 65535.f / (bb[2] - bb[0]) : 0.f;
+  const float sy = bb[3] > bb[1] ? 65535.f / (bb[3] - bb[1]) : 0.f;
+  DevArray<uint32_t> keys(n, s), ids(n, s);
+  hipLaunchKernelGGL(k_point_keys, blocks_for(n), kBlock, 0, s, g.xy.p, g.type.p, n, bb[0], bb[1], sx, sy, keys.p,
+                     ids.p);
+  SJ_CHECK(hipGetLastError());
+  sort_keys(keys, ids, n, s);
+  order = std::move(ids);
+  if (frame) {
+    frame->x0 = bb[0];
+    frame->y0 = bb[1];
+    frame->sx = sx;
+    frame->sy = sy;
+  }
+  if (sorted_keys) *sorted_keys = std::move(keys);
+  return nv;
+}
+
 __global__ void k_fill_u64(unsigned long long* p, int64_t n, unsigned long long v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;

@@ -754,16 +754,25 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
   const bool dwithin = o.predicate == 100;
   if (!dwithin) check_relation_types(rf->build.type_mask, probe.type_mask, o.predicate);
   DevArray<float4> q;
+  // point probes of the point-in-polygon refine are ordered by the Morton
+  // keys of their point rects, computed from the points (the rects are
+  // made below only where a filter reads them)
+  const bool point_keys = o.sort_probes && !dwithin && probe.pt_valid >= 0 &&
+                          pip_refine_applies(rf->build, rf->ring_index, probe, o.predicate);
   // probe rects: GPU operator semantics (points nearest) for relation
   // predicates; outward + distance expansion for DWithin
-  compute_rects(probe, nullptr, !dwithin, dwithin ? o.distance : NAN, q, s);
+  auto make_rects = [&] {
+    if (!q.p) compute_rects(probe, nullptr, !dwithin, dwithin ? o.distance : NAN, q, s);
+  };
+  if (!point_keys) make_rects();
   mark(1);
   DevArray<uint32_t> order, mkeys;
   MortonFrame frame;
   int64_t nq = probe.n_rows;
   const uint32_t* ord = nullptr;
   if (o.sort_probes) {
-    nq = morton_order(q.p, probe.n_rows, order, s, &frame, &mkeys);
+    nq = point_keys ? morton_order_points(probe, order, s, &frame, &mkeys)
+                    : morton_order(q.p, probe.n_rows, order, s, &frame, &mkeys);
     ord = order.p;
   }
   // Point probes refined by the point-in-polygon kernel are gathered into
@@ -821,9 +830,11 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
         n_open = pr.open;
       }
     }
-    if (!grouped)
+    if (!grouped) {
+      make_rects();
       nc = filter_pairs(ix->tree, q.p, nq, ord, ws->cand, s, ev.on ? &ft : nullptr, ranked,
                         probe.points_only && !dwithin, ranked ? rprobe.xy.p : nullptr);
+    }
     mark(3);
     // point in polygon, polygon-major order: candidates in clean raster
     // cells are decided first, so that the regrouping sorts only the rest

@@ -31,6 +31,14 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// order-preserving uint -> float (host side of the bounds atomics)
+inline float o2f_host(uint32_t u) {
+  const uint32_t b = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+  float f;
+  std::memcpy(&f, &b, 4);
+  return f;
+}
+
 // Error word: (row << 8) | code; the smallest row wins.
 enum DecodeErr : uint32_t {
   kErrTruncated = 1,
@@ -340,16 +348,64 @@ __device__ __forceinline__ void decode_point_row(const Src& s, int64_t r, double
   }
 }
 
-__global__ void k_points(Src s, int64_t n, double2* xy, uint8_t* type, unsigned long long* err,
-                         unsigned int* type_mask) {
-  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  double2 out;
-  uint8_t t;
-  decode_point_row(s, r, out, t, err);
-  xy[r] = out;
-  type[r] = t;
+// float <-> order-preserving uint (for min / max atomics on floats)
+__device__ __forceinline__ uint32_t pt_f2o(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// Decodes point rows, one per thread, and accumulates the bounds of the
+// valid points rounded to FP32 -- the centres of their point rects -- in
+// bounds[0..3] (ordered uints: min x, min y, max x, max y) and the number
+// of rows that are not valid points in bounds[4].  A block issues a bounds
+// atomic only when it improves on the value it reads (few blocks do once
+// the extremes are found) and a count atomic only when it has such rows.
+__global__ __launch_bounds__(kBlock) void k_points(Src s, int64_t n, double2* xy, uint8_t* type,
+                                                   unsigned long long* err, unsigned int* type_mask,
+                                                   unsigned int* bounds) {
+  __shared__ unsigned int sb[5][kBlock / 64];
+  unsigned int mnx = 0xffffffffu, mny = 0xffffffffu, mxx = 0, mxy = 0, bad = 0;
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint8_t t = 0;
+  if (r < n) {
+    double2 out;
+    decode_point_row(s, r, out, t, err);
+    xy[r] = out;
+    type[r] = t;
+    const float fx = (float)out.x, fy = (float)out.y;
+    if (t != 0 && fx == fx && fy == fy) {
+      const float cx = 0.5f * fx + 0.5f * fx, cy = 0.5f * fy + 0.5f * fy;
+      mnx = mxx = pt_f2o(cx);
+      mny = mxy = pt_f2o(cy);
+    } else {
+      bad = 1;
+    }
+  }
   or_type_mask(type_mask, t);
+  for (int o = 32; o > 0; o >>= 1) {
+    mnx = min(mnx, (unsigned int)__shfl_xor((int)mnx, o));
+    mny = min(mny, (unsigned int)__shfl_xor((int)mny, o));
+    mxx = max(mxx, (unsigned int)__shfl_xor((int)mxx, o));
+    mxy = max(mxy, (unsigned int)__shfl_xor((int)mxy, o));
+    bad += (unsigned int)__shfl_xor((int)bad, o);
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    sb[0][wv] = mnx; sb[1][wv] = mny; sb[2][wv] = mxx; sb[3][wv] = mxy; sb[4][wv] = bad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / 64; w++) {
+      mnx = min(mnx, sb[0][w]); mny = min(mny, sb[1][w]);
+      mxx = max(mxx, sb[2][w]); mxy = max(mxy, sb[3][w]);
+      bad += sb[4][w];
+    }
+    if (mnx < __atomic_load_n(&bounds[0], __ATOMIC_RELAXED)) atomicMin(&bounds[0], mnx);
+    if (mny < __atomic_load_n(&bounds[1], __ATOMIC_RELAXED)) atomicMin(&bounds[1], mny);
+    if (mxx > __atomic_load_n(&bounds[2], __ATOMIC_RELAXED)) atomicMax(&bounds[2], mxx);
+    if (mxy > __atomic_load_n(&bounds[3], __ATOMIC_RELAXED)) atomicMax(&bounds[3], mxy);
+    if (bad) atomicAdd(&bounds[4], bad);
+  }
 }
 
 // The same for a column described in device memory, over a fixed capacity
@@ -520,10 +576,12 @@ void decode_wkb(const WkbInput& in0, GeomColumn& g, hipStream_t s) {
   if (!in0.view && !in0.on_device) src.bit_offset = in0.offset;
 
   DevArray<unsigned long long> d_err(1, s);
-  DevArray<unsigned int> d_flags(2, s);  // [0] type mask, [1] not-all-points
+  // [0] type mask, [1] not-all-points, [2..6] point bounds and non-point rows (k_points)
+  DevArray<unsigned int> d_flags(7, s);
   unsigned long long init_err = ~0ull;
   SJ_CHECK(hipMemcpyAsync(d_err.p, &init_err, 8, hipMemcpyHostToDevice, s));
-  SJ_CHECK(hipMemsetAsync(d_flags.p, 0, 8, s));
+  const unsigned int init_flags[7] = {0u, 0u, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+  SJ_CHECK(hipMemcpyAsync(d_flags.p, init_flags, sizeof(init_flags), hipMemcpyHostToDevice, s));
   g.type.alloc(n > 0 ? n : 1, s);
 
   if (n > 0) hipLaunchKernelGGL(k_all_points, blocks_for(n), kBlock, 0, s, src, n, d_flags.p + 1);
@@ -534,10 +592,16 @@ void decode_wkb(const WkbInput& in0, GeomColumn& g, hipStream_t s) {
   if (flags[1] == 0) {
     g.points_only = true;
     g.xy.alloc(n > 0 ? n : 1, s);
-    if (n > 0) hipLaunchKernelGGL(k_points, blocks_for(n), kBlock, 0, s, src, n, g.xy.p, g.type.p, d_err.p, d_flags.p);
+    if (n > 0)
+      hipLaunchKernelGGL(k_points, blocks_for(n), kBlock, 0, s, src, n, g.xy.p, g.type.p, d_err.p, d_flags.p,
+                         d_flags.p + 2);
+    unsigned int pf[7];
     SJ_CHECK(hipMemcpyAsync(&herr, d_err.p, 8, hipMemcpyDeviceToHost, s));
-    SJ_CHECK(hipMemcpyAsync(flags, d_flags.p, 4, hipMemcpyDeviceToHost, s));
+    SJ_CHECK(hipMemcpyAsync(pf, d_flags.p, sizeof(pf), hipMemcpyDeviceToHost, s));
     SJ_CHECK(hipStreamSynchronize(s));
+    flags[0] = pf[0];
+    g.pt_valid = n - (int64_t)pf[6];
+    for (int k = 0; k < 4; k++) g.pt_bounds[k] = g.pt_valid > 0 ? o2f_host(pf[2 + k]) : 0.f;
     g.n_parts = g.n_rings = g.n_pts = n;
   } else {
     DevArray<int64_t> cp(n, s), cr(n, s), cv(n, s);

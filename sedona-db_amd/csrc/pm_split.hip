@@ -138,7 +138,9 @@ __global__ __launch_bounds__(kPmBlock) void k_pm_count(PmGrid g, const float4* _
 }
 
 // One record per step: leaf rect index, first point, points in the step.
-// One wavefront per rect: cells in row order, lanes over a cell's steps.
+// One wavefront per rect: lanes load the starts of 64 cells at a time (row
+// order) and scan their step counts; then each cell's records are written
+// by the lanes in turn.
 __global__ __launch_bounds__(kPmBlock) void k_pm_fill(PmGrid g, const float4* __restrict__ rects, int64_t n,
                                                       const uint32_t* __restrict__ start,
                                                       const uint64_t* __restrict__ off, uint32_t* __restrict__ rec_i,
@@ -152,21 +154,40 @@ __global__ __launch_bounds__(kPmBlock) void k_pm_fill(PmGrid g, const float4* __
   const float4 r = rects[i];
   uint32_t cx0, cx1, cy0, cy1;
   pm_range(g, r, cx0, cx1, cy0, cy1);
-  for (uint32_t cy = cy0; cy <= cy1; cy++)
-    for (uint32_t cx = cx0; cx <= cx1; cx++) {
+  const uint32_t w = cx1 - cx0 + 1;
+  const int64_t cells = (int64_t)w * (int64_t)(cy1 - cy0 + 1);
+  for (int64_t c0 = 0; c0 < cells && k < end; c0 += 64) {
+    const int64_t idx = c0 + lane;
+    uint32_t a = 0, b = 0;
+    if (idx < cells) {
+      const uint32_t cx = cx0 + (uint32_t)(idx % w), cy = cy0 + (uint32_t)(idx / w);
       const uint32_t c = pm_spread16(cx) | (pm_spread16(cy) << 1);
-      const uint32_t a = start[c], b = start[c + 1];
-      const uint32_t ns = (b - a + 63u) >> 6;
-      for (uint32_t j = lane; j < ns; j += 64) {
-        const uint64_t kk = k + j;
+      a = start[c];
+      b = start[c + 1];
+    }
+    const uint32_t ns = (b - a + 63u) >> 6;
+    uint32_t incl = ns;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t v = __shfl_up(incl, o);
+      if (lane >= o) incl += v;
+    }
+    const uint32_t excl = incl - ns;
+    const int nc = (int)min<int64_t>(64, cells - c0);
+    for (int j = 0; j < nc; j++) {
+      const uint32_t nsj = rl32(ns, j);
+      if (nsj == 0) continue;
+      const uint32_t aj = rl32(a, j), bj = rl32(b, j), ej = rl32(excl, j);
+      for (uint32_t t = lane; t < nsj; t += 64) {
+        const uint64_t kk = k + ej + t;
         if (kk < end) {
           rec_i[kk] = (uint32_t)i;
-          rec_s[kk] = a + 64u * j;
-          rec_n[kk] = (uint8_t)min(64u, b - a - 64u * j);
+          rec_s[kk] = aj + 64u * t;
+          rec_n[kk] = (uint8_t)min(64u, bj - aj - 64u * t);
         }
       }
-      k += ns;
     }
+    k += rl32(incl, 63);
+  }
 }
 
 // Per step (64 per wavefront): the hit lanes split into kept / open masks
@@ -193,16 +214,19 @@ __global__ __launch_bounds__(kPmBlock) void k_pm_test(const float4* __restrict__
   uint32_t cur = 0xffffffffu;
   float4 r = make_float4(0, 0, 0, 0);
   PolyDesc d{};
+  // the next step's point is loaded before this step is tested
+  double2 nxt = make_double2(0.0, 0.0);
+  if ((uint32_t)lane < rl32(my_n, 0)) nxt = xy[rl32(my_s, 0) + lane];
   for (int j = 0; j < steps; j++) {
-    const uint32_t i = rl32(my_i, j), st = rl32(my_s, j), n = rl32(my_n, j);
+    const uint32_t i = rl32(my_i, j), n = rl32(my_n, j);
+    const double2 pt = nxt;
+    if (j + 1 < steps && (uint32_t)lane < rl32(my_n, j + 1)) nxt = xy[rl32(my_s, j + 1) + lane];
     if (i != cur) {
       cur = i;
       r = rects[i];
       if (RASTER) d = R.desc[ids[i]];
     }
     const bool in = (uint32_t)lane < n;
-    double2 pt = make_double2(0.0, 0.0);
-    if (in) pt = xy[st + lane];
     const float fx = (float)pt.x, fy = (float)pt.y;
     const bool h = in && !(fx > r.z || fx < r.x || fy > r.w || fy < r.y);
     bool keep = false, open = h;
@@ -225,9 +249,32 @@ __global__ __launch_bounds__(kPmBlock) void k_pm_test(const float4* __restrict__
   if (lane == 0) whits[w] = hits;  // (one 64-bit word per wavefront: the sum needs no wider accumulator)
 }
 
-// Writes each step's kept pairs (build id, probe_ids[position]) at its
-// scanned kept offset and its open pairs (build id, position) at its open
-// offset.
+// The r-th (0-based) set bit of m, which has more than r set bits.
+__device__ __forceinline__ uint32_t pm_nth_bit(uint64_t m, uint32_t r) {
+  uint32_t pos = 0, v = (uint32_t)m, c = (uint32_t)__popc(v);
+  if (r >= c) {
+    r -= c;
+    v = (uint32_t)(m >> 32);
+    pos = 32;
+  }
+  c = (uint32_t)__popc(v & 0xffffu);
+  if (r >= c) { r -= c; v >>= 16; pos += 16; }
+  c = (uint32_t)__popc(v & 0xffu);
+  if (r >= c) { r -= c; v >>= 8; pos += 8; }
+  c = (uint32_t)__popc(v & 0xfu);
+  if (r >= c) { r -= c; v >>= 4; pos += 4; }
+  c = (uint32_t)__popc(v & 0x3u);
+  if (r >= c) { r -= c; v >>= 2; pos += 2; }
+  if (r >= (v & 1u)) pos += 1;
+  return pos;
+}
+
+// Writes the kept pairs (build id, probe_ids[position]) and the open pairs
+// (build id, position) of 64 steps per wavefront.  The steps' outputs are
+// contiguous (their offsets come from one scan), so each lane takes every
+// 64th output of the wavefront's range, finds its step by binary search
+// over the step offsets (in LDS) and its point as the n-th set bit of the
+// step's mask: every store is coalesced.
 template <bool RASTER>
 __global__ __launch_bounds__(kPmBlock) void k_pm_emit(const uint32_t* __restrict__ ids,
                                                       const uint32_t* __restrict__ rec_i,
@@ -237,42 +284,49 @@ __global__ __launch_bounds__(kPmBlock) void k_pm_emit(const uint32_t* __restrict
                                                       const uint32_t* __restrict__ probe_ids, uint32_t* __restrict__ kb,
                                                       uint32_t* __restrict__ kp, uint32_t* __restrict__ ob,
                                                       uint32_t* __restrict__ op) {
-  const int lane = threadIdx.x & 63;
+  constexpr int kW = kPmBlock / 64;
+  __shared__ uint32_t s_ko[kW][64], s_oo[kW][64], s_b[kW][64], s_st[kW][64];
+  __shared__ uint64_t s_mk[kW][64], s_mo[kW][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint64_t w = ((uint64_t)blockIdx.x * kPmBlock + threadIdx.x) >> 6;
   const uint64_t k0 = w * 64;
-  if (k0 >= S) return;
+  const int steps = k0 < S ? (int)min<uint64_t>(64, S - k0) : 0;
   const uint64_t k = k0 + lane;
-  const bool have = k < S;
-  ulonglong2 m = make_ulonglong2(0, 0);
-  uint64_t o = 0;
-  uint32_t my_i = 0, my_s = 0;
-  if (have) {
-    m = masks[k];
+  uint64_t o = 0, o_end = 0;
+  if (lane < steps) {
+    const ulonglong2 m = masks[k];
     o = off2[k];
-    my_i = rec_i[k];
-    my_s = rec_s[k];
+    s_mk[wv][lane] = m.x;
+    s_mo[wv][lane] = m.y;
+    s_b[wv][lane] = ids[rec_i[k]];
+    s_st[wv][lane] = rec_s[k];
   }
-  const int steps = (int)min<uint64_t>(64, S - k0);
-  const uint64_t lt = (1ull << lane) - 1ull;
-  uint32_t cur = 0xffffffffu, b = 0;
-  for (int j = 0; j < steps; j++) {
-    const uint64_t mk = rl64(m.x, j), mo = rl64(m.y, j);
-    if ((mk | mo) == 0) continue;
-    const uint32_t i = rl32(my_i, j), st = rl32(my_s, j);
-    const uint64_t oj = rl64(o, j);
-    if (i != cur) {
-      cur = i;
-      b = ids[i];
-    }
-    if (RASTER && ((mk >> lane) & 1)) {
-      const uint64_t pos = (oj >> 32) + (uint64_t)__popcll(mk & lt);
-      kb[pos] = b;
-      kp[pos] = probe_ids[st + lane];
-    }
-    if ((mo >> lane) & 1) {
-      const uint64_t pos = (oj & 0xffffffffull) + (uint64_t)__popcll(mo & lt);
-      ob[pos] = b;
-      op[pos] = st + lane;
+  if (steps > 0 && lane == 0) o_end = off2[k0 + steps];  // (off2 holds S + 1 entries)
+  const uint64_t base = rl64(o, 0), stop = rl64(o_end, 0);
+  // offsets relative to the wavefront's first step (below 64 * 64 each)
+  if (lane < steps) {
+    s_ko[wv][lane] = (uint32_t)((o >> 32) - (base >> 32));
+    s_oo[wv][lane] = (uint32_t)((o & 0xffffffffull) - (base & 0xffffffffull));
+  }
+  __syncthreads();
+  if (steps == 0) return;
+  for (int list = RASTER ? 0 : 1; list < 2; list++) {
+    const uint32_t* so = list == 0 ? s_ko[wv] : s_oo[wv];
+    const uint64_t* sm = list == 0 ? s_mk[wv] : s_mo[wv];
+    const uint64_t b0 = list == 0 ? (base >> 32) : (base & 0xffffffffull);
+    const uint32_t total = (uint32_t)((list == 0 ? (stop >> 32) : (stop & 0xffffffffull)) - b0);
+    for (uint32_t x = lane; x < total; x += 64) {
+      int j = 0;
+      for (int d = 32; d > 0; d >>= 1)
+        if (j + d < steps && so[j + d] <= x) j += d;
+      const uint32_t pos = s_st[wv][j] + pm_nth_bit(sm[j], x - so[j]);
+      if (list == 0) {
+        kb[b0 + x] = s_b[wv][j];
+        kp[b0 + x] = probe_ids[pos];
+      } else {
+        ob[b0 + x] = s_b[wv][j];
+        op[b0 + x] = pos;
+      }
     }
   }
 }

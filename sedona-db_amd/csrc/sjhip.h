@@ -184,6 +184,11 @@ struct GeomColumn {
   int64_t n_rows = 0, n_parts = 0, n_rings = 0, n_pts = 0;
   bool points_only = false;
   uint32_t type_mask = 0;  // bit t set when a row of type t is present
+  // point columns decoded from WKB: bounds of the valid points rounded to
+  // FP32 (min x, min y, max x, max y; the centres of their point rects)
+  // and their number, or pt_valid < 0 when not computed
+  float pt_bounds[4] = {0.f, 0.f, 0.f, 0.f};
+  int64_t pt_valid = -1;
   DevArray<uint8_t> type;
   DevArray<int64_t> part_off, ring_off, pt_off;
   DevArray<double2> xy;
@@ -435,6 +440,13 @@ struct MortonFrame {
 // queries').
 int64_t morton_order(const float4* rects, int64_t n, DevArray<uint32_t>& order, hipStream_t s,
                      MortonFrame* frame = nullptr, DevArray<uint32_t>* sorted_keys = nullptr);
+
+// Morton order of the valid rows of a point column by the keys its point
+// rects (rounded to nearest) would get from morton_order: the same order,
+// frame and sorted keys, without the rects.  Needs the bounds decode_wkb
+// records for point columns.
+int64_t morton_order_points(const GeomColumn& points, DevArray<uint32_t>& order, hipStream_t s, MortonFrame* frame,
+                            DevArray<uint32_t>* sorted_keys);
 
 struct PmSplitResult {
   uint64_t candidates = 0;  // (build, probe) rect hits
