@@ -206,6 +206,20 @@ def main():
         f"filter {stage[2]:.2f} refine {stage[3]:.2f}")
 
     last.release()
+    # untimed: the probe-major filter (one lane per Morton-ordered point,
+    # the rects of its grid cell; then the raster split and the regrouping
+    # sort) in place of the build-rect pass, for its own roofline
+    ix.set_filter_mode(3)
+    pmaj = []
+    for _ in range(2):
+        r = step(args.precision)
+        pmaj.append((list(r.res.stage_ms), r.size))
+        r.release()
+    ix.set_filter_mode(0)
+    assert all(x[1] == pairs for x in pmaj), "probe-major pairs differ"
+    pmaj_stage = np.mean([x[0] for x in pmaj], axis=0)
+    log(f"probe-major filter: {pmaj_stage[2]:.2f} ms, split and regroup {pmaj_stage[5]:.2f} ms")
+
     # untimed: the same join with its pairs sorted by (probe, build) on the
     # device, for the exact pair-set comparisons below
     t_sorted = time.time()
@@ -275,10 +289,20 @@ def main():
     refine_flops = FLOPS_PER_SEGMENT_TEST * seg_tests
     traffic, traffic_source = read_traffic(os.path.join(ROOT, "profiles", "traffic_pip_zipf_100Mx1M_contains.md"))
     full = args.scale == 1.0 and args.precision == 0
+    # the filter stage of the timed joins is the build-rect pass (k_pm_*:
+    # each build rect tests the Morton cells it covers, the cell-raster
+    # split decided in the same pass); its algorithmic bytes are the
+    # filter's (it also replaces the raster pass and the regrouping sort)
     roof_filter = dict(bound="hbm", achieved=filter_bytes / (filt_ms / 1e3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
-                       traffic=traffic.get("k_filter_grid") if full else None, kernel="k_filter_grid", ms=filt_ms,
+                       traffic=None, kernel="k_pm_ends+k_pm_count+k_pm_fill+k_pm_test+k_pm_emit", ms=filt_ms,
                        algorithmic_bytes=filter_bytes)
     roof_filter["frac"] = roof_filter["achieved"] / HBM_PEAK_GBS
+    pm_ms = pmaj_stage[2]
+    roof_filter_pm = dict(bound="hbm", achieved=filter_bytes / (pm_ms / 1e3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
+                          traffic=traffic.get("k_filter_grid") if full else None,
+                          kernel="k_filter_grid", ms=pm_ms, algorithmic_bytes=filter_bytes,
+                          split_and_regroup_ms=pmaj_stage[5])
+    roof_filter_pm["frac"] = roof_filter_pm["achieved"] / HBM_PEAK_GBS
     ach_tf = refine_flops / (ref_ms / 1e3) / 1e12
     roof_refine = dict(bound="hbm", achieved=refine_bytes / (ref_ms / 1e3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
                        traffic=traffic.get("k_walk_rings") if full and order_used == 1 else None,
@@ -331,7 +355,7 @@ def main():
         "roofline_kernel": dominant["kernel"],
         "roofline_traffic_source": traffic_source,
         "roofline_valu_fp32": dominant.get("valu_fp32"),
-        "roofline_stages": {"filter": roof_filter, "refine": roof_refine},
+        "roofline_stages": {"filter_split": roof_filter, "filter_probe_major": roof_filter_pm, "refine": roof_refine},
         "cpu_baseline": cpu,
         "pairs_per_step": int(pairs),
         "candidates_per_step": int(cands),
