@@ -1020,6 +1020,77 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8
   }
 }
 
+// The FP64 pass over the candidates the certified walks deferred (list[0,
+// *n_list) of candidate indices), one wavefront per candidate: for a row
+// of one ring the lanes take the ring's segments with stride 64 and vote
+// (any ON: allow_on_edge, else the parity of IN), which is the full-scan
+// form of tg_ring_contains_point along +x (tg.c:2809-2822) that the
+// per-lane FP64 walk prunes; other rows are decided by lane 0 as in
+// k_refine_pip<1>.  The deferred rays were long or met a vertex, so a lane
+// per candidate left most of its wavefront idle.
+__global__ __launch_bounds__(kBlock) void k_pip_redo_wave(Ctx c, int allow_on_edge, const uint32_t* cb,
+                                                          const uint32_t* cp, const uint32_t* list,
+                                                          const unsigned long long* n_list, const uint32_t* probe_ids,
+                                                          uint32_t* ob, uint32_t* op, uint64_t cap,
+                                                          unsigned long long* counter) {
+  SJ_WAVE_OUT(wo);
+  const int lane = threadIdx.x & 63;
+  const uint64_t nl = *n_list;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t rounds = (nl + nwaves - 1) / nwaves;  // (every wavefront runs the same rounds: emit is wave-wide)
+  for (uint64_t t = 0; t < rounds; t++) {
+    const uint64_t k = wave + t * nwaves;
+    bool keep = false;
+    uint32_t b = 0, p = 0;
+    if (k < nl) {
+      const uint32_t i = list[k];
+      b = cb[i];
+      p = cp[i];
+      const double2 pt = c.P.xy[p];
+      if (pt.x == pt.x || pt.y == pt.y) {
+        const PolyDesc d = c.R.desc[b];
+        int cls = 0;
+        if (d.raster_off >= 0 && rect_covers(d.rect, pt.x, pt.y)) {
+          const double cw = (d.rect.z - d.rect.x) / kRasterCells, ch = (d.rect.w - d.rect.y) / kRasterCells;
+          const int ci = min(max((int)floor((pt.x - d.rect.x) / cw), 0), kRasterCells - 1);
+          const int cj = min(max((int)floor((pt.y - d.rect.y) / ch), 0), kRasterCells - 1);
+          const int cell = cj * kRasterCells + ci;
+          cls = (c.R.raster[d.raster_off + (cell >> 4)] >> (2 * (cell & 15))) & 3;
+        }
+        if (cls) {
+          keep = cls == 1;
+        } else if (d.npts >= 0) {
+          if (rect_covers(d.rect, pt.x, pt.y)) {
+            RingRef r;
+            r.xy = c.B.xy + d.v0;
+            r.v0 = 0;
+            r.npts = d.npts;
+            const int64_t ns = d.npts >= 3 ? d.npts - 1 : 0;
+            int on = 0, in = 0;
+            uint64_t tests = 0;
+            ring_segments_f64(r, lane, ns, 64, pt.x, pt.y, on, in, tests);
+            const bool any_on = __ballot(on != 0) != 0;
+            keep = any_on ? allow_on_edge != 0 : (__popcll(__ballot(in != 0)) & 1) != 0;
+          }
+        } else if (c.B.type[b] != kNull) {
+          uint64_t tests = 0, fallbacks = 0;
+          bool k0 = false;
+          if (lane == 0) {
+            const int64_t q1 = c.B.part_off[b + 1];
+            for (int64_t q = c.B.part_off[b]; q < q1 && !k0; q++)
+              k0 = build_poly_contains<1>(c, q, pt.x, pt.y, allow_on_edge != 0, tests, fallbacks);
+          }
+          keep = __shfl((int)k0, 0) != 0;
+        }
+      }
+      if (keep && probe_ids) p = probe_ids[p];
+    }
+    emit(wo, lane == 0 && keep, b, p, ob, op, cap, counter);
+  }
+  wave_flush(wo, ob, op, cap, counter);
+}
+
 // tg_ring_intersects_ring for one candidate with the whole wavefront: the
 // lanes take the smaller-rect ring's segments with stride 64 and the
 // wavefront votes.  Every lane returns the same value.
@@ -2262,8 +2333,8 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
       // (the launches the roofline figures describe)
       if (timer) timer->record_stop(s);
       timer = nullptr;
-      hipLaunchKernelGGL(k_refine_pip<1>, 1024, kBlock, 0, s, c, allow, wb, wp, n, redo.p, out.b.p, out.p.p,
-                         out.cap(), out.counter.p, nullptr, probe_ids, nullptr, nullptr, aux.p);
+      hipLaunchKernelGGL(k_pip_redo_wave, 2048, kBlock, 0, s, c, allow, wb, wp, redo.p, aux.p, probe_ids, out.b.p,
+                         out.p.p, out.cap(), out.counter.p);
       // (read before open_count is released on the stream)
       if (n_open) SJ_CHECK(hipMemcpyAsync(&walked, n_open, 8, hipMemcpyDeviceToHost, s));
     } else if (precision == 0) {
