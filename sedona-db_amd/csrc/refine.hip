@@ -1022,11 +1022,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8
 
 // The FP64 pass over the candidates the certified walks deferred (list[0,
 // *n_list) of candidate indices), one wavefront per candidate: for a row
-// of one ring the lanes take the ring's segments with stride 64 and vote
-// (any ON: allow_on_edge, else the parity of IN), which is the full-scan
-// form of tg_ring_contains_point along +x (tg.c:2809-2822) that the
-// per-lane FP64 walk prunes; other rows are decided by lane 0 as in
-// k_refine_pip<1>.  The deferred rays were long or met a vertex, so a lane
+// of one ring the lanes take the ring's segments with stride 64 (or its
+// tree's leaves, skipping those a +x ray cannot meet, as the per-lane
+// FP64 walk prunes them) and vote (any ON: allow_on_edge, else the parity
+// of IN): tg_ring_contains_point along +x (tg.c:2809-2822).  Other rows
+// are decided by lane 0 as in k_refine_pip<1>.  The deferred rays were long or met a vertex, so a lane
 // per candidate left most of its wavefront idle.
 __global__ __launch_bounds__(kBlock) void k_pip_redo_wave(Ctx c, int allow_on_edge, const uint32_t* cb,
                                                           const uint32_t* cp, const uint32_t* list,
@@ -1069,7 +1069,25 @@ __global__ __launch_bounds__(kBlock) void k_pip_redo_wave(Ctx c, int allow_on_ed
             const int64_t ns = d.npts >= 3 ? d.npts - 1 : 0;
             int on = 0, in = 0;
             uint64_t tests = 0;
-            ring_segments_f64(r, lane, ns, 64, pt.x, pt.y, on, in, tests);
+            if (d.tree_off >= 0) {
+              // lanes take the ring tree's leaves (level 0: one FP32
+              // ring-local box per 2^ls segments) and scan those a +x ray
+              // can meet, with the tree walk's margin of 2E
+              const float4* lv0 = c.R.seg_tree + d.tree_off;
+              const int ls = c.R.seg_leaf_shift;
+              const int64_t nleaf = (ns + (1 << ls) - 1) >> ls;
+              const double2 o = ring_origin(d.rect);
+              const float pxl = (float)(pt.x - o.x), pyl = (float)(pt.y - o.y), m = 2.0f * d.err;
+              for (int64_t lf = lane; lf < nleaf; lf += 64) {
+                const float4 bx = lv0[lf];
+                if (pyl >= bx.y - m && pyl <= bx.w + m && pxl <= bx.z + m) {
+                  const int64_t s0 = lf << ls;
+                  ring_segments_f64(r, s0, min(s0 + ((int64_t)1 << ls), ns), 1, pt.x, pt.y, on, in, tests);
+                }
+              }
+            } else {
+              ring_segments_f64(r, lane, ns, 64, pt.x, pt.y, on, in, tests);
+            }
             const bool any_on = __ballot(on != 0) != 0;
             keep = any_on ? allow_on_edge != 0 : (__popcll(__ballot(in != 0)) & 1) != 0;
           }
