@@ -858,11 +858,19 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
                              ws->out, &st, s, ev.on ? &rt : nullptr, ranked ? ord : nullptr, ix->tuning.pip_waves,
                              ws, split);
     mark(8);
-    jr->pairs.b.alloc(kept > 0 ? kept : 1, s);
-    jr->pairs.p.alloc(kept > 0 ? kept : 1, s);
-    if (kept) {
-      SJ_CHECK(hipMemcpyAsync(jr->pairs.b.p, ws->out.b.p, kept * 4, hipMemcpyDeviceToDevice, s));
-      SJ_CHECK(hipMemcpyAsync(jr->pairs.p.p, ws->out.p.p, kept * 4, hipMemcpyDeviceToDevice, s));
+    if (kept > 0 && kept * 4 >= ws->out.cap()) {
+      // the kept pairs fill a good part of the workspace's output: the
+      // result takes those buffers (the workspace gets new ones from the
+      // pool on its next join) instead of a copy
+      jr->pairs.b = std::move(ws->out.b);
+      jr->pairs.p = std::move(ws->out.p);
+    } else {
+      jr->pairs.b.alloc(kept > 0 ? kept : 1, s);
+      jr->pairs.p.alloc(kept > 0 ? kept : 1, s);
+      if (kept) {
+        SJ_CHECK(hipMemcpyAsync(jr->pairs.b.p, ws->out.b.p, kept * 4, hipMemcpyDeviceToDevice, s));
+        SJ_CHECK(hipMemcpyAsync(jr->pairs.p.p, ws->out.p.p, kept * 4, hipMemcpyDeviceToDevice, s));
+      }
     }
   }
   if (fused) mark(8);
