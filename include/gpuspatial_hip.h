@@ -468,9 +468,11 @@ struct SjhipServerConfig {
                               point probes in device memory (up to 65536
                               rows; point-in-polygon predicates and DWithin):
                               one graph per slot, predicate and power-of-two
-                              size bucket, captured on first use.  Switched
-                              off in this build: SjhipServerCreate fails
-                              with EINVAL unless graphs is 0 */
+                              size bucket, captured on first use; a graph
+                              holds kernels and copies only (no memset or
+                              allocation node).  Larger requests, and any
+                              whose decode or candidates exceed the plan,
+                              take the eager path */
 };
 
 /* Timing of one request, ms on a steady clock since the server's

@@ -258,6 +258,26 @@ def test_device_join_uniform_stars_golden(precision, sort_probes, opts):
     assert np.array_equal(gp, E["uniform_small/intersects/probe"])
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_c1_full_size_vs_oracle(precision, opts):
+    """The oracle's own configuration at its full size: 1M uniform points x
+    10k stars (8..32 vertices, radius lognormal with mean 0.5), ST_Intersects,
+    through the fused device join and through the reference ABI flow
+    (index probe, then refine of the host candidates)."""
+    b, p = D.workload_pip_uniform(1_000_000, 10_000, seed=1)
+    ob, op = O.join(O.Store(b.as_tuple()), O.Store(p.as_tuple()), "intersects", nthreads=8)
+    say("C1 full size, precision", precision, "oracle pairs", len(ob))
+    assert 600_000 < len(ob) < 700_000
+    ix, rf = build_pair(b.to_arrow(), opts)
+    rf.set_precision(precision)
+    r = G.join_device(ix, rf, G.DeviceWkb.from_host(p), "intersects", precision=precision)
+    gb, gp = r.to_host()
+    assert np.array_equal(gb, ob) and np.array_equal(gp, op), (len(gb), len(ob))
+    cb, cp = ix.probe(G.compute_rects(p.to_arrow(), point_nearest=True))
+    hb, hp = rf.refine(p.to_arrow(), "intersects", cb, cp)
+    assert np.array_equal(hb, ob) and np.array_equal(hp, op), (len(hb), len(ob))
+
+
 @pytest.mark.parametrize("pred", ["contains", "intersects", "covers"])
 @pytest.mark.parametrize("precision", [0, 1])
 def test_device_join_zipf_stars_vs_oracle(pred, precision, opts):
