@@ -754,11 +754,24 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
   const bool dwithin = o.predicate == 100;
   if (!dwithin) check_relation_types(rf->build.type_mask, probe.type_mask, o.predicate);
   DevArray<float4> q;
-  // point probes of the point-in-polygon refine are ordered by the Morton
-  // keys of their point rects, computed from the points (the rects are
-  // made below only where a filter reads them)
-  const bool point_keys = o.sort_probes && !dwithin && probe.pt_valid >= 0 &&
-                          pip_refine_applies(rf->build, rf->ring_index, probe, o.predicate);
+  // refine order: the caller's, or (refine_order < 0) the tuned one, else
+  // the default of the refine kind
+  const bool pip_kind = !dwithin && pip_refine_applies(rf->build, rf->ring_index, probe, o.predicate);
+  const int kind = dwithin ? 1 : pip_kind && o.sort_probes ? 0 : 2;
+  int order_used = o.refine_order;
+  if (order_used < 0) order_used = ix->tuning.refine_order[kind];
+  // default: polygon-major; DWithin in filter order unless the build side
+  // has line rasters (whose split leaves few enough candidates to sort)
+  if (order_used < 0) order_used = dwithin && rf->ring_index.n_line_rasters == 0 ? 0 : 1;
+  // ST_DWithin of point probes against single lines with line rasters, in
+  // line-major order: the build-rect pass, over the points in Morton order
+  const bool dist_pm = dwithin && o.sort_probes && order_used == 1 && probe.pt_valid >= 0 &&
+                       ix->tree.filter_mode == 0 &&
+                       line_rasters_apply(rf->build, rf->ring_index, probe, o.distance, o.precision);
+  // point probes of the point-in-polygon refine (and of dist_pm) are
+  // ordered by the Morton keys of their point rects, computed from the
+  // points (the rects are made below only where a filter reads them)
+  const bool point_keys = o.sort_probes && probe.pt_valid >= 0 && (pip_kind || dist_pm);
   // probe rects: GPU operator semantics (points nearest) for relation
   // predicates; outward + distance expansion for DWithin
   auto make_rects = [&] {
@@ -779,23 +792,15 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
   // Morton order: candidates then name positions in that order, so that the
   // points of one polygon's candidates lie together in memory.  The refine
   // maps kept positions back to probe rows.
-  const bool ranked = ord != nullptr && !dwithin && pip_refine_applies(rf->build, rf->ring_index, probe, o.predicate);
+  const bool ranked = ord != nullptr && pip_kind;
   GeomColumn rprobe;
-  if (ranked) gather_points(probe, ord, nq, rprobe, s);
+  if (ranked || (dist_pm && ord)) gather_points(probe, ord, nq, rprobe, s);
   const GeomColumn& rp = ranked ? rprobe : probe;
   mark(2);
   std::unique_ptr<JoinResultImpl> jr(new JoinResultImpl());
   jr->device_id = rf->rt->device_id;
   RefineStats st;
   uint64_t kept = 0, nc = 0;
-  // refine order: the caller's, or (refine_order < 0) the tuned one, else
-  // the default of the refine kind
-  const int kind = dwithin ? 1 : ranked ? 0 : 2;
-  int order_used = o.refine_order;
-  if (order_used < 0) order_used = ix->tuning.refine_order[kind];
-  // default: polygon-major; DWithin in filter order unless the build side
-  // has line rasters (whose split leaves few enough candidates to sort)
-  if (order_used < 0) order_used = dwithin && rf->ring_index.n_line_rasters == 0 ? 0 : 1;
   // refine_order 2 (DWithin over point probes): filter and distance refine
   // in one kernel, candidates never written out
   const bool fused = dwithin && probe.points_only && o.precision != 1 && order_used == 2;
@@ -812,6 +817,7 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
     Workspace* ws = lease.ws.get();
     uint64_t n_open = 0;
     bool split = false, grouped = false;
+    PmSplitResult pr;
     // point in polygon over Morton-ordered points, polygon-major order:
     // each build rect tests the points of the Morton cells it covers, so
     // the candidates come out grouped by build row, and the cell-raster
@@ -819,7 +825,6 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
     if (order_used == 1 && ranked && ix->tree.filter_mode == 0) {
       const RingIndex& ri = rf->ring_index;
       const bool pm_raster = ri.n_rasters > 0 && o.precision == 0 && !rf->build.points_only && ri.ring_rect.p;
-      PmSplitResult pr;
       if (ev.on) ft.record_start(s);
       grouped = pm_split(ix->tree, ri, rprobe.xy.p, mkeys.p, nq, frame, ord, pm_raster, ws->out,
                          pm_raster ? ws->open : ws->cand, ws->pm, &pr, s);
@@ -827,6 +832,18 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
       if (grouped) {
         nc = pr.candidates;
         split = pm_raster;
+        n_open = pr.open;
+      }
+    } else if (dist_pm && nq > 0) {
+      // the line rasters' drops decided in the pass; the open candidates
+      // (build row, Morton position) grouped by line
+      if (ev.on) ft.record_start(s);
+      grouped = pm_split(ix->tree, rf->ring_index, rprobe.xy.p, mkeys.p, nq, frame, ord, false, ws->out, ws->open,
+                         ws->pm, &pr, s, o.distance);
+      if (ev.on) ft.record_stop(s);
+      if (grouped) {
+        nc = pr.candidates;
+        split = true;
         n_open = pr.open;
       }
     }
@@ -851,8 +868,9 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
     // polygon-major order: one wavefront then walks few distinct rings
     if (order_used == 1 && !grouped) sort_pairs_by_build(cand, n_cand, rf->build.n_rows, s);
     if (dwithin)
-      kept = refine_distance(rf->build, rf->ring_index, probe, o.distance, cand.b.p, cand.p.p, n_cand, ws->out, s,
-                             ev.on ? &rt : nullptr, o.precision, &st, ws, split);
+      kept = refine_distance(rf->build, rf->ring_index, grouped ? rprobe : probe, o.distance, cand.b.p, cand.p.p,
+                             n_cand, ws->out, s, ev.on ? &rt : nullptr, o.precision, &st, ws, split,
+                             grouped ? ord : nullptr);
     else
       kept = refine_relation(rf->build, rf->ring_index, rp, o.predicate, cand.b.p, cand.p.p, n_cand, o.precision,
                              ws->out, &st, s, ev.on ? &rt : nullptr, ranked ? ord : nullptr, ix->tuning.pip_waves,

@@ -354,4 +354,31 @@ SJ_HD int segment_within_f32(float ax, float ay, float bx, float by, float pxl, 
   return -1;
 }
 
+// ---------------------------------------------------------------------------
+// Rects of geometries (a1/a2).
+
+// f64 interval -> f32, outward (rust/sedona-spatial-join/src/utils/bounds.rs:147-162)
+SJ_HD float2 outward(double lo, double hi) {
+  float l = (float)lo, h = (float)hi;
+  if ((double)l > lo) l = nextafterf(l, -INFINITY);
+  if ((double)h < hi) h = nextafterf(h, INFINITY);
+  return make_float2(l, h);
+}
+
+SJ_HD float4 finish_rect(double xl, double yl, double xh, double yh, bool nearest_point, double expand) {
+  float4 r;
+  if (nearest_point) {
+    r = make_float4((float)xl, (float)yl, (float)xh, (float)yh);
+  } else {
+    float2 x = outward(xl, xh), y = outward(yl, yh);
+    r = make_float4(x.x, y.x, x.y, y.y);
+  }
+  if (expand == expand && expand >= 0.0) {  // interval.rs:269-275
+    float2 x = outward((double)r.x - expand, (double)r.z + expand);
+    float2 y = outward((double)r.y - expand, (double)r.w + expand);
+    r = make_float4(x.x, y.x, x.y, y.y);
+  }
+  return r;
+}
+
 }  // namespace sjhip

@@ -51,6 +51,7 @@ struct PmGrid {
   float x0, y0, sx, sy;  // Morton frame of the probe keys
   int s;                 // cell = 2^s x 2^s Morton units
   int L;                 // cells per axis = 2^L (L = 16 - s)
+  double dist;           // ST_DWithin distance (< 0 or NaN: a relation join)
 };
 
 __device__ __forceinline__ uint32_t pm_spread16(uint32_t x) {
@@ -88,13 +89,29 @@ __global__ void k_pm_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
   if (i < n) p[i] = v;
 }
 
-// The cell range a build rect covers.
+// The cell range that holds every point whose probe rect can meet build
+// rect r.  For ST_DWithin the probe rect is the point's outward FP32 box
+// widened by dist and rounded outward again (finish_rect), which meets r
+// only when the point lies within dist plus a few FP32 ulps of r: r is
+// widened by dist + 2^-20 (|r| + dist), and the bounds rounded outward to
+// FP32 hold every such point rounded to nearest, which is what its key is
+// made from.
 __device__ __forceinline__ void pm_range(const PmGrid& g, const float4& r, uint32_t& cx0, uint32_t& cx1,
                                          uint32_t& cy0, uint32_t& cy1) {
-  cx0 = pm_q(r.x, g.x0, g.sx) >> g.s;
-  cx1 = pm_q(r.z, g.x0, g.sx) >> g.s;
-  cy0 = pm_q(r.y, g.y0, g.sy) >> g.s;
-  cy1 = pm_q(r.w, g.y0, g.sy) >> g.s;
+  float x0 = r.x, x1 = r.z, y0 = r.y, y1 = r.w;
+  if (g.dist >= 0.0) {
+    const double mx = g.dist + 0x1p-20 * (fmax(fabs((double)r.x), fabs((double)r.z)) + g.dist) + 1e-30;
+    const double my = g.dist + 0x1p-20 * (fmax(fabs((double)r.y), fabs((double)r.w)) + g.dist) + 1e-30;
+    const float2 bx = outward((double)r.x - mx, (double)r.z + mx), by = outward((double)r.y - my, (double)r.w + my);
+    x0 = bx.x;
+    x1 = bx.y;
+    y0 = by.x;
+    y1 = by.y;
+  }
+  cx0 = pm_q(x0, g.x0, g.sx) >> g.s;
+  cx1 = pm_q(x1, g.x0, g.sx) >> g.s;
+  cy0 = pm_q(y0, g.y0, g.sy) >> g.s;
+  cy1 = pm_q(y1, g.y0, g.sy) >> g.s;
 }
 
 // cnt[i] = 64-point steps over the cells of leaf rect i (cnt[n] = 0), one
@@ -191,16 +208,18 @@ __global__ __launch_bounds__(kPmBlock) void k_pm_fill(PmGrid g, const float4* __
 }
 
 // Per step (64 per wavefront): the hit lanes split into kept / open masks
-// (RASTER), or all hits open; packed counts (kept << 32 | open) for the
-// scan; the wavefront's hits in whits.
-template <bool RASTER>
+// (MODE 1: point in polygon with cell rasters), all hits open (MODE 0), or
+// (MODE 2: ST_DWithin over single lines) the point's widened rect tested
+// and the hits the line raster rules out dropped; packed counts
+// (kept << 32 | open) for the scan; the wavefront's hits in whits.
+template <int MODE>
 __global__ __launch_bounds__(kPmBlock) void k_pm_test(const float4* __restrict__ rects,
                                                       const uint32_t* __restrict__ ids, RingIndexView R,
                                                       const double2* __restrict__ xy, const uint32_t* __restrict__ rec_i,
                                                       const uint32_t* __restrict__ rec_s,
                                                       const uint8_t* __restrict__ rec_n, uint64_t S,
                                                       ulonglong2* __restrict__ masks, uint64_t* __restrict__ cnt2,
-                                                      uint64_t* __restrict__ whits) {
+                                                      uint64_t* __restrict__ whits, double dist) {
   const int lane = threadIdx.x & 63;
   const uint64_t w = ((uint64_t)blockIdx.x * kPmBlock + threadIdx.x) >> 6;
   const uint64_t k0 = w * 64;
@@ -211,7 +230,7 @@ __global__ __launch_bounds__(kPmBlock) void k_pm_test(const float4* __restrict__
   const int steps = (int)min<uint64_t>(64, S - k0);
   uint64_t my_mk = 0, my_mo = 0;
   uint32_t hits = 0;
-  uint32_t cur = 0xffffffffu;
+  uint32_t cur = 0xffffffffu, bid = 0;
   float4 r = make_float4(0, 0, 0, 0);
   PolyDesc d{};
   // the next step's point is loaded before this step is tested
@@ -224,17 +243,27 @@ __global__ __launch_bounds__(kPmBlock) void k_pm_test(const float4* __restrict__
     if (i != cur) {
       cur = i;
       r = rects[i];
-      if (RASTER) d = R.desc[ids[i]];
+      bid = ids[i];
+      if (MODE == 1) d = R.desc[bid];
     }
     const bool in = (uint32_t)lane < n;
-    const float fx = (float)pt.x, fy = (float)pt.y;
-    const bool h = in && !(fx > r.z || fx < r.x || fy > r.w || fy < r.y);
+    bool h;
+    if (MODE == 2) {
+      // the probe rect of the ST_DWithin filter (compute_rects with the
+      // distance) against the build rect
+      const float4 q = finish_rect(pt.x, pt.y, pt.x, pt.y, false, dist);
+      h = in && !(q.x > r.z || q.z < r.x || q.y > r.w || q.w < r.y);
+    } else {
+      const float fx = (float)pt.x, fy = (float)pt.y;
+      h = in && !(fx > r.z || fx < r.x || fy > r.w || fy < r.y);
+    }
     bool keep = false, open = h;
-    if (RASTER && h) {
+    if (MODE == 1 && h) {
       const int cls = raster_class(R, d, pt);
       keep = cls == 1;
       open = cls == 0;
     }
+    if (MODE == 2 && h) open = line_raster_may_hold(R, bid, pt, dist);
     const uint64_t mk = __ballot(keep), mo = __ballot(open);
     hits += (uint32_t)__popcll(__ballot(h));
     if (lane == j) {
@@ -337,11 +366,14 @@ __global__ void k_pm_set_u64(unsigned long long* p, unsigned long long v) { *p =
 
 bool pm_split(const RectTree& t, const RingIndex& ri, const double2* xy, const uint32_t* keys, int64_t nq,
               const MortonFrame& f, const uint32_t* probe_ids, bool raster, PairBuf& out, PairBuf& open,
-              PmScratch& sc, PmSplitResult* res, hipStream_t s) {
+              PmScratch& sc, PmSplitResult* res, hipStream_t s, double dist) {
   *res = PmSplitResult{};
   if (t.grid_nx <= 0 || t.n_valid <= 0 || nq <= 0 || nq >= (int64_t)0xffffffffll || !(t.grid_inv_x > 0.f))
     return false;
+  const bool dwithin = dist >= 0.0 && dist < INFINITY;
+  if (dwithin && (raster || ri.desc.p == nullptr || ri.n_line_rasters == 0)) return false;
   if (raster && (ri.desc.p == nullptr || ri.n_rasters == 0)) return false;
+  const int mode = dwithin ? 2 : raster ? 1 : 0;
   // cells about half the median build rect wide: the rect grid's cells are
   // the median extent (coarser where the rects are large)
   const double cell_world = 1.0 / (double)t.grid_inv_x;
@@ -352,7 +384,7 @@ bool pm_split(const RectTree& t, const RingIndex& ri, const double2* xy, const u
     sh = units <= 1.0 ? 0 : (int)std::lround(std::log2(units));
   }
   sh = std::min(15, std::max(4, sh));  // 4^(16 - sh) + 1 cell starts: at most 2^24 + 1
-  PmGrid g{f.x0, f.y0, f.sx, f.sy, sh, 16 - sh};
+  PmGrid g{f.x0, f.y0, f.sx, f.sy, sh, 16 - sh, dwithin ? dist : -1.0};
   const int64_t ncell = (int64_t)1 << (2 * g.L);
   const int64_t n = t.n_valid;
   fit(sc.start, (size_t)(ncell + 1), s);
@@ -410,12 +442,13 @@ bool pm_split(const RectTree& t, const RingIndex& ri, const double2* xy, const u
                      sc.start.p, sc.off.p, sc.rec_i.p, sc.rec_s.p, sc.rec_n.p);
   const unsigned bs = (unsigned)((S + kPmBlock - 1) / kPmBlock);
   const RingIndexView R = ri.view();
-  if (raster)
-    hipLaunchKernelGGL(k_pm_test<true>, bs, kPmBlock, 0, s, t.leaf_rects.p, t.leaf_ids.p, R, xy, sc.rec_i.p,
-                       sc.rec_s.p, sc.rec_n.p, S, sc.masks.p, sc.cnt2.p, sc.whits.p);
-  else
-    hipLaunchKernelGGL(k_pm_test<false>, bs, kPmBlock, 0, s, t.leaf_rects.p, t.leaf_ids.p, R, xy, sc.rec_i.p,
-                       sc.rec_s.p, sc.rec_n.p, S, sc.masks.p, sc.cnt2.p, sc.whits.p);
+  auto test = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, bs, kPmBlock, 0, s, t.leaf_rects.p, t.leaf_ids.p, R, xy, sc.rec_i.p, sc.rec_s.p,
+                       sc.rec_n.p, S, sc.masks.p, sc.cnt2.p, sc.whits.p, dist);
+  };
+  if (mode == 1) test(k_pm_test<1>);
+  else if (mode == 2) test(k_pm_test<2>);
+  else test(k_pm_test<0>);
   SJ_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_pm_set_u64, 1, 1, 0, s, (unsigned long long*)(sc.cnt2.p + S), 0ull);
   {

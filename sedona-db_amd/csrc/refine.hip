@@ -130,12 +130,15 @@ __device__ bool any_poly_contains(const GeomView& g, int64_t part, double px, do
 }
 
 __device__ __forceinline__ bool point_of(const GeomView& g, int64_t row, double& x, double& y) {
-  if (g.type[row] != kPoint) return false;
   if (g.points_only) {
+    // (a point column gathered into another order keeps no type array: its
+    // rows are valid points or NaN)
+    if (g.type && g.type[row] != kPoint) return false;
     double2 p = g.xy[row];
     x = p.x; y = p.y;
     return p.x == p.x || p.y == p.y;
   }
+  if (g.type[row] != kPoint) return false;
   if (g.part_off[row] == g.part_off[row + 1]) return false;
   double2 p = g.xy[g.pt_off[g.ring_off[g.part_off[row]]]];
   x = p.x; y = p.y;
@@ -1362,32 +1365,7 @@ __global__ __launch_bounds__(kBlock) void k_open_scatter(const uint64_t* mask, c
 // or the square misses the line's rect, or the point has no coordinates.
 // Rows without a raster answer true.
 __device__ __forceinline__ bool line_may_be_within(const Ctx& c, uint32_t b, double2 pt, double dist) {
-  if (!(pt.x == pt.x || pt.y == pt.y)) return false;
-  const PolyDesc d = c.R.desc[b];
-  if (d.raster_off < 0) return true;
-  const double cw = (d.rect.z - d.rect.x) / kLineCells, ch = (d.rect.w - d.rect.y) / kLineCells;
-  const double fx0 = floor((pt.x - dist - d.rect.x) / cw) - 1.0, fx1 = floor((pt.x + dist - d.rect.x) / cw) + 1.0;
-  const double fy0 = floor((pt.y - dist - d.rect.y) / ch) - 1.0, fy1 = floor((pt.y + dist - d.rect.y) / ch) + 1.0;
-  if (fx1 < 0.0 || fy1 < 0.0 || fx0 > kLineCells - 1.0 || fy0 > kLineCells - 1.0) return false;
-  if (!(fy1 - fy0 < 24.0)) return true;  // a window of many rows: left to the walk
-  const int i0 = (int)fmax(fx0, 0.0), i1 = (int)fmin(fx1, kLineCells - 1.0);
-  const int j0 = (int)fmax(fy0, 0.0), j1 = (int)fmin(fy1, kLineCells - 1.0);
-  if (i0 > i1 || j0 > j1) return true;
-  // bits i0..i1 of a row, split over its words
-  uint64_t m[kLineRowWords];
-#pragma unroll
-  for (int w = 0; w < kLineRowWords; w++) {
-    const int lo = max(i0 - 64 * w, 0), hi = min(i1 - 64 * w, 63);
-    m[w] = lo <= hi ? (~0ull >> (63 - (hi - lo))) << lo : 0ull;
-  }
-  for (int r = j0; r <= j1; r++) {
-    const uint64_t* rw = c.R.line_raster + d.raster_off + (int64_t)r * kLineRowWords;
-    bool any = false;
-#pragma unroll
-    for (int w = 0; w < kLineRowWords; w++) any |= (rw[w] & m[w]) != 0;
-    if (any) return true;
-  }
-  return false;
+  return line_raster_may_hold(c.R, b, pt, dist);
 }
 
 // Cell-raster pass of the distance refine over single-line build rows and
@@ -1707,7 +1685,8 @@ __global__ __launch_bounds__(kBlock) void k_refine_distance(Ctx c, double dist, 
                                                             const uint32_t* cp, uint64_t n, const uint32_t* list,
                                                             uint32_t* ob, uint32_t* op, uint64_t cap,
                                                             unsigned long long* counter, unsigned long long* stats,
-                                                            const unsigned long long* n_dev) {
+                                                            const unsigned long long* n_dev,
+                                                            const uint32_t* probe_ids) {
   const GeomView& B = c.B;
   const GeomView& P = c.P;
   SJ_WAVE_OUT(wo);
@@ -1727,6 +1706,7 @@ __global__ __launch_bounds__(kBlock) void k_refine_distance(Ctx c, double dist, 
       if (bok && point_of(P, p, px, py))
         keep = plain ? distance_to_point(B, b, px, py) <= dist
                      : within_distance<KIND>(c, b, px, py, dist, tests, exact_scans, fb);
+      if (keep && probe_ids) p = probe_ids[p];
     }
     emit(wo, keep, b, p, ob, op, cap, counter);
   }
@@ -2247,6 +2227,12 @@ bool raster_split(const GeomColumn& build, const RingIndex& ri, const GeomColumn
   return true;
 }
 
+bool line_rasters_apply(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
+                        int precision) {
+  return precision != 1 && dist_kind(build.type_mask, build.points_only, ri) == kDistLines && ri.n_line_rasters > 0 &&
+         probe.points_only && d >= 0 && d < INFINITY;
+}
+
 bool raster_split_distance(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
                            int precision, const uint32_t* cb, const uint32_t* cp, uint64_t n, PairBuf& open,
                            DevArray<uint64_t>& mask, uint64_t* n_open, hipStream_t s) {
@@ -2462,7 +2448,8 @@ void pair_distances(const GeomColumn& build, const GeomColumn& probe, const uint
 
 uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
                          const uint32_t* cb, const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s,
-                         KernelTimer* timer, int precision, RefineStats* stats, Workspace* ws, bool raster_done) {
+                         KernelTimer* timer, int precision, RefineStats* stats, Workspace* ws, bool raster_done,
+                         const uint32_t* probe_ids) {
   out.counter.reserve(1, s);
   fit(out.b, n > 0 ? n : 1, s);
   fit(out.p, n > 0 ? n : 1, s);
@@ -2500,15 +2487,15 @@ uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const Geo
   switch (kind) {
     case kDistLines:
       hipLaunchKernelGGL(k_refine_distance<kDistLines>, grid_for(n), kBlock, 0, s, c, d, 0, wb, wp, n, nullptr,
-                         out.b.p, out.p.p, out.cap(), out.counter.p, aux.p, n_open);
+                         out.b.p, out.p.p, out.cap(), out.counter.p, aux.p, n_open, probe_ids);
       break;
     case kDistPolygons:
       hipLaunchKernelGGL(k_refine_distance<kDistPolygons>, grid_for(n), kBlock, 0, s, c, d, 0, wb, wp, n, nullptr,
-                         out.b.p, out.p.p, out.cap(), out.counter.p, aux.p, n_open);
+                         out.b.p, out.p.p, out.cap(), out.counter.p, aux.p, n_open, probe_ids);
       break;
     default:
       hipLaunchKernelGGL(k_refine_distance<kDistAny>, grid_for(n), kBlock, 0, s, c, d, plain, cb, cp, n,
-                         nullptr, out.b.p, out.p.p, out.cap(), out.counter.p, aux.p, nullptr);
+                         nullptr, out.b.p, out.p.p, out.cap(), out.counter.p, aux.p, nullptr, probe_ids);
   }
   SJ_CHECK(hipGetLastError());
   if (timer) timer->record_stop(s);
@@ -2557,15 +2544,15 @@ void launch_distance_refine_dev(const GeomColumn& build, const RingIndex& ri, co
   switch (dist_kind(build.type_mask, build.points_only, ri)) {
     case kDistLines:
       hipLaunchKernelGGL(k_refine_distance<kDistLines>, grid_for(n_cap), kBlock, 0, s, c, d, 0, cb, cp, n_cap, nullptr, ob,
-                         op, out_cap, out_n, stats, n_dev);
+                         op, out_cap, out_n, stats, n_dev, nullptr);
       break;
     case kDistPolygons:
       hipLaunchKernelGGL(k_refine_distance<kDistPolygons>, grid_for(n_cap), kBlock, 0, s, c, d, 0, cb, cp, n_cap, nullptr,
-                         ob, op, out_cap, out_n, stats, n_dev);
+                         ob, op, out_cap, out_n, stats, n_dev, nullptr);
       break;
     default:
       hipLaunchKernelGGL(k_refine_distance<kDistAny>, grid_for(n_cap), kBlock, 0, s, c, d, 0, cb, cp, n_cap, nullptr, ob,
-                         op, out_cap, out_n, stats, n_dev);
+                         op, out_cap, out_n, stats, n_dev, nullptr);
   }
   SJ_CHECK(hipGetLastError());
 }

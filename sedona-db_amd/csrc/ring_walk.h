@@ -439,6 +439,39 @@ __device__ __forceinline__ int raster_class(const RingIndexView& R, const PolyDe
   return (R.raster[d.raster_off + (cell >> 4)] >> (2 * (cell & 15))) & 3;
 }
 
+// Whether a line's cell raster leaves a segment possibly within dist of the
+// point: some cell of the square [pt - dist, pt + dist] (widened by one
+// cell) is set.  False also when the square misses the line's rect or the
+// point has no coordinates; rows without a raster answer true.
+__device__ __forceinline__ bool line_raster_may_hold(const RingIndexView& R, uint32_t b, double2 pt, double dist) {
+  if (!(pt.x == pt.x || pt.y == pt.y)) return false;
+  const PolyDesc d = R.desc[b];
+  if (d.raster_off < 0) return true;
+  const double cw = (d.rect.z - d.rect.x) / kLineCells, ch = (d.rect.w - d.rect.y) / kLineCells;
+  const double fx0 = floor((pt.x - dist - d.rect.x) / cw) - 1.0, fx1 = floor((pt.x + dist - d.rect.x) / cw) + 1.0;
+  const double fy0 = floor((pt.y - dist - d.rect.y) / ch) - 1.0, fy1 = floor((pt.y + dist - d.rect.y) / ch) + 1.0;
+  if (fx1 < 0.0 || fy1 < 0.0 || fx0 > kLineCells - 1.0 || fy0 > kLineCells - 1.0) return false;
+  if (!(fy1 - fy0 < 24.0)) return true;  // a window of many rows: left to the walk
+  const int i0 = (int)fmax(fx0, 0.0), i1 = (int)fmin(fx1, kLineCells - 1.0);
+  const int j0 = (int)fmax(fy0, 0.0), j1 = (int)fmin(fy1, kLineCells - 1.0);
+  if (i0 > i1 || j0 > j1) return true;
+  // bits i0..i1 of a row, split over its words
+  uint64_t m[kLineRowWords];
+#pragma unroll
+  for (int w = 0; w < kLineRowWords; w++) {
+    const int lo = max(i0 - 64 * w, 0), hi = min(i1 - 64 * w, 63);
+    m[w] = lo <= hi ? (~0ull >> (63 - (hi - lo))) << lo : 0ull;
+  }
+  for (int r = j0; r <= j1; r++) {
+    const uint64_t* rw = R.line_raster + d.raster_off + (int64_t)r * kLineRowWords;
+    bool any = false;
+#pragma unroll
+    for (int w = 0; w < kLineRowWords; w++) any |= (rw[w] & m[w]) != 0;
+    if (any) return true;
+  }
+  return false;
+}
+
 // The FP32 frame origin of a ring: the centre of its rect (the record of a
 // single-ring row stores the rect, so the origin is recomputed from it).
 __device__ __forceinline__ double2 ring_origin(const double4& rb) {

@@ -463,12 +463,15 @@ struct PmSplitResult {
 // present) the candidates in clean raster cells are decided: kept pairs
 // (build, probe row) go to out (out.counter = their number) and the rest
 // to open as (build, position); without, every candidate goes to open.
-// Returns false, having written nothing the caller reads, where it does
-// not apply (no cell grid, a rect over too many cells); the caller then
-// takes the probe-major filter.
+// With dist >= 0 (ST_DWithin; build rows single lines with line rasters,
+// raster false) each point's probe rect is its outward box widened by
+// dist, and the hits the line rasters rule out are dropped; the rest go to
+// open.  Returns false, having written nothing the caller reads, where it
+// does not apply (no cell grid, a rect over too many cells); the caller
+// then takes the probe-major filter.
 bool pm_split(const RectTree& t, const RingIndex& ri, const double2* xy, const uint32_t* keys, int64_t nq,
               const MortonFrame& f, const uint32_t* probe_ids, bool raster, PairBuf& out, PairBuf& open,
-              PmScratch& sc, PmSplitResult* res, hipStream_t s);
+              PmScratch& sc, PmSplitResult* res, hipStream_t s, double dist = -1.0);
 
 // Refine statistics
 struct RefineStats {
@@ -509,6 +512,11 @@ bool raster_split(const GeomColumn& build, const RingIndex& ri, const GeomColumn
 // out are copied, in candidate order, to open, for
 // refine_distance(..., raster_done = true).  Returns false, doing nothing,
 // where the line rasters do not apply.
+// Whether the line rasters of the distance refine apply: build rows single
+// lines with rasters, point probes, a finite distance >= 0, not the
+// all-FP64 form.
+bool line_rasters_apply(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
+                        int precision);
 bool raster_split_distance(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
                            int precision, const uint32_t* cb, const uint32_t* cp, uint64_t n, PairBuf& open,
                            DevArray<uint64_t>& mask, uint64_t* n_open, hipStream_t s);
@@ -519,10 +527,12 @@ bool raster_split_distance(const GeomColumn& build, const RingIndex& ri, const G
 // FP64 reference distance (build row cb[i], probe point cp[i]) into out[i].
 void pair_distances(const GeomColumn& build, const GeomColumn& probe, const uint32_t* cb, const uint32_t* cp,
                     uint64_t n, double* out, hipStream_t s);
+// probe_ids (optional): cp holds positions in probe, and the kept pairs
+// carry probe_ids[position].
 uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
                          const uint32_t* cb, const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s,
                          KernelTimer* timer = nullptr, int precision = 0, RefineStats* stats = nullptr,
-                         Workspace* ws = nullptr, bool raster_done = false);
+                         Workspace* ws = nullptr, bool raster_done = false, const uint32_t* probe_ids = nullptr);
 
 // ST_DWithin join of point probes in one kernel: the filter walk decides
 // the distance predicate for each rect hit, so only kept pairs are

@@ -161,3 +161,57 @@ def test_every_candidate_decided_by_the_rasters(mode, opts):
         assert len(ob) == 500
         assert r.res.refine_candidates == 0
         assert np.array_equal(gb, ob) and np.array_equal(gp, op)
+
+
+def _walk_probes(d):
+    """Random-walk lines; probes: points at the lines' FP32 rect bounds
+    widened by d (and one FP64 ulp either side), nulls, empty points and
+    uniform points."""
+    b, p = D.workload_dwithin_walks(10_000, 1_000, seed=33)
+    assert b.data[0] == 1 and int.from_bytes(bytes(b.data[1:5]), "little") == 2  # little-endian linestrings
+    r = G.compute_rects(b.to_arrow()).astype(np.float64)
+    rng = np.random.default_rng(34)
+    xs, ys = [], []
+    for dx in (-np.inf, 0.0, np.inf):
+        for side_x, side_y in ((r[:, 0] - d, 0.5 * (r[:, 1] + r[:, 3])), (r[:, 2] + d, 0.5 * (r[:, 1] + r[:, 3])),
+                               (0.5 * (r[:, 0] + r[:, 2]), r[:, 1] - d), (0.5 * (r[:, 0] + r[:, 2]), r[:, 3] + d)):
+            xs.append(side_x if dx == 0.0 else np.nextafter(side_x, dx))
+            ys.append(side_y)
+    xy = p.data.reshape(-1, 21)[:, 5:].copy().view(np.float64).reshape(-1, 2)
+    xs.append(xy[:, 0])
+    ys.append(xy[:, 1])
+    # the vertices of the first lines: at distance 0 from them
+    for i in range(200):
+        v = b.data[b.offsets[i] + 9:b.offsets[i + 1]].copy().view(np.float64).reshape(-1, 2)
+        xs.append(v[:, 0])
+        ys.append(v[:, 1])
+    x, y = np.concatenate(xs), np.concatenate(ys)
+    x[::89] = np.nan
+    y[::89] = np.nan
+    valid = np.ones(len(x), bool)
+    valid[3::97] = False
+    perm = rng.permutation(len(x))
+    return b, _points(x[perm], y[perm], valid)
+
+
+@pytest.mark.parametrize("d", [0.0, 0.05, 0.7])
+def test_dwithin_build_rect_pass_equals_probe_major_and_oracle(d, opts):
+    """ST_DWithin over single lines with line rasters: the build-rect pass
+    (each line's rect widened by d visits the Morton cells of the points,
+    the line rasters' drops decided in the pass) against the tree filter,
+    the raster split and the grouping sort, and against the oracle."""
+    b, p = _walk_probes(d)
+    ix, rf = _pair(b.to_arrow(), opts)
+    say("dwithin build-rect pass", d, "lines", len(b), "probes", len(p))
+    ix.set_filter_mode(0)
+    r0 = G.join_device(ix, rf, p.to_arrow(), "dwithin", distance=d)
+    ix.set_filter_mode(3)
+    r3 = G.join_device(ix, rf, p.to_arrow(), "dwithin", distance=d)
+    ix.set_filter_mode(0)
+    b0, p0 = r0.to_host()
+    b3, p3 = r3.to_host()
+    ob, op = O.join(O.Store(b.as_tuple()), O.Store(p.as_tuple()), "dwithin", distance=d, nthreads=8)
+    assert r0.candidates == r3.candidates, (r0.candidates, r3.candidates)
+    assert np.array_equal(b0, ob) and np.array_equal(p0, op), (len(b0), len(ob))
+    assert np.array_equal(b3, ob) and np.array_equal(p3, op), (len(b3), len(ob))
+    assert len(ob) > 1000
