@@ -760,14 +760,18 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
   const int kind = dwithin ? 1 : pip_kind && o.sort_probes ? 0 : 2;
   int order_used = o.refine_order;
   if (order_used < 0) order_used = ix->tuning.refine_order[kind];
-  // default: polygon-major; DWithin in filter order unless the build side
-  // has line rasters (whose split leaves few enough candidates to sort)
-  if (order_used < 0) order_used = dwithin && rf->ring_index.n_line_rasters == 0 ? 0 : 1;
-  // ST_DWithin of point probes against single lines with line rasters, in
-  // line-major order: the build-rect pass, over the points in Morton order
-  const bool dist_pm = dwithin && o.sort_probes && order_used == 1 && probe.pt_valid >= 0 &&
-                       ix->tree.filter_mode == 0 &&
-                       line_rasters_apply(rf->build, rf->ring_index, probe, o.distance, o.precision);
+  // ST_DWithin of Morton-ordered point probes in build-row-major order:
+  // the build-rect pass (with the build side's rasters, if any)
+  const bool dist_pm_ok = dwithin && o.sort_probes && probe.pt_valid >= 0 && probe.points_only &&
+                          ix->tree.filter_mode == 0 && o.precision != 1 && o.distance >= 0 &&
+                          o.distance < INFINITY && !rf->build.points_only && rf->ring_index.desc.p != nullptr;
+  // default: build-row-major; DWithin in filter order when neither the
+  // build-rect pass nor line rasters (whose split leaves few enough
+  // candidates to sort) apply
+  if (order_used < 0) order_used = dwithin && !dist_pm_ok && rf->ring_index.n_line_rasters == 0 ? 0 : 1;
+  const bool dist_pm = dist_pm_ok && order_used == 1;
+  const int dist_raster =
+      dist_pm ? dist_raster_kind(rf->build, rf->ring_index, probe, o.distance, o.precision) : 0;
   // point probes of the point-in-polygon refine (and of dist_pm) are
   // ordered by the Morton keys of their point rects, computed from the
   // points (the rects are made below only where a filter reads them)
@@ -839,7 +843,7 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
       // (build row, Morton position) grouped by line
       if (ev.on) ft.record_start(s);
       grouped = pm_split(ix->tree, rf->ring_index, rprobe.xy.p, mkeys.p, nq, frame, ord, false, ws->out, ws->open,
-                         ws->pm, &pr, s, o.distance);
+                         ws->pm, &pr, s, o.distance, dist_raster);
       if (ev.on) ft.record_stop(s);
       if (grouped) {
         nc = pr.candidates;
@@ -869,8 +873,9 @@ void JoinEngine::run(IndexImpl* ix, RefinerImpl* rf, const WkbInput& in, const S
     if (order_used == 1 && !grouped) sort_pairs_by_build(cand, n_cand, rf->build.n_rows, s);
     if (dwithin)
       kept = refine_distance(rf->build, rf->ring_index, grouped ? rprobe : probe, o.distance, cand.b.p, cand.p.p,
-                             n_cand, ws->out, s, ev.on ? &rt : nullptr, o.precision, &st, ws, split,
-                             grouped ? ord : nullptr);
+                             n_cand, ws->out, s, ev.on ? &rt : nullptr, o.precision, &st, ws,
+                             split && (!grouped || dist_raster == 1), grouped ? ord : nullptr,
+                             grouped && dist_raster == 2);
     else
       kept = refine_relation(rf->build, rf->ring_index, rp, o.predicate, cand.b.p, cand.p.p, n_cand, o.precision,
                              ws->out, &st, s, ev.on ? &rt : nullptr, ranked ? ord : nullptr, ix->tuning.pip_waves,

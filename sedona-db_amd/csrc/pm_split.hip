@@ -209,9 +209,11 @@ __global__ __launch_bounds__(kPmBlock) void k_pm_fill(PmGrid g, const float4* __
 
 // Per step (64 per wavefront): the hit lanes split into kept / open masks
 // (MODE 1: point in polygon with cell rasters), all hits open (MODE 0), or
-// (MODE 2: ST_DWithin over single lines) the point's widened rect tested
-// and the hits the line raster rules out dropped; packed counts
-// (kept << 32 | open) for the scan; the wavefront's hits in whits.
+// (MODE 2: ST_DWithin) the point's widened rect tested, then by the build
+// rows' rasters (draster 1: line rasters drop the hits no segment can be
+// within dist of; 2: polygon cell rasters keep the hits in clean inside
+// cells, at distance 0; 0: none) ; packed counts (kept << 32 | open) for
+// the scan; the wavefront's hits in whits.
 template <int MODE>
 __global__ __launch_bounds__(kPmBlock) void k_pm_test(const float4* __restrict__ rects,
                                                       const uint32_t* __restrict__ ids, RingIndexView R,
@@ -219,7 +221,7 @@ __global__ __launch_bounds__(kPmBlock) void k_pm_test(const float4* __restrict__
                                                       const uint32_t* __restrict__ rec_s,
                                                       const uint8_t* __restrict__ rec_n, uint64_t S,
                                                       ulonglong2* __restrict__ masks, uint64_t* __restrict__ cnt2,
-                                                      uint64_t* __restrict__ whits, double dist) {
+                                                      uint64_t* __restrict__ whits, double dist, int draster) {
   const int lane = threadIdx.x & 63;
   const uint64_t w = ((uint64_t)blockIdx.x * kPmBlock + threadIdx.x) >> 6;
   const uint64_t k0 = w * 64;
@@ -244,7 +246,7 @@ __global__ __launch_bounds__(kPmBlock) void k_pm_test(const float4* __restrict__
       cur = i;
       r = rects[i];
       bid = ids[i];
-      if (MODE == 1) d = R.desc[bid];
+      if (MODE == 1 || (MODE == 2 && draster == 2)) d = R.desc[bid];
     }
     const bool in = (uint32_t)lane < n;
     bool h;
@@ -263,7 +265,11 @@ __global__ __launch_bounds__(kPmBlock) void k_pm_test(const float4* __restrict__
       keep = cls == 1;
       open = cls == 0;
     }
-    if (MODE == 2 && h) open = line_raster_may_hold(R, bid, pt, dist);
+    if (MODE == 2 && h && draster == 1) open = line_raster_may_hold(R, bid, pt, dist);
+    if (MODE == 2 && h && draster == 2) {
+      keep = raster_class(R, d, pt) == 1;
+      open = !keep;
+    }
     const uint64_t mk = __ballot(keep), mo = __ballot(open);
     hits += (uint32_t)__popcll(__ballot(h));
     if (lane == j) {
@@ -366,14 +372,18 @@ __global__ void k_pm_set_u64(unsigned long long* p, unsigned long long v) { *p =
 
 bool pm_split(const RectTree& t, const RingIndex& ri, const double2* xy, const uint32_t* keys, int64_t nq,
               const MortonFrame& f, const uint32_t* probe_ids, bool raster, PairBuf& out, PairBuf& open,
-              PmScratch& sc, PmSplitResult* res, hipStream_t s, double dist) {
+              PmScratch& sc, PmSplitResult* res, hipStream_t s, double dist, int dist_raster) {
   *res = PmSplitResult{};
   if (t.grid_nx <= 0 || t.n_valid <= 0 || nq <= 0 || nq >= (int64_t)0xffffffffll || !(t.grid_inv_x > 0.f))
     return false;
   const bool dwithin = dist >= 0.0 && dist < INFINITY;
-  if (dwithin && (raster || ri.desc.p == nullptr || ri.n_line_rasters == 0)) return false;
-  if (raster && (ri.desc.p == nullptr || ri.n_rasters == 0)) return false;
+  if (dwithin && dist_raster != 0 && ri.desc.p == nullptr) return false;
+  if (dwithin && dist_raster == 1 && ri.n_line_rasters == 0) return false;
+  if (dwithin && dist_raster == 2 && ri.n_rasters == 0) return false;
+  if (raster && (dwithin || ri.desc.p == nullptr || ri.n_rasters == 0)) return false;
   const int mode = dwithin ? 2 : raster ? 1 : 0;
+  // kept pairs: point in polygon with rasters, or ST_DWithin with polygon rasters
+  const bool kept_list = raster || (dwithin && dist_raster == 2);
   // cells about half the median build rect wide: the rect grid's cells are
   // the median extent (coarser where the rects are large)
   const double cell_world = 1.0 / (double)t.grid_inv_x;
@@ -444,7 +454,7 @@ bool pm_split(const RectTree& t, const RingIndex& ri, const double2* xy, const u
   const RingIndexView R = ri.view();
   auto test = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, bs, kPmBlock, 0, s, t.leaf_rects.p, t.leaf_ids.p, R, xy, sc.rec_i.p, sc.rec_s.p,
-                       sc.rec_n.p, S, sc.masks.p, sc.cnt2.p, sc.whits.p, dist);
+                       sc.rec_n.p, S, sc.masks.p, sc.cnt2.p, sc.whits.p, dist, dwithin ? dist_raster : 0);
   };
   if (mode == 1) test(k_pm_test<1>);
   else if (mode == 2) test(k_pm_test<2>);
@@ -470,7 +480,7 @@ bool pm_split(const RectTree& t, const RingIndex& ri, const double2* xy, const u
   res->kept = kept;
   res->open = n_open;
   // the walk appends its kept pairs after these (capacity: every candidate)
-  if (raster) {
+  if (kept_list) {
     out.counter.reserve(1, s);
     fit(out.b, hits > 0 ? hits : 1, s);
     fit(out.p, hits > 0 ? hits : 1, s);
@@ -478,7 +488,7 @@ bool pm_split(const RectTree& t, const RingIndex& ri, const double2* xy, const u
   }
   fit(open.b, n_open > 0 ? n_open : 1, s);
   fit(open.p, n_open > 0 ? n_open : 1, s);
-  if (raster)
+  if (kept_list)
     hipLaunchKernelGGL(k_pm_emit<true>, bs, kPmBlock, 0, s, t.leaf_ids.p, sc.rec_i.p, sc.rec_s.p, S, sc.masks.p,
                        sc.off2.p, probe_ids, out.b.p, out.p.p, open.b.p, open.p.p);
   else

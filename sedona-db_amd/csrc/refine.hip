@@ -2233,6 +2233,14 @@ bool line_rasters_apply(const GeomColumn& build, const RingIndex& ri, const Geom
          probe.points_only && d >= 0 && d < INFINITY;
 }
 
+int dist_raster_kind(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d, int precision) {
+  if (line_rasters_apply(build, ri, probe, d, precision)) return 1;
+  if (precision == 0 && dist_kind(build.type_mask, build.points_only, ri) == kDistPolygons && ri.n_rasters > 0 &&
+      probe.points_only && d >= 0 && d < INFINITY)
+    return 2;
+  return 0;
+}
+
 bool raster_split_distance(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
                            int precision, const uint32_t* cb, const uint32_t* cp, uint64_t n, PairBuf& open,
                            DevArray<uint64_t>& mask, uint64_t* n_open, hipStream_t s) {
@@ -2449,14 +2457,20 @@ void pair_distances(const GeomColumn& build, const GeomColumn& probe, const uint
 uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
                          const uint32_t* cb, const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s,
                          KernelTimer* timer, int precision, RefineStats* stats, Workspace* ws, bool raster_done,
-                         const uint32_t* probe_ids) {
+                         const uint32_t* probe_ids, bool out_holds_kept) {
   out.counter.reserve(1, s);
   fit(out.b, n > 0 ? n : 1, s);
   fit(out.p, n > 0 ? n : 1, s);
-  if (n == 0) return 0;
+  if (n == 0) {
+    if (!out_holds_kept) return 0;
+    unsigned long long cnt = 0;
+    SJ_CHECK(hipMemcpyAsync(&cnt, out.counter.p, 8, hipMemcpyDeviceToHost, s));
+    SJ_CHECK(hipStreamSynchronize(s));
+    return cnt;
+  }
   if (!probe.points_only && (probe.type_mask & ~(1u << kPoint)))
     throw Error("distance refine: probe rows must be points");
-  SJ_CHECK(hipMemsetAsync(out.counter.p, 0, 8, s));
+  if (!out_holds_kept) SJ_CHECK(hipMemsetAsync(out.counter.p, 0, 8, s));
   if (timer) timer->record_start(s);
   Ctx c;
   c.B = build.view();

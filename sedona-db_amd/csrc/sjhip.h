@@ -463,15 +463,17 @@ struct PmSplitResult {
 // present) the candidates in clean raster cells are decided: kept pairs
 // (build, probe row) go to out (out.counter = their number) and the rest
 // to open as (build, position); without, every candidate goes to open.
-// With dist >= 0 (ST_DWithin; build rows single lines with line rasters,
-// raster false) each point's probe rect is its outward box widened by
-// dist, and the hits the line rasters rule out are dropped; the rest go to
-// open.  Returns false, having written nothing the caller reads, where it
+// With dist >= 0 (ST_DWithin, raster false) each point's probe rect is its
+// outward box widened by dist; dist_raster 1 (single-line build rows with
+// line rasters) drops the hits the line rasters rule out, 2 (polygon build
+// rows with cell rasters) keeps the hits in clean inside cells (distance
+// 0) in out like the point-in-polygon split, 0 uses no raster; the rest go
+// to open.  Returns false, having written nothing the caller reads, where it
 // does not apply (no cell grid, a rect over too many cells); the caller
 // then takes the probe-major filter.
 bool pm_split(const RectTree& t, const RingIndex& ri, const double2* xy, const uint32_t* keys, int64_t nq,
               const MortonFrame& f, const uint32_t* probe_ids, bool raster, PairBuf& out, PairBuf& open,
-              PmScratch& sc, PmSplitResult* res, hipStream_t s, double dist = -1.0);
+              PmScratch& sc, PmSplitResult* res, hipStream_t s, double dist = -1.0, int dist_raster = 0);
 
 // Refine statistics
 struct RefineStats {
@@ -517,6 +519,10 @@ bool raster_split(const GeomColumn& build, const RingIndex& ri, const GeomColumn
 // all-FP64 form.
 bool line_rasters_apply(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
                         int precision);
+// The rasters the ST_DWithin build-rect pass uses for this build side: 1
+// line rasters (line_rasters_apply), 2 polygon cell rasters (build rows
+// (multi)polygons, the certified refine), 0 none.
+int dist_raster_kind(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d, int precision);
 bool raster_split_distance(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
                            int precision, const uint32_t* cb, const uint32_t* cp, uint64_t n, PairBuf& open,
                            DevArray<uint64_t>& mask, uint64_t* n_open, hipStream_t s);
@@ -528,11 +534,13 @@ bool raster_split_distance(const GeomColumn& build, const RingIndex& ri, const G
 void pair_distances(const GeomColumn& build, const GeomColumn& probe, const uint32_t* cb, const uint32_t* cp,
                     uint64_t n, double* out, hipStream_t s);
 // probe_ids (optional): cp holds positions in probe, and the kept pairs
-// carry probe_ids[position].
+// carry probe_ids[position].  out_holds_kept: out already holds kept pairs
+// (out.counter of them), which the refine appends to.
 uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const GeomColumn& probe, double d,
                          const uint32_t* cb, const uint32_t* cp, uint64_t n, PairBuf& out, hipStream_t s,
                          KernelTimer* timer = nullptr, int precision = 0, RefineStats* stats = nullptr,
-                         Workspace* ws = nullptr, bool raster_done = false, const uint32_t* probe_ids = nullptr);
+                         Workspace* ws = nullptr, bool raster_done = false, const uint32_t* probe_ids = nullptr,
+                         bool out_holds_kept = false);
 
 // ST_DWithin join of point probes in one kernel: the filter walk decides
 // the distance predicate for each rect hit, so only kept pairs are

@@ -215,3 +215,38 @@ def test_dwithin_build_rect_pass_equals_probe_major_and_oracle(d, opts):
     assert np.array_equal(b0, ob) and np.array_equal(p0, op), (len(b0), len(ob))
     assert np.array_equal(b3, ob) and np.array_equal(p3, op), (len(b3), len(ob))
     assert len(ob) > 1000
+
+
+@pytest.mark.parametrize("d", [0.0, 0.05, 0.7])
+def test_dwithin_polygons_build_rect_pass_equals_probe_major_and_oracle(d, opts):
+    """ST_DWithin over polygons (Zipf stars with cell rasters, plus rows with
+    holes and multipolygons): the build-rect pass keeps the points in clean
+    inside cells (distance 0) and walks the rest; against the tree filter in
+    filter order and against the oracle."""
+    b, p = D.workload_pip_zipf(60_000, 600, seed=35)
+    hb = C.column(C.hole_polygons())
+    off = np.concatenate([b.offsets, b.offsets[-1] + hb[0][1:]])
+    build = D.WkbColumn(off, np.concatenate([b.data, hb[1]]))
+    rng = np.random.default_rng(36)
+    xy = p.data.reshape(-1, 21)[:, 5:].copy().view(np.float64).reshape(-1, 2)
+    x = np.concatenate([xy[:, 0], rng.uniform(-1, 17, 20_000)])
+    y = np.concatenate([xy[:, 1], rng.uniform(-1, 17, 20_000)])
+    x[::83] = np.nan
+    y[::83] = np.nan
+    valid = np.ones(len(x), bool)
+    valid[7::101] = False
+    pts = _points(x, y, valid)
+    ix, rf = _pair(build.to_arrow(), opts)
+    say("dwithin polygons build-rect pass", d, "rows", len(build), "probes", len(pts))
+    ix.set_filter_mode(0)
+    r0 = G.join_device(ix, rf, pts.to_arrow(), "dwithin", distance=d)
+    ix.set_filter_mode(3)
+    r3 = G.join_device(ix, rf, pts.to_arrow(), "dwithin", distance=d, refine_order=0)
+    ix.set_filter_mode(0)
+    b0, p0 = r0.to_host()
+    b3, p3 = r3.to_host()
+    ob, op = O.join(O.Store(build.as_tuple()), O.Store(pts.as_tuple()), "dwithin", distance=d, nthreads=8)
+    assert r0.candidates == r3.candidates, (r0.candidates, r3.candidates)
+    assert np.array_equal(b0, ob) and np.array_equal(p0, op), (len(b0), len(ob))
+    assert np.array_equal(b3, ob) and np.array_equal(p3, op), (len(b3), len(ob))
+    assert len(ob) > 1000
