@@ -258,7 +258,224 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(WAVE
   }
 }
 
+// ---- ST_DWithin over single-line build rows ---------------------------------
+//
+// The decision dist(line, point) <= d of rust/sedona-spatial-join/src/
+// refine/geo.rs:312-335 (a minimum over the line's segments, so <= d exactly
+// when one segment term is) with certified FP32 point-segment terms
+// (segment_within_f32) only: a lane stops at its first segment that is
+// certainly within d; a candidate with no such segment but an uncertain one
+// is listed for the FP64 kernel, which decides it in the reference's
+// operation order.  Tree boxes, and segments of rows without a tree, are
+// pruned by the query square [p - D, p + D] with D = d (1 + 1e-4) + 4E,
+// the bound the FP64 kernel prunes with.
+
+// resident waves per SIMD of the line walk (at 6 it holds its state in 79
+// VGPRs without scratch; at 8 it spills)
+#ifndef SJ_LW_WAVES
+#define SJ_LW_WAVES 6
+#endif
+
+struct LineWalk {
+  const float4* box;  // the line's tree, level 0 first (nullptr: none)
+  const float2* q32;  // the line's FP32 line-local vertices
+  float px, py;       // the point, line-local FP32
+  float D;            // pruning half-width of the query square
+  float E, Ep;        // bounds of the vertices' and the point's FP32 rounding
+  int ns;
+  uint32_t st;        // bit 0: a segment certainly within d; bits 1..: uncertain tests
+};
+
+__device__ __forceinline__ bool line_box_hit(const LineWalk& w, const float4& b) {
+  return b.x <= w.px + w.D && b.z >= w.px - w.D && b.y <= w.py + w.D && b.w >= w.py - w.D;
+}
+
+__device__ __forceinline__ void line_segment_test(LineWalk& w, uint32_t& tests, float2 a, float2 b, double d) {
+  const float gx = fmaxf(fminf(a.x, b.x) - w.px, w.px - fmaxf(a.x, b.x));
+  const float gy = fmaxf(fminf(a.y, b.y) - w.py, w.py - fmaxf(a.y, b.y));
+  if (fmaxf(gx, gy) > w.D) return;
+  tests++;
+  const int r = segment_within_f32(a.x, a.y, b.x, b.y, w.px, w.py, w.E, w.Ep, d);
+  w.st |= r == 1 ? 1u : 0u;
+  w.st += r < 0 ? 2u : 0u;
+}
+
+// Segments [s0, s1), kSegBatch vertex loads in flight; stops once a
+// segment is certainly within d.
+__device__ __forceinline__ void line_seg_scan(LineWalk& w, uint32_t& tests, const int s0, const int s1, double d) {
+  float2 a = w.q32[s0];
+  for (int c0 = s0; c0 < s1 && !(w.st & 1u); c0 += kSegBatch) {
+    float2 v[kSegBatch];
+#pragma unroll
+    for (int k = 0; k < kSegBatch; k++) v[k] = c0 + k + 1 <= s1 ? w.q32[c0 + k + 1] : a;
+    line_segment_test(w, tests, a, v[0], d);
+#pragma unroll
+    for (int k = 1; k < kSegBatch; k++)
+      if (c0 + k < s1) line_segment_test(w, tests, v[k - 1], v[k], d);
+    a = v[kSegBatch - 1];
+  }
+}
+
+__device__ __forceinline__ uint32_t line_hit_mask(const LineWalk& w, const float4* b, int n) {
+  uint32_t mask = 0;
+#pragma unroll
+  for (int c0 = 0; c0 < kSegFanout; c0 += kWalkBoxBatch) {
+    float4 bx[kWalkBoxBatch];
+#pragma unroll
+    for (int k = 0; k < kWalkBoxBatch; k++) bx[k] = c0 + k < n ? b[c0 + k] : make_float4(1.f, 1.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < kWalkBoxBatch; k++)
+      if (c0 + k < n && line_box_hit(w, bx[k])) mask |= 1u << (c0 + k);
+  }
+  return mask;
+}
+
+// Depth-first walk over the line's segment tree (the layout of walk_ring),
+// ending at the first segment certainly within d.
+__device__ __forceinline__ void walk_line(LineWalk& w, int ls, uint32_t& tests, double d) {
+  if (!w.box) {
+    line_seg_scan(w, tests, 0, w.ns, d);
+    return;
+  }
+  const int top = seg_tree_shape(w.ns, ls).levels - 1;
+  int otop, ntop;
+  level_geom(w.ns, ls, top, otop, ntop);
+  const float4* btop = w.box + otop;
+  for (int c0 = 0; c0 < ntop && !(w.st & 1u); c0 += kSegFanout) {
+    uint64_t masks = (uint64_t)line_hit_mask(w, btop + c0, min(kSegFanout, ntop - c0)) << (8 * top);
+    int lvl = top, base = c0;
+    while (masks && !(w.st & 1u)) {
+      const uint32_t m = (uint32_t)(masks >> (8 * lvl)) & 0xffu;
+      if (m == 0) {
+        lvl++;
+        base = (base >> 3) & ~(kSegFanout - 1);
+        continue;
+      }
+      const int k = __ffs(m) - 1;
+      masks &= ~(1ull << (8 * lvl + k));
+      const int node = base + k;
+      if (lvl == 0) {
+        const int s0 = node << ls;
+        line_seg_scan(w, tests, s0, min(s0 + (1 << ls), w.ns), d);
+        continue;
+      }
+      const int cb = node * kSegFanout;
+      int loff, lcnt;
+      level_geom(w.ns, ls, lvl - 1, loff, lcnt);
+      const uint32_t cm = line_hit_mask(w, w.box + loff + cb, min(kSegFanout, lcnt - cb));
+      if (cm) {
+        lvl--;
+        base = cb;
+        masks |= (uint64_t)cm << (8 * lvl);
+      }
+    }
+  }
+}
+
+// Candidates whose build row is not a single line with an FP32 frame, and
+// those with an uncertain test and no certain one, go to `redo` for the
+// FP64 kernel.
+template <int WAVES>
+__global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_walk_lines(
+    RingIndexView R, const double2* __restrict__ pts, const uint32_t* __restrict__ cb,
+    const uint32_t* __restrict__ cp, uint64_t n, const unsigned long long* n_dev, const uint32_t* probe_ids,
+    double dist, uint32_t* ob, uint32_t* op, uint64_t cap, unsigned long long* counter, unsigned long long* stats,
+    uint32_t* redo, unsigned long long* n_redo, unsigned long long* queue) {
+  __shared__ uint32_t wo_b[kWalkBlock / 64][kWaveOutCap], wo_p[kWalkBlock / 64][kWaveOutCap];
+  WaveOut wo{wo_b[threadIdx.x >> 6], wo_p[threadIdx.x >> 6], 0};
+  if (n_dev) n = min(n, (uint64_t)*n_dev);  // candidate count produced on the device (n: its bound)
+  const int ls = R.seg_leaf_shift;
+  const uint32_t n32 = (uint32_t)n;  // (the launcher admits at most 2^32 - 65 candidates)
+  const int lane = threadIdx.x & 63;
+  // the FP64 kernel's pruning bound, without E
+  const float Dd = (float)(dist * 1.0001);
+  uint32_t chunk_end = 0;
+  auto next = [&](uint32_t cur) -> uint32_t {
+    if (cur + 64 < chunk_end) return cur + 64;
+    unsigned long long v = 0;
+    if (lane == 0) v = atomicAdd(queue, (unsigned long long)kQueueChunk);
+    v = __shfl(v, 0);
+    chunk_end = v + kQueueChunk < n32 ? (uint32_t)(v + kQueueChunk) : n32;
+    return v < n32 ? (uint32_t)v : n32;
+  };
+  uint32_t tests = 0, unsure = 0;
+  for (uint32_t wbase = next(0); wbase < n32; wbase = next(wbase)) {
+    const uint32_t i = wbase + lane;
+    bool keep = false, walk = false, defer = false;
+    LineWalk w;
+    if (i < n32) {
+      const uint32_t b = cb[i];
+      // the probe is a point column: null and empty rows hold NaN
+      const double2 pt = pts[cp[i]];
+      if (pt.x == pt.x || pt.y == pt.y) {
+        const PolyDesc dsc = R.desc[b];
+        if (dsc.npts < 2 || !(dsc.err < INFINITY)) {
+          // other rows, lines of fewer than 2 vertices (distance 0) and
+          // lines without an FP32 frame: the FP64 kernel
+          defer = true;
+        } else {
+          walk = true;
+          const double2 o = ring_origin(dsc.rect);
+          w.px = (float)(pt.x - o.x);
+          w.py = (float)(pt.y - o.y);
+          w.E = dsc.err;
+          w.D = Dd + 4.0f * dsc.err;
+          w.Ep = fmaxf(fabsf(w.px), fabsf(w.py)) * 5.9604645e-08f * 1.0001f;
+          w.box = dsc.tree_off >= 0 ? R.seg_tree + dsc.tree_off : nullptr;
+          w.q32 = R.xy32 + dsc.v0;
+          w.ns = dsc.npts - 1;
+          w.st = 0;
+        }
+      }
+    }
+    if (walk) {
+      walk_line(w, ls, tests, dist);
+      keep = (w.st & 1u) != 0;
+      defer = !keep && w.st > 1u;
+      unsure += w.st >> 1;
+    }
+    wave_push(defer, i, redo, n_redo);
+    uint32_t b = 0, p = 0;
+    if (keep) {
+      b = cb[i];
+      p = cp[i];
+      if (probe_ids) p = probe_ids[p];
+    }
+    wave_append(wo, keep, b, p, ob, op, cap, counter);
+  }
+  wave_flush(wo, ob, op, cap, counter);
+  for (int o = 32; o > 0; o >>= 1) {
+    tests += __shfl_xor(tests, o);
+    unsure += __shfl_xor(unsure, o);
+  }
+  if ((threadIdx.x & 63) == 0 && stats && (tests | unsure)) {
+    atomicAdd(&stats[0], (unsigned long long)tests);
+    atomicAdd(&stats[1], (unsigned long long)unsure);
+  }
+}
+
 }  // namespace
+
+void launch_line_walk(const RingIndex& ri, const GeomColumn& probe, const uint32_t* cb, const uint32_t* cp,
+                      uint64_t n, const unsigned long long* n_dev, const uint32_t* probe_ids, double d, uint32_t* ob,
+                      uint32_t* op, uint64_t cap, unsigned long long* counter, unsigned long long* stats,
+                      uint32_t* redo, unsigned long long* n_redo, hipStream_t s, unsigned long long* queue) {
+  if (n == 0) return;
+  if (!probe.points_only) throw Error("launch_line_walk: the probe column must hold points only");
+  if (ri.desc.p == nullptr) throw Error("launch_line_walk: the ring index has no row records");
+  if (!queue) throw Error("launch_line_walk: needs a queue counter");
+  if (n >= 0xffffffffull - 64) throw Error("launch_line_walk: 2^32 - 64 candidates or more");
+  const uint64_t blocks = (n + kWalkBlock - 1) / kWalkBlock;
+  int dev = 0, cus = 0;
+  SJ_CHECK(hipGetDevice(&dev));
+  SJ_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  // the blocks that can be resident at once
+  const unsigned g =
+      (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, (uint64_t)cus * 4 * SJ_LW_WAVES / (kWalkBlock / 64)));
+  hipLaunchKernelGGL(k_walk_lines<SJ_LW_WAVES>, g, kWalkBlock, 0, s, ri.view(), probe.xy.p, cb, cp, n, n_dev, probe_ids, d, ob,
+                     op, cap, counter, stats, redo, n_redo, queue);
+  SJ_CHECK(hipGetLastError());
+}
 
 void launch_ring_walk(const RingIndex& ri, const GeomColumn& probe, const uint32_t* cb, const uint32_t* cp,
                       uint64_t n, const unsigned long long* n_dev, const uint32_t* probe_ids, bool raster_checked,

@@ -1925,13 +1925,13 @@ __global__ void k_poly_desc(GeomView g, RingIndexView R, PolyDesc* out) {
   out[r] = d;
 }
 
-// Rows that get a cell raster: one polygon ring with a segment tree and a
-// rect of positive width and height.
+// Rows that get a cell raster: one polygon ring (with or without a
+// segment tree) with an FP32 frame and a rect of positive width and height.
 __global__ void k_raster_flags(GeomView g, const PolyDesc* desc, int64_t* flag) {
   int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= g.n_rows) return;
   const PolyDesc d = desc[r];
-  flag[r] = is_polyish(g.type[r]) && d.npts >= 4 && d.tree_off >= 0 && d.rect.z > d.rect.x && d.rect.w > d.rect.y &&
+  flag[r] = is_polyish(g.type[r]) && d.npts >= 4 && d.rect.z > d.rect.x && d.rect.w > d.rect.y &&
                     d.err < INFINITY
                 ? 1
                 : 0;
@@ -1973,7 +1973,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_build(Ctx c, PolyDesc* desc, 
   wave_lds_sync();
   // lane k writes words k, k + 64, ...: 16 cells each
   const double2 o = ring_origin(rb);
-  const float4* tree = c.R.seg_tree + d.tree_off;
+  const float4* tree = d.tree_off >= 0 ? c.R.seg_tree + d.tree_off : nullptr;  // (short rings: scanned whole)
   const float2* q32 = c.R.xy32 + d.v0;
   for (int wi = lane; wi < kRasterWords; wi += 64) {
     uint32_t word = 0;
@@ -2498,7 +2498,24 @@ uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const Geo
         cb, cp, open, mask, open_count, &n_open, s);
   const uint32_t* wb = n_open ? open.b.p : cb;
   const uint32_t* wp = n_open ? open.p.p : cp;
-  switch (kind) {
+  // lines, certified form: the lean FP32 walk, then the FP64 kernel over
+  // the candidates it lists (other rows, uncertain tests)
+  const bool lean = kind == kDistLines && precision == 0 && ri.desc.p != nullptr && probe.points_only &&
+                    d >= 0 && d < INFINITY && n < 0xffffffffull - 64;
+  DevArray<uint32_t> redo_own;
+  DevArray<unsigned long long> walk_aux;  // redo count, queue, tests, uncertain tests
+  switch (lean ? -1 : kind) {
+    case -1: {
+      DevArray<uint32_t>& redo = ws ? ws->redo : redo_own;
+      fit(redo, n, s);
+      walk_aux.alloc(4, s);
+      SJ_CHECK(hipMemsetAsync(walk_aux.p, 0, 32, s));
+      launch_line_walk(ri, probe, wb, wp, n, n_open, probe_ids, d, out.b.p, out.p.p, out.cap(), out.counter.p,
+                       walk_aux.p + 2, redo.p, walk_aux.p, s, walk_aux.p + 1);
+      hipLaunchKernelGGL(k_refine_distance<kDistLines>, std::min(grid_for(n), 2048u), kBlock, 0, s, c, d, 0, wb, wp,
+                         n, redo.p, out.b.p, out.p.p, out.cap(), out.counter.p, aux.p, walk_aux.p, probe_ids);
+      break;
+    }
     case kDistLines:
       hipLaunchKernelGGL(k_refine_distance<kDistLines>, grid_for(n), kBlock, 0, s, c, d, 0, wb, wp, n, nullptr,
                          out.b.p, out.p.p, out.cap(), out.counter.p, aux.p, n_open, probe_ids);
@@ -2513,13 +2530,14 @@ uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const Geo
   }
   SJ_CHECK(hipGetLastError());
   if (timer) timer->record_stop(s);
-  unsigned long long cnt = 0, h[3], walked = n;
+  unsigned long long cnt = 0, h[3], walked = n, wh[4] = {0, 0, 0, 0};
   SJ_CHECK(hipMemcpyAsync(&cnt, out.counter.p, 8, hipMemcpyDeviceToHost, s));
   SJ_CHECK(hipMemcpyAsync(h, aux.p, 24, hipMemcpyDeviceToHost, s));
+  if (walk_aux.p) SJ_CHECK(hipMemcpyAsync(wh, walk_aux.p, 32, hipMemcpyDeviceToHost, s));
   if (n_open) SJ_CHECK(hipMemcpyAsync(&walked, n_open, 8, hipMemcpyDeviceToHost, s));
   SJ_CHECK(hipStreamSynchronize(s));
   if (stats) {
-    stats->segment_tests += h[0];
+    stats->segment_tests += h[0] + wh[2];
     stats->fallback_tests += h[1] + h[2];  // FP64 re-evaluations: exact ring scans and segment terms
     stats->walked += walked;
   }

@@ -273,7 +273,7 @@ void compute_rects(const GeomColumn& g, const RingIndex* ri, bool point_nearest,
 // Ring refine index for a build column.
 struct RingIndexParams {
   int min_segs_for_tree = 32;  // rings with fewer segments are scanned whole
-  bool raster = true;          // cell rasters for single-ring polygon rows with trees
+  bool raster = true;          // cell rasters for single-ring polygon rows
   int seg_leaf_shift = kSegLeafShift;  // segments per tree leaf: 1 << seg_leaf_shift (4 or 5)
 };
 void build_ring_index(const GeomColumn& g, RingIndex& ri, const RingIndexParams& p, hipStream_t s);
@@ -645,6 +645,16 @@ void launch_ring_walk(const RingIndex& ri, const GeomColumn& probe, const uint32
                       uint32_t* ob, uint32_t* op, uint64_t cap, unsigned long long* counter,
                       unsigned long long* stats, uint32_t* redo, unsigned long long* n_redo, uint32_t* rest,
                       unsigned long long* n_rest, int waves, hipStream_t s, unsigned long long* queue = nullptr);
+
+// The certified FP32 ST_DWithin walk of candidates whose build row is one
+// line: pairs certainly within d are appended to (ob, op); candidates of
+// other rows, and those with an uncertain test and no certain one, are
+// listed in redo (count *n_redo) for the FP64 kernel.  stats[0..1]: tests
+// and uncertain tests.  queue: a counter that is zero at launch.
+void launch_line_walk(const RingIndex& ri, const GeomColumn& probe, const uint32_t* cb, const uint32_t* cp,
+                      uint64_t n, const unsigned long long* n_dev, const uint32_t* probe_ids, double d, uint32_t* ob,
+                      uint32_t* op, uint64_t cap, unsigned long long* counter, unsigned long long* stats,
+                      uint32_t* redo, unsigned long long* n_redo, hipStream_t s, unsigned long long* queue);
 
 // Throws "build index out of range: v" / "probe index out of range: v" for
 // the first position (build index first) with cb[i] >= nb or cp[i] >= np.
