@@ -30,6 +30,10 @@ double SjhipHostCoordToLine(double px, double py, double sx, double sy, double e
  * d, 0 certainly not, -1 undecided (FP64 fallback needed). */
 int SjhipHostSegmentWithinF32(float ax, float ay, float bx, float by, float px, float py, float e_ab, float e_p,
                               double d);
+/* Certified FP32 form of the segment intersection test for segments a-b
+ * and c-d given within e of exact per coordinate: 1 certainly
+ * intersecting, 0 certainly not, -1 undecided (FP64 fallback needed). */
+int SjhipHostSegmentsCrossF32(float ax, float ay, float bx, float by, float cx, float cy, float dx, float dy, float e);
 /* Batched forms over arrays (n entries). */
 void SjhipHostRaycastBatch(const double* seg_pts, int64_t n, int32_t* out);
 void SjhipHostHypotBatch(const double* xy, int64_t n, double* out);

@@ -573,6 +573,11 @@ int64_t morton_order_points(const GeomColumn& g, DevArray<uint32_t>& order, hipS
   hipLaunchKernelGGL(k_point_keys, blocks_for(n), kBlock, 0, s, g.xy.p, g.type.p, n, bb[0], bb[1], sx, sy, keys.p,
                      ids.p);
   SJ_CHECK(hipGetLastError());
+  // (all 32 bits: rocPRIM's merge-sort path, which it takes below about 1M
+  // keys, builds its comparator's mask as (1 << (begin_bit + bits)) - 1,
+  // a 32-bit shift when a range [begin_bit > 0, 32) is asked for, which
+  // yields the low bits instead; its merge phase then works on runs sorted
+  // by another key than its own and indexes out of bounds)
   sort_keys(keys, ids, n, s);
   order = std::move(ids);
   if (frame) {

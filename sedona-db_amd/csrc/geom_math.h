@@ -169,6 +169,40 @@ SJ_HD bool segment_intersects(double ax, double ay, double bx, double by, double
   return (t >= 0) && (t <= 1) && (u >= 0) && (u <= 1);
 }
 
+// Certified FP32 orientation of c relative to the directed line a -> b:
+// 1 left, -1 right, 0 uncertain.  Inputs are FP32 values within e (per
+// coordinate) of exact ones; the bound has the form raycast_f32_certified
+// uses for the same 2 x 2 determinant.
+SJ_HD int orient_f32_certified(float ax, float ay, float bx, float by, float cx, float cy, float e) {
+  const float d1 = bx - ax, d2 = cy - ay, d3 = by - ay, d4 = cx - ax;
+  const float det = d1 * d2 - d3 * d4;
+  const float a1 = fabsf(d1), a2 = fabsf(d2), a3 = fabsf(d3), a4 = fabsf(d4);
+  const float u = 5.9604645e-08f;  // 2^-24
+  const float bound = 3.0f * e * (a1 + a2 + a3 + a4) + 8.0f * u * (a1 * a2 + a3 * a4) + 8.0f * e * e;
+  return det > bound ? 1 : det < -bound ? -1 : 0;
+}
+
+// Certified FP32 form of segment_intersects (below) for segments a-b and
+// c-d given within e of exact: 1 when the segments certainly cross at a
+// point interior to both (each one's endpoints strictly on opposite sides
+// of the other's line), 0 when one segment certainly lies strictly on one
+// side of the other's line, -1 otherwise.  Where it decides, the exact
+// closed segments intersect (or not) with a margin far above the rounding
+// of the reference's FP64 expressions, so segment_intersects agrees.
+SJ_HD int segments_cross_f32_certified(float ax, float ay, float bx, float by, float cx, float cy, float dx, float dy,
+                                       float e) {
+  const float e2 = 2.0f * e;
+  if (fminf(cx, dx) > fmaxf(ax, bx) + e2 || fmaxf(cx, dx) < fminf(ax, bx) - e2 ||
+      fminf(cy, dy) > fmaxf(ay, by) + e2 || fmaxf(cy, dy) < fminf(ay, by) - e2)
+    return 0;
+  const int o1 = orient_f32_certified(cx, cy, dx, dy, ax, ay, e), o2 = orient_f32_certified(cx, cy, dx, dy, bx, by, e);
+  if (o1 != 0 && o1 == o2) return 0;
+  const int o3 = orient_f32_certified(ax, ay, bx, by, cx, cy, e), o4 = orient_f32_certified(ax, ay, bx, by, dx, dy, e);
+  if (o3 != 0 && o3 == o4) return 0;
+  if (o1 != 0 && o2 != 0 && o3 != 0 && o4 != 0) return 1;
+  return -1;
+}
+
 // ---------------------------------------------------------------------------
 // Exact orientation sign (Shewchuk's expansion arithmetic, "Adaptive
 // Precision Floating-Point Arithmetic and Fast Robust Geometric Predicates",

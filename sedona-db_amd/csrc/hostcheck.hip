@@ -36,6 +36,9 @@ int SjhipHostSegmentWithinF32(float ax, float ay, float bx, float by, float px, 
                               double d) {
   return segment_within_f32(ax, ay, bx, by, px, py, e_ab, e_p, d);
 }
+int SjhipHostSegmentsCrossF32(float ax, float ay, float bx, float by, float cx, float cy, float dx, float dy, float e) {
+  return segments_cross_f32_certified(ax, ay, bx, by, cx, cy, dx, dy, e);
+}
 void SjhipHostHypotBatch(const double* xy, int64_t n, double* out) {
   for (int64_t i = 0; i < n; i++) out[i] = hypot_ref(xy[2 * i], xy[2 * i + 1]);
 }

@@ -1143,6 +1143,239 @@ __device__ bool wave_iring_intersects(IRing ring, IRing other, int lane) {
   return false;
 }
 
+// Certified FP32 form of ring_two_crossings: 1 when two segments of the
+// ring certainly intersect segment p-q (segment_intersects), 0 when fewer
+// than two can, -1 otherwise.  The segment is taken into the ring's FP32
+// frame (its rounding Ep added to the ring's bound E); tree boxes farther
+// than the test's own bounding-box margin are pruned.
+__device__ __forceinline__ int ring_two_crossings_cert(IRing ring, double px, double py, double qx,
+                                                       double qy) {
+  if (!(ring.E < INFINITY)) return -1;
+  const int ns = ring.npts >= 3 ? (int)(ring.npts - 1) : 0;
+  const float pxl = (float)(px - ring.o.x), pyl = (float)(py - ring.o.y);
+  const float qxl = (float)(qx - ring.o.x), qyl = (float)(qy - ring.o.y);
+  const float Ep = fmaxf(fmaxf(fabsf(pxl), fabsf(pyl)), fmaxf(fabsf(qxl), fabsf(qyl))) * 5.9604645e-08f * 1.0001f;
+  const float e = ring.E + Ep;
+  int yes = 0, unc = 0;
+  const float2* q32 = ring.q32;
+  if (!ring.tree) {
+    for (int j = 0; j < ns && yes < 2; j++) {
+      const float2 c = q32[j], d = q32[j + 1];
+      const int r = segments_cross_f32_certified(pxl, pyl, qxl, qyl, c.x, c.y, d.x, d.y, e);
+      yes += r == 1 ? 1 : 0;
+      unc += r < 0 ? 1 : 0;
+    }
+  } else {
+    // depth-first over the tree (byte l of `masks`: the children of level
+    // l still to visit), the level geometry recomputed rather than held
+    const float m = 2.0f * e + 4.0f * ring.E;
+    const float x0 = fminf(pxl, qxl) - m, x1 = fmaxf(pxl, qxl) + m;
+    const float y0 = fminf(pyl, qyl) - m, y1 = fmaxf(pyl, qyl) + m;
+    const int ls = ring.ls;
+    auto hits = [&](const float4* b, int n) {
+      uint32_t mask = 0;
+      for (int k = 0; k < n; k++) {
+        const float4 bx = b[k];
+        if (bx.x <= x1 && bx.z >= x0 && bx.y <= y1 && bx.w >= y0) mask |= 1u << k;
+      }
+      return mask;
+    };
+    const int top = seg_tree_shape(ns, ls).levels - 1;
+    int otop, ntop;
+    level_geom(ns, ls, top, otop, ntop);
+    bool done = false;
+    for (int c0 = 0; c0 < ntop && !done; c0 += kSegFanout) {
+      uint64_t masks = (uint64_t)hits(ring.tree + otop + c0, min(kSegFanout, ntop - c0)) << (8 * top);
+      int lvl = top, base = c0;
+      while (masks && !done) {
+        const uint32_t mk = (uint32_t)(masks >> (8 * lvl)) & 0xffu;
+        if (mk == 0) {
+          lvl++;
+          base = (base >> 3) & ~(kSegFanout - 1);
+          continue;
+        }
+        const int k = __ffs(mk) - 1;
+        masks &= ~(1ull << (8 * lvl + k));
+        const int node = base + k;
+        if (lvl == 0) {
+          const int s0 = node << ls, s1 = min(s0 + (1 << ls), ns);
+          for (int j = s0; j < s1 && !done; j++) {
+            const float2 c = q32[j], d = q32[j + 1];
+            const int r = segments_cross_f32_certified(pxl, pyl, qxl, qyl, c.x, c.y, d.x, d.y, e);
+            yes += r == 1 ? 1 : 0;
+            unc += r < 0 ? 1 : 0;
+            done = yes >= 2;
+          }
+          continue;
+        }
+        const int cb = node * kSegFanout;
+        int loff, lcnt;
+        level_geom(ns, ls, lvl - 1, loff, lcnt);
+        const uint32_t cm = hits(ring.tree + loff + cb, min(kSegFanout, lcnt - cb));
+        if (cm) {
+          lvl--;
+          base = cb;
+          masks |= (uint64_t)cm << (8 * lvl);
+        }
+      }
+    }
+  }
+  return yes >= 2 ? 1 : (yes + unc >= 2 ? -1 : 0);
+}
+
+// tg_ring_intersects_ring as wave_iring_intersects decides it, in two
+// phases run by two kernels, with certified FP32 tests only.  The answer
+// is an OR over the segments of the smaller-rect ring of two tests, so the
+// order of evaluation does not change it:
+// phase 1, one vertex of that ring per lane (every segment endpoint is a
+// vertex of the closed ring) covered by the other ring;
+// phase 2, only when no vertex is covered: the two-crossings count of each
+// segment.
+
+// The operands of a candidate: 1 when both rows are one polygon with one
+// ring whose rects meet (ring: the larger-rect ring, other: the smaller),
+// 0 when the pair certainly does not intersect (not both polygons, an empty
+// part, disjoint rects, a ring of fewer than 3 points), 2 when the rows
+// have several parts or rings (left to k_refine_polypoly).
+__device__ __forceinline__ int pp_operands(const Ctx& c, uint32_t b, uint32_t p, IRing& ring, IRing& other) {
+  if (!(is_polyish(c.B.type[b]) && is_polyish(c.P.type[p]))) return 0;
+  if (c.B.part_off[b + 1] - c.B.part_off[b] != 1 || c.P.part_off[p + 1] - c.P.part_off[p] != 1) return 2;
+  const int64_t qa = c.B.part_off[b], qb = c.P.part_off[p];
+  const int64_t a0 = c.B.ring_off[qa], a1 = c.B.ring_off[qa + 1];
+  const int64_t b0 = c.P.ring_off[qb], b1 = c.P.ring_off[qb + 1];
+  if (a0 == a1 || b0 == b1) return 0;  // an empty part meets nothing
+  if (a1 - a0 != 1 || b1 - b0 != 1) return 2;
+  ring = iring(c.B, c.R, a0);
+  other = iring(c.P, c.PR, b0);
+  if (ring.npts < 3 || other.npts < 3) return 0;
+  const double4 a = ring.rect, r = other.rect;
+  if (r.x > a.z || r.z < a.x || r.y > a.w || r.w < a.y) return 0;
+  if ((r.z - r.x) * (r.w - r.y) > (a.z - a.x) * (a.w - a.y)) {
+    IRing t = ring; ring = other; other = t;
+  }
+  return 1;
+}
+
+// Phase 1: 1 a vertex is certainly covered, 0 none is, -1 none certainly
+// is but some test was uncertain.
+__device__ int wave_vertices_covered(const IRing& ring, const IRing& other, int lane) {
+  const double4 a = ring.rect;
+  const int64_t ns = other.npts - 1;
+  bool unsure = false;
+  for (int64_t base = 0; base < ns; base += 64) {
+    const int64_t i = base + lane;
+    bool hit = false, uns = false;
+    if (i < ns) {
+      const double2 p = other.xy[i];
+      if (rect_covers(a, p.x, p.y)) {
+        uint64_t tests = 0, fb = 0;
+        const bool in = ring_contains_run<kPrecDeferred>(ring.xy, ring.q32, ring.tree, ring.ls, ring.npts, a, ring.o,
+                                                         ring.E, p.x, p.y, true, tests, fb);
+        hit = fb == 0 && in;
+        uns = fb != 0;
+      }
+    }
+    if (__ballot(hit)) return 1;
+    unsure |= __ballot(uns) != 0;
+  }
+  return unsure ? -1 : 0;
+}
+
+// Phase 2: 1 some segment certainly crosses the ring twice, 0 none can, -1
+// undecided.
+__device__ int wave_two_crossings(const IRing& ring, const IRing& other, int lane) {
+  const double4 a = ring.rect;
+  const int64_t ns = other.npts - 1;
+  bool unsure = false;
+  for (int64_t base = 0; base < ns; base += 64) {
+    const int64_t i = base + lane;
+    bool hit = false, uns = false;
+    if (i < ns) {
+      const double2 p = other.xy[i], q = other.xy[i + 1];
+      const double sx0 = fmin0(p.x, q.x), sy0 = fmin0(p.y, q.y), sx1 = fmax0(p.x, q.x), sy1 = fmax0(p.y, q.y);
+      if (!(a.x > sx1 || a.z < sx0 || a.y > sy1 || a.w < sy0)) {
+        const int r = ring_two_crossings_cert(ring, p.x, p.y, q.x, q.y);
+        hit = r == 1;
+        uns = r < 0;
+      }
+    }
+    if (__ballot(hit)) return 1;
+    unsure |= __ballot(uns) != 0;
+  }
+  return unsure ? -1 : 0;
+}
+
+// One pair appended to a list by lane 0 of the wavefront: list (lb, lp)
+// with counter *cnt, at the front (k) or, with from_back, at cap - 1 - k.
+__device__ __forceinline__ void pp_push(int lane, uint32_t b, uint32_t p, uint32_t* lb, uint32_t* lp,
+                                        unsigned long long* cnt, uint64_t cap, bool from_back) {
+  if (lane != 0) return;
+  const unsigned long long k = atomicAdd(cnt, 1ull);
+  const uint64_t at = from_back ? cap - 1 - k : k;
+  lb[at] = b;
+  lp[at] = p;
+}
+
+// Polygon x polygon ST_Intersects, certified form, phase 1 (one wavefront
+// per candidate).  Pairs without a covered vertex go to (lb, lp): from the
+// front (count cnt[0]) when every test was certain, from the back (count
+// cnt[1]) when one was not; rows of several parts or rings go to (db, dp)
+// (count cnt[2]).  Capacities: n each.
+__global__ __launch_bounds__(kBlock) void k_pp_vertices(Ctx c, const uint32_t* cb, const uint32_t* cp, uint64_t n,
+                                                        const unsigned long long* n_dev, uint32_t* ob, uint32_t* op,
+                                                        uint64_t cap, unsigned long long* counter, uint32_t* lb,
+                                                        uint32_t* lp, uint32_t* db, uint32_t* dp,
+                                                        unsigned long long* cnt) {
+  SJ_WAVE_OUT(wo);
+  const uint64_t n0 = n;
+  if (n_dev) n = min(n, (uint64_t)*n_dev);  // candidate count produced on the device (n: its bound)
+  const int lane = threadIdx.x & 63;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t i = wave; i < n; i += nwaves) {
+    const uint32_t b = cb[i], p = cp[i];
+    IRing ring, other;
+    const int k = pp_operands(c, b, p, ring, other);
+    int r = 0;
+    if (k == 1) {
+      r = wave_vertices_covered(ring, other, lane);
+      if (r == 0) pp_push(lane, b, p, lb, lp, &cnt[0], n0, false);
+      if (r < 0) pp_push(lane, b, p, lb, lp, &cnt[1], n0, true);
+    } else if (k == 2) {
+      pp_push(lane, b, p, db, dp, &cnt[2], n0, false);
+    }
+    emit(wo, lane == 0 && k == 1 && r == 1, b, p, ob, op, cap, counter);
+  }
+  wave_flush(wo, ob, op, cap, counter);
+}
+
+// Phase 2 over the pairs phase 1 listed (UNSURE: those with an uncertain
+// vertex test, read from the back of the list): kept when a segment
+// certainly crosses twice; else to (db, dp) when a test was uncertain in
+// either phase, else dropped.
+template <bool UNSURE>
+__global__ __launch_bounds__(kBlock) void k_pp_crossings(Ctx c, const uint32_t* lb, const uint32_t* lp,
+                                                         uint64_t cap_l, const unsigned long long* n_dev,
+                                                         uint32_t* ob, uint32_t* op, uint64_t cap,
+                                                         unsigned long long* counter, uint32_t* db, uint32_t* dp,
+                                                         unsigned long long* n_defer, uint64_t cap_d) {
+  SJ_WAVE_OUT(wo);
+  const uint64_t n = min(cap_l, (uint64_t)*n_dev);
+  const int lane = threadIdx.x & 63;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t i = wave; i < n; i += nwaves) {
+    const uint64_t at = UNSURE ? cap_l - 1 - i : i;
+    const uint32_t b = lb[at], p = lp[at];
+    IRing ring, other;
+    int r = 0;
+    if (pp_operands(c, b, p, ring, other) == 1) r = wave_two_crossings(ring, other, lane);
+    if (r < 0 || (UNSURE && r == 0)) pp_push(lane, b, p, db, dp, n_defer, cap_d, false);
+    emit(wo, lane == 0 && r == 1, b, p, ob, op, cap, counter);
+  }
+  wave_flush(wo, ob, op, cap, counter);
+}
+
 // Polygon x polygon ST_Intersects with rings indexed on both sides, one
 // wavefront per candidate (single-ring parts voted by the wavefront;
 // parts with holes decided by every lane alike).
@@ -2399,8 +2632,30 @@ uint64_t refine_relation(const GeomColumn& build, const RingIndex& ri, const Geo
     const uint32_t* wb = n_open ? open.b.p : cb;
     const uint32_t* wp = n_open ? open.p.p : cp;
     const uint64_t waves = std::min<uint64_t>(n, 1u << 18);
-    hipLaunchKernelGGL(k_refine_polypoly, (unsigned int)((waves * 64 + kBlock - 1) / kBlock), kBlock, 0, s, c, wb, wp,
-                       n, out.b.p, out.p.p, out.cap(), out.counter.p, n_open);
+    const unsigned int grid = (unsigned int)((waves * 64 + kBlock - 1) / kBlock);
+    if (precision == 0 && n <= 0xffffffffull) {
+      // the certified phases (covered vertices, then two crossings, each
+      // kernel lean), then the inline-precision kernel over the pairs they
+      // leave (rows of several parts or rings, undecided)
+      DevArray<uint32_t> lb_own, lp_own, db(n, s), dp(n, s);
+      DevArray<uint32_t>& lb = ws ? ws->redo : lb_own;
+      DevArray<uint32_t>& lp = ws ? ws->rest : lp_own;
+      fit(lb, n, s);
+      fit(lp, n, s);
+      DevArray<unsigned long long> cnt(3, s);  // certain list, unsure list, deferred
+      SJ_CHECK(hipMemsetAsync(cnt.p, 0, 24, s));
+      hipLaunchKernelGGL(k_pp_vertices, grid, kBlock, 0, s, c, wb, wp, n, n_open, out.b.p, out.p.p, out.cap(),
+                         out.counter.p, lb.p, lp.p, db.p, dp.p, cnt.p);
+      hipLaunchKernelGGL(k_pp_crossings<false>, grid, kBlock, 0, s, c, lb.p, lp.p, n, cnt.p, out.b.p, out.p.p,
+                         out.cap(), out.counter.p, db.p, dp.p, cnt.p + 2, n);
+      hipLaunchKernelGGL(k_pp_crossings<true>, grid, kBlock, 0, s, c, lb.p, lp.p, n, cnt.p + 1, out.b.p, out.p.p,
+                         out.cap(), out.counter.p, db.p, dp.p, cnt.p + 2, n);
+      hipLaunchKernelGGL(k_refine_polypoly, std::min(grid, 4096u), kBlock, 0, s, c, db.p, dp.p, n, out.b.p, out.p.p,
+                         out.cap(), out.counter.p, cnt.p + 2);
+    } else {
+      hipLaunchKernelGGL(k_refine_polypoly, grid, kBlock, 0, s, c, wb, wp, n, out.b.p, out.p.p, out.cap(),
+                         out.counter.p, n_open);
+    }
     SJ_CHECK(hipGetLastError());
     if (timer) timer->record_stop(s);
     unsigned long long cnt = 0, walked = n;

@@ -207,6 +207,7 @@ EXPORTED = [
     "SjhipHostOrient2d", "SjhipHostHypot", "SjhipHostCoordToLine", "SjhipHostRaycastBatch", "SjhipHostHypotBatch",
     "SjhipTestingGraphSortPairs", "SjhipJoinDeviceNoAlloc", "SjhipRefinerKnnTies", "SjhipHostSegmentWithinF32",
     "SjhipRefinerPairDistances", "SjhipIndexSetFilterMode", "SjhipJoinHostStream", "SjhipPartitionedJoin",
+    "SjhipHostSegmentsCrossF32",
 ]
 
 _lib = None
@@ -284,6 +285,7 @@ def lib():
         L.SjhipHostRaycastBatch.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
         L.SjhipHostHypotBatch.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
         L.SjhipHostSegmentWithinF32.argtypes = [ctypes.c_float] * 8 + [ctypes.c_double]
+        L.SjhipHostSegmentsCrossF32.argtypes = [ctypes.c_float] * 9
         L.SjhipTestingGraphSortPairs.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                                                          ctypes.c_int]
         _lib = L

@@ -209,9 +209,6 @@ def test_device_rects_match_reference_rounding():
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
 
 
-@pytest.mark.skip(reason="two GPU boxes ended with a host SIGSEGV and a GPU fault 1.8 s into runs whose first "
-                         "test was this one (precision 0, then 1); no cause found in the code it runs, so it "
-                         "stays off the GPU")
 @pytest.mark.parametrize("precision", [0, 1])
 def test_corpus_all_pairs_match_reference(precision, opts):
     """Every (polygon, point) pair of the adversarial corpus, both roles."""

@@ -208,3 +208,60 @@ def test_fp32_certified_distance_never_contradicts_fp64():
     # most samples sit within 1e-7 of the threshold on purpose; those far
     # from it are decided both ways
     assert decided > 0.4 * total and min(seen.values()) > 5000, (decided, total, seen)
+
+
+def test_fp32_certified_segment_crossing_never_contradicts_fp64():
+    """The certified FP32 segment x segment test never decides otherwise
+    than the reference's FP64 segment_intersects (tg.c:1392-1443), with
+    ring-local FP32 coordinates and the bound the polygon x polygon kernels
+    use (the ring's E = 2^-22 H plus the other segment's own rounding):
+    crossing, touching (an endpoint on the other segment or one ulp off it),
+    shared endpoints, collinear overlapping and disjoint, near-parallel and
+    disjoint pairs, rings of extent 1e-3 .. 1e3 at coordinates up to 1e4."""
+    rng = np.random.default_rng(23)
+    lib = L.lib()
+    seen = {0: 0, 1: 0, -1: 0}
+    for _ in range(60000):
+        o = rng.uniform(-1e4, 1e4, 2)
+        H = 10.0 ** rng.uniform(-3, 3)
+        a = o + rng.uniform(-H, H, 2)
+        b = a + rng.normal(0, H * 10.0 ** rng.uniform(-3, 0), 2)
+        kind = rng.integers(0, 6)
+        if kind == 0:  # general position
+            c = o + rng.uniform(-H, H, 2)
+            d = c + rng.normal(0, H * 10.0 ** rng.uniform(-3, 0), 2)
+        elif kind == 1:  # an endpoint on a-b, or a few ulps off it
+            t = rng.random()
+            c = a + t * (b - a) + rng.integers(-2, 3, 2) * np.spacing(np.abs(a) + 1.0)
+            d = c + rng.normal(0, H * 0.3, 2)
+        elif kind == 2:  # shared endpoint
+            c = b.copy()
+            d = c + rng.normal(0, H * 0.3, 2)
+        elif kind == 3:  # collinear, overlapping or not
+            t0, t1 = rng.uniform(-0.5, 1.5, 2)
+            c, d = a + t0 * (b - a), a + t1 * (b - a)
+        elif kind == 4:  # near-parallel, offset by a tiny distance
+            v = b - a
+            nrm = np.array([-v[1], v[0]]) / (np.hypot(*v) or 1.0)
+            off = H * 10.0 ** rng.uniform(-9, -3) * rng.choice([-1.0, 1.0])
+            c = a + rng.uniform(-0.5, 0.5) * v + off * nrm
+            d = b + rng.uniform(-0.5, 0.5) * v + off * nrm * rng.uniform(0.5, 1.5)
+        else:  # crossing near an endpoint of c-d
+            t = rng.random()
+            m = a + t * (b - a)
+            dirn = rng.normal(0, 1, 2)
+            c = m + dirn * H * 10.0 ** rng.uniform(-8, -1)
+            d = m - dirn * H * rng.uniform(0.1, 1.0)
+        E = np.float32(H * 2.384185791015625e-07) * np.float32(1.0001)
+        loc = [np.float32(x) for x in (a[0] - o[0], a[1] - o[1], b[0] - o[0], b[1] - o[1])]
+        q = [np.float32(x) for x in (c[0] - o[0], c[1] - o[1], d[0] - o[0], d[1] - o[1])]
+        Ep = np.float32(max(abs(v) for v in q)) * np.float32(5.9604645e-08) * np.float32(1.0001)
+        # the kernels pass the other ring's segment first, the ring's second
+        f = lib.SjhipHostSegmentsCrossF32(*[float(x) for x in q], *[float(x) for x in loc], float(E + Ep))
+        seen[f] += 1
+        if f < 0:
+            continue
+        want = lib.SjhipHostSegmentIntersects(c[0], c[1], d[0], d[1], a[0], a[1], b[0], b[1])
+        assert bool(f) == bool(want), (a, b, c, d, f, kind)
+    assert seen[0] > 10000 and seen[1] > 5000 and seen[-1] > 1000, seen
+
