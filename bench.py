@@ -992,11 +992,32 @@ def run_poly_poly(args):
     walked = int(r.refine_candidates)
     log(f"timed: {ms_per_step:.1f} ms/step, {pairs} pairs, {cands} candidates ({walked} left to the wavefront "
         f"kernel by the raster pass), stages {stage}")
+    # the certified kernels against the inline-precision kernel alone
+    # (precision 1: every open candidate to k_refine_polypoly), both pair
+    # sets sorted by (probe, build) on the device
+    plain = None
+    if not args.no_plain:
+        r.release()
+        r = G.join_device(ix, rf, pdev, "intersects", sort_output=True, refine_order=args.refine_order)
+        gb, gp = r.to_host()
+        r.release()
+        t1 = time.time()
+        r1 = G.join_device(ix, rf, pdev, "intersects", precision=1, sort_output=True, refine_order=args.refine_order,
+                           profile=True)
+        plain_ms = (time.time() - t1) * 1000
+        pb, pp = r1.to_host()
+        plain = {"ms_per_step": plain_ms, "refine_ms": float(r1.res.stage_ms[3]), "pairs": int(r1.size),
+                 "pair_set_diff": pair_set_diff(gb, gp, pb, pp)}
+        r1.release()
+        del pb, pp
+        log(f"inline-precision kernel alone: {plain_ms:.1f} ms, pair-set diff {plain['pair_set_diff']}")
     cpu = None
     if strip is not None:
-        gb, gp = r.to_host()
+        if plain is None:
+            gb, gp = r.to_host()
         cpu = cpu_baseline(strip, gb, gp, args, predicate="intersects", what="polygons")
-    r.release()
+    if plain is None:
+        r.release()
     out = {
         "metric": "refined join pairs/s (ST_Intersects, clustered star polygons x star polygons, device-resident)",
         "value": pairs / (ms_per_step / 1000), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
@@ -1007,6 +1028,7 @@ def run_poly_poly(args):
         "pairs_per_step": pairs, "candidates_per_step": cands, "walked_candidates_per_step": walked,
         "stage_ms": {"decode_rects": stage[0], "morton": stage[1], "filter": stage[2], "group_by_polygon": stage[5],
                      "refine": stage[3]},
+        "inline_precision_kernel": plain,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

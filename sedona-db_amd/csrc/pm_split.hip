@@ -235,6 +235,7 @@ __global__ __launch_bounds__(kPmBlock) void k_pm_test(const float4* __restrict__
   uint32_t cur = 0xffffffffu, bid = 0;
   float4 r = make_float4(0, 0, 0, 0);
   PolyDesc d{};
+  double2 inv = make_double2(0.0, 0.0);  // the raster's cells per unit of the current row
   // the next step's point is loaded before this step is tested
   double2 nxt = make_double2(0.0, 0.0);
   if ((uint32_t)lane < rl32(my_n, 0)) nxt = xy[rl32(my_s, 0) + lane];
@@ -246,7 +247,11 @@ __global__ __launch_bounds__(kPmBlock) void k_pm_test(const float4* __restrict__
       cur = i;
       r = rects[i];
       bid = ids[i];
-      if (MODE == 1 || (MODE == 2 && draster == 2)) d = R.desc[bid];
+      if (MODE == 1 || (MODE == 2 && draster != 0)) {
+        d = R.desc[bid];
+        if (d.raster_off >= 0)
+          inv = cells_per_unit(d.rect, MODE == 2 && draster == 1 ? (double)kLineCells : (double)kRasterCells);
+      }
     }
     const bool in = (uint32_t)lane < n;
     bool h;
@@ -261,13 +266,13 @@ __global__ __launch_bounds__(kPmBlock) void k_pm_test(const float4* __restrict__
     }
     bool keep = false, open = h;
     if (MODE == 1 && h) {
-      const int cls = raster_class(R, d, pt);
+      const int cls = raster_class_inv(R, d, pt, inv);
       keep = cls == 1;
       open = cls == 0;
     }
-    if (MODE == 2 && h && draster == 1) open = line_raster_may_hold(R, bid, pt, dist);
+    if (MODE == 2 && h && draster == 1) open = line_raster_may_hold_d(R, d, inv, pt, dist);
     if (MODE == 2 && h && draster == 2) {
-      keep = raster_class(R, d, pt) == 1;
+      keep = raster_class_inv(R, d, pt, inv) == 1;
       open = !keep;
     }
     const uint64_t mk = __ballot(keep), mo = __ballot(open);

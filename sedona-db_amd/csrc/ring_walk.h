@@ -426,30 +426,44 @@ __device__ __forceinline__ bool ring_contains_run(const double2* xy, const float
   return on ? allow_on_edge : (in != 0);
 }
 
+// Cells per unit length along x and y of a raster of `cells` cells per
+// side over rect r (positive width and height).
+__device__ __forceinline__ double2 cells_per_unit(const double4& r, double cells) {
+  return make_double2(cells / (r.z - r.x), cells / (r.w - r.y));
+}
+
 // Cell-raster class of a point for a single-ring polygon (record d, its
-// ring index R): 1 inside, 2 outside (a clean cell, or outside the ring's
-// rect), 0 undecided (a cell near the boundary, or no raster).
-__device__ __forceinline__ int raster_class(const RingIndexView& R, const PolyDesc& d, double2 pt) {
+// ring index R, inv = cells_per_unit(d.rect, kRasterCells)): 1 inside, 2
+// outside (a clean cell, or outside the ring's rect), 0 undecided (a cell
+// near the boundary, or no raster).  A clean cell's 8 neighbours hold no
+// boundary either, so a point whose computed cell is one off its own (the
+// rounding of the product) still gets its own class.
+__device__ __forceinline__ int raster_class_inv(const RingIndexView& R, const PolyDesc& d, double2 pt, double2 inv) {
   if (d.raster_off < 0) return 0;
   if (!rect_covers(d.rect, pt.x, pt.y)) return 2;
-  const double cw = (d.rect.z - d.rect.x) / kRasterCells, ch = (d.rect.w - d.rect.y) / kRasterCells;
-  const int ci = min(max((int)floor((pt.x - d.rect.x) / cw), 0), kRasterCells - 1);
-  const int cj = min(max((int)floor((pt.y - d.rect.y) / ch), 0), kRasterCells - 1);
+  const int ci = min(max((int)floor((pt.x - d.rect.x) * inv.x), 0), kRasterCells - 1);
+  const int cj = min(max((int)floor((pt.y - d.rect.y) * inv.y), 0), kRasterCells - 1);
   const int cell = cj * kRasterCells + ci;
   return (R.raster[d.raster_off + (cell >> 4)] >> (2 * (cell & 15))) & 3;
 }
 
+__device__ __forceinline__ int raster_class(const RingIndexView& R, const PolyDesc& d, double2 pt) {
+  if (d.raster_off < 0) return 0;
+  return raster_class_inv(R, d, pt, cells_per_unit(d.rect, kRasterCells));
+}
+
 // Whether a line's cell raster leaves a segment possibly within dist of the
 // point: some cell of the square [pt - dist, pt + dist] (widened by one
-// cell) is set.  False also when the square misses the line's rect or the
-// point has no coordinates; rows without a raster answer true.
-__device__ __forceinline__ bool line_raster_may_hold(const RingIndexView& R, uint32_t b, double2 pt, double dist) {
+// cell, which also covers a computed cell one off the exact one) is set.
+// False also when the square misses the line's rect or the point has no
+// coordinates; rows without a raster answer true.  inv =
+// cells_per_unit(d.rect, kLineCells).
+__device__ __forceinline__ bool line_raster_may_hold_d(const RingIndexView& R, const PolyDesc& d, double2 inv,
+                                                       double2 pt, double dist) {
   if (!(pt.x == pt.x || pt.y == pt.y)) return false;
-  const PolyDesc d = R.desc[b];
   if (d.raster_off < 0) return true;
-  const double cw = (d.rect.z - d.rect.x) / kLineCells, ch = (d.rect.w - d.rect.y) / kLineCells;
-  const double fx0 = floor((pt.x - dist - d.rect.x) / cw) - 1.0, fx1 = floor((pt.x + dist - d.rect.x) / cw) + 1.0;
-  const double fy0 = floor((pt.y - dist - d.rect.y) / ch) - 1.0, fy1 = floor((pt.y + dist - d.rect.y) / ch) + 1.0;
+  const double fx0 = floor((pt.x - dist - d.rect.x) * inv.x) - 1.0, fx1 = floor((pt.x + dist - d.rect.x) * inv.x) + 1.0;
+  const double fy0 = floor((pt.y - dist - d.rect.y) * inv.y) - 1.0, fy1 = floor((pt.y + dist - d.rect.y) * inv.y) + 1.0;
   if (fx1 < 0.0 || fy1 < 0.0 || fx0 > kLineCells - 1.0 || fy0 > kLineCells - 1.0) return false;
   if (!(fy1 - fy0 < 24.0)) return true;  // a window of many rows: left to the walk
   const int i0 = (int)fmax(fx0, 0.0), i1 = (int)fmin(fx1, kLineCells - 1.0);
@@ -470,6 +484,13 @@ __device__ __forceinline__ bool line_raster_may_hold(const RingIndexView& R, uin
     if (any) return true;
   }
   return false;
+}
+
+__device__ __forceinline__ bool line_raster_may_hold(const RingIndexView& R, uint32_t b, double2 pt, double dist) {
+  if (!(pt.x == pt.x || pt.y == pt.y)) return false;
+  const PolyDesc d = R.desc[b];
+  if (d.raster_off < 0) return true;
+  return line_raster_may_hold_d(R, d, cells_per_unit(d.rect, kLineCells), pt, dist);
 }
 
 // The FP32 frame origin of a ring: the centre of its rect (the record of a
