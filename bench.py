@@ -294,7 +294,8 @@ def main():
     # split decided in the same pass); its algorithmic bytes are the
     # filter's (it also replaces the raster pass and the regrouping sort)
     roof_filter = dict(bound="hbm", achieved=filter_bytes / (filt_ms / 1e3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
-                       traffic=None, kernel="k_pm_ends+k_pm_count+k_pm_fill+k_pm_test+k_pm_emit", ms=filt_ms,
+                       traffic=traffic.get("k_pm_split") if full and order_used == 1 else None,
+                       kernel="k_pm_ends+k_pm_count+k_pm_fill+k_pm_test+k_pm_emit", ms=filt_ms,
                        algorithmic_bytes=filter_bytes)
     roof_filter["frac"] = roof_filter["achieved"] / HBM_PEAK_GBS
     pm_ms = pmaj_stage[2]

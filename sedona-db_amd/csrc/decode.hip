@@ -360,16 +360,29 @@ __device__ __forceinline__ uint32_t pt_f2o(float f) {
 // of rows that are not valid points in bounds[4].  A block issues a bounds
 // atomic only when it improves on the value it reads (few blocks do once
 // the extremes are found) and a count atomic only when it has such rows.
+// A valid row that is not a 21-byte point record (of byte order 0 or 1 and
+// type 1) sets *not_points and is left empty: the caller then decodes the
+// column with the general passes, which report its errors.
 __global__ __launch_bounds__(kBlock) void k_points(Src s, int64_t n, double2* xy, uint8_t* type,
-                                                   unsigned long long* err, unsigned int* type_mask,
-                                                   unsigned int* bounds) {
+                                                   unsigned long long* err, unsigned int* not_points,
+                                                   unsigned int* type_mask, unsigned int* bounds) {
   __shared__ unsigned int sb[5][kBlock / 64];
   unsigned int mnx = 0xffffffffu, mny = 0xffffffffu, mxx = 0, mxy = 0, bad = 0;
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint8_t t = 0;
+  bool other = false;
   if (r < n) {
-    double2 out;
-    decode_point_row(s, r, out, t, err);
+    double2 out = make_double2(NAN, NAN);
+    if (row_valid(s, r)) {
+      const int64_t b = row_begin(s, r), e = row_begin(s, r + 1);
+      bool ok = false;
+      if (e - b == 21) {
+        const uint8_t bo = s.data[b];
+        if (bo <= 1) ok = ld_u32(s.data, b + 1, bo == 1) == 1;
+      }
+      if (ok) decode_point_row(s, r, out, t, err);
+      other = !ok;
+    }
     xy[r] = out;
     type[r] = t;
     const float fx = (float)out.x, fy = (float)out.y;
@@ -381,6 +394,7 @@ __global__ __launch_bounds__(kBlock) void k_points(Src s, int64_t n, double2* xy
       bad = 1;
     }
   }
+  if (__ballot(other) && (threadIdx.x & 63) == 0) atomicOr(not_points, 1u);
   or_type_mask(type_mask, t);
   for (int o = 32; o > 0; o >>= 1) {
     mnx = min(mnx, (unsigned int)__shfl_xor((int)mnx, o));
@@ -430,20 +444,6 @@ __global__ void k_points_desc(const PointsDesc* d, int64_t cap, double2* xy, uin
   }
   xy[r] = out;
   type[r] = t;
-}
-
-// Whether every valid row is a 21-byte point record (cheap pre-check).
-__global__ void k_all_points(Src s, int64_t n, unsigned int* not_points) {
-  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  if (!row_valid(s, r)) return;
-  int64_t b = row_begin(s, r), e = row_begin(s, r + 1);
-  bool ok = false;
-  if (e - b == 21) {
-    uint8_t bo = s.data[b];
-    if (bo <= 1) ok = ld_u32(s.data, b + 1, bo == 1) == 1;
-  }
-  if (!ok) atomicOr(not_points, 1u);
 }
 
 // Int32 BinaryView -> int64 offsets + packed data (host side of the view
@@ -584,26 +584,30 @@ void decode_wkb(const WkbInput& in0, GeomColumn& g, hipStream_t s) {
   SJ_CHECK(hipMemcpyAsync(d_flags.p, init_flags, sizeof(init_flags), hipMemcpyHostToDevice, s));
   g.type.alloc(n > 0 ? n : 1, s);
 
-  if (n > 0) hipLaunchKernelGGL(k_all_points, blocks_for(n), kBlock, 0, s, src, n, d_flags.p + 1);
+  // a column of point records is decoded in one pass; any other row sends
+  // the column through the general passes
   unsigned int flags[2];
-  SJ_CHECK(hipMemcpyAsync(flags, d_flags.p, 8, hipMemcpyDeviceToHost, s));
-  SJ_CHECK(hipStreamSynchronize(s));
   unsigned long long herr = ~0ull;
+  g.xy.alloc(n > 0 ? n : 1, s);
+  if (n > 0)
+    hipLaunchKernelGGL(k_points, blocks_for(n), kBlock, 0, s, src, n, g.xy.p, g.type.p, d_err.p, d_flags.p + 1,
+                       d_flags.p, d_flags.p + 2);
+  unsigned int pf[7];
+  SJ_CHECK(hipMemcpyAsync(&herr, d_err.p, 8, hipMemcpyDeviceToHost, s));
+  SJ_CHECK(hipMemcpyAsync(pf, d_flags.p, sizeof(pf), hipMemcpyDeviceToHost, s));
+  SJ_CHECK(hipStreamSynchronize(s));
+  flags[0] = pf[0];
+  flags[1] = pf[1];
   if (flags[1] == 0) {
     g.points_only = true;
-    g.xy.alloc(n > 0 ? n : 1, s);
-    if (n > 0)
-      hipLaunchKernelGGL(k_points, blocks_for(n), kBlock, 0, s, src, n, g.xy.p, g.type.p, d_err.p, d_flags.p,
-                         d_flags.p + 2);
-    unsigned int pf[7];
-    SJ_CHECK(hipMemcpyAsync(&herr, d_err.p, 8, hipMemcpyDeviceToHost, s));
-    SJ_CHECK(hipMemcpyAsync(pf, d_flags.p, sizeof(pf), hipMemcpyDeviceToHost, s));
-    SJ_CHECK(hipStreamSynchronize(s));
-    flags[0] = pf[0];
     g.pt_valid = n - (int64_t)pf[6];
     for (int k = 0; k < 4; k++) g.pt_bounds[k] = g.pt_valid > 0 ? o2f_host(pf[2 + k]) : 0.f;
     g.n_parts = g.n_rings = g.n_pts = n;
   } else {
+    g.xy.release();
+    // (the type mask k_points gathered is dropped: k_scan gathers it again;
+    // k_points reported nothing, it decoded point records only)
+    SJ_CHECK(hipMemsetAsync(d_flags.p, 0, 4, s));
     DevArray<int64_t> cp(n, s), cr(n, s), cv(n, s);
     hipLaunchKernelGGL(k_scan, blocks_for(n), kBlock, 0, s, src, n, g.type.p, cp.p, cr.p, cv.p, d_err.p, d_flags.p);
     DevArray<int64_t> sp(n + 1, s), sr(n + 1, s), sv(n + 1, s);
