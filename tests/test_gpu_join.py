@@ -577,6 +577,78 @@ def test_poly_poly_raster_cases_vs_oracle(opts):
     assert (1, 2) in set(zip(ob.tolist(), op.tolist())) and len(ob) >= 6
 
 
+def _near_contact_pairs(rng, n_base=40):
+    """Star polygons and polygons in exact or near contact with them."""
+    def close(r):
+        r = np.asarray(r, np.float64).copy()
+        r[-1] = r[0]
+        return r
+
+    build, probe = [], []
+    for _ in range(n_base):
+        c = rng.uniform(0, 1000, 2)
+        n = int(rng.integers(4, 300))
+        R = float(rng.uniform(0.5, 3.0))
+        A = _star(c[0], c[1], 0.7 * R, R, n, phase=float(rng.uniform(0, 1)))
+        build.append(A)
+        m = len(A) - 1
+        j = int(rng.integers(0, m))
+        # a copy moved so that its first vertex lands exactly on vertex j
+        B = _star(c[0], c[1], 0.6 * R, 0.9 * R, int(rng.integers(4, 64)))
+        B = close(B - B[0] + A[j])
+        B[0] = A[j]
+        B[-1] = A[j]
+        probe.append(B)
+        # the mirror image across the line of edge (j, j + 1): the edge is shared
+        a, b = A[j], A[j + 1]
+        d = (b - a) / np.hypot(*(b - a))
+        v = A - a
+        M = a + 2 * (v @ d)[:, None] * d[None, :] - v
+        M[j], M[j + 1] = a, b
+        probe.append(close(M) if j != 0 else np.vstack([M[:-1], M[:1]]))
+        # every vertex moved by a few ulps
+        P = A[:-1] + np.spacing(A[:-1]) * rng.integers(-3, 4, A[:-1].shape)
+        probe.append(close(np.vstack([P, P[:1]])))
+        # the mirror image across the vertical line through the rightmost vertex
+        k = int(np.argmax(A[:-1, 0]))
+        X = A.copy()
+        X[:, 0] = 2 * A[k, 0] - A[:, 0]
+        X[k] = A[k]
+        probe.append(close(X) if k != 0 else np.vstack([X[:-1], X[:1]]))
+        # a copy scaled by 1 + 1e-9 about a far point
+        far = c + 1e4
+        probe.append(close(far + (A - far) * (1 + 1e-9)))
+    return build, probe
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_poly_poly_near_contact_vs_oracle(precision, opts):
+    """Polygon pairs in exact or near contact, where the certified FP32
+    point and crossing tests of the two-phase kernels are uncertain and hand
+    pairs to the inline-precision kernel: a shared vertex, a shared edge,
+    every vertex a few ulps off, a touching mirror image and a copy scaled
+    by 1 + 1e-9; every (build, probe) pair through the refiner, certified
+    (precision 0) and inline (precision 1), against the oracle."""
+    build, probe = _near_contact_pairs(np.random.default_rng(31))
+    b = D.rings_wkb([len(r) for r in build], np.vstack(build), 3)
+    p = D.rings_wkb([len(r) for r in probe], np.vstack(probe), 3)
+    B, P = O.Store(b.as_tuple()), O.Store(p.as_tuple())
+    bi = np.repeat(np.arange(B.n), P.n).astype(np.uint32)
+    pi = np.tile(np.arange(P.n), B.n).astype(np.uint32)
+    rf = G.GpuSpatialRefiner(opts)
+    rf.init_build_schema(b.to_arrow())
+    rf.push_build(b.to_arrow())
+    rf.finish_building()
+    rf.set_precision(precision)
+    ob, op = rf.refine(p.to_arrow(), "intersects", bi, pi)
+    want = O.refine(B, P, "intersects", bi, pi)
+    say("near contact", precision, int(want.sum()))
+    eq, miss, extra = pairs_equal((ob, op), (bi[want], pi[want]))
+    assert eq, (precision, miss, extra)
+    # every base polygon meets the five polygons made from it
+    assert int(want.sum()) >= 5 * B.n
+
+
 @pytest.mark.parametrize("rev", [False, True])
 def test_hole_corpus_polypoly_vs_reference_goldens(rev, opts):
     """Polygons with convex, concave and pentagram holes x small polygons,
