@@ -372,15 +372,38 @@ __device__ __forceinline__ void walk_line(LineWalk& w, int ls, uint32_t& tests, 
   }
 }
 
-// Candidates whose build row is not a single line with an FP32 frame, and
-// those with an uncertain test and no certain one, go to `redo` for the
-// FP64 kernel.
-template <int WAVES>
+// Appends the pair (b, p) to lists (lb, lp) (count *cnt) for every lane
+// with f set: one atomic per wavefront.  Called by all lanes.
+__device__ __forceinline__ void wave_push_pair(bool f, uint32_t b, uint32_t p, uint32_t* lb, uint32_t* lp,
+                                               unsigned long long* cnt) {
+  const uint64_t m = __ballot(f);
+  if (!m) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(cnt, (unsigned long long)__popcll(m));
+  base = __shfl(base, leader);
+  if (f) {
+    const unsigned long long k = base + __popcll(m & ((1ull << lane) - 1ull));
+    lb[k] = b;
+    lp[k] = p;
+  }
+}
+
+// Candidates whose build row is not a single line (POLY: a single-ring
+// polygon) with an FP32 frame, and those with an uncertain test and no
+// certain one, go to `redo` for the FP64 kernel.  POLY: the ring's
+// segments are walked as the line's are (a single-ring polygon is within d
+// of the point when a segment is, or when it holds the point); the
+// candidates with every segment certainly farther than d go, as pairs, to
+// (pb, pp) (count *n_p) for the point-in-polygon walk.
+template <int WAVES, bool POLY>
 __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void k_walk_lines(
     RingIndexView R, const double2* __restrict__ pts, const uint32_t* __restrict__ cb,
     const uint32_t* __restrict__ cp, uint64_t n, const unsigned long long* n_dev, const uint32_t* probe_ids,
     double dist, uint32_t* ob, uint32_t* op, uint64_t cap, unsigned long long* counter, unsigned long long* stats,
-    uint32_t* redo, unsigned long long* n_redo, unsigned long long* queue) {
+    uint32_t* redo, unsigned long long* n_redo, unsigned long long* queue, uint32_t* pb, uint32_t* pp,
+    unsigned long long* n_p) {
   __shared__ uint32_t wo_b[kWalkBlock / 64][kWaveOutCap], wo_p[kWalkBlock / 64][kWaveOutCap];
   WaveOut wo{wo_b[threadIdx.x >> 6], wo_p[threadIdx.x >> 6], 0};
   if (n_dev) n = min(n, (uint64_t)*n_dev);  // candidate count produced on the device (n: its bound)
@@ -401,7 +424,7 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(WAVE
   uint32_t tests = 0, unsure = 0;
   for (uint32_t wbase = next(0); wbase < n32; wbase = next(wbase)) {
     const uint32_t i = wbase + lane;
-    bool keep = false, walk = false, defer = false;
+    bool keep = false, walk = false, defer = false, inside_test = false;
     LineWalk w;
     if (i < n32) {
       const uint32_t b = cb[i];
@@ -409,9 +432,9 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(WAVE
       const double2 pt = pts[cp[i]];
       if (pt.x == pt.x || pt.y == pt.y) {
         const PolyDesc dsc = R.desc[b];
-        if (dsc.npts < 2 || !(dsc.err < INFINITY)) {
-          // other rows, lines of fewer than 2 vertices (distance 0) and
-          // lines without an FP32 frame: the FP64 kernel
+        if (dsc.npts < (POLY ? 4 : 2) || !(dsc.err < INFINITY)) {
+          // other rows, lines of fewer than 2 vertices (distance 0), rings
+          // of fewer than 4 and rows without an FP32 frame: the FP64 kernel
           defer = true;
         } else {
           walk = true;
@@ -432,9 +455,11 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(WAVE
       walk_line(w, ls, tests, dist);
       keep = (w.st & 1u) != 0;
       defer = !keep && w.st > 1u;
+      inside_test = POLY && !keep && !defer;
       unsure += w.st >> 1;
     }
     wave_push(defer, i, redo, n_redo);
+    if (POLY) wave_push_pair(inside_test, inside_test ? cb[i] : 0u, inside_test ? cp[i] : 0u, pb, pp, n_p);
     uint32_t b = 0, p = 0;
     if (keep) {
       b = cb[i];
@@ -459,7 +484,8 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(WAVE
 void launch_line_walk(const RingIndex& ri, const GeomColumn& probe, const uint32_t* cb, const uint32_t* cp,
                       uint64_t n, const unsigned long long* n_dev, const uint32_t* probe_ids, double d, uint32_t* ob,
                       uint32_t* op, uint64_t cap, unsigned long long* counter, unsigned long long* stats,
-                      uint32_t* redo, unsigned long long* n_redo, hipStream_t s, unsigned long long* queue) {
+                      uint32_t* redo, unsigned long long* n_redo, hipStream_t s, unsigned long long* queue,
+                      uint32_t* pb, uint32_t* pp, unsigned long long* n_p) {
   if (n == 0) return;
   if (!probe.points_only) throw Error("launch_line_walk: the probe column must hold points only");
   if (ri.desc.p == nullptr) throw Error("launch_line_walk: the ring index has no row records");
@@ -472,8 +498,15 @@ void launch_line_walk(const RingIndex& ri, const GeomColumn& probe, const uint32
   // the blocks that can be resident at once
   const unsigned g =
       (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, (uint64_t)cus * 4 * SJ_LW_WAVES / (kWalkBlock / 64)));
-  hipLaunchKernelGGL(k_walk_lines<SJ_LW_WAVES>, g, kWalkBlock, 0, s, ri.view(), probe.xy.p, cb, cp, n, n_dev, probe_ids, d, ob,
-                     op, cap, counter, stats, redo, n_redo, queue);
+  if (pb) {
+    if (!pp || !n_p) throw Error("launch_line_walk: the polygon form needs its pair list");
+    hipLaunchKernelGGL((k_walk_lines<SJ_LW_WAVES, true>), g, kWalkBlock, 0, s, ri.view(), probe.xy.p, cb, cp, n,
+                       n_dev, probe_ids, d, ob, op, cap, counter, stats, redo, n_redo, queue, pb, pp, n_p);
+  } else {
+    hipLaunchKernelGGL((k_walk_lines<SJ_LW_WAVES, false>), g, kWalkBlock, 0, s, ri.view(), probe.xy.p, cb, cp, n,
+                       n_dev, probe_ids, d, ob, op, cap, counter, stats, redo, n_redo, queue, nullptr, nullptr,
+                       nullptr);
+  }
   SJ_CHECK(hipGetLastError());
 }
 

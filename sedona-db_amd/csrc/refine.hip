@@ -2755,11 +2755,35 @@ uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const Geo
   const uint32_t* wp = n_open ? open.p.p : cp;
   // lines, certified form: the lean FP32 walk, then the FP64 kernel over
   // the candidates it lists (other rows, uncertain tests)
-  const bool lean = kind == kDistLines && precision == 0 && ri.desc.p != nullptr && probe.points_only &&
-                    d >= 0 && d < INFINITY && n < 0xffffffffull - 64;
+  const bool lean = (kind == kDistLines || kind == kDistPolygons) && precision == 0 && ri.desc.p != nullptr &&
+                    probe.points_only && d >= 0 && d < INFINITY && n < 0xffffffffull - 64;
   DevArray<uint32_t> redo_own;
   DevArray<unsigned long long> walk_aux;  // redo count, queue, tests, uncertain tests
-  switch (lean ? -1 : kind) {
+  switch (lean ? (kind == kDistLines ? -1 : -2) : kind) {
+    case -2: {
+      // single-ring polygons: the segments first (certainly within d:
+      // kept); where every segment is certainly farther, the point-in-
+      // polygon walk keeps the points certainly inside (with the cell
+      // rasters) and lists the uncertain ones for its FP64 pass (inside or
+      // on the boundary: distance 0); rows of several rings or parts and
+      // uncertain segment terms go to the FP64 distance kernel
+      DevArray<uint32_t>& redo = ws ? ws->redo : redo_own;
+      fit(redo, n, s);
+      DevArray<uint32_t> pb(n, s), pp(n, s), redo2(n, s);
+      // [0] redo count, [1] queue, [2..3] walk stats, [4] pairs to the
+      // inside test, [5] its redo count, [6] its queue, [7..8] its stats
+      walk_aux.alloc(9, s);
+      SJ_CHECK(hipMemsetAsync(walk_aux.p, 0, 72, s));
+      launch_line_walk(ri, probe, wb, wp, n, n_open, probe_ids, d, out.b.p, out.p.p, out.cap(), out.counter.p,
+                       walk_aux.p + 2, redo.p, walk_aux.p, s, walk_aux.p + 1, pb.p, pp.p, walk_aux.p + 4);
+      launch_ring_walk(ri, probe, pb.p, pp.p, n, walk_aux.p + 4, probe_ids, false, out.b.p, out.p.p, out.cap(),
+                       out.counter.p, walk_aux.p + 7, redo2.p, walk_aux.p + 5, nullptr, nullptr, 8, s, walk_aux.p + 6);
+      hipLaunchKernelGGL(k_pip_redo_wave, 2048, kBlock, 0, s, c, 1, pb.p, pp.p, redo2.p, walk_aux.p + 5, probe_ids,
+                         out.b.p, out.p.p, out.cap(), out.counter.p);
+      hipLaunchKernelGGL(k_refine_distance<kDistPolygons>, std::min(grid_for(n), 2048u), kBlock, 0, s, c, d, 0, wb,
+                         wp, n, redo.p, out.b.p, out.p.p, out.cap(), out.counter.p, aux.p, walk_aux.p, probe_ids);
+      break;
+    }
     case -1: {
       DevArray<uint32_t>& redo = ws ? ws->redo : redo_own;
       fit(redo, n, s);
@@ -2785,14 +2809,14 @@ uint64_t refine_distance(const GeomColumn& build, const RingIndex& ri, const Geo
   }
   SJ_CHECK(hipGetLastError());
   if (timer) timer->record_stop(s);
-  unsigned long long cnt = 0, h[3], walked = n, wh[4] = {0, 0, 0, 0};
+  unsigned long long cnt = 0, h[3], walked = n, wh[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   SJ_CHECK(hipMemcpyAsync(&cnt, out.counter.p, 8, hipMemcpyDeviceToHost, s));
   SJ_CHECK(hipMemcpyAsync(h, aux.p, 24, hipMemcpyDeviceToHost, s));
-  if (walk_aux.p) SJ_CHECK(hipMemcpyAsync(wh, walk_aux.p, 32, hipMemcpyDeviceToHost, s));
+  if (walk_aux.p) SJ_CHECK(hipMemcpyAsync(wh, walk_aux.p, 8 * walk_aux.n, hipMemcpyDeviceToHost, s));
   if (n_open) SJ_CHECK(hipMemcpyAsync(&walked, n_open, 8, hipMemcpyDeviceToHost, s));
   SJ_CHECK(hipStreamSynchronize(s));
   if (stats) {
-    stats->segment_tests += h[0] + wh[2];
+    stats->segment_tests += h[0] + wh[2] + wh[7];
     stats->fallback_tests += h[1] + h[2];  // FP64 re-evaluations: exact ring scans and segment terms
     stats->walked += walked;
   }

@@ -650,11 +650,16 @@ void launch_ring_walk(const RingIndex& ri, const GeomColumn& probe, const uint32
 // line: pairs certainly within d are appended to (ob, op); candidates of
 // other rows, and those with an uncertain test and no certain one, are
 // listed in redo (count *n_redo) for the FP64 kernel.  stats[0..1]: tests
-// and uncertain tests.  queue: a counter that is zero at launch.
+// and uncertain tests.  queue: a counter that is zero at launch.  With a
+// pair list (pb, pp, n_p), the build rows are single-ring polygons: the
+// candidates whose ring segments are all certainly farther than d are
+// listed there as (build row, probe position) for the point-in-polygon
+// walk.
 void launch_line_walk(const RingIndex& ri, const GeomColumn& probe, const uint32_t* cb, const uint32_t* cp,
                       uint64_t n, const unsigned long long* n_dev, const uint32_t* probe_ids, double d, uint32_t* ob,
                       uint32_t* op, uint64_t cap, unsigned long long* counter, unsigned long long* stats,
-                      uint32_t* redo, unsigned long long* n_redo, hipStream_t s, unsigned long long* queue);
+                      uint32_t* redo, unsigned long long* n_redo, hipStream_t s, unsigned long long* queue,
+                      uint32_t* pb = nullptr, uint32_t* pp = nullptr, unsigned long long* n_p = nullptr);
 
 // Throws "build index out of range: v" / "probe index out of range: v" for
 // the first position (build index first) with cb[i] >= nb or cp[i] >= np.
