@@ -448,9 +448,10 @@ int SjhipIndexSetFilterMode(struct SedonaFloatIndex2D* index, int32_t mode);
 
 /* Precision of the refiner's refine and SjhipRefinerRefineDistance calls:
  * 0 (the default) runs the certified FP32 tests with FP64 re-evaluation
- * where a sign is uncertain, 1 the all-FP64 plain kernels.  Both give the
- * same pair set; 1 exists for comparisons.  No counterpart in the
- * reference. */
+ * where a sign is uncertain, 1 the all-FP64 plain kernels (for polygon x
+ * polygon Intersects: the kernel that decides uncertain segments in FP64
+ * inline, without the certified phases).  Both give the same pair set; 1
+ * exists for comparisons.  No counterpart in the reference. */
 int SjhipRefinerSetPrecision(struct SedonaSpatialRefiner* refiner, int32_t precision);
 
 /* Request loop over a resident index and refiner (the serving path of the

@@ -221,7 +221,8 @@ __global__ __launch_bounds__(kPmBlock) void k_pm_test(const float4* __restrict__
                                                       const uint32_t* __restrict__ rec_s,
                                                       const uint8_t* __restrict__ rec_n, uint64_t S,
                                                       ulonglong2* __restrict__ masks, uint64_t* __restrict__ cnt2,
-                                                      uint64_t* __restrict__ whits, double dist, int draster) {
+                                                      uint64_t* __restrict__ whits, double dist, int draster,
+                                                      const float4* __restrict__ qrects) {
   const int lane = threadIdx.x & 63;
   const uint64_t w = ((uint64_t)blockIdx.x * kPmBlock + threadIdx.x) >> 6;
   const uint64_t k0 = w * 64;
@@ -236,13 +237,22 @@ __global__ __launch_bounds__(kPmBlock) void k_pm_test(const float4* __restrict__
   float4 r = make_float4(0, 0, 0, 0);
   PolyDesc d{};
   double2 inv = make_double2(0.0, 0.0);  // the raster's cells per unit of the current row
-  // the next step's point is loaded before this step is tested
+  // the next step's point (MODE 2: its probe rect) is loaded before this
+  // step is tested
   double2 nxt = make_double2(0.0, 0.0);
-  if ((uint32_t)lane < rl32(my_n, 0)) nxt = xy[rl32(my_s, 0) + lane];
+  float4 nxq = make_float4(0.f, 0.f, 0.f, 0.f);
+  if ((uint32_t)lane < rl32(my_n, 0)) {
+    if (MODE == 2) nxq = qrects[rl32(my_s, 0) + lane];
+    else nxt = xy[rl32(my_s, 0) + lane];
+  }
   for (int j = 0; j < steps; j++) {
     const uint32_t i = rl32(my_i, j), n = rl32(my_n, j);
-    const double2 pt = nxt;
-    if (j + 1 < steps && (uint32_t)lane < rl32(my_n, j + 1)) nxt = xy[rl32(my_s, j + 1) + lane];
+    double2 pt = nxt;
+    const float4 q = nxq;
+    if (j + 1 < steps && (uint32_t)lane < rl32(my_n, j + 1)) {
+      if (MODE == 2) nxq = qrects[rl32(my_s, j + 1) + lane];
+      else nxt = xy[rl32(my_s, j + 1) + lane];
+    }
     if (i != cur) {
       cur = i;
       r = rects[i];
@@ -257,9 +267,10 @@ __global__ __launch_bounds__(kPmBlock) void k_pm_test(const float4* __restrict__
     bool h;
     if (MODE == 2) {
       // the probe rect of the ST_DWithin filter (compute_rects with the
-      // distance) against the build rect
-      const float4 q = finish_rect(pt.x, pt.y, pt.x, pt.y, false, dist);
+      // distance, made once per point by k_pm_probe_rects) against the
+      // build rect; the point itself only where a raster reads it
       h = in && !(q.x > r.z || q.z < r.x || q.y > r.w || q.w < r.y);
+      if (h && draster != 0) pt = xy[rl32(my_s, j) + lane];
     } else {
       const float fx = (float)pt.x, fy = (float)pt.y;
       h = in && !(fx > r.z || fx < r.x || fy > r.w || fy < r.y);
@@ -373,6 +384,15 @@ __global__ __launch_bounds__(kPmBlock) void k_pm_emit(const uint32_t* __restrict
 
 __global__ void k_pm_set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
 
+// The ST_DWithin probe rect of each point (compute_rects' form: the point's
+// outward FP32 box widened by dist, rounded outward again).
+__global__ void k_pm_probe_rects(const double2* __restrict__ xy, int64_t n, double dist, float4* __restrict__ q) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double2 p = xy[i];
+  q[i] = finish_rect(p.x, p.y, p.x, p.y, false, dist);
+}
+
 }  // namespace
 
 bool pm_split(const RectTree& t, const RingIndex& ri, const double2* xy, const uint32_t* keys, int64_t nq,
@@ -457,9 +477,15 @@ bool pm_split(const RectTree& t, const RingIndex& ri, const double2* xy, const u
                      sc.start.p, sc.off.p, sc.rec_i.p, sc.rec_s.p, sc.rec_n.p);
   const unsigned bs = (unsigned)((S + kPmBlock - 1) / kPmBlock);
   const RingIndexView R = ri.view();
+  if (mode == 2) {
+    fit(sc.qrect, (size_t)nq, s);
+    hipLaunchKernelGGL(k_pm_probe_rects, (unsigned)((nq + kPmBlock - 1) / kPmBlock), kPmBlock, 0, s, xy, nq, dist,
+                       sc.qrect.p);
+  }
   auto test = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, bs, kPmBlock, 0, s, t.leaf_rects.p, t.leaf_ids.p, R, xy, sc.rec_i.p, sc.rec_s.p,
-                       sc.rec_n.p, S, sc.masks.p, sc.cnt2.p, sc.whits.p, dist, dwithin ? dist_raster : 0);
+                       sc.rec_n.p, S, sc.masks.p, sc.cnt2.p, sc.whits.p, dist, dwithin ? dist_raster : 0,
+                       mode == 2 ? sc.qrect.p : nullptr);
   };
   if (mode == 1) test(k_pm_test<1>);
   else if (mode == 2) test(k_pm_test<2>);

@@ -382,10 +382,11 @@ struct PmScratch {
   DevArray<uint64_t> off, flag, cnt2, off2, whits, hits;
   DevArray<uint8_t> rec_n, tmp;
   DevArray<ulonglong2> masks;
+  DevArray<float4> qrect;  // ST_DWithin: the probe rect of each point
   void set_stream(hipStream_t s) {
     start.s = ends.s = cnt.s = rec_i.s = rec_s.s = s;
     off.s = flag.s = cnt2.s = off2.s = whits.s = hits.s = s;
-    rec_n.s = tmp.s = masks.s = s;
+    rec_n.s = tmp.s = masks.s = qrect.s = s;
   }
 };
 
