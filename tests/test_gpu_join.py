@@ -959,6 +959,50 @@ def test_nulls_empties_big_endian_large_and_view(opts):
         assert set(zip(ob.tolist(), op.tolist())) == {(0, 0), (2, 1), (2, 5)}, typ
 
 
+@pytest.mark.parametrize("pred,d", [("contains", 0.0), ("intersects", 0.0), ("dwithin", 0.05)])
+def test_invalid_probe_rows_sort_after_corner_points(pred, d, opts):
+    """The fused join orders the valid probe points by Morton key and drops
+    the invalid rows (null, empty, NaN), whose key sorts after every valid
+    one.  Valid points at the far corner of the points' frame get keys whose
+    top bits equal the invalid key's; with null, empty and NaN rows mixed
+    among them, every corner point must still be joined, in every candidate
+    order."""
+    import pyarrow as pa
+    rng = np.random.default_rng(43)
+    n = 50_000
+    x, y = rng.uniform(0, 100, n), rng.uniform(0, 100, n)
+    # the frame's far corner: the largest x and y, and points within a few
+    # key units of it
+    x[:200] = 100.0 - rng.uniform(0, 1e-3, 200)
+    y[:200] = 100.0 - rng.uniform(0, 1e-3, 200)
+    x[0], y[0] = 100.0, 100.0
+    col = D.points_wkb(x, y)
+    rows = [bytes(col.data[col.offsets[i]:col.offsets[i + 1]]) for i in range(n)]
+    # null, empty and NaN rows spread among the valid ones
+    for i in rng.choice(np.arange(200, n), 3000, replace=False):
+        k = int(rng.integers(0, 3))
+        rows[i] = None if k == 0 else (C.point_wkb(float("nan"), float("nan")) if k == 1 else
+                                       b"\x01" + (1).to_bytes(4, "little") + np.array([np.nan, np.nan]).tobytes())
+    p = pa.array(rows, type=pa.binary())
+    # polygons over the corner and over the rest
+    b, _ = D.workload_pip_uniform(10, 300, seed=44, extent=100.0)
+    corner = [np.array([[99.0, 99.0], [101.0, 99.0], [101.0, 101.0], [99.0, 101.0], [99.0, 99.0]])]
+    bc = D.rings_wkb([5], np.vstack(corner), 3)
+    bcol = pa.array([bytes(bc.data[bc.offsets[0]:bc.offsets[1]])] +
+                    [bytes(b.data[b.offsets[i]:b.offsets[i + 1]]) for i in range(len(b.offsets) - 1)], type=pa.binary())
+    ix, rf = build_pair(bcol, opts)
+    ob, op = O.join(O.Store(bcol), O.Store(p), pred, distance=d, nthreads=8)
+    got = set()
+    for order in ((0, 1, 2) if pred == "dwithin" else (0, 1)):
+        say("corner points", pred, d, order)
+        r = G.join_device(ix, rf, p, pred, distance=d, refine_order=order)
+        gb, gp = r.to_host()
+        assert np.array_equal(gb, ob) and np.array_equal(gp, op), (pred, order, len(gb), len(ob))
+        got = set(gp[gb == 0].tolist())
+    # every corner point lies inside the corner square (build row 0)
+    assert set(range(200)) <= got, sorted(set(range(200)) - got)[:10]
+
+
 def test_empty_build_and_probe(opts):
     import pyarrow as pa
     empty = pa.array([], type=pa.binary())
