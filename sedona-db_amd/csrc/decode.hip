@@ -314,7 +314,28 @@ __device__ __forceinline__ void ld_point_record(const uint8_t* d, int64_t o, uin
   w[5] = __builtin_amdgcn_alignbyte(0u, a5, sh);
 }
 
-// Points-only fast path: one thread per row.
+// The fields of a 21-byte point record read by ld_point_record: byte
+// order (byte 0), geometry type (bytes 1..4, in that order) and
+// coordinates (bytes 5..20).
+struct PointRecord {
+  uint32_t bo, ty;
+  double2 xy;
+};
+
+__device__ __forceinline__ PointRecord parse_point_record(const uint32_t w[6]) {
+  PointRecord p;
+  p.bo = w[0] & 0xff;
+  const bool le = p.bo == 1;
+  // shift the stream by one byte: c[k] = bytes 1+4k .. 4+4k
+  const uint32_t c0 = __builtin_amdgcn_alignbyte(w[1], w[0], 1), c1 = __builtin_amdgcn_alignbyte(w[2], w[1], 1);
+  const uint32_t c2 = __builtin_amdgcn_alignbyte(w[3], w[2], 1), c3 = __builtin_amdgcn_alignbyte(w[4], w[3], 1);
+  const uint32_t c4 = __builtin_amdgcn_alignbyte(w[5], w[4], 1);
+  p.ty = le ? c0 : __builtin_bswap32(c0);
+  const uint32_t cw[4] = {c1, c2, c3, c4};
+  p.xy = words_to_xy(cw, le);
+  return p;
+}
+
 // One row of a column of 21-byte WKB points: coordinates (NaN for null
 // and empty rows) and type (0 for null rows).
 __device__ __forceinline__ void decode_point_row(const Src& s, int64_t r, double2& out, uint8_t& t,
@@ -328,18 +349,11 @@ __device__ __forceinline__ void decode_point_row(const Src& s, int64_t r, double
     } else {
       uint32_t w[6];
       ld_point_record(s.data, b, w);
-      uint32_t bo = w[0] & 0xff;
-      bool le = bo == 1;
-      // shift the stream by one byte: c[k] = bytes 1+4k .. 4+4k
-      uint32_t c0 = __builtin_amdgcn_alignbyte(w[1], w[0], 1), c1 = __builtin_amdgcn_alignbyte(w[2], w[1], 1);
-      uint32_t c2 = __builtin_amdgcn_alignbyte(w[3], w[2], 1), c3 = __builtin_amdgcn_alignbyte(w[4], w[3], 1);
-      uint32_t c4 = __builtin_amdgcn_alignbyte(w[5], w[4], 1);
-      uint32_t ty = le ? c0 : __builtin_bswap32(c0);
-      if (bo > 1) report(err, r, kErrByteOrder);
-      else if (ty != 1) report(err, r, kErrType);
+      const PointRecord pr = parse_point_record(w);
+      if (pr.bo > 1) report(err, r, kErrByteOrder);
+      else if (pr.ty != 1) report(err, r, kErrType);
       else {
-        uint32_t cw[4] = {c1, c2, c3, c4};
-        out = words_to_xy(cw, le);
+        out = pr.xy;
         bool empty = (out.x != out.x) && (out.y != out.y);
         t = kPoint;
         if (empty) out = make_double2(NAN, NAN);
@@ -377,10 +391,17 @@ __global__ __launch_bounds__(kBlock) void k_points(Src s, int64_t n, double2* xy
       const int64_t b = row_begin(s, r), e = row_begin(s, r + 1);
       bool ok = false;
       if (e - b == 21) {
-        const uint8_t bo = s.data[b];
-        if (bo <= 1) ok = ld_u32(s.data, b + 1, bo == 1) == 1;
+        // the record is read once, as aligned dwords
+        uint32_t w[6];
+        ld_point_record(s.data, b, w);
+        const PointRecord pr = parse_point_record(w);
+        ok = pr.bo <= 1 && pr.ty == 1;
+        if (ok) {
+          out = pr.xy;
+          t = kPoint;
+          if (out.x != out.x && out.y != out.y) out = make_double2(NAN, NAN);  // empty
+        }
       }
-      if (ok) decode_point_row(s, r, out, t, err);
       other = !ok;
     }
     xy[r] = out;
@@ -694,9 +715,20 @@ __global__ void k_rects_rows(GeomView g, int nearest, double expand, float4* out
   }
 }
 
+// The reads of xy are random; the order is read and the output written
+// once, as streams (non-temporal: they need not stay in the caches, which
+// keeps more of them for the random reads).
 __global__ void k_gather_points(const double2* xy, const uint32_t* order, int64_t n, double2* out) {
+  typedef double d2v __attribute__((ext_vector_type(2)));
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = xy[order[i]];
+  if (i < n) {
+    const uint32_t o = __builtin_nontemporal_load(order + i);
+    const double2 v = xy[o];
+    d2v w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, (d2v*)(out + i));
+  }
 }
 
 }  // namespace
