@@ -74,6 +74,8 @@ def main():
                     help="pip_zipf_contains: resident waves of the point-in-polygon kernel (4, 6, 8) set before the "
                          "joins (0 = keep)")
     ap.add_argument("--probe-batch", type=int, default=5_000_000, help="dwithin_walks: probe points per batch")
+    ap.add_argument("--poly-batch", type=int, default=500_000,
+                    help="poly_poly: probe polygons per join call (0: all; each call's candidates must stay below 2^32)")
     ap.add_argument("--serve-requests", type=int, default=96,
                     help="requests in the serving measurement (0 = skip it)")
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -975,29 +977,53 @@ def run_poly_poly(args):
     ix.build_from_refiner(rf)
     pdev = G.DeviceWkb.from_host(b)
 
+    # probe polygons per join call.  The clusters keep their width at every
+    # scale, so candidates grow about as the square of the scale (5.5e8 at
+    # 0.1 in one call); slices of the probe column keep each call's
+    # candidates far below 2^32 (the pair-set comparisons run on one-call
+    # scales)
+    nb = na if args.poly_batch <= 0 else min(args.poly_batch, na)
+    batched = nb < na
+    if batched and strip is not None:
+        log("probe side in slices: the CPU strip comparison runs at one-call scales (--scale 0.1)")
+        strip = None
+
     def step():
-        return G.join_device(ix, rf, pdev, "intersects", sort_output=False, refine_order=args.refine_order,
-                             profile=True)
+        if not batched:
+            r = G.join_device(ix, rf, pdev, "intersects", sort_output=False, refine_order=args.refine_order,
+                              profile=True)
+            return r, [int(r.size), int(r.candidates), int(r.refine_candidates)], list(r.res.stage_ms)
+        tot, stage = [0, 0, 0], [0.0] * 6
+        for s0 in range(0, na, nb):
+            r = G.join_device(ix, rf, pdev.slice(s0, min(na, s0 + nb)), "intersects", sort_output=False,
+                              refine_order=args.refine_order, profile=True)
+            for k, v in enumerate((r.size, r.candidates, r.refine_candidates)):
+                tot[k] += int(v)
+            stage = [a + float(b) for a, b in zip(stage, r.res.stage_ms)]
+            r.release()
+            if (s0 // nb) % 10 == 9:
+                log(f"  {min(na, s0 + nb)} of {na} probe polygons joined, {tot[0]} pairs so far")
+        return None, tot, stage
 
     for i in range(args.warmup):
-        r = step()
-        log(f"warmup {i}: {r.size} pairs, {r.candidates} candidates, stages {list(r.res.stage_ms)}")
-        r.release()
+        r, tot, stage = step()
+        log(f"warmup {i}: {tot[0]} pairs, {tot[1]} candidates, stages {stage}")
+        if r is not None:
+            r.release()
     t = time.time()
     for i in range(args.steps):
-        r = step()
-        if i < args.steps - 1:
+        r, tot, stage = step()
+        if i < args.steps - 1 and r is not None:
             r.release()
     ms_per_step = (time.time() - t) * 1000 / args.steps
-    pairs, cands, stage = int(r.size), int(r.candidates), list(r.res.stage_ms)
-    walked = int(r.refine_candidates)
+    pairs, cands, walked = tot
     log(f"timed: {ms_per_step:.1f} ms/step, {pairs} pairs, {cands} candidates ({walked} left to the wavefront "
         f"kernel by the raster pass), stages {stage}")
     # the certified kernels against the inline-precision kernel alone
     # (precision 1: every open candidate to k_refine_polypoly), both pair
     # sets sorted by (probe, build) on the device
     plain = None
-    if not args.no_plain:
+    if not args.no_plain and not batched:
         r.release()
         r = G.join_device(ix, rf, pdev, "intersects", sort_output=True, refine_order=args.refine_order)
         gb, gp = r.to_host()
@@ -1017,7 +1043,7 @@ def run_poly_poly(args):
         if plain is None:
             gb, gp = r.to_host()
         cpu = cpu_baseline(strip, gb, gp, args, predicate="intersects", what="polygons")
-    if plain is None:
+    if plain is None and r is not None:
         r.release()
     out = {
         "metric": "refined join pairs/s (ST_Intersects, clustered star polygons x star polygons, device-resident)",
@@ -1025,7 +1051,7 @@ def run_poly_poly(args):
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded, generated on the host)",
         "config": {"workload": "poly_poly", "polygons_each_side": na, "scale_of_5M": args.scale,
-                   "predicate": "ST_Intersects", "parallelism": "single-gpu"},
+                   "predicate": "ST_Intersects", "probe_polygons_per_call": nb, "parallelism": "single-gpu"},
         "pairs_per_step": pairs, "candidates_per_step": cands, "walked_candidates_per_step": walked,
         "stage_ms": {"decode_rects": stage[0], "morton": stage[1], "filter": stage[2], "group_by_polygon": stage[5],
                      "refine": stage[3]},
