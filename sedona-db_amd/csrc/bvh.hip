@@ -507,6 +507,8 @@ uint64_t filter_pairs(const RectTree& t, const float4* q, int64_t nq, const uint
     SJ_CHECK(hipMemcpyAsync(&total, pb.counter.p, 8, hipMemcpyDeviceToHost, s));
     SJ_CHECK(hipStreamSynchronize(s));
     if (nq > 0) t.cand_per_query_x16.store((uint64_t)((total * 16) / (uint64_t)nq + 1), std::memory_order_relaxed);
+    // the refine kernels index candidates with 32 bits
+    if (total >= 0xffffffffull - 64) throw Error("filter: 2^32 - 64 candidates or more: probe in smaller batches");
     if (total <= pb.cap()) return total;
     size_t cap = (size_t)(total + total / 8 + 1024);
     pb.b.alloc(cap, s);
